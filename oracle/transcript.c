@@ -1,0 +1,166 @@
+/* Challenge sources for the oracle (test infrastructure).
+ *
+ * The reference keeps Fiat-Shamir in spongefish (crates/transcript/src/
+ * {domain.rs,prover.rs,challenge.rs}); that crate is absent here, so the
+ * oracle exposes transcripts as callbacks and offers two stand-ins:
+ *  - seeded: challenges from the splitmix64 stream the reference's GKR tests
+ *    use (crates/gkr/src/lib.rs:986-999), F128::from([w0, w1]);
+ *  - duplex: a SHA-256 ratchet over every absorbed message, so challenges
+ *    depend on every earlier message. */
+#include <stdlib.h>
+#include <string.h>
+
+#include "bitz_oracle.h"
+
+/* ---- FIPS 180-4 SHA-256 ---------------------------------------------------- */
+static const uint32_t K256[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+static inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void sha256_block(uint32_t h[8], const uint8_t* p) {
+    uint32_t w[64];
+    for (int i = 0; i < 16; ++i)
+        w[i] = ((uint32_t)p[4 * i] << 24) | ((uint32_t)p[4 * i + 1] << 16) |
+               ((uint32_t)p[4 * i + 2] << 8) | (uint32_t)p[4 * i + 3];
+    for (int i = 16; i < 64; ++i) {
+        uint32_t s0 = rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3);
+        uint32_t s1 = rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10);
+        w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+    }
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+    for (int i = 0; i < 64; ++i) {
+        uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
+        uint32_t ch = (e & f) ^ (~e & g);
+        uint32_t t1 = hh + S1 + ch + K256[i] + w[i];
+        uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
+        uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+        uint32_t t2 = S0 + mj;
+        hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
+    }
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
+void oracle_sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
+    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                     0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    size_t full = len / 64;
+    for (size_t i = 0; i < full; ++i) sha256_block(h, msg + 64 * i);
+    uint8_t tail[128];
+    size_t rem = len - 64 * full;
+    memset(tail, 0, sizeof tail);
+    memcpy(tail, msg + 64 * full, rem);
+    tail[rem] = 0x80;
+    size_t tail_len = (rem + 9 <= 64) ? 64 : 128;
+    uint64_t bits = (uint64_t)len * 8;
+    for (int i = 0; i < 8; ++i) tail[tail_len - 1 - i] = (uint8_t)(bits >> (8 * i));
+    sha256_block(h, tail);
+    if (tail_len == 128) sha256_block(h, tail + 64);
+    for (int i = 0; i < 8; ++i) {
+        out[4 * i] = (uint8_t)(h[i] >> 24);
+        out[4 * i + 1] = (uint8_t)(h[i] >> 16);
+        out[4 * i + 2] = (uint8_t)(h[i] >> 8);
+        out[4 * i + 3] = (uint8_t)h[i];
+    }
+}
+
+static void narg_append(uint8_t** buf, size_t* len, size_t* cap, const uint8_t* bytes, size_t n) {
+    if (*len + n > *cap) {
+        size_t c = *cap ? *cap : 256;
+        while (c < *len + n) c *= 2;
+        *buf = (uint8_t*)realloc(*buf, c);
+        *cap = c;
+    }
+    memcpy(*buf + *len, bytes, n);
+    *len += n;
+}
+
+/* ---- seeded ------------------------------------------------------------------ */
+uint64_t oracle_splitmix_next(uint64_t* state) {
+    *state += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static void seeded_absorb(void* ctx, int kind, const uint8_t* bytes, size_t len) {
+    oracle_seeded* s = (oracle_seeded*)ctx;
+    s->absorbed_bytes += len;
+    if (kind == 1) narg_append(&s->narg, &s->narg_len, &s->narg_cap, bytes, len);
+}
+
+static void seeded_squeeze(void* ctx, uint8_t out[16]) {
+    oracle_seeded* s = (oracle_seeded*)ctx;
+    uint64_t w0 = oracle_splitmix_next(&s->state);
+    uint64_t w1 = oracle_splitmix_next(&s->state);
+    memcpy(out, &w0, 8);
+    memcpy(out + 8, &w1, 8);
+}
+
+void oracle_seeded_init(oracle_seeded* s, uint64_t seed) {
+    memset(s, 0, sizeof *s);
+    s->state = seed;
+}
+
+void oracle_seeded_free(oracle_seeded* s) {
+    free(s->narg);
+    s->narg = NULL;
+}
+
+oracle_transcript oracle_seeded_transcript(oracle_seeded* s) {
+    oracle_transcript t = {s, seeded_absorb, seeded_squeeze};
+    return t;
+}
+
+/* ---- SHA-256 duplex ----------------------------------------------------------
+ * absorb(kind, m): state = SHA256(state || "A" || kind || le64(len) || m)
+ * squeeze:         out = SHA256(state || "S")[0..16];
+ *                  state = SHA256(state || "R") */
+static void duplex_absorb(void* ctx, int kind, const uint8_t* bytes, size_t len) {
+    oracle_duplex* d = (oracle_duplex*)ctx;
+    size_t n = 32 + 2 + 8 + len;
+    uint8_t* buf = (uint8_t*)malloc(n);
+    memcpy(buf, d->state, 32);
+    buf[32] = 'A';
+    buf[33] = (uint8_t)kind;
+    uint64_t l = len;
+    memcpy(buf + 34, &l, 8);
+    if (len) memcpy(buf + 42, bytes, len);
+    oracle_sha256(buf, n, d->state);
+    free(buf);
+    if (kind == 1) narg_append(&d->narg, &d->narg_len, &d->narg_cap, bytes, len);
+}
+
+static void duplex_squeeze(void* ctx, uint8_t out[16]) {
+    oracle_duplex* d = (oracle_duplex*)ctx;
+    uint8_t buf[33], h[32];
+    memcpy(buf, d->state, 32);
+    buf[32] = 'S';
+    oracle_sha256(buf, 33, h);
+    memcpy(out, h, 16);
+    buf[32] = 'R';
+    oracle_sha256(buf, 33, d->state);
+}
+
+void oracle_duplex_init(oracle_duplex* d, const uint8_t* domain, size_t len) {
+    memset(d, 0, sizeof *d);
+    oracle_sha256(domain, len, d->state);
+}
+
+void oracle_duplex_free(oracle_duplex* d) {
+    free(d->narg);
+    d->narg = NULL;
+}
+
+oracle_transcript oracle_duplex_transcript(oracle_duplex* d) {
+    oracle_transcript t = {d, duplex_absorb, duplex_squeeze};
+    return t;
+}
