@@ -1,0 +1,165 @@
+"""Benchmark: the Spartan outer sumcheck over GF(2^128) on one MI355X.
+
+Workload: three multilinears A, B, C of 2^n GF(2^128) elements (default
+n = 24), uniform from the seed, an eq polynomial at a uniform point, and the
+degree-3 round polynomial of eq * (A * B + C) over all n rounds with
+challenges drawn from the seed.  A step is one complete proof through the C
+ABI (bitz_prove_outer_sumcheck) with the tables already resident in HBM.
+
+Prints one JSON line: throughput, p50/p99 proof latency, the roofline of the
+dominant kernel (timed live with HIP events on its stream), and the CPU
+oracle's rate on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SHA-256 blocks proved/sec (sumcheck stage) + p50/p99 proof latency, 1 MI355X")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    a, b, c = (rng.integers(0, 2**64, size=(1 << n, 2), dtype=np.uint64) for _ in range(3))
+    tau = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
+    return a, b, c, tau
+
+
+def cpu_baseline(n_sample: int, n_target: int) -> dict:
+    """The oracle (bit-serial multiply, the kernel the reference selects on
+    x86) proving one outer sumcheck at 2^n_sample on one host core, scaled
+    linearly in the table length to 2^n_target."""
+    from oracle import oracle as O
+    O.set_kernel("portable")
+    a, b, c, tau = make_inputs(n_sample, 7)
+    lo, hi = O.eq_table(tau[: n_sample // 2]), O.eq_table(tau[n_sample // 2:])
+    t0 = time.perf_counter()
+    rc, _ = O.outer_sumcheck(O.SeededTranscript(1), 0, lo, hi, a, b, c)
+    dt = time.perf_counter() - t0
+    assert rc == 0
+    per_proof = dt * (1 << (n_target - n_sample))
+    return {
+        "value": 1.0 / per_proof,
+        "unit": "outer-sumcheck proofs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle outer sumcheck, bit-serial portable multiply, 3 tables of 2^{n_sample} "
+                   f"elements, 1 thread: {dt:.2f} s, scaled x{1 << (n_target - n_sample)} to 2^{n_target}"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-sample-log-n", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(a proof shards by high index bits; see the design notes). Use --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import bitz_amd as G
+
+    n = args.log_n
+    G.init(0)
+    log(f"bench: outer sumcheck n={n}, generating inputs")
+    a, b, c, tau = make_inputs(n, args.seed)
+    lo, hi = G.eq_table(tau[: n // 2]), G.eq_table(tau[n // 2:])
+    prods = G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c))
+    del a, b, c
+    claim = G.outer_claim((lo, hi), prods)
+    log("bench: inputs resident in HBM; initial claim computed on device")
+
+    def step(i: int) -> float:
+        t = G.Transcript.seeded(args.seed + i)
+        t0 = time.perf_counter()
+        out = G.prove_outer_sumcheck(t, claim, (lo, hi), prods)
+        dt = time.perf_counter() - t0
+        assert out.round_polynomials.shape[0] == n
+        return dt
+
+    for i in range(args.warmup):
+        log(f"bench: warmup {i + 1}/{args.warmup}: {step(i) * 1e3:.2f} ms")
+    G.synchronize()
+    G.profile_enable(True)
+    times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        dt = step(1000 + i)
+        times.append(dt)
+        log(f"bench: step {i + 1}/{args.steps}: {dt * 1e3:.2f} ms")
+    G.synchronize()
+    wall = time.perf_counter() - t_start
+    fold = G.profile_read("outer_fold_next")
+    r0 = G.profile_read("outer_round0")
+
+    N = 1 << n
+    ms_per_step = wall / args.steps * 1e3
+    achieved = fold["bytes"] / (fold["ms"] * 1e-3) / 1e9 if fold["ms"] else 0.0
+    total_algo = 192.0 * N  # bytes the whole proof must move (read 3N, then fold rounds)
+    line = {
+        "metric": METRIC,
+        "value": args.steps / wall,
+        "unit": "outer-sumcheck proofs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (uniform random tables and point from the seed)",
+        "config": {
+            "workload": f"outer sumcheck eq*(A*B+C), 3 tables of 2^{n} GF(2^128) elements, "
+                        f"{n} rounds, challenges from the seed",
+            "log_n": n,
+            "elements_per_table": N,
+            "parallelism": "single GPU",
+        },
+        "elements_folded_per_s": 3 * (N - 1) * args.steps / wall,
+        "proof_hbm_gbs": total_algo * args.steps / wall / 1e9,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_outer_fold_next",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "launches": fold["launches"],
+            "kernel_ms_per_step": fold["ms"] / args.steps,
+            "round0_ms_per_step": r0["ms"] / args.steps,
+        },
+    }
+    if not args.no_cpu:
+        log(f"bench: CPU oracle sample at 2^{args.cpu_sample_log_n}")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log_n, n)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,593 @@
+"""bitz_amd — MI355X (gfx950) GF(2^128) sumcheck engine for BitZ's provers.
+
+Python mirror of the reference's prover interface for the sumcheck path,
+over the C ABI in include/bitz_hip.h (libbitz_hip.so, built in-tree by
+bitz_amd/build.py).  Names, argument meaning and error behaviour follow the
+reference crates:
+
+  poly:     eq_table, make_equality_factors, mle_fold, mle_evaluate
+            (crates/poly/src/{eq.rs,mle.rs})
+  spartan:  prove_outer_sumcheck, prove_inner_sumcheck,
+            PreparedConstraintMatrices.bind_and_batch
+            (crates/spartan/src/{sumcheck.rs,matrix.rs})
+  circuit:  MaterializedMTranspose.apply / transpose_inner
+            (crates/circuit/src/matrix_transpose.rs)
+  gkr:      grand_product, gpgkr_prove (crates/gkr/src/lib.rs)
+
+Field elements are numpy uint64 arrays of shape (n, 2) = [lo, hi], the byte
+layout of the reference's F128.  Every computation runs in the HIP library;
+there is no CPU fallback, and importing this package raises if the library
+is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbitz_hip.so")
+
+DEVICE = 1  # BITZ_DEVICE
+
+
+class BitzError(RuntimeError):
+    """A failure reported by the C ABI (status code + message)."""
+
+    NAMES = {
+        -1: "InvalidProductDimensions",
+        -2: "InvalidEqualityDimensions",
+        -3: "InvalidRoundClaim",
+        -4: "InvalidTerminalClaim",
+        -5: "InvalidRowPointLength",
+        -6: "TooManyChallenges",
+        -7: "InvalidArgument",
+        -8: "InvalidState",
+        -9: "InvalidPointLength",
+        -100: "DeviceError",
+        -101: "AllocationError",
+    }
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.variant = self.NAMES.get(code, f"Error{code}")
+        super().__init__(f"{self.variant} ({code}): {message}")
+
+
+class SumcheckError(BitzError):
+    """crates/spartan/src/sumcheck.rs:39-47 SumcheckError."""
+
+
+class F128(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libbitz_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python bitz_amd/build.py` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        P, U32, SZ, I32 = C.c_void_p, C.c_uint32, C.c_size_t, C.c_int32
+        sig = {
+            "bitz_init": (I32, [I32]),
+            "bitz_last_error": (C.c_char_p, []),
+            "bitz_device_alloc": (I32, [C.POINTER(C.c_void_p), SZ]),
+            "bitz_device_free": (I32, [P]),
+            "bitz_memcpy_h2d": (I32, [P, P, SZ]),
+            "bitz_memcpy_d2h": (I32, [P, P, SZ]),
+            "bitz_synchronize": (I32, []),
+            "bitz_profile_enable": (I32, [I32]),
+            "bitz_profile_read": (I32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_double)]),
+            "bitz_transcript_callbacks": (P, [P, P, P]),
+            "bitz_transcript_seeded": (P, [C.c_uint64]),
+            "bitz_transcript_duplex": (P, [P, SZ]),
+            "bitz_transcript_free": (None, [P]),
+            "bitz_transcript_absorb": (I32, [P, I32, P, SZ]),
+            "bitz_transcript_squeeze": (I32, [P, P]),
+            "bitz_transcript_narg": (SZ, [P, P, SZ]),
+            "bitz_mul_batch": (I32, [P, P, P, SZ, U32]),
+            "bitz_eq_table": (I32, [P, U32, P, U32]),
+            "bitz_mle_fold": (I32, [P, C.POINTER(U32), P, U32, U32]),
+            "bitz_mle_evaluate": (I32, [P, U32, P, U32, P, U32]),
+            "bitz_prove_outer_sumcheck": (I32, [P, F128, P, U32, P, U32, P, U32, P, U32, P, U32, U32,
+                                                P, P, P, P]),
+            "bitz_outer_claim": (I32, [P, U32, P, U32, P, P, P, U32, U32, P]),
+            "bitz_outer_begin": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32, P, U32, P, U32, U32]),
+            "bitz_outer_round0": (I32, [P, P]),
+            "bitz_outer_fold_next": (I32, [P, F128, P]),
+            "bitz_outer_finish": (I32, [P, P]),
+            "bitz_outer_tables": (I32, [P, P, P, P, C.POINTER(SZ)]),
+            "bitz_outer_free": (None, [P]),
+            "bitz_prove_inner_sumcheck": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
+            "bitz_inner_begin": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, U32]),
+            "bitz_inner_round0": (I32, [P, P]),
+            "bitz_inner_fold_next": (I32, [P, F128, P]),
+            "bitz_inner_finish": (I32, [P, P]),
+            "bitz_inner_free": (None, [P]),
+            "bitz_matrices_prepare": (I32, [C.POINTER(C.c_void_p), SZ, U32, U32, P, P, P]),
+            "bitz_bind_and_batch": (I32, [P, P, U32, F128, P, U32]),
+            "bitz_matrices_free": (None, [P]),
+            "bitz_mt_prepare": (I32, [C.POINTER(C.c_void_p), SZ, SZ, P, P]),
+            "bitz_mt_apply": (I32, [P, P, SZ, P, U32]),
+            "bitz_mt_transpose_inner": (I32, [P, P, SZ, P, SZ, P, U32]),
+            "bitz_mt_free": (None, [P]),
+            "bitz_gkr_build": (I32, [C.POINTER(C.c_void_p), P, U32, U32, P, U32]),
+            "bitz_gkr_prove": (I32, [P, P, P, U32, P, P]),
+            "bitz_gkr_layer": (I32, [P, U32, P, SZ, C.POINTER(SZ)]),
+            "bitz_gkr_free": (None, [P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, cls=BitzError) -> None:
+    if rc != 0:
+        raise cls(rc, (lib().bitz_last_error() or b"").decode())
+
+
+_initialized = False
+
+
+def init(device: int = 0) -> None:
+    global _initialized
+    if not _initialized:
+        _check(lib().bitz_init(device))
+        _initialized = True
+
+
+# ---------------------------------------------------------------- elements --
+def elems(a) -> np.ndarray:
+    """(n, 2) uint64 contiguous view/copy of a table of field elements."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a.reshape(-1, 2)
+
+
+def _p(a):
+    if isinstance(a, DeviceTable):
+        return C.c_void_p(a.ptr)
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def f128(x) -> F128:
+    if isinstance(x, F128):
+        return x
+    if isinstance(x, (tuple, list, np.ndarray)):
+        return F128(int(x[0]), int(x[1]))
+    x = int(x)
+    return F128(x & (2**64 - 1), x >> 64)
+
+
+def to_int(e) -> int:
+    if isinstance(e, F128):
+        return e.lo | (e.hi << 64)
+    return int(e[0]) | (int(e[1]) << 64)
+
+
+def _nvars(n: int) -> int:
+    if n <= 0 or n & (n - 1):
+        raise BitzError(-7, f"table length {n} is not a power of two")
+    return n.bit_length() - 1
+
+
+class DeviceTable:
+    """A table of field elements resident in HBM (bitz_device_alloc)."""
+
+    def __init__(self, host: np.ndarray | None = None, length: int | None = None):
+        init()
+        if host is not None:
+            host = elems(host)
+            length = len(host)
+        self.len = int(length)
+        ptr = C.c_void_p()
+        _check(lib().bitz_device_alloc(C.byref(ptr), max(self.len, 1) * 16))
+        self.ptr = ptr.value
+        if host is not None and self.len:
+            _check(lib().bitz_memcpy_h2d(C.c_void_p(self.ptr), _p(host), self.len * 16))
+
+    def to_host(self) -> np.ndarray:
+        out = np.zeros((self.len, 2), dtype=np.uint64)
+        if self.len:
+            _check(lib().bitz_memcpy_d2h(_p(out), C.c_void_p(self.ptr), self.len * 16))
+        return out
+
+    def __len__(self):
+        return self.len
+
+    def free(self):
+        if self.ptr:
+            lib().bitz_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _table(t):
+    """(pointer-able, length, flags) for a host array or DeviceTable."""
+    if isinstance(t, DeviceTable):
+        return t, t.len, DEVICE
+    t = elems(t)
+    return t, len(t), 0
+
+
+def profile_enable(on: bool = True) -> None:
+    """Time the sumcheck kernels with HIP events on their own streams."""
+    init()
+    _check(lib().bitz_profile_enable(1 if on else 0))
+
+
+def profile_read(kernel: str) -> dict:
+    ms, n, by = C.c_double(), C.c_uint64(), C.c_double()
+    _check(lib().bitz_profile_read(kernel.encode(), C.byref(ms), C.byref(n), C.byref(by)))
+    return {"ms": ms.value, "launches": n.value, "bytes": by.value}
+
+
+def synchronize() -> None:
+    _check(lib().bitz_synchronize())
+
+
+# ------------------------------------------------------------- transcripts --
+ABSORB_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.POINTER(C.c_uint8), C.c_size_t)
+SQUEEZE_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_uint8))
+
+
+class Transcript:
+    """Fiat-Shamir transcript behind the C ABI's bitz_transcript."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise BitzError(-7, "transcript creation failed")
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def seeded(cls, seed: int) -> "Transcript":
+        init()
+        return cls(lib().bitz_transcript_seeded(seed & (2**64 - 1)))
+
+    @classmethod
+    def duplex(cls, domain: bytes) -> "Transcript":
+        init()
+        buf = C.create_string_buffer(domain, len(domain))
+        return cls(lib().bitz_transcript_duplex(buf, len(domain)))
+
+    @classmethod
+    def callbacks(cls, absorb, squeeze) -> "Transcript":
+        """absorb(kind: int, data: bytes) -> None; squeeze() -> 16 bytes."""
+        init()
+
+        def _a(ctx, kind, ptr, n):
+            absorb(kind, C.string_at(ptr, n) if n else b"")
+
+        def _s(ctx, out):
+            data = squeeze()
+            C.memmove(out, data, 16)
+
+        a, s = ABSORB_FN(_a), SQUEEZE_FN(_s)
+        t = cls(lib().bitz_transcript_callbacks(None, C.cast(a, C.c_void_p), C.cast(s, C.c_void_p)))
+        t._keep = (a, s)
+        return t
+
+    def absorb(self, kind: int, data: bytes) -> None:
+        buf = C.create_string_buffer(data, max(len(data), 1))
+        _check(lib().bitz_transcript_absorb(self.h, kind, buf, len(data)))
+
+    def public_message(self, data) -> None:
+        self.absorb(0, bytes(elems(data)) if not isinstance(data, bytes) else data)
+
+    def prover_message(self, data) -> None:
+        self.absorb(1, bytes(elems(data)) if not isinstance(data, bytes) else data)
+
+    def squeeze(self) -> int:
+        out = F128()
+        _check(lib().bitz_transcript_squeeze(self.h, C.byref(out)))
+        return to_int(out)
+
+    def narg(self) -> bytes:
+        n = lib().bitz_transcript_narg(self.h, None, 0)
+        buf = (C.c_uint8 * max(n, 1))()
+        lib().bitz_transcript_narg(self.h, buf, n)
+        return bytes(buf[:n])
+
+    def __del__(self):
+        try:
+            lib().bitz_transcript_free(self.h)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------- poly ---
+def mul_batch(a, b) -> np.ndarray:
+    init()
+    a, b = elems(a), elems(b)
+    out = np.zeros_like(a)
+    _check(lib().bitz_mul_batch(_p(a), _p(b), _p(out), len(a), 0))
+    return out
+
+
+def eq_table(r) -> np.ndarray:
+    """poly::eq_table (crates/poly/src/eq.rs:63-99)."""
+    init()
+    r = elems(r)
+    out = np.zeros((1 << len(r), 2), dtype=np.uint64)
+    _check(lib().bitz_eq_table(_p(r), len(r), _p(out), 0))
+    return out
+
+
+def make_equality_factors(point):
+    """poly::make_equality_factors (eq.rs:107-115): split at len/2."""
+    point = elems(point)
+    split = len(point) // 2
+    return eq_table(point[:split]), eq_table(point[split:])
+
+
+def mle_fold(evaluations, r) -> np.ndarray:
+    """DenseMultilinearExtension::fold (mle.rs:136-156), returning the folded table."""
+    init()
+    ev = elems(evaluations).copy()
+    r = elems(r)
+    nv = C.c_uint32(_nvars(len(ev)))
+    _check(lib().bitz_mle_fold(_p(ev), C.byref(nv), _p(r), len(r), 0))
+    return ev[: 1 << nv.value]
+
+
+def mle_evaluate(evaluations, point) -> int:
+    """DenseMultilinearExtension::evaluate (mle.rs:158-172)."""
+    init()
+    ev, point = elems(evaluations), elems(point)
+    out = F128()
+    _check(lib().bitz_mle_evaluate(_p(ev), _nvars(len(ev)), _p(point), len(point), C.byref(out), 0))
+    return to_int(out)
+
+
+# ---------------------------------------------------------------- spartan ---
+@dataclass
+class R1csProductMles:
+    """sumcheck.rs:122-128: dense Boolean-row MLEs of Az, Bz, Cz."""
+    az: object
+    bz: object
+    cz: object
+
+
+@dataclass
+class OuterSumcheckOutput:
+    """sumcheck.rs:143-185: round polynomials [c0,c1,c2,c3], r_x, final claim,
+    and the terminal claims [Az(r_x), Bz(r_x), Cz(r_x)]."""
+    round_polynomials: np.ndarray  # (n, 4, 2)
+    eval_points: np.ndarray        # (n, 2)
+    final_claim: int
+    az_mle_claim: int
+    bz_mle_claim: int
+    cz_mle_claim: int
+
+
+@dataclass
+class InnerSumcheckOutput:
+    """sumcheck.rs:199-212."""
+    round_polynomials: np.ndarray  # (n, 3, 2)
+    eval_points: np.ndarray
+    final_claim: int
+    batched_matrix_evaluation: int
+    witness_evaluation: int
+
+
+def prove_outer_sumcheck(transcript: Transcript, initial_claim, eq_factors, products: R1csProductMles):
+    """prove_outer_sumcheck (crates/spartan/src/sumcheck.rs:252-384) on the GPU."""
+    init()
+    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
+    (az, na, fa), (bz, nb, fb), (cz, nc, fc) = (_table(t) for t in (products.az, products.bz, products.cz))
+    flags = fa
+    if not (fa == fb == fc):
+        raise BitzError(-7, "products must all be host arrays or all DeviceTables")
+    if flags & DEVICE:
+        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    n = _nvars(na)
+    rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
+    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    final = F128()
+    claims = np.zeros((3, 2), dtype=np.uint64)
+    rc = lib().bitz_prove_outer_sumcheck(
+        transcript.h, f128(initial_claim), _p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)),
+        _p(az), n, _p(bz), _nvars(nb), _p(cz), _nvars(nc), flags, _p(rounds), _p(points), C.byref(final),
+        _p(claims))
+    _check(rc, SumcheckError)
+    return OuterSumcheckOutput(rounds[: 4 * n].reshape(n, 4, 2), points[:n], to_int(final),
+                               to_int(claims[0]), to_int(claims[1]), to_int(claims[2]))
+
+
+def outer_claim(eq_factors, products: R1csProductMles) -> int:
+    """sum_x eq(x) (Az Bz + Cz)(x) on the device (sumcheck.rs:367-370)."""
+    init()
+    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
+    (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
+    if fa & DEVICE:
+        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    out = F128()
+    _check(lib().bitz_outer_claim(_p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)), _p(az),
+                                  _p(bz), _p(cz), _nvars(na), fa, C.byref(out)), SumcheckError)
+    return to_int(out)
+
+
+class OuterSumcheck:
+    """Per-round handle (bitz_outer_*): the caller keeps the transcript."""
+
+    def __init__(self, eq_factors, products: R1csProductMles):
+        init()
+        eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
+        (az, na, fa), (bz, nb, _), (cz, nc, _) = (_table(t) for t in (products.az, products.bz, products.cz))
+        if fa & DEVICE:
+            eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+        self._keep = (eq_low, eq_high, az, bz, cz)
+        h = C.c_void_p()
+        _check(lib().bitz_outer_begin(C.byref(h), _p(eq_low), _nvars(len(eq_low)), _p(eq_high),
+                                      _nvars(len(eq_high)), _p(az), _nvars(na), _p(bz), _nvars(nb), _p(cz),
+                                      _nvars(nc), fa), SumcheckError)
+        self.h = h
+
+    def round0(self):
+        out = np.zeros((3, 2), dtype=np.uint64)
+        _check(lib().bitz_outer_round0(self.h, _p(out)))
+        return out
+
+    def fold_next(self, r):
+        out = np.zeros((3, 2), dtype=np.uint64)
+        _check(lib().bitz_outer_fold_next(self.h, f128(r), _p(out)))
+        return out
+
+    def finish(self):
+        out = np.zeros((3, 2), dtype=np.uint64)
+        _check(lib().bitz_outer_finish(self.h, _p(out)))
+        return out
+
+    def tables(self):
+        n = C.c_size_t()
+        _check(lib().bitz_outer_tables(self.h, None, None, None, C.byref(n)))
+        a, b, c = (np.zeros((n.value, 2), dtype=np.uint64) for _ in range(3))
+        _check(lib().bitz_outer_tables(self.h, _p(a), _p(b), _p(c), C.byref(n)))
+        return a, b, c
+
+    def __del__(self):
+        try:
+            lib().bitz_outer_free(self.h)
+        except Exception:
+            pass
+
+
+def prove_inner_sumcheck(transcript: Transcript, initial_claim, batched_matrix, witness):
+    """prove_inner_sumcheck (crates/spartan/src/sumcheck.rs:442-552) on the GPU."""
+    init()
+    (d, nd, fd), (w, nw, fw) = _table(batched_matrix), _table(witness)
+    if fd != fw:
+        raise BitzError(-7, "matrix and witness must both be host arrays or both DeviceTables")
+    n = _nvars(nd)
+    rounds = np.zeros((max(n, 1) * 3, 2), dtype=np.uint64)
+    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    final = F128()
+    evals = np.zeros((2, 2), dtype=np.uint64)
+    rc = lib().bitz_prove_inner_sumcheck(transcript.h, f128(initial_claim), _p(d), n, _p(w), _nvars(nw), fd,
+                                         _p(rounds), _p(points), C.byref(final), _p(evals))
+    _check(rc, SumcheckError)
+    return InnerSumcheckOutput(rounds[: 3 * n].reshape(n, 3, 2), points[:n], to_int(final),
+                               to_int(evals[0]), to_int(evals[1]))
+
+
+class PreparedConstraintMatrices:
+    """crates/spartan/src/matrix.rs:54-64: A, B, C prepared once (CSR in,
+    column-major on device) for repeated bind_and_batch calls."""
+
+    def __init__(self, rows: int, num_row_vars: int, num_column_vars: int, csr):
+        """csr: three (row_starts[rows+1], columns, values) triples for A, B, C."""
+        init()
+        self.rows, self.num_row_vars, self.num_column_vars = rows, num_row_vars, num_column_vars
+        starts = [np.ascontiguousarray(m[0], dtype=np.uint64) for m in csr]
+        cols = [np.ascontiguousarray(m[1], dtype=np.uint32) for m in csr]
+        vals = [elems(m[2]) if len(m[2]) else np.zeros((1, 2), np.uint64) for m in csr]
+        self._keep = (starts, cols, vals)
+        arr_s = (C.c_void_p * 3)(*[s.ctypes.data for s in starts])
+        arr_c = (C.c_void_p * 3)(*[c.ctypes.data if len(c) else None for c in cols])
+        arr_v = (C.c_void_p * 3)(*[v.ctypes.data for v in vals])
+        h = C.c_void_p()
+        _check(lib().bitz_matrices_prepare(C.byref(h), rows, num_row_vars, num_column_vars, arr_s, arr_c, arr_v))
+        self.h = h
+
+    def bind_and_batch(self, row_point, rho) -> np.ndarray:
+        """matrix.rs:429-468: D(j) = sum_i eq(i, r_x)(A + rho B + rho^2 C)[i, j]."""
+        row_point = elems(row_point) if len(row_point) else np.zeros((0, 2), np.uint64)
+        out = np.zeros((1 << self.num_column_vars, 2), dtype=np.uint64)
+        _check(lib().bitz_bind_and_batch(self.h, _p(row_point), len(row_point), f128(rho), _p(out), 0))
+        return out
+
+    def __del__(self):
+        try:
+            lib().bitz_matrices_free(self.h)
+        except Exception:
+            pass
+
+
+class MaterializedMTranspose:
+    """crates/circuit/src/matrix_transpose.rs:282-441 (CSC of M)."""
+
+    def __init__(self, row_count: int, column_offsets, row_indices):
+        init()
+        self.offsets = np.ascontiguousarray(column_offsets, dtype=np.uint32)
+        self.rows = np.ascontiguousarray(row_indices, dtype=np.uint32)
+        self.row_count = row_count
+        self.columns = len(self.offsets) - 1
+        h = C.c_void_p()
+        _check(lib().bitz_mt_prepare(C.byref(h), row_count, self.columns, _p(self.offsets),
+                                     _p(self.rows) if len(self.rows) else None))
+        self.h = h
+
+    def apply(self, challenges) -> np.ndarray:
+        ch = elems(challenges) if len(challenges) else np.zeros((0, 2), np.uint64)
+        out = np.zeros((self.columns, 2), dtype=np.uint64)
+        _check(lib().bitz_mt_apply(self.h, _p(ch), len(ch), _p(out), 0))
+        return out
+
+    def transpose_inner(self, row_weights, column_weights) -> np.ndarray:
+        rw, cw = elems(row_weights), elems(column_weights)
+        out = np.zeros((self.columns, 2), dtype=np.uint64)
+        _check(lib().bitz_mt_transpose_inner(self.h, _p(rw), len(rw), _p(cw), len(cw), _p(out), 0))
+        return out
+
+    def __del__(self):
+        try:
+            lib().bitz_mt_free(self.h)
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------------- gkr ---
+class LayerWitnesses:
+    """The product tree's layers, resident on device (crates/gkr/src/lib.rs:896-904)."""
+
+    def __init__(self, handle, log_leaves, log_groups):
+        self.h, self.log_leaves, self.log_groups = handle, log_leaves, log_groups
+
+    def layer(self, j: int) -> np.ndarray:
+        n = C.c_size_t()
+        _check(lib().bitz_gkr_layer(self.h, j, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 2), dtype=np.uint64)
+        _check(lib().bitz_gkr_layer(self.h, j, _p(out), n.value, C.byref(n)))
+        return out
+
+    def __del__(self):
+        try:
+            lib().bitz_gkr_free(self.h)
+        except Exception:
+            pass
+
+
+def grand_product(leaves, log_groups: int):
+    """fold_layers (lib.rs:874-894) over dense leaves: returns (output, LayerWitnesses)."""
+    init()
+    lv, n, flags = _table(leaves)
+    log_leaves = _nvars(n)
+    out = np.zeros((1 << log_groups, 2), dtype=np.uint64)
+    h = C.c_void_p()
+    _check(lib().bitz_gkr_build(C.byref(h), _p(lv), log_leaves, log_groups, _p(out), flags))
+    return out, LayerWitnesses(h, log_leaves, log_groups)
+
+
+def gpgkr_prove(transcript: Transcript, point, witnesses: LayerWitnesses):
+    """gpgkr_prove (lib.rs:24-79) over dense layers: returns (leaf point, claim)."""
+    point = elems(point) if len(point) else np.zeros((0, 2), np.uint64)
+    leaf_point = np.zeros((witnesses.log_leaves, 2), dtype=np.uint64)
+    claim = F128()
+    _check(lib().bitz_gkr_prove(witnesses.h, transcript.h, _p(point), len(point), _p(leaf_point),
+                                C.byref(claim)))
+    return leaf_point, to_int(claim)

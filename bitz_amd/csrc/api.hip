@@ -1,0 +1,1176 @@
+// C ABI of the GF(2^128) sumcheck engine: device memory, per-round handles
+// and the full prover drivers (host loop around the HIP kernels).
+//
+// Each handle owns one HIP stream.  A round launches its kernels, copies the
+// 16..48-byte round sums into pinned host memory and synchronises; the
+// challenge for the next round then comes from the caller (handle API) or
+// the transcript (drivers).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bitz_hip.h"
+#include "kernels.hpp"
+#include "transcript.hpp"
+
+using bitz::kBlock;
+using bitz::Transcript;
+
+struct bitz_transcript {
+    Transcript t;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int32_t fail(int32_t code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(BITZ_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int32_t rc_ = (expr);      \
+        if (rc_ != BITZ_OK) return rc_; \
+    } while (0)
+
+inline f128 F(bitz_f128 x) { return f128{x.lo, x.hi}; }
+inline bitz_f128 B(f128 x) { return bitz_f128{x.lo, x.hi}; }
+
+// A device allocation owned by a handle.
+struct Dev {
+    void* p = nullptr;
+    size_t bytes = 0;
+    Dev() = default;
+    Dev(const Dev&) = delete;
+    Dev& operator=(const Dev&) = delete;
+    ~Dev() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int32_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            return fail(BITZ_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+        return BITZ_OK;
+    }
+    f128* f() const { return static_cast<f128*>(p); }
+    uint32_t* u32() const { return static_cast<uint32_t*>(p); }
+};
+
+// Pinned host staging for round sums.
+struct Pinned {
+    void* p = nullptr;
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+    int32_t alloc(size_t n) {
+        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+        }
+        return BITZ_OK;
+    }
+};
+
+unsigned grid_for(size_t work) {
+    size_t g = (work + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (unsigned)g;
+}
+
+int32_t check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(BITZ_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return BITZ_OK;
+}
+
+// Copies n elements from the caller (host or device) into a fresh device buffer.
+int32_t upload(Dev& d, const bitz_f128* src, size_t n, uint32_t flags, hipStream_t s) {
+    TRY(d.alloc(n * sizeof(f128)));
+    HIP_TRY(hipMemcpyAsync(d.p, src, n * sizeof(f128),
+                           (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    return BITZ_OK;
+}
+
+int32_t download(bitz_f128* dst, const f128* src, size_t n, uint32_t flags, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(f128),
+                           (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
+// eq(b, r) over b in {0,1}^k into out (2^k entries); r is a device array.
+// Up to 2^12 entries one workgroup doubles the table level by level; above,
+// two half tables are built that way and multiplied entrywise.
+int32_t eq_build(const f128* r_dev, int k, f128* out, Dev& scratch, hipStream_t s) {
+    if (k <= 12) {
+        hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev, k, out);
+        return check_launch();
+    }
+    const int kl = k / 2, kh = k - kl;
+    const size_t nl = (size_t)1 << kl, nh = (size_t)1 << kh;
+    if (scratch.bytes < (nl + nh) * sizeof(f128)) TRY(scratch.alloc((nl + nh) * sizeof(f128)));
+    f128* low = scratch.f();
+    f128* high = low + nl;
+    hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev, kl, low);
+    hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev + kl, kh, high);
+    const size_t n = (size_t)1 << k;
+    hipLaunchKernelGGL(bitz::k_eq_combine, dim3(grid_for(n)), dim3(kBlock), 0, s, low, kl, high, out, n);
+    return check_launch();
+}
+
+int log2_exact(size_t n) {
+    int k = 0;
+    while (((size_t)1 << k) < n) ++k;
+    return k;
+}
+
+struct StreamOwner {
+    hipStream_t s = nullptr;
+    ~StreamOwner() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    int32_t create() {
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        return BITZ_OK;
+    }
+};
+
+// Per-kernel timing: when enabled, launches on the sumcheck paths are
+// bracketed by HIP events on their own stream; durations are read back when
+// the round's sums are (the stream is synchronised there anyway).
+struct KernelStat {
+    double ms = 0, bytes = 0;
+    uint64_t launches = 0;
+};
+struct Profiler {
+    bool on = false;
+    std::mutex mu;
+    std::map<std::string, KernelStat> stats;
+} g_prof;
+
+struct Timed {
+    hipEvent_t start = nullptr, stop = nullptr;
+    const char* name = nullptr;
+    double bytes = 0;
+    ~Timed() {
+        if (start) (void)hipEventDestroy(start);
+        if (stop) (void)hipEventDestroy(stop);
+    }
+    // Records the start event if profiling is on.
+    void begin(hipStream_t s, const char* kernel, double algo_bytes) {
+        name = nullptr;
+        if (!g_prof.on) return;
+        if (!start) {
+            (void)hipEventCreate(&start);
+            (void)hipEventCreate(&stop);
+        }
+        name = kernel;
+        bytes = algo_bytes;
+        (void)hipEventRecord(start, s);
+    }
+    void end(hipStream_t s) {
+        if (name) (void)hipEventRecord(stop, s);
+    }
+    // Call after the stream has been synchronised.
+    void collect() {
+        if (!name) return;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, start, stop) == hipSuccess) {
+            std::lock_guard<std::mutex> lock(g_prof.mu);
+            KernelStat& k = g_prof.stats[name];
+            k.ms += ms;
+            k.bytes += bytes;
+            k.launches += 1;
+        }
+        name = nullptr;
+    }
+};
+
+}  // namespace
+
+// =============================================================== runtime ===
+extern "C" int32_t bitz_init(int32_t device) {
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipFree(nullptr));
+    return BITZ_OK;
+}
+
+extern "C" const char* bitz_last_error(void) { return g_err.c_str(); }
+
+extern "C" int32_t bitz_device_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return fail(BITZ_ERR_ARG, "null pointer");
+    if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) return fail(BITZ_ERR_ALLOC, "hipMalloc failed");
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_device_free(void* ptr) {
+    HIP_TRY(hipFree(ptr));
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_profile_enable(int32_t on) {
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    g_prof.stats.clear();
+    g_prof.on = on != 0;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_profile_read(const char* kernel, double* total_ms, uint64_t* launches,
+                                     double* algo_bytes) {
+    if (!kernel || !total_ms || !launches || !algo_bytes) return fail(BITZ_ERR_ARG, "null pointer");
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    auto it = g_prof.stats.find(kernel);
+    if (it == g_prof.stats.end()) {
+        *total_ms = 0;
+        *launches = 0;
+        *algo_bytes = 0;
+    } else {
+        *total_ms = it->second.ms;
+        *launches = it->second.launches;
+        *algo_bytes = it->second.bytes;
+    }
+    return BITZ_OK;
+}
+
+// =========================================================== transcripts ===
+extern "C" bitz_transcript* bitz_transcript_callbacks(void* ctx, bitz_absorb_fn absorb,
+                                                      bitz_squeeze_fn squeeze) {
+    if (!absorb || !squeeze) return nullptr;
+    auto* t = new bitz_transcript;
+    t->t.type = Transcript::Type::kCallbacks;
+    t->t.cb_ctx = ctx;
+    t->t.cb_absorb = absorb;
+    t->t.cb_squeeze = squeeze;
+    return t;
+}
+extern "C" bitz_transcript* bitz_transcript_seeded(uint64_t seed) {
+    auto* t = new bitz_transcript;
+    t->t.type = Transcript::Type::kSeeded;
+    t->t.seed = seed;
+    return t;
+}
+extern "C" bitz_transcript* bitz_transcript_duplex(const uint8_t* domain, size_t len) {
+    auto* t = new bitz_transcript;
+    t->t.type = Transcript::Type::kDuplex;
+    bitz::sha256(domain, len, t->t.state);
+    return t;
+}
+extern "C" void bitz_transcript_free(bitz_transcript* t) { delete t; }
+extern "C" int32_t bitz_transcript_absorb(bitz_transcript* t, int32_t kind, const uint8_t* bytes,
+                                          size_t len) {
+    if (!t || (len && !bytes)) return fail(BITZ_ERR_ARG, "null transcript or message");
+    t->t.absorb(kind, bytes, len);
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_transcript_squeeze(bitz_transcript* t, bitz_f128* out) {
+    if (!t || !out) return fail(BITZ_ERR_ARG, "null transcript or output");
+    *out = B(t->t.squeeze());
+    return BITZ_OK;
+}
+extern "C" size_t bitz_transcript_narg(const bitz_transcript* t, uint8_t* out, size_t cap) {
+    if (!t) return 0;
+    if (out) memcpy(out, t->t.narg.data(), std::min(cap, t->t.narg.size()));
+    return t->t.narg.size();
+}
+
+// ============================================================ primitives ===
+extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f128* out, size_t n,
+                                  uint32_t flags) {
+    if (n && (!a || !b || !out)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (n == 0) return BITZ_OK;
+    StreamOwner so;
+    TRY(so.create());
+    Dev da, db, dout;
+    TRY(upload(da, a, n, flags, so.s));
+    TRY(upload(db, b, n, flags, so.s));
+    TRY(dout.alloc(n * sizeof(f128)));
+    hipLaunchKernelGGL(bitz::k_mul, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
+    TRY(check_launch());
+    return download(out, dout.f(), n, flags, so.s);
+}
+
+extern "C" int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags) {
+    if (!out || (k && !r)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (k > 34) return fail(BITZ_ERR_ARG, "eq table too large");
+    StreamOwner so;
+    TRY(so.create());
+    Dev dr, dout, scratch;
+    TRY(upload(dr, r, k, flags, so.s));
+    const size_t n = (size_t)1 << k;
+    TRY(dout.alloc(n * sizeof(f128)));
+    TRY(eq_build(dr.f(), (int)k, dout.f(), scratch, so.s));
+    return download(out, dout.f(), n, flags, so.s);
+}
+
+extern "C" int32_t bitz_mle_fold(bitz_f128* evals, uint32_t* num_vars, const bitz_f128* r, uint32_t m,
+                                 uint32_t flags) {
+    if (!evals || !num_vars || (m && !r)) return fail(BITZ_ERR_ARG, "null pointer");
+    // mle.rs:136-142: too many challenges is rejected before mutating.
+    if (m > *num_vars) return fail(BITZ_ERR_TOO_MANY, "more challenges than variables");
+    if (m == 0) return BITZ_OK;
+    StreamOwner so;
+    TRY(so.create());
+    const size_t n = (size_t)1 << *num_vars;
+    Dev a, b;
+    TRY(upload(a, evals, n, flags, so.s));
+    TRY(b.alloc(n / 2 * sizeof(f128)));
+    std::vector<bitz_f128> rr(m);
+    HIP_TRY(hipMemcpy(rr.data(), r, m * sizeof(f128),
+                      (flags & BITZ_DEVICE) ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    f128* cur = a.f();
+    f128* nxt = b.f();
+    size_t len = n;
+    for (uint32_t i = 0; i < m; ++i) {
+        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(len / 2)), dim3(kBlock), 0, so.s, cur, nxt,
+                           len / 2, F(rr[i]));
+        TRY(check_launch());
+        std::swap(cur, nxt);
+        len /= 2;
+    }
+    TRY(download(evals, cur, len, flags, so.s));
+    *num_vars -= m;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, const bitz_f128* point,
+                                     uint32_t point_len, bitz_f128* out, uint32_t flags) {
+    if (!evals || !out || (point_len && !point)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (point_len != num_vars) return fail(BITZ_ERR_ARG, "point width differs from num_vars");
+    StreamOwner so;
+    TRY(so.create());
+    const size_t n = (size_t)1 << num_vars;
+    Dev a, b;
+    TRY(upload(a, evals, n, flags, so.s));
+    TRY(b.alloc(std::max<size_t>(n / 2, 1) * sizeof(f128)));
+    std::vector<bitz_f128> rr(point_len);
+    if (point_len)
+        HIP_TRY(hipMemcpy(rr.data(), point, point_len * sizeof(f128),
+                          (flags & BITZ_DEVICE) ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    f128* cur = a.f();
+    f128* nxt = b.f();
+    size_t len = n;
+    for (uint32_t i = 0; i < point_len; ++i) {
+        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(len / 2)), dim3(kBlock), 0, so.s, cur, nxt,
+                           len / 2, F(rr[i]));
+        TRY(check_launch());
+        std::swap(cur, nxt);
+        len /= 2;
+    }
+    bitz_f128 v;
+    HIP_TRY(hipMemcpyAsync(&v, cur, sizeof v, hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipStreamSynchronize(so.s));
+    if (flags & BITZ_DEVICE)
+        HIP_TRY(hipMemcpy(out, &v, sizeof v, hipMemcpyHostToDevice));
+    else
+        *out = v;
+    return BITZ_OK;
+}
+
+// ======================================================= outer sumcheck ===
+struct bitz_outer {
+    StreamOwner so;
+    int n = 0;
+    size_t len = 0, low_len = 0, high_len = 0;
+    int round = 0;
+    bool started = false, finished = false;
+    Dev in, P, Q;      // inputs (owned copy when given on host), ping-pong scratch
+    const f128* cur[3] = {nullptr, nullptr, nullptr};
+    int where = 0;     // 0 = inputs, 1 = P, 2 = Q
+    size_t capP = 0, capQ = 0;
+    Dev eqL[2], eqH[2];
+    int curL = 0, curH = 0;
+    Dev acc;
+    Pinned host;
+    Timed tm;
+
+    int32_t read_slot(int slot, bitz_f128* out3) {
+        HIP_TRY(hipMemcpyAsync(host.p, acc.u32() + 12 * slot, 48, hipMemcpyDeviceToHost, so.s));
+        HIP_TRY(hipStreamSynchronize(so.s));
+        memcpy(out3, host.p, 48);
+        tm.collect();
+        return BITZ_OK;
+    }
+};
+
+static int32_t outer_check(uint32_t low_vars, uint32_t high_vars, uint32_t az_vars, uint32_t bz_vars,
+                           uint32_t cz_vars) {
+    // sumcheck.rs:258-269, before any transcript mutation.
+    if (bz_vars != az_vars || cz_vars != az_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "product tables have different num_vars");
+    if ((uint64_t)low_vars + high_vars != az_vars)
+        return fail(BITZ_ERR_EQUALITY_DIMS, "eq factor num_vars do not sum to the product num_vars");
+    if (az_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
+                                    const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az,
+                                    uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars,
+                                    const bitz_f128* cz, uint32_t cz_vars, uint32_t flags) {
+    if (!out || !eq_low || !eq_high || !az || !bz || !cz) return fail(BITZ_ERR_ARG, "null pointer");
+    TRY(outer_check(low_vars, high_vars, az_vars, bz_vars, cz_vars));
+    auto* h = new bitz_outer;
+    auto cleanup = [&](int32_t rc) {
+        delete h;
+        return rc;
+    };
+    int32_t rc;
+    if ((rc = h->so.create())) return cleanup(rc);
+    hipStream_t s = h->so.s;
+    h->n = (int)az_vars;
+    const size_t N = (size_t)1 << az_vars;
+    h->len = N;
+    h->low_len = (size_t)1 << low_vars;
+    h->high_len = (size_t)1 << high_vars;
+    if (flags & BITZ_DEVICE) {
+        h->cur[0] = reinterpret_cast<const f128*>(az);
+        h->cur[1] = reinterpret_cast<const f128*>(bz);
+        h->cur[2] = reinterpret_cast<const f128*>(cz);
+    } else {
+        if ((rc = h->in.alloc(3 * N * sizeof(f128)))) return cleanup(rc);
+        const bitz_f128* src[3] = {az, bz, cz};
+        for (int t = 0; t < 3; ++t) {
+            if (hipMemcpyAsync(h->in.f() + t * N, src[t], N * sizeof(f128), hipMemcpyHostToDevice, s) !=
+                hipSuccess)
+                return cleanup(fail(BITZ_ERR_DEVICE, "upload of product tables failed"));
+            h->cur[t] = h->in.f() + t * N;
+        }
+    }
+    h->capP = std::max<size_t>(N / 2, 1);
+    h->capQ = std::max<size_t>(N / 4, 1);
+    if ((rc = h->P.alloc(3 * h->capP * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->Q.alloc(3 * h->capQ * sizeof(f128)))) return cleanup(rc);
+    if ((rc = upload(h->eqL[0], eq_low, h->low_len, flags, s))) return cleanup(rc);
+    if ((rc = upload(h->eqH[0], eq_high, h->high_len, flags, s))) return cleanup(rc);
+    if ((rc = h->eqL[1].alloc(std::max<size_t>(h->low_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->eqH[1].alloc(std::max<size_t>(h->high_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->acc.alloc((size_t)(h->n + 2) * 12 * sizeof(uint32_t)))) return cleanup(rc);
+    if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
+    if ((rc = h->host.alloc(64))) return cleanup(rc);
+    *out = h;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
+    if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
+    if (h->started) return fail(BITZ_ERR_STATE, "round0 called twice");
+    h->started = true;
+    if (h->n == 0) {
+        memset(co, 0, 48);
+        return BITZ_OK;
+    }
+    const size_t pairs = h->len / 2;
+    h->tm.begin(h->so.s, "outer_round0", 48.0 * (double)h->len);
+    hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, h->so.s, h->cur[0],
+                       h->cur[1], h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len),
+                       h->eqH[h->curH].f(), h->acc.u32());
+    h->tm.end(h->so.s);
+    TRY(check_launch());
+    return h->read_slot(0, co);
+}
+
+extern "C" int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 rb, bitz_f128* co) {
+    if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
+    if (!h->started || h->round >= h->n) return fail(BITZ_ERR_STATE, "no round left to fold");
+    hipStream_t s = h->so.s;
+    const f128 r = F(rb);
+    const size_t next_len = h->len / 2;
+    // Output buffers: P after the inputs or Q, Q after P.
+    Dev& dst = (h->where == 1) ? h->Q : h->P;
+    const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+    f128* out[3] = {dst.f(), dst.f() + cap, dst.f() + 2 * cap};
+    uint32_t* slot = h->acc.u32() + 12 * (h->round + 1);
+    bool has_next = false;
+    if (h->low_len > 1) {
+        // Phase 1 (sumcheck.rs:294-323): fold eq_low, then the products.
+        const size_t nl = h->low_len / 2;
+        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nl)), dim3(kBlock), 0, s, h->eqL[h->curL].f(),
+                           h->eqL[1 - h->curL].f(), nl, r);
+        h->curL ^= 1;
+        h->low_len = nl;
+        if (next_len > 1) {
+            h->tm.begin(s, "outer_fold_next", 72.0 * (double)h->len);
+            hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
+                               h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], next_len / 2, r,
+                               h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(), slot);
+            h->tm.end(s);
+            has_next = true;
+        } else {
+            hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                               out[0], out[1], out[2], (size_t)1, r);
+        }
+    } else {
+        // Phase 2 (sumcheck.rs:328-362): fold eq_high with the products.
+        const size_t nh = h->high_len / 2;
+        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nh)), dim3(kBlock), 0, s, h->eqH[h->curH].f(),
+                           h->eqH[1 - h->curH].f(), nh, r);
+        h->curH ^= 1;
+        h->high_len = nh;
+        if (nh == 1) {
+            hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                               out[0], out[1], out[2], (size_t)1, r);
+        } else {
+            h->tm.begin(s, "outer_fold_next", 72.0 * (double)h->len);
+            hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(nh / 2)), dim3(kBlock), 0, s,
+                               h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], nh / 2, r,
+                               h->eqL[h->curL].f(), 0, h->eqH[h->curH].f(), slot);
+            h->tm.end(s);
+            has_next = true;
+        }
+    }
+    TRY(check_launch());
+    for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+    h->where = (h->where == 1) ? 2 : 1;
+    h->len = next_len;
+    h->round += 1;
+    if (has_next) return h->read_slot(h->round, co);
+    memset(co, 0, 48);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_finish(bitz_outer* h, bitz_f128* claims) {
+    if (!h || !claims) return fail(BITZ_ERR_ARG, "null pointer");
+    if (h->round != h->n) return fail(BITZ_ERR_STATE, "rounds remain");
+    for (int t = 0; t < 3; ++t)
+        HIP_TRY(hipMemcpyAsync(claims + t, h->cur[t], sizeof(f128), hipMemcpyDeviceToHost, h->so.s));
+    HIP_TRY(hipStreamSynchronize(h->so.s));
+    h->finished = true;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_tables(bitz_outer* h, bitz_f128* az, bitz_f128* bz, bitz_f128* cz,
+                                     size_t* len) {
+    if (!h || !len) return fail(BITZ_ERR_ARG, "null pointer");
+    *len = h->len;
+    bitz_f128* dst[3] = {az, bz, cz};
+    for (int t = 0; t < 3; ++t)
+        if (dst[t])
+            HIP_TRY(hipMemcpyAsync(dst[t], h->cur[t], h->len * sizeof(f128), hipMemcpyDeviceToHost, h->so.s));
+    HIP_TRY(hipStreamSynchronize(h->so.s));
+    return BITZ_OK;
+}
+
+extern "C" void bitz_outer_free(bitz_outer* h) {
+    if (h) {
+        (void)hipStreamSynchronize(h->so.s);
+        delete h;
+    }
+}
+
+extern "C" int32_t bitz_outer_claim(const bitz_f128* eq_low, uint32_t low_vars, const bitz_f128* eq_high,
+                                    uint32_t high_vars, const bitz_f128* az, const bitz_f128* bz,
+                                    const bitz_f128* cz, uint32_t num_vars, uint32_t flags, bitz_f128* out) {
+    if (!eq_low || !eq_high || !az || !bz || !cz || !out) return fail(BITZ_ERR_ARG, "null pointer");
+    TRY(outer_check(low_vars, high_vars, num_vars, num_vars, num_vars));
+    StreamOwner so;
+    TRY(so.create());
+    const size_t n = (size_t)1 << num_vars;
+    Dev in, lo, hi, acc;
+    const f128* t[3];
+    if (flags & BITZ_DEVICE) {
+        t[0] = reinterpret_cast<const f128*>(az);
+        t[1] = reinterpret_cast<const f128*>(bz);
+        t[2] = reinterpret_cast<const f128*>(cz);
+    } else {
+        TRY(in.alloc(3 * n * sizeof(f128)));
+        const bitz_f128* src[3] = {az, bz, cz};
+        for (int k = 0; k < 3; ++k) {
+            HIP_TRY(hipMemcpyAsync(in.f() + k * n, src[k], n * sizeof(f128), hipMemcpyHostToDevice, so.s));
+            t[k] = in.f() + k * n;
+        }
+    }
+    TRY(upload(lo, eq_low, (size_t)1 << low_vars, flags, so.s));
+    TRY(upload(hi, eq_high, (size_t)1 << high_vars, flags, so.s));
+    TRY(acc.alloc(16));
+    HIP_TRY(hipMemsetAsync(acc.p, 0, 16, so.s));
+    hipLaunchKernelGGL(bitz::k_outer_claim, dim3(grid_for(n)), dim3(kBlock), 0, so.s, t[0], t[1], t[2], n, lo.f(),
+                       (int)low_vars, hi.f(), acc.u32());
+    TRY(check_launch());
+    bitz_f128 v;
+    HIP_TRY(hipMemcpyAsync(&v, acc.p, 16, hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipStreamSynchronize(so.s));
+    *out = v;
+    return BITZ_OK;
+}
+
+// sumcheck.rs:760-829: c1 = claim - c0 - (c0 + c2 + ... + cD), which in
+// characteristic 2 is claim + c2 + ... + cD; absorb [c0..cD], squeeze r,
+// claim <- g(r).
+static f128 recover_round(Transcript& t, f128& claim, const bitz_f128* without_linear, int degree,
+                          bitz_f128* round_out, bitz_f128* point_out) {
+    f128 c[4];
+    c[0] = F(without_linear[0]);
+    f128 c1 = claim;
+    for (int i = 2; i <= degree; ++i) {
+        c[i] = F(without_linear[i - 1]);
+        c1 = hostf::add(c1, c[i]);
+    }
+    c[1] = c1;
+    t.absorb_elems(0, c, (size_t)degree + 1);
+    const f128 r = t.squeeze();
+    claim = hostf::horner(c, degree + 1, r);
+    for (int i = 0; i <= degree; ++i) round_out[i] = B(c[i]);
+    *point_out = B(r);
+    return r;
+}
+
+extern "C" int32_t bitz_prove_outer_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
+                                             const bitz_f128* eq_low, uint32_t low_vars,
+                                             const bitz_f128* eq_high, uint32_t high_vars,
+                                             const bitz_f128* az, uint32_t az_vars, const bitz_f128* bz,
+                                             uint32_t bz_vars, const bitz_f128* cz, uint32_t cz_vars,
+                                             uint32_t flags, bitz_f128* round_polys,
+                                             bitz_f128* eval_points, bitz_f128* final_claim,
+                                             bitz_f128* claims) {
+    if (!t || !final_claim || !claims || (az_vars && (!round_polys || !eval_points)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    TRY(outer_check(low_vars, high_vars, az_vars, bz_vars, cz_vars));
+    bitz_outer* h = nullptr;
+    TRY(bitz_outer_begin(&h, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars,
+                         flags));
+    int32_t rc = BITZ_OK;
+    bitz_f128 co[3];
+    f128 claim = F(initial_claim);
+    if ((rc = bitz_outer_round0(h, co))) goto done;
+    for (uint32_t k = 0; k < az_vars; ++k) {
+        const f128 r = recover_round(t->t, claim, co, 3, round_polys + 4 * k, eval_points + k);
+        if ((rc = bitz_outer_fold_next(h, B(r), co))) goto done;
+    }
+    if ((rc = bitz_outer_finish(h, claims))) goto done;
+    {
+        f128 cl[3] = {F(claims[0]), F(claims[1]), F(claims[2])};
+        t->t.absorb_elems(0, cl, 3);  // sumcheck.rs:372
+    }
+    *final_claim = B(claim);
+done:
+    bitz_outer_free(h);
+    return rc;
+}
+
+// ======================================================= inner sumcheck ===
+struct bitz_inner {
+    StreamOwner so;
+    int n = 0;
+    size_t len = 0;
+    int round = 0;
+    bool started = false;
+    Dev in_d, in_w, P, Q;
+    const f128* cur[2] = {nullptr, nullptr};
+    int where = 0;
+    size_t capP = 0, capQ = 0;
+    Dev acc;
+    Pinned host;
+    Timed tm;
+};
+
+extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
+                                    const bitz_f128* witness, uint32_t witness_vars, uint32_t flags) {
+    if (!out || !matrix || !witness) return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    if (matrix_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
+    auto* h = new bitz_inner;
+    int32_t rc;
+    auto cleanup = [&](int32_t c) {
+        delete h;
+        return c;
+    };
+    if ((rc = h->so.create())) return cleanup(rc);
+    h->n = (int)matrix_vars;
+    const size_t N = (size_t)1 << matrix_vars;
+    h->len = N;
+    if (flags & BITZ_DEVICE) {
+        h->cur[0] = reinterpret_cast<const f128*>(matrix);
+        h->cur[1] = reinterpret_cast<const f128*>(witness);
+    } else {
+        if ((rc = upload(h->in_d, matrix, N, flags, h->so.s))) return cleanup(rc);
+        if ((rc = upload(h->in_w, witness, N, flags, h->so.s))) return cleanup(rc);
+        h->cur[0] = h->in_d.f();
+        h->cur[1] = h->in_w.f();
+    }
+    h->capP = std::max<size_t>(N / 2, 1);
+    h->capQ = std::max<size_t>(N / 4, 1);
+    if ((rc = h->P.alloc(2 * h->capP * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->Q.alloc(2 * h->capQ * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->acc.alloc((size_t)(h->n + 2) * 8 * sizeof(uint32_t)))) return cleanup(rc);
+    if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, h->so.s) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
+    if ((rc = h->host.alloc(64))) return cleanup(rc);
+    *out = h;
+    return BITZ_OK;
+}
+
+static int32_t inner_read(bitz_inner* h, int slot, bitz_f128* co) {
+    HIP_TRY(hipMemcpyAsync(h->host.p, h->acc.u32() + 8 * slot, 32, hipMemcpyDeviceToHost, h->so.s));
+    HIP_TRY(hipStreamSynchronize(h->so.s));
+    memcpy(co, h->host.p, 32);
+    h->tm.collect();
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* co) {
+    if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
+    if (h->started) return fail(BITZ_ERR_STATE, "round0 called twice");
+    h->started = true;
+    if (h->n == 0) {
+        memset(co, 0, 32);
+        return BITZ_OK;
+    }
+    const size_t pairs = h->len / 2;
+    h->tm.begin(h->so.s, "inner_round0", 32.0 * (double)h->len);
+    hipLaunchKernelGGL(bitz::k_inner_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, h->so.s, h->cur[0],
+                       h->cur[1], pairs, h->acc.u32());
+    h->tm.end(h->so.s);
+    TRY(check_launch());
+    return inner_read(h, 0, co);
+}
+
+extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* co) {
+    if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
+    if (!h->started || h->round >= h->n) return fail(BITZ_ERR_STATE, "no round left to fold");
+    hipStream_t s = h->so.s;
+    const f128 r = F(rb);
+    const size_t next_len = h->len / 2;
+    Dev& dst = (h->where == 1) ? h->Q : h->P;
+    const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+    f128* out[2] = {dst.f(), dst.f() + cap};
+    bool has_next = false;
+    if (next_len == 1) {
+        hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[0], out[0], (size_t)1, r);
+        hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[1], out[1], (size_t)1, r);
+    } else {
+        h->tm.begin(s, "inner_fold_next", 48.0 * (double)h->len);
+        hipLaunchKernelGGL(bitz::k_inner_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
+                           h->cur[0], h->cur[1], out[0], out[1], next_len / 2, r,
+                           h->acc.u32() + 8 * (h->round + 1));
+        h->tm.end(s);
+        has_next = true;
+    }
+    TRY(check_launch());
+    h->cur[0] = out[0];
+    h->cur[1] = out[1];
+    h->where = (h->where == 1) ? 2 : 1;
+    h->len = next_len;
+    h->round += 1;
+    if (has_next) return inner_read(h, h->round, co);
+    memset(co, 0, 32);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_inner_finish(bitz_inner* h, bitz_f128* evals) {
+    if (!h || !evals) return fail(BITZ_ERR_ARG, "null pointer");
+    if (h->round != h->n) return fail(BITZ_ERR_STATE, "rounds remain");
+    HIP_TRY(hipMemcpyAsync(evals, h->cur[0], sizeof(f128), hipMemcpyDeviceToHost, h->so.s));
+    HIP_TRY(hipMemcpyAsync(evals + 1, h->cur[1], sizeof(f128), hipMemcpyDeviceToHost, h->so.s));
+    HIP_TRY(hipStreamSynchronize(h->so.s));
+    return BITZ_OK;
+}
+
+extern "C" void bitz_inner_free(bitz_inner* h) {
+    if (h) {
+        (void)hipStreamSynchronize(h->so.s);
+        delete h;
+    }
+}
+
+extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
+                                             const bitz_f128* matrix, uint32_t matrix_vars,
+                                             const bitz_f128* witness, uint32_t witness_vars,
+                                             uint32_t flags, bitz_f128* round_polys,
+                                             bitz_f128* eval_points, bitz_f128* final_claim,
+                                             bitz_f128* evals) {
+    if (!t || !final_claim || !evals || (matrix_vars && (!round_polys || !eval_points)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    bitz_inner* h = nullptr;
+    TRY(bitz_inner_begin(&h, matrix, matrix_vars, witness, witness_vars, flags));
+    int32_t rc = BITZ_OK;
+    bitz_f128 co[2];
+    f128 claim = F(initial_claim);
+    if ((rc = bitz_inner_round0(h, co))) goto done;
+    for (uint32_t k = 0; k < matrix_vars; ++k) {
+        const f128 r = recover_round(t->t, claim, co, 2, round_polys + 3 * k, eval_points + k);
+        if ((rc = bitz_inner_fold_next(h, B(r), co))) goto done;
+    }
+    if ((rc = bitz_inner_finish(h, evals))) goto done;
+    *final_claim = B(claim);
+done:
+    bitz_inner_free(h);
+    return rc;
+}
+
+// ========================================================= matrices =======
+struct bitz_matrices {
+    StreamOwner so;
+    size_t rows = 0, rows_padded = 0, cols = 0;
+    int row_vars = 0, col_vars = 0;
+    Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
+};
+
+extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint32_t row_vars,
+                                         uint32_t col_vars, const uint64_t* const row_starts[3],
+                                         const uint32_t* const columns[3],
+                                         const bitz_f128* const values[3]) {
+    if (!out || !row_starts || !columns || !values) return fail(BITZ_ERR_ARG, "null pointer");
+    if (row_vars > 30 || col_vars > 31 || rows > ((size_t)1 << row_vars))
+        return fail(BITZ_ERR_ARG, "matrix shape out of range");
+    const size_t cols = (size_t)1 << col_vars;
+    // Column-major copy of all three matrices: counting sort by column.
+    std::vector<uint32_t> count(cols + 1, 0);
+    size_t nnz = 0;
+    for (int m = 0; m < 3; ++m) {
+        if (!row_starts[m]) return fail(BITZ_ERR_ARG, "null row_starts");
+        const size_t mn = row_starts[m][rows];
+        for (size_t k = 0; k < mn; ++k) {
+            if (columns[m][k] >= cols) return fail(BITZ_ERR_ARG, "column index out of range");
+            count[columns[m][k] + 1]++;
+        }
+        nnz += mn;
+    }
+    if (nnz >= 0xFFFFFFFFull) return fail(BITZ_ERR_ARG, "too many nonzeros");
+    for (size_t j = 0; j < cols; ++j) count[j + 1] += count[j];
+    std::vector<uint32_t> starts(count), rowv(nnz);
+    std::vector<uint8_t> whichv(nnz);
+    std::vector<bitz_f128> vals(std::max<size_t>(nnz, 1));
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for (int m = 0; m < 3; ++m)
+        for (size_t i = 0; i < rows; ++i)
+            for (uint64_t k = row_starts[m][i]; k < row_starts[m][i + 1]; ++k) {
+                const uint32_t pos = fill[columns[m][k]]++;
+                rowv[pos] = (uint32_t)i;
+                whichv[pos] = (uint8_t)m;
+                vals[pos] = values[m][k];
+            }
+    auto* h = new bitz_matrices;
+    auto cleanup = [&](int32_t c) {
+        delete h;
+        return c;
+    };
+    int32_t rc;
+    if ((rc = h->so.create())) return cleanup(rc);
+    h->rows = rows;
+    h->rows_padded = (size_t)1 << row_vars;
+    h->cols = cols;
+    h->row_vars = (int)row_vars;
+    h->col_vars = (int)col_vars;
+    if ((rc = h->col_starts.alloc(starts.size() * 4))) return cleanup(rc);
+    if ((rc = h->row_idx.alloc(std::max<size_t>(nnz, 1) * 4))) return cleanup(rc);
+    if ((rc = h->which.alloc(std::max<size_t>(nnz, 1)))) return cleanup(rc);
+    if ((rc = h->values.alloc(vals.size() * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->eq_rows.alloc(h->rows_padded * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->row_scale.alloc(3 * h->rows_padded * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->scales.alloc(3 * sizeof(f128)))) return cleanup(rc);
+    if ((rc = h->point.alloc(std::max<uint32_t>(row_vars, 1) * sizeof(f128)))) return cleanup(rc);
+    if (hipMemcpy(h->col_starts.p, starts.data(), starts.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && hipMemcpy(h->row_idx.p, rowv.data(), nnz * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nnz && hipMemcpy(h->which.p, whichv.data(), nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(h->values.p, vals.data(), vals.size() * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "matrix upload failed"));
+    *out = h;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
+                                       bitz_f128 rho, bitz_f128* out, uint32_t flags) {
+    if (!h || !out || (point_len && !row_point)) return fail(BITZ_ERR_ARG, "null pointer");
+    // matrix.rs:435-441: the row point must have num_row_vars coordinates.
+    if (point_len != (uint32_t)h->row_vars) return fail(BITZ_ERR_ROW_POINT, "row point length");
+    hipStream_t s = h->so.s;
+    if (point_len)
+        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, point_len * sizeof(f128),
+                               (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    TRY(eq_build(h->point.f(), h->row_vars, h->eq_rows.f(), h->scratch, s));
+    const f128 r = F(rho);
+    const f128 sc[3] = {hostf::one(), r, hostf::mul(r, r)};  // matrix.rs:533-545
+    HIP_TRY(hipMemcpyAsync(h->scales.p, sc, sizeof sc, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // sc lives on this stack frame
+    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s,
+                       h->eq_rows.f(), h->scales.f(), h->row_scale.f(), h->rows_padded);
+    Dev dout;
+    f128* target = reinterpret_cast<f128*>(out);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(dout.alloc(h->cols * sizeof(f128)));
+        target = dout.f();
+    }
+    hipLaunchKernelGGL(bitz::k_bind_columns, dim3(grid_for(h->cols)), dim3(kBlock), 0, s, h->col_starts.u32(),
+                       h->row_idx.u32(), static_cast<const uint8_t*>(h->which.p), h->values.f(),
+                       h->row_scale.f(), h->rows_padded, target, h->cols);
+    TRY(check_launch());
+    if (!(flags & BITZ_DEVICE))
+        HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
+extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
+
+// ================================================================ M^T =====
+struct bitz_mt {
+    StreamOwner so;
+    size_t row_count = 0, cols = 0;
+    Dev offsets, rows;
+};
+
+extern "C" int32_t bitz_mt_prepare(bitz_mt** out, size_t row_count, size_t columns,
+                                   const uint32_t* column_offsets, const uint32_t* row_indices) {
+    if (!out || !column_offsets) return fail(BITZ_ERR_ARG, "null pointer");
+    const size_t nnz = column_offsets[columns];
+    if (nnz && !row_indices) return fail(BITZ_ERR_ARG, "null row indices");
+    for (size_t k = 0; k < nnz; ++k)
+        if (row_indices[k] >= row_count) return fail(BITZ_ERR_ARG, "row index out of range");
+    for (size_t c = 0; c < columns; ++c)
+        if (column_offsets[c] > column_offsets[c + 1]) return fail(BITZ_ERR_ARG, "offsets not monotone");
+    auto* h = new bitz_mt;
+    auto cleanup = [&](int32_t c) {
+        delete h;
+        return c;
+    };
+    int32_t rc;
+    if ((rc = h->so.create())) return cleanup(rc);
+    h->row_count = row_count;
+    h->cols = columns;
+    if ((rc = h->offsets.alloc((columns + 1) * 4))) return cleanup(rc);
+    if ((rc = h->rows.alloc(std::max<size_t>(nnz, 1) * 4))) return cleanup(rc);
+    if (hipMemcpy(h->offsets.p, column_offsets, (columns + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && hipMemcpy(h->rows.p, row_indices, nnz * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return cleanup(fail(BITZ_ERR_DEVICE, "upload failed"));
+    *out = h;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_mt_apply(bitz_mt* h, const bitz_f128* challenges, size_t n, bitz_f128* out,
+                                 uint32_t flags) {
+    if (!h || !out || (n && !challenges)) return fail(BITZ_ERR_ARG, "null pointer");
+    // matrix_transpose.rs:346-351: one challenge per row of M.
+    if (n != h->row_count) return fail(BITZ_ERR_ARG, "challenge count != row count");
+    hipStream_t s = h->so.s;
+    Dev dc, dout;
+    if (n)
+        TRY(upload(dc, challenges, n, flags, s));
+    else
+        TRY(dc.alloc(sizeof(f128)));
+    f128* target = reinterpret_cast<f128*>(out);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(dout.alloc(h->cols * sizeof(f128)));
+        target = dout.f();
+    }
+    hipLaunchKernelGGL(bitz::k_mt_apply, dim3(grid_for(h->cols)), dim3(kBlock), 0, s, h->offsets.u32(),
+                       h->rows.u32(), dc.f(), target, h->cols);
+    TRY(check_launch());
+    return (flags & BITZ_DEVICE) ? (hipStreamSynchronize(s) == hipSuccess ? BITZ_OK : BITZ_ERR_DEVICE)
+                                 : download(out, target, h->cols, 0, s);
+}
+
+extern "C" int32_t bitz_mt_transpose_inner(bitz_mt* h, const bitz_f128* row_w, size_t n_row,
+                                           const bitz_f128* col_w, size_t n_col, bitz_f128* out,
+                                           uint32_t flags) {
+    if (!h || !out || !row_w || !col_w) return fail(BITZ_ERR_ARG, "null pointer");
+    // matrix_transpose.rs:386-392: power-of-two rows per column covering M.
+    if (n_row == 0 || (n_row & (n_row - 1)) || (n_col && n_row * n_col < h->row_count) || n_col == 0)
+        return fail(BITZ_ERR_ARG, "weight count mismatch");
+    const int log_rows = log2_exact(n_row);
+    // Every row index must land inside col_w.
+    if (((h->row_count ? h->row_count - 1 : 0) >> log_rows) >= n_col)
+        return fail(BITZ_ERR_ARG, "weight count mismatch");
+    hipStream_t s = h->so.s;
+    Dev dr, dc, dout;
+    TRY(upload(dr, row_w, n_row, flags, s));
+    TRY(upload(dc, col_w, n_col, flags, s));
+    f128* target = reinterpret_cast<f128*>(out);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(dout.alloc(h->cols * sizeof(f128)));
+        target = dout.f();
+    }
+    hipLaunchKernelGGL(bitz::k_mt_transpose_inner, dim3(grid_for(h->cols)), dim3(kBlock), 0, s,
+                       h->offsets.u32(), h->rows.u32(), dr.f(), log_rows, dc.f(), target, h->cols);
+    TRY(check_launch());
+    return (flags & BITZ_DEVICE) ? (hipStreamSynchronize(s) == hipSuccess ? BITZ_OK : BITZ_ERR_DEVICE)
+                                 : download(out, target, h->cols, 0, s);
+}
+
+extern "C" void bitz_mt_free(bitz_mt* h) { delete h; }
+
+// ================================================================== GKR ===
+struct bitz_gkr {
+    StreamOwner so;
+    int log_leaves = 0, log_groups = 0;
+    Dev store, eq, eq_scratch, point, acc;
+    std::vector<size_t> off;  // offset of layer j (0 = leaves)
+    Pinned host;
+    bool consumed = false;
+    Timed tm;
+};
+
+extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint32_t log_leaves,
+                                  uint32_t log_groups, bitz_f128* output, uint32_t flags) {
+    if (!out || !leaves || !output) return fail(BITZ_ERR_ARG, "null pointer");
+    if (log_groups > log_leaves || log_leaves > 32) return fail(BITZ_ERR_ARG, "bad tree shape");
+    auto* h = new bitz_gkr;
+    auto cleanup = [&](int32_t c) {
+        delete h;
+        return c;
+    };
+    int32_t rc;
+    if ((rc = h->so.create())) return cleanup(rc);
+    hipStream_t s = h->so.s;
+    h->log_leaves = (int)log_leaves;
+    h->log_groups = (int)log_groups;
+    size_t total = 0;
+    for (int j = 0; j <= (int)(log_leaves - log_groups); ++j) {
+        h->off.push_back(total);
+        total += (size_t)1 << (log_leaves - j);
+    }
+    if ((rc = h->store.alloc(total * sizeof(f128)))) return cleanup(rc);
+    if (hipMemcpyAsync(h->store.p, leaves, ((size_t)1 << log_leaves) * sizeof(f128),
+                       (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "leaf upload failed"));
+    for (int j = 0; j < (int)(log_leaves - log_groups); ++j) {
+        const size_t half = (size_t)1 << (log_leaves - j - 1);
+        hipLaunchKernelGGL(bitz::k_gkr_layer, dim3(grid_for(half)), dim3(kBlock), 0, s,
+                           h->store.f() + h->off[j], h->store.f() + h->off[j + 1], half);
+    }
+    if ((rc = check_launch())) return cleanup(rc);
+    if ((rc = download(output, h->store.f() + h->off.back(), (size_t)1 << log_groups, flags, s)))
+        return cleanup(rc);
+    if ((rc = h->host.alloc(64))) return cleanup(rc);
+    *out = h;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_t cap, size_t* len) {
+    if (!h || !len) return fail(BITZ_ERR_ARG, "null pointer");
+    if (j >= h->off.size()) return fail(BITZ_ERR_ARG, "no such layer");
+    const size_t n = (size_t)1 << (h->log_leaves - (int)j);
+    *len = n;
+    if (out) {
+        HIP_TRY(hipMemcpyAsync(out, h->store.f() + h->off[j], std::min(cap, n) * sizeof(f128),
+                               hipMemcpyDeviceToHost, h->so.s));
+        HIP_TRY(hipStreamSynchronize(h->so.s));
+    }
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point,
+                                  uint32_t point_len, bitz_f128* leaf_point, bitz_f128* leaf_claim) {
+    if (!h || !t || !leaf_point || !leaf_claim || (point_len && !point))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    // lib.rs:39-43: one coordinate per tree bit; EqTables needs at least one.
+    if ((int)point_len != h->log_groups || point_len == 0)
+        return fail(BITZ_ERR_POINT_LEN, "the output has one value per tree, so the point has a coordinate per tree bit");
+    if (h->consumed) return fail(BITZ_ERR_STATE, "layers already consumed");
+    h->consumed = true;
+    hipStream_t s = h->so.s;
+    const int layers = h->log_leaves - h->log_groups;
+    int total_rounds = 0;
+    for (int n = h->log_groups; n < h->log_leaves; ++n) total_rounds += n;
+    TRY(h->acc.alloc((size_t)(total_rounds + 1) * 8 * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s));
+    TRY(h->eq.alloc(((size_t)1 << std::max(h->log_leaves - 2, 0)) * sizeof(f128)));
+    TRY(h->point.alloc((size_t)(h->log_leaves + 1) * sizeof(f128)));
+    std::vector<f128> pt(point_len);
+    for (uint32_t i = 0; i < point_len; ++i) pt[i] = F(point[i]);
+    int slot = 0;
+    f128 claim{0, 0};
+    for (int j = layers - 1; j >= 0; --j) {
+        const int n = (int)pt.size();
+        f128* left = h->store.f() + h->off[j];
+        f128* right = left + ((size_t)1 << n);
+        // EqTables (lib.rs:663-693): only the largest, eq(z_<n-1, .), is built;
+        // each round halves it for the next.
+        if (n > 1) HIP_TRY(hipMemcpy(h->point.p, pt.data(), (n - 1) * sizeof(f128), hipMemcpyHostToDevice));
+        TRY(eq_build(h->point.f(), n - 1, h->eq.f(), h->eq_scratch, s));
+        std::vector<f128> next(n + 1);
+        f128 eq_bound = hostf::one();
+        f128 r_prev{0, 0};
+        for (int i = n - 1; i >= 0; --i) {
+            const f128 z = pt[i];
+            const int at_one = hostf::is_zero(z);  // sends_one (lib.rs:622-624)
+            uint32_t* acc = h->acc.u32() + 8 * slot;
+            if (i == n - 1) {
+                const size_t half = (size_t)1 << i;
+                h->tm.begin(s, "gkr_sums", 80.0 * (double)half);
+                hipLaunchKernelGGL(bitz::k_gkr_sums, dim3(grid_for(half)), dim3(kBlock), 0, s, left, right,
+                                   h->eq.f(), half, at_one, acc);
+            } else {
+                const size_t hh = (size_t)1 << i;
+                h->tm.begin(s, "gkr_bind_next", 240.0 * (double)hh);
+                hipLaunchKernelGGL(bitz::k_gkr_bind_next, dim3(grid_for(hh)), dim3(kBlock), 0, s, left, right,
+                                   h->eq.f(), hh, r_prev, at_one, acc);
+            }
+            h->tm.end(s);
+            TRY(check_launch());
+            HIP_TRY(hipMemcpyAsync(h->host.p, acc, 32, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            h->tm.collect();
+            ++slot;
+            f128 sums[2];
+            memcpy(sums, h->host.p, 32);
+            // lib.rs:142: prover_message([eq_bound * sum_b, eq_bound * sum_inf]).
+            f128 msg[2] = {hostf::mul(eq_bound, sums[0]), hostf::mul(eq_bound, sums[1])};
+            t->t.absorb_elems(1, msg, 2);
+            // Grinding is zero bits for these rounds at 100-bit security
+            // (crates/common/src/security.rs:25-32).
+            const f128 r = t->t.squeeze();
+            next[i] = r;
+            eq_bound = hostf::mul(eq_bound, f128{1 ^ r.lo ^ z.lo, r.hi ^ z.hi});  // eq_factor
+            r_prev = r;
+        }
+        hipLaunchKernelGGL(bitz::k_gkr_bind, dim3(1), dim3(kBlock), 0, s, left, right, (size_t)1, r_prev);
+        TRY(check_launch());
+        f128 lr[2];
+        HIP_TRY(hipMemcpyAsync(h->host.p, left, 16, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(h->host.p) + 16, right, 16, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        memcpy(lr, h->host.p, 32);
+        t->t.absorb_elems(1, lr, 2);  // lib.rs:154
+        const f128 tt = t->t.squeeze();
+        next[n] = tt;
+        claim = hostf::add(lr[0], hostf::mul(tt, hostf::add(lr[1], lr[0])));
+        pt = next;
+    }
+    for (size_t i = 0; i < pt.size(); ++i) leaf_point[i] = B(pt[i]);
+    *leaf_claim = B(claim);
+    return BITZ_OK;
+}
+
+extern "C" void bitz_gkr_free(bitz_gkr* h) {
+    if (h) {
+        (void)hipStreamSynchronize(h->so.s);
+        delete h;
+    }
+}

@@ -1,0 +1,211 @@
+// GF(2^128) = F2[X]/(X^128 + X^7 + X^2 + X + 1), non-reflected basis: bit i
+// of lo||hi is the coefficient of X^i (crates/field/src/gf128.rs:1-8, :52).
+//
+// Device side: elements live in four 32-bit VGPRs (w0 = bits 0..31, ...).
+// The carry-less product is a comb over the multiplier's bits with the
+// 256-bit accumulator shifted one bit per step; each masked XOR is a single
+// v_bitop3_b32.  Reduction folds the high 128 bits with X^128 = X^7+X^2+X+1.
+//
+// Host side: PCLMULQDQ products for the drivers' scalar work (round
+// polynomial reconstruction, Horner evaluation, eq factors, inverses).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+struct f128 {
+    uint64_t lo, hi;
+};
+
+#if defined(__HIPCC__)
+
+// (a & m) ^ c in one v_bitop3_b32: LUT index = a*4 + m*2 + c.
+__device__ __forceinline__ uint32_t xor_and(uint32_t c, uint32_t a, uint32_t m) {
+    return __builtin_amdgcn_bitop3_b32(a, m, c, 0x6A);
+}
+// a ^ b ^ c in one v_bitop3_b32.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+// High 32 bits of (hi:lo) << s, s in [1, 31]: v_alignbit_b32(hi, lo, 32 - s).
+__device__ __forceinline__ uint32_t shl_pair(uint32_t hi, uint32_t lo, int s) {
+    return __builtin_amdgcn_alignbit(hi, lo, 32 - s);
+}
+
+struct g4 {
+    uint32_t w[4];
+};
+
+__device__ __forceinline__ g4 g_load(const f128* p) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    g4 r;
+    r.w[0] = v.x; r.w[1] = v.y; r.w[2] = v.z; r.w[3] = v.w;
+    return r;
+}
+__device__ __forceinline__ void g_store(f128* p, const g4& a) {
+    *reinterpret_cast<uint4*>(p) = make_uint4(a.w[0], a.w[1], a.w[2], a.w[3]);
+}
+__device__ __forceinline__ g4 g_zero() {
+    g4 r;
+    r.w[0] = r.w[1] = r.w[2] = r.w[3] = 0;
+    return r;
+}
+__device__ __forceinline__ g4 g_one() {
+    g4 r = g_zero();
+    r.w[0] = 1;
+    return r;
+}
+__device__ __forceinline__ g4 g_from(f128 x) {
+    g4 r;
+    r.w[0] = (uint32_t)x.lo; r.w[1] = (uint32_t)(x.lo >> 32);
+    r.w[2] = (uint32_t)x.hi; r.w[3] = (uint32_t)(x.hi >> 32);
+    return r;
+}
+__device__ __forceinline__ g4 g_add(const g4& a, const g4& b) {
+    g4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.w[i] = a.w[i] ^ b.w[i];
+    return r;
+}
+__device__ __forceinline__ g4 g_add3(const g4& a, const g4& b, const g4& c) {
+    g4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.w[i] = xor3(a.w[i], b.w[i], c.w[i]);
+    return r;
+}
+
+// Unreduced 256-bit value: eight words, least significant first.
+struct g8 {
+    uint32_t w[8];
+};
+__device__ __forceinline__ g8 w_zero() {
+    g8 r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r.w[i] = 0;
+    return r;
+}
+__device__ __forceinline__ void w_acc(g8& acc, const g8& x) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc.w[i] ^= x.w[i];
+}
+
+// Carry-less 128x128 -> 256 product, accumulated into `c` (c ^= a*b).
+// Comb: for bit j = 31..0 of every multiplier word, shift the accumulator
+// left one bit, then XOR in a shifted by 32*w wherever bit j of b.w[w] is set.
+__device__ __forceinline__ void clmul_acc(g8& c, const g4& a, const g4& b) {
+    g8 t = w_zero();
+#pragma unroll 4
+    for (int j = 31; j >= 0; --j) {
+#pragma unroll
+        for (int i = 7; i >= 1; --i) t.w[i] = shl_pair(t.w[i], t.w[i - 1], 1);
+        t.w[0] <<= 1;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)b.w[w], j, 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) t.w[w + i] = xor_and(t.w[w + i], a.w[i], m);
+        }
+    }
+    w_acc(c, t);
+}
+
+__device__ __forceinline__ g8 clmul(const g4& a, const g4& b) {
+    g8 c = w_zero();
+    clmul_acc(c, a, b);
+    return c;
+}
+
+// Reduce modulo X^128 + X^7 + X^2 + X + 1 (the fold of portable.rs:50-57,
+// in 32-bit words): H = high 128 bits, H*g = H ^ H<<1 ^ H<<2 ^ H<<7 spills at
+// most 7 bits above X^128, folded once more into word 0.
+__device__ __forceinline__ g4 g_reduce(const g8& c) {
+    const uint32_t h0 = c.w[4], h1 = c.w[5], h2 = c.w[6], h3 = c.w[7];
+    g4 r;
+    r.w[0] = xor3(c.w[0] ^ h0, h0 << 1, xor3(h0 << 2, h0 << 7, 0u));
+    r.w[1] = xor3(c.w[1] ^ h1, shl_pair(h1, h0, 1), xor3(shl_pair(h1, h0, 2), shl_pair(h1, h0, 7), 0u));
+    r.w[2] = xor3(c.w[2] ^ h2, shl_pair(h2, h1, 1), xor3(shl_pair(h2, h1, 2), shl_pair(h2, h1, 7), 0u));
+    r.w[3] = xor3(c.w[3] ^ h3, shl_pair(h3, h2, 1), xor3(shl_pair(h3, h2, 2), shl_pair(h3, h2, 7), 0u));
+    const uint32_t s = xor3(h3 >> 31, h3 >> 30, h3 >> 25);  // < 2^7
+    r.w[0] ^= xor3(s, s << 1, xor3(s << 2, s << 7, 0u));
+    return r;
+}
+
+__device__ __forceinline__ g4 g_mul(const g4& a, const g4& b) { return g_reduce(clmul(a, b)); }
+
+#endif  // __HIPCC__
+
+// ---------------------------------------------------------------- host -----
+// Host-only scalar arithmetic.  The device compilation pass parses (but never
+// emits) host code, so it sees a shift/XOR body instead of PCLMULQDQ.
+#if !defined(__HIP_DEVICE_COMPILE__)
+#include <wmmintrin.h>
+#endif
+
+namespace hostf {
+
+inline f128 add(f128 a, f128 b) { return f128{a.lo ^ b.lo, a.hi ^ b.hi}; }
+inline bool eq(f128 a, f128 b) { return a.lo == b.lo && a.hi == b.hi; }
+inline bool is_zero(f128 a) { return (a.lo | a.hi) == 0; }
+inline f128 one() { return f128{1, 0}; }
+
+inline f128 reduce4(uint64_t r0, uint64_t r1, uint64_t r2, uint64_t r3) {
+    auto times_g = [](uint64_t w, uint64_t& lo, uint64_t& hi) {
+        lo = w ^ (w << 1) ^ (w << 2) ^ (w << 7);
+        hi = (w >> 63) ^ (w >> 62) ^ (w >> 57);
+    };
+    uint64_t a0, a1, b0, b1;
+    times_g(r2, a0, a1);
+    times_g(r3, b0, b1);
+    return f128{r0 ^ a0 ^ b1 ^ (b1 << 1) ^ (b1 << 2) ^ (b1 << 7), r1 ^ a1 ^ b0};
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+inline f128 mul(f128 a, f128 b) {
+    uint64_t r[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 128; ++i)
+        if (((i < 64 ? b.lo >> i : b.hi >> (i - 64)) & 1)) {
+            const int w = i / 64, s = i % 64;
+            r[w] ^= a.lo << s;
+            r[w + 1] ^= (s ? a.lo >> (64 - s) : 0) ^ (a.hi << s);
+            r[w + 2] ^= s ? a.hi >> (64 - s) : 0;
+        }
+    return reduce4(r[0], r[1], r[2], r[3]);
+}
+#else
+__attribute__((target("pclmul,sse2"))) inline f128 mul(f128 a, f128 b) {
+    __m128i x = _mm_set_epi64x((long long)a.hi, (long long)a.lo);
+    __m128i y = _mm_set_epi64x((long long)b.hi, (long long)b.lo);
+    __m128i l = _mm_clmulepi64_si128(x, y, 0x00);
+    __m128i h = _mm_clmulepi64_si128(x, y, 0x11);
+    __m128i m = _mm_xor_si128(_mm_clmulepi64_si128(x, y, 0x10), _mm_clmulepi64_si128(x, y, 0x01));
+    uint64_t r0 = (uint64_t)_mm_cvtsi128_si64(l);
+    uint64_t r1 = (uint64_t)_mm_cvtsi128_si64(_mm_unpackhi_epi64(l, l)) ^ (uint64_t)_mm_cvtsi128_si64(m);
+    uint64_t r2 = (uint64_t)_mm_cvtsi128_si64(h) ^ (uint64_t)_mm_cvtsi128_si64(_mm_unpackhi_epi64(m, m));
+    uint64_t r3 = (uint64_t)_mm_cvtsi128_si64(_mm_unpackhi_epi64(h, h));
+    return reduce4(r0, r1, r2, r3);
+}
+#endif
+
+inline f128 square(f128 a) { return mul(a, a); }
+
+// a^(2^128 - 2) = a^-1 (a != 0).
+inline f128 inv(f128 a) {
+    f128 acc = one(), p = a;
+    for (int i = 1; i < 128; ++i) {
+        p = square(p);
+        acc = mul(acc, p);
+    }
+    return acc;
+}
+
+// Horner from the top coefficient (crates/spartan/src/sumcheck.rs:784-794).
+inline f128 horner(const f128* c, int n, f128 x) {
+    f128 v{0, 0};
+    for (int i = n - 1; i >= 0; --i) v = add(mul(v, x), c[i]);
+    return v;
+}
+
+}  // namespace hostf

@@ -1,0 +1,352 @@
+// HIP kernels for the GF(2^128) sumcheck path on gfx950 (MI355X).
+//
+// Tables are arrays of 16-byte elements (the reference's F128 layout), read
+// and written as one dwordx4 per lane so every wave moves 1 KiB contiguous.
+// Round sums are XOR-reduced inside each wave with lane shuffles, across the
+// waves of a workgroup through LDS, and across workgroups with one 32-bit
+// atomic XOR per word into a per-round slot (XOR is associative and
+// commutative, so the result does not depend on arrival order).
+#include <hip/hip_runtime.h>
+
+#include "gf128.hpp"
+#include "kernels.hpp"
+
+namespace bitz {
+
+// ---------------------------------------------------------------- helpers --
+template <int K>
+__device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
+    __shared__ uint32_t red[kBlock / 64][4 * K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t x = v[k].w[i];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
+            v[k].w[i] = x;
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * K) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) x ^= red[w][threadIdx.x];
+        if (x) atomicXor(dst + threadIdx.x, x);
+    }
+}
+
+__device__ __forceinline__ g4 interp(const g4& z, const g4& o, const g4& r) {
+    return g_add(z, g_mul(r, g_add(o, z)));
+}
+
+// EqualityPairs::pair (crates/spartan/src/sumcheck.rs:835-871).
+__device__ __forceinline__ void eq_pair(const f128* low, int low_bits, const f128* high,
+                                        size_t p, g4& e0, g4& e1) {
+    if (low_bits == 0) {
+        const g4 s = g_load(low);
+        e0 = g_mul(s, g_load(high + 2 * p));
+        e1 = g_mul(s, g_load(high + 2 * p + 1));
+        return;
+    }
+    const size_t lp = p & ((((size_t)1) << (low_bits - 1)) - 1);
+    const g4 hw = g_load(high + (p >> (low_bits - 1)));
+    e0 = g_mul(g_load(low + 2 * lp), hw);
+    e1 = g_mul(g_load(low + 2 * lp + 1), hw);
+}
+
+// cubic_contribution (sumcheck.rs:724-744), accumulated unreduced.
+__device__ __forceinline__ void cubic_acc(const g4& e0, const g4& e1, const g4& a0, const g4& a1,
+                                          const g4& b0, const g4& b1, const g4& c0, const g4& c1,
+                                          g8& s0, g8& s2, g8& s3) {
+    const g4 ed = g_add(e1, e0), ad = g_add(a1, a0), bd = g_add(b1, b0), cd = g_add(c1, c0);
+    const g4 p00 = g_mul(a0, b0);
+    const g4 pdd = g_mul(ad, bd);
+    const g4 p11 = g_mul(a1, b1);
+    // a0*bd + ad*b0 = a1*b1 + a0*b0 + ad*bd (characteristic 2)
+    const g4 r0 = g_add(p00, c0);
+    const g4 rl = g_add3(p11, p00, g_add(pdd, cd));
+    clmul_acc(s0, e0, r0);
+    clmul_acc(s2, e0, pdd);
+    clmul_acc(s2, ed, rl);
+    clmul_acc(s3, ed, pdd);
+}
+
+// ------------------------------------------------------------ primitives --
+__global__ __launch_bounds__(kBlock) void k_mul(const f128* a, const f128* b, f128* out, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        g_store(out + i, g_mul(g_load(a + i), g_load(b + i)));
+}
+
+// In-place level doubling of a small eq table by one workgroup
+// (crates/poly/src/eq.rs:63-99).
+__global__ __launch_bounds__(kBlock) void k_eq_small(const f128* r, int k, f128* table) {
+    if (threadIdx.x == 0) g_store(table, g_one());
+    for (size_t j = 1 + threadIdx.x; j < ((size_t)1 << k); j += blockDim.x) g_store(table + j, g_zero());
+    __syncthreads();
+    for (int i = 0; i < k; ++i) {
+        const size_t half = (size_t)1 << i;
+        const g4 ri = g_load(r + i);
+        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
+            const g4 parent = g_load(table + j);
+            const g4 one = g_mul(parent, ri);
+            g_store(table + j, g_add(parent, one));
+            g_store(table + j + half, one);
+        }
+        __syncthreads();
+    }
+}
+
+// table[j] = low[j & mask] * high[j >> low_bits]: every entry is the product
+// of its per-coordinate factors, split into a low and a high group.
+__global__ __launch_bounds__(kBlock) void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n) {
+    const size_t mask = ((size_t)1 << low_bits) - 1;
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n;
+         j += (size_t)gridDim.x * blockDim.x)
+        g_store(out + j, g_mul(g_load(low + (j & mask)), g_load(high + (j >> low_bits))));
+}
+
+// f'(b) = f(0,b) + r (f(1,b) - f(0,b)) on adjacent pairs (mle.rs:240-258).
+__global__ __launch_bounds__(kBlock) void k_fold(const f128* in, f128* out, size_t out_len, f128 r) {
+    const g4 rr = g_from(r);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < out_len;
+         i += (size_t)gridDim.x * blockDim.x)
+        g_store(out + i, interp(g_load(in + 2 * i), g_load(in + 2 * i + 1), rr));
+}
+
+__global__ __launch_bounds__(kBlock) void k_fold3(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
+                        size_t out_len, f128 r) {
+    const g4 rr = g_from(r);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < out_len;
+         i += (size_t)gridDim.x * blockDim.x) {
+        g_store(ao + i, interp(g_load(a + 2 * i), g_load(a + 2 * i + 1), rr));
+        g_store(bo + i, interp(g_load(b + 2 * i), g_load(b + 2 * i + 1), rr));
+        g_store(co + i, interp(g_load(c + 2 * i), g_load(c + 2 * i + 1), rr));
+    }
+}
+
+// ------------------------------------------------------- outer sumcheck --
+// Round sums [c0, c2, c3] over adjacent pairs of the current tables
+// (compute_coefficients_without_linear, sumcheck.rs:874-889).
+__global__ __launch_bounds__(kBlock) void k_outer_round0(const f128* a, const f128* b, const f128* c, size_t pairs,
+                               const f128* low, int low_bits, const f128* high, uint32_t* acc) {
+    g8 s0 = w_zero(), s2 = w_zero(), s3 = w_zero();
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pairs;
+         p += (size_t)gridDim.x * blockDim.x) {
+        g4 e0, e1;
+        eq_pair(low, low_bits, high, p, e0, e1);
+        cubic_acc(e0, e1, g_load(a + 2 * p), g_load(a + 2 * p + 1), g_load(b + 2 * p),
+                  g_load(b + 2 * p + 1), g_load(c + 2 * p), g_load(c + 2 * p + 1), s0, s2, s3);
+    }
+    g4 v[3] = {g_reduce(s0), g_reduce(s2), g_reduce(s3)};
+    block_xor_to_global<3>(v, acc);
+}
+
+// Fold A, B, C at r and accumulate the next round's [c0, c2, c3] from the
+// freshly folded pairs (fold_products_and_compute_next, sumcheck.rs:963-1025).
+// One output pair = four input elements per table.
+__global__ __launch_bounds__(kBlock) void k_outer_fold_next(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo,
+                                  f128* co, size_t out_pairs, f128 r, const f128* low,
+                                  int low_bits, const f128* high, uint32_t* acc) {
+    const g4 rr = g_from(r);
+    g8 s0 = w_zero(), s2 = w_zero(), s3 = w_zero();
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < out_pairs;
+         p += (size_t)gridDim.x * blockDim.x) {
+        const g4 a0 = interp(g_load(a + 4 * p), g_load(a + 4 * p + 1), rr);
+        const g4 a1 = interp(g_load(a + 4 * p + 2), g_load(a + 4 * p + 3), rr);
+        const g4 b0 = interp(g_load(b + 4 * p), g_load(b + 4 * p + 1), rr);
+        const g4 b1 = interp(g_load(b + 4 * p + 2), g_load(b + 4 * p + 3), rr);
+        const g4 c0 = interp(g_load(c + 4 * p), g_load(c + 4 * p + 1), rr);
+        const g4 c1 = interp(g_load(c + 4 * p + 2), g_load(c + 4 * p + 3), rr);
+        g_store(ao + 2 * p, a0); g_store(ao + 2 * p + 1, a1);
+        g_store(bo + 2 * p, b0); g_store(bo + 2 * p + 1, b1);
+        g_store(co + 2 * p, c0); g_store(co + 2 * p + 1, c1);
+        g4 e0, e1;
+        eq_pair(low, low_bits, high, p, e0, e1);
+        cubic_acc(e0, e1, a0, a1, b0, b1, c0, c1, s0, s2, s3);
+    }
+    g4 v[3] = {g_reduce(s0), g_reduce(s2), g_reduce(s3)};
+    block_xor_to_global<3>(v, acc);
+}
+
+// The outer sumcheck's claim sum_x eq(x) (A(x) B(x) + C(x)) with
+// eq(x) = low[x & mask] * high[x >> low_bits] (sumcheck.rs:367-370 checks the
+// prover's claim against exactly this sum).
+__global__ __launch_bounds__(kBlock) void k_outer_claim(const f128* a, const f128* b, const f128* c, size_t n,
+                                                        const f128* low, int low_bits, const f128* high,
+                                                        uint32_t* acc) {
+    const size_t mask = ((size_t)1 << low_bits) - 1;
+    g8 s = w_zero();
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const g4 e = g_mul(g_load(low + (x & mask)), g_load(high + (x >> low_bits)));
+        clmul_acc(s, e, g_add(g_mul(g_load(a + x), g_load(b + x)), g_load(c + x)));
+    }
+    g4 v[1] = {g_reduce(s)};
+    block_xor_to_global<1>(v, acc);
+}
+
+// ------------------------------------------------------- inner sumcheck --
+// [c0, c2] = [sum m0 w0, sum (m1 - m0)(w1 - w0)] (sumcheck.rs:555-609).
+__global__ __launch_bounds__(kBlock) void k_inner_round0(const f128* d, const f128* w, size_t pairs, uint32_t* acc) {
+    g8 s0 = w_zero(), s2 = w_zero();
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pairs;
+         p += (size_t)gridDim.x * blockDim.x) {
+        const g4 m0 = g_load(d + 2 * p), m1 = g_load(d + 2 * p + 1);
+        const g4 w0 = g_load(w + 2 * p), w1 = g_load(w + 2 * p + 1);
+        clmul_acc(s0, m0, w0);
+        clmul_acc(s2, g_add(m1, m0), g_add(w1, w0));
+    }
+    g4 v[2] = {g_reduce(s0), g_reduce(s2)};
+    block_xor_to_global<2>(v, acc);
+}
+
+// fold_and_compute_next_inner_round_coefficients_without_linear
+// (sumcheck.rs:641-688).
+__global__ __launch_bounds__(kBlock) void k_inner_fold_next(const f128* d, const f128* w, f128* dout, f128* wout,
+                                  size_t out_pairs, f128 r, uint32_t* acc) {
+    const g4 rr = g_from(r);
+    g8 s0 = w_zero(), s2 = w_zero();
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < out_pairs;
+         p += (size_t)gridDim.x * blockDim.x) {
+        const g4 m0 = interp(g_load(d + 4 * p), g_load(d + 4 * p + 1), rr);
+        const g4 m1 = interp(g_load(d + 4 * p + 2), g_load(d + 4 * p + 3), rr);
+        const g4 w0 = interp(g_load(w + 4 * p), g_load(w + 4 * p + 1), rr);
+        const g4 w1 = interp(g_load(w + 4 * p + 2), g_load(w + 4 * p + 3), rr);
+        g_store(dout + 2 * p, m0); g_store(dout + 2 * p + 1, m1);
+        g_store(wout + 2 * p, w0); g_store(wout + 2 * p + 1, w1);
+        clmul_acc(s0, m0, w0);
+        clmul_acc(s2, g_add(m1, m0), g_add(w1, w0));
+    }
+    g4 v[2] = {g_reduce(s0), g_reduce(s2)};
+    block_xor_to_global<2>(v, acc);
+}
+
+// ------------------------------------------------------------------ gkr --
+// Product tree layer: next[k] = cur[k] * cur[k + half] (crates/gkr/src/lib.rs:874-894).
+__global__ __launch_bounds__(kBlock) void k_gkr_layer(const f128* cur, f128* next, size_t half) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < half;
+         k += (size_t)gridDim.x * blockDim.x)
+        g_store(next + k, g_mul(g_load(cur + k), g_load(cur + k + half)));
+}
+
+// Round sums of a dense layer (round_sums, lib.rs:164-219) for the round
+// whose halves are [0, half) and [half, 2 half):
+//   sum_b   = sum_x eq[x] L_b(x) R_b(x)     (b = 1 when at_one)
+//   sum_inf = sum_x eq[x] dL(x) dR(x)
+__global__ __launch_bounds__(kBlock) void k_gkr_sums(const f128* left, const f128* right, const f128* eq, size_t half,
+                           int at_one, uint32_t* acc) {
+    g8 sb = w_zero(), si = w_zero();
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < half;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const g4 l0 = g_load(left + x), l1 = g_load(left + x + half);
+        const g4 r0 = g_load(right + x), r1 = g_load(right + x + half);
+        const g4 e = g_load(eq + x);
+        clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
+        clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
+    }
+    g4 v[2] = {g_reduce(sb), g_reduce(si)};
+    block_xor_to_global<2>(v, acc);
+}
+
+// Bind the round with halves [0, 2h) x [2h, 4h) at r (lib.rs:223-243) and
+// accumulate the next round's sums over halves [0, h), [h, 2h) of the bound
+// arrays.  The next round's eq table is the current one halved:
+// eq(z_<i-1, x) = eq(z_<i, x) + eq(z_<i, x + h), written in place.
+__global__ __launch_bounds__(kBlock) void k_gkr_bind_next(f128* left, f128* right, f128* eq, size_t h, f128 r,
+                                int at_one, uint32_t* acc) {
+    const g4 rr = g_from(r);
+    const size_t hh = 2 * h;
+    g8 sb = w_zero(), si = w_zero();
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < h;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const g4 l0 = interp(g_load(left + x), g_load(left + x + hh), rr);
+        const g4 l1 = interp(g_load(left + x + h), g_load(left + x + h + hh), rr);
+        const g4 r0 = interp(g_load(right + x), g_load(right + x + hh), rr);
+        const g4 r1 = interp(g_load(right + x + h), g_load(right + x + h + hh), rr);
+        g_store(left + x, l0); g_store(left + x + h, l1);
+        g_store(right + x, r0); g_store(right + x + h, r1);
+        const g4 e = g_add(g_load(eq + x), g_load(eq + x + h));
+        g_store(eq + x, e);
+        clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
+        clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
+    }
+    g4 v[2] = {g_reduce(sb), g_reduce(si)};
+    block_xor_to_global<2>(v, acc);
+}
+
+// Bind only (the layer's last round): l0 += r (l1 - l0), same for right.
+__global__ __launch_bounds__(kBlock) void k_gkr_bind(f128* left, f128* right, size_t half, f128 r) {
+    const g4 rr = g_from(r);
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < half;
+         x += (size_t)gridDim.x * blockDim.x) {
+        g_store(left + x, interp(g_load(left + x), g_load(left + x + half), rr));
+        g_store(right + x, interp(g_load(right + x), g_load(right + x + half), rr));
+    }
+}
+
+// --------------------------------------------------------- sparse gathers --
+// D(j) = sum over column j's nonzeros of W[row] * value, W[row] the row's eq
+// weight times the matrix's batching scale (bind_and_batch, matrix.rs:429-468),
+// over a column-major copy of the three matrices.
+__global__ __launch_bounds__(kBlock) void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
+                               const uint8_t* which, const f128* values, const f128* row_scale,
+                               size_t rows_padded, f128* out, size_t cols) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < cols;
+         j += (size_t)gridDim.x * blockDim.x) {
+        g8 s = w_zero();
+        for (uint32_t k = col_starts[j]; k < col_starts[j + 1]; ++k)
+            clmul_acc(s, g_load(row_scale + which[k] * rows_padded + rows[k]), g_load(values + k));
+        g_store(out + j, g_reduce(s));
+    }
+}
+
+// row_scale[m][i] = eq(i, r_x) * scale_m.
+__global__ __launch_bounds__(kBlock) void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
+                             size_t rows_padded) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < rows_padded;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const g4 w = g_load(eq_rows + i);
+        g_store(row_scale + i, w);
+        g_store(row_scale + rows_padded + i, g_mul(w, g_load(scales + 1)));
+        g_store(row_scale + 2 * rows_padded + i, g_mul(w, g_load(scales + 2)));
+    }
+}
+
+// r * M over GF(2^128): out[col] = XOR of challenges[row] over the column
+// (MaterializedMTranspose::apply, matrix_transpose.rs:327-375).
+__global__ __launch_bounds__(kBlock) void k_mt_apply(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
+                           f128* out, size_t cols) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < cols;
+         c += (size_t)gridDim.x * blockDim.x) {
+        g4 s = g_zero();
+        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) s = g_add(s, g_load(ch + rows[k]));
+        g_store(out + c, s);
+    }
+}
+
+// transpose_inner (matrix_transpose.rs:377-441): one unreduced product per
+// nonzero, one reduction per column.
+__global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* rows,
+                                     const f128* row_w, int log_rows, const f128* col_w,
+                                     f128* out, size_t cols) {
+    const uint32_t mask = (1u << log_rows) - 1;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < cols;
+         c += (size_t)gridDim.x * blockDim.x) {
+        g8 s = w_zero();
+        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) {
+            const uint32_t idx = rows[k];
+            clmul_acc(s, g_load(row_w + (idx & mask)), g_load(col_w + (idx >> log_rows)));
+        }
+        g_store(out + c, g_reduce(s));
+    }
+}
+
+}  // namespace bitz

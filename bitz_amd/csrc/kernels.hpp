@@ -1,0 +1,47 @@
+// Kernel declarations shared by the kernel and driver translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gf128.hpp"
+
+namespace bitz {
+
+constexpr int kBlock = 256;
+
+__global__ void k_mul(const f128* a, const f128* b, f128* out, size_t n);
+__global__ void k_eq_small(const f128* r, int k, f128* table);
+__global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);
+__global__ void k_fold(const f128* in, f128* out, size_t out_len, f128 r);
+__global__ void k_fold3(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
+                        size_t out_len, f128 r);
+__global__ void k_outer_round0(const f128* a, const f128* b, const f128* c, size_t pairs,
+                               const f128* low, int low_bits, const f128* high, uint32_t* acc);
+__global__ void k_outer_fold_next(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo,
+                                  f128* co, size_t out_pairs, f128 r, const f128* low,
+                                  int low_bits, const f128* high, uint32_t* acc);
+__global__ void k_outer_claim(const f128* a, const f128* b, const f128* c, size_t n, const f128* low,
+                              int low_bits, const f128* high, uint32_t* acc);
+__global__ void k_inner_round0(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
+__global__ void k_inner_fold_next(const f128* d, const f128* w, f128* dout, f128* wout,
+                                  size_t out_pairs, f128 r, uint32_t* acc);
+__global__ void k_gkr_layer(const f128* cur, f128* next, size_t half);
+__global__ void k_gkr_sums(const f128* left, const f128* right, const f128* eq, size_t half,
+                           int at_one, uint32_t* acc);
+__global__ void k_gkr_bind_next(f128* left, f128* right, f128* eq, size_t h, f128 r,
+                                int at_one, uint32_t* acc);
+__global__ void k_gkr_bind(f128* left, f128* right, size_t half, f128 r);
+__global__ void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
+                               const uint8_t* which, const f128* values, const f128* row_scale,
+                               size_t rows_padded, f128* out, size_t cols);
+__global__ void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
+                             size_t rows_padded);
+__global__ void k_mt_apply(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
+                           f128* out, size_t cols);
+__global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* rows,
+                                     const f128* row_w, int log_rows, const f128* col_w,
+                                     f128* out, size_t cols);
+
+}  // namespace bitz

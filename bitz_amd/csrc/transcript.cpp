@@ -1,0 +1,137 @@
+// Host-side transcripts behind the C ABI's bitz_transcript.
+//
+// The reference's Fiat–Shamir sponge is spongefish (crates/transcript), kept
+// on the caller's side of the boundary: callers pass absorb/squeeze
+// callbacks.  Two built-in sources serve benchmarks and tests:
+//  - seeded: splitmix64(seed) challenges, F128::from([w0, w1]), the stream
+//    the reference's GKR tests draw test data from (crates/gkr/src/lib.rs:986-999);
+//  - duplex: a SHA-256 ratchet binding every challenge to every earlier message.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "transcript.hpp"
+
+namespace bitz {
+
+namespace {
+const uint32_t K256[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+void compress(uint32_t h[8], const uint8_t* p) {
+    uint32_t w[64];
+    for (int i = 0; i < 16; ++i)
+        w[i] = (uint32_t)p[4 * i] << 24 | (uint32_t)p[4 * i + 1] << 16 | (uint32_t)p[4 * i + 2] << 8 |
+               (uint32_t)p[4 * i + 3];
+    for (int i = 16; i < 64; ++i) {
+        const uint32_t s0 = rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3);
+        const uint32_t s1 = rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10);
+        w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+    }
+    uint32_t v[8];
+    memcpy(v, h, sizeof v);
+    for (int i = 0; i < 64; ++i) {
+        const uint32_t t1 = v[7] + (rotr(v[4], 6) ^ rotr(v[4], 11) ^ rotr(v[4], 25)) +
+                            ((v[4] & v[5]) ^ (~v[4] & v[6])) + K256[i] + w[i];
+        const uint32_t t2 = (rotr(v[0], 2) ^ rotr(v[0], 13) ^ rotr(v[0], 22)) +
+                            ((v[0] & v[1]) ^ (v[0] & v[2]) ^ (v[1] & v[2]));
+        memmove(v + 1, v, 7 * sizeof(uint32_t));
+        v[4] += t1;
+        v[0] = t1 + t2;
+    }
+    for (int i = 0; i < 8; ++i) h[i] += v[i];
+}
+}  // namespace
+
+void sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
+    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                     0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    size_t full = len / 64;
+    for (size_t i = 0; i < full; ++i) compress(h, msg + 64 * i);
+    uint8_t tail[128] = {0};
+    const size_t rem = len - 64 * full;
+    memcpy(tail, msg + 64 * full, rem);
+    tail[rem] = 0x80;
+    const size_t tl = rem + 9 <= 64 ? 64 : 128;
+    const uint64_t bits = (uint64_t)len * 8;
+    for (int i = 0; i < 8; ++i) tail[tl - 1 - i] = (uint8_t)(bits >> (8 * i));
+    compress(h, tail);
+    if (tl == 128) compress(h, tail + 64);
+    for (int i = 0; i < 8; ++i) {
+        out[4 * i] = (uint8_t)(h[i] >> 24);
+        out[4 * i + 1] = (uint8_t)(h[i] >> 16);
+        out[4 * i + 2] = (uint8_t)(h[i] >> 8);
+        out[4 * i + 3] = (uint8_t)h[i];
+    }
+}
+
+uint64_t splitmix_next(uint64_t& x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void Transcript::absorb(int kind, const uint8_t* bytes, size_t len) {
+    if (kind == 1) narg.insert(narg.end(), bytes, bytes + len);
+    switch (type) {
+        case Type::kCallbacks:
+            cb_absorb(cb_ctx, kind, bytes, len);
+            break;
+        case Type::kSeeded:
+            break;
+        case Type::kDuplex: {
+            std::vector<uint8_t> buf(32 + 2 + 8 + len);
+            memcpy(buf.data(), state, 32);
+            buf[32] = 'A';
+            buf[33] = (uint8_t)kind;
+            const uint64_t l = len;
+            memcpy(buf.data() + 34, &l, 8);
+            if (len) memcpy(buf.data() + 42, bytes, len);
+            sha256(buf.data(), buf.size(), state);
+            break;
+        }
+    }
+}
+
+f128 Transcript::squeeze() {
+    uint8_t out[16];
+    switch (type) {
+        case Type::kCallbacks:
+            cb_squeeze(cb_ctx, out);
+            break;
+        case Type::kSeeded: {
+            const uint64_t w0 = splitmix_next(seed), w1 = splitmix_next(seed);
+            memcpy(out, &w0, 8);
+            memcpy(out + 8, &w1, 8);
+            break;
+        }
+        case Type::kDuplex: {
+            uint8_t buf[33], h[32];
+            memcpy(buf, state, 32);
+            buf[32] = 'S';
+            sha256(buf, 33, h);
+            memcpy(out, h, 16);
+            buf[32] = 'R';
+            sha256(buf, 33, state);
+            break;
+        }
+    }
+    f128 r;
+    memcpy(&r.lo, out, 8);
+    memcpy(&r.hi, out + 8, 8);
+    return r;
+}
+
+}  // namespace bitz

@@ -1,0 +1,207 @@
+/*
+ * bitz_hip.h — C ABI of the MI355X (gfx950) GF(2^128) sumcheck engine.
+ *
+ * The drop-in boundary for BitZ's Spartan and GKR provers.  Plain pointers and
+ * sizes only; no exceptions cross it; every call returns a status (0 = ok).
+ *
+ * Element layout: bitz_f128 is byte-identical to the reference's
+ * `#[repr(C, align(16))] struct F128 { lo: u64, hi: u64 }`
+ * (crates/field/src/gf128.rs:60-68): bit i of lo||hi is the coefficient of
+ * X^i in GF(2)[X]/(X^128 + X^7 + X^2 + X + 1).
+ *
+ * Pointers: unless a call's flags include BITZ_DEVICE, element arrays are
+ * host memory owned by the caller (copied in / out).  With BITZ_DEVICE they
+ * are device pointers from bitz_device_alloc (inputs stay resident in HBM).
+ *
+ * Fiat–Shamir stays with the caller: the full provers take a bitz_transcript
+ * (two callbacks: absorb a message, squeeze 16 bytes), and the per-round
+ * handle API (bitz_outer_*, bitz_inner_*) takes each challenge as an argument,
+ * so a Rust wrapper can keep spongefish on its side of the boundary.
+ *
+ * Handles are not thread-safe; use one per host thread.
+ */
+#ifndef BITZ_HIP_H
+#define BITZ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint64_t lo, hi;
+} bitz_f128;
+
+/* Status codes.  The negative protocol errors mirror the reference's enums. */
+enum {
+    BITZ_OK = 0,
+    BITZ_ERR_PRODUCT_DIMS = -1,  /* SumcheckError::InvalidProductDimensions (crates/spartan/src/sumcheck.rs:44) */
+    BITZ_ERR_EQUALITY_DIMS = -2, /* SumcheckError::InvalidEqualityDimensions (sumcheck.rs:45) */
+    BITZ_ERR_ROUND_CLAIM = -3,   /* SumcheckError::InvalidRoundClaim (sumcheck.rs:42) */
+    BITZ_ERR_TERMINAL = -4,      /* SumcheckError::InvalidTerminalClaim (sumcheck.rs:43) */
+    BITZ_ERR_ROW_POINT = -5,     /* SpartanMatrixError::InvalidRowPointLength (crates/spartan/src/matrix.rs:25) */
+    BITZ_ERR_TOO_MANY = -6,      /* DenseMleError::TooManyChallenges (crates/poly/src/mle.rs:23) */
+    BITZ_ERR_ARG = -7,           /* bad argument (null pointer, size out of range) */
+    BITZ_ERR_STATE = -8,         /* handle used out of order */
+    BITZ_ERR_POINT_LEN = -9,     /* GKR point length != tree bits (crates/gkr/src/lib.rs:39-43 panics) */
+    BITZ_ERR_DEVICE = -100,      /* HIP runtime error; see bitz_last_error() */
+    BITZ_ERR_ALLOC = -101        /* device allocation failed */
+};
+
+#define BITZ_DEVICE 1u /* element arrays are device pointers */
+
+/* ---- runtime --------------------------------------------------------------- */
+int32_t bitz_init(int32_t device);
+const char* bitz_last_error(void);
+int32_t bitz_device_alloc(void** ptr, size_t bytes);
+int32_t bitz_device_free(void* ptr);
+int32_t bitz_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int32_t bitz_synchronize(void);
+
+/* Kernel timing: with profiling on, the sumcheck kernels ("outer_round0",
+ * "outer_fold_next", "inner_round0", "inner_fold_next", "gkr_sums",
+ * "gkr_bind_next") are bracketed by HIP events on their own stream.  read
+ * returns the summed duration, launch count and algorithmic bytes (the table
+ * bytes each launch must read and write).  No reference counterpart: the
+ * reference times stages with tracing spans (tooling/cli/src/span_stats.rs). */
+int32_t bitz_profile_enable(int32_t on);
+int32_t bitz_profile_read(const char* kernel, double* total_ms, uint64_t* launches, double* algo_bytes);
+
+/* ---- transcripts ----------------------------------------------------------- */
+/* A transcript absorbs messages (kind 0 = public message, 1 = prover message
+ * also appended to the narg string; crates/transcript/src/prover.rs:44-62) and
+ * squeezes 16-byte challenges (verifier_message::<u128>, challenge.rs:85-90;
+ * F128 challenges are F128::from(u128), little-endian lo first). */
+typedef struct bitz_transcript bitz_transcript;
+typedef void (*bitz_absorb_fn)(void* ctx, int32_t kind, const uint8_t* bytes, size_t len);
+typedef void (*bitz_squeeze_fn)(void* ctx, uint8_t out[16]);
+/* Caller-provided sponge (e.g. a Rust closure around transcript::ProverState). */
+bitz_transcript* bitz_transcript_callbacks(void* ctx, bitz_absorb_fn absorb, bitz_squeeze_fn squeeze);
+/* Challenges from the splitmix64 stream of the seed, F128::from([w0, w1])
+ * (the generator of crates/gkr/src/lib.rs:986-999); messages are recorded,
+ * not absorbed. */
+bitz_transcript* bitz_transcript_seeded(uint64_t seed);
+/* SHA-256 ratchet over every message: state = SHA256(state||'A'||kind||le64(len)||m),
+ * challenge = SHA256(state||'S')[0..16], state = SHA256(state||'R'). */
+bitz_transcript* bitz_transcript_duplex(const uint8_t* domain, size_t len);
+void bitz_transcript_free(bitz_transcript* t);
+int32_t bitz_transcript_absorb(bitz_transcript* t, int32_t kind, const uint8_t* bytes, size_t len);
+int32_t bitz_transcript_squeeze(bitz_transcript* t, bitz_f128* out);
+/* Copies up to cap narg bytes; returns the total narg length. */
+size_t bitz_transcript_narg(const bitz_transcript* t, uint8_t* out, size_t cap);
+
+/* ---- field and multilinear primitives -------------------------------------- */
+/* out[i] = a[i] * b[i] (F128 Mul, crates/field/src/gf128.rs:275-281). */
+int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f128* out, size_t n, uint32_t flags);
+/* poly::eq_table (crates/poly/src/eq.rs:63-99): out has 2^k entries. */
+int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags);
+/* DenseMultilinearExtension::fold (crates/poly/src/mle.rs:136-156): binds the
+ * lowest variables to r[0..m), in place; *num_vars is updated. */
+int32_t bitz_mle_fold(bitz_f128* evals, uint32_t* num_vars, const bitz_f128* r, uint32_t m, uint32_t flags);
+/* DenseMultilinearExtension::evaluate (mle.rs:158-172). */
+int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, const bitz_f128* point,
+                          uint32_t point_len, bitz_f128* out, uint32_t flags);
+
+/* ---- Spartan outer sumcheck ------------------------------------------------- */
+/* prove_outer_sumcheck (crates/spartan/src/sumcheck.rs:252-384).
+ * eq factors are consumed, products borrowed (never modified).
+ * Outputs: round_polys[num_vars][4] = [c0,c1,c2,c3], eval_points[num_vars],
+ * *final_claim, claims[3] = [Az(r_x), Bz(r_x), Cz(r_x)] (absorbed last). */
+int32_t bitz_prove_outer_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
+                                  const bitz_f128* eq_low, uint32_t low_vars,
+                                  const bitz_f128* eq_high, uint32_t high_vars,
+                                  const bitz_f128* az, uint32_t az_vars,
+                                  const bitz_f128* bz, uint32_t bz_vars,
+                                  const bitz_f128* cz, uint32_t cz_vars, uint32_t flags,
+                                  bitz_f128* round_polys, bitz_f128* eval_points,
+                                  bitz_f128* final_claim, bitz_f128* claims);
+
+/* The claim the outer sumcheck proves: sum_x eq(x) (Az(x) Bz(x) + Cz(x)) with
+ * eq(x) = eq_low[x & (2^low_vars - 1)] * eq_high[x >> low_vars] — the identity
+ * prove_outer_sumcheck asserts at its end (sumcheck.rs:367-370).  A satisfied
+ * R1CS gives 0, which prove_spartan_piop passes (crates/spartan/src/piop.rs:91). */
+int32_t bitz_outer_claim(const bitz_f128* eq_low, uint32_t low_vars, const bitz_f128* eq_high,
+                         uint32_t high_vars, const bitz_f128* az, const bitz_f128* bz,
+                         const bitz_f128* cz, uint32_t num_vars, uint32_t flags, bitz_f128* out);
+
+/* Per-round form of the same prover for callers that keep the transcript:
+ * begin -> round0 ([c0,c2,c3]) -> fold_next(r_k) per round (returns the next
+ * round's [c0,c2,c3], zeros after the last) -> finish ([Az,Bz,Cz](r_x)).
+ * The caller reconstructs c1 = claim - 2 c0 - c2 - c3, absorbs, squeezes
+ * (recover_full_round_polynomial_and_sample_next_challenge, sumcheck.rs:803-829). */
+typedef struct bitz_outer bitz_outer;
+int32_t bitz_outer_begin(bitz_outer** h, const bitz_f128* eq_low, uint32_t low_vars,
+                         const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az,
+                         uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars,
+                         const bitz_f128* cz, uint32_t cz_vars, uint32_t flags);
+int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* coeffs3);
+int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 r, bitz_f128* coeffs3);
+int32_t bitz_outer_finish(bitz_outer* h, bitz_f128* claims3);
+/* Copies the current (folded) A, B, C tables, `len` entries each, to host. */
+int32_t bitz_outer_tables(bitz_outer* h, bitz_f128* az, bitz_f128* bz, bitz_f128* cz, size_t* len);
+void bitz_outer_free(bitz_outer* h);
+
+/* ---- Spartan inner sumcheck ------------------------------------------------- */
+/* prove_inner_sumcheck (sumcheck.rs:442-552): round_polys[num_vars][3],
+ * evals[2] = [D(r_y), h(r_y)]. */
+int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
+                                  const bitz_f128* matrix, uint32_t matrix_vars,
+                                  const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
+                                  bitz_f128* round_polys, bitz_f128* eval_points,
+                                  bitz_f128* final_claim, bitz_f128* evals);
+typedef struct bitz_inner bitz_inner;
+int32_t bitz_inner_begin(bitz_inner** h, const bitz_f128* matrix, uint32_t matrix_vars,
+                         const bitz_f128* witness, uint32_t witness_vars, uint32_t flags);
+int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* coeffs2);
+int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 r, bitz_f128* coeffs2);
+int32_t bitz_inner_finish(bitz_inner* h, bitz_f128* evals2);
+void bitz_inner_free(bitz_inner* h);
+
+/* ---- Spartan constraint matrices ------------------------------------------- */
+/* PreparedConstraintMatrices (crates/spartan/src/matrix.rs:54-64, 71-125):
+ * A, B, C in compressed-row form (row_starts[rows+1], columns, values). */
+typedef struct bitz_matrices bitz_matrices;
+int32_t bitz_matrices_prepare(bitz_matrices** h, size_t rows, uint32_t row_vars, uint32_t col_vars,
+                              const uint64_t* const row_starts[3], const uint32_t* const columns[3],
+                              const bitz_f128* const values[3]);
+/* bind_and_batch (matrix.rs:429-468): out[2^col_vars] = D. */
+int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
+                            bitz_f128 rho, bitz_f128* out, uint32_t flags);
+void bitz_matrices_free(bitz_matrices* h);
+
+/* ---- M^T (crates/circuit/src/matrix_transpose.rs) ------------------------- */
+typedef struct bitz_mt bitz_mt;
+/* MaterializedMTranspose: CSC of M (column_offsets[columns+1], row_indices). */
+int32_t bitz_mt_prepare(bitz_mt** h, size_t row_count, size_t columns, const uint32_t* column_offsets,
+                        const uint32_t* row_indices);
+/* apply (matrix_transpose.rs:327-375): out[columns] = r * M. */
+int32_t bitz_mt_apply(bitz_mt* h, const bitz_f128* challenges, size_t n, bitz_f128* out, uint32_t flags);
+/* transpose_inner (matrix_transpose.rs:377-441): out[columns], column 0 first. */
+int32_t bitz_mt_transpose_inner(bitz_mt* h, const bitz_f128* row_weights, size_t n_row,
+                                const bitz_f128* col_weights, size_t n_col, bitz_f128* out,
+                                uint32_t flags);
+void bitz_mt_free(bitz_mt* h);
+
+/* ---- GKR grand product (crates/gkr/src/lib.rs) ---------------------------- */
+typedef struct bitz_gkr bitz_gkr;
+/* grand_product / fold_layers (lib.rs:785-802, 874-894) over dense leaves:
+ * builds every layer on device; output[2^log_groups] = per-tree products
+ * (tree c = leaves c, c + 2^log_groups, ...). */
+int32_t bitz_gkr_build(bitz_gkr** h, const bitz_f128* leaves, uint32_t log_leaves,
+                       uint32_t log_groups, bitz_f128* output, uint32_t flags);
+/* gpgkr_prove (lib.rs:24-79) over the dense layers: point has log_groups
+ * coordinates; leaf_point receives log_leaves coordinates and *leaf_claim
+ * the claim on the leaves.  Consumes the layers (binds them in place). */
+int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point, uint32_t point_len,
+                       bitz_f128* leaf_point, bitz_f128* leaf_claim);
+/* Copies layer j (0 = leaves) to host; *len receives its length. */
+int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_t cap, size_t* len);
+void bitz_gkr_free(bitz_gkr* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs, bit for bit.  Every test here needs an MI355X."""
+import numpy as np
+import pytest
+
+import bitz_amd as G
+from common import elements, elements_fast, from_ints, ints, to_int
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _fast_oracle():
+    O.set_kernel("pclmul")  # same products as the bit-serial kernel, pinned by test_oracle_field
+    yield
+    O.set_kernel("portable")
+
+
+EDGES = [0, 1, 2, 0x87, 1 << 63, 1 << 64, 1 << 127, 2**128 - 1, (1 << 128) - 2]
+
+
+def test_mul_batch_matches_oracle():
+    a = [x for x in EDGES for _ in EDGES] + ints(elements(1, 4096))
+    b = [y for _ in EDGES for y in EDGES] + ints(elements(2, 4096))
+    A, B = from_ints(a), from_ints(b)
+    assert np.array_equal(G.mul_batch(A, B), O.mul_batch(A, B))
+    A, B = elements_fast(3, 1 << 20), elements_fast(4, 1 << 20)
+    assert np.array_equal(G.mul_batch(A, B), O.mul_batch(A, B))
+
+
+@pytest.mark.parametrize("k", [0, 1, 5, 12, 13, 20])
+def test_eq_table_matches_oracle(k):
+    r = elements(10 + k, k) if k else np.zeros((0, 2), np.uint64)
+    assert np.array_equal(G.eq_table(r), O.eq_table(r))
+
+
+def test_mle_fold_and_evaluate_match_oracle():
+    t = elements_fast(5, 1 << 14)
+    pt = elements(6, 14)
+    cur = t
+    for i in range(5):
+        cur = O.mle_fold(cur, pt[i])
+    assert np.array_equal(G.mle_fold(t, pt[:5]), cur)
+    assert G.mle_evaluate(t, pt) == O.mle_evaluate(t, pt)
+    with pytest.raises(G.BitzError) as e:
+        G.mle_fold(t[:4], pt[:3])
+    assert e.value.variant == "TooManyChallenges"
+
+
+def outer_case(n, split, seed, satisfied=False):
+    a = elements_fast(seed, 1 << n)
+    b = elements_fast(seed + 1, 1 << n)
+    c = O.mul_batch(a, b) if satisfied else elements_fast(seed + 2, 1 << n)
+    tau = elements(seed + 3, n) if n else np.zeros((0, 2), np.uint64)
+    lo = O.eq_table(tau[:split]) if split else from_ints([1])
+    hi = O.eq_table(tau[split:]) if split < n else from_ints([1])
+    return a, b, c, tau, lo, hi
+
+
+def run_outer_both(n, split, seed, claim=0, satisfied=False, domain=b"outer/gpu"):
+    a, b, c, tau, lo, hi = outer_case(n, split, seed, satisfied)
+    ot = O.DuplexTranscript(domain)
+    rc, ref = O.outer_sumcheck(ot, claim, lo, hi, a, b, c)
+    assert rc == 0
+    gt = G.Transcript.duplex(domain)
+    out = G.prove_outer_sumcheck(gt, claim, (lo, hi), G.R1csProductMles(a, b, c))
+    return ref, out, (a, b, c, tau, lo, hi), (ot, gt)
+
+
+def assert_outer_equal(ref, out):
+    assert np.array_equal(out.round_polynomials, ref["rounds"])
+    assert np.array_equal(out.eval_points, ref["points"])
+    assert out.final_claim == ref["final_claim"]
+    assert [out.az_mle_claim, out.bz_mle_claim, out.cz_mle_claim] == ints(ref["claims"])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 6, 11, 16])
+def test_outer_sumcheck_matches_oracle(n):
+    ref, out, _, tr = run_outer_both(n, n // 2, 100 + n, claim=0x1234 + n)
+    assert_outer_equal(ref, out)
+    # the transcripts saw the same bytes: their next challenges agree
+    assert O.squeeze(tr[0]) == tr[1].squeeze()
+
+
+@pytest.mark.parametrize("split", range(6))
+def test_outer_every_split(split):
+    ref, out, *_ = run_outer_both(5, split, 7, satisfied=True)
+    assert_outer_equal(ref, out)
+
+
+def test_outer_handle_tables_match_oracle_folds():
+    n = 9
+    a, b, c, tau, lo, hi = outer_case(n, 4, 55)
+    h = G.OuterSumcheck((lo, hi), G.R1csProductMles(a, b, c))
+    h.round0()
+    rs = elements(56, n)
+    ta, tb, tc = a, b, c
+    for k in range(n):
+        h.fold_next(rs[k])
+        ta, tb, tc = O.mle_fold(ta, rs[k]), O.mle_fold(tb, rs[k]), O.mle_fold(tc, rs[k])
+        ga, gb, gc = h.tables()
+        assert np.array_equal(ga, ta) and np.array_equal(gb, tb) and np.array_equal(gc, tc), k
+    assert np.array_equal(h.finish(), np.vstack([ta, tb, tc]))
+
+
+def test_outer_dimension_errors_leave_transcript_untouched():
+    a, b, c, tau, lo, hi = outer_case(3, 1, 9)
+    t = G.Transcript.duplex(b"dims")
+    with pytest.raises(G.SumcheckError) as e:
+        G.prove_outer_sumcheck(t, 0, (lo, hi), G.R1csProductMles(a, b[:4], c))
+    assert e.value.variant == "InvalidProductDimensions"
+    with pytest.raises(G.SumcheckError) as e:
+        G.prove_outer_sumcheck(t, 0, (lo, lo), G.R1csProductMles(a, b, c))
+    assert e.value.variant == "InvalidEqualityDimensions"
+    assert t.squeeze() == G.Transcript.duplex(b"dims").squeeze()
+
+
+def true_claim(a, b, c, tau):
+    eq = O.eq_table(tau)
+    return O.inner_product(eq, O.mul_batch(a, b)) ^ O.inner_product(eq, c)
+
+
+def test_outer_n20_matches_oracle_and_verifies():
+    a, b, c, tau, lo, hi = outer_case(20, 10, 2024)
+    claim = true_claim(a, b, c, tau)
+    ref, out, *_ = run_outer_both(20, 10, 2024, claim=claim)
+    assert_outer_equal(ref, out)
+    vt = O.DuplexTranscript(b"outer/gpu")
+    claims = np.array([[out.az_mle_claim & (2**64 - 1), out.az_mle_claim >> 64],
+                       [out.bz_mle_claim & (2**64 - 1), out.bz_mle_claim >> 64],
+                       [out.cz_mle_claim & (2**64 - 1), out.cz_mle_claim >> 64]], dtype=np.uint64)
+    assert O.outer_verify(vt, claim, tau, out.round_polynomials, claims) == 0
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 12, 13, 18])
+def test_inner_sumcheck_matches_oracle(n):
+    d, w = elements_fast(300 + n, 1 << n), elements_fast(400 + n, 1 << n)
+    claim = O.inner_product(d, w)
+    ot = O.DuplexTranscript(b"inner/gpu")
+    rc, ref = O.inner_sumcheck(ot, claim, d, w)
+    assert rc == 0
+    gt = G.Transcript.duplex(b"inner/gpu")
+    out = G.prove_inner_sumcheck(gt, claim, d, w)
+    assert np.array_equal(out.round_polynomials, ref["rounds"])
+    assert np.array_equal(out.eval_points, ref["points"])
+    assert out.final_claim == ref["final_claim"]
+    assert [out.batched_matrix_evaluation, out.witness_evaluation] == ints(ref["evals"])
+    assert O.squeeze(ot) == gt.squeeze()
+
+
+def test_inner_dimension_error():
+    with pytest.raises(G.SumcheckError) as e:
+        G.prove_inner_sumcheck(G.Transcript.seeded(1), 0, from_ints([1, 2]), from_ints([1, 2, 3, 4]))
+    assert e.value.variant == "InvalidProductDimensions"
+
+
+def random_csr(rng, rows, cols, per_row, seed):
+    starts, colsl = [0], []
+    for _ in range(rows):
+        cs = np.unique(rng.integers(0, cols, size=per_row))
+        colsl += cs.tolist()
+        starts.append(len(colsl))
+    return np.array(starts, np.uint64), np.array(colsl, np.uint32), elements_fast(seed, len(colsl))
+
+
+@pytest.mark.parametrize("rows,row_vars,col_vars,per_row", [(13, 4, 5, 6), (700, 10, 17, 24), (1, 0, 3, 2)])
+def test_bind_and_batch_matches_oracle(rows, row_vars, col_vars, per_row):
+    rng = np.random.default_rng(rows)
+    csr = [random_csr(rng, rows, 1 << col_vars, per_row, 500 + m) for m in range(3)]
+    rx = elements(7, row_vars) if row_vars else np.zeros((0, 2), np.uint64)
+    rho = 0xDEADBEEF0123456789ABCDEF55AA55AA
+    prepared = G.PreparedConstraintMatrices(rows, row_vars, col_vars, csr)
+    d = prepared.bind_and_batch(rx, rho)
+    mats = [O.CsrMatrix(*m) for m in csr]
+    assert np.array_equal(d, O.bind_and_batch(mats, rows, row_vars, col_vars, rx, rho))
+    with pytest.raises(G.BitzError) as e:
+        prepared.bind_and_batch(elements(8, row_vars + 1), rho)
+    assert e.value.variant == "InvalidRowPointLength"
+
+
+def test_mt_apply_and_transpose_inner_match_oracle():
+    rng = np.random.default_rng(3)
+    rows, cols = 5000, 3000
+    offsets, idx = [0], []
+    for _ in range(cols):
+        idx += sorted(set(rng.integers(0, rows, size=int(rng.integers(0, 9))).tolist()))
+        offsets.append(len(idx))
+    r = elements_fast(9, rows)
+    mt = G.MaterializedMTranspose(rows, offsets, idx)
+    assert np.array_equal(mt.apply(r), O.mt_apply(offsets, idx, r))
+    rw, cw = elements_fast(10, 64), elements_fast(11, 128)
+    assert np.array_equal(mt.transpose_inner(rw, cw), O.mt_transpose_inner(offsets, idx, rw, cw))
+
+
+@pytest.mark.parametrize("log_leaves,log_groups,zero", [(6, 2, None), (10, 3, 0), (14, 7, 6), (16, 1, None)])
+def test_gkr_matches_oracle(log_leaves, log_groups, zero):
+    leaves = elements_fast(70 + log_leaves, 1 << log_leaves)
+    store, layers = O.gkr_fold_layers(leaves, log_groups)
+    out, wit = G.grand_product(leaves, log_groups)
+    assert np.array_equal(out, layers[-1])
+    for j in (0, 1, len(layers) - 1):
+        assert np.array_equal(wit.layer(j), layers[j])
+    point = elements(80, log_groups)
+    if zero is not None:
+        point[zero] = 0
+    ot = O.DuplexTranscript(b"gkr/gpu")
+    ref_point, ref_claim = O.gkr_prove(ot, point, store, log_leaves)
+    gt = G.Transcript.duplex(b"gkr/gpu")
+    gp, gc = G.gpgkr_prove(gt, point, wit)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim
+    assert gt.narg() == ot.narg()
+    assert gc == O.mle_evaluate(leaves, gp)
+
+
+def test_gkr_rejects_a_wrong_point_length():
+    leaves = elements_fast(1, 1 << 6)
+    _, wit = G.grand_product(leaves, 2)
+    with pytest.raises(G.BitzError) as e:
+        G.gpgkr_prove(G.Transcript.seeded(1), elements(2, 3), wit)
+    assert e.value.variant == "InvalidPointLength"
