@@ -101,6 +101,7 @@ def lib():
                                                 P, P, P, P]),
             "bitz_outer_claim": (I32, [P, U32, P, U32, P, P, P, U32, U32, P]),
             "bitz_outer_begin": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32, P, U32, P, U32, U32]),
+            "bitz_outer_eq_mode": (I32, [P]),
             "bitz_outer_round0": (I32, [P, P]),
             "bitz_outer_fold_next": (I32, [P, F128, P]),
             "bitz_outer_finish": (I32, [P, P]),
@@ -437,6 +438,10 @@ class OuterSumcheck:
                                       _nvars(len(eq_high)), _p(az), _nvars(na), _p(bz), _nvars(nb), _p(cz),
                                       _nvars(nc), fa), SumcheckError)
         self.h = h
+
+    @property
+    def eq_mode(self) -> bool:
+        return bool(lib().bitz_outer_eq_mode(self.h))
 
     def round0(self):
         out = np.zeros((3, 2), dtype=np.uint64)

@@ -416,6 +416,18 @@ struct bitz_outer {
     Pinned host;
     Timed tm;
 
+    // eq-factored mode (the factors are eq tables of a point tau)
+    bool eqmode = false;
+    std::vector<f128> tau, tau_inv;
+    Dev tau_dev, weights, jobs;
+    struct Geo {
+        size_t threads, iters, la_off, lb_off, h_off;
+        int la_bits;
+    };
+    std::vector<Geo> geo;
+    f128 S{1, 0}, sigma{0, 0};
+    f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
+
     int32_t read_slot(int slot, bitz_f128* out3) {
         HIP_TRY(hipMemcpyAsync(host.p, acc.u32() + 12 * slot, 48, hipMemcpyDeviceToHost, so.s));
         HIP_TRY(hipStreamSynchronize(so.s));
@@ -434,6 +446,181 @@ static int32_t outer_check(uint32_t low_vars, uint32_t high_vars, uint32_t az_va
         return fail(BITZ_ERR_EQUALITY_DIMS, "eq factor num_vars do not sum to the product num_vars");
     if (az_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
     return BITZ_OK;
+}
+
+// Montgomery batch inversion of the nonzero entries (zeros stay zero).
+static void batch_inverse(std::vector<f128>& v) {
+    std::vector<f128> pre(v.size());
+    f128 acc = hostf::one();
+    for (size_t i = 0; i < v.size(); ++i) {
+        pre[i] = acc;
+        if (!hostf::is_zero(v[i])) acc = hostf::mul(acc, v[i]);
+    }
+    f128 inv = hostf::inv(acc);
+    for (size_t i = v.size(); i-- > 0;) {
+        if (hostf::is_zero(v[i])) continue;
+        const f128 vi = v[i];
+        v[i] = hostf::mul(inv, pre[i]);
+        inv = hostf::mul(inv, vi);
+    }
+}
+
+// Recovers tau from eq_table(tau) (crates/poly/src/eq.rs:63-99): with
+// T[0] = prod (1 + tau_j) and T[2^i] = tau_i prod_{j != i} (1 + tau_j),
+// tau_i = T[2^i] / (T[0] + T[2^i]).  False if a denominator vanishes.
+static bool recover_tau(const std::vector<f128>& probes, int k, std::vector<f128>& out) {
+    // probes = [T[0], T[1], T[2], T[4], ...]
+    std::vector<f128> den(k);
+    for (int i = 0; i < k; ++i) {
+        den[i] = hostf::add(probes[0], probes[1 + i]);
+        if (hostf::is_zero(den[i])) return false;
+    }
+    batch_inverse(den);
+    for (int i = 0; i < k; ++i) out.push_back(hostf::mul(probes[1 + i], den[i]));
+    return true;
+}
+
+// Sets up the eq-factored mode: recover tau, build every round's weight
+// tables on the device, and check eq_table(tau_low), eq_table(tau_high)
+// against the given factors there.  Returns false (generic mode) when the
+// factors are not eq tables of a point.
+static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
+    hipStream_t s = h->so.s;
+    const int n = h->n;
+    if (n == 0) return false;
+    // Probe entries T[0], T[2^i] of both factors.
+    std::vector<f128> pl(low_vars + 1), ph(high_vars + 1);
+    if (hipMemcpyAsync(pl.data(), h->eqL[0].f(), sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+    for (int i = 0; i < low_vars; ++i)
+        if (hipMemcpyAsync(&pl[1 + i], h->eqL[0].f() + ((size_t)1 << i), sizeof(f128), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return false;
+    if (hipMemcpyAsync(ph.data(), h->eqH[0].f(), sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+    for (int i = 0; i < high_vars; ++i)
+        if (hipMemcpyAsync(&ph[1 + i], h->eqH[0].f() + ((size_t)1 << i), sizeof(f128), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return false;
+    if (hipStreamSynchronize(s) != hipSuccess) return false;
+    std::vector<f128> tau;
+    if (!recover_tau(pl, low_vars, tau) || !recover_tau(ph, high_vars, tau)) return false;
+    h->tau = tau;
+    h->tau_inv = tau;
+    batch_inverse(h->tau_inv);
+
+    // Geometry and weight-table jobs: job = (offset, tau start, k).
+    std::vector<uint32_t> jobs;
+    size_t off = 0;
+    auto add_job = [&](size_t start, int k) {
+        const size_t o = off;
+        jobs.push_back((uint32_t)o);
+        jobs.push_back((uint32_t)start);
+        jobs.push_back((uint32_t)k);
+        off += (size_t)1 << k;
+        return o;
+    };
+    h->geo.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const int pbits = n - k - 1;             // log2 pairs of round k
+        const int tbits = std::min(pbits, 18);   // log2 threads
+        auto& g = h->geo[k];
+        g.threads = (size_t)1 << tbits;
+        g.iters = (size_t)1 << (pbits - tbits);
+        g.la_bits = std::min(tbits, 9);
+        g.la_off = add_job(k + 1, g.la_bits);
+        g.lb_off = add_job(k + 1 + g.la_bits, tbits - g.la_bits);
+        g.h_off = add_job(k + 1 + tbits, pbits - tbits);
+    }
+    const size_t check_low = add_job(0, low_vars), check_high = add_job(low_vars, high_vars);
+    const size_t n_jobs = jobs.size() / 3;
+    if (h->tau_dev.alloc(n * sizeof(f128)) || h->weights.alloc(off * sizeof(f128)) ||
+        h->jobs.alloc(jobs.size() * 4))
+        return false;
+    if (hipMemcpyAsync(h->tau_dev.p, h->tau.data(), n * sizeof(f128), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(h->jobs.p, jobs.data(), jobs.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+        return false;
+    hipLaunchKernelGGL(bitz::k_eq_jobs, dim3((unsigned)n_jobs), dim3(kBlock), 0, s, h->tau_dev.f(), h->jobs.u32(),
+                       h->weights.f());
+    if (hipGetLastError() != hipSuccess) return false;
+    // Compare the rebuilt factors with the given ones.
+    const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
+    std::vector<f128> a(nl), b(nl), c(nh), d(nh);
+    if (hipMemcpyAsync(a.data(), h->weights.f() + check_low, nl * sizeof(f128), hipMemcpyDeviceToHost, s) ||
+        hipMemcpyAsync(b.data(), h->eqL[0].f(), nl * sizeof(f128), hipMemcpyDeviceToHost, s) ||
+        hipMemcpyAsync(c.data(), h->weights.f() + check_high, nh * sizeof(f128), hipMemcpyDeviceToHost, s) ||
+        hipMemcpyAsync(d.data(), h->eqH[0].f(), nh * sizeof(f128), hipMemcpyDeviceToHost, s) ||
+        hipStreamSynchronize(s))
+        return false;
+    if (memcmp(a.data(), b.data(), nl * sizeof(f128)) || memcmp(c.data(), d.data(), nh * sizeof(f128))) return false;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<true, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        attr_set = true;
+    }
+    return true;
+}
+
+// Launches the eq-factored round-k kernel (folding at r when k > 0) into
+// slot k.
+static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3]) {
+    const auto& g = h->geo[k];
+    const bool q1 = k == 0 || hostf::is_zero(h->tau[k]);
+    const unsigned block = (unsigned)std::min<size_t>(g.threads, 1024);
+    const unsigned grid = (unsigned)(g.threads / block);
+    const f128* w = h->weights.f();
+    uint32_t* slot = h->acc.u32() + 12 * k;
+    hipStream_t s = h->so.s;
+    const size_t pairs = g.threads * g.iters;
+    h->tm.begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 64.0 * pairs : 288.0 * pairs);
+    const size_t smem = bitz::kOuterEqSmem;
+#define OUTER_EQ_ARGS                                                                                     \
+    h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
+        g.threads, g.iters, r, w + g.la_off, g.la_bits, w + g.lb_off, w + g.h_off, slot
+    if (k == 0)
+        hipLaunchKernelGGL((bitz::k_outer_eq<false, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+    else if (q1)
+        hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+    else
+        hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+#undef OUTER_EQ_ARGS
+    h->tm.end(s);
+    return check_launch();
+}
+
+// Round k's [c0, c2, c3] from q(0), q(inf) and q(1) (read or recovered):
+// g_k(X) = S_k (1 + tau_k + X) q_k(X), q_k(X) = q0 + q1 X + q2 X^2.
+static void outer_eq_coeffs(bitz_outer* h, int k, const bitz_f128* raw, bitz_f128* co) {
+    const f128 tau = h->tau[k];
+    const f128 one_tau = hostf::add(hostf::one(), tau);
+    const f128 q0 = F(raw[0]), q2 = F(raw[1]);
+    f128 q_at1;
+    if (k == 0 || hostf::is_zero(tau)) {
+        q_at1 = F(raw[2]);
+    } else {
+        // sigma_k = q_{k-1}(r_{k-1}) = (1 + tau_k) q(0) + tau_k q(1)
+        q_at1 = hostf::mul(hostf::add(h->sigma, hostf::mul(one_tau, q0)), h->tau_inv[k]);
+    }
+    const f128 q1 = hostf::add(hostf::add(q_at1, q0), q2);
+    h->q[0] = q0;
+    h->q[1] = q1;
+    h->q[2] = q2;
+    const f128 S = h->S;
+    co[0] = B(hostf::mul(S, hostf::mul(one_tau, q0)));
+    co[1] = B(hostf::mul(S, hostf::add(hostf::mul(one_tau, q2), q1)));
+    co[2] = B(hostf::mul(S, q2));
+}
+
+// After round k's challenge r: sigma_{k+1} = q_k(r), S_{k+1} = S_k eq(tau_k, r).
+static void outer_eq_advance(bitz_outer* h, int k, f128 r) {
+    h->sigma = hostf::add(h->q[0], hostf::mul(r, hostf::add(h->q[1], hostf::mul(r, h->q[2]))));
+    const f128 e = f128{1 ^ h->tau[k].lo ^ r.lo, h->tau[k].hi ^ r.hi};
+    h->S = hostf::mul(h->S, e);
 }
 
 extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
@@ -481,9 +668,13 @@ extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, u
     if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
     if ((rc = h->host.alloc(64))) return cleanup(rc);
+    h->eqmode = outer_setup_eq(h, (int)low_vars, (int)high_vars);
+    (void)hipGetLastError();  // a failed probe must not leak into later checks
     *out = h;
     return BITZ_OK;
 }
+
+extern "C" int32_t bitz_outer_eq_mode(const bitz_outer* h) { return h && h->eqmode ? 1 : 0; }
 
 extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
     if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
@@ -491,6 +682,13 @@ extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
     h->started = true;
     if (h->n == 0) {
         memset(co, 0, 48);
+        return BITZ_OK;
+    }
+    if (h->eqmode) {
+        TRY(outer_eq_launch(h, 0, f128{0, 0}, nullptr));
+        bitz_f128 raw[3];
+        TRY(h->read_slot(0, raw));
+        outer_eq_coeffs(h, 0, raw, co);
         return BITZ_OK;
     }
     const size_t pairs = h->len / 2;
@@ -515,6 +713,30 @@ extern "C" int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 rb, bitz_f128* 
     f128* out[3] = {dst.f(), dst.f() + cap, dst.f() + 2 * cap};
     uint32_t* slot = h->acc.u32() + 12 * (h->round + 1);
     bool has_next = false;
+    if (h->eqmode) {
+        const int k = h->round;  // r is round k's challenge
+        outer_eq_advance(h, k, r);
+        if (k + 1 < h->n) {
+            TRY(outer_eq_launch(h, k + 1, r, out));
+            for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+            h->where = (h->where == 1) ? 2 : 1;
+            h->len = next_len;
+            h->round += 1;
+            bitz_f128 raw[3];
+            TRY(h->read_slot(h->round, raw));
+            outer_eq_coeffs(h, h->round, raw, co);
+            return BITZ_OK;
+        }
+        hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2], out[0],
+                           out[1], out[2], (size_t)1, r);
+        TRY(check_launch());
+        for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+        h->where = (h->where == 1) ? 2 : 1;
+        h->len = next_len;
+        h->round += 1;
+        memset(co, 0, 48);
+        return BITZ_OK;
+    }
     if (h->low_len > 1) {
         // Phase 1 (sumcheck.rs:294-323): fold eq_low, then the products.
         const size_t nl = h->low_len / 2;

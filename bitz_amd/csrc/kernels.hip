@@ -349,4 +349,139 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
     }
 }
 
+// ============================================ outer sumcheck, eq-factored ===
+// Used when the eq factors are eq tables of a point tau (checked by the
+// driver).  Round k's polynomial is g_k(X) = S_k eq(tau_k, X) q_k(X) with
+//   q_k(X) = sum_y E_k(y) (A_k B_k + C_k)(X, y),   E_k(y) = eq(tau_{>k}, y),
+// a quadratic; the kernels return q(0), q(inf) and, when asked, q(1).  The
+// pair index y = i * T + t (t = global thread, i = iteration) splits the
+// weight as E_k(y) = L(t) H(i): H(i) is the same for every thread of an
+// iteration and is applied through an LDS constant table; L(t) is applied
+// once per thread at the end, from two small tables La (low bits) and Lb.
+namespace {
+
+struct Smem {
+    uint4 r_tab[kConstTableEntries];
+    uint4 h_tab[kConstTableEntries];
+    uint4 basis[128];
+};
+
+template <bool FOLD>
+__device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f128* c, f128* ao,
+                                          f128* bo, f128* co, size_t y, const uint4* r_tab, const g4& rr,
+                                          bool use_tab, g4& a0, g4& a1, g4& b0, g4& b1, g4& c0, g4& c1) {
+    if (!FOLD) {
+        a0 = g_load(a + 2 * y); a1 = g_load(a + 2 * y + 1);
+        b0 = g_load(b + 2 * y); b1 = g_load(b + 2 * y + 1);
+        c0 = g_load(c + 2 * y); c1 = g_load(c + 2 * y + 1);
+        return;
+    }
+    g4 in[3][4];
+    const f128* src[3] = {a, b, c};
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) in[t][j] = g_load(src[t] + 4 * y + j);
+    g4 out[3][2];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const g4 d = g_add(in[t][2 * j + 1], in[t][2 * j]);
+            out[t][j] = g_add(in[t][2 * j], use_tab ? g_mul_table(r_tab, d) : g_mul(rr, d));
+        }
+    g_store(ao + 2 * y, out[0][0]); g_store(ao + 2 * y + 1, out[0][1]);
+    g_store(bo + 2 * y, out[1][0]); g_store(bo + 2 * y + 1, out[1][1]);
+    g_store(co + 2 * y, out[2][0]); g_store(co + 2 * y + 1, out[2][1]);
+    a0 = out[0][0]; a1 = out[0][1];
+    b0 = out[1][0]; b1 = out[1][1];
+    c0 = out[2][0]; c1 = out[2][1];
+}
+
+}  // namespace
+
+template <bool FOLD, bool Q1>
+__global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao,
+                                                   f128* bo, f128* co, size_t threads, size_t iters, f128 r,
+                                                   const f128* la, int la_bits, const f128* lb,
+                                                   const f128* h, uint32_t* acc) {
+    extern __shared__ uint4 smem_raw[];
+    Smem& sm = *reinterpret_cast<Smem*>(smem_raw);
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const g4 rr = g_from(r);
+    // Tables pay off only when a workgroup handles enough pairs.
+    const bool tabs = iters > 1 || threads >= 1024;
+    if (FOLD && tabs) build_const_table(sm.r_tab, sm.basis, rr);
+    g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
+    for (size_t i = 0; i < iters; ++i) {
+        const g4 hw = g_load(h + i);
+        if (iters > 1) {
+            __syncthreads();  // previous iteration's readers are done with h_tab
+            build_const_table(sm.h_tab, sm.basis, hw);
+        }
+        if (t < threads) {
+            const size_t y = i * threads + t;
+            g4 a0, a1, b0, b1, c0, c1;
+            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, rr, tabs, a0, a1, b0, b1, c0, c1);
+            const g4 p0 = g_add(g_mul(a0, b0), c0);
+            const g4 pi = g_mul(g_add(a0, a1), g_add(b0, b1));
+            if (iters > 1) {
+                s0 = g_add(s0, g_mul_table(sm.h_tab, p0));
+                si = g_add(si, g_mul_table(sm.h_tab, pi));
+            } else {
+                s0 = g_add(s0, g_mul(hw, p0));
+                si = g_add(si, g_mul(hw, pi));
+            }
+            if (Q1) {
+                const g4 p1 = g_add(g_mul(a1, b1), c1);
+                s1 = g_add(s1, iters > 1 ? g_mul_table(sm.h_tab, p1) : g_mul(hw, p1));
+            }
+        }
+    }
+    g4 v[3] = {g_zero(), g_zero(), g_zero()};
+    if (t < threads) {
+        const size_t mask = ((size_t)1 << la_bits) - 1;
+        const g4 L = g_mul(g_load(la + (t & mask)), g_load(lb + (t >> la_bits)));
+        v[0] = g_mul(L, s0);
+        v[1] = g_mul(L, si);
+        if (Q1) v[2] = g_mul(L, s1);
+    }
+    block_xor_to_global<3>(v, acc);
+}
+
+template __global__ void k_outer_eq<false, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
+                                                  size_t, size_t, f128, const f128*, int, const f128*,
+                                                  const f128*, uint32_t*);
+template __global__ void k_outer_eq<false, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
+                                                 size_t, size_t, f128, const f128*, int, const f128*,
+                                                 const f128*, uint32_t*);
+template __global__ void k_outer_eq<true, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
+                                                 size_t, size_t, f128, const f128*, int, const f128*,
+                                                 const f128*, uint32_t*);
+template __global__ void k_outer_eq<true, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
+                                                size_t, size_t, f128, const f128*, int, const f128*,
+                                                const f128*, uint32_t*);
+
+// Builds many small eq tables at once: job j writes eq(tau[start_j ..
+// start_j + k_j), .) (2^k_j entries) at out + offset_j, one workgroup per job
+// (crates/poly/src/eq.rs:63-99 level doubling).
+__global__ __launch_bounds__(kBlock) void k_eq_jobs(const f128* tau, const uint32_t* jobs, f128* out) {
+    const uint32_t offset = jobs[3 * blockIdx.x], start = jobs[3 * blockIdx.x + 1],
+                   k = jobs[3 * blockIdx.x + 2];
+    f128* table = out + offset;
+    if (threadIdx.x == 0) g_store(table, g_one());
+    __syncthreads();
+    for (uint32_t i = 0; i < k; ++i) {
+        const size_t half = (size_t)1 << i;
+        const g4 ri = g_load(tau + start + i);
+        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
+            const g4 parent = g_load(table + j);
+            const g4 one = g_mul(parent, ri);
+            g_store(table + j, g_add(parent, one));
+            g_store(table + j + half, one);
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace bitz

@@ -44,4 +44,11 @@ __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* ro
                                      const f128* row_w, int log_rows, const f128* col_w,
                                      f128* out, size_t cols);
 
+template <bool FOLD, bool Q1>
+__global__ void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
+                           size_t threads, size_t iters, f128 r, const f128* la, int la_bits, const f128* lb,
+                           const f128* h, uint32_t* acc);
+__global__ void k_eq_jobs(const f128* tau, const uint32_t* jobs, f128* out);
+constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries + 128);
+
 }  // namespace bitz

@@ -218,6 +218,29 @@ class DuplexTranscript:
         return C.string_at(self._d.narg, self._d.narg_len) if self._d.narg_len else b""
 
 
+class ListTranscript:
+    """Challenges from a fixed list (tests that need chosen challenges);
+    messages are recorded."""
+
+    def __init__(self, challenges):
+        self.challenges = [int(c) for c in challenges]
+        self.messages = []
+
+        def _a(ctx, kind, ptr, n):
+            self.messages.append((kind, C.string_at(ptr, n) if n else b""))
+
+        def _s(ctx, out):
+            v = self.challenges.pop(0)
+            C.memmove(out, v.to_bytes(16, "little"), 16)
+
+        self._a, self._s = ABSORB(_a), SQUEEZE(_s)
+        self.t = Transcript(None, C.cast(self._a, C.c_void_p), C.cast(self._s, C.c_void_p))
+
+    @property
+    def ref(self):
+        return C.byref(self.t)
+
+
 def squeeze(tr) -> int:
     out = (C.c_uint8 * 16)()
     fn = SQUEEZE(tr.t.squeeze)

@@ -219,3 +219,60 @@ def test_gkr_rejects_a_wrong_point_length():
     with pytest.raises(G.BitzError) as e:
         G.gpgkr_prove(G.Transcript.seeded(1), elements(2, 3), wit)
     assert e.value.variant == "InvalidPointLength"
+
+
+# ---- the two outer-sumcheck modes ---------------------------------------------
+def outer_with(a, b, c, lo, hi, claim, challenges):
+    ot = O.ListTranscript(challenges)
+    rc, ref = O.outer_sumcheck(ot, claim, lo, hi, a, b, c)
+    assert rc == 0
+    it = iter(int(x) for x in challenges)
+    gt = G.Transcript.callbacks(lambda kind, data: None, lambda: next(it).to_bytes(16, "little"))
+    out = G.prove_outer_sumcheck(gt, claim, (lo, hi), G.R1csProductMles(a, b, c))
+    assert_outer_equal(ref, out)
+    return out
+
+
+def mode_of(lo, hi, a, b, c):
+    return G.OuterSumcheck((lo, hi), G.R1csProductMles(a, b, c)).eq_mode
+
+
+def test_outer_eq_mode_is_used_for_eq_factors():
+    a, b, c, tau, lo, hi = outer_case(12, 6, 91)
+    assert mode_of(lo, hi, a, b, c)
+    ch = ints(elements(92, 12))
+    outer_with(a, b, c, lo, hi, 5, ch)
+
+
+@pytest.mark.parametrize("zero_at", [0, 1, 5, 11])
+def test_outer_eq_mode_with_zero_tau(zero_at):
+    n = 12
+    a, b, c = elements_fast(93, 1 << n), elements_fast(94, 1 << n), elements_fast(95, 1 << n)
+    tau = elements(96, n)
+    tau[zero_at] = 0
+    lo, hi = O.eq_table(tau[:6]), O.eq_table(tau[6:])
+    assert mode_of(lo, hi, a, b, c)
+    outer_with(a, b, c, lo, hi, 0, ints(elements(97, n)))
+
+
+def test_outer_generic_mode_for_non_eq_factors_and_tau_one():
+    n = 10
+    a, b, c = elements_fast(98, 1 << n), elements_fast(99, 1 << n), elements_fast(100, 1 << n)
+    lo, hi = elements_fast(101, 1 << 5), elements_fast(102, 1 << 5)  # arbitrary tables
+    assert not mode_of(lo, hi, a, b, c)
+    outer_with(a, b, c, lo, hi, 3, ints(elements(103, n)))
+    tau = elements(104, n)
+    tau[2] = 1  # eq_table(tau) has T[0] = 0: tau cannot be recovered
+    lo, hi = O.eq_table(tau[:5]), O.eq_table(tau[5:])
+    assert not mode_of(lo, hi, a, b, c)
+    outer_with(a, b, c, lo, hi, 0, ints(elements(105, n)))
+
+
+def test_outer_eq_mode_when_eq_bound_vanishes():
+    """r_k = 1 + tau_k makes eq(tau, r) zero from round k on."""
+    n = 12
+    a, b, c, tau, lo, hi = outer_case(n, 6, 106)
+    ch = ints(elements(107, n))
+    ch[3] = to_int(tau[3]) ^ 1
+    out = outer_with(a, b, c, lo, hi, 9, ch)
+    assert all(to_int(x) == 0 for x in out.round_polynomials[5][[0, 2, 3]])
