@@ -83,6 +83,7 @@ def lib():
             "bitz_memcpy_h2d": (I32, [P, P, SZ]),
             "bitz_memcpy_d2h": (I32, [P, P, SZ]),
             "bitz_synchronize": (I32, []),
+            "bitz_release_cache": (I32, []),
             "bitz_profile_enable": (I32, [I32]),
             "bitz_profile_read": (I32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_double)]),
@@ -99,6 +100,8 @@ def lib():
             "bitz_mle_evaluate": (I32, [P, U32, P, U32, P, U32]),
             "bitz_prove_outer_sumcheck": (I32, [P, F128, P, U32, P, U32, P, U32, P, U32, P, U32, U32,
                                                 P, P, P, P]),
+            "bitz_prove_outer_sumcheck_with_challenges": (I32, [P, F128, P, U32, P, U32, P, U32, P, U32, P, U32,
+                                                                U32, P, P, P, P]),
             "bitz_outer_claim": (I32, [P, U32, P, U32, P, P, P, U32, U32, P]),
             "bitz_outer_begin": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32, P, U32, P, U32, U32]),
             "bitz_outer_eq_mode": (I32, [P]),
@@ -404,6 +407,31 @@ def prove_outer_sumcheck(transcript: Transcript, initial_claim, eq_factors, prod
     rc = lib().bitz_prove_outer_sumcheck(
         transcript.h, f128(initial_claim), _p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)),
         _p(az), n, _p(bz), _nvars(nb), _p(cz), _nvars(nc), flags, _p(rounds), _p(points), C.byref(final),
+        _p(claims))
+    _check(rc, SumcheckError)
+    return OuterSumcheckOutput(rounds[: 4 * n].reshape(n, 4, 2), points[:n], to_int(final),
+                               to_int(claims[0]), to_int(claims[1]), to_int(claims[2]))
+
+
+def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, products: R1csProductMles):
+    """prove_outer_sumcheck against challenges fixed in advance: every round
+    is queued back to back, the round sums are read once."""
+    init()
+    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
+    (az, na, fa), (bz, nb, fb), (cz, nc, fc) = (_table(t) for t in (products.az, products.bz, products.cz))
+    if not (fa == fb == fc):
+        raise BitzError(-7, "products must all be host arrays or all DeviceTables")
+    if fa & DEVICE:
+        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    n = _nvars(na)
+    ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
+    rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
+    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    final = F128()
+    claims = np.zeros((3, 2), dtype=np.uint64)
+    rc = lib().bitz_prove_outer_sumcheck_with_challenges(
+        _p(ch), f128(initial_claim), _p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)),
+        _p(az), n, _p(bz), _nvars(nb), _p(cz), _nvars(nc), fa, _p(rounds), _p(points), C.byref(final),
         _p(claims))
     _check(rc, SumcheckError)
     return OuterSumcheckOutput(rounds[: 4 * n].reshape(n, 4, 2), points[:n], to_int(final),

@@ -50,6 +50,46 @@ int32_t fail(int32_t code, const std::string& msg) {
 inline f128 F(bitz_f128 x) { return f128{x.lo, x.hi}; }
 inline bitz_f128 B(f128 x) { return bitz_f128{x.lo, x.hi}; }
 
+// Device allocations are recycled through a process-wide cache keyed by
+// size: a proof's scratch (tens of MiB to GiB) would otherwise pay hipMalloc
+// and hipFree on every call.
+struct DeviceCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_blocks;
+    size_t cached_bytes = 0;
+    void* get(size_t n) {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = free_blocks.lower_bound(n);
+            if (it != free_blocks.end() && it->first <= 2 * n) {
+                void* p = it->second;
+                cached_bytes -= it->first;
+                free_blocks.erase(it);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            // drop the cache and retry once
+            release_all();
+            if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+        }
+        return p;
+    }
+    void put(void* p, size_t n) {
+        std::lock_guard<std::mutex> lock(mu);
+        free_blocks.emplace(n, p);
+        cached_bytes += n;
+    }
+    void release_all() {
+        std::lock_guard<std::mutex> lock(mu);
+        (void)hipDeviceSynchronize();
+        for (auto& kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached_bytes = 0;
+    }
+} g_cache;
+
 // A device allocation owned by a handle.
 struct Dev {
     void* p = nullptr;
@@ -59,17 +99,21 @@ struct Dev {
     Dev& operator=(const Dev&) = delete;
     ~Dev() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) g_cache.put(p, bytes);
         p = nullptr;
         bytes = 0;
     }
     int32_t alloc(size_t n) {
-        release();
-        if (n == 0) n = 16;
-        if (hipMalloc(&p, n) != hipSuccess) {
-            p = nullptr;
-            return fail(BITZ_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        if (p) {
+            // Replacing a buffer a queued kernel may still use: let the
+            // device drain before the block returns to the cache.
+            (void)hipDeviceSynchronize();
+            release();
         }
+        if (n == 0) n = 16;
+        n = (n + 255) & ~(size_t)255;
+        p = g_cache.get(n);
+        if (!p) return fail(BITZ_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
         bytes = n;
         return BITZ_OK;
     }
@@ -149,7 +193,10 @@ int log2_exact(size_t n) {
 struct StreamOwner {
     hipStream_t s = nullptr;
     ~StreamOwner() {
-        if (s) (void)hipStreamDestroy(s);
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     }
     int32_t create() {
         HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -236,6 +283,11 @@ extern "C" int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return BITZ_OK;
 }
+extern "C" int32_t bitz_release_cache(void) {
+    g_cache.release_all();
+    return BITZ_OK;
+}
+
 extern "C" int32_t bitz_synchronize(void) {
     HIP_TRY(hipDeviceSynchronize());
     return BITZ_OK;
@@ -311,9 +363,9 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
                                   uint32_t flags) {
     if (n && (!a || !b || !out)) return fail(BITZ_ERR_ARG, "null pointer");
     if (n == 0) return BITZ_OK;
+    Dev da, db, dout;
     StreamOwner so;
     TRY(so.create());
-    Dev da, db, dout;
     TRY(upload(da, a, n, flags, so.s));
     TRY(upload(db, b, n, flags, so.s));
     TRY(dout.alloc(n * sizeof(f128)));
@@ -325,9 +377,9 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
 extern "C" int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags) {
     if (!out || (k && !r)) return fail(BITZ_ERR_ARG, "null pointer");
     if (k > 34) return fail(BITZ_ERR_ARG, "eq table too large");
+    Dev dr, dout, scratch;
     StreamOwner so;
     TRY(so.create());
-    Dev dr, dout, scratch;
     TRY(upload(dr, r, k, flags, so.s));
     const size_t n = (size_t)1 << k;
     TRY(dout.alloc(n * sizeof(f128)));
@@ -341,10 +393,10 @@ extern "C" int32_t bitz_mle_fold(bitz_f128* evals, uint32_t* num_vars, const bit
     // mle.rs:136-142: too many challenges is rejected before mutating.
     if (m > *num_vars) return fail(BITZ_ERR_TOO_MANY, "more challenges than variables");
     if (m == 0) return BITZ_OK;
+    Dev a, b;
     StreamOwner so;
     TRY(so.create());
     const size_t n = (size_t)1 << *num_vars;
-    Dev a, b;
     TRY(upload(a, evals, n, flags, so.s));
     TRY(b.alloc(n / 2 * sizeof(f128)));
     std::vector<bitz_f128> rr(m);
@@ -369,10 +421,10 @@ extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, 
                                      uint32_t point_len, bitz_f128* out, uint32_t flags) {
     if (!evals || !out || (point_len && !point)) return fail(BITZ_ERR_ARG, "null pointer");
     if (point_len != num_vars) return fail(BITZ_ERR_ARG, "point width differs from num_vars");
+    Dev a, b;
     StreamOwner so;
     TRY(so.create());
     const size_t n = (size_t)1 << num_vars;
-    Dev a, b;
     TRY(upload(a, evals, n, flags, so.s));
     TRY(b.alloc(std::max<size_t>(n / 2, 1) * sizeof(f128)));
     std::vector<bitz_f128> rr(point_len);
@@ -401,7 +453,6 @@ extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, 
 
 // ======================================================= outer sumcheck ===
 struct bitz_outer {
-    StreamOwner so;
     int n = 0;
     size_t len = 0, low_len = 0, high_len = 0;
     int round = 0;
@@ -435,6 +486,7 @@ struct bitz_outer {
         tm.collect();
         return BITZ_OK;
     }
+    StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
 static int32_t outer_check(uint32_t low_vars, uint32_t high_vars, uint32_t az_vars, uint32_t bz_vars,
@@ -488,19 +540,20 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     hipStream_t s = h->so.s;
     const int n = h->n;
     if (n == 0) return false;
-    // Probe entries T[0], T[2^i] of both factors.
+    // The factors are small (2^(n/2) entries for the reference's split); the
+    // recovered tau is checked against all of them below.
+    const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
+    if (nl + nh > ((size_t)1 << 20)) return false;
+    std::vector<f128> fl(nl), fh(nh);
+    if (hipMemcpyAsync(fl.data(), h->eqL[0].f(), nl * sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(fh.data(), h->eqH[0].f(), nh * sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return false;
     std::vector<f128> pl(low_vars + 1), ph(high_vars + 1);
-    if (hipMemcpyAsync(pl.data(), h->eqL[0].f(), sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
-    for (int i = 0; i < low_vars; ++i)
-        if (hipMemcpyAsync(&pl[1 + i], h->eqL[0].f() + ((size_t)1 << i), sizeof(f128), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-            return false;
-    if (hipMemcpyAsync(ph.data(), h->eqH[0].f(), sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
-    for (int i = 0; i < high_vars; ++i)
-        if (hipMemcpyAsync(&ph[1 + i], h->eqH[0].f() + ((size_t)1 << i), sizeof(f128), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-            return false;
-    if (hipStreamSynchronize(s) != hipSuccess) return false;
+    pl[0] = fl[0];
+    ph[0] = fh[0];
+    for (int i = 0; i < low_vars; ++i) pl[1 + i] = fl[(size_t)1 << i];
+    for (int i = 0; i < high_vars; ++i) ph[1 + i] = fh[(size_t)1 << i];
     std::vector<f128> tau;
     if (!recover_tau(pl, low_vars, tau) || !recover_tau(ph, high_vars, tau)) return false;
     h->tau = tau;
@@ -542,15 +595,12 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
                        h->weights.f());
     if (hipGetLastError() != hipSuccess) return false;
     // Compare the rebuilt factors with the given ones.
-    const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
-    std::vector<f128> a(nl), b(nl), c(nh), d(nh);
+    std::vector<f128> a(nl), c(nh);
     if (hipMemcpyAsync(a.data(), h->weights.f() + check_low, nl * sizeof(f128), hipMemcpyDeviceToHost, s) ||
-        hipMemcpyAsync(b.data(), h->eqL[0].f(), nl * sizeof(f128), hipMemcpyDeviceToHost, s) ||
         hipMemcpyAsync(c.data(), h->weights.f() + check_high, nh * sizeof(f128), hipMemcpyDeviceToHost, s) ||
-        hipMemcpyAsync(d.data(), h->eqH[0].f(), nh * sizeof(f128), hipMemcpyDeviceToHost, s) ||
         hipStreamSynchronize(s))
         return false;
-    if (memcmp(a.data(), b.data(), nl * sizeof(f128)) || memcmp(c.data(), d.data(), nh * sizeof(f128))) return false;
+    if (memcmp(a.data(), fl.data(), nl * sizeof(f128)) || memcmp(c.data(), fh.data(), nh * sizeof(f128))) return false;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
@@ -568,16 +618,19 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
 
 // Launches the eq-factored round-k kernel (folding at r when k > 0) into
 // slot k.
-static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3]) {
+static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3], Timed* tm = nullptr) {
+    if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
     const bool q1 = k == 0 || hostf::is_zero(h->tau[k]);
-    const unsigned block = (unsigned)std::min<size_t>(g.threads, 1024);
-    const unsigned grid = (unsigned)(g.threads / block);
+    // Whole waves only (the block reduction needs them); threads past
+    // g.threads contribute zero.
+    const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, 1024));
+    const unsigned grid = (unsigned)((g.threads + block - 1) / block);
     const f128* w = h->weights.f();
     uint32_t* slot = h->acc.u32() + 12 * k;
     hipStream_t s = h->so.s;
     const size_t pairs = g.threads * g.iters;
-    h->tm.begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 64.0 * pairs : 288.0 * pairs);
+    tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 64.0 * pairs : 288.0 * pairs);
     const size_t smem = bitz::kOuterEqSmem;
 #define OUTER_EQ_ARGS                                                                                     \
     h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
@@ -589,7 +642,7 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3])
     else
         hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
 #undef OUTER_EQ_ARGS
-    h->tm.end(s);
+    tm->end(s);
     return check_launch();
 }
 
@@ -818,10 +871,10 @@ extern "C" int32_t bitz_outer_claim(const bitz_f128* eq_low, uint32_t low_vars, 
                                     const bitz_f128* cz, uint32_t num_vars, uint32_t flags, bitz_f128* out) {
     if (!eq_low || !eq_high || !az || !bz || !cz || !out) return fail(BITZ_ERR_ARG, "null pointer");
     TRY(outer_check(low_vars, high_vars, num_vars, num_vars, num_vars));
+    Dev in, lo, hi, acc;
     StreamOwner so;
     TRY(so.create());
     const size_t n = (size_t)1 << num_vars;
-    Dev in, lo, hi, acc;
     const f128* t[3];
     if (flags & BITZ_DEVICE) {
         t[0] = reinterpret_cast<const f128*>(az);
@@ -903,9 +956,123 @@ done:
     return rc;
 }
 
+// prove_outer_sumcheck against challenges fixed in advance (a transcript
+// whose challenges do not depend on the messages, such as the seeded
+// stream): every round is enqueued back to back and the round sums are read
+// once at the end.  Outputs as bitz_prove_outer_sumcheck; eval_points are the
+// given challenges.
+extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
+    const bitz_f128* challenges, bitz_f128 initial_claim, const bitz_f128* eq_low, uint32_t low_vars,
+    const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az, uint32_t az_vars, const bitz_f128* bz,
+    uint32_t bz_vars, const bitz_f128* cz, uint32_t cz_vars, uint32_t flags, bitz_f128* round_polys,
+    bitz_f128* eval_points, bitz_f128* final_claim, bitz_f128* claims) {
+    if (!final_claim || !claims || (az_vars && (!round_polys || !eval_points || !challenges)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    TRY(outer_check(low_vars, high_vars, az_vars, bz_vars, cz_vars));
+    bitz_outer* h = nullptr;
+    TRY(bitz_outer_begin(&h, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars, flags));
+    const int n = (int)az_vars;
+    int32_t rc = BITZ_OK;
+    hipStream_t s = h->so.s;
+    std::vector<bitz_f128> raw(3 * (size_t)std::max(n, 1));
+    std::vector<Timed> timers(n + 1);
+    f128 claim = F(initial_claim);
+    if (n == 0) {
+        rc = bitz_outer_finish(h, claims);
+        *final_claim = initial_claim;
+        bitz_outer_free(h);
+        return rc;
+    }
+    h->started = true;
+    // Enqueue every round.
+    if (h->eqmode) {
+        if ((rc = outer_eq_launch(h, 0, f128{0, 0}, nullptr, &timers[0]))) goto done;
+    } else {
+        const size_t pairs = h->len / 2;
+        hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, s, h->cur[0], h->cur[1],
+                           h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(),
+                           h->acc.u32());
+        if ((rc = check_launch())) goto done;
+    }
+    for (int k = 0; k < n; ++k) {
+        const f128 r = F(challenges[k]);
+        const size_t next_len = h->len / 2;
+        Dev& dst = (h->where == 1) ? h->Q : h->P;
+        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+        f128* out[3] = {dst.f(), dst.f() + cap, dst.f() + 2 * cap};
+        if (h->eqmode) {
+            if (k + 1 < n) {
+                if ((rc = outer_eq_launch(h, k + 1, r, out, &timers[k + 1]))) goto done;
+            } else {
+                hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                                   out[0], out[1], out[2], (size_t)1, r);
+            }
+        } else {
+            uint32_t* slot = h->acc.u32() + 12 * (k + 1);
+            if (h->low_len > 1) {
+                const size_t nl = h->low_len / 2;
+                hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nl)), dim3(kBlock), 0, s, h->eqL[h->curL].f(),
+                                   h->eqL[1 - h->curL].f(), nl, r);
+                h->curL ^= 1;
+                h->low_len = nl;
+                if (next_len > 1)
+                    hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
+                                       h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], next_len / 2, r,
+                                       h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(), slot);
+                else
+                    hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                                       out[0], out[1], out[2], (size_t)1, r);
+            } else {
+                const size_t nh = h->high_len / 2;
+                hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nh)), dim3(kBlock), 0, s, h->eqH[h->curH].f(),
+                                   h->eqH[1 - h->curH].f(), nh, r);
+                h->curH ^= 1;
+                h->high_len = nh;
+                if (nh == 1)
+                    hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                                       out[0], out[1], out[2], (size_t)1, r);
+                else
+                    hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(nh / 2)), dim3(kBlock), 0, s,
+                                       h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], nh / 2, r,
+                                       h->eqL[h->curL].f(), 0, h->eqH[h->curH].f(), slot);
+            }
+        }
+        if ((rc = check_launch())) goto done;
+        for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+        h->where = (h->where == 1) ? 2 : 1;
+        h->len = next_len;
+        h->round += 1;
+    }
+    // One read of every round's sums and of the terminal evaluations.
+    if (hipMemcpyAsync(raw.data(), h->acc.p, 48 * (size_t)n, hipMemcpyDeviceToHost, s) != hipSuccess) {
+        rc = fail(BITZ_ERR_DEVICE, "round sums readback failed");
+        goto done;
+    }
+    if ((rc = bitz_outer_finish(h, claims))) goto done;
+    for (auto& tmr : timers) tmr.collect();
+    for (int k = 0; k < n; ++k) {
+        bitz_f128 co[3];
+        if (h->eqmode) {
+            outer_eq_coeffs(h, k, &raw[3 * k], co);
+            outer_eq_advance(h, k, F(challenges[k]));
+        } else {
+            memcpy(co, &raw[3 * k], sizeof co);
+        }
+        // sumcheck.rs:760-794: c1 = claim + c2 + c3, claim <- g(r).
+        f128 c[4] = {F(co[0]), hostf::add(claim, hostf::add(F(co[1]), F(co[2]))), F(co[1]), F(co[2])};
+        const f128 r = F(challenges[k]);
+        claim = hostf::horner(c, 4, r);
+        for (int i = 0; i < 4; ++i) round_polys[4 * k + i] = B(c[i]);
+        eval_points[k] = challenges[k];
+    }
+    *final_claim = B(claim);
+done:
+    bitz_outer_free(h);
+    return rc;
+}
+
 // ======================================================= inner sumcheck ===
 struct bitz_inner {
-    StreamOwner so;
     int n = 0;
     size_t len = 0;
     int round = 0;
@@ -917,6 +1084,7 @@ struct bitz_inner {
     Dev acc;
     Pinned host;
     Timed tm;
+    StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
 extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
@@ -1058,10 +1226,10 @@ done:
 
 // ========================================================= matrices =======
 struct bitz_matrices {
-    StreamOwner so;
     size_t rows = 0, rows_padded = 0, cols = 0;
     int row_vars = 0, col_vars = 0;
     Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
+    StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
 extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint32_t row_vars,
@@ -1163,9 +1331,9 @@ extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
 
 // ================================================================ M^T =====
 struct bitz_mt {
-    StreamOwner so;
     size_t row_count = 0, cols = 0;
     Dev offsets, rows;
+    StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
 extern "C" int32_t bitz_mt_prepare(bitz_mt** out, size_t row_count, size_t columns,
@@ -1249,13 +1417,13 @@ extern "C" void bitz_mt_free(bitz_mt* h) { delete h; }
 
 // ================================================================== GKR ===
 struct bitz_gkr {
-    StreamOwner so;
     int log_leaves = 0, log_groups = 0;
     Dev store, eq, eq_scratch, point, acc;
     std::vector<size_t> off;  // offset of layer j (0 = leaves)
     Pinned host;
     bool consumed = false;
     Timed tm;
+    StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
 extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint32_t log_leaves,

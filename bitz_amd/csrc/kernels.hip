@@ -14,9 +14,10 @@
 namespace bitz {
 
 // ---------------------------------------------------------------- helpers --
+// Requires blockDim.x to be a multiple of 64 (whole waves), at most 1024.
 template <int K>
 __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
-    __shared__ uint32_t red[kBlock / 64][4 * K];
+    __shared__ uint32_t red[1024 / 64][4 * K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -34,11 +35,11 @@ __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
             for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
     }
     __syncthreads();
-    if (threadIdx.x < 4 * K) {
+    const int waves = (int)(blockDim.x >> 6);
+    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
         uint32_t x = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) x ^= red[w][threadIdx.x];
-        if (x) atomicXor(dst + threadIdx.x, x);
+        for (int w = 0; w < waves; ++w) x ^= red[w][j];
+        if (x) atomicXor(dst + j, x);
     }
 }
 
@@ -409,8 +410,9 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
     Smem& sm = *reinterpret_cast<Smem*>(smem_raw);
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     const g4 rr = g_from(r);
-    // Tables pay off only when a workgroup handles enough pairs.
-    const bool tabs = iters > 1 || threads >= 1024;
+    // Folding through the LDS table of r also shortens the per-thread chain
+    // of dependent multiplies, which bounds the small tail rounds.
+    const bool tabs = true;
     if (FOLD && tabs) build_const_table(sm.r_tab, sm.basis, rr);
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {

@@ -60,6 +60,9 @@ int32_t bitz_device_free(void* ptr);
 int32_t bitz_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int32_t bitz_synchronize(void);
+/* Device scratch is recycled across calls through a size-keyed cache; this
+ * returns every cached block to the runtime. */
+int32_t bitz_release_cache(void);
 
 /* Kernel timing: with profiling on, the sumcheck kernels ("outer_round0",
  * "outer_fold_next", "inner_round0", "inner_fold_next", "gkr_sums",
@@ -126,6 +129,19 @@ int32_t bitz_prove_outer_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
 int32_t bitz_outer_claim(const bitz_f128* eq_low, uint32_t low_vars, const bitz_f128* eq_high,
                          uint32_t high_vars, const bitz_f128* az, const bitz_f128* bz,
                          const bitz_f128* cz, uint32_t num_vars, uint32_t flags, bitz_f128* out);
+
+/* The same prover against challenges fixed in advance, challenges[num_vars]
+ * (a transcript whose challenges do not depend on the messages, such as the
+ * seeded stream): all rounds are queued back to back with one read of the
+ * round sums at the end.  eval_points receives the challenges. */
+int32_t bitz_prove_outer_sumcheck_with_challenges(const bitz_f128* challenges, bitz_f128 initial_claim,
+                                                  const bitz_f128* eq_low, uint32_t low_vars,
+                                                  const bitz_f128* eq_high, uint32_t high_vars,
+                                                  const bitz_f128* az, uint32_t az_vars,
+                                                  const bitz_f128* bz, uint32_t bz_vars,
+                                                  const bitz_f128* cz, uint32_t cz_vars, uint32_t flags,
+                                                  bitz_f128* round_polys, bitz_f128* eval_points,
+                                                  bitz_f128* final_claim, bitz_f128* claims);
 
 /* Per-round form of the same prover for callers that keep the transcript:
  * begin -> round0 ([c0,c2,c3]) -> fold_next(r_k) per round (returns the next

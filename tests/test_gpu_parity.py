@@ -262,7 +262,7 @@ def test_outer_generic_mode_for_non_eq_factors_and_tau_one():
     assert not mode_of(lo, hi, a, b, c)
     outer_with(a, b, c, lo, hi, 3, ints(elements(103, n)))
     tau = elements(104, n)
-    tau[2] = 1  # eq_table(tau) has T[0] = 0: tau cannot be recovered
+    tau[2] = (1, 0)  # the element 1: eq_table(tau) has T[0] = 0, tau cannot be recovered
     lo, hi = O.eq_table(tau[:5]), O.eq_table(tau[5:])
     assert not mode_of(lo, hi, a, b, c)
     outer_with(a, b, c, lo, hi, 0, ints(elements(105, n)))
@@ -276,3 +276,18 @@ def test_outer_eq_mode_when_eq_bound_vanishes():
     ch[3] = to_int(tau[3]) ^ 1
     out = outer_with(a, b, c, lo, hi, 9, ch)
     assert all(to_int(x) == 0 for x in out.round_polynomials[5][[0, 2, 3]])
+
+
+@pytest.mark.parametrize("n,split,eq", [(1, 0, True), (9, 4, True), (14, 7, True), (10, 5, False)])
+def test_outer_with_challenges_matches_oracle(n, split, eq):
+    a, b, c, tau, lo, hi = outer_case(n, split, 120 + n)
+    if not eq:
+        lo, hi = elements_fast(121, len(lo)), elements_fast(122, len(hi))
+    ch = elements(123, n)
+    ot = O.ListTranscript(ints(ch))
+    rc, ref = O.outer_sumcheck(ot, 77, lo, hi, a, b, c)
+    assert rc == 0
+    out = G.prove_outer_sumcheck_with_challenges(ch, 77, (lo, hi), G.R1csProductMles(a, b, c))
+    assert_outer_equal(ref, out)
+    dev = G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c))
+    assert_outer_equal(ref, G.prove_outer_sumcheck_with_challenges(ch, 77, (lo, hi), dev))
