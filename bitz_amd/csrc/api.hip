@@ -470,9 +470,11 @@ struct bitz_outer {
     // eq-factored mode (the factors are eq tables of a point tau)
     bool eqmode = false;
     std::vector<f128> tau, tau_inv;
-    Dev tau_dev, weights, jobs;
+    Dev tau_dev, weights, jobs, lfull, htabs, rtab;
     struct Geo {
         size_t threads, iters, la_off, lb_off, h_off;
+        size_t lfull_off = SIZE_MAX;  // full L table (single-iteration rounds)
+        size_t htab_off = 0;          // first H constant table (multi-iteration rounds)
         int la_bits;
     };
     std::vector<Geo> geo;
@@ -601,6 +603,34 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
         hipStreamSynchronize(s))
         return false;
     if (memcmp(a.data(), fl.data(), nl * sizeof(f128)) || memcmp(c.data(), fh.data(), nh * sizeof(f128))) return false;
+    // Single-iteration rounds read L(t) whole; multi-iteration rounds read one
+    // LDS constant table per H(i).
+    size_t lfull_total = 0, htab_total = 0;
+    for (int k = 0; k < n; ++k) {
+        auto& g = h->geo[k];
+        if (g.iters == 1) {
+            g.lfull_off = lfull_total;
+            lfull_total += g.threads;
+        } else {
+            g.htab_off = htab_total;
+            htab_total += g.iters;
+        }
+    }
+    if (h->lfull.alloc(std::max<size_t>(lfull_total, 1) * sizeof(f128)) ||
+        h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
+        h->rtab.alloc(kConstTableEntries * sizeof(uint4)))
+        return false;
+    for (int k = 0; k < n; ++k) {
+        const auto& g = h->geo[k];
+        const f128* w = h->weights.f();
+        if (g.iters == 1)
+            hipLaunchKernelGGL(bitz::k_eq_combine, dim3(grid_for(g.threads)), dim3(kBlock), 0, s, w + g.la_off,
+                               g.la_bits, w + g.lb_off, h->lfull.f() + g.lfull_off, g.threads);
+        else
+            hipLaunchKernelGGL(bitz::k_const_tables, dim3((unsigned)g.iters), dim3(1024), 0, s, w + g.h_off,
+                               static_cast<uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries);
+    }
+    if (hipGetLastError() != hipSuccess) return false;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
@@ -622,19 +652,26 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3],
     if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
     const bool q1 = k == 0 || hostf::is_zero(h->tau[k]);
-    // Whole waves only (the block reduction needs them); threads past
-    // g.threads contribute zero.
-    const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, 1024));
+    const bool fold = k > 0;
+    // Multi-iteration rounds: 1024-thread workgroups share each LDS table
+    // copy.  Single-iteration rounds: 256-thread workgroups, so small rounds
+    // spread over every CU.  Whole waves only (the block reduction needs them).
+    const size_t want = g.iters > 1 ? 1024 : 256;
+    const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, want));
     const unsigned grid = (unsigned)((g.threads + block - 1) / block);
     const f128* w = h->weights.f();
     uint32_t* slot = h->acc.u32() + 12 * k;
     hipStream_t s = h->so.s;
+    uint4* rtab = static_cast<uint4*>(h->rtab.p);
+    if (fold) hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab);
     const size_t pairs = g.threads * g.iters;
     tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 64.0 * pairs : 288.0 * pairs);
-    const size_t smem = bitz::kOuterEqSmem;
+    const size_t smem = (fold ? 2 : 1) * kConstTableEntries * sizeof(uint4);
+    const f128* lfull = g.iters == 1 ? h->lfull.f() + g.lfull_off : nullptr;
+    const uint4* htabs = static_cast<const uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries;
 #define OUTER_EQ_ARGS                                                                                     \
     h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
-        g.threads, g.iters, r, w + g.la_off, g.la_bits, w + g.lb_off, w + g.h_off, slot
+        g.threads, g.iters, rtab, w + g.la_off, g.la_bits, w + g.lb_off, lfull, htabs, slot
     if (k == 0)
         hipLaunchKernelGGL((bitz::k_outer_eq<false, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
     else if (q1)

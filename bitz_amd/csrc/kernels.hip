@@ -350,6 +350,27 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
     }
 }
 
+// Constant tables are built once into global memory (one workgroup per
+// constant) and copied into LDS by every workgroup that multiplies by them.
+__global__ __launch_bounds__(1024) void k_const_tables(const f128* consts, uint4* out) {
+    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 basis[128];
+    build_const_table(tab, basis, g_load(consts + blockIdx.x));
+    uint4* dst = out + (size_t)blockIdx.x * kConstTableEntries;
+    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
+}
+
+__global__ __launch_bounds__(1024) void k_const_table1(f128 c, uint4* out) {
+    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 basis[128];
+    build_const_table(tab, basis, g_from(c));
+    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) out[e] = tab[e];
+}
+
+__device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__ src) {
+    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) lds[e] = src[e];
+}
+
 // ============================================ outer sumcheck, eq-factored ===
 // Used when the eq factors are eq tables of a point tau (checked by the
 // driver).  Round k's polynomial is g_k(X) = S_k eq(tau_k, X) q_k(X) with
@@ -361,10 +382,10 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
 // once per thread at the end, from two small tables La (low bits) and Lb.
 namespace {
 
+// h_tab first: a kernel that does not fold needs only the first 64 KiB.
 struct Smem {
-    uint4 r_tab[kConstTableEntries];
     uint4 h_tab[kConstTableEntries];
-    uint4 basis[128];
+    uint4 r_tab[kConstTableEntries];
 };
 
 template <bool FOLD>
@@ -403,47 +424,49 @@ __device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f1
 
 template <bool FOLD, bool Q1>
 __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao,
-                                                   f128* bo, f128* co, size_t threads, size_t iters, f128 r,
-                                                   const f128* la, int la_bits, const f128* lb,
-                                                   const f128* h, uint32_t* acc) {
+                                                   f128* bo, f128* co, size_t threads, size_t iters,
+                                                   const uint4* r_tab, const f128* la, int la_bits,
+                                                   const f128* lb, const f128* l_full, const uint4* h_tabs,
+                                                   uint32_t* acc) {
     extern __shared__ uint4 smem_raw[];
     Smem& sm = *reinterpret_cast<Smem*>(smem_raw);
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    const g4 rr = g_from(r);
-    // Folding through the LDS table of r also shortens the per-thread chain
-    // of dependent multiplies, which bounds the small tail rounds.
-    const bool tabs = true;
-    if (FOLD && tabs) build_const_table(sm.r_tab, sm.basis, rr);
+    if (FOLD) copy_table(sm.r_tab, r_tab);
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {
-        const g4 hw = g_load(h + i);
         if (iters > 1) {
-            __syncthreads();  // previous iteration's readers are done with h_tab
-            build_const_table(sm.h_tab, sm.basis, hw);
+            __syncthreads();  // the previous iteration's readers are done with h_tab
+            copy_table(sm.h_tab, h_tabs + i * kConstTableEntries);
         }
+        __syncthreads();
         if (t < threads) {
             const size_t y = i * threads + t;
             g4 a0, a1, b0, b1, c0, c1;
-            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, rr, tabs, a0, a1, b0, b1, c0, c1);
+            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, g_zero(), true, a0, a1, b0, b1, c0, c1);
             const g4 p0 = g_add(g_mul(a0, b0), c0);
             const g4 pi = g_mul(g_add(a0, a1), g_add(b0, b1));
+            g4 p1;
+            if (Q1) p1 = g_add(g_mul(a1, b1), c1);
             if (iters > 1) {
                 s0 = g_add(s0, g_mul_table(sm.h_tab, p0));
                 si = g_add(si, g_mul_table(sm.h_tab, pi));
-            } else {
-                s0 = g_add(s0, g_mul(hw, p0));
-                si = g_add(si, g_mul(hw, pi));
-            }
-            if (Q1) {
-                const g4 p1 = g_add(g_mul(a1, b1), c1);
-                s1 = g_add(s1, iters > 1 ? g_mul_table(sm.h_tab, p1) : g_mul(hw, p1));
+                if (Q1) s1 = g_add(s1, g_mul_table(sm.h_tab, p1));
+            } else {  // H is eq over no coordinates: 1
+                s0 = p0;
+                si = pi;
+                if (Q1) s1 = p1;
             }
         }
     }
     g4 v[3] = {g_zero(), g_zero(), g_zero()};
     if (t < threads) {
-        const size_t mask = ((size_t)1 << la_bits) - 1;
-        const g4 L = g_mul(g_load(la + (t & mask)), g_load(lb + (t >> la_bits)));
+        g4 L;
+        if (l_full) {
+            L = g_load(l_full + t);
+        } else {
+            const size_t mask = ((size_t)1 << la_bits) - 1;
+            L = g_mul(g_load(la + (t & mask)), g_load(lb + (t >> la_bits)));
+        }
         v[0] = g_mul(L, s0);
         v[1] = g_mul(L, si);
         if (Q1) v[2] = g_mul(L, s1);
@@ -451,18 +474,10 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
     block_xor_to_global<3>(v, acc);
 }
 
-template __global__ void k_outer_eq<false, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
-                                                  size_t, size_t, f128, const f128*, int, const f128*,
-                                                  const f128*, uint32_t*);
-template __global__ void k_outer_eq<false, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
-                                                 size_t, size_t, f128, const f128*, int, const f128*,
-                                                 const f128*, uint32_t*);
-template __global__ void k_outer_eq<true, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
-                                                 size_t, size_t, f128, const f128*, int, const f128*,
-                                                 const f128*, uint32_t*);
-template __global__ void k_outer_eq<true, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*,
-                                                size_t, size_t, f128, const f128*, int, const f128*,
-                                                const f128*, uint32_t*);
+template __global__ void k_outer_eq<false, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
+template __global__ void k_outer_eq<false, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
+template __global__ void k_outer_eq<true, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
+template __global__ void k_outer_eq<true, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
 
 // Builds many small eq tables at once: job j writes eq(tau[start_j ..
 // start_j + k_j), .) (2^k_j entries) at out + offset_j, one workgroup per job

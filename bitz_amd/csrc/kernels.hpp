@@ -46,9 +46,11 @@ __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* ro
 
 template <bool FOLD, bool Q1>
 __global__ void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
-                           size_t threads, size_t iters, f128 r, const f128* la, int la_bits, const f128* lb,
-                           const f128* h, uint32_t* acc);
+                           size_t threads, size_t iters, const uint4* r_tab, const f128* la, int la_bits,
+                           const f128* lb, const f128* l_full, const uint4* h_tabs, uint32_t* acc);
+__global__ void k_const_tables(const f128* consts, uint4* out);
+__global__ void k_const_table1(f128 c, uint4* out);
 __global__ void k_eq_jobs(const f128* tau, const uint32_t* jobs, f128* out);
-constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries + 128);
+constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
 }  // namespace bitz
