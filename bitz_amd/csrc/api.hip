@@ -121,17 +121,43 @@ struct Dev {
     uint32_t* u32() const { return static_cast<uint32_t*>(p); }
 };
 
-// Pinned host staging for round sums.
+// Pinned host staging for round sums: fixed-size blocks recycled through a
+// process-wide pool (hipHostMalloc per proof would cost more than a round).
+struct PinnedPool {
+    static constexpr size_t kBytes = 64 * 1024;
+    std::mutex mu;
+    std::vector<void*> free_blocks;
+    void* get() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!free_blocks.empty()) {
+                void* p = free_blocks.back();
+                free_blocks.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, kBytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+        return p;
+    }
+    void put(void* p) {
+        std::lock_guard<std::mutex> lock(mu);
+        free_blocks.push_back(p);
+    }
+} g_pinned;
+
 struct Pinned {
     void* p = nullptr;
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
     ~Pinned() {
-        if (p) (void)hipHostFree(p);
+        if (p) g_pinned.put(p);
     }
     int32_t alloc(size_t n) {
-        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
-        }
+        if (n > PinnedPool::kBytes) return fail(BITZ_ERR_ARG, "pinned staging request too large");
+        if (!p) p = g_pinned.get();
+        if (!p) return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
         return BITZ_OK;
     }
 };
@@ -190,16 +216,41 @@ int log2_exact(size_t n) {
     return k;
 }
 
+// Streams are recycled through a pool as well: one per live handle, all at
+// the default priority.
+struct StreamPool {
+    std::mutex mu;
+    std::vector<hipStream_t> free_streams;
+    hipStream_t get() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!free_streams.empty()) {
+                hipStream_t s = free_streams.back();
+                free_streams.pop_back();
+                return s;
+            }
+        }
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        return s;
+    }
+    void put(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu);
+        free_streams.push_back(s);
+    }
+} g_streams;
+
 struct StreamOwner {
     hipStream_t s = nullptr;
     ~StreamOwner() {
         if (s) {
             (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
+            g_streams.put(s);
         }
     }
     int32_t create() {
-        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        s = g_streams.get();
+        if (!s) return fail(BITZ_ERR_DEVICE, "hipStreamCreateWithFlags failed");
         return BITZ_OK;
     }
 };
@@ -217,22 +268,47 @@ struct Profiler {
     std::map<std::string, KernelStat> stats;
 } g_prof;
 
+// Events come from a pool so that a profiled proof creates none.
+struct EventPool {
+    std::mutex mu;
+    std::vector<hipEvent_t> free_events;
+    hipEvent_t get() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!free_events.empty()) {
+                hipEvent_t e = free_events.back();
+                free_events.pop_back();
+                return e;
+            }
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void put(hipEvent_t e) {
+        std::lock_guard<std::mutex> lock(mu);
+        free_events.push_back(e);
+    }
+} g_events;
+
 struct Timed {
     hipEvent_t start = nullptr, stop = nullptr;
     const char* name = nullptr;
     double bytes = 0;
+    Timed() = default;
+    Timed(const Timed&) = delete;
+    Timed& operator=(const Timed&) = delete;
     ~Timed() {
-        if (start) (void)hipEventDestroy(start);
-        if (stop) (void)hipEventDestroy(stop);
+        if (start) g_events.put(start);
+        if (stop) g_events.put(stop);
     }
     // Records the start event if profiling is on.
     void begin(hipStream_t s, const char* kernel, double algo_bytes) {
         name = nullptr;
         if (!g_prof.on) return;
-        if (!start) {
-            (void)hipEventCreate(&start);
-            (void)hipEventCreate(&stop);
-        }
+        if (!start) start = g_events.get();
+        if (!stop) stop = g_events.get();
+        if (!start || !stop) return;
         name = kernel;
         bytes = algo_bytes;
         (void)hipEventRecord(start, s);
@@ -454,10 +530,11 @@ extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, 
 // ======================================================= outer sumcheck ===
 struct bitz_outer {
     int n = 0;
-    size_t len = 0, low_len = 0, high_len = 0;
+    size_t N = 0, len = 0, low_len = 0, high_len = 0;
     int round = 0;
     bool started = false, finished = false;
     Dev in, P, Q;      // inputs (owned copy when given on host), ping-pong scratch
+    const f128* inputs[3] = {nullptr, nullptr, nullptr};
     const f128* cur[3] = {nullptr, nullptr, nullptr};
     int where = 0;     // 0 = inputs, 1 = P, 2 = Q
     size_t capP = 0, capQ = 0;
@@ -467,25 +544,60 @@ struct bitz_outer {
     Pinned host;
     Timed tm;
 
-    // eq-factored mode (the factors are eq tables of a point tau)
-    bool eqmode = false;
+    // Eq-factored mode: the factors are eq tables of a point tau.  The
+    // device checks them (k_eq_check) while the first round runs; the host
+    // reads the verdict with the first round sums.
+    bool eqmode = false;   // eq-factored kernels in use
+    bool eq_valid = false; // the host has read and accepted the check
+    int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
-    Dev tau_dev, weights, jobs, lfull, htabs, rtab;
+    std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
+    Dev htabs, rtabs, chal;
+    struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
+        int table, a, b;
+    };
     struct Geo {
-        size_t threads, iters, la_off, lb_off, h_off;
-        size_t lfull_off = SIZE_MAX;  // full L table (single-iteration rounds)
-        size_t htab_off = 0;          // first H constant table (multi-iteration rounds)
+        size_t threads, iters;
+        Run la, lb;
         int la_bits;
+        int h_start;       // first iteration variable
+        size_t htab_off;   // first H(i) table (multi-iteration rounds)
     };
     std::vector<Geo> geo;
     f128 S{1, 0}, sigma{0, 0};
     f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
 
+    // acc layout (32-bit words): round k's sums at 12 k; the eq check at
+    // eq_off(); the pipelined prover's final A(r), B(r), C(r) at fin_off().
+    size_t eq_off() const { return 12 * (size_t)(n + 2); }
+    size_t fin_off() const { return eq_off() + 12 + 4 * (size_t)(n + 2); }
+    size_t acc_words() const { return fin_off() + 12; }
+    const f128* table_ptr(int t) const { return t == 0 ? eqL[0].f() : eqH[0].f(); }
+    int table_base(int t) const { return t == 0 ? 0 : ell; }
+
+    int32_t read_words(size_t words) {
+        HIP_TRY(hipMemcpyAsync(host.p, acc.p, words * 4, hipMemcpyDeviceToHost, so.s));
+        HIP_TRY(hipStreamSynchronize(so.s));
+        tm.collect();
+        return BITZ_OK;
+    }
     int32_t read_slot(int slot, bitz_f128* out3) {
         HIP_TRY(hipMemcpyAsync(host.p, acc.u32() + 12 * slot, 48, hipMemcpyDeviceToHost, so.s));
         HIP_TRY(hipStreamSynchronize(so.s));
         memcpy(out3, host.p, 48);
         tm.collect();
+        return BITZ_OK;
+    }
+    const uint32_t* host_words() const { return static_cast<const uint32_t*>(host.p); }
+    // Back to the round-0 state (for a second pipelined attempt).
+    int32_t rewind() {
+        for (int t = 0; t < 3; ++t) cur[t] = inputs[t];
+        where = 0;
+        len = N;
+        round = 0;
+        S = f128{1, 0};
+        sigma = f128{0, 0};
+        HIP_TRY(hipMemsetAsync(acc.p, 0, acc.bytes, so.s));
         return BITZ_OK;
     }
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
@@ -519,139 +631,168 @@ static void batch_inverse(std::vector<f128>& v) {
     }
 }
 
-// Recovers tau from eq_table(tau) (crates/poly/src/eq.rs:63-99): with
-// T[0] = prod (1 + tau_j) and T[2^i] = tau_i prod_{j != i} (1 + tau_j),
-// tau_i = T[2^i] / (T[0] + T[2^i]).  False if a denominator vanishes.
-static bool recover_tau(const std::vector<f128>& probes, int k, std::vector<f128>& out) {
-    // probes = [T[0], T[1], T[2], T[4], ...]
-    std::vector<f128> den(k);
-    for (int i = 0; i < k; ++i) {
-        den[i] = hostf::add(probes[0], probes[1 + i]);
-        if (hostf::is_zero(den[i])) return false;
-    }
-    batch_inverse(den);
-    for (int i = 0; i < k; ++i) out.push_back(hostf::mul(probes[1 + i], den[i]));
-    return true;
+static f128 f128_at(const uint32_t* w) {
+    f128 x;
+    memcpy(&x, w, sizeof x);
+    return x;
 }
 
-// Sets up the eq-factored mode: recover tau, build every round's weight
-// tables on the device, and check eq_table(tau_low), eq_table(tau_high)
-// against the given factors there.  Returns false (generic mode) when the
-// factors are not eq tables of a point.
+// Rounds whose tables hold at most 2^kTailLog elements run in the
+// single-workgroup tail kernel (pipelined prover only).
+constexpr int kTailLog = 10;
+static_assert(kTailLog <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
+
+// Plans the eq-factored mode and launches the device check of the factors
+// and the H(i) tables.  Returns false (general kernels) when the factors are
+// too large to check cheaply.  Nothing here reads back: the verdict arrives
+// with the first round sums.
 static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     hipStream_t s = h->so.s;
     const int n = h->n;
     if (n == 0) return false;
-    // The factors are small (2^(n/2) entries for the reference's split); the
-    // recovered tau is checked against all of them below.
     const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
-    if (nl + nh > ((size_t)1 << 20)) return false;
-    std::vector<f128> fl(nl), fh(nh);
-    if (hipMemcpyAsync(fl.data(), h->eqL[0].f(), nl * sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(fh.data(), h->eqH[0].f(), nh * sizeof(f128), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return false;
-    std::vector<f128> pl(low_vars + 1), ph(high_vars + 1);
-    pl[0] = fl[0];
-    ph[0] = fh[0];
-    for (int i = 0; i < low_vars; ++i) pl[1 + i] = fl[(size_t)1 << i];
-    for (int i = 0; i < high_vars; ++i) ph[1 + i] = fh[(size_t)1 << i];
-    std::vector<f128> tau;
-    if (!recover_tau(pl, low_vars, tau) || !recover_tau(ph, high_vars, tau)) return false;
-    h->tau = tau;
-    h->tau_inv = tau;
-    batch_inverse(h->tau_inv);
-
-    // Geometry and weight-table jobs: job = (offset, tau start, k).
-    std::vector<uint32_t> jobs;
-    size_t off = 0;
-    auto add_job = [&](size_t start, int k) {
-        const size_t o = off;
-        jobs.push_back((uint32_t)o);
-        jobs.push_back((uint32_t)start);
-        jobs.push_back((uint32_t)k);
-        off += (size_t)1 << k;
-        return o;
-    };
+    if (nl + nh > ((size_t)1 << 21)) return false;
+    h->ell = low_vars;
+    h->m = high_vars;
+    const int ell = low_vars;
+    bitz::HtabDesc desc{};
     h->geo.resize(n);
+    size_t htab_total = 0;
     for (int k = 0; k < n; ++k) {
-        const int pbits = n - k - 1;             // log2 pairs of round k
-        const int tbits = std::min(pbits, 18);   // log2 threads
+        const int pbits = n - k - 1;            // log2 pairs of round k
+        const int tbits = std::min(pbits, 18);  // log2 threads
         auto& g = h->geo[k];
         g.threads = (size_t)1 << tbits;
         g.iters = (size_t)1 << (pbits - tbits);
-        g.la_bits = std::min(tbits, 9);
-        g.la_off = add_job(k + 1, g.la_bits);
-        g.lb_off = add_job(k + 1 + g.la_bits, tbits - g.la_bits);
-        g.h_off = add_job(k + 1 + tbits, pbits - tbits);
-    }
-    const size_t check_low = add_job(0, low_vars), check_high = add_job(low_vars, high_vars);
-    const size_t n_jobs = jobs.size() / 3;
-    if (h->tau_dev.alloc(n * sizeof(f128)) || h->weights.alloc(off * sizeof(f128)) ||
-        h->jobs.alloc(jobs.size() * 4))
-        return false;
-    if (hipMemcpyAsync(h->tau_dev.p, h->tau.data(), n * sizeof(f128), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(h->jobs.p, jobs.data(), jobs.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
-        return false;
-    hipLaunchKernelGGL(bitz::k_eq_jobs, dim3((unsigned)n_jobs), dim3(kBlock), 0, s, h->tau_dev.f(), h->jobs.u32(),
-                       h->weights.f());
-    if (hipGetLastError() != hipSuccess) return false;
-    // Compare the rebuilt factors with the given ones.
-    std::vector<f128> a(nl), c(nh);
-    if (hipMemcpyAsync(a.data(), h->weights.f() + check_low, nl * sizeof(f128), hipMemcpyDeviceToHost, s) ||
-        hipMemcpyAsync(c.data(), h->weights.f() + check_high, nh * sizeof(f128), hipMemcpyDeviceToHost, s) ||
-        hipStreamSynchronize(s))
-        return false;
-    if (memcmp(a.data(), fl.data(), nl * sizeof(f128)) || memcmp(c.data(), fh.data(), nh * sizeof(f128))) return false;
-    // Single-iteration rounds read L(t) whole; multi-iteration rounds read one
-    // LDS constant table per H(i).
-    size_t lfull_total = 0, htab_total = 0;
-    for (int k = 0; k < n; ++k) {
-        auto& g = h->geo[k];
-        if (g.iters == 1) {
-            g.lfull_off = lfull_total;
-            lfull_total += g.threads;
+        const int a = k + 1, b = a + tbits;
+        if (a < ell && ell < b) {  // the thread variables cross from eq_low into eq_high
+            g.la = {0, a, ell};
+            g.lb = {1, ell, b};
         } else {
+            const int t = a < ell ? 0 : 1;
+            const int lab = std::min(tbits, 9);
+            g.la = {t, a, a + lab};
+            g.lb = {t, a + lab, b};
+        }
+        g.la_bits = g.la.b - g.la.a;
+        g.h_start = b;
+        g.htab_off = 0;
+        if (g.iters > 1) {
+            desc.first[desc.rounds] = (uint32_t)htab_total;
+            desc.start[desc.rounds] = b;
+            desc.rounds += 1;
             g.htab_off = htab_total;
             htab_total += g.iters;
         }
     }
-    if (h->lfull.alloc(std::max<size_t>(lfull_total, 1) * sizeof(f128)) ||
-        h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
-        h->rtab.alloc(kConstTableEntries * sizeof(uint4)))
+    desc.first[desc.rounds] = (uint32_t)htab_total;
+    if (h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
+        h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
+        h->chal.alloc((size_t)n * sizeof(f128)))
         return false;
-    for (int k = 0; k < n; ++k) {
-        const auto& g = h->geo[k];
-        const f128* w = h->weights.f();
-        if (g.iters == 1)
-            hipLaunchKernelGGL(bitz::k_eq_combine, dim3(grid_for(g.threads)), dim3(kBlock), 0, s, w + g.la_off,
-                               g.la_bits, w + g.lb_off, h->lfull.f() + g.lfull_off, g.threads);
-        else
-            hipLaunchKernelGGL(bitz::k_const_tables, dim3((unsigned)g.iters), dim3(1024), 0, s, w + g.h_off,
-                               static_cast<uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries);
-    }
+    const unsigned bl = (unsigned)std::max<size_t>(1, (nl + kBlock - 1) / kBlock);
+    const unsigned bh = (unsigned)std::max<size_t>(1, (nh + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(bitz::k_eq_check, dim3(bl + bh), dim3(kBlock), 0, s, h->eqL[0].f(), low_vars,
+                       h->eqH[0].f(), high_vars, h->acc.u32() + h->eq_off(), bl);
+    if (htab_total)
+        hipLaunchKernelGGL(bitz::k_htabs, dim3((unsigned)htab_total), dim3(1024), 0, s, h->eqL[0].f(), low_vars,
+                           h->eqH[0].f(), desc, static_cast<uint4*>(h->htabs.p));
     if (hipGetLastError() != hipSuccess) return false;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<false, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_eq<true, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        const void* fns[4] = {reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
+                              reinterpret_cast<const void*>(&bitz::k_outer_eq<false, true>),
+                              reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
+                              reinterpret_cast<const void*>(&bitz::k_outer_eq<true, true>)};
+        for (const void* f : fns)
+            (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(3 * sizeof(uint4) << bitz::kTailMaxLog));
         attr_set = true;
     }
     return true;
 }
 
-// Launches the eq-factored round-k kernel (folding at r when k > 0) into
-// slot k.
-static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3], Timed* tm = nullptr) {
+// Reads the device check (eq region of the host copy of acc) and, when the
+// factors are eq tables, recovers tau and the constants the kernels' weights
+// carry.  For a factor T with variables V and a run R of them, T[x << s]
+// = c_R eq(tau_R, x) with c_R = prod_{i in V \ R} (1 + tau_i), since
+// T[0] = prod_V (1 + tau_i) and T[2^i] / T[0] = tau_i / (1 + tau_i).
+static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
+    const int n = h->n, ell = h->ell;
+    if (region[0]) return false;
+    if (!hostf::eq(f128_at(region + 4), hostf::one()) || !hostf::eq(f128_at(region + 8), hostf::one()))
+        return false;
+    const uint32_t* probes = region + 12;
+    std::vector<f128> den(n), num(n);
+    for (int t = 0; t < 2; ++t) {
+        const int base = t == 0 ? 0 : ell, bits = t == 0 ? ell : h->m;
+        const uint32_t* P = probes + 4 * (t == 0 ? 0 : ell + 1);
+        const f128 t0 = f128_at(P);
+        if (hostf::is_zero(t0)) return false;
+        for (int i = 0; i < bits; ++i) {
+            num[base + i] = f128_at(P + 4 * (1 + i));
+            den[base + i] = hostf::add(t0, num[base + i]);
+            if (hostf::is_zero(den[base + i])) return false;
+        }
+    }
+    batch_inverse(den);
+    std::vector<f128> tau(n), one_tau(n);
+    for (int i = 0; i < n; ++i) {
+        tau[i] = hostf::mul(num[i], den[i]);
+        one_tau[i] = hostf::add(hostf::one(), tau[i]);
+    }
+    // Per factor t with variables [lo, hi): pre[t][j] = prod_{lo <= i < lo + j}
+    // (1 + tau_i), suf[t][j] = prod_{lo + j <= i < hi}.  For a run [a, b)
+    // of its variables, c = pre[t][a - lo] * suf[t][b - lo].
+    std::vector<f128> pre[2], suf[2];
+    for (int t = 0; t < 2; ++t) {
+        const int lo = t == 0 ? 0 : ell, hi = t == 0 ? ell : n, w = hi - lo;
+        pre[t].assign(w + 1, hostf::one());
+        suf[t].assign(w + 1, hostf::one());
+        for (int j = 0; j < w; ++j) pre[t][j + 1] = hostf::mul(pre[t][j], one_tau[lo + j]);
+        for (int j = w; j-- > 0;) suf[t][j] = hostf::mul(suf[t][j + 1], one_tau[lo + j]);
+    }
+    auto kappa = [&](int t, int a, int b) {
+        const int lo = t == 0 ? 0 : ell, hi = t == 0 ? ell : n;
+        a = std::max(lo, std::min(a, hi));
+        b = std::max(a, std::min(b, hi));
+        return hostf::mul(pre[t][a - lo], suf[t][b - lo]);
+    };
+    std::vector<f128> kap(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const auto& g = h->geo[k];
+        f128 c = hostf::mul(kappa(g.la.table, g.la.a, g.la.b), kappa(g.lb.table, g.lb.a, g.lb.b));
+        if (g.iters > 1) {
+            if (g.h_start < ell)
+                c = hostf::mul(c, hostf::mul(kappa(0, g.h_start, ell), kappa(1, ell, n)));
+            else
+                c = hostf::mul(c, kappa(1, g.h_start, n));
+        }
+        kap[k] = c;
+        // Tail weights: eq_low[y_lo << (k+1)] * eq_high[y_hi << max(0, k+1-ell)].
+        if (k + 1 < ell)
+            kap[n + k] = hostf::mul(kappa(0, k + 1, ell), kappa(1, ell, n));
+        else
+            kap[n + k] = kappa(1, k + 1, n);
+    }
+    batch_inverse(kap);
+    h->scale_big.assign(kap.begin(), kap.begin() + n);
+    h->scale_tail.assign(kap.begin() + n, kap.end());
+    h->tau = tau;
+    h->tau_inv = tau;
+    batch_inverse(h->tau_inv);
+    h->eq_valid = true;
+    return true;
+}
+
+// Launches the eq-factored round-k kernel (folding the previous tables
+// through the LDS table rtab when k > 0) into slot k.
+static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab, bool q1, f128* const out[3],
+                               Timed* tm = nullptr) {
     if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
-    const bool q1 = k == 0 || hostf::is_zero(h->tau[k]);
     const bool fold = k > 0;
     // Multi-iteration rounds: 1024-thread workgroups share each LDS table
     // copy.  Single-iteration rounds: 256-thread workgroups, so small rounds
@@ -659,21 +800,22 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3],
     const size_t want = g.iters > 1 ? 1024 : 256;
     const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, want));
     const unsigned grid = (unsigned)((g.threads + block - 1) / block);
-    const f128* w = h->weights.f();
     uint32_t* slot = h->acc.u32() + 12 * k;
     hipStream_t s = h->so.s;
-    uint4* rtab = static_cast<uint4*>(h->rtab.p);
-    if (fold) hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab);
     const size_t pairs = g.threads * g.iters;
-    tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 64.0 * pairs : 288.0 * pairs);
+    tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 96.0 * pairs : 288.0 * pairs);
     const size_t smem = (fold ? 2 : 1) * kConstTableEntries * sizeof(uint4);
-    const f128* lfull = g.iters == 1 ? h->lfull.f() + g.lfull_off : nullptr;
     const uint4* htabs = static_cast<const uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries;
+    const f128* la = h->table_ptr(g.la.table);
+    const f128* lb = h->table_ptr(g.lb.table);
+    const int la_shift = g.la.a - h->table_base(g.la.table), lb_shift = g.lb.a - h->table_base(g.lb.table);
 #define OUTER_EQ_ARGS                                                                                     \
     h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
-        g.threads, g.iters, rtab, w + g.la_off, g.la_bits, w + g.lb_off, lfull, htabs, slot
-    if (k == 0)
+        g.threads, g.iters, rtab, la, g.la_bits, la_shift, lb, lb_shift, htabs, slot
+    if (!fold && q1)
         hipLaunchKernelGGL((bitz::k_outer_eq<false, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+    else if (!fold)
+        hipLaunchKernelGGL((bitz::k_outer_eq<false, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
     else if (q1)
         hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
     else
@@ -683,19 +825,19 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, f128 r, f128* const out[3],
     return check_launch();
 }
 
-// Round k's [c0, c2, c3] from q(0), q(inf) and q(1) (read or recovered):
-// g_k(X) = S_k (1 + tau_k + X) q_k(X), q_k(X) = q0 + q1 X + q2 X^2.
-static void outer_eq_coeffs(bitz_outer* h, int k, const bitz_f128* raw, bitz_f128* co) {
+// Round k's [c0, c2, c3] from the kernel's sums (times the round's scale):
+// q(0), q(inf) and q(1) when computed, else q(1) recovered from
+// sigma_k = (1 + tau_k) q(0) + tau_k q(1).  g_k(X) = S_k (1 + tau_k + X) q_k(X).
+static void outer_eq_coeffs(bitz_outer* h, int k, const bitz_f128* raw, bool have_q1, const f128& scale,
+                            bitz_f128* co) {
     const f128 tau = h->tau[k];
     const f128 one_tau = hostf::add(hostf::one(), tau);
-    const f128 q0 = F(raw[0]), q2 = F(raw[1]);
+    const f128 q0 = hostf::mul(F(raw[0]), scale), q2 = hostf::mul(F(raw[1]), scale);
     f128 q_at1;
-    if (k == 0 || hostf::is_zero(tau)) {
-        q_at1 = F(raw[2]);
-    } else {
-        // sigma_k = q_{k-1}(r_{k-1}) = (1 + tau_k) q(0) + tau_k q(1)
+    if (have_q1)
+        q_at1 = hostf::mul(F(raw[2]), scale);
+    else
         q_at1 = hostf::mul(hostf::add(h->sigma, hostf::mul(one_tau, q0)), h->tau_inv[k]);
-    }
     const f128 q1 = hostf::add(hostf::add(q_at1, q0), q2);
     h->q[0] = q0;
     h->q[1] = q1;
@@ -729,13 +871,14 @@ extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, u
     hipStream_t s = h->so.s;
     h->n = (int)az_vars;
     const size_t N = (size_t)1 << az_vars;
+    h->N = N;
     h->len = N;
     h->low_len = (size_t)1 << low_vars;
     h->high_len = (size_t)1 << high_vars;
     if (flags & BITZ_DEVICE) {
-        h->cur[0] = reinterpret_cast<const f128*>(az);
-        h->cur[1] = reinterpret_cast<const f128*>(bz);
-        h->cur[2] = reinterpret_cast<const f128*>(cz);
+        h->inputs[0] = reinterpret_cast<const f128*>(az);
+        h->inputs[1] = reinterpret_cast<const f128*>(bz);
+        h->inputs[2] = reinterpret_cast<const f128*>(cz);
     } else {
         if ((rc = h->in.alloc(3 * N * sizeof(f128)))) return cleanup(rc);
         const bitz_f128* src[3] = {az, bz, cz};
@@ -743,9 +886,10 @@ extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, u
             if (hipMemcpyAsync(h->in.f() + t * N, src[t], N * sizeof(f128), hipMemcpyHostToDevice, s) !=
                 hipSuccess)
                 return cleanup(fail(BITZ_ERR_DEVICE, "upload of product tables failed"));
-            h->cur[t] = h->in.f() + t * N;
+            h->inputs[t] = h->in.f() + t * N;
         }
     }
+    for (int t = 0; t < 3; ++t) h->cur[t] = h->inputs[t];
     h->capP = std::max<size_t>(N / 2, 1);
     h->capQ = std::max<size_t>(N / 4, 1);
     if ((rc = h->P.alloc(3 * h->capP * sizeof(f128)))) return cleanup(rc);
@@ -754,17 +898,39 @@ extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, u
     if ((rc = upload(h->eqH[0], eq_high, h->high_len, flags, s))) return cleanup(rc);
     if ((rc = h->eqL[1].alloc(std::max<size_t>(h->low_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->eqH[1].alloc(std::max<size_t>(h->high_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
-    if ((rc = h->acc.alloc((size_t)(h->n + 2) * 12 * sizeof(uint32_t)))) return cleanup(rc);
+    if ((rc = h->acc.alloc(h->acc_words() * sizeof(uint32_t)))) return cleanup(rc);
     if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
-    if ((rc = h->host.alloc(64))) return cleanup(rc);
+    if ((rc = h->host.alloc(h->acc_words() * sizeof(uint32_t)))) return cleanup(rc);
     h->eqmode = outer_setup_eq(h, (int)low_vars, (int)high_vars);
     (void)hipGetLastError();  // a failed probe must not leak into later checks
     *out = h;
     return BITZ_OK;
 }
 
-extern "C" int32_t bitz_outer_eq_mode(const bitz_outer* h) { return h && h->eqmode ? 1 : 0; }
+// Reads the device check of the factors if the host has not yet.
+static int32_t outer_confirm_eq(bitz_outer* h) {
+    if (!h->eqmode || h->eq_valid) return BITZ_OK;
+    TRY(h->read_words(h->fin_off()));
+    if (!outer_validate_eq(h, h->host_words() + h->eq_off())) h->eqmode = false;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_eq_mode(bitz_outer* h) {
+    if (!h) return 0;
+    if (outer_confirm_eq(h)) return 0;
+    return h->eqmode ? 1 : 0;
+}
+
+static int32_t outer_generic_round0(bitz_outer* h) {
+    const size_t pairs = h->len / 2;
+    h->tm.begin(h->so.s, "outer_round0", 48.0 * (double)h->len);
+    hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, h->so.s, h->cur[0],
+                       h->cur[1], h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len),
+                       h->eqH[h->curH].f(), h->acc.u32());
+    h->tm.end(h->so.s);
+    return check_launch();
+}
 
 extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
     if (!h || !co) return fail(BITZ_ERR_ARG, "null pointer");
@@ -775,19 +941,25 @@ extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
         return BITZ_OK;
     }
     if (h->eqmode) {
-        TRY(outer_eq_launch(h, 0, f128{0, 0}, nullptr));
+        // q(1) is computed: the claim the caller holds is not trusted here.
+        // The factor check is read with the first sums.
+        TRY(outer_eq_launch(h, 0, nullptr, true, nullptr));
         bitz_f128 raw[3];
-        TRY(h->read_slot(0, raw));
-        outer_eq_coeffs(h, 0, raw, co);
-        return BITZ_OK;
+        if (h->eq_valid) {
+            TRY(h->read_slot(0, raw));
+        } else {
+            TRY(h->read_words(h->fin_off()));
+            memcpy(raw, h->host.p, 48);
+            if (!outer_validate_eq(h, h->host_words() + h->eq_off())) h->eqmode = false;
+        }
+        if (h->eqmode) {
+            outer_eq_coeffs(h, 0, raw, true, h->scale_big[0], co);
+            return BITZ_OK;
+        }
+        // Not eq tables of a point: the general kernels take over.
+        HIP_TRY(hipMemsetAsync(h->acc.p, 0, 48, h->so.s));
     }
-    const size_t pairs = h->len / 2;
-    h->tm.begin(h->so.s, "outer_round0", 48.0 * (double)h->len);
-    hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, h->so.s, h->cur[0],
-                       h->cur[1], h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len),
-                       h->eqH[h->curH].f(), h->acc.u32());
-    h->tm.end(h->so.s);
-    TRY(check_launch());
+    TRY(outer_generic_round0(h));
     return h->read_slot(0, co);
 }
 
@@ -807,14 +979,17 @@ extern "C" int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 rb, bitz_f128* 
         const int k = h->round;  // r is round k's challenge
         outer_eq_advance(h, k, r);
         if (k + 1 < h->n) {
-            TRY(outer_eq_launch(h, k + 1, r, out));
+            uint4* rtab = static_cast<uint4*>(h->rtabs.p);
+            hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab);
+            const bool q1 = hostf::is_zero(h->tau[k + 1]);
+            TRY(outer_eq_launch(h, k + 1, rtab, q1, out));
             for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
             h->where = (h->where == 1) ? 2 : 1;
             h->len = next_len;
             h->round += 1;
             bitz_f128 raw[3];
             TRY(h->read_slot(h->round, raw));
-            outer_eq_coeffs(h, h->round, raw, co);
+            outer_eq_coeffs(h, h->round, raw, q1, h->scale_big[h->round], co);
             return BITZ_OK;
         }
         hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2], out[0],
@@ -993,11 +1168,168 @@ done:
     return rc;
 }
 
+// Enqueues a whole eq-factored proof with the challenges already on the
+// device (h->chal): rounds 0 .. k0-1 as grid-wide kernels, the rest in the
+// tail kernel, then one read of every round's sums, the factor check and
+// the final evaluations.  q1mask selects the rounds that compute q(1).
+static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Timed>& timers) {
+    hipStream_t s = h->so.s;
+    const int n = h->n;
+    const int k0 = std::max(0, n - kTailLog);
+    uint4* rtabs = static_cast<uint4*>(h->rtabs.p);
+    if (k0 >= 2)
+        hipLaunchKernelGGL(bitz::k_const_tables, dim3((unsigned)(k0 - 1)), dim3(1024), 0, s, h->chal.f(), rtabs);
+    for (int k = 0; k < k0; ++k) {
+        f128* out[3] = {nullptr, nullptr, nullptr};
+        if (k > 0) {
+            Dev& dst = (h->where == 1) ? h->Q : h->P;
+            const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+            out[0] = dst.f();
+            out[1] = dst.f() + cap;
+            out[2] = dst.f() + 2 * cap;
+        }
+        TRY(outer_eq_launch(h, k, k > 0 ? rtabs + (size_t)(k - 1) * kConstTableEntries : nullptr,
+                            (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k]));
+        if (k > 0) {
+            for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+            h->where = (h->where == 1) ? 2 : 1;
+            h->len /= 2;
+        }
+    }
+    bitz::TailArgs ta{};
+    ta.a = h->cur[0];
+    ta.b = h->cur[1];
+    ta.c = h->cur[2];
+    ta.chal = h->chal.f();
+    ta.L = h->eqL[0].f();
+    ta.H = h->eqH[0].f();
+    ta.ell = h->ell;
+    ta.n = n;
+    ta.k0 = k0;
+    ta.fold_first = k0 > 0;
+    ta.len_in_log = k0 > 0 ? n - k0 + 1 : n;
+    ta.q1mask = q1mask;
+    ta.acc = h->acc.u32();
+    ta.fin = reinterpret_cast<f128*>(h->acc.u32() + h->fin_off());
+    const size_t smem = 3 * sizeof(uint4) << (n - k0);
+    timers[n].begin(s, "outer_tail", 0.0);
+    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(1024), smem, s, ta);
+    timers[n].end(s);
+    TRY(check_launch());
+    h->round = n;
+    HIP_TRY(hipMemcpyAsync(h->host.p, h->acc.p, h->acc_words() * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (auto& tmr : timers) tmr.collect();
+    return BITZ_OK;
+}
+
+// Turns the pipelined eq-factored sums into round coefficients (without
+// c1).  With check_claim (round 0 without q(1)) the chain of q(1)
+// recoveries started from the caller's claim, and the last round's
+// q_{n-1}(r) must equal A(r) B(r) + C(r); with every r_k nonzero a wrong
+// claim changes it.  Returns false when the sums cannot be used as they
+// are (a recovery would divide by tau_k = 0, or the check fails).
+static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 initial_claim,
+                              uint64_t q1mask, bool check_claim, std::vector<bitz_f128>& co) {
+    const int n = h->n, k0 = std::max(0, n - kTailLog);
+    const uint32_t* w = h->host_words();
+    for (int k = 0; k < n; ++k)
+        if (!((q1mask >> k) & 1) && hostf::is_zero(h->tau[k])) return false;
+    h->S = hostf::one();
+    h->sigma = initial_claim;
+    co.resize(3 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        bitz_f128 raw[3];
+        memcpy(raw, w + 12 * k, 48);
+        const f128& sc = k >= k0 ? h->scale_tail[k] : h->scale_big[k];
+        outer_eq_coeffs(h, k, raw, (q1mask >> k) & 1, sc, &co[3 * (size_t)k]);
+        outer_eq_advance(h, k, F(challenges[k]));
+    }
+    if (check_claim) {
+        const uint32_t* fin = w + h->fin_off();
+        const f128 v = hostf::add(hostf::mul(f128_at(fin), f128_at(fin + 4)), f128_at(fin + 8));
+        if (!hostf::eq(v, h->sigma)) return false;
+    }
+    return true;
+}
+
+// The general-factor kernels, all rounds queued back to back; the sums are
+// read with the final evaluations.
+static int32_t outer_pipeline_generic(bitz_outer* h, const bitz_f128* challenges, std::vector<Timed>& timers,
+                                      std::vector<bitz_f128>& co, bitz_f128* claims) {
+    hipStream_t s = h->so.s;
+    const int n = h->n;
+    timers[0].begin(s, "outer_round0", 48.0 * (double)h->len);
+    const size_t pairs = h->len / 2;
+    hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, s, h->cur[0], h->cur[1],
+                       h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(),
+                       h->acc.u32());
+    timers[0].end(s);
+    TRY(check_launch());
+    for (int k = 0; k < n; ++k) {
+        const f128 r = F(challenges[k]);
+        const size_t next_len = h->len / 2;
+        Dev& dst = (h->where == 1) ? h->Q : h->P;
+        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+        f128* out[3] = {dst.f(), dst.f() + cap, dst.f() + 2 * cap};
+        uint32_t* slot = h->acc.u32() + 12 * (k + 1);
+        if (h->low_len > 1) {
+            const size_t nl = h->low_len / 2;
+            hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nl)), dim3(kBlock), 0, s, h->eqL[h->curL].f(),
+                               h->eqL[1 - h->curL].f(), nl, r);
+            h->curL ^= 1;
+            h->low_len = nl;
+            if (next_len > 1) {
+                timers[k + 1].begin(s, "outer_fold_next", 72.0 * (double)h->len);
+                hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
+                                   h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], next_len / 2, r,
+                                   h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(), slot);
+                timers[k + 1].end(s);
+            } else {
+                hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                                   out[0], out[1], out[2], (size_t)1, r);
+            }
+        } else {
+            const size_t nh = h->high_len / 2;
+            hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nh)), dim3(kBlock), 0, s, h->eqH[h->curH].f(),
+                               h->eqH[1 - h->curH].f(), nh, r);
+            h->curH ^= 1;
+            h->high_len = nh;
+            if (nh == 1) {
+                hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
+                                   out[0], out[1], out[2], (size_t)1, r);
+            } else {
+                timers[k + 1].begin(s, "outer_fold_next", 72.0 * (double)h->len);
+                hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(nh / 2)), dim3(kBlock), 0, s,
+                                   h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], nh / 2, r,
+                                   h->eqL[h->curL].f(), 0, h->eqH[h->curH].f(), slot);
+                timers[k + 1].end(s);
+            }
+        }
+        TRY(check_launch());
+        for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+        h->where = (h->where == 1) ? 2 : 1;
+        h->len = next_len;
+        h->round += 1;
+    }
+    HIP_TRY(hipMemcpyAsync(h->host.p, h->acc.p, 48 * (size_t)n, hipMemcpyDeviceToHost, s));
+    TRY(bitz_outer_finish(h, claims));  // synchronises
+    for (auto& tmr : timers) tmr.collect();
+    co.assign(static_cast<const bitz_f128*>(h->host.p), static_cast<const bitz_f128*>(h->host.p) + 3 * (size_t)n);
+    return BITZ_OK;
+}
+
 // prove_outer_sumcheck against challenges fixed in advance (a transcript
 // whose challenges do not depend on the messages, such as the seeded
 // stream): every round is enqueued back to back and the round sums are read
 // once at the end.  Outputs as bitz_prove_outer_sumcheck; eval_points are the
 // given challenges.
+//
+// Eq-factored factors take a first, optimistic pass: round 0 skips q(1)
+// (recovered from the claim) unless a challenge is zero, and no later round
+// computes it.  If the factor check, a tau_k = 0 or the claim check rejects
+// that pass, a second pass computes q(1) wherever the sums need it; factors
+// that are not eq tables fall back to the general kernels.
 extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
     const bitz_f128* challenges, bitz_f128 initial_claim, const bitz_f128* eq_low, uint32_t low_vars,
     const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az, uint32_t az_vars, const bitz_f128* bz,
@@ -1010,9 +1342,7 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
     TRY(bitz_outer_begin(&h, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars, flags));
     const int n = (int)az_vars;
     int32_t rc = BITZ_OK;
-    hipStream_t s = h->so.s;
-    std::vector<bitz_f128> raw(3 * (size_t)std::max(n, 1));
-    std::vector<Timed> timers(n + 1);
+    std::vector<bitz_f128> co;
     f128 claim = F(initial_claim);
     if (n == 0) {
         rc = bitz_outer_finish(h, claims);
@@ -1021,82 +1351,45 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
         return rc;
     }
     h->started = true;
-    // Enqueue every round.
     if (h->eqmode) {
-        if ((rc = outer_eq_launch(h, 0, f128{0, 0}, nullptr, &timers[0]))) goto done;
-    } else {
-        const size_t pairs = h->len / 2;
-        hipLaunchKernelGGL(bitz::k_outer_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, s, h->cur[0], h->cur[1],
-                           h->cur[2], pairs, h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(),
-                           h->acc.u32());
-        if ((rc = check_launch())) goto done;
-    }
-    for (int k = 0; k < n; ++k) {
-        const f128 r = F(challenges[k]);
-        const size_t next_len = h->len / 2;
-        Dev& dst = (h->where == 1) ? h->Q : h->P;
-        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
-        f128* out[3] = {dst.f(), dst.f() + cap, dst.f() + 2 * cap};
-        if (h->eqmode) {
-            if (k + 1 < n) {
-                if ((rc = outer_eq_launch(h, k + 1, r, out, &timers[k + 1]))) goto done;
-            } else {
-                hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
-                                   out[0], out[1], out[2], (size_t)1, r);
-            }
-        } else {
-            uint32_t* slot = h->acc.u32() + 12 * (k + 1);
-            if (h->low_len > 1) {
-                const size_t nl = h->low_len / 2;
-                hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nl)), dim3(kBlock), 0, s, h->eqL[h->curL].f(),
-                                   h->eqL[1 - h->curL].f(), nl, r);
-                h->curL ^= 1;
-                h->low_len = nl;
-                if (next_len > 1)
-                    hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
-                                       h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], next_len / 2, r,
-                                       h->eqL[h->curL].f(), log2_exact(h->low_len), h->eqH[h->curH].f(), slot);
-                else
-                    hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
-                                       out[0], out[1], out[2], (size_t)1, r);
-            } else {
-                const size_t nh = h->high_len / 2;
-                hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(nh)), dim3(kBlock), 0, s, h->eqH[h->curH].f(),
-                                   h->eqH[1 - h->curH].f(), nh, r);
-                h->curH ^= 1;
-                h->high_len = nh;
-                if (nh == 1)
-                    hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2],
-                                       out[0], out[1], out[2], (size_t)1, r);
-                else
-                    hipLaunchKernelGGL(bitz::k_outer_fold_next, dim3(grid_for(nh / 2)), dim3(kBlock), 0, s,
-                                       h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], nh / 2, r,
-                                       h->eqL[h->curL].f(), 0, h->eqH[h->curH].f(), slot);
+        std::vector<Timed> timers(n + 1);
+        if (hipMemcpyAsync(h->chal.p, challenges, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, h->so.s) !=
+            hipSuccess) {
+            rc = fail(BITZ_ERR_DEVICE, "challenge upload failed");
+            goto done;
+        }
+        bool zero_r = false;
+        for (int k = 0; k < n; ++k) zero_r |= hostf::is_zero(F(challenges[k]));
+        uint64_t q1mask = zero_r ? 1 : 0;
+        if ((rc = outer_pipeline_eq(h, q1mask, timers))) goto done;
+        if (!outer_validate_eq(h, h->host_words() + h->eq_off())) {
+            h->eqmode = false;
+            if ((rc = h->rewind())) goto done;
+        } else if (!outer_assemble_eq(h, challenges, claim, q1mask, !(q1mask & 1), co)) {
+            q1mask = 1;
+            for (int k = 0; k < n; ++k)
+                if (hostf::is_zero(h->tau[k])) q1mask |= (uint64_t)1 << k;
+            std::vector<Timed> timers2(n + 1);
+            if ((rc = h->rewind())) goto done;
+            if ((rc = outer_pipeline_eq(h, q1mask, timers2))) goto done;
+            if (!outer_assemble_eq(h, challenges, claim, q1mask, false, co)) {
+                rc = fail(BITZ_ERR_STATE, "eq-factored sums rejected twice");
+                goto done;
             }
         }
-        if ((rc = check_launch())) goto done;
-        for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
-        h->where = (h->where == 1) ? 2 : 1;
-        h->len = next_len;
-        h->round += 1;
-    }
-    // One read of every round's sums and of the terminal evaluations.
-    if (hipMemcpyAsync(raw.data(), h->acc.p, 48 * (size_t)n, hipMemcpyDeviceToHost, s) != hipSuccess) {
-        rc = fail(BITZ_ERR_DEVICE, "round sums readback failed");
-        goto done;
-    }
-    if ((rc = bitz_outer_finish(h, claims))) goto done;
-    for (auto& tmr : timers) tmr.collect();
-    for (int k = 0; k < n; ++k) {
-        bitz_f128 co[3];
         if (h->eqmode) {
-            outer_eq_coeffs(h, k, &raw[3 * k], co);
-            outer_eq_advance(h, k, F(challenges[k]));
-        } else {
-            memcpy(co, &raw[3 * k], sizeof co);
+            memcpy(claims, h->host_words() + h->fin_off(), 48);
+            h->finished = true;
         }
+    }
+    if (!h->eqmode) {
+        std::vector<Timed> timers(n + 1);
+        if ((rc = outer_pipeline_generic(h, challenges, timers, co, claims))) goto done;
+    }
+    for (int k = 0; k < n; ++k) {
         // sumcheck.rs:760-794: c1 = claim + c2 + c3, claim <- g(r).
-        f128 c[4] = {F(co[0]), hostf::add(claim, hostf::add(F(co[1]), F(co[2]))), F(co[1]), F(co[2])};
+        f128 c[4] = {F(co[3 * k]), hostf::add(claim, hostf::add(F(co[3 * k + 1]), F(co[3 * k + 2]))),
+                     F(co[3 * k + 1]), F(co[3 * k + 2])};
         const f128 r = F(challenges[k]);
         claim = hostf::horner(c, 4, r);
         for (int i = 0; i < 4; ++i) round_polys[4 * k + i] = B(c[i]);

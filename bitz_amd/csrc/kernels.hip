@@ -372,14 +372,76 @@ __device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__
 }
 
 // ============================================ outer sumcheck, eq-factored ===
-// Used when the eq factors are eq tables of a point tau (checked by the
-// driver).  Round k's polynomial is g_k(X) = S_k eq(tau_k, X) q_k(X) with
+// Used when the eq factors are eq tables of a point tau.  Round k's
+// polynomial is g_k(X) = S_k eq(tau_k, X) q_k(X) with
 //   q_k(X) = sum_y E_k(y) (A_k B_k + C_k)(X, y),   E_k(y) = eq(tau_{>k}, y),
-// a quadratic; the kernels return q(0), q(inf) and, when asked, q(1).  The
-// pair index y = i * T + t (t = global thread, i = iteration) splits the
-// weight as E_k(y) = L(t) H(i): H(i) is the same for every thread of an
-// iteration and is applied through an LDS constant table; L(t) is applied
-// once per thread at the end, from two small tables La (low bits) and Lb.
+// a quadratic; the kernels return q(0), q(inf) and, when asked, q(1).
+//
+// The weights are read straight from the given factors, without tau: for a
+// run R = [s, s + j) of variables inside one factor T, U(x) = T[x << s'] (s'
+// = s minus the factor's first variable) equals eq(tau_R, x) times a
+// constant (the product of (1 + tau_i) over T's other variables), which the
+// host divides out of the round sums.  The pair index y = i * T + t
+// (t = global thread, i = iteration) splits the weight as L(t) H(i): H(i) is
+// the same for every thread of an iteration and is applied through an LDS
+// constant table; L(t) = La(t_low) Lb(t_high) is applied once per thread.
+
+// Checks that a factor is an eq table: T[x] T[0] = T[x - 2^top] T[2^top]
+// for every x > 0 (so T[x] = T[0] prod_{i in x} rho_i) and XOR-sums the
+// entries (eq tables sum to 1; the host checks the sums and the probes).
+// out: [0] mismatch flag, [4..7] sum of L, [8..11] sum of H, from [12] the
+// probes T[0], T[1], T[2], T[4], ... of L, then of H.
+__global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, const f128* H, int m,
+                                                     uint32_t* out, unsigned blocks_l) {
+    const bool in_h = blockIdx.x >= blocks_l;
+    const f128* T = in_h ? H : L;
+    const int bits = in_h ? m : ell;
+    const size_t x = (size_t)(in_h ? blockIdx.x - blocks_l : blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t size = (size_t)1 << bits;
+    f128* probes = reinterpret_cast<f128*>(out + 12) + (in_h ? ell + 1 : 0);
+    g4 v[1] = {g_zero()};
+    if (x < size) {
+        const g4 tx = g_load(T + x);
+        v[0] = tx;
+        if (x == 0) {
+            g_store(probes, tx);
+        } else {
+            const int top = 63 - __builtin_clzll((unsigned long long)x);
+            const size_t bit = (size_t)1 << top;
+            const g4 lhs = g_mul(tx, g_load(T));
+            const g4 rhs = g_mul(g_load(T + (x ^ bit)), g_load(T + bit));
+            if ((lhs.w[0] ^ rhs.w[0]) | (lhs.w[1] ^ rhs.w[1]) | (lhs.w[2] ^ rhs.w[2]) | (lhs.w[3] ^ rhs.w[3]))
+                atomicOr(out, 1u);
+            if (x == bit) g_store(probes + 1 + top, tx);
+        }
+    }
+    block_xor_to_global<1>(v, out + (in_h ? 8 : 4));
+}
+
+// H(i) constant tables for the multi-iteration rounds: table b belongs to
+// the round j with first[j] <= b < first[j + 1]; its constant is the weight
+// of iteration i = b - first[j] over the variables [start[j], n), read from
+// the factors (a product when the run crosses from L into H).
+__global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out) {
+    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 basis[128];
+    int j = 0;
+    while (j + 1 < d.rounds && blockIdx.x >= d.first[j + 1]) ++j;
+    const uint32_t i = blockIdx.x - d.first[j];
+    const int s = d.start[j];
+    g4 c;
+    if (s < ell) {
+        const int lo_bits = ell - s;
+        const uint32_t lo = i & ((1u << lo_bits) - 1), hi = i >> lo_bits;
+        c = g_mul(g_load(L + ((size_t)lo << s)), g_load(H + hi));
+    } else {
+        c = g_load(H + ((size_t)i << (s - ell)));
+    }
+    build_const_table(tab, basis, c);
+    uint4* dst = out + (size_t)blockIdx.x * kConstTableEntries;
+    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
+}
+
 namespace {
 
 // h_tab first: a kernel that does not fold needs only the first 64 KiB.
@@ -390,8 +452,8 @@ struct Smem {
 
 template <bool FOLD>
 __device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f128* c, f128* ao,
-                                          f128* bo, f128* co, size_t y, const uint4* r_tab, const g4& rr,
-                                          bool use_tab, g4& a0, g4& a1, g4& b0, g4& b1, g4& c0, g4& c1) {
+                                          f128* bo, f128* co, size_t y, const uint4* r_tab,
+                                          g4& a0, g4& a1, g4& b0, g4& b1, g4& c0, g4& c1) {
     if (!FOLD) {
         a0 = g_load(a + 2 * y); a1 = g_load(a + 2 * y + 1);
         b0 = g_load(b + 2 * y); b1 = g_load(b + 2 * y + 1);
@@ -410,7 +472,7 @@ __device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f1
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const g4 d = g_add(in[t][2 * j + 1], in[t][2 * j]);
-            out[t][j] = g_add(in[t][2 * j], use_tab ? g_mul_table(r_tab, d) : g_mul(rr, d));
+            out[t][j] = g_add(in[t][2 * j], g_mul_table(r_tab, d));
         }
     g_store(ao + 2 * y, out[0][0]); g_store(ao + 2 * y + 1, out[0][1]);
     g_store(bo + 2 * y, out[1][0]); g_store(bo + 2 * y + 1, out[1][1]);
@@ -420,13 +482,43 @@ __device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f1
     c0 = out[2][0]; c1 = out[2][1];
 }
 
+// XOR of v over the workgroup, stored (not accumulated) at dst: for kernels
+// that run as a single workgroup.  Whole waves only.
+template <int K>
+__device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst) {
+    __shared__ uint32_t red[1024 / 64][4 * K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t x = v[k].w[i];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
+            v[k].w[i] = x;
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
+    }
+    __syncthreads();
+    const int waves = (int)(blockDim.x >> 6);
+    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
+        uint32_t x = 0;
+        for (int w = 0; w < waves; ++w) x ^= red[w][j];
+        dst[j] = x;
+    }
+}
+
 }  // namespace
 
 template <bool FOLD, bool Q1>
 __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao,
                                                    f128* bo, f128* co, size_t threads, size_t iters,
-                                                   const uint4* r_tab, const f128* la, int la_bits,
-                                                   const f128* lb, const f128* l_full, const uint4* h_tabs,
+                                                   const uint4* r_tab, const f128* la, int la_bits, int la_shift,
+                                                   const f128* lb, int lb_shift, const uint4* h_tabs,
                                                    uint32_t* acc) {
     extern __shared__ uint4 smem_raw[];
     Smem& sm = *reinterpret_cast<Smem*>(smem_raw);
@@ -442,7 +534,7 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
         if (t < threads) {
             const size_t y = i * threads + t;
             g4 a0, a1, b0, b1, c0, c1;
-            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, g_zero(), true, a0, a1, b0, b1, c0, c1);
+            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, a0, a1, b0, b1, c0, c1);
             const g4 p0 = g_add(g_mul(a0, b0), c0);
             const g4 pi = g_mul(g_add(a0, a1), g_add(b0, b1));
             g4 p1;
@@ -451,7 +543,7 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
                 s0 = g_add(s0, g_mul_table(sm.h_tab, p0));
                 si = g_add(si, g_mul_table(sm.h_tab, pi));
                 if (Q1) s1 = g_add(s1, g_mul_table(sm.h_tab, p1));
-            } else {  // H is eq over no coordinates: 1
+            } else {  // no iteration variables: H = 1
                 s0 = p0;
                 si = pi;
                 if (Q1) s1 = p1;
@@ -460,13 +552,8 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
     }
     g4 v[3] = {g_zero(), g_zero(), g_zero()};
     if (t < threads) {
-        g4 L;
-        if (l_full) {
-            L = g_load(l_full + t);
-        } else {
-            const size_t mask = ((size_t)1 << la_bits) - 1;
-            L = g_mul(g_load(la + (t & mask)), g_load(lb + (t >> la_bits)));
-        }
+        const size_t mask = ((size_t)1 << la_bits) - 1;
+        const g4 L = g_mul(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)));
         v[0] = g_mul(L, s0);
         v[1] = g_mul(L, si);
         if (Q1) v[2] = g_mul(L, s1);
@@ -474,31 +561,109 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
     block_xor_to_global<3>(v, acc);
 }
 
-template __global__ void k_outer_eq<false, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
-template __global__ void k_outer_eq<false, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
-template __global__ void k_outer_eq<true, false>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
-template __global__ void k_outer_eq<true, true>(const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, const f128*, const f128*, const uint4*, uint32_t*);
+#define OUTER_EQ_SIG                                                                                       \
+    const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, \
+        int, const f128*, int, const uint4*, uint32_t*
+template __global__ void k_outer_eq<false, false>(OUTER_EQ_SIG);
+template __global__ void k_outer_eq<false, true>(OUTER_EQ_SIG);
+template __global__ void k_outer_eq<true, false>(OUTER_EQ_SIG);
+template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
+#undef OUTER_EQ_SIG
 
-// Builds many small eq tables at once: job j writes eq(tau[start_j ..
-// start_j + k_j), .) (2^k_j entries) at out + offset_j, one workgroup per job
-// (crates/poly/src/eq.rs:63-99 level doubling).
-__global__ __launch_bounds__(kBlock) void k_eq_jobs(const f128* tau, const uint32_t* jobs, f128* out) {
-    const uint32_t offset = jobs[3 * blockIdx.x], start = jobs[3 * blockIdx.x + 1],
-                   k = jobs[3 * blockIdx.x + 2];
-    f128* table = out + offset;
-    if (threadIdx.x == 0) g_store(table, g_one());
+// The last rounds of an eq-factored proof with challenges known in advance,
+// in one workgroup: the three tables (at most kTailMax elements each after
+// the first fold) stay in LDS, and every round is a fold by r_{k-1} followed
+// by the round sums, separated by barriers instead of kernel launches.
+// Weights are read from the factors as in k_outer_eq; the round sums are
+// stored at acc + 12 k and the final A(r), B(r), C(r) at fin[0..2].
+__global__ __launch_bounds__(1024) void k_outer_tail(TailArgs A) {
+    extern __shared__ uint4 lds_raw[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int lg = A.len_in_log;
+    auto src = [&](int t) { return t == 0 ? A.a : (t == 1 ? A.b : A.c); };
+    if (A.fold_first) {
+        const g4 r = g_load(A.chal + (A.k0 - 1));
+        lg -= 1;
+        const size_t len = (size_t)1 << lg;
+        for (size_t e = tid; e < 3 * len; e += nt) {
+            const int t = (int)(e >> lg);
+            const size_t j = e & (len - 1);
+            const g4 lo = g_load(src(t) + 2 * j), hi = g_load(src(t) + 2 * j + 1);
+            const g4 v = g_add(lo, g_mul(r, g_add(lo, hi)));
+            lds_raw[e] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+        }
+    } else {
+        const size_t len = (size_t)1 << lg;
+        for (size_t e = tid; e < 3 * len; e += nt) {
+            const int t = (int)(e >> lg);
+            lds_raw[e] = *reinterpret_cast<const uint4*>(src(t) + (e & (len - 1)));
+        }
+    }
     __syncthreads();
-    for (uint32_t i = 0; i < k; ++i) {
-        const size_t half = (size_t)1 << i;
-        const g4 ri = g_load(tau + start + i);
-        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
-            const g4 parent = g_load(table + j);
-            const g4 one = g_mul(parent, ri);
-            g_store(table + j, g_add(parent, one));
-            g_store(table + j + half, one);
+    const int cap_lg = lg;  // each table occupies 2^cap_lg entries of LDS
+    auto ld = [&](int t, size_t j) {
+        const uint4 u = lds_raw[((size_t)t << cap_lg) + j];
+        g4 r;
+        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
+        return r;
+    };
+    for (int k = A.k0; k < A.n; ++k) {
+        const int pl = lg - 1;  // log2 pairs
+        const size_t P = (size_t)1 << pl;
+        const bool q1 = (A.q1mask >> k) & 1;
+        const size_t items = (q1 ? 3 : 2) * P;
+        const int lo_bits = A.ell > k + 1 ? A.ell - (k + 1) : 0;
+        const int hshift = k + 1 > A.ell ? k + 1 - A.ell : 0;
+        g4 s[3] = {g_zero(), g_zero(), g_zero()};
+        for (size_t it = tid; it < items; it += nt) {
+            const size_t y = it & (P - 1);
+            const int which = (int)(it >> pl);
+            const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
+            const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
+            g4 p;
+            if (which == 0)
+                p = g_add(g_mul(a0, b0), ld(2, 2 * y));
+            else if (which == 1)
+                p = g_mul(g_add(a0, a1), g_add(b0, b1));
+            else
+                p = g_add(g_mul(a1, b1), ld(2, 2 * y + 1));
+            const size_t ylo = y & (((size_t)1 << lo_bits) - 1), yhi = y >> lo_bits;
+            g4 w = g_load(A.H + (yhi << hshift));
+            if (lo_bits) w = g_mul(w, g_load(A.L + (ylo << (k + 1))));
+            const g4 wp = g_mul(w, p);
+            if (which == 0) s[0] = g_add(s[0], wp);
+            else if (which == 1) s[1] = g_add(s[1], wp);
+            else s[2] = g_add(s[2], wp);
+        }
+        block_xor_store<3>(s, A.acc + 12 * k);
+        // Fold by r_k: read everything first, then write in place.
+        const g4 r = g_load(A.chal + k);
+        const size_t half = P;
+        g4 nv[kTailFoldPerThread];
+#pragma unroll
+        for (int q = 0; q < kTailFoldPerThread; ++q) {
+            const size_t e = tid + (size_t)q * nt;
+            if (e < 3 * half) {
+                const int t = (int)(e >> pl);
+                const size_t j = e & (half - 1);
+                const g4 lo = ld(t, 2 * j), hi = ld(t, 2 * j + 1);
+                nv[q] = g_add(lo, g_mul(r, g_add(lo, hi)));
+            }
         }
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kTailFoldPerThread; ++q) {
+            const size_t e = tid + (size_t)q * nt;
+            if (e < 3 * half) {
+                const int t = (int)(e >> pl);
+                const size_t j = e & (half - 1);
+                lds_raw[((size_t)t << cap_lg) + j] = make_uint4(nv[q].w[0], nv[q].w[1], nv[q].w[2], nv[q].w[3]);
+            }
+        }
+        __syncthreads();
+        lg = pl;
     }
+    if (tid < 3) g_store(A.fin + tid, ld(tid, 0));
 }
 
 }  // namespace bitz

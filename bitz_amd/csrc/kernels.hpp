@@ -47,10 +47,36 @@ __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* ro
 template <bool FOLD, bool Q1>
 __global__ void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
                            size_t threads, size_t iters, const uint4* r_tab, const f128* la, int la_bits,
-                           const f128* lb, const f128* l_full, const uint4* h_tabs, uint32_t* acc);
+                           int la_shift, const f128* lb, int lb_shift, const uint4* h_tabs, uint32_t* acc);
 __global__ void k_const_tables(const f128* consts, uint4* out);
 __global__ void k_const_table1(f128 c, uint4* out);
-__global__ void k_eq_jobs(const f128* tau, const uint32_t* jobs, f128* out);
+__global__ void k_eq_check(const f128* L, int ell, const f128* H, int m, uint32_t* out, unsigned blocks_l);
+
+// H(i) tables of the multi-iteration rounds (see k_htabs).
+constexpr int kMaxRounds = 36;
+struct HtabDesc {
+    int rounds;
+    uint32_t first[kMaxRounds + 1];
+    int start[kMaxRounds];
+};
+__global__ void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out);
+
+// Single-workgroup tail of an eq-factored proof (see k_outer_tail).
+constexpr int kTailMaxLog = 11;         // at most 2^11 elements per table in LDS
+constexpr int kTailFoldPerThread = 3;   // 3 * 2^10 fold outputs over 1024 threads
+struct TailArgs {
+    const f128* a;
+    const f128* b;
+    const f128* c;
+    const f128* chal;  // chal[k] = r_k
+    const f128* L;
+    const f128* H;
+    int ell, n, k0, fold_first, len_in_log;
+    uint64_t q1mask;
+    uint32_t* acc;  // round k's sums at acc + 12 k
+    f128* fin;      // A(r), B(r), C(r)
+};
+__global__ void k_outer_tail(TailArgs args);
 constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
 }  // namespace bitz

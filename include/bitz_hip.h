@@ -154,8 +154,9 @@ int32_t bitz_outer_begin(bitz_outer** h, const bitz_f128* eq_low, uint32_t low_v
                          uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars,
                          const bitz_f128* cz, uint32_t cz_vars, uint32_t flags);
 /* 1 when the factors were recognised as eq tables of a point tau and the
- * eq-factored kernels run; 0 for the general per-pair eq kernels. */
-int32_t bitz_outer_eq_mode(const bitz_outer* h);
+ * eq-factored kernels run; 0 for the general per-pair eq kernels.  Waits for
+ * the device's check of the factors if round0 has not read it yet. */
+int32_t bitz_outer_eq_mode(bitz_outer* h);
 int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* coeffs3);
 int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 r, bitz_f128* coeffs3);
 int32_t bitz_outer_finish(bitz_outer* h, bitz_f128* claims3);

@@ -291,3 +291,95 @@ def test_outer_with_challenges_matches_oracle(n, split, eq):
     assert_outer_equal(ref, out)
     dev = G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c))
     assert_outer_equal(ref, G.prove_outer_sumcheck_with_challenges(ch, 77, (lo, hi), dev))
+
+
+# ---- pipelined prover: the optimistic pass and its second pass ----------------
+def pipelined(ch, claim, lo, hi, a, b, c):
+    """The pipelined prover and how many times its tail kernel ran (1 = the
+    optimistic pass was accepted, 2 = a second pass was needed)."""
+    G.profile_enable(True)
+    try:
+        out = G.prove_outer_sumcheck_with_challenges(ch, claim, (lo, hi), G.R1csProductMles(a, b, c))
+        passes = G.profile_read("outer_tail")["launches"]
+    finally:
+        G.profile_enable(False)
+    return out, passes
+
+
+def oracle_with(ch, claim, lo, hi, a, b, c):
+    rc, ref = O.outer_sumcheck(O.ListTranscript(ints(ch)), claim, lo, hi, a, b, c)
+    assert rc == 0
+    return ref
+
+
+@pytest.mark.parametrize("n,split", [(1, 0), (1, 1), (2, 1), (6, 0), (6, 6), (10, 3), (11, 5), (13, 0),
+                                     (13, 6), (13, 13), (16, 9), (20, 10)])
+def test_pipelined_true_claim_takes_one_pass(n, split):
+    a, b, c, tau, lo, hi = outer_case(n, split, 300 + 7 * n + split)
+    claim = true_claim(a, b, c, tau)
+    ch = elements(301 + n, n)
+    out, passes = pipelined(ch, claim, lo, hi, a, b, c)
+    assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+    assert passes == 1
+
+
+@pytest.mark.parametrize("n", [3, 12, 15])
+def test_pipelined_wrong_claim_is_caught(n):
+    """Round 0 recovers q(1) from the claim; a wrong claim fails the final
+    check and the second pass computes q(1) from the tables."""
+    a, b, c, tau, lo, hi = outer_case(n, n // 2, 400 + n)
+    claim = true_claim(a, b, c, tau) ^ 0x5A5A
+    ch = elements(401 + n, n)
+    out, passes = pipelined(ch, claim, lo, hi, a, b, c)
+    assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+    assert passes == 2
+
+
+@pytest.mark.parametrize("zero_at", [0, 2, 11, 13])
+def test_pipelined_zero_tau(zero_at):
+    n = 14
+    a, b, c = elements_fast(410, 1 << n), elements_fast(411, 1 << n), elements_fast(412, 1 << n)
+    tau = elements(413, n)
+    tau[zero_at] = 0
+    lo, hi = O.eq_table(tau[:7]), O.eq_table(tau[7:])
+    claim = true_claim(a, b, c, tau)
+    ch = elements(414, n)
+    out, passes = pipelined(ch, claim, lo, hi, a, b, c)
+    assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+    assert passes == 2
+
+
+@pytest.mark.parametrize("zero_r", [0, 5, 13])
+def test_pipelined_zero_challenge_computes_q1_from_the_start(zero_r):
+    n = 14
+    a, b, c, tau, lo, hi = outer_case(n, 7, 420)
+    claim = true_claim(a, b, c, tau) ^ 1  # wrong, and undetectable through r = 0
+    ch = elements(421, n)
+    ch[zero_r] = 0
+    out, passes = pipelined(ch, claim, lo, hi, a, b, c)
+    assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+    assert passes == 1
+
+
+def test_pipelined_non_eq_factors_use_the_general_kernels():
+    n = 12
+    a, b, c = elements_fast(430, 1 << n), elements_fast(431, 1 << n), elements_fast(432, 1 << n)
+    tau = elements(433, n)
+    lo, hi = O.eq_table(tau[:6]), O.eq_table(tau[6:])
+    bad = [lo.copy(), hi.copy()]
+    bad[0][5, 0] ^= 1                                  # one entry off
+    scaled = O.mul_batch(hi, np.tile(from_ints([3]), (len(hi), 1)))  # c * eq: products agree, sum != 1
+    for lo2, hi2 in [(bad[0], hi), (lo, scaled)]:
+        ch = elements(434, n)
+        out, passes = pipelined(ch, 0, lo2, hi2, a, b, c)
+        assert_outer_equal(oracle_with(ch, 0, lo2, hi2, a, b, c), out)
+        assert passes == 1
+        assert not mode_of(lo2, hi2, a, b, c)
+
+
+@pytest.mark.parametrize("split", [0, 3, 9, 13])
+def test_per_round_eq_mode_every_split(split):
+    n = 13
+    a, b, c, tau, lo, hi = outer_case(n, split, 440 + split)
+    assert mode_of(lo, hi, a, b, c)
+    outer_with(a, b, c, lo, hi, 11, ints(elements(441, n)))
