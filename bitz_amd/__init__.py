@@ -85,6 +85,7 @@ def lib():
             "bitz_synchronize": (I32, []),
             "bitz_release_cache": (I32, []),
             "bitz_profile_enable": (I32, [I32]),
+            "bitz_profile_only": (I32, [C.c_char_p]),
             "bitz_profile_read": (I32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_double)]),
             "bitz_transcript_callbacks": (P, [P, P, P]),
@@ -233,6 +234,12 @@ def profile_enable(on: bool = True) -> None:
     """Time the sumcheck kernels with HIP events on their own streams."""
     init()
     _check(lib().bitz_profile_enable(1 if on else 0))
+
+
+def profile_only(kernel: str | None) -> None:
+    """Time only launches named `kernel` (None: all)."""
+    init()
+    _check(lib().bitz_profile_only(kernel.encode() if kernel else None))
 
 
 def profile_read(kernel: str) -> dict:
@@ -389,16 +396,24 @@ class InnerSumcheckOutput:
     witness_evaluation: int
 
 
+def _factor(f, flags):
+    """An eq factor on the side the products are on (DeviceTables pass through)."""
+    if isinstance(f, DeviceTable):
+        if not flags & DEVICE:
+            raise BitzError(-7, "device eq factors need device products")
+        return f
+    f = elems(f)
+    return DeviceTable(f) if flags & DEVICE else f
+
+
 def prove_outer_sumcheck(transcript: Transcript, initial_claim, eq_factors, products: R1csProductMles):
     """prove_outer_sumcheck (crates/spartan/src/sumcheck.rs:252-384) on the GPU."""
     init()
-    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
     (az, na, fa), (bz, nb, fb), (cz, nc, fc) = (_table(t) for t in (products.az, products.bz, products.cz))
     flags = fa
     if not (fa == fb == fc):
         raise BitzError(-7, "products must all be host arrays or all DeviceTables")
-    if flags & DEVICE:
-        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    eq_low, eq_high = (_factor(f, flags) for f in eq_factors)
     n = _nvars(na)
     rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
     points = np.zeros((max(n, 1), 2), dtype=np.uint64)
@@ -417,12 +432,10 @@ def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, 
     """prove_outer_sumcheck against challenges fixed in advance: every round
     is queued back to back, the round sums are read once."""
     init()
-    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
     (az, na, fa), (bz, nb, fb), (cz, nc, fc) = (_table(t) for t in (products.az, products.bz, products.cz))
     if not (fa == fb == fc):
         raise BitzError(-7, "products must all be host arrays or all DeviceTables")
-    if fa & DEVICE:
-        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
     n = _nvars(na)
     ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
     rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
@@ -441,10 +454,8 @@ def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, 
 def outer_claim(eq_factors, products: R1csProductMles) -> int:
     """sum_x eq(x) (Az Bz + Cz)(x) on the device (sumcheck.rs:367-370)."""
     init()
-    eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
     (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
-    if fa & DEVICE:
-        eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+    eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
     out = F128()
     _check(lib().bitz_outer_claim(_p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)), _p(az),
                                   _p(bz), _p(cz), _nvars(na), fa, C.byref(out)), SumcheckError)
@@ -456,10 +467,8 @@ class OuterSumcheck:
 
     def __init__(self, eq_factors, products: R1csProductMles):
         init()
-        eq_low, eq_high = elems(eq_factors[0]), elems(eq_factors[1])
         (az, na, fa), (bz, nb, _), (cz, nc, _) = (_table(t) for t in (products.az, products.bz, products.cz))
-        if fa & DEVICE:
-            eq_low, eq_high = DeviceTable(eq_low), DeviceTable(eq_high)
+        eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
         self._keep = (eq_low, eq_high, az, bz, cz)
         h = C.c_void_p()
         _check(lib().bitz_outer_begin(C.byref(h), _p(eq_low), _nvars(len(eq_low)), _p(eq_high),

@@ -264,6 +264,7 @@ struct KernelStat {
 };
 struct Profiler {
     bool on = false;
+    std::string only;  // when set, only launches of this name are timed
     std::mutex mu;
     std::map<std::string, KernelStat> stats;
 } g_prof;
@@ -306,6 +307,7 @@ struct Timed {
     void begin(hipStream_t s, const char* kernel, double algo_bytes) {
         name = nullptr;
         if (!g_prof.on) return;
+        if (!g_prof.only.empty() && g_prof.only != kernel) return;
         if (!start) start = g_events.get();
         if (!stop) stop = g_events.get();
         if (!start || !stop) return;
@@ -373,6 +375,12 @@ extern "C" int32_t bitz_profile_enable(int32_t on) {
     std::lock_guard<std::mutex> lock(g_prof.mu);
     g_prof.stats.clear();
     g_prof.on = on != 0;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_profile_only(const char* kernel) {
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    g_prof.only = kernel ? kernel : "";
     return BITZ_OK;
 }
 
@@ -552,7 +560,7 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
-    Dev htabs, rtabs, chal;
+    Dev htabs, rtabs, rtabs4, chal;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
     };
@@ -564,6 +572,8 @@ struct bitz_outer {
         size_t htab_off;   // first H(i) table (multi-iteration rounds)
     };
     std::vector<Geo> geo;
+    bitz::HtabDesc hdesc{};
+    size_t htab_total = 0;
     f128 S{1, 0}, sigma{0, 0};
     f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
 
@@ -642,12 +652,12 @@ static f128 f128_at(const uint32_t* w) {
 constexpr int kTailLog = 10;
 static_assert(kTailLog <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
 
-// Plans the eq-factored mode and launches the device check of the factors
-// and the H(i) tables.  Returns false (general kernels) when the factors are
-// too large to check cheaply.  Nothing here reads back: the verdict arrives
-// with the first round sums.
+// Plans the eq-factored mode (round geometry, H(i) tables) and allocates
+// its buffers.  Returns false (general kernels) when the factors are too
+// large to check cheaply.  The check itself is launched by
+// outer_launch_checks or, in the pipelined prover, by its prologue; nothing
+// reads back here: the verdict arrives with the first round sums.
 static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
-    hipStream_t s = h->so.s;
     const int n = h->n;
     if (n == 0) return false;
     const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
@@ -686,18 +696,13 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
         }
     }
     desc.first[desc.rounds] = (uint32_t)htab_total;
+    h->hdesc = desc;
+    h->htab_total = htab_total;
     if (h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
+        h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->chal.alloc((size_t)n * sizeof(f128)))
         return false;
-    const unsigned bl = (unsigned)std::max<size_t>(1, (nl + kBlock - 1) / kBlock);
-    const unsigned bh = (unsigned)std::max<size_t>(1, (nh + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(bitz::k_eq_check, dim3(bl + bh), dim3(kBlock), 0, s, h->eqL[0].f(), low_vars,
-                       h->eqH[0].f(), high_vars, h->acc.u32() + h->eq_off(), bl);
-    if (htab_total)
-        hipLaunchKernelGGL(bitz::k_htabs, dim3((unsigned)htab_total), dim3(1024), 0, s, h->eqL[0].f(), low_vars,
-                           h->eqH[0].f(), desc, static_cast<uint4*>(h->htabs.p));
-    if (hipGetLastError() != hipSuccess) return false;
     static bool attr_set = false;
     if (!attr_set) {
         const void* fns[4] = {reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
@@ -712,6 +717,22 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
         attr_set = true;
     }
     return true;
+}
+
+static unsigned eq_check_blocks(size_t entries, unsigned block) {
+    return (unsigned)std::max<size_t>(1, (entries + block - 1) / block);
+}
+
+// The factor check and the H(i) tables, for the per-round prover.
+static int32_t outer_launch_checks(bitz_outer* h) {
+    hipStream_t s = h->so.s;
+    const unsigned bl = eq_check_blocks(h->low_len, kBlock), bh = eq_check_blocks(h->high_len, kBlock);
+    hipLaunchKernelGGL(bitz::k_eq_check, dim3(bl + bh), dim3(kBlock), 0, s, h->eqL[0].f(), h->ell,
+                       h->eqH[0].f(), h->m, h->acc.u32() + h->eq_off(), bl);
+    if (h->htab_total)
+        hipLaunchKernelGGL(bitz::k_htabs, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
+                           h->eqH[0].f(), h->hdesc, static_cast<uint4*>(h->htabs.p));
+    return check_launch();
 }
 
 // Reads the device check (eq region of the host copy of acc) and, when the
@@ -855,10 +876,12 @@ static void outer_eq_advance(bitz_outer* h, int k, f128 r) {
     h->S = hostf::mul(h->S, e);
 }
 
-extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
-                                    const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az,
-                                    uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars,
-                                    const bitz_f128* cz, uint32_t cz_vars, uint32_t flags) {
+// launch_checks: launch the factor check and H(i) tables now (per-round
+// prover); the pipelined prover folds them into its prologue instead.
+static int32_t outer_begin_impl(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
+                                const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az,
+                                uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars, const bitz_f128* cz,
+                                uint32_t cz_vars, uint32_t flags, bool launch_checks) {
     if (!out || !eq_low || !eq_high || !az || !bz || !cz) return fail(BITZ_ERR_ARG, "null pointer");
     TRY(outer_check(low_vars, high_vars, az_vars, bz_vars, cz_vars));
     auto* h = new bitz_outer;
@@ -903,9 +926,18 @@ extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, u
         return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
     if ((rc = h->host.alloc(h->acc_words() * sizeof(uint32_t)))) return cleanup(rc);
     h->eqmode = outer_setup_eq(h, (int)low_vars, (int)high_vars);
+    if (h->eqmode && launch_checks && outer_launch_checks(h)) h->eqmode = false;
     (void)hipGetLastError();  // a failed probe must not leak into later checks
     *out = h;
     return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_begin(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
+                                    const bitz_f128* eq_high, uint32_t high_vars, const bitz_f128* az,
+                                    uint32_t az_vars, const bitz_f128* bz, uint32_t bz_vars,
+                                    const bitz_f128* cz, uint32_t cz_vars, uint32_t flags) {
+    return outer_begin_impl(out, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars,
+                            flags, true);
 }
 
 // Reads the device check of the factors if the host has not yet.
@@ -1168,50 +1200,124 @@ done:
     return rc;
 }
 
+// Round schedule of the pipelined eq-factored prover: rounds [0, kc) as
+// grid-wide kernels, [kc, k0) in the chunked kernel, [k0, n) in the tail.
+struct PipeSchedule {
+    int kc, k0;
+};
+static PipeSchedule pipe_schedule(int n) {
+    const int k0 = std::max(0, n - kTailLog);
+    const int kc = std::max(1, n - 19);  // rounds before kc have more pairs than one launch's threads
+    if (k0 - kc >= 1) return {kc, k0};
+    return {k0, k0};
+}
+
+// Uploads the challenges (through the handle's pinned block, so the copy
+// is asynchronous) and launches the prologue: factor check, H(i) tables and
+// the challenges' constant tables.
+static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenges) {
+    hipStream_t s = h->so.s;
+    const int n = h->n;
+    const size_t bytes = (size_t)n * sizeof(f128);
+    char* staging = static_cast<char*>(h->host.p) + PinnedPool::kBytes / 2;
+    if (h->acc_words() * 4 > PinnedPool::kBytes / 2) return fail(BITZ_ERR_ARG, "num_vars too large");
+    memcpy(staging, challenges, bytes);
+    HIP_TRY(hipMemcpyAsync(h->chal.p, staging, bytes, hipMemcpyHostToDevice, s));
+    const PipeSchedule ps = pipe_schedule(n);
+    bitz::PrologueArgs pa{};
+    pa.L = h->eqL[0].f();
+    pa.H = h->eqH[0].f();
+    pa.ell = h->ell;
+    pa.m = h->m;
+    pa.eq_blocks_l = eq_check_blocks(h->low_len, 1024);
+    pa.eq_blocks = pa.eq_blocks_l + eq_check_blocks(h->high_len, 1024);
+    pa.eq_out = h->acc.u32() + h->eq_off();
+    pa.htab_blocks = (unsigned)h->htab_total;
+    pa.hdesc = h->hdesc;
+    pa.htabs = static_cast<uint4*>(h->htabs.p);
+    pa.chal = h->chal.f();
+    pa.t4_blocks = (unsigned)n;
+    pa.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
+    pa.t8_blocks = (unsigned)std::max(0, ps.kc - 1);
+    pa.rtabs = static_cast<uint4*>(h->rtabs.p);
+    const unsigned grid = pa.eq_blocks + pa.htab_blocks + pa.t4_blocks + pa.t8_blocks;
+    hipLaunchKernelGGL(bitz::k_prologue, dim3(grid), dim3(1024), 0, s, pa);
+    return check_launch();
+}
+
 // Enqueues a whole eq-factored proof with the challenges already on the
-// device (h->chal): rounds 0 .. k0-1 as grid-wide kernels, the rest in the
-// tail kernel, then one read of every round's sums, the factor check and
-// the final evaluations.  q1mask selects the rounds that compute q(1).
+// device (h->chal), then reads every round's sums, the factor check and the
+// final evaluations at once.  q1mask selects the rounds that compute q(1).
 static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Timed>& timers) {
     hipStream_t s = h->so.s;
     const int n = h->n;
-    const int k0 = std::max(0, n - kTailLog);
+    const PipeSchedule ps = pipe_schedule(n);
     uint4* rtabs = static_cast<uint4*>(h->rtabs.p);
-    if (k0 >= 2)
-        hipLaunchKernelGGL(bitz::k_const_tables, dim3((unsigned)(k0 - 1)), dim3(1024), 0, s, h->chal.f(), rtabs);
-    for (int k = 0; k < k0; ++k) {
+    const uint4* rtabs4 = static_cast<const uint4*>(h->rtabs4.p);
+    auto next_out = [&](f128* (&out)[3]) {
+        Dev& dst = (h->where == 1) ? h->Q : h->P;
+        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+        out[0] = dst.f();
+        out[1] = dst.f() + cap;
+        out[2] = dst.f() + 2 * cap;
+    };
+    auto advance = [&](f128* const (&out)[3], size_t new_len) {
+        for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
+        h->where = (h->where == 1) ? 2 : 1;
+        h->len = new_len;
+    };
+    for (int k = 0; k < ps.kc; ++k) {
         f128* out[3] = {nullptr, nullptr, nullptr};
-        if (k > 0) {
-            Dev& dst = (h->where == 1) ? h->Q : h->P;
-            const size_t cap = (h->where == 1) ? h->capQ : h->capP;
-            out[0] = dst.f();
-            out[1] = dst.f() + cap;
-            out[2] = dst.f() + 2 * cap;
-        }
+        if (k > 0) next_out(out);
         TRY(outer_eq_launch(h, k, k > 0 ? rtabs + (size_t)(k - 1) * kConstTableEntries : nullptr,
                             (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k]));
-        if (k > 0) {
-            for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
-            h->where = (h->where == 1) ? 2 : 1;
-            h->len /= 2;
-        }
+        if (k > 0) advance(out, h->len / 2);
+    }
+    if (ps.kc < ps.k0) {
+        const int cs = bitz::kChunkMaxLog, steps = ps.k0 - ps.kc;
+        f128* out[3];
+        next_out(out);
+        bitz::ChunkArgs ca{};
+        ca.a = h->cur[0];
+        ca.b = h->cur[1];
+        ca.c = h->cur[2];
+        ca.ao = out[0];
+        ca.bo = out[1];
+        ca.co = out[2];
+        ca.rtabs4 = rtabs4;
+        ca.L = h->eqL[0].f();
+        ca.H = h->eqH[0].f();
+        ca.ell = h->ell;
+        ca.n = n;
+        ca.kc = ps.kc;
+        ca.cs = cs;
+        ca.s = steps;
+        ca.q1mask = q1mask;
+        ca.acc = h->acc.u32();
+        const unsigned chunks = (unsigned)(h->len >> cs);
+        const size_t smem = 3 * sizeof(uint4) << (cs - 1);
+        timers[ps.kc].begin(s, "outer_chunk", 0.0);
+        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(256), smem, s, ca);
+        timers[ps.kc].end(s);
+        TRY(check_launch());
+        advance(out, h->len >> steps);
     }
     bitz::TailArgs ta{};
     ta.a = h->cur[0];
     ta.b = h->cur[1];
     ta.c = h->cur[2];
-    ta.chal = h->chal.f();
+    ta.rtabs4 = rtabs4;
     ta.L = h->eqL[0].f();
     ta.H = h->eqH[0].f();
     ta.ell = h->ell;
     ta.n = n;
-    ta.k0 = k0;
-    ta.fold_first = k0 > 0;
-    ta.len_in_log = k0 > 0 ? n - k0 + 1 : n;
+    ta.k0 = ps.k0;
+    ta.fold_first = ps.k0 > 0;
+    ta.len_in_log = ps.k0 > 0 ? n - ps.k0 + 1 : n;
     ta.q1mask = q1mask;
     ta.acc = h->acc.u32();
     ta.fin = reinterpret_cast<f128*>(h->acc.u32() + h->fin_off());
-    const size_t smem = 3 * sizeof(uint4) << (n - k0);
+    const size_t smem = 3 * sizeof(uint4) << (n - ps.k0);
     timers[n].begin(s, "outer_tail", 0.0);
     hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(1024), smem, s, ta);
     timers[n].end(s);
@@ -1231,7 +1337,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
 // are (a recovery would divide by tau_k = 0, or the check fails).
 static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 initial_claim,
                               uint64_t q1mask, bool check_claim, std::vector<bitz_f128>& co) {
-    const int n = h->n, k0 = std::max(0, n - kTailLog);
+    const int n = h->n, kc = pipe_schedule(n).kc;
     const uint32_t* w = h->host_words();
     for (int k = 0; k < n; ++k)
         if (!((q1mask >> k) & 1) && hostf::is_zero(h->tau[k])) return false;
@@ -1241,7 +1347,7 @@ static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 i
     for (int k = 0; k < n; ++k) {
         bitz_f128 raw[3];
         memcpy(raw, w + 12 * k, 48);
-        const f128& sc = k >= k0 ? h->scale_tail[k] : h->scale_big[k];
+        const f128& sc = k >= kc ? h->scale_tail[k] : h->scale_big[k];
         outer_eq_coeffs(h, k, raw, (q1mask >> k) & 1, sc, &co[3 * (size_t)k]);
         outer_eq_advance(h, k, F(challenges[k]));
     }
@@ -1339,7 +1445,8 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
         return fail(BITZ_ERR_ARG, "null pointer");
     TRY(outer_check(low_vars, high_vars, az_vars, bz_vars, cz_vars));
     bitz_outer* h = nullptr;
-    TRY(bitz_outer_begin(&h, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars, flags));
+    TRY(outer_begin_impl(&h, eq_low, low_vars, eq_high, high_vars, az, az_vars, bz, bz_vars, cz, cz_vars, flags,
+                         false));
     const int n = (int)az_vars;
     int32_t rc = BITZ_OK;
     std::vector<bitz_f128> co;
@@ -1353,11 +1460,7 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
     h->started = true;
     if (h->eqmode) {
         std::vector<Timed> timers(n + 1);
-        if (hipMemcpyAsync(h->chal.p, challenges, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, h->so.s) !=
-            hipSuccess) {
-            rc = fail(BITZ_ERR_DEVICE, "challenge upload failed");
-            goto done;
-        }
+        if ((rc = outer_pipeline_prologue(h, challenges))) goto done;
         bool zero_r = false;
         for (int k = 0; k < n; ++k) zero_r |= hostf::is_zero(F(challenges[k]));
         uint64_t q1mask = zero_r ? 1 : 0;

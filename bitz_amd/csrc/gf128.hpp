@@ -282,4 +282,47 @@ __device__ __forceinline__ g4 g_mul_table(const uint4* tab, const g4& x) {
     }
     return r;
 }
+
+// 4-bit windows: 32 x 16 entries (8 KiB), entry (w, v) = c * v * X^(4w).
+// Half the lookups' reach of the 8-bit table, an eighth of its size: cheap
+// to copy into LDS once per round.
+constexpr int kConstTable4Entries = 32 * 16;
+
+// Builds c's 4-bit table in `tab` (uint4[512]), `basis` (uint4[128]) as
+// scratch.  All threads of the block must call it; it ends with a barrier.
+__device__ __forceinline__ void build_const_table4(uint4* tab, uint4* basis, const g4& c) {
+    for (int j = threadIdx.x; j < 128; j += blockDim.x) {
+        const g4 b = g_mul_xpow(c, j);
+        basis[j] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) {
+        const int w = e >> 4, v = e & 15;
+        uint4 x = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int bit = 0; bit < 4; ++bit)
+            if ((v >> bit) & 1) {
+                const uint4 y = basis[4 * w + bit];
+                x.x ^= y.x; x.y ^= y.y; x.z ^= y.z; x.w ^= y.w;
+            }
+        tab[e] = x;
+    }
+    __syncthreads();
+}
+
+// c * x through c's 4-bit table.
+__device__ __forceinline__ g4 g_mul_table4(const uint4* tab, const g4& x) {
+    g4 r = g_zero();
+#pragma unroll
+    for (int k = 0; k < 32; k += 2) {
+        const uint32_t n0 = __builtin_amdgcn_ubfe(x.w[k >> 3], 4 * (k & 7), 4);
+        const uint32_t n1 = __builtin_amdgcn_ubfe(x.w[(k + 1) >> 3], 4 * ((k + 1) & 7), 4);
+        const uint4 e0 = tab[16 * k + n0], e1 = tab[16 * (k + 1) + n1];
+        r.w[0] = xor3(r.w[0], e0.x, e1.x);
+        r.w[1] = xor3(r.w[1], e0.y, e1.y);
+        r.w[2] = xor3(r.w[2], e0.z, e1.z);
+        r.w[3] = xor3(r.w[3], e0.w, e1.w);
+    }
+    return r;
+}
 #endif  // __HIPCC__

@@ -367,6 +367,14 @@ __global__ __launch_bounds__(1024) void k_const_table1(f128 c, uint4* out) {
     for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) out[e] = tab[e];
 }
 
+__global__ __launch_bounds__(512) void k_const_tables4(const f128* consts, uint4* out) {
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    build_const_table4(tab, basis, g_load(consts + blockIdx.x));
+    uint4* dst = out + (size_t)blockIdx.x * kConstTable4Entries;
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
+}
+
 __device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__ src) {
     for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) lds[e] = src[e];
 }
@@ -391,12 +399,12 @@ __device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__
 // entries (eq tables sum to 1; the host checks the sums and the probes).
 // out: [0] mismatch flag, [4..7] sum of L, [8..11] sum of H, from [12] the
 // probes T[0], T[1], T[2], T[4], ... of L, then of H.
-__global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, const f128* H, int m,
-                                                     uint32_t* out, unsigned blocks_l) {
-    const bool in_h = blockIdx.x >= blocks_l;
+__device__ __forceinline__ void eq_check_block(const f128* L, int ell, const f128* H, int m, uint32_t* out,
+                                               unsigned blocks_l, unsigned bidx) {
+    const bool in_h = bidx >= blocks_l;
     const f128* T = in_h ? H : L;
     const int bits = in_h ? m : ell;
-    const size_t x = (size_t)(in_h ? blockIdx.x - blocks_l : blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t x = (size_t)(in_h ? bidx - blocks_l : bidx) * blockDim.x + threadIdx.x;
     const size_t size = (size_t)1 << bits;
     f128* probes = reinterpret_cast<f128*>(out + 12) + (in_h ? ell + 1 : 0);
     g4 v[1] = {g_zero()};
@@ -418,16 +426,20 @@ __global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, con
     block_xor_to_global<1>(v, out + (in_h ? 8 : 4));
 }
 
+__global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, const f128* H, int m,
+                                                     uint32_t* out, unsigned blocks_l) {
+    eq_check_block(L, ell, H, m, out, blocks_l, blockIdx.x);
+}
+
 // H(i) constant tables for the multi-iteration rounds: table b belongs to
 // the round j with first[j] <= b < first[j + 1]; its constant is the weight
 // of iteration i = b - first[j] over the variables [start[j], n), read from
 // the factors (a product when the run crosses from L into H).
-__global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out) {
-    __shared__ uint4 tab[kConstTableEntries];
-    __shared__ uint4 basis[128];
+__device__ __forceinline__ void htab_block(const f128* L, int ell, const f128* H, const HtabDesc& d, uint4* out,
+                                           unsigned bidx, uint4* tab, uint4* basis) {
     int j = 0;
-    while (j + 1 < d.rounds && blockIdx.x >= d.first[j + 1]) ++j;
-    const uint32_t i = blockIdx.x - d.first[j];
+    while (j + 1 < d.rounds && bidx >= d.first[j + 1]) ++j;
+    const uint32_t i = bidx - d.first[j];
     const int s = d.start[j];
     g4 c;
     if (s < ell) {
@@ -438,7 +450,43 @@ __global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f1
         c = g_load(H + ((size_t)i << (s - ell)));
     }
     build_const_table(tab, basis, c);
-    uint4* dst = out + (size_t)blockIdx.x * kConstTableEntries;
+    uint4* dst = out + (size_t)bidx * kConstTableEntries;
+    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
+}
+
+__global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out) {
+    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 basis[128];
+    htab_block(L, ell, H, d, out, blockIdx.x, tab, basis);
+}
+
+// Everything the pipelined prover needs before round 0, in one launch of
+// 1024-thread workgroups: the factor check, the H(i) tables, the 4-bit
+// tables of every challenge and the 8-bit tables of the challenges the
+// grid-wide rounds fold with.
+__global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
+    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 basis[128];
+    unsigned b = blockIdx.x;
+    if (b < A.eq_blocks) {
+        eq_check_block(A.L, A.ell, A.H, A.m, A.eq_out, A.eq_blocks_l, b);
+        return;
+    }
+    b -= A.eq_blocks;
+    if (b < A.htab_blocks) {
+        htab_block(A.L, A.ell, A.H, A.hdesc, A.htabs, b, tab, basis);
+        return;
+    }
+    b -= A.htab_blocks;
+    if (b < A.t4_blocks) {
+        build_const_table4(tab, basis, g_load(A.chal + b));
+        uint4* dst = A.rtabs4 + (size_t)b * kConstTable4Entries;
+        for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
+        return;
+    }
+    b -= A.t4_blocks;
+    build_const_table(tab, basis, g_load(A.chal + b));
+    uint4* dst = A.rtabs + (size_t)b * kConstTableEntries;
     for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
 }
 
@@ -570,100 +618,180 @@ template __global__ void k_outer_eq<true, false>(OUTER_EQ_SIG);
 template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
 #undef OUTER_EQ_SIG
 
-// The last rounds of an eq-factored proof with challenges known in advance,
-// in one workgroup: the three tables (at most kTailMax elements each after
-// the first fold) stay in LDS, and every round is a fold by r_{k-1} followed
-// by the round sums, separated by barriers instead of kernel launches.
-// Weights are read from the factors as in k_outer_eq; the round sums are
-// stored at acc + 12 k and the final A(r), B(r), C(r) at fin[0..2].
-__global__ __launch_bounds__(1024) void k_outer_tail(TailArgs A) {
-    extern __shared__ uint4 lds_raw[];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    int lg = A.len_in_log;
-    auto src = [&](int t) { return t == 0 ? A.a : (t == 1 ? A.b : A.c); };
-    if (A.fold_first) {
-        const g4 r = g_load(A.chal + (A.k0 - 1));
-        lg -= 1;
-        const size_t len = (size_t)1 << lg;
-        for (size_t e = tid; e < 3 * len; e += nt) {
-            const int t = (int)(e >> lg);
-            const size_t j = e & (len - 1);
-            const g4 lo = g_load(src(t) + 2 * j), hi = g_load(src(t) + 2 * j + 1);
-            const g4 v = g_add(lo, g_mul(r, g_add(lo, hi)));
-            lds_raw[e] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
-        }
-    } else {
-        const size_t len = (size_t)1 << lg;
-        for (size_t e = tid; e < 3 * len; e += nt) {
-            const int t = (int)(e >> lg);
-            lds_raw[e] = *reinterpret_cast<const uint4*>(src(t) + (e & (len - 1)));
-        }
-    }
-    __syncthreads();
-    const int cap_lg = lg;  // each table occupies 2^cap_lg entries of LDS
+// Eq weight of pair y in round k, read from the factors (the variables
+// above k: eq_low's part times eq_high's part).
+__device__ __forceinline__ g4 tail_weight(const f128* L, int ell, const f128* H, int k, size_t y) {
+    const int lo_bits = ell > k + 1 ? ell - (k + 1) : 0;
+    const int hshift = k + 1 > ell ? k + 1 - ell : 0;
+    const size_t ylo = y & (((size_t)1 << lo_bits) - 1), yhi = y >> lo_bits;
+    g4 w = g_load(H + (yhi << hshift));
+    if (lo_bits) w = g_mul(w, g_load(L + (ylo << (k + 1))));
+    return w;
+}
+
+// Round k's sums over the pairs held in LDS (three tables at stride 2^cap_lg,
+// 2^pl pairs; global pair index = base + local), XORed into acc_k.
+__device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int pl, size_t base, int k,
+                                               bool q1, const f128* L, int ell, const f128* H,
+                                               uint32_t* acc_k, bool single_block) {
+    const size_t P = (size_t)1 << pl;
+    const size_t items = (q1 ? 3 : 2) * P;
     auto ld = [&](int t, size_t j) {
-        const uint4 u = lds_raw[((size_t)t << cap_lg) + j];
+        const uint4 u = lds[((size_t)t << cap_lg) + j];
         g4 r;
         r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
         return r;
     };
+    g4 s[3] = {g_zero(), g_zero(), g_zero()};
+    for (size_t it = threadIdx.x; it < items; it += blockDim.x) {
+        const size_t y = it & (P - 1);
+        const int which = (int)(it >> pl);
+        const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
+        const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
+        g4 p;
+        if (which == 0)
+            p = g_add(g_mul(a0, b0), ld(2, 2 * y));
+        else if (which == 1)
+            p = g_mul(g_add(a0, a1), g_add(b0, b1));
+        else
+            p = g_add(g_mul(a1, b1), ld(2, 2 * y + 1));
+        const g4 wp = g_mul(tail_weight(L, ell, H, k, base + y), p);
+        if (which == 0) s[0] = g_add(s[0], wp);
+        else if (which == 1) s[1] = g_add(s[1], wp);
+        else s[2] = g_add(s[2], wp);
+    }
+    if (single_block)
+        block_xor_store<3>(s, acc_k);
+    else
+        block_xor_to_global<3>(s, acc_k);
+}
+
+// In-place fold of the three LDS tables (2^(pl+1) entries each at stride
+// 2^cap_lg) through the 4-bit table of r in rt: read everything, barrier,
+// write.  At most kFoldPerThread outputs per thread.
+template <int kFoldPerThread>
+__device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const uint4* rt) {
+    const size_t half = (size_t)1 << pl;
+    g4 nv[kFoldPerThread];
+#pragma unroll
+    for (int q = 0; q < kFoldPerThread; ++q) {
+        const size_t e = threadIdx.x + (size_t)q * blockDim.x;
+        if (e < 3 * half) {
+            const int t = (int)(e >> pl);
+            const size_t j = e & (half - 1);
+            const uint4 u0 = lds[((size_t)t << cap_lg) + 2 * j], u1 = lds[((size_t)t << cap_lg) + 2 * j + 1];
+            g4 lo, d;
+            lo.w[0] = u0.x; lo.w[1] = u0.y; lo.w[2] = u0.z; lo.w[3] = u0.w;
+            d.w[0] = u0.x ^ u1.x; d.w[1] = u0.y ^ u1.y; d.w[2] = u0.z ^ u1.z; d.w[3] = u0.w ^ u1.w;
+            nv[q] = g_add(lo, g_mul_table4(rt, d));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kFoldPerThread; ++q) {
+        const size_t e = threadIdx.x + (size_t)q * blockDim.x;
+        if (e < 3 * half) {
+            const int t = (int)(e >> pl);
+            const size_t j = e & (half - 1);
+            lds[((size_t)t << cap_lg) + j] = make_uint4(nv[q].w[0], nv[q].w[1], nv[q].w[2], nv[q].w[3]);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void copy_table4(uint4* lds, const uint4* __restrict__ src) {
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) lds[e] = src[e];
+}
+
+// Loads 2^lg folded entries per table (chunk of 2^(lg+1) inputs starting at
+// element `off`) into LDS: out[j] = in[2j] + r (in[2j] + in[2j+1]).
+__device__ __forceinline__ void lds_load_folded(uint4* lds, int lg, const f128* a, const f128* b,
+                                                const f128* c, size_t off, const uint4* rt) {
+    const size_t len = (size_t)1 << lg;
+    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+        const int t = (int)(e >> lg);
+        const size_t j = e & (len - 1);
+        const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off;
+        const g4 lo = g_load(src + 2 * j), hi = g_load(src + 2 * j + 1);
+        const g4 v = g_add(lo, g_mul_table4(rt, g_add(lo, hi)));
+        lds[e] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+    }
+}
+
+// Middle rounds kc .. kc+s-1 of an eq-factored proof with challenges known in
+// advance.  Adjacent-pair folding keeps chunks independent: workgroup g
+// loads inputs [g 2^cs, (g+1) 2^cs) of each table (the tables before the
+// fold by r_{kc-1}), folds them in LDS round after round and adds its pairs'
+// share of every round's sums into the round's slot; it writes its 2^(cs-s)
+// entries of the tables of round kc+s-1 (before the fold by r_{kc+s-1}).
+__global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
+    __shared__ uint4 rt[kConstTable4Entries];
+    extern __shared__ uint4 lds_raw[];
+    const size_t g = blockIdx.x;
+    int lg = A.cs - 1;
+    const int cap_lg = lg;
+    copy_table4(rt, A.rtabs4 + (size_t)(A.kc - 1) * kConstTable4Entries);
+    __syncthreads();
+    lds_load_folded(lds_raw, lg, A.a, A.b, A.c, g << A.cs, rt);
+    __syncthreads();
+    for (int l = 0; l < A.s; ++l) {
+        const int k = A.kc + l;
+        const int pl = lg - 1;
+        lds_round_sums(lds_raw, cap_lg, pl, g << pl, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k,
+                       false);
+        if (l + 1 < A.s) {
+            __syncthreads();  // every reader of rt is done
+            copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
+            __syncthreads();
+            lds_fold<kChunkFoldPerThread>(lds_raw, cap_lg, pl, rt);
+            lg = pl;
+        }
+    }
+    __syncthreads();
+    const size_t len = (size_t)1 << lg;
+    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+        const int t = (int)(e >> lg);
+        const size_t j = e & (len - 1);
+        f128* dst = (t == 0 ? A.ao : (t == 1 ? A.bo : A.co)) + (g << lg) + j;
+        *reinterpret_cast<uint4*>(dst) = lds_raw[((size_t)t << cap_lg) + j];
+    }
+}
+
+// The last rounds of an eq-factored proof with challenges known in advance,
+// in one workgroup: the three tables (at most 2^kTailMaxLog elements each
+// after the first fold) stay in LDS, and every round is a fold by r_{k-1}
+// followed by the round sums, separated by barriers instead of kernel
+// launches.  The round sums are stored at acc + 12 k and the final A(r),
+// B(r), C(r) at fin[0..2].
+__global__ __launch_bounds__(1024) void k_outer_tail(TailArgs A) {
+    __shared__ uint4 rt[kConstTable4Entries];
+    extern __shared__ uint4 lds_raw[];
+    int lg = A.len_in_log;
+    if (A.fold_first) {
+        copy_table4(rt, A.rtabs4 + (size_t)(A.k0 - 1) * kConstTable4Entries);
+        __syncthreads();
+        lg -= 1;
+        lds_load_folded(lds_raw, lg, A.a, A.b, A.c, 0, rt);
+    } else {
+        const size_t len = (size_t)1 << lg;
+        for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+            const int t = (int)(e >> lg);
+            const f128* src = t == 0 ? A.a : (t == 1 ? A.b : A.c);
+            lds_raw[e] = *reinterpret_cast<const uint4*>(src + (e & (len - 1)));
+        }
+    }
+    __syncthreads();
+    const int cap_lg = lg;
     for (int k = A.k0; k < A.n; ++k) {
-        const int pl = lg - 1;  // log2 pairs
-        const size_t P = (size_t)1 << pl;
-        const bool q1 = (A.q1mask >> k) & 1;
-        const size_t items = (q1 ? 3 : 2) * P;
-        const int lo_bits = A.ell > k + 1 ? A.ell - (k + 1) : 0;
-        const int hshift = k + 1 > A.ell ? k + 1 - A.ell : 0;
-        g4 s[3] = {g_zero(), g_zero(), g_zero()};
-        for (size_t it = tid; it < items; it += nt) {
-            const size_t y = it & (P - 1);
-            const int which = (int)(it >> pl);
-            const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
-            const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
-            g4 p;
-            if (which == 0)
-                p = g_add(g_mul(a0, b0), ld(2, 2 * y));
-            else if (which == 1)
-                p = g_mul(g_add(a0, a1), g_add(b0, b1));
-            else
-                p = g_add(g_mul(a1, b1), ld(2, 2 * y + 1));
-            const size_t ylo = y & (((size_t)1 << lo_bits) - 1), yhi = y >> lo_bits;
-            g4 w = g_load(A.H + (yhi << hshift));
-            if (lo_bits) w = g_mul(w, g_load(A.L + (ylo << (k + 1))));
-            const g4 wp = g_mul(w, p);
-            if (which == 0) s[0] = g_add(s[0], wp);
-            else if (which == 1) s[1] = g_add(s[1], wp);
-            else s[2] = g_add(s[2], wp);
-        }
-        block_xor_store<3>(s, A.acc + 12 * k);
-        // Fold by r_k: read everything first, then write in place.
-        const g4 r = g_load(A.chal + k);
-        const size_t half = P;
-        g4 nv[kTailFoldPerThread];
-#pragma unroll
-        for (int q = 0; q < kTailFoldPerThread; ++q) {
-            const size_t e = tid + (size_t)q * nt;
-            if (e < 3 * half) {
-                const int t = (int)(e >> pl);
-                const size_t j = e & (half - 1);
-                const g4 lo = ld(t, 2 * j), hi = ld(t, 2 * j + 1);
-                nv[q] = g_add(lo, g_mul(r, g_add(lo, hi)));
-            }
-        }
+        const int pl = lg - 1;
+        lds_round_sums(lds_raw, cap_lg, pl, 0, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k, true);
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < kTailFoldPerThread; ++q) {
-            const size_t e = tid + (size_t)q * nt;
-            if (e < 3 * half) {
-                const int t = (int)(e >> pl);
-                const size_t j = e & (half - 1);
-                lds_raw[((size_t)t << cap_lg) + j] = make_uint4(nv[q].w[0], nv[q].w[1], nv[q].w[2], nv[q].w[3]);
-            }
-        }
+        copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
         __syncthreads();
+        lds_fold<kTailFoldPerThread>(lds_raw, cap_lg, pl, rt);
         lg = pl;
     }
-    if (tid < 3) g_store(A.fin + tid, ld(tid, 0));
+    if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
 }
 
 }  // namespace bitz

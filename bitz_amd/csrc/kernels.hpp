@@ -61,6 +61,25 @@ struct HtabDesc {
 };
 __global__ void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out);
 
+__global__ void k_const_tables4(const f128* consts, uint4* out);
+
+struct PrologueArgs {
+    const f128* L;
+    const f128* H;
+    int ell, m;
+    unsigned eq_blocks, eq_blocks_l;  // factor check: 1024 entries per workgroup
+    uint32_t* eq_out;
+    unsigned htab_blocks;
+    HtabDesc hdesc;
+    uint4* htabs;
+    const f128* chal;
+    unsigned t4_blocks;  // 4-bit tables of chal[0 ..)
+    uint4* rtabs4;
+    unsigned t8_blocks;  // 8-bit tables of chal[0 ..)
+    uint4* rtabs;
+};
+__global__ void k_prologue(PrologueArgs args);
+
 // Single-workgroup tail of an eq-factored proof (see k_outer_tail).
 constexpr int kTailMaxLog = 11;         // at most 2^11 elements per table in LDS
 constexpr int kTailFoldPerThread = 3;   // 3 * 2^10 fold outputs over 1024 threads
@@ -68,7 +87,7 @@ struct TailArgs {
     const f128* a;
     const f128* b;
     const f128* c;
-    const f128* chal;  // chal[k] = r_k
+    const uint4* rtabs4;  // 4-bit table of r_k at rtabs4 + 512 k
     const f128* L;
     const f128* H;
     int ell, n, k0, fold_first, len_in_log;
@@ -77,6 +96,26 @@ struct TailArgs {
     f128* fin;      // A(r), B(r), C(r)
 };
 __global__ void k_outer_tail(TailArgs args);
+
+// Chunked middle rounds (see k_outer_chunk): 256-thread workgroups, chunks
+// of 2^cs inputs with cs <= kChunkMaxLog.
+constexpr int kChunkMaxLog = 10;
+constexpr int kChunkFoldPerThread = 3;  // 3 * 2^(kChunkMaxLog - 2) fold outputs over 256 threads
+struct ChunkArgs {
+    const f128* a;
+    const f128* b;
+    const f128* c;
+    f128* ao;
+    f128* bo;
+    f128* co;
+    const uint4* rtabs4;
+    const f128* L;
+    const f128* H;
+    int ell, n, kc, cs, s;
+    uint64_t q1mask;
+    uint32_t* acc;
+};
+__global__ void k_outer_chunk(ChunkArgs args);
 constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
 }  // namespace bitz

@@ -71,6 +71,9 @@ int32_t bitz_release_cache(void);
  * bytes each launch must read and write).  No reference counterpart: the
  * reference times stages with tracing spans (tooling/cli/src/span_stats.rs). */
 int32_t bitz_profile_enable(int32_t on);
+/* Restricts timing to launches of one name (NULL or "" = all): every timed
+ * launch adds two event records to its stream. */
+int32_t bitz_profile_only(const char* kernel);
 int32_t bitz_profile_read(const char* kernel, double* total_ms, uint64_t* launches, double* algo_bytes);
 
 /* ---- transcripts ----------------------------------------------------------- */
