@@ -560,7 +560,7 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
-    Dev htabs, rtabs, rtabs4, chal;
+    Dev htabs, htabs4, rtabs, rtabs4, chal;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
     };
@@ -574,6 +574,7 @@ struct bitz_outer {
     std::vector<Geo> geo;
     bitz::HtabDesc hdesc{};
     size_t htab_total = 0;
+    size_t h4_count = 0;  // round 0's 4-bit H(i) tables
     f128 S{1, 0}, sigma{0, 0};
     f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
 
@@ -669,8 +670,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     h->geo.resize(n);
     size_t htab_total = 0;
     for (int k = 0; k < n; ++k) {
-        const int pbits = n - k - 1;            // log2 pairs of round k
-        const int tbits = std::min(pbits, 18);  // log2 threads
+        const int pbits = n - k - 1;  // log2 pairs of round k
+        // log2 threads: round 0's lane-table kernel keeps two 256-thread
+        // workgroups on every CU; the fold rounds run 2^18 threads.
+        const int tbits = std::min(pbits, k == 0 ? bitz::kR0ThreadBits : 18);
         auto& g = h->geo[k];
         g.threads = (size_t)1 << tbits;
         g.iters = (size_t)1 << (pbits - tbits);
@@ -687,7 +690,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
         g.la_bits = g.la.b - g.la.a;
         g.h_start = b;
         g.htab_off = 0;
-        if (g.iters > 1) {
+        if (g.iters > 1 && k > 0) {
             desc.first[desc.rounds] = (uint32_t)htab_total;
             desc.start[desc.rounds] = b;
             desc.rounds += 1;
@@ -698,19 +701,23 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     desc.first[desc.rounds] = (uint32_t)htab_total;
     h->hdesc = desc;
     h->htab_total = htab_total;
+    h->h4_count = h->geo[0].iters > 1 ? h->geo[0].iters : 0;
     if (h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
+        h->htabs4.alloc(std::max<size_t>(h->h4_count, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
         h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->chal.alloc((size_t)n * sizeof(f128)))
         return false;
     static bool attr_set = false;
     if (!attr_set) {
-        const void* fns[4] = {reinterpret_cast<const void*>(&bitz::k_outer_eq<false, false>),
-                              reinterpret_cast<const void*>(&bitz::k_outer_eq<false, true>),
-                              reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
+        const void* fns[2] = {reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
                               reinterpret_cast<const void*>(&bitz::k_outer_eq<true, true>)};
         for (const void* f : fns)
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_r0<false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_r0<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(3 * sizeof(uint4) << bitz::kTailMaxLog));
@@ -732,6 +739,9 @@ static int32_t outer_launch_checks(bitz_outer* h) {
     if (h->htab_total)
         hipLaunchKernelGGL(bitz::k_htabs, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
                            h->eqH[0].f(), h->hdesc, static_cast<uint4*>(h->htabs.p));
+    if (h->h4_count)
+        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->h4_count), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
+                           h->eqH[0].f(), h->geo[0].h_start, static_cast<uint4*>(h->htabs4.p));
     return check_launch();
 }
 
@@ -808,8 +818,9 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     return true;
 }
 
-// Launches the eq-factored round-k kernel (folding the previous tables
-// through the LDS table rtab when k > 0) into slot k.
+// Launches the eq-factored round-k kernel into slot k: round 0 through lane
+// tables (k_outer_r0), later rounds folding the previous tables through the
+// LDS table rtab of r_{k-1} (k_outer_eq).
 static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab, bool q1, f128* const out[3],
                                Timed* tm = nullptr) {
     if (!tm) tm = &h->tm;
@@ -825,19 +836,30 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab, bool q1,
     hipStream_t s = h->so.s;
     const size_t pairs = g.threads * g.iters;
     tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 96.0 * pairs : 288.0 * pairs);
-    const size_t smem = (fold ? 2 : 1) * kConstTableEntries * sizeof(uint4);
+    const size_t smem = 2 * kConstTableEntries * sizeof(uint4);
     const uint4* htabs = static_cast<const uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries;
     const f128* la = h->table_ptr(g.la.table);
     const f128* lb = h->table_ptr(g.lb.table);
     const int la_shift = g.la.a - h->table_base(g.la.table), lb_shift = g.lb.a - h->table_base(g.lb.table);
+    if (!fold) {
+        const unsigned g0 = (unsigned)((g.threads + bitz::kR0Block - 1) / bitz::kR0Block);
+        const uint4* h4 = static_cast<const uint4*>(h->htabs4.p);
+#define OUTER_R0_ARGS \
+    h->cur[0], h->cur[1], h->cur[2], g.threads, g.iters, la, g.la_bits, la_shift, lb, lb_shift, h4, slot
+        if (q1)
+            hipLaunchKernelGGL((bitz::k_outer_r0<true>), dim3(g0), dim3(bitz::kR0Block), bitz::kR0Smem, s,
+                               OUTER_R0_ARGS);
+        else
+            hipLaunchKernelGGL((bitz::k_outer_r0<false>), dim3(g0), dim3(bitz::kR0Block), bitz::kR0Smem, s,
+                               OUTER_R0_ARGS);
+#undef OUTER_R0_ARGS
+        tm->end(s);
+        return check_launch();
+    }
 #define OUTER_EQ_ARGS                                                                                     \
     h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
         g.threads, g.iters, rtab, la, g.la_bits, la_shift, lb, lb_shift, htabs, slot
-    if (!fold && q1)
-        hipLaunchKernelGGL((bitz::k_outer_eq<false, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
-    else if (!fold)
-        hipLaunchKernelGGL((bitz::k_outer_eq<false, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
-    else if (q1)
+    if (q1)
         hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
     else
         hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
@@ -1235,12 +1257,15 @@ static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenge
     pa.htab_blocks = (unsigned)h->htab_total;
     pa.hdesc = h->hdesc;
     pa.htabs = static_cast<uint4*>(h->htabs.p);
+    pa.h4_blocks = (unsigned)h->h4_count;
+    pa.h4_start = h->geo[0].h_start;
+    pa.htabs4 = static_cast<uint4*>(h->htabs4.p);
     pa.chal = h->chal.f();
     pa.t4_blocks = (unsigned)n;
     pa.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
     pa.t8_blocks = (unsigned)std::max(0, ps.kc - 1);
     pa.rtabs = static_cast<uint4*>(h->rtabs.p);
-    const unsigned grid = pa.eq_blocks + pa.htab_blocks + pa.t4_blocks + pa.t8_blocks;
+    const unsigned grid = pa.eq_blocks + pa.htab_blocks + pa.h4_blocks + pa.t4_blocks + pa.t8_blocks;
     hipLaunchKernelGGL(bitz::k_prologue, dim3(grid), dim3(1024), 0, s, pa);
     return check_launch();
 }

@@ -325,4 +325,79 @@ __device__ __forceinline__ g4 g_mul_table4(const uint4* tab, const g4& x) {
     }
     return r;
 }
+
+// ------------------------------------- general product through LDS ---
+// a * b with a per-lane 4-bit window table of a in LDS: the table holds
+// T[v] = a' v (v < 16) for a' = a without its top three bits (so every entry
+// fits 128 bits), and b is consumed one nibble per lookup, most significant
+// nibble position first, shifting the 256-bit accumulator by 4 between
+// positions.  a's top bits (125..127) add (top * b) X^125.  `col` is this
+// lane's column: entry v at col[64 v] (entry-major, one wave of 64 lanes per
+// row), so a wave's lookups of any entries are one conflict-free
+// ds_read_b128 each.  Entry 0 must hold zero (written once by the caller).
+constexpr int kLaneTableEntries = 16;  // per lane; a wave's table is 16 KiB
+
+__device__ __forceinline__ void lane_table_clear(uint4* col) { col[0] = make_uint4(0, 0, 0, 0); }
+
+__device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
+    g4 al = a;
+    al.w[3] &= 0x1FFFFFFFu;
+    g4 s1, s2, s3;
+    s1.w[0] = al.w[0] << 1;
+    s2.w[0] = al.w[0] << 2;
+    s3.w[0] = al.w[0] << 3;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        s1.w[i] = shl_pair(al.w[i], al.w[i - 1], 1);
+        s2.w[i] = shl_pair(al.w[i], al.w[i - 1], 2);
+        s3.w[i] = shl_pair(al.w[i], al.w[i - 1], 3);
+    }
+    auto put = [&](int v, const g4& x) { col[64 * v] = make_uint4(x.w[0], x.w[1], x.w[2], x.w[3]); };
+    put(1, al);
+    put(2, s1);
+    put(4, s2);
+    put(8, s3);
+    const g4 t3 = g_add(al, s1), t5 = g_add(al, s2), t6 = g_add(s1, s2), t7 = g_add(t3, s2);
+    put(3, t3);
+    put(5, t5);
+    put(6, t6);
+    put(7, t7);
+    put(9, g_add(al, s3));
+    put(10, g_add(s1, s3));
+    put(11, g_add(t3, s3));
+    put(12, g_add(s2, s3));
+    put(13, g_add(t5, s3));
+    put(14, g_add(t6, s3));
+    put(15, g_add(t7, s3));
+    g8 acc = w_zero();
+#pragma unroll
+    for (int k = 7; k >= 0; --k) {
+        if (k != 7) {
+#pragma unroll
+            for (int i = 7; i >= 1; --i) acc.w[i] = shl_pair(acc.w[i], acc.w[i - 1], 4);
+            acc.w[0] <<= 4;
+        }
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t nib = __builtin_amdgcn_ubfe(b.w[w], 4 * k, 4);
+            const uint4 e = col[64 * nib];
+            acc.w[w] ^= e.x;
+            acc.w[w + 1] ^= e.y;
+            acc.w[w + 2] ^= e.z;
+            acc.w[w + 3] ^= e.w;
+        }
+    }
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 29, 1);
+    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 30, 1);
+    const uint32_t m2 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 31, 1);
+    uint32_t u[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const uint32_t bi = i < 4 ? b.w[i] : 0u, bp = i > 0 ? b.w[i - 1] : 0u;
+        u[i] = xor_and(xor_and(bi & m0, shl_pair(bi, bp, 1), m1), shl_pair(bi, bp, 2), m2);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) acc.w[3 + i] ^= __builtin_amdgcn_alignbit(u[i], i ? u[i - 1] : 0u, 3);
+    return g_reduce(acc);
+}
 #endif  // __HIPCC__

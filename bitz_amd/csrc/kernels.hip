@@ -460,6 +460,29 @@ __global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f1
     htab_block(L, ell, H, d, out, blockIdx.x, tab, basis);
 }
 
+// 4-bit H(i) tables of round 0: table i is the weight of iteration i over
+// the variables [start, n).
+__device__ __forceinline__ void h4_block(const f128* L, int ell, const f128* H, int start, uint4* out,
+                                         unsigned i, uint4* tab, uint4* basis) {
+    g4 c;
+    if (start < ell) {
+        const int lo_bits = ell - start;
+        const uint32_t lo = i & ((1u << lo_bits) - 1), hi = i >> lo_bits;
+        c = g_mul(g_load(L + ((size_t)lo << start)), g_load(H + hi));
+    } else {
+        c = g_load(H + ((size_t)i << (start - ell)));
+    }
+    build_const_table4(tab, basis, c);
+    uint4* dst = out + (size_t)i * kConstTable4Entries;
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
+}
+
+__global__ __launch_bounds__(1024) void k_htabs4(const f128* L, int ell, const f128* H, int start, uint4* out) {
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    h4_block(L, ell, H, start, out, blockIdx.x, tab, basis);
+}
+
 // Everything the pipelined prover needs before round 0, in one launch of
 // 1024-thread workgroups: the factor check, the H(i) tables, the 4-bit
 // tables of every challenge and the 8-bit tables of the challenges the
@@ -478,6 +501,11 @@ __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
         return;
     }
     b -= A.htab_blocks;
+    if (b < A.h4_blocks) {
+        h4_block(A.L, A.ell, A.H, A.h4_start, A.htabs4, b, tab, basis);
+        return;
+    }
+    b -= A.h4_blocks;
     if (b < A.t4_blocks) {
         build_const_table4(tab, basis, g_load(A.chal + b));
         uint4* dst = A.rtabs4 + (size_t)b * kConstTable4Entries;
@@ -609,11 +637,68 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
     block_xor_to_global<3>(v, acc);
 }
 
+// Round 0 of the eq-factored outer sumcheck with the general products
+// through per-lane LDS tables (g_mul_lane): 256-thread workgroups, each
+// wave owning 16 KiB of lane tables, H(i) through a 4-bit constant table
+// (8 KiB) copied per iteration.  Pair y = i * threads + t as in k_outer_eq.
+template <bool Q1>
+__global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
+                                                  size_t iters, const f128* la, int la_bits, int la_shift,
+                                                  const f128* lb, int lb_shift, const uint4* h_tabs4,
+                                                  uint32_t* acc) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* htab = smem_raw;
+    uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
+    for (size_t i = 0; i < iters; ++i) {
+        if (iters > 1) {
+            __syncthreads();  // the previous iteration's readers are done with htab
+            for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
+                htab[e] = h_tabs4[i * kConstTable4Entries + e];
+            __syncthreads();
+        }
+        if (t < threads) {
+            const size_t y = i * threads + t;
+            const g4 a0 = g_load(a + 2 * y), a1 = g_load(a + 2 * y + 1);
+            const g4 b0 = g_load(b + 2 * y), b1 = g_load(b + 2 * y + 1);
+            const g4 p0 = g_add(g_mul_lane(a0, b0, col), g_load(c + 2 * y));
+            const g4 pi = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
+            g4 p1;
+            if (Q1) p1 = g_add(g_mul_lane(a1, b1, col), g_load(c + 2 * y + 1));
+            if (iters > 1) {
+                s0 = g_add(s0, g_mul_table4(htab, p0));
+                si = g_add(si, g_mul_table4(htab, pi));
+                if (Q1) s1 = g_add(s1, g_mul_table4(htab, p1));
+            } else {
+                s0 = p0;
+                si = pi;
+                if (Q1) s1 = p1;
+            }
+        }
+    }
+    g4 v[3] = {g_zero(), g_zero(), g_zero()};
+    if (t < threads) {
+        const size_t mask = ((size_t)1 << la_bits) - 1;
+        const g4 L = g_mul_lane(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)), col);
+        v[0] = g_mul_lane(L, s0, col);
+        v[1] = g_mul_lane(L, si, col);
+        if (Q1) v[2] = g_mul_lane(L, s1, col);
+    }
+    block_xor_to_global<3>(v, acc);
+}
+
+#define OUTER_R0_SIG                                                                                        \
+    const f128*, const f128*, const f128*, size_t, size_t, const f128*, int, int, const f128*, int, const uint4*, \
+        uint32_t*
+template __global__ void k_outer_r0<false>(OUTER_R0_SIG);
+template __global__ void k_outer_r0<true>(OUTER_R0_SIG);
+#undef OUTER_R0_SIG
+
 #define OUTER_EQ_SIG                                                                                       \
     const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, \
         int, const f128*, int, const uint4*, uint32_t*
-template __global__ void k_outer_eq<false, false>(OUTER_EQ_SIG);
-template __global__ void k_outer_eq<false, true>(OUTER_EQ_SIG);
 template __global__ void k_outer_eq<true, false>(OUTER_EQ_SIG);
 template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
 #undef OUTER_EQ_SIG

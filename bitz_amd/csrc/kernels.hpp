@@ -72,6 +72,9 @@ struct PrologueArgs {
     unsigned htab_blocks;
     HtabDesc hdesc;
     uint4* htabs;
+    unsigned h4_blocks;  // 4-bit H(i) tables of round 0
+    int h4_start;
+    uint4* htabs4;
     const f128* chal;
     unsigned t4_blocks;  // 4-bit tables of chal[0 ..)
     uint4* rtabs4;
@@ -79,6 +82,16 @@ struct PrologueArgs {
     uint4* rtabs;
 };
 __global__ void k_prologue(PrologueArgs args);
+__global__ void k_htabs4(const f128* L, int ell, const f128* H, int start, uint4* out);
+
+template <bool Q1>
+__global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads, size_t iters,
+                           const f128* la, int la_bits, int la_shift, const f128* lb, int lb_shift,
+                           const uint4* h_tabs4, uint32_t* acc);
+// Round 0 with lane tables: 256 threads, 4 waves x 16 KiB + one 4-bit table.
+constexpr int kR0Block = 256;
+constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
+constexpr int kR0ThreadBits = 17;  // 2^17 threads: two workgroups on each of 256 CUs
 
 // Single-workgroup tail of an eq-factored proof (see k_outer_tail).
 constexpr int kTailMaxLog = 11;         // at most 2^11 elements per table in LDS
