@@ -560,7 +560,7 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
-    Dev htabs, htabs4, rtabs, rtabs4, chal;
+    Dev htabs4, rtabs, rtabs4, chal;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
     };
@@ -569,12 +569,11 @@ struct bitz_outer {
         Run la, lb;
         int la_bits;
         int h_start;       // first iteration variable
-        size_t htab_off;   // first H(i) table (multi-iteration rounds)
+        size_t htab_off;   // first 4-bit H(i) table (multi-iteration rounds)
     };
     std::vector<Geo> geo;
     bitz::HtabDesc hdesc{};
-    size_t htab_total = 0;
-    size_t h4_count = 0;  // round 0's 4-bit H(i) tables
+    size_t htab_total = 0;  // 4-bit H(i) tables of the multi-iteration rounds
     f128 S{1, 0}, sigma{0, 0};
     f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
 
@@ -671,9 +670,9 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     size_t htab_total = 0;
     for (int k = 0; k < n; ++k) {
         const int pbits = n - k - 1;  // log2 pairs of round k
-        // log2 threads: round 0's lane-table kernel keeps two 256-thread
-        // workgroups on every CU; the fold rounds run 2^18 threads.
-        const int tbits = std::min(pbits, k == 0 ? bitz::kR0ThreadBits : 18);
+        // log2 threads: one wave of lane-table workgroups across the chip;
+        // larger rounds iterate.
+        const int tbits = std::min(pbits, bitz::kThreadBits);
         auto& g = h->geo[k];
         g.threads = (size_t)1 << tbits;
         g.iters = (size_t)1 << (pbits - tbits);
@@ -690,7 +689,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
         g.la_bits = g.la.b - g.la.a;
         g.h_start = b;
         g.htab_off = 0;
-        if (g.iters > 1 && k > 0) {
+        if (g.iters > 1) {
             desc.first[desc.rounds] = (uint32_t)htab_total;
             desc.start[desc.rounds] = b;
             desc.rounds += 1;
@@ -701,9 +700,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     desc.first[desc.rounds] = (uint32_t)htab_total;
     h->hdesc = desc;
     h->htab_total = htab_total;
-    h->h4_count = h->geo[0].iters > 1 ? h->geo[0].iters : 0;
-    if (h->htabs.alloc(std::max<size_t>(htab_total, 1) * kConstTableEntries * sizeof(uint4)) ||
-        h->htabs4.alloc(std::max<size_t>(h->h4_count, 1) * kConstTable4Entries * sizeof(uint4)) ||
+    if (h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
         h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->chal.alloc((size_t)n * sizeof(f128)))
@@ -718,6 +715,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_r0<true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_fold<false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_fold<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(3 * sizeof(uint4) << bitz::kTailMaxLog));
@@ -737,11 +738,8 @@ static int32_t outer_launch_checks(bitz_outer* h) {
     hipLaunchKernelGGL(bitz::k_eq_check, dim3(bl + bh), dim3(kBlock), 0, s, h->eqL[0].f(), h->ell,
                        h->eqH[0].f(), h->m, h->acc.u32() + h->eq_off(), bl);
     if (h->htab_total)
-        hipLaunchKernelGGL(bitz::k_htabs, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
-                           h->eqH[0].f(), h->hdesc, static_cast<uint4*>(h->htabs.p));
-    if (h->h4_count)
-        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->h4_count), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
-                           h->eqH[0].f(), h->geo[0].h_start, static_cast<uint4*>(h->htabs4.p));
+        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
+                           h->eqH[0].f(), h->hdesc, static_cast<uint4*>(h->htabs4.p));
     return check_launch();
 }
 
@@ -818,52 +816,60 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     return true;
 }
 
-// Launches the eq-factored round-k kernel into slot k: round 0 through lane
-// tables (k_outer_r0), later rounds folding the previous tables through the
-// LDS table rtab of r_{k-1} (k_outer_eq).
-static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab, bool q1, f128* const out[3],
-                               Timed* tm = nullptr) {
+// Launches the eq-factored round-k kernel into slot k: round 0 and the
+// multi-iteration fold rounds through lane tables (k_outer_r0,
+// k_outer_fold, folding with the 4-bit table rtab4 of r_{k-1}), the
+// single-iteration fold rounds through k_outer_eq (8-bit table rtab8).
+static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab8, const uint4* rtab4, bool q1,
+                               f128* const out[3], Timed* tm = nullptr) {
     if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
-    const bool fold = k > 0;
-    // Multi-iteration rounds: 1024-thread workgroups share each LDS table
-    // copy.  Single-iteration rounds: 256-thread workgroups, so small rounds
-    // spread over every CU.  Whole waves only (the block reduction needs them).
-    const size_t want = g.iters > 1 ? 1024 : 256;
-    const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, want));
-    const unsigned grid = (unsigned)((g.threads + block - 1) / block);
     uint32_t* slot = h->acc.u32() + 12 * k;
     hipStream_t s = h->so.s;
     const size_t pairs = g.threads * g.iters;
     tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 96.0 * pairs : 288.0 * pairs);
-    const size_t smem = 2 * kConstTableEntries * sizeof(uint4);
-    const uint4* htabs = static_cast<const uint4*>(h->htabs.p) + g.htab_off * kConstTableEntries;
+    const uint4* h4 = static_cast<const uint4*>(h->htabs4.p) + g.htab_off * kConstTable4Entries;
     const f128* la = h->table_ptr(g.la.table);
     const f128* lb = h->table_ptr(g.lb.table);
     const int la_shift = g.la.a - h->table_base(g.la.table), lb_shift = g.lb.a - h->table_base(g.lb.table);
-    if (!fold) {
-        const unsigned g0 = (unsigned)((g.threads + bitz::kR0Block - 1) / bitz::kR0Block);
-        const uint4* h4 = static_cast<const uint4*>(h->htabs4.p);
+    if (k == 0) {
+        const unsigned grid = (unsigned)((g.threads + bitz::kR0Block - 1) / bitz::kR0Block);
 #define OUTER_R0_ARGS \
     h->cur[0], h->cur[1], h->cur[2], g.threads, g.iters, la, g.la_bits, la_shift, lb, lb_shift, h4, slot
         if (q1)
-            hipLaunchKernelGGL((bitz::k_outer_r0<true>), dim3(g0), dim3(bitz::kR0Block), bitz::kR0Smem, s,
+            hipLaunchKernelGGL((bitz::k_outer_r0<true>), dim3(grid), dim3(bitz::kR0Block), bitz::kR0Smem, s,
                                OUTER_R0_ARGS);
         else
-            hipLaunchKernelGGL((bitz::k_outer_r0<false>), dim3(g0), dim3(bitz::kR0Block), bitz::kR0Smem, s,
+            hipLaunchKernelGGL((bitz::k_outer_r0<false>), dim3(grid), dim3(bitz::kR0Block), bitz::kR0Smem, s,
                                OUTER_R0_ARGS);
 #undef OUTER_R0_ARGS
-        tm->end(s);
-        return check_launch();
-    }
+    } else if (g.iters > 1) {
+        const unsigned grid = (unsigned)((g.threads + bitz::kFoldBlock - 1) / bitz::kFoldBlock);
+#define OUTER_FOLD_ARGS                                                                                  \
+    h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], g.threads, g.iters, rtab4, la, g.la_bits, la_shift, \
+        lb, lb_shift, h4, slot
+        if (q1)
+            hipLaunchKernelGGL((bitz::k_outer_fold<true>), dim3(grid), dim3(bitz::kFoldBlock), bitz::kFoldSmem, s,
+                               OUTER_FOLD_ARGS);
+        else
+            hipLaunchKernelGGL((bitz::k_outer_fold<false>), dim3(grid), dim3(bitz::kFoldBlock), bitz::kFoldSmem, s,
+                               OUTER_FOLD_ARGS);
+#undef OUTER_FOLD_ARGS
+    } else {
+        // Single iteration: 256-thread workgroups (whole waves) so small
+        // rounds spread over every CU.
+        const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, 256));
+        const unsigned grid = (unsigned)((g.threads + block - 1) / block);
+        const size_t smem = 2 * kConstTableEntries * sizeof(uint4);
 #define OUTER_EQ_ARGS                                                                                     \
-    h->cur[0], h->cur[1], h->cur[2], out ? out[0] : nullptr, out ? out[1] : nullptr, out ? out[2] : nullptr, \
-        g.threads, g.iters, rtab, la, g.la_bits, la_shift, lb, lb_shift, htabs, slot
-    if (q1)
-        hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
-    else
-        hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+    h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], g.threads, g.iters, rtab8, la, g.la_bits, la_shift, \
+        lb, lb_shift, nullptr, slot
+        if (q1)
+            hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+        else
+            hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
 #undef OUTER_EQ_ARGS
+    }
     tm->end(s);
     return check_launch();
 }
@@ -997,7 +1003,7 @@ extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
     if (h->eqmode) {
         // q(1) is computed: the claim the caller holds is not trusted here.
         // The factor check is read with the first sums.
-        TRY(outer_eq_launch(h, 0, nullptr, true, nullptr));
+        TRY(outer_eq_launch(h, 0, nullptr, nullptr, true, nullptr));
         bitz_f128 raw[3];
         if (h->eq_valid) {
             TRY(h->read_slot(0, raw));
@@ -1033,10 +1039,14 @@ extern "C" int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 rb, bitz_f128* 
         const int k = h->round;  // r is round k's challenge
         outer_eq_advance(h, k, r);
         if (k + 1 < h->n) {
-            uint4* rtab = static_cast<uint4*>(h->rtabs.p);
-            hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab);
+            uint4* rtab8 = static_cast<uint4*>(h->rtabs.p);
+            uint4* rtab4 = static_cast<uint4*>(h->rtabs4.p);
+            if (h->geo[k + 1].iters > 1)
+                hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, rtab4);
+            else
+                hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab8);
             const bool q1 = hostf::is_zero(h->tau[k + 1]);
-            TRY(outer_eq_launch(h, k + 1, rtab, q1, out));
+            TRY(outer_eq_launch(h, k + 1, rtab8, rtab4, q1, out));
             for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
             h->where = (h->where == 1) ? 2 : 1;
             h->len = next_len;
@@ -1229,7 +1239,7 @@ struct PipeSchedule {
 };
 static PipeSchedule pipe_schedule(int n) {
     const int k0 = std::max(0, n - kTailLog);
-    const int kc = std::max(1, n - 19);  // rounds before kc have more pairs than one launch's threads
+    const int kc = std::max(1, n - 1 - bitz::kThreadBits);  // rounds before kc iterate over their pairs
     if (k0 - kc >= 1) return {kc, k0};
     return {k0, k0};
 }
@@ -1254,18 +1264,19 @@ static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenge
     pa.eq_blocks_l = eq_check_blocks(h->low_len, 1024);
     pa.eq_blocks = pa.eq_blocks_l + eq_check_blocks(h->high_len, 1024);
     pa.eq_out = h->acc.u32() + h->eq_off();
-    pa.htab_blocks = (unsigned)h->htab_total;
+    pa.h4_blocks = (unsigned)h->htab_total;
     pa.hdesc = h->hdesc;
-    pa.htabs = static_cast<uint4*>(h->htabs.p);
-    pa.h4_blocks = (unsigned)h->h4_count;
-    pa.h4_start = h->geo[0].h_start;
     pa.htabs4 = static_cast<uint4*>(h->htabs4.p);
     pa.chal = h->chal.f();
     pa.t4_blocks = (unsigned)n;
     pa.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
-    pa.t8_blocks = (unsigned)std::max(0, ps.kc - 1);
+    // 8-bit tables only for grid-wide rounds that run single-iteration.
+    unsigned t8 = 0;
+    for (int k = 1; k < ps.kc; ++k)
+        if (h->geo[k].iters == 1) t8 = (unsigned)k;
+    pa.t8_blocks = t8;
     pa.rtabs = static_cast<uint4*>(h->rtabs.p);
-    const unsigned grid = pa.eq_blocks + pa.htab_blocks + pa.h4_blocks + pa.t4_blocks + pa.t8_blocks;
+    const unsigned grid = pa.eq_blocks + pa.h4_blocks + pa.t4_blocks + pa.t8_blocks;
     hipLaunchKernelGGL(bitz::k_prologue, dim3(grid), dim3(1024), 0, s, pa);
     return check_launch();
 }
@@ -1295,7 +1306,8 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         f128* out[3] = {nullptr, nullptr, nullptr};
         if (k > 0) next_out(out);
         TRY(outer_eq_launch(h, k, k > 0 ? rtabs + (size_t)(k - 1) * kConstTableEntries : nullptr,
-                            (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k]));
+                            k > 0 ? rtabs4 + (size_t)(k - 1) * kConstTable4Entries : nullptr, (q1mask >> k) & 1,
+                            k > 0 ? out : nullptr, &timers[k]));
         if (k > 0) advance(out, h->len / 2);
     }
     if (ps.kc < ps.k0) {

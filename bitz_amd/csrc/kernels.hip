@@ -431,12 +431,12 @@ __global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, con
     eq_check_block(L, ell, H, m, out, blocks_l, blockIdx.x);
 }
 
-// H(i) constant tables for the multi-iteration rounds: table b belongs to
-// the round j with first[j] <= b < first[j + 1]; its constant is the weight
-// of iteration i = b - first[j] over the variables [start[j], n), read from
-// the factors (a product when the run crosses from L into H).
-__device__ __forceinline__ void htab_block(const f128* L, int ell, const f128* H, const HtabDesc& d, uint4* out,
-                                           unsigned bidx, uint4* tab, uint4* basis) {
+// H(i) tables (4-bit windows) of the multi-iteration rounds: table b belongs
+// to the round j with first[j] <= b < first[j + 1]; its constant is the
+// weight of iteration i = b - first[j] over the variables [start[j], n),
+// read from the factors (a product when the run crosses from L into H).
+__device__ __forceinline__ void h4_block(const f128* L, int ell, const f128* H, const HtabDesc& d, uint4* out,
+                                         unsigned bidx, uint4* tab, uint4* basis) {
     int j = 0;
     while (j + 1 < d.rounds && bidx >= d.first[j + 1]) ++j;
     const uint32_t i = bidx - d.first[j];
@@ -449,44 +449,28 @@ __device__ __forceinline__ void htab_block(const f128* L, int ell, const f128* H
     } else {
         c = g_load(H + ((size_t)i << (s - ell)));
     }
-    build_const_table(tab, basis, c);
-    uint4* dst = out + (size_t)bidx * kConstTableEntries;
-    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
-}
-
-__global__ __launch_bounds__(1024) void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out) {
-    __shared__ uint4 tab[kConstTableEntries];
-    __shared__ uint4 basis[128];
-    htab_block(L, ell, H, d, out, blockIdx.x, tab, basis);
-}
-
-// 4-bit H(i) tables of round 0: table i is the weight of iteration i over
-// the variables [start, n).
-__device__ __forceinline__ void h4_block(const f128* L, int ell, const f128* H, int start, uint4* out,
-                                         unsigned i, uint4* tab, uint4* basis) {
-    g4 c;
-    if (start < ell) {
-        const int lo_bits = ell - start;
-        const uint32_t lo = i & ((1u << lo_bits) - 1), hi = i >> lo_bits;
-        c = g_mul(g_load(L + ((size_t)lo << start)), g_load(H + hi));
-    } else {
-        c = g_load(H + ((size_t)i << (start - ell)));
-    }
     build_const_table4(tab, basis, c);
-    uint4* dst = out + (size_t)i * kConstTable4Entries;
+    uint4* dst = out + (size_t)bidx * kConstTable4Entries;
     for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
 }
 
-__global__ __launch_bounds__(1024) void k_htabs4(const f128* L, int ell, const f128* H, int start, uint4* out) {
+__global__ __launch_bounds__(1024) void k_htabs4(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out) {
     __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
-    h4_block(L, ell, H, start, out, blockIdx.x, tab, basis);
+    h4_block(L, ell, H, d, out, blockIdx.x, tab, basis);
+}
+
+__global__ __launch_bounds__(1024) void k_const_table4_1(f128 c, uint4* out) {
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    build_const_table4(tab, basis, g_from(c));
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) out[e] = tab[e];
 }
 
 // Everything the pipelined prover needs before round 0, in one launch of
 // 1024-thread workgroups: the factor check, the H(i) tables, the 4-bit
 // tables of every challenge and the 8-bit tables of the challenges the
-// grid-wide rounds fold with.
+// single-iteration grid-wide rounds fold with.
 __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
     __shared__ uint4 tab[kConstTableEntries];
     __shared__ uint4 basis[128];
@@ -496,13 +480,8 @@ __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
         return;
     }
     b -= A.eq_blocks;
-    if (b < A.htab_blocks) {
-        htab_block(A.L, A.ell, A.H, A.hdesc, A.htabs, b, tab, basis);
-        return;
-    }
-    b -= A.htab_blocks;
     if (b < A.h4_blocks) {
-        h4_block(A.L, A.ell, A.H, A.h4_start, A.htabs4, b, tab, basis);
+        h4_block(A.L, A.ell, A.H, A.hdesc, A.htabs4, b, tab, basis);
         return;
     }
     b -= A.h4_blocks;
@@ -695,6 +674,87 @@ __global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, 
 template __global__ void k_outer_r0<false>(OUTER_R0_SIG);
 template __global__ void k_outer_r0<true>(OUTER_R0_SIG);
 #undef OUTER_R0_SIG
+
+// Rounds k >= 1 with more pairs than threads: the fold by r_{k-1} through
+// r's 4-bit table, the general products through per-lane LDS tables
+// (g_mul_lane) and H(i) through 4-bit tables.  512-thread workgroups: eight
+// waves x 16 KiB of lane tables plus the two 8 KiB constant tables.
+template <bool Q1>
+__global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao,
+                                                    f128* bo, f128* co, size_t threads, size_t iters,
+                                                    const uint4* r_tab4, const f128* la, int la_bits,
+                                                    int la_shift, const f128* lb, int lb_shift,
+                                                    const uint4* h_tabs4, uint32_t* acc) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* rt = smem_raw;
+    uint4* ht = smem_raw + kConstTable4Entries;
+    uint4* col = smem_raw + 2 * kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
+                 (threadIdx.x & 63);
+    lane_table_clear(col);
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = r_tab4[e];
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
+    for (size_t i = 0; i < iters; ++i) {
+        if (iters > 1) {
+            __syncthreads();  // the previous iteration's readers are done with ht
+            for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
+                ht[e] = h_tabs4[i * kConstTable4Entries + e];
+        }
+        __syncthreads();
+        if (t < threads) {
+            const size_t y = i * threads + t;
+            g4 in[3][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                in[0][j] = g_load(a + 4 * y + j);
+                in[1][j] = g_load(b + 4 * y + j);
+                in[2][j] = g_load(c + 4 * y + j);
+            }
+            g4 f[3][2];
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    f[u][j] = g_add(in[u][2 * j], g_mul_table4(rt, g_add(in[u][2 * j], in[u][2 * j + 1])));
+            g_store(ao + 2 * y, f[0][0]);
+            g_store(ao + 2 * y + 1, f[0][1]);
+            g_store(bo + 2 * y, f[1][0]);
+            g_store(bo + 2 * y + 1, f[1][1]);
+            g_store(co + 2 * y, f[2][0]);
+            g_store(co + 2 * y + 1, f[2][1]);
+            const g4 p0 = g_add(g_mul_lane(f[0][0], f[1][0], col), f[2][0]);
+            const g4 pi = g_mul_lane(g_add(f[0][0], f[0][1]), g_add(f[1][0], f[1][1]), col);
+            g4 p1;
+            if (Q1) p1 = g_add(g_mul_lane(f[0][1], f[1][1], col), f[2][1]);
+            if (iters > 1) {
+                s0 = g_add(s0, g_mul_table4(ht, p0));
+                si = g_add(si, g_mul_table4(ht, pi));
+                if (Q1) s1 = g_add(s1, g_mul_table4(ht, p1));
+            } else {
+                s0 = p0;
+                si = pi;
+                if (Q1) s1 = p1;
+            }
+        }
+    }
+    g4 v[3] = {g_zero(), g_zero(), g_zero()};
+    if (t < threads) {
+        const size_t mask = ((size_t)1 << la_bits) - 1;
+        const g4 L =
+            g_mul_lane(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)), col);
+        v[0] = g_mul_lane(L, s0, col);
+        v[1] = g_mul_lane(L, si, col);
+        if (Q1) v[2] = g_mul_lane(L, s1, col);
+    }
+    block_xor_to_global<3>(v, acc);
+}
+
+#define OUTER_FOLD_SIG                                                                                    \
+    const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, \
+        int, const f128*, int, const uint4*, uint32_t*
+template __global__ void k_outer_fold<false>(OUTER_FOLD_SIG);
+template __global__ void k_outer_fold<true>(OUTER_FOLD_SIG);
+#undef OUTER_FOLD_SIG
 
 #define OUTER_EQ_SIG                                                                                       \
     const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, \

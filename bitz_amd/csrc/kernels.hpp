@@ -52,14 +52,13 @@ __global__ void k_const_tables(const f128* consts, uint4* out);
 __global__ void k_const_table1(f128 c, uint4* out);
 __global__ void k_eq_check(const f128* L, int ell, const f128* H, int m, uint32_t* out, unsigned blocks_l);
 
-// H(i) tables of the multi-iteration rounds (see k_htabs).
+// H(i) tables of the multi-iteration rounds (see k_htabs4).
 constexpr int kMaxRounds = 36;
 struct HtabDesc {
     int rounds;
     uint32_t first[kMaxRounds + 1];
     int start[kMaxRounds];
 };
-__global__ void k_htabs(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out);
 
 __global__ void k_const_tables4(const f128* consts, uint4* out);
 
@@ -69,11 +68,8 @@ struct PrologueArgs {
     int ell, m;
     unsigned eq_blocks, eq_blocks_l;  // factor check: 1024 entries per workgroup
     uint32_t* eq_out;
-    unsigned htab_blocks;
+    unsigned h4_blocks;  // 4-bit H(i) tables of the multi-iteration rounds
     HtabDesc hdesc;
-    uint4* htabs;
-    unsigned h4_blocks;  // 4-bit H(i) tables of round 0
-    int h4_start;
     uint4* htabs4;
     const f128* chal;
     unsigned t4_blocks;  // 4-bit tables of chal[0 ..)
@@ -82,7 +78,16 @@ struct PrologueArgs {
     uint4* rtabs;
 };
 __global__ void k_prologue(PrologueArgs args);
-__global__ void k_htabs4(const f128* L, int ell, const f128* H, int start, uint4* out);
+__global__ void k_htabs4(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out);
+__global__ void k_const_table4_1(f128 c, uint4* out);
+
+template <bool Q1>
+__global__ void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
+                             size_t threads, size_t iters, const uint4* r_tab4, const f128* la, int la_bits,
+                             int la_shift, const f128* lb, int lb_shift, const uint4* h_tabs4, uint32_t* acc);
+constexpr int kFoldBlock = 512;
+constexpr size_t kFoldSmem =
+    sizeof(uint4) * (2 * kConstTable4Entries + (kFoldBlock / 64) * 64 * kLaneTableEntries);
 
 template <bool Q1>
 __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads, size_t iters,
@@ -91,7 +96,7 @@ __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t t
 // Round 0 with lane tables: 256 threads, 4 waves x 16 KiB + one 4-bit table.
 constexpr int kR0Block = 256;
 constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
-constexpr int kR0ThreadBits = 17;  // 2^17 threads: two workgroups on each of 256 CUs
+constexpr int kThreadBits = 17;  // threads of a multi-iteration round: one wave of workgroups
 
 // Single-workgroup tail of an eq-factored proof (see k_outer_tail).
 constexpr int kTailMaxLog = 11;         // at most 2^11 elements per table in LDS
