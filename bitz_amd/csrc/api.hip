@@ -546,8 +546,13 @@ struct bitz_outer {
     const f128* cur[3] = {nullptr, nullptr, nullptr};
     int where = 0;     // 0 = inputs, 1 = P, 2 = Q
     size_t capP = 0, capQ = 0;
+    // eq factors: eq_in[] are the caller's device tables (or an uploaded
+    // copy); the general kernels fold an owned copy (eqL/eqH ping-pong).
+    const f128* eq_in[2] = {nullptr, nullptr};
+    bool factors_owned = false;
     Dev eqL[2], eqH[2];
     int curL = 0, curH = 0;
+    unsigned eq_bl = 1, eq_bh = 1;  // factor-check workgroups over eq_low, eq_high
     Dev acc;
     Pinned host;
     Timed tm;
@@ -560,7 +565,7 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
-    Dev htabs4, rtabs, rtabs4, chal;
+    Dev htabs4, rtabs, rtabs4;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
     };
@@ -577,12 +582,13 @@ struct bitz_outer {
     f128 S{1, 0}, sigma{0, 0};
     f128 q[3] = {{0, 0}, {0, 0}, {0, 0}};  // current round's q0, q1, q2
 
-    // acc layout (32-bit words): round k's sums at 12 k; the eq check at
-    // eq_off(); the pipelined prover's final A(r), B(r), C(r) at fin_off().
+    // acc layout (32-bit words): round k's sums at 12 k; the factor check at
+    // eq_off() (n + 2 probes, then 8 words per check workgroup); the
+    // pipelined prover's final A(r), B(r), C(r) at fin_off().
     size_t eq_off() const { return 12 * (size_t)(n + 2); }
-    size_t fin_off() const { return eq_off() + 12 + 4 * (size_t)(n + 2); }
+    size_t fin_off() const { return eq_off() + 4 * (size_t)(n + 2) + 8 * (size_t)(eq_bl + eq_bh); }
     size_t acc_words() const { return fin_off() + 12; }
-    const f128* table_ptr(int t) const { return t == 0 ? eqL[0].f() : eqH[0].f(); }
+    const f128* table_ptr(int t) const { return eq_in[t]; }
     int table_base(int t) const { return t == 0 ? 0 : ell; }
 
     int32_t read_words(size_t words) {
@@ -661,7 +667,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     const int n = h->n;
     if (n == 0) return false;
     const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
-    if (nl + nh > ((size_t)1 << 21)) return false;
+    if (nl + nh > ((size_t)1 << 18)) return false;
     h->ell = low_vars;
     h->m = high_vars;
     const int ell = low_vars;
@@ -702,8 +708,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     h->htab_total = htab_total;
     if (h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
-        h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
-        h->chal.alloc((size_t)n * sizeof(f128)))
+        h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)))
         return false;
     static bool attr_set = false;
     if (!attr_set) {
@@ -734,12 +739,11 @@ static unsigned eq_check_blocks(size_t entries, unsigned block) {
 // The factor check and the H(i) tables, for the per-round prover.
 static int32_t outer_launch_checks(bitz_outer* h) {
     hipStream_t s = h->so.s;
-    const unsigned bl = eq_check_blocks(h->low_len, kBlock), bh = eq_check_blocks(h->high_len, kBlock);
-    hipLaunchKernelGGL(bitz::k_eq_check, dim3(bl + bh), dim3(kBlock), 0, s, h->eqL[0].f(), h->ell,
-                       h->eqH[0].f(), h->m, h->acc.u32() + h->eq_off(), bl);
+    hipLaunchKernelGGL(bitz::k_eq_check, dim3(h->eq_bl + h->eq_bh), dim3(1024), 0, s, h->eq_in[0], h->ell,
+                       h->eq_in[1], h->m, h->acc.u32() + h->eq_off(), h->eq_bl);
     if (h->htab_total)
-        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eqL[0].f(), h->ell,
-                           h->eqH[0].f(), h->hdesc, static_cast<uint4*>(h->htabs4.p));
+        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eq_in[0], h->ell,
+                           h->eq_in[1], h->hdesc, static_cast<uint4*>(h->htabs4.p));
     return check_launch();
 }
 
@@ -750,10 +754,15 @@ static int32_t outer_launch_checks(bitz_outer* h) {
 // T[0] = prod_V (1 + tau_i) and T[2^i] / T[0] = tau_i / (1 + tau_i).
 static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     const int n = h->n, ell = h->ell;
-    if (region[0]) return false;
-    if (!hostf::eq(f128_at(region + 4), hostf::one()) || !hostf::eq(f128_at(region + 8), hostf::one()))
-        return false;
-    const uint32_t* probes = region + 12;
+    const uint32_t* probes = region;
+    const uint32_t* rec = region + 4 * (n + 2);
+    f128 sum[2] = {{0, 0}, {0, 0}};
+    for (unsigned b = 0; b < h->eq_bl + h->eq_bh; ++b) {
+        if (rec[8 * b]) return false;
+        const f128 x = f128_at(rec + 8 * b + 4);
+        sum[b >= h->eq_bl] = hostf::add(sum[b >= h->eq_bl], x);
+    }
+    if (!hostf::eq(sum[0], hostf::one()) || !hostf::eq(sum[1], hostf::one())) return false;
     std::vector<f128> den(n), num(n);
     for (int t = 0; t < 2; ++t) {
         const int base = t == 0 ? 0 : ell, bits = t == 0 ? ell : h->m;
@@ -904,6 +913,22 @@ static void outer_eq_advance(bitz_outer* h, int k, f128 r) {
     h->S = hostf::mul(h->S, e);
 }
 
+// The general kernels fold the eq factors: give them an owned, writable
+// copy (the caller's tables are borrowed) and the half-size ping-pong tables.
+static int32_t outer_own_factors(bitz_outer* h) {
+    hipStream_t s = h->so.s;
+    if (!h->factors_owned) {
+        TRY(h->eqL[0].alloc(h->low_len * sizeof(f128)));
+        TRY(h->eqH[0].alloc(h->high_len * sizeof(f128)));
+        HIP_TRY(hipMemcpyAsync(h->eqL[0].p, h->eq_in[0], h->low_len * sizeof(f128), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(h->eqH[0].p, h->eq_in[1], h->high_len * sizeof(f128), hipMemcpyDeviceToDevice, s));
+        h->factors_owned = true;
+    }
+    if (!h->eqL[1].p) TRY(h->eqL[1].alloc(std::max<size_t>(h->low_len / 2, 1) * sizeof(f128)));
+    if (!h->eqH[1].p) TRY(h->eqH[1].alloc(std::max<size_t>(h->high_len / 2, 1) * sizeof(f128)));
+    return BITZ_OK;
+}
+
 // launch_checks: launch the factor check and H(i) tables now (per-round
 // prover); the pipelined prover folds them into its prologue instead.
 static int32_t outer_begin_impl(bitz_outer** out, const bitz_f128* eq_low, uint32_t low_vars,
@@ -945,16 +970,28 @@ static int32_t outer_begin_impl(bitz_outer** out, const bitz_f128* eq_low, uint3
     h->capQ = std::max<size_t>(N / 4, 1);
     if ((rc = h->P.alloc(3 * h->capP * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->Q.alloc(3 * h->capQ * sizeof(f128)))) return cleanup(rc);
-    if ((rc = upload(h->eqL[0], eq_low, h->low_len, flags, s))) return cleanup(rc);
-    if ((rc = upload(h->eqH[0], eq_high, h->high_len, flags, s))) return cleanup(rc);
-    if ((rc = h->eqL[1].alloc(std::max<size_t>(h->low_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
-    if ((rc = h->eqH[1].alloc(std::max<size_t>(h->high_len / 2, 1) * sizeof(f128)))) return cleanup(rc);
+    if (flags & BITZ_DEVICE) {
+        h->eq_in[0] = reinterpret_cast<const f128*>(eq_low);
+        h->eq_in[1] = reinterpret_cast<const f128*>(eq_high);
+    } else {
+        if ((rc = upload(h->eqL[0], eq_low, h->low_len, flags, s))) return cleanup(rc);
+        if ((rc = upload(h->eqH[0], eq_high, h->high_len, flags, s))) return cleanup(rc);
+        h->eq_in[0] = h->eqL[0].f();
+        h->eq_in[1] = h->eqH[0].f();
+        h->factors_owned = true;
+    }
+    h->eq_bl = eq_check_blocks(h->low_len, 1024);
+    h->eq_bh = eq_check_blocks(h->high_len, 1024);
     if ((rc = h->acc.alloc(h->acc_words() * sizeof(uint32_t)))) return cleanup(rc);
-    if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s) != hipSuccess)
+    if (h->acc_words() * sizeof(uint32_t) > PinnedPool::kBytes / 2)
+        return cleanup(fail(BITZ_ERR_ARG, "eq factors too large"));
+    // The pipelined prover's prologue clears what it needs.
+    if (launch_checks && hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
     if ((rc = h->host.alloc(h->acc_words() * sizeof(uint32_t)))) return cleanup(rc);
     h->eqmode = outer_setup_eq(h, (int)low_vars, (int)high_vars);
     if (h->eqmode && launch_checks && outer_launch_checks(h)) h->eqmode = false;
+    if (!h->eqmode && (rc = outer_own_factors(h))) return cleanup(rc);
     (void)hipGetLastError();  // a failed probe must not leak into later checks
     *out = h;
     return BITZ_OK;
@@ -1018,6 +1055,7 @@ extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
         }
         // Not eq tables of a point: the general kernels take over.
         HIP_TRY(hipMemsetAsync(h->acc.p, 0, 48, h->so.s));
+        TRY(outer_own_factors(h));
     }
     TRY(outer_generic_round0(h));
     return h->read_slot(0, co);
@@ -1244,30 +1282,28 @@ static PipeSchedule pipe_schedule(int n) {
     return {k0, k0};
 }
 
-// Uploads the challenges (through the handle's pinned block, so the copy
-// is asynchronous) and launches the prologue: factor check, H(i) tables and
-// the challenges' constant tables.
+// Launches the prologue: clearing of the round-sum slots, factor check,
+// H(i) tables and the challenges' constant tables (the challenges travel
+// as kernel arguments).
 static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenges) {
     hipStream_t s = h->so.s;
     const int n = h->n;
-    const size_t bytes = (size_t)n * sizeof(f128);
-    char* staging = static_cast<char*>(h->host.p) + PinnedPool::kBytes / 2;
-    if (h->acc_words() * 4 > PinnedPool::kBytes / 2) return fail(BITZ_ERR_ARG, "num_vars too large");
-    memcpy(staging, challenges, bytes);
-    HIP_TRY(hipMemcpyAsync(h->chal.p, staging, bytes, hipMemcpyHostToDevice, s));
+    if (n > bitz::kMaxRounds) return fail(BITZ_ERR_ARG, "num_vars too large");
     const PipeSchedule ps = pipe_schedule(n);
     bitz::PrologueArgs pa{};
-    pa.L = h->eqL[0].f();
-    pa.H = h->eqH[0].f();
+    pa.L = h->eq_in[0];
+    pa.H = h->eq_in[1];
     pa.ell = h->ell;
     pa.m = h->m;
-    pa.eq_blocks_l = eq_check_blocks(h->low_len, 1024);
-    pa.eq_blocks = pa.eq_blocks_l + eq_check_blocks(h->high_len, 1024);
+    pa.eq_blocks_l = h->eq_bl;
+    pa.eq_blocks = h->eq_bl + h->eq_bh;
     pa.eq_out = h->acc.u32() + h->eq_off();
+    pa.zero = h->acc.u32();
+    pa.zero_words = (unsigned)h->eq_off();
+    for (int k = 0; k < n; ++k) pa.chal[k] = F(challenges[k]);
     pa.h4_blocks = (unsigned)h->htab_total;
     pa.hdesc = h->hdesc;
     pa.htabs4 = static_cast<uint4*>(h->htabs4.p);
-    pa.chal = h->chal.f();
     pa.t4_blocks = (unsigned)n;
     pa.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
     // 8-bit tables only for grid-wide rounds that run single-iteration.
@@ -1281,9 +1317,10 @@ static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenge
     return check_launch();
 }
 
-// Enqueues a whole eq-factored proof with the challenges already on the
-// device (h->chal), then reads every round's sums, the factor check and the
-// final evaluations at once.  q1mask selects the rounds that compute q(1).
+// Enqueues a whole eq-factored proof after its prologue (the challenges'
+// constant tables are on the device), then reads every round's sums, the
+// factor check and the final evaluations at once.  q1mask selects the
+// rounds that compute q(1).
 static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Timed>& timers) {
     hipStream_t s = h->so.s;
     const int n = h->n;
@@ -1322,8 +1359,8 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         ca.bo = out[1];
         ca.co = out[2];
         ca.rtabs4 = rtabs4;
-        ca.L = h->eqL[0].f();
-        ca.H = h->eqH[0].f();
+        ca.L = h->eq_in[0];
+        ca.H = h->eq_in[1];
         ca.ell = h->ell;
         ca.n = n;
         ca.kc = ps.kc;
@@ -1344,8 +1381,8 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     ta.b = h->cur[1];
     ta.c = h->cur[2];
     ta.rtabs4 = rtabs4;
-    ta.L = h->eqL[0].f();
-    ta.H = h->eqH[0].f();
+    ta.L = h->eq_in[0];
+    ta.H = h->eq_in[1];
     ta.ell = h->ell;
     ta.n = n;
     ta.k0 = ps.k0;
@@ -1504,7 +1541,7 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
         if ((rc = outer_pipeline_eq(h, q1mask, timers))) goto done;
         if (!outer_validate_eq(h, h->host_words() + h->eq_off())) {
             h->eqmode = false;
-            if ((rc = h->rewind())) goto done;
+            if ((rc = h->rewind()) || (rc = outer_own_factors(h))) goto done;
         } else if (!outer_assemble_eq(h, challenges, claim, q1mask, !(q1mask & 1), co)) {
             q1mask = 1;
             for (int k = 0; k < n; ++k)

@@ -43,6 +43,36 @@ __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
     }
 }
 
+// XOR of v over the workgroup, stored (not accumulated) at dst: for kernels
+// that run as a single workgroup.  Whole waves only.
+template <int K>
+__device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst) {
+    __shared__ uint32_t red[1024 / 64][4 * K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t x = v[k].w[i];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
+            v[k].w[i] = x;
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
+    }
+    __syncthreads();
+    const int waves = (int)(blockDim.x >> 6);
+    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
+        uint32_t x = 0;
+        for (int w = 0; w < waves; ++w) x ^= red[w][j];
+        dst[j] = x;
+    }
+}
+
 __device__ __forceinline__ g4 interp(const g4& z, const g4& o, const g4& r) {
     return g_add(z, g_mul(r, g_add(o, z)));
 }
@@ -394,11 +424,6 @@ __device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__
 // the same for every thread of an iteration and is applied through an LDS
 // constant table; L(t) = La(t_low) Lb(t_high) is applied once per thread.
 
-// Checks that a factor is an eq table: T[x] T[0] = T[x - 2^top] T[2^top]
-// for every x > 0 (so T[x] = T[0] prod_{i in x} rho_i) and XOR-sums the
-// entries (eq tables sum to 1; the host checks the sums and the probes).
-// out: [0] mismatch flag, [4..7] sum of L, [8..11] sum of H, from [12] the
-// probes T[0], T[1], T[2], T[4], ... of L, then of H.
 __device__ __forceinline__ void eq_check_block(const f128* L, int ell, const f128* H, int m, uint32_t* out,
                                                unsigned blocks_l, unsigned bidx) {
     const bool in_h = bidx >= blocks_l;
@@ -406,8 +431,10 @@ __device__ __forceinline__ void eq_check_block(const f128* L, int ell, const f12
     const int bits = in_h ? m : ell;
     const size_t x = (size_t)(in_h ? bidx - blocks_l : bidx) * blockDim.x + threadIdx.x;
     const size_t size = (size_t)1 << bits;
-    f128* probes = reinterpret_cast<f128*>(out + 12) + (in_h ? ell + 1 : 0);
+    f128* probes = reinterpret_cast<f128*>(out) + (in_h ? ell + 1 : 0);
+    uint32_t* rec = out + 4 * (ell + m + 2) + 8 * bidx;
     g4 v[1] = {g_zero()};
+    int bad = 0;
     if (x < size) {
         const g4 tx = g_load(T + x);
         v[0] = tx;
@@ -418,16 +445,23 @@ __device__ __forceinline__ void eq_check_block(const f128* L, int ell, const f12
             const size_t bit = (size_t)1 << top;
             const g4 lhs = g_mul(tx, g_load(T));
             const g4 rhs = g_mul(g_load(T + (x ^ bit)), g_load(T + bit));
-            if ((lhs.w[0] ^ rhs.w[0]) | (lhs.w[1] ^ rhs.w[1]) | (lhs.w[2] ^ rhs.w[2]) | (lhs.w[3] ^ rhs.w[3]))
-                atomicOr(out, 1u);
+            bad = ((lhs.w[0] ^ rhs.w[0]) | (lhs.w[1] ^ rhs.w[1]) | (lhs.w[2] ^ rhs.w[2]) | (lhs.w[3] ^ rhs.w[3])) != 0;
             if (x == bit) g_store(probes + 1 + top, tx);
         }
     }
-    block_xor_to_global<1>(v, out + (in_h ? 8 : 4));
+    const int any_bad = __syncthreads_or(bad);
+    block_xor_store<1>(v, rec + 4);
+    if (threadIdx.x == 0) rec[0] = (uint32_t)any_bad;
 }
 
-__global__ __launch_bounds__(kBlock) void k_eq_check(const f128* L, int ell, const f128* H, int m,
-                                                     uint32_t* out, unsigned blocks_l) {
+// Checks that a factor is an eq table: T[x] T[0] = T[x - 2^top] T[2^top]
+// for every x > 0 (so T[x] = T[0] prod_{i in x} rho_i), and XOR-sums the
+// entries (eq tables sum to 1; the host checks the sums and the probes).
+// out: the probes T[0], T[1], T[2], T[4], ... of L, then of H (n + 2
+// elements), then per workgroup [mismatch flag, 0, 0, 0, XOR of its
+// entries] (8 words).  Plain stores only: out needs no clearing.
+__global__ __launch_bounds__(1024) void k_eq_check(const f128* L, int ell, const f128* H, int m, uint32_t* out,
+                                                   unsigned blocks_l) {
     eq_check_block(L, ell, H, m, out, blocks_l, blockIdx.x);
 }
 
@@ -468,13 +502,15 @@ __global__ __launch_bounds__(1024) void k_const_table4_1(f128 c, uint4* out) {
 }
 
 // Everything the pipelined prover needs before round 0, in one launch of
-// 1024-thread workgroups: the factor check, the H(i) tables, the 4-bit
-// tables of every challenge and the 8-bit tables of the challenges the
-// single-iteration grid-wide rounds fold with.
+// 1024-thread workgroups: clearing the round-sum slots, the factor check,
+// the H(i) tables, the 4-bit tables of every challenge and the 8-bit tables
+// of the challenges the single-iteration grid-wide rounds fold with.
 __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
     __shared__ uint4 tab[kConstTableEntries];
     __shared__ uint4 basis[128];
     unsigned b = blockIdx.x;
+    if (b == 0)
+        for (unsigned e = threadIdx.x; e < A.zero_words; e += blockDim.x) A.zero[e] = 0;
     if (b < A.eq_blocks) {
         eq_check_block(A.L, A.ell, A.H, A.m, A.eq_out, A.eq_blocks_l, b);
         return;
@@ -486,13 +522,13 @@ __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
     }
     b -= A.h4_blocks;
     if (b < A.t4_blocks) {
-        build_const_table4(tab, basis, g_load(A.chal + b));
+        build_const_table4(tab, basis, g_from(A.chal[b]));
         uint4* dst = A.rtabs4 + (size_t)b * kConstTable4Entries;
         for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
         return;
     }
     b -= A.t4_blocks;
-    build_const_table(tab, basis, g_load(A.chal + b));
+    build_const_table(tab, basis, g_from(A.chal[b]));
     uint4* dst = A.rtabs + (size_t)b * kConstTableEntries;
     for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
 }
@@ -535,36 +571,6 @@ __device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f1
     a0 = out[0][0]; a1 = out[0][1];
     b0 = out[1][0]; b1 = out[1][1];
     c0 = out[2][0]; c1 = out[2][1];
-}
-
-// XOR of v over the workgroup, stored (not accumulated) at dst: for kernels
-// that run as a single workgroup.  Whole waves only.
-template <int K>
-__device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst) {
-    __shared__ uint32_t red[1024 / 64][4 * K];
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint32_t x = v[k].w[i];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
-            v[k].w[i] = x;
-        }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
-    }
-    __syncthreads();
-    const int waves = (int)(blockDim.x >> 6);
-    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
-        uint32_t x = 0;
-        for (int w = 0; w < waves; ++w) x ^= red[w][j];
-        dst[j] = x;
-    }
 }
 
 }  // namespace

@@ -66,12 +66,14 @@ struct PrologueArgs {
     const f128* L;
     const f128* H;
     int ell, m;
-    unsigned eq_blocks, eq_blocks_l;  // factor check: 1024 entries per workgroup
+    unsigned eq_blocks, eq_blocks_l;  // factor check (k_eq_check layout): 1024 entries per workgroup
     uint32_t* eq_out;
     unsigned h4_blocks;  // 4-bit H(i) tables of the multi-iteration rounds
     HtabDesc hdesc;
     uint4* htabs4;
-    const f128* chal;
+    uint32_t* zero;  // round-sum slots, cleared by workgroup 0
+    unsigned zero_words;
+    f128 chal[kMaxRounds];  // the challenges, by value
     unsigned t4_blocks;  // 4-bit tables of chal[0 ..)
     uint4* rtabs4;
     unsigned t8_blocks;  // 8-bit tables of chal[0 ..)
