@@ -656,7 +656,7 @@ static f128 f128_at(const uint32_t* w) {
 // Rounds whose tables hold at most 2^kTailLog elements run in the
 // single-workgroup tail kernel (pipelined prover only).
 constexpr int kTailLog = 10;
-static_assert(kTailLog <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
+static_assert(kTailLog + 1 <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
 
 // Plans the eq-factored mode (round geometry, H(i) tables) and allocates
 // its buffers.  Returns false (general kernels) when the factors are too
@@ -726,7 +726,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(3 * sizeof(uint4) << bitz::kTailMaxLog));
+                                  (int)(sizeof(uint4) * ((3 << (bitz::kTailMaxLog - 1)) + bitz::kLaneTables256)));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_chunk),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(uint4) * ((3 << (bitz::kChunkMaxLog - 1)) + bitz::kLaneTables256)));
         attr_set = true;
     }
     return true;
@@ -1369,9 +1372,9 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         ca.q1mask = q1mask;
         ca.acc = h->acc.u32();
         const unsigned chunks = (unsigned)(h->len >> cs);
-        const size_t smem = 3 * sizeof(uint4) << (cs - 1);
+        const size_t smem = sizeof(uint4) * (((size_t)3 << (cs - 1)) + bitz::kLaneTables256);
         timers[ps.kc].begin(s, "outer_chunk", 0.0);
-        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(256), smem, s, ca);
+        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(bitz::kChunkBlock), smem, s, ca);
         timers[ps.kc].end(s);
         TRY(check_launch());
         advance(out, h->len >> steps);
@@ -1391,9 +1394,9 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     ta.q1mask = q1mask;
     ta.acc = h->acc.u32();
     ta.fin = reinterpret_cast<f128*>(h->acc.u32() + h->fin_off());
-    const size_t smem = 3 * sizeof(uint4) << (n - ps.k0);
+    const size_t smem = sizeof(uint4) * (((size_t)3 << (n - ps.k0)) + bitz::kLaneTables256);
     timers[n].begin(s, "outer_tail", 0.0);
-    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(1024), smem, s, ta);
+    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(bitz::kTailBlock), smem, s, ta);
     timers[n].end(s);
     TRY(check_launch());
     h->round = n;

@@ -771,20 +771,21 @@ template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
 
 // Eq weight of pair y in round k, read from the factors (the variables
 // above k: eq_low's part times eq_high's part).
-__device__ __forceinline__ g4 tail_weight(const f128* L, int ell, const f128* H, int k, size_t y) {
+__device__ __forceinline__ g4 tail_weight(const f128* L, int ell, const f128* H, int k, size_t y, uint4* col) {
     const int lo_bits = ell > k + 1 ? ell - (k + 1) : 0;
     const int hshift = k + 1 > ell ? k + 1 - ell : 0;
     const size_t ylo = y & (((size_t)1 << lo_bits) - 1), yhi = y >> lo_bits;
     g4 w = g_load(H + (yhi << hshift));
-    if (lo_bits) w = g_mul(w, g_load(L + (ylo << (k + 1))));
+    if (lo_bits) w = g_mul_lane(w, g_load(L + (ylo << (k + 1))), col);
     return w;
 }
 
 // Round k's sums over the pairs held in LDS (three tables at stride 2^cap_lg,
-// 2^pl pairs; global pair index = base + local), XORed into acc_k.
+// 2^pl pairs; global pair index = base + local), XORed into acc_k.  Products
+// through this lane's LDS table col.
 __device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int pl, size_t base, int k,
                                                bool q1, const f128* L, int ell, const f128* H,
-                                               uint32_t* acc_k, bool single_block) {
+                                               uint32_t* acc_k, bool single_block, uint4* col) {
     const size_t P = (size_t)1 << pl;
     const size_t items = (q1 ? 3 : 2) * P;
     auto ld = [&](int t, size_t j) {
@@ -801,12 +802,12 @@ __device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int
         const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
         g4 p;
         if (which == 0)
-            p = g_add(g_mul(a0, b0), ld(2, 2 * y));
+            p = g_add(g_mul_lane(a0, b0, col), ld(2, 2 * y));
         else if (which == 1)
-            p = g_mul(g_add(a0, a1), g_add(b0, b1));
+            p = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
         else
-            p = g_add(g_mul(a1, b1), ld(2, 2 * y + 1));
-        const g4 wp = g_mul(tail_weight(L, ell, H, k, base + y), p);
+            p = g_add(g_mul_lane(a1, b1, col), ld(2, 2 * y + 1));
+        const g4 wp = g_mul_lane(tail_weight(L, ell, H, k, base + y, col), p, col);
         if (which == 0) s[0] = g_add(s[0], wp);
         else if (which == 1) s[1] = g_add(s[1], wp);
         else s[2] = g_add(s[2], wp);
@@ -875,12 +876,16 @@ __device__ __forceinline__ void lds_load_folded(uint4* lds, int lg, const f128* 
 // fold by r_{kc-1}), folds them in LDS round after round and adds its pairs'
 // share of every round's sums into the round's slot; it writes its 2^(cs-s)
 // entries of the tables of round kc+s-1 (before the fold by r_{kc+s-1}).
+// Dynamic LDS: the three tables (3 x 2^(cs-1) entries), then 16 KiB of lane
+// tables per wave.
 __global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
     __shared__ uint4 rt[kConstTable4Entries];
     extern __shared__ uint4 lds_raw[];
     const size_t g = blockIdx.x;
     int lg = A.cs - 1;
     const int cap_lg = lg;
+    uint4* col = lds_raw + ((size_t)3 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
     copy_table4(rt, A.rtabs4 + (size_t)(A.kc - 1) * kConstTable4Entries);
     __syncthreads();
     lds_load_folded(lds_raw, lg, A.a, A.b, A.c, g << A.cs, rt);
@@ -889,7 +894,7 @@ __global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
         const int k = A.kc + l;
         const int pl = lg - 1;
         lds_round_sums(lds_raw, cap_lg, pl, g << pl, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k,
-                       false);
+                       false, col);
         if (l + 1 < A.s) {
             __syncthreads();  // every reader of rt is done
             copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
@@ -913,8 +918,9 @@ __global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
 // after the first fold) stay in LDS, and every round is a fold by r_{k-1}
 // followed by the round sums, separated by barriers instead of kernel
 // launches.  The round sums are stored at acc + 12 k and the final A(r),
-// B(r), C(r) at fin[0..2].
-__global__ __launch_bounds__(1024) void k_outer_tail(TailArgs A) {
+// B(r), C(r) at fin[0..2].  256 threads; dynamic LDS: the three tables, then
+// 16 KiB of lane tables per wave.
+__global__ __launch_bounds__(256) void k_outer_tail(TailArgs A) {
     __shared__ uint4 rt[kConstTable4Entries];
     extern __shared__ uint4 lds_raw[];
     int lg = A.len_in_log;
@@ -933,9 +939,12 @@ __global__ __launch_bounds__(1024) void k_outer_tail(TailArgs A) {
     }
     __syncthreads();
     const int cap_lg = lg;
+    uint4* col = lds_raw + ((size_t)3 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
     for (int k = A.k0; k < A.n; ++k) {
         const int pl = lg - 1;
-        lds_round_sums(lds_raw, cap_lg, pl, 0, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k, true);
+        lds_round_sums(lds_raw, cap_lg, pl, 0, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k, true,
+                       col);
         __syncthreads();
         copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
         __syncthreads();

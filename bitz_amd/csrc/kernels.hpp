@@ -98,11 +98,12 @@ __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t t
 // Round 0 with lane tables: 256 threads, 4 waves x 16 KiB + one 4-bit table.
 constexpr int kR0Block = 256;
 constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
-constexpr int kThreadBits = 17;  // threads of a multi-iteration round: one wave of workgroups
+constexpr int kThreadBits = 17;  // threads of a multi-iteration round: one wave of lane-table workgroups
 
 // Single-workgroup tail of an eq-factored proof (see k_outer_tail).
-constexpr int kTailMaxLog = 11;         // at most 2^11 elements per table in LDS
-constexpr int kTailFoldPerThread = 3;   // 3 * 2^10 fold outputs over 1024 threads
+constexpr int kTailMaxLog = 11;        // at most 2^11 input elements per table (2^10 in LDS)
+constexpr int kTailBlock = 256;
+constexpr int kTailFoldPerThread = 6;  // 3 * 2^9 fold outputs over 256 threads
 struct TailArgs {
     const f128* a;
     const f128* b;
@@ -119,8 +120,11 @@ __global__ void k_outer_tail(TailArgs args);
 
 // Chunked middle rounds (see k_outer_chunk): 256-thread workgroups, chunks
 // of 2^cs inputs with cs <= kChunkMaxLog.
-constexpr int kChunkMaxLog = 10;
-constexpr int kChunkFoldPerThread = 3;  // 3 * 2^(kChunkMaxLog - 2) fold outputs over 256 threads
+constexpr int kChunkMaxLog = 11;
+constexpr int kChunkBlock = 256;
+constexpr int kChunkFoldPerThread = 6;  // 3 * 2^(kChunkMaxLog - 2) fold outputs over 256 threads
+// Lane tables of a 256-thread workgroup (4 waves x 64 lanes x 16 entries).
+constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
 struct ChunkArgs {
     const f128* a;
     const f128* b;
