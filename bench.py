@@ -27,6 +27,16 @@ METRIC = ("SHA-256 blocks proved/sec (sumcheck stage) + p50/p99 proof latency, 1
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(n: int):
+    """HBM bytes per fold launch from the calibrated rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded under profiles/ (2^24 workload only)."""
+    path = os.path.join(ROOT, "profiles", "outer_fold_n24_traffic.json")
+    if n != 24 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_launch"]
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -70,6 +80,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device twice (the known-size reference for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -79,6 +92,15 @@ def main() -> int:
         return 2
 
     import bitz_amd as G
+
+    if args.copy_calibration:
+        G.init(0)
+        src, dst = G.DeviceTable(length=1 << 24), G.DeviceTable(length=1 << 24)
+        for _ in range(2):
+            dst.copy_from(src)
+        G.synchronize()
+        log("bench: copied 2 x 256 MiB device to device")
+        return 0
 
     n = args.log_n
     G.init(0)
@@ -168,12 +190,12 @@ def main() -> int:
         "proof_hbm_gbs": total_algo * args.steps / wall / 1e9,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_outer_eq<FOLD=true> (grid-wide fold rounds)",
+            "kernel": "k_outer_fold (grid-wide fold rounds 1-5)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(n),
             "launches": fold["launches"],
             "kernel_ms_per_step": fold["ms"] / args.steps,
         },

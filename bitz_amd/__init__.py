@@ -82,6 +82,7 @@ def lib():
             "bitz_device_free": (I32, [P]),
             "bitz_memcpy_h2d": (I32, [P, P, SZ]),
             "bitz_memcpy_d2h": (I32, [P, P, SZ]),
+            "bitz_memcpy_d2d": (I32, [P, P, SZ]),
             "bitz_synchronize": (I32, []),
             "bitz_release_cache": (I32, []),
             "bitz_profile_enable": (I32, [I32]),
@@ -209,6 +210,12 @@ class DeviceTable:
 
     def __len__(self):
         return self.len
+
+    def copy_from(self, other: "DeviceTable") -> None:
+        """Device-to-device copy of other's elements into this table."""
+        n = min(self.len, other.len)
+        if n:
+            _check(lib().bitz_memcpy_d2d(C.c_void_p(self.ptr), C.c_void_p(other.ptr), n * 16))
 
     def free(self):
         if self.ptr:

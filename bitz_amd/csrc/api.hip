@@ -361,6 +361,10 @@ extern "C" int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return BITZ_OK;
 }
+extern "C" int32_t bitz_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return BITZ_OK;
+}
 extern "C" int32_t bitz_release_cache(void) {
     g_cache.release_all();
     return BITZ_OK;

@@ -59,6 +59,9 @@ int32_t bitz_device_alloc(void** ptr, size_t bytes);
 int32_t bitz_device_free(void* ptr);
 int32_t bitz_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int32_t bitz_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy (a plain streaming copy kernel; also the known-size
+ * reference for HBM traffic counters). */
+int32_t bitz_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int32_t bitz_synchronize(void);
 /* Device scratch is recycled across calls through a size-keyed cache; this
  * returns every cached block to the runtime. */
