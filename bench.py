@@ -80,6 +80,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["outer", "batch"], default="outer",
+                    help="outer: one 2^log_n outer sumcheck per step (default); batch: 256 outer sumchecks "
+                         "of 2^12..2^22 elements per step, batched")
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device twice (the known-size reference for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -101,6 +105,9 @@ def main() -> int:
         G.synchronize()
         log("bench: copied 2 x 256 MiB device to device")
         return 0
+
+    if args.workload == "batch":
+        return bench_batch(G, args)
 
     n = args.log_n
     G.init(0)
@@ -203,6 +210,82 @@ def main() -> int:
     if not args.no_cpu:
         log(f"bench: CPU oracle sample at 2^{args.cpu_sample_log_n}")
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log_n, n)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_batch(G, args) -> int:
+    """256 independent outer sumchecks, num_vars uniform in [12, 22] from the
+    seed, each with A, B, C, tau and challenges as the single workload.  A
+    step proves the whole batch in one bitz_prove_outer_batch call; the same
+    instances proved one call each give the plain single-instance rate."""
+    G.init(0)
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(x) for x in rng.integers(12, 23, size=args.batch)]
+    pool_log = max(sizes) + 1
+    log(f"bench: batch of {args.batch} outer sumchecks, n in [12, 22], total {sum(1 << n for n in sizes)} "
+        f"elements per table")
+    pools = [G.DeviceTable(rng.integers(0, 2**64, size=(1 << pool_log, 2), dtype=np.uint64)) for _ in range(3)]
+
+    class View(G.DeviceTable):  # a window of a device pool (owns nothing)
+        def __init__(self, pool, off, n):
+            self.ptr, self.len, self.pool = pool.ptr + 16 * off, n, pool
+
+        def free(self):
+            pass
+
+    insts = []
+    for i, n in enumerate(sizes):
+        off = int(rng.integers(0, (1 << pool_log) - (1 << n) + 1))
+        prods = G.R1csProductMles(*(View(p, off, 1 << n) for p in pools))
+        tau = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
+        lo, hi = G.make_equality_factors(tau)
+        eq = (G.DeviceTable(lo), G.DeviceTable(hi))
+        claim = G.outer_claim(eq, prods)
+        ch = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
+        insts.append((ch, claim, eq, prods))
+    folded = sum(3 * ((1 << n) - 1) for n in sizes)
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        G.prove_outer_batch(insts)
+        log(f"bench: warmup {i + 1}: {(time.perf_counter() - t0) * 1e3:.2f} ms")
+    G.synchronize()
+    times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        G.prove_outer_batch(insts)
+        times.append(time.perf_counter() - t0)
+    G.synchronize()
+    wall = time.perf_counter() - t_start
+    # The plain path: the same instances, one pipelined proof per call.
+    t0 = time.perf_counter()
+    for ch, claim, eq, prods in insts:
+        G.prove_outer_sumcheck_with_challenges(ch, claim, eq, prods)
+    G.synchronize()
+    plain = time.perf_counter() - t0
+    line = {
+        "metric": "GF(2^128) elements folded per second, batched outer sumcheck",
+        "value": folded * args.steps / wall,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (uniform random tables, points and challenges from the seed)",
+        "config": {"workload": f"{args.batch} outer sumchecks, num_vars uniform in [12, 22] (seed {args.seed}), "
+                               "challenges fixed in advance, one batched call per step",
+                   "instances": args.batch, "elements_per_table": sum(1 << n for n in sizes)},
+        "plain_single_instance": {"ms_for_all_instances": plain * 1e3,
+                                  "elements_folded_per_s": folded / plain},
+        "speedup_vs_plain": (folded * args.steps / wall) / (folded / plain),
+    }
     print(json.dumps(line), flush=True)
     return 0
 

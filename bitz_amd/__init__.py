@@ -87,6 +87,7 @@ def lib():
             "bitz_release_cache": (I32, []),
             "bitz_profile_enable": (I32, [I32]),
             "bitz_profile_only": (I32, [C.c_char_p]),
+            "bitz_prove_outer_batch": (I32, [P, U32, U32]),
             "bitz_profile_read": (I32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_double)]),
             "bitz_transcript_callbacks": (P, [P, P, P]),
@@ -456,6 +457,52 @@ def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, 
     _check(rc, SumcheckError)
     return OuterSumcheckOutput(rounds[: 4 * n].reshape(n, 4, 2), points[:n], to_int(final),
                                to_int(claims[0]), to_int(claims[1]), to_int(claims[2]))
+
+
+class _OuterInstance(C.Structure):
+    """bitz_outer_instance (include/bitz_hip.h)."""
+    _fields_ = [("challenges", C.c_void_p), ("initial_claim", F128),
+                ("eq_low", C.c_void_p), ("low_vars", C.c_uint32),
+                ("eq_high", C.c_void_p), ("high_vars", C.c_uint32),
+                ("az", C.c_void_p), ("bz", C.c_void_p), ("cz", C.c_void_p), ("num_vars", C.c_uint32),
+                ("round_polys", C.c_void_p), ("eval_points", C.c_void_p),
+                ("final_claim", C.c_void_p), ("claims", C.c_void_p)]
+
+
+def prove_outer_batch(instances) -> list:
+    """Many independent outer sumchecks with challenges fixed in advance, in
+    one batched call (bitz_prove_outer_batch).  instances: iterable of
+    (challenges, initial_claim, (eq_low, eq_high), R1csProductMles) with
+    DeviceTables for the products and factors.  Returns one
+    OuterSumcheckOutput per instance, equal to
+    prove_outer_sumcheck_with_challenges on the same inputs."""
+    init()
+    instances = list(instances)
+    arr = (_OuterInstance * max(len(instances), 1))()
+    keep, outs = [], []
+    for i, (ch, claim, eq, prods) in enumerate(instances):
+        (az, na, fa), (bz, nb, _), (cz, nc, _) = (_table(t) for t in (prods.az, prods.bz, prods.cz))
+        if not fa & DEVICE:
+            raise BitzError(-7, "batched proofs take DeviceTables")
+        lo, hi = (_factor(f, fa) for f in eq)
+        n = _nvars(na)
+        chs = elems(ch) if n else np.zeros((1, 2), np.uint64)
+        rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
+        points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+        final = np.zeros((1, 2), dtype=np.uint64)
+        claims = np.zeros((3, 2), dtype=np.uint64)
+        x = arr[i]
+        x.challenges, x.initial_claim = _p(chs).value, f128(claim)
+        x.eq_low, x.low_vars = _p(lo).value, _nvars(len(lo))
+        x.eq_high, x.high_vars = _p(hi).value, _nvars(len(hi))
+        x.az, x.bz, x.cz, x.num_vars = _p(az).value, _p(bz).value, _p(cz).value, n
+        x.round_polys, x.eval_points = _p(rounds).value, _p(points).value
+        x.final_claim, x.claims = _p(final).value, _p(claims).value
+        keep.append((chs, lo, hi, rounds, points, final, claims, az, bz, cz))
+        outs.append((n, rounds, points, final, claims))
+    _check(lib().bitz_prove_outer_batch(C.cast(arr, C.c_void_p), len(instances), DEVICE), SumcheckError)
+    return [OuterSumcheckOutput(r[: 4 * n].reshape(n, 4, 2), p[:n], to_int(f[0]), to_int(c[0]), to_int(c[1]),
+                                to_int(c[2])) for n, r, p, f, c in outs]
 
 
 def outer_claim(eq_factors, products: R1csProductMles) -> int:

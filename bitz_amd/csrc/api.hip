@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <cstring>
 #include <string>
@@ -569,6 +570,8 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
+    std::vector<f128> scale_batch;            // batched rounds (when set, used for every round)
+    std::vector<f128> kpre[2], ksuf[2];       // prefix / suffix products of (1 + tau_i) per factor
     Dev htabs4, rtabs, rtabs4;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
@@ -608,7 +611,12 @@ struct bitz_outer {
         tm.collect();
         return BITZ_OK;
     }
-    const uint32_t* host_words() const { return static_cast<const uint32_t*>(host.p); }
+    // A batch member's sums live in the batch's shared buffers.
+    bool batch_member = false;
+    const uint32_t* host_ext = nullptr;
+    const uint32_t* host_words() const {
+        return host_ext ? host_ext : static_cast<const uint32_t*>(host.p);
+    }
     // Back to the round-0 state (for a second pipelined attempt).
     int32_t rewind() {
         for (int t = 0; t < 3; ++t) cur[t] = inputs[t];
@@ -667,7 +675,7 @@ static_assert(kTailLog + 1 <= bitz::kTailMaxLog, "tail tables must fit the kerne
 // large to check cheaply.  The check itself is launched by
 // outer_launch_checks or, in the pipelined prover, by its prologue; nothing
 // reads back here: the verdict arrives with the first round sums.
-static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
+static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batch_member = false) {
     const int n = h->n;
     if (n == 0) return false;
     const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
@@ -710,9 +718,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars) {
     desc.first[desc.rounds] = (uint32_t)htab_total;
     h->hdesc = desc;
     h->htab_total = htab_total;
-    if (h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
-        h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4)) ||
-        h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)))
+    if (h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4))) return false;
+    if (!batch_member &&
+        (h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
+         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4))))
         return false;
     static bool attr_set = false;
     if (!attr_set) {
@@ -791,7 +800,8 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     // Per factor t with variables [lo, hi): pre[t][j] = prod_{lo <= i < lo + j}
     // (1 + tau_i), suf[t][j] = prod_{lo + j <= i < hi}.  For a run [a, b)
     // of its variables, c = pre[t][a - lo] * suf[t][b - lo].
-    std::vector<f128> pre[2], suf[2];
+    std::vector<f128>(&pre)[2] = h->kpre;
+    std::vector<f128>(&suf)[2] = h->ksuf;
     for (int t = 0; t < 2; ++t) {
         const int lo = t == 0 ? 0 : ell, hi = t == 0 ? ell : n, w = hi - lo;
         pre[t].assign(w + 1, hostf::one());
@@ -1418,7 +1428,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
 // are (a recovery would divide by tau_k = 0, or the check fails).
 static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 initial_claim,
                               uint64_t q1mask, bool check_claim, std::vector<bitz_f128>& co) {
-    const int n = h->n, kc = pipe_schedule(n).kc;
+    const int n = h->n, kc = h->batch_member ? 0 : pipe_schedule(n).kc;
     const uint32_t* w = h->host_words();
     for (int k = 0; k < n; ++k)
         if (!((q1mask >> k) & 1) && hostf::is_zero(h->tau[k])) return false;
@@ -1428,7 +1438,7 @@ static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 i
     for (int k = 0; k < n; ++k) {
         bitz_f128 raw[3];
         memcpy(raw, w + 12 * k, 48);
-        const f128& sc = k >= kc ? h->scale_tail[k] : h->scale_big[k];
+        const f128& sc = !h->scale_batch.empty() ? h->scale_batch[k] : k >= kc ? h->scale_tail[k] : h->scale_big[k];
         outer_eq_coeffs(h, k, raw, (q1mask >> k) & 1, sc, &co[3 * (size_t)k]);
         outer_eq_advance(h, k, F(challenges[k]));
     }
@@ -1582,6 +1592,356 @@ extern "C" int32_t bitz_prove_outer_sumcheck_with_challenges(
     *final_claim = B(claim);
 done:
     bitz_outer_free(h);
+    return rc;
+}
+
+// ============================================ batched outer sumchecks ===
+// Many independent eq-factored outer sumchecks with challenges fixed in
+// advance, proved together: one prologue launch for every instance, then
+// stages of k_outer_batch in which every (instance, chunk) is a workgroup
+// (up to nine rounds of every instance per launch), then one read-back.
+// Each instance's result is exactly bitz_prove_outer_sumcheck_with_challenges';
+// an instance whose optimistic pass is rejected (factors that are not eq
+// tables, tau_k = 0, a wrong claim) is proved again on its own.
+
+namespace {
+
+// Host staging that outlives a call: pinned, grown on demand, one batch at a
+// time.
+struct BigPinned {
+    std::mutex mu;
+    void* p = nullptr;
+    size_t bytes = 0;
+    int32_t reserve(size_t n) {
+        if (n <= bytes) return BITZ_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+        }
+        bytes = n;
+        return BITZ_OK;
+    }
+} g_batch_pinned;
+
+struct BatchStage {
+    int k_first, s, cs_in, fold_first, final_stage;
+    int len_log_in, len_log_out;  // log2 table length before / after the stage
+};
+
+// Stages of one instance of n rounds: chunks of at most 2^kBatchMaxLog
+// entries per table in LDS, nine rounds per non-final stage.
+std::vector<BatchStage> plan_batch(int n) {
+    std::vector<BatchStage> st;
+    int k = 0, len_log = n, fold_first = 0;
+    while (k < n) {
+        BatchStage b{};
+        b.k_first = k;
+        b.fold_first = fold_first;
+        b.len_log_in = len_log;
+        const int cap = len_log - fold_first;
+        if (cap <= bitz::kBatchMaxLog) {
+            b.cs_in = len_log;
+            b.s = n - k;
+            b.final_stage = 1;
+            b.len_log_out = 0;
+            st.push_back(b);
+            break;
+        }
+        b.cs_in = bitz::kBatchMaxLog + fold_first;
+        b.s = bitz::kBatchMaxLog - 1;
+        b.final_stage = 0;
+        b.len_log_out = (len_log - b.cs_in) + (bitz::kBatchMaxLog - b.s + 1);
+        st.push_back(b);
+        k += b.s;
+        len_log = b.len_log_out;
+        fold_first = 1;
+    }
+    return st;
+}
+
+// A batch member: the eq-factored state of one instance (geometry, tau and
+// constants after validation, coefficient recovery) without scratch of its
+// own.
+int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in) {
+    auto* h = new bitz_outer;
+    h->batch_member = true;
+    h->n = (int)in.num_vars;
+    h->N = h->len = (size_t)1 << in.num_vars;
+    h->low_len = (size_t)1 << in.low_vars;
+    h->high_len = (size_t)1 << in.high_vars;
+    h->inputs[0] = reinterpret_cast<const f128*>(in.az);
+    h->inputs[1] = reinterpret_cast<const f128*>(in.bz);
+    h->inputs[2] = reinterpret_cast<const f128*>(in.cz);
+    h->eq_in[0] = reinterpret_cast<const f128*>(in.eq_low);
+    h->eq_in[1] = reinterpret_cast<const f128*>(in.eq_high);
+    h->eq_bl = eq_check_blocks(h->low_len, 1024);
+    h->eq_bh = eq_check_blocks(h->high_len, 1024);
+    h->eqmode = outer_setup_eq(h, (int)in.low_vars, (int)in.high_vars, true);
+    *out = h;
+    return BITZ_OK;
+}
+
+// Constant carried by run_weight's product over the variables [a, b)
+// (kernels.hip): eq_low's part times eq_high's part, each c_R =
+// prod over that factor's other variables of (1 + tau_i).
+f128 run_constant(const bitz_outer* h, int a, int b) {
+    const int ell = h->ell, n = h->n;
+    auto kap = [&](int t, int x, int y) {
+        const int lo = t == 0 ? 0 : ell;
+        return hostf::mul(h->kpre[t][x - lo], h->ksuf[t][y - lo]);
+    };
+    if (b <= ell) return kap(0, a, b);
+    if (a >= ell) return kap(1, a, b);
+    (void)n;
+    return hostf::mul(kap(0, a, ell), kap(1, ell, b));
+}
+
+// Round k of a batched stage weighs pair y = g 2^pl + yl by
+// run_weight(k+1, k+1+pl, yl) run_weight(k+1+pl, n, g).
+void batch_scales(bitz_outer* h, const std::vector<BatchStage>& plan) {
+    std::vector<f128> kap(h->n);
+    for (const auto& st : plan)
+        for (int l = 0; l < st.s; ++l) {
+            const int k = st.k_first + l;
+            const int pl = (st.cs_in - st.fold_first) - 1 - l;
+            kap[k] = hostf::mul(run_constant(h, k + 1, k + 1 + pl), run_constant(h, k + 1 + pl, h->n));
+        }
+    batch_inverse(kap);
+    h->scale_batch = kap;
+}
+
+}  // namespace
+
+extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint32_t count, uint32_t flags) {
+    if (!inst && count) return fail(BITZ_ERR_ARG, "null pointer");
+    if (!(flags & BITZ_DEVICE)) return fail(BITZ_ERR_ARG, "batched proofs take device tables (BITZ_DEVICE)");
+    for (uint32_t i = 0; i < count; ++i) {
+        const auto& x = inst[i];
+        if (!x.final_claim || !x.claims || (x.num_vars && (!x.round_polys || !x.eval_points || !x.challenges)) ||
+            !x.eq_low || !x.eq_high || !x.az || !x.bz || !x.cz)
+            return fail(BITZ_ERR_ARG, "instance " + std::to_string(i) + ": null pointer");
+        const int32_t rc = outer_check(x.low_vars, x.high_vars, x.num_vars, x.num_vars, x.num_vars);
+        if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
+    }
+    if (!count) return BITZ_OK;
+    // Members; instances the batched kernels cannot take go alone.
+    std::vector<std::unique_ptr<bitz_outer, void (*)(bitz_outer*)>> hs;
+    std::vector<uint32_t> members, alone;
+    for (uint32_t i = 0; i < count; ++i) {
+        bitz_outer* h = nullptr;
+        TRY(batch_member(&h, inst[i]));
+        hs.emplace_back(h, [](bitz_outer* p) { delete p; });
+        if (h->eqmode && h->n >= 1 && h->n <= bitz::kMaxRounds)
+            members.push_back(i);
+        else
+            alone.push_back(i);
+    }
+    int32_t rc = BITZ_OK;
+    if (!members.empty()) {
+        StreamOwner so;
+        TRY(so.create());
+        hipStream_t s = so.s;
+        // Layout of the shared sums buffer and of the host staging.
+        std::vector<size_t> acc_off(count, 0), ch_off(count, 0);
+        size_t acc_words = 0, ch_total = 0;
+        std::vector<std::vector<BatchStage>> plans(count);
+        size_t n_stages = 0;
+        for (uint32_t i : members) {
+            bitz_outer* h = hs[i].get();
+            acc_off[i] = acc_words;
+            acc_words += (h->acc_words() + 63) & ~(size_t)63;
+            ch_off[i] = ch_total;
+            ch_total += (size_t)h->n;
+            plans[i] = plan_batch(h->n);
+            n_stages = std::max(n_stages, plans[i].size());
+        }
+        // Stage scratch: the largest stage output, ping-ponged.
+        std::vector<size_t> out_len(n_stages, 0);  // elements per table summed over instances
+        std::vector<std::vector<size_t>> out_off(count);
+        for (uint32_t i : members) {
+            out_off[i].assign(plans[i].size(), 0);
+            for (size_t j = 0; j < plans[i].size(); ++j)
+                if (!plans[i][j].final_stage) {
+                    out_off[i][j] = out_len[j];
+                    out_len[j] += (size_t)1 << plans[i][j].len_log_out;
+                }
+        }
+        size_t scratch = 1;
+        for (size_t v : out_len) scratch = std::max(scratch, v);
+        Dev acc, chal, buf[2], desc;
+        TRY(acc.alloc(acc_words * 4));
+        TRY(chal.alloc(std::max<size_t>(ch_total, 1) * sizeof(f128)));
+        TRY(buf[0].alloc(3 * scratch * sizeof(f128)));
+        TRY(buf[1].alloc(3 * scratch * sizeof(f128)));
+        // Descriptors: prologue items, then every stage's instances.
+        std::vector<bitz::BatchPrologue> pro;
+        uint32_t pro_blocks = 0;
+        for (uint32_t i : members) {
+            bitz_outer* h = hs[i].get();
+            bitz::BatchPrologue p{};
+            p.L = h->eq_in[0];
+            p.H = h->eq_in[1];
+            p.ell = h->ell;
+            p.m = h->m;
+            p.n = h->n;
+            p.zero = acc.u32() + acc_off[i];
+            p.zero_words = (unsigned)h->eq_off();
+            p.eq_blocks_l = h->eq_bl;
+            p.eq_blocks = h->eq_bl + h->eq_bh;
+            p.eq_out = acc.u32() + acc_off[i] + h->eq_off();
+            p.chal = chal.f() + ch_off[i];
+            p.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
+            p.block_base = pro_blocks;
+            pro_blocks += p.eq_blocks + (unsigned)h->n;
+            pro.push_back(p);
+        }
+        std::vector<std::vector<bitz::BatchInst>> stage_items(n_stages);
+        std::vector<uint32_t> stage_chunks(n_stages, 0);
+        for (size_t j = 0; j < n_stages; ++j) {
+            for (uint32_t i : members) {
+                if (j >= plans[i].size()) continue;
+                bitz_outer* h = hs[i].get();
+                const BatchStage& st = plans[i][j];
+                bitz::BatchInst b{};
+                const f128* src = j == 0 ? nullptr : buf[(j - 1) & 1].f();
+                const size_t src_stride = j == 0 ? 0 : scratch;
+                if (j == 0) {
+                    b.a = h->inputs[0];
+                    b.b = h->inputs[1];
+                    b.c = h->inputs[2];
+                } else {
+                    const size_t o = out_off[i][j - 1];
+                    b.a = src + o;
+                    b.b = src + src_stride + o;
+                    b.c = src + 2 * src_stride + o;
+                }
+                if (!st.final_stage) {
+                    f128* dst = buf[j & 1].f();
+                    const size_t o = out_off[i][j];
+                    b.ao = dst + o;
+                    b.bo = dst + scratch + o;
+                    b.co = dst + 2 * scratch + o;
+                }
+                b.L = h->eq_in[0];
+                b.H = h->eq_in[1];
+                b.rtabs4 = static_cast<const uint4*>(h->rtabs4.p);
+                b.acc = acc.u32() + acc_off[i];
+                b.fin = reinterpret_cast<f128*>(acc.u32() + acc_off[i] + h->fin_off());
+                b.ell = h->ell;
+                b.n = h->n;
+                b.k_first = st.k_first;
+                b.s = st.s;
+                b.cs_in = st.cs_in;
+                b.fold_first = st.fold_first;
+                b.final_stage = st.final_stage;
+                b.chunk_base = stage_chunks[j];
+                stage_chunks[j] += (uint32_t)1 << (st.len_log_in - st.cs_in);
+                stage_items[j].push_back(b);
+            }
+        }
+        // Round 0 skips q(1) unless a challenge is zero (the claim check at
+        // the end catches a wrong claim).
+        std::vector<uint64_t> q1mask(count, 0);
+        for (uint32_t i : members) {
+            for (uint32_t k = 0; k < inst[i].num_vars; ++k)
+                if (hostf::is_zero(F(inst[i].challenges[k]))) q1mask[i] = 1;
+        }
+        {
+            // Stage items were appended in member order: the same order here.
+            std::vector<size_t> pos(n_stages, 0);
+            for (uint32_t i : members)
+                for (size_t j = 0; j < plans[i].size(); ++j) stage_items[j][pos[j]++].q1mask = q1mask[i];
+        }
+        // Staging: descriptors and challenges up, sums down.
+        std::vector<size_t> stage_desc_off(n_stages);
+        size_t desc_bytes = pro.size() * sizeof(bitz::BatchPrologue);
+        for (size_t j = 0; j < n_stages; ++j) {
+            desc_bytes = (desc_bytes + 255) & ~(size_t)255;
+            stage_desc_off[j] = desc_bytes;
+            desc_bytes += stage_items[j].size() * sizeof(bitz::BatchInst);
+        }
+        const size_t up_bytes = ((desc_bytes + 255) & ~(size_t)255) + ch_total * sizeof(f128);
+        const size_t stage_bytes = up_bytes + acc_words * 4;
+        std::lock_guard<std::mutex> lock(g_batch_pinned.mu);
+        TRY(g_batch_pinned.reserve(stage_bytes));
+        char* pin = static_cast<char*>(g_batch_pinned.p);
+        memcpy(pin, pro.data(), pro.size() * sizeof(bitz::BatchPrologue));
+        for (size_t j = 0; j < n_stages; ++j)
+            memcpy(pin + stage_desc_off[j], stage_items[j].data(), stage_items[j].size() * sizeof(bitz::BatchInst));
+        const size_t ch_bytes_off = (desc_bytes + 255) & ~(size_t)255;
+        for (uint32_t i : members)
+            memcpy(pin + ch_bytes_off + ch_off[i] * sizeof(f128), inst[i].challenges,
+                   inst[i].num_vars * sizeof(f128));
+        TRY(desc.alloc(std::max<size_t>(desc_bytes, 16)));
+        HIP_TRY(hipMemcpyAsync(desc.p, pin, desc_bytes, hipMemcpyHostToDevice, s));
+        if (ch_total)
+            HIP_TRY(hipMemcpyAsync(chal.p, pin + ch_bytes_off, ch_total * sizeof(f128), hipMemcpyHostToDevice, s));
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_batch),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(sizeof(uint4) * ((3 << bitz::kBatchMaxLog) + bitz::kLaneTables256)));
+            attr = true;
+        }
+        Timed tp;
+        tp.begin(s, "outer_batch_prologue", 0.0);
+        hipLaunchKernelGGL(bitz::k_batch_prologue, dim3(pro_blocks), dim3(1024), 0, s,
+                           static_cast<const bitz::BatchPrologue*>(desc.p), (int)pro.size());
+        tp.end(s);
+        TRY(check_launch());
+        std::vector<Timed> ts(n_stages);
+        const size_t smem = sizeof(uint4) * ((3 << bitz::kBatchMaxLog) + bitz::kLaneTables256);
+        for (size_t j = 0; j < n_stages; ++j) {
+            ts[j].begin(s, "outer_batch_stage", 0.0);
+            hipLaunchKernelGGL(bitz::k_outer_batch, dim3(stage_chunks[j]), dim3(256), smem, s,
+                               reinterpret_cast<const bitz::BatchInst*>(static_cast<char*>(desc.p) +
+                                                                        stage_desc_off[j]),
+                               (int)stage_items[j].size());
+            ts[j].end(s);
+            TRY(check_launch());
+        }
+        uint32_t* host_acc = reinterpret_cast<uint32_t*>(pin + up_bytes);
+        HIP_TRY(hipMemcpyAsync(host_acc, acc.p, acc_words * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        tp.collect();
+        for (auto& t : ts) t.collect();
+        // Per instance: validate, recover the coefficients, check the claim.
+        for (uint32_t i : members) {
+            bitz_outer* h = hs[i].get();
+            h->host_ext = host_acc + acc_off[i];
+            const auto& x = inst[i];
+            std::vector<bitz_f128> co;
+            const bool ok = outer_validate_eq(h, h->host_words() + h->eq_off()) &&
+                            (batch_scales(h, plans[i]), true) &&
+                            outer_assemble_eq(h, x.challenges, F(x.initial_claim), q1mask[i], !(q1mask[i] & 1), co);
+            if (!ok) {
+                alone.push_back(i);
+                continue;
+            }
+            f128 claim = F(x.initial_claim);
+            for (int k = 0; k < h->n; ++k) {
+                f128 c[4] = {F(co[3 * k]), hostf::add(claim, hostf::add(F(co[3 * k + 1]), F(co[3 * k + 2]))),
+                             F(co[3 * k + 1]), F(co[3 * k + 2])};
+                const f128 r = F(x.challenges[k]);
+                claim = hostf::horner(c, 4, r);
+                for (int t = 0; t < 4; ++t) x.round_polys[4 * k + t] = B(c[t]);
+                x.eval_points[k] = x.challenges[k];
+            }
+            *x.final_claim = B(claim);
+            memcpy(x.claims, h->host_words() + h->fin_off(), 48);
+        }
+    }
+    for (uint32_t i : alone) {
+        const auto& x = inst[i];
+        rc = bitz_prove_outer_sumcheck_with_challenges(x.challenges, x.initial_claim, x.eq_low, x.low_vars,
+                                                       x.eq_high, x.high_vars, x.az, x.num_vars, x.bz, x.num_vars,
+                                                       x.cz, x.num_vars, flags, x.round_polys, x.eval_points,
+                                                       x.final_claim, x.claims);
+        if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
+    }
     return rc;
 }
 

@@ -954,4 +954,145 @@ __global__ __launch_bounds__(256) void k_outer_tail(TailArgs A) {
     if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
 }
 
+
+// ======================================= batches of independent proofs ===
+__global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* items, int count) {
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block_base <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchPrologue& P = items[lo];
+    const unsigned b = blockIdx.x - P.block_base;
+    if (b == 0)
+        for (unsigned e = threadIdx.x; e < P.zero_words; e += blockDim.x) P.zero[e] = 0;
+    if (b < P.eq_blocks) {
+        eq_check_block(P.L, P.ell, P.H, P.m, P.eq_out, P.eq_blocks_l, b);
+        return;
+    }
+    const unsigned t = b - P.eq_blocks;
+    build_const_table4(tab, basis, g_load(P.chal + t));
+    uint4* dst = P.rtabs4 + (size_t)t * kConstTable4Entries;
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
+}
+
+// Product of the factor entries over the variables [a, b) of the weight,
+// at index idx (bit j of idx is variable a + j): eq_low's part times
+// eq_high's part, which is eq(tau_[a,b), idx) times a constant the host
+// divides out.  The run picks eq_low alone when b <= ell, eq_high alone when
+// a >= ell (an empty run reads that factor's entry 0).
+__device__ __forceinline__ g4 run_weight(const f128* L, int ell, const f128* H, int a, int b, size_t idx,
+                                         uint4* col) {
+    if (b <= ell) return g_load(L + (idx << a));
+    if (a >= ell) return g_load(H + (idx << (a - ell)));
+    const int lo_bits = ell - a;
+    return g_mul_lane(g_load(L + ((idx & (((size_t)1 << lo_bits) - 1)) << a)), g_load(H + (idx >> lo_bits)), col);
+}
+
+// Round k's sums over a chunk's pairs in LDS (2^pl pairs; the chunk is pair
+// block g of the round): sum_yl W_lo(yl) p(yl) per thread, one pair per
+// thread at a time, then times W_hi(g), XORed into acc_k.  W_lo covers the
+// variables [k+1, k+1+pl), W_hi the rest up to n.
+__device__ __forceinline__ void chunk_round_sums(const uint4* lds, int cap_lg, int pl, size_t g, int k, int n,
+                                                 bool q1, const f128* L, int ell, const f128* H,
+                                                 uint32_t* acc_k, uint4* col) {
+    const size_t P = (size_t)1 << pl;
+    const size_t items = (q1 ? 3 : 2) * P;
+    auto ld = [&](int t, size_t j) {
+        const uint4 u = lds[((size_t)t << cap_lg) + j];
+        g4 r;
+        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
+        return r;
+    };
+    g4 s[3] = {g_zero(), g_zero(), g_zero()};
+    for (size_t it = threadIdx.x; it < items; it += blockDim.x) {
+        const size_t y = it & (P - 1);
+        const int which = (int)(it >> pl);
+        const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
+        const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
+        g4 p;
+        if (which == 0)
+            p = g_add(g_mul_lane(a0, b0, col), ld(2, 2 * y));
+        else if (which == 1)
+            p = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
+        else
+            p = g_add(g_mul_lane(a1, b1, col), ld(2, 2 * y + 1));
+        const g4 wp = g_mul_lane(run_weight(L, ell, H, k + 1, k + 1 + pl, y, col), p, col);
+        if (which == 0) s[0] = g_add(s[0], wp);
+        else if (which == 1) s[1] = g_add(s[1], wp);
+        else s[2] = g_add(s[2], wp);
+    }
+    if (threadIdx.x < items) {
+        const g4 wh = run_weight(L, ell, H, k + 1 + pl, n, g, col);
+        for (int j = 0; j < 3; ++j) s[j] = g_mul_lane(wh, s[j], col);
+    }
+    block_xor_to_global<3>(s, acc_k);
+}
+
+// One workgroup per (instance, chunk) of the current stage: the chunked
+// rounds of k_outer_chunk, for many instances of different sizes at once,
+// from round 0 (no fold first) to the last round (final stage).
+__global__ __launch_bounds__(256) void k_outer_batch(const BatchInst* insts, int count) {
+    __shared__ uint4 rt[kConstTable4Entries];
+    extern __shared__ uint4 lds_raw[];
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (insts[mid].chunk_base <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchInst& A = insts[lo];
+    const size_t g = blockIdx.x - A.chunk_base;
+    int lg = A.cs_in - A.fold_first;
+    const int cap_lg = lg;
+    uint4* col = lds_raw + ((size_t)3 << kBatchMaxLog) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
+                 (threadIdx.x & 63);
+    lane_table_clear(col);
+    if (A.fold_first) {
+        copy_table4(rt, A.rtabs4 + (size_t)(A.k_first - 1) * kConstTable4Entries);
+        __syncthreads();
+        lds_load_folded(lds_raw, lg, A.a, A.b, A.c, g << A.cs_in, rt);
+    } else {
+        const size_t len = (size_t)1 << lg;
+        for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+            const int t = (int)(e >> lg);
+            const f128* src = (t == 0 ? A.a : (t == 1 ? A.b : A.c)) + (g << lg);
+            lds_raw[e] = *reinterpret_cast<const uint4*>(src + (e & (len - 1)));
+        }
+    }
+    __syncthreads();
+    for (int l = 0; l < A.s; ++l) {
+        const int k = A.k_first + l;
+        const int pl = lg - 1;
+        chunk_round_sums(lds_raw, cap_lg, pl, g, k, A.n, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k,
+                         col);
+        if (l + 1 < A.s || A.final_stage) {
+            __syncthreads();  // every reader of rt is done
+            copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
+            __syncthreads();
+            lds_fold<kBatchFoldPerThread>(lds_raw, cap_lg, pl, rt);
+            lg = pl;
+        }
+    }
+    __syncthreads();
+    if (A.final_stage) {
+        if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
+        return;
+    }
+    const size_t len = (size_t)1 << lg;
+    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+        const int t = (int)(e >> lg);
+        const size_t j = e & (len - 1);
+        f128* dst = (t == 0 ? A.ao : (t == 1 ? A.bo : A.co)) + (g << lg) + j;
+        *reinterpret_cast<uint4*>(dst) = lds_raw[((size_t)t << cap_lg) + j];
+    }
+}
+
 }  // namespace bitz

@@ -142,4 +142,46 @@ struct ChunkArgs {
 __global__ void k_outer_chunk(ChunkArgs args);
 constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
+// ---- batches of independent eq-factored proofs (k_outer_batch) ----------
+// One stage of one instance: rounds [k_first, k_first + s) over chunks of
+// 2^cs_in inputs (the tables before the fold by r_{k_first - 1} when
+// fold_first), chunk g = work item chunk_base + g.  A final stage folds the
+// last round too and stores A(r), B(r), C(r) at fin; other stages store
+// their 2^(cs_in - fold_first - s) entries per chunk and table at ao/bo/co.
+struct BatchInst {
+    const f128* a;
+    const f128* b;
+    const f128* c;
+    f128* ao;
+    f128* bo;
+    f128* co;
+    const f128* L;
+    const f128* H;
+    const uint4* rtabs4;  // 4-bit table of r_k at rtabs4 + 512 k
+    uint32_t* acc;        // round k's sums at acc + 12 k
+    f128* fin;
+    uint64_t q1mask;
+    int ell, n, k_first, s, cs_in, fold_first, final_stage;
+    uint32_t chunk_base;
+};
+constexpr int kBatchMaxLog = 10;  // at most 2^10 entries per table in LDS
+constexpr int kBatchFoldPerThread = 6;  // 3 * 2^9 fold outputs over 256 threads
+__global__ void k_outer_batch(const BatchInst* insts, int count);
+
+// Prologue of a batch: per instance, clearing of its sums, the factor check
+// (k_eq_check layout at eq_out) and the 4-bit tables of its challenges.
+struct BatchPrologue {
+    const f128* L;
+    const f128* H;
+    int ell, m, n;
+    uint32_t* zero;
+    unsigned zero_words;
+    unsigned eq_blocks_l, eq_blocks;
+    uint32_t* eq_out;
+    const f128* chal;  // device array, n entries
+    uint4* rtabs4;
+    uint32_t block_base;  // first workgroup: eq check, then n table builds
+};
+__global__ void k_batch_prologue(const BatchPrologue* items, int count);
+
 }  // namespace bitz

@@ -149,6 +149,30 @@ int32_t bitz_prove_outer_sumcheck_with_challenges(const bitz_f128* challenges, b
                                                   bitz_f128* round_polys, bitz_f128* eval_points,
                                                   bitz_f128* final_claim, bitz_f128* claims);
 
+/* A batch of independent outer sumchecks with challenges fixed in advance
+ * (no counterpart in the reference, which proves one instance per call:
+ * the batched mode behind many uneven instances on one GPU).  Every
+ * instance's outputs equal bitz_prove_outer_sumcheck_with_challenges' on
+ * the same inputs.  Tables and factors are device pointers (flags must
+ * include BITZ_DEVICE); challenges and outputs are host memory. */
+typedef struct {
+    const bitz_f128* challenges; /* num_vars entries */
+    bitz_f128 initial_claim;
+    const bitz_f128* eq_low;
+    uint32_t low_vars;
+    const bitz_f128* eq_high;
+    uint32_t high_vars;
+    const bitz_f128* az;
+    const bitz_f128* bz;
+    const bitz_f128* cz;
+    uint32_t num_vars;
+    bitz_f128* round_polys; /* num_vars x [c0, c1, c2, c3] */
+    bitz_f128* eval_points; /* num_vars */
+    bitz_f128* final_claim; /* 1 */
+    bitz_f128* claims;      /* [Az, Bz, Cz](r_x) */
+} bitz_outer_instance;
+int32_t bitz_prove_outer_batch(const bitz_outer_instance* instances, uint32_t count, uint32_t flags);
+
 /* Per-round form of the same prover for callers that keep the transcript:
  * begin -> round0 ([c0,c2,c3]) -> fold_next(r_k) per round (returns the next
  * round's [c0,c2,c3], zeros after the last) -> finish ([Az,Bz,Cz](r_x)).

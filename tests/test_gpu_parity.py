@@ -383,3 +383,52 @@ def test_per_round_eq_mode_every_split(split):
     a, b, c, tau, lo, hi = outer_case(n, split, 440 + split)
     assert mode_of(lo, hi, a, b, c)
     outer_with(a, b, c, lo, hi, 11, ints(elements(441, n)))
+
+
+# ---- batched prover -----------------------------------------------------------
+def test_outer_batch_matches_oracle_per_instance():
+    """Uneven instances in one batch, each equal to the oracle on its own
+    challenges: true and wrong claims, a zero challenge, a zero tau, factors
+    that are not eq tables, every split."""
+    cases = []
+    specs = [(1, 0, True), (2, 1, True), (5, 2, False), (10, 5, True), (11, 0, True), (12, 6, True),
+             (13, 13, False), (14, 7, True), (16, 9, True), (20, 10, True), (21, 10, True), (12, 6, "zero_r"),
+             (12, 6, "zero_tau"), (12, 6, "not_eq")]
+    for j, (n, split, kind) in enumerate(specs):
+        a, b, c, tau, lo, hi = outer_case(n, split, 500 + 11 * j)
+        ch = elements(501 + j, n)
+        claim = true_claim(a, b, c, tau)
+        if kind is False:
+            claim ^= 0xABC
+        elif kind == "zero_r":
+            ch[3] = 0
+            claim ^= 7
+        elif kind == "zero_tau":
+            tau[8] = 0
+            lo, hi = O.eq_table(tau[:split]), O.eq_table(tau[split:])
+            claim = true_claim(a, b, c, tau)
+        elif kind == "not_eq":
+            hi = hi.copy()
+            hi[3, 1] ^= 1
+        cases.append((ch, claim, lo, hi, a, b, c))
+    insts = [(ch, claim, (G.DeviceTable(lo), G.DeviceTable(hi)),
+              G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c)))
+             for ch, claim, lo, hi, a, b, c in cases]
+    outs = G.prove_outer_batch(insts)
+    for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
+        assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+
+
+def test_outer_batch_of_many_small_instances():
+    rng = np.random.default_rng(7)
+    cases = []
+    for j in range(48):
+        n = int(rng.integers(1, 15))
+        a, b, c, tau, lo, hi = outer_case(n, n // 2, 900 + 13 * j)
+        cases.append((elements(901 + j, n), true_claim(a, b, c, tau), lo, hi, a, b, c))
+    insts = [(ch, claim, (G.DeviceTable(lo), G.DeviceTable(hi)),
+              G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c)))
+             for ch, claim, lo, hi, a, b, c in cases]
+    outs = G.prove_outer_batch(insts)
+    for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
+        assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
