@@ -739,10 +739,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batc
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(sizeof(uint4) * ((3 << (bitz::kTailMaxLog - 1)) + bitz::kLaneTables256)));
+                                  (int)bitz::stage_smem(bitz::kTailMaxLog - 1));
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_chunk),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(sizeof(uint4) * ((3 << (bitz::kChunkMaxLog - 1)) + bitz::kLaneTables256)));
+                                  (int)bitz::stage_smem(bitz::kChunkMaxLog - 1));
         attr_set = true;
     }
     return true;
@@ -761,6 +761,46 @@ static int32_t outer_launch_checks(bitz_outer* h) {
         hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eq_in[0], h->ell,
                            h->eq_in[1], h->hdesc, static_cast<uint4*>(h->htabs4.p));
     return check_launch();
+}
+
+// Round schedule of the pipelined eq-factored prover: rounds [0, kc) as
+// grid-wide kernels, [kc, k0) in the chunked kernel, [k0, n) in the tail.
+struct PipeSchedule {
+    int kc, k0;
+};
+static PipeSchedule pipe_schedule(int n) {
+    const int k0 = std::max(0, n - kTailLog);
+    const int kc = std::max(1, n - 1 - bitz::kThreadBits);  // rounds before kc iterate over their pairs
+    if (k0 - kc >= 1) return {kc, k0};
+    return {k0, k0};
+}
+
+// Constant carried by run_weight's product over the variables [a, b)
+// (kernels.hip): eq_low's part times eq_high's part, each the product of
+// 1 + tau_i over that factor's variables outside the run.
+static f128 run_constant(const bitz_outer* h, int a, int b) {
+    const int ell = h->ell;
+    auto kap = [&](int t, int x, int y) {
+        const int lo = t == 0 ? 0 : ell;
+        return hostf::mul(h->kpre[t][x - lo], h->ksuf[t][y - lo]);
+    };
+    if (b <= ell) return kap(0, a, b);
+    if (a >= ell) return kap(1, a, b);
+    return hostf::mul(kap(0, a, ell), kap(1, ell, b));
+}
+
+// Constants of the weights of an LDS stage (lds_stage in kernels.hip):
+// round k = k_first + l weighs its pairs by E(y 2^l) W_hi(g), where E runs
+// over [a0, b) = [k_first + 1, k_first + 1 + pl0) and W_hi over [b, n), and
+// the l zero variables of y 2^l contribute prod_{a0 <= i <= k} (1 + tau_i).
+static void stage_constants(const bitz_outer* h, int k_first, int s, int pl0, f128* kap) {
+    const int a0 = k_first + 1, b = a0 + pl0;
+    f128 c = hostf::mul(run_constant(h, a0, b), run_constant(h, b, h->n));
+    for (int l = 0; l < s; ++l) {
+        const int k = k_first + l;
+        if (l > 0) c = hostf::mul(c, hostf::add(hostf::one(), h->tau[k]));
+        kap[k] = c;
+    }
 }
 
 // Reads the device check (eq region of the host copy of acc) and, when the
@@ -826,16 +866,16 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
                 c = hostf::mul(c, kappa(1, g.h_start, n));
         }
         kap[k] = c;
-        // Tail weights: eq_low[y_lo << (k+1)] * eq_high[y_hi << max(0, k+1-ell)].
-        if (k + 1 < ell)
-            kap[n + k] = hostf::mul(kappa(0, k + 1, ell), kappa(1, ell, n));
-        else
-            kap[n + k] = kappa(1, k + 1, n);
+        kap[n + k] = hostf::one();
     }
+    // Rounds of the pipelined prover's LDS stages (chunk and tail).
+    h->tau = tau;
+    const PipeSchedule ps = pipe_schedule(n);
+    if (ps.kc < ps.k0) stage_constants(h, ps.kc, ps.k0 - ps.kc, bitz::kChunkMaxLog - 2, kap.data() + n);
+    stage_constants(h, ps.k0, n - ps.k0, n - ps.k0 - 1, kap.data() + n);
     batch_inverse(kap);
     h->scale_big.assign(kap.begin(), kap.begin() + n);
     h->scale_tail.assign(kap.begin() + n, kap.end());
-    h->tau = tau;
     h->tau_inv = tau;
     batch_inverse(h->tau_inv);
     h->eq_valid = true;
@@ -1287,18 +1327,6 @@ done:
     return rc;
 }
 
-// Round schedule of the pipelined eq-factored prover: rounds [0, kc) as
-// grid-wide kernels, [kc, k0) in the chunked kernel, [k0, n) in the tail.
-struct PipeSchedule {
-    int kc, k0;
-};
-static PipeSchedule pipe_schedule(int n) {
-    const int k0 = std::max(0, n - kTailLog);
-    const int kc = std::max(1, n - 1 - bitz::kThreadBits);  // rounds before kc iterate over their pairs
-    if (k0 - kc >= 1) return {kc, k0};
-    return {k0, k0};
-}
-
 // Launches the prologue: clearing of the round-sum slots, factor check,
 // H(i) tables and the challenges' constant tables (the challenges travel
 // as kernel arguments).
@@ -1368,7 +1396,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         const int cs = bitz::kChunkMaxLog, steps = ps.k0 - ps.kc;
         f128* out[3];
         next_out(out);
-        bitz::ChunkArgs ca{};
+        bitz::LdsStage ca{};
         ca.a = h->cur[0];
         ca.b = h->cur[1];
         ca.c = h->cur[2];
@@ -1380,20 +1408,21 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         ca.H = h->eq_in[1];
         ca.ell = h->ell;
         ca.n = n;
-        ca.kc = ps.kc;
-        ca.cs = cs;
+        ca.k_first = ps.kc;
         ca.s = steps;
+        ca.lg_in = cs;
+        ca.fold_first = 1;
         ca.q1mask = q1mask;
         ca.acc = h->acc.u32();
         const unsigned chunks = (unsigned)(h->len >> cs);
-        const size_t smem = sizeof(uint4) * (((size_t)3 << (cs - 1)) + bitz::kLaneTables256);
         timers[ps.kc].begin(s, "outer_chunk", 0.0);
-        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(bitz::kChunkBlock), smem, s, ca);
+        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(bitz::kStageBlock), bitz::stage_smem(cs - 1), s,
+                           ca);
         timers[ps.kc].end(s);
         TRY(check_launch());
         advance(out, h->len >> steps);
     }
-    bitz::TailArgs ta{};
+    bitz::LdsStage ta{};
     ta.a = h->cur[0];
     ta.b = h->cur[1];
     ta.c = h->cur[2];
@@ -1402,15 +1431,17 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     ta.H = h->eq_in[1];
     ta.ell = h->ell;
     ta.n = n;
-    ta.k0 = ps.k0;
+    ta.k_first = ps.k0;
+    ta.s = n - ps.k0;
     ta.fold_first = ps.k0 > 0;
-    ta.len_in_log = ps.k0 > 0 ? n - ps.k0 + 1 : n;
+    ta.lg_in = ps.k0 > 0 ? n - ps.k0 + 1 : n;
+    ta.final_stage = 1;
+    ta.store_sums = 1;
     ta.q1mask = q1mask;
     ta.acc = h->acc.u32();
     ta.fin = reinterpret_cast<f128*>(h->acc.u32() + h->fin_off());
-    const size_t smem = sizeof(uint4) * (((size_t)3 << (n - ps.k0)) + bitz::kLaneTables256);
     timers[n].begin(s, "outer_tail", 0.0);
-    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(bitz::kTailBlock), smem, s, ta);
+    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(bitz::kStageBlock), bitz::stage_smem(n - ps.k0), s, ta);
     timers[n].end(s);
     TRY(check_launch());
     h->round = n;
@@ -1627,11 +1658,11 @@ struct BigPinned {
 } g_batch_pinned;
 
 struct BatchStage {
-    int k_first, s, cs_in, fold_first, final_stage;
+    int k_first, s, lg_in, fold_first, final_stage;
     int len_log_in, len_log_out;  // log2 table length before / after the stage
 };
 
-// Stages of one instance of n rounds: chunks of at most 2^kBatchMaxLog
+// Stages of one instance of n rounds: chunks of at most 2^kStageMaxLog
 // entries per table in LDS, nine rounds per non-final stage.
 std::vector<BatchStage> plan_batch(int n) {
     std::vector<BatchStage> st;
@@ -1642,18 +1673,18 @@ std::vector<BatchStage> plan_batch(int n) {
         b.fold_first = fold_first;
         b.len_log_in = len_log;
         const int cap = len_log - fold_first;
-        if (cap <= bitz::kBatchMaxLog) {
-            b.cs_in = len_log;
+        if (cap <= bitz::kStageMaxLog) {
+            b.lg_in = len_log;
             b.s = n - k;
             b.final_stage = 1;
             b.len_log_out = 0;
             st.push_back(b);
             break;
         }
-        b.cs_in = bitz::kBatchMaxLog + fold_first;
-        b.s = bitz::kBatchMaxLog - 1;
+        b.lg_in = bitz::kStageMaxLog + fold_first;
+        b.s = bitz::kStageMaxLog - 1;
         b.final_stage = 0;
-        b.len_log_out = (len_log - b.cs_in) + (bitz::kBatchMaxLog - b.s + 1);
+        b.len_log_out = (len_log - b.lg_in) + (bitz::kStageMaxLog - b.s + 1);
         st.push_back(b);
         k += b.s;
         len_log = b.len_log_out;
@@ -1684,31 +1715,10 @@ int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in) {
     return BITZ_OK;
 }
 
-// Constant carried by run_weight's product over the variables [a, b)
-// (kernels.hip): eq_low's part times eq_high's part, each c_R =
-// prod over that factor's other variables of (1 + tau_i).
-f128 run_constant(const bitz_outer* h, int a, int b) {
-    const int ell = h->ell, n = h->n;
-    auto kap = [&](int t, int x, int y) {
-        const int lo = t == 0 ? 0 : ell;
-        return hostf::mul(h->kpre[t][x - lo], h->ksuf[t][y - lo]);
-    };
-    if (b <= ell) return kap(0, a, b);
-    if (a >= ell) return kap(1, a, b);
-    (void)n;
-    return hostf::mul(kap(0, a, ell), kap(1, ell, b));
-}
-
-// Round k of a batched stage weighs pair y = g 2^pl + yl by
-// run_weight(k+1, k+1+pl, yl) run_weight(k+1+pl, n, g).
+// The constants of every stage's weights.
 void batch_scales(bitz_outer* h, const std::vector<BatchStage>& plan) {
-    std::vector<f128> kap(h->n);
-    for (const auto& st : plan)
-        for (int l = 0; l < st.s; ++l) {
-            const int k = st.k_first + l;
-            const int pl = (st.cs_in - st.fold_first) - 1 - l;
-            kap[k] = hostf::mul(run_constant(h, k + 1, k + 1 + pl), run_constant(h, k + 1 + pl, h->n));
-        }
+    std::vector<f128> kap(h->n, hostf::one());
+    for (const auto& st : plan) stage_constants(h, st.k_first, st.s, st.lg_in - st.fold_first - 1, kap.data());
     batch_inverse(kap);
     h->scale_batch = kap;
 }
@@ -1798,14 +1808,14 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
             pro_blocks += p.eq_blocks + (unsigned)h->n;
             pro.push_back(p);
         }
-        std::vector<std::vector<bitz::BatchInst>> stage_items(n_stages);
+        std::vector<std::vector<bitz::LdsStage>> stage_items(n_stages);
         std::vector<uint32_t> stage_chunks(n_stages, 0);
         for (size_t j = 0; j < n_stages; ++j) {
             for (uint32_t i : members) {
                 if (j >= plans[i].size()) continue;
                 bitz_outer* h = hs[i].get();
                 const BatchStage& st = plans[i][j];
-                bitz::BatchInst b{};
+                bitz::LdsStage b{};
                 const f128* src = j == 0 ? nullptr : buf[(j - 1) & 1].f();
                 const size_t src_stride = j == 0 ? 0 : scratch;
                 if (j == 0) {
@@ -1834,11 +1844,11 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
                 b.n = h->n;
                 b.k_first = st.k_first;
                 b.s = st.s;
-                b.cs_in = st.cs_in;
+                b.lg_in = st.lg_in;
                 b.fold_first = st.fold_first;
                 b.final_stage = st.final_stage;
                 b.chunk_base = stage_chunks[j];
-                stage_chunks[j] += (uint32_t)1 << (st.len_log_in - st.cs_in);
+                stage_chunks[j] += (uint32_t)1 << (st.len_log_in - st.lg_in);
                 stage_items[j].push_back(b);
             }
         }
@@ -1861,7 +1871,7 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         for (size_t j = 0; j < n_stages; ++j) {
             desc_bytes = (desc_bytes + 255) & ~(size_t)255;
             stage_desc_off[j] = desc_bytes;
-            desc_bytes += stage_items[j].size() * sizeof(bitz::BatchInst);
+            desc_bytes += stage_items[j].size() * sizeof(bitz::LdsStage);
         }
         const size_t up_bytes = ((desc_bytes + 255) & ~(size_t)255) + ch_total * sizeof(f128);
         const size_t stage_bytes = up_bytes + acc_words * 4;
@@ -1870,7 +1880,7 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         char* pin = static_cast<char*>(g_batch_pinned.p);
         memcpy(pin, pro.data(), pro.size() * sizeof(bitz::BatchPrologue));
         for (size_t j = 0; j < n_stages; ++j)
-            memcpy(pin + stage_desc_off[j], stage_items[j].data(), stage_items[j].size() * sizeof(bitz::BatchInst));
+            memcpy(pin + stage_desc_off[j], stage_items[j].data(), stage_items[j].size() * sizeof(bitz::LdsStage));
         const size_t ch_bytes_off = (desc_bytes + 255) & ~(size_t)255;
         for (uint32_t i : members)
             memcpy(pin + ch_bytes_off + ch_off[i] * sizeof(f128), inst[i].challenges,
@@ -1883,7 +1893,7 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         if (!attr) {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_batch),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(sizeof(uint4) * ((3 << bitz::kBatchMaxLog) + bitz::kLaneTables256)));
+                                      (int)bitz::stage_smem(bitz::kStageMaxLog));
             attr = true;
         }
         Timed tp;
@@ -1893,11 +1903,11 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         tp.end(s);
         TRY(check_launch());
         std::vector<Timed> ts(n_stages);
-        const size_t smem = sizeof(uint4) * ((3 << bitz::kBatchMaxLog) + bitz::kLaneTables256);
+        const size_t smem = bitz::stage_smem(bitz::kStageMaxLog);
         for (size_t j = 0; j < n_stages; ++j) {
             ts[j].begin(s, "outer_batch_stage", 0.0);
             hipLaunchKernelGGL(bitz::k_outer_batch, dim3(stage_chunks[j]), dim3(256), smem, s,
-                               reinterpret_cast<const bitz::BatchInst*>(static_cast<char*>(desc.p) +
+                               reinterpret_cast<const bitz::LdsStage*>(static_cast<char*>(desc.p) +
                                                                         stage_desc_off[j]),
                                (int)stage_items[j].size());
             ts[j].end(s);

@@ -45,6 +45,16 @@ __device__ __forceinline__ g4 g_load(const f128* p) {
     r.w[0] = v.x; r.w[1] = v.y; r.w[2] = v.z; r.w[3] = v.w;
     return r;
 }
+// Load from device memory through a global-address-space pointer: the
+// compiler then emits global loads, which (unlike flat loads of pointers it
+// cannot place) do not hold up waits on LDS traffic.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ g4 g_load_hbm(const f128* p) {
+    const u32x4 v = *(const __attribute__((address_space(1))) u32x4*)(p);
+    g4 r;
+    r.w[0] = v.x; r.w[1] = v.y; r.w[2] = v.z; r.w[3] = v.w;
+    return r;
+}
 __device__ __forceinline__ void g_store(f128* p, const g4& a) {
     *reinterpret_cast<uint4*>(p) = make_uint4(a.w[0], a.w[1], a.w[2], a.w[3]);
 }

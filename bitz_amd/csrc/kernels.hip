@@ -16,7 +16,7 @@ namespace bitz {
 // ---------------------------------------------------------------- helpers --
 // Requires blockDim.x to be a multiple of 64 (whole waves), at most 1024.
 template <int K>
-__device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
+__device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst, int nthreads = blockDim.x) {
     __shared__ uint32_t red[1024 / 64][4 * K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -35,8 +35,8 @@ __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
             for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
     }
     __syncthreads();
-    const int waves = (int)(blockDim.x >> 6);
-    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
+    const int waves = nthreads >> 6;
+    for (int j = threadIdx.x; j < 4 * K; j += nthreads) {
         uint32_t x = 0;
         for (int w = 0; w < waves; ++w) x ^= red[w][j];
         if (x) atomicXor(dst + j, x);
@@ -46,7 +46,7 @@ __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst) {
 // XOR of v over the workgroup, stored (not accumulated) at dst: for kernels
 // that run as a single workgroup.  Whole waves only.
 template <int K>
-__device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst) {
+__device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst, int nthreads = blockDim.x) {
     __shared__ uint32_t red[1024 / 64][4 * K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -65,8 +65,8 @@ __device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst) {
             for (int i = 0; i < 4; ++i) red[wave][4 * k + i] = v[k].w[i];
     }
     __syncthreads();
-    const int waves = (int)(blockDim.x >> 6);
-    for (int j = threadIdx.x; j < 4 * K; j += blockDim.x) {
+    const int waves = nthreads >> 6;
+    for (int j = threadIdx.x; j < 4 * K; j += nthreads) {
         uint32_t x = 0;
         for (int w = 0; w < waves; ++w) x ^= red[w][j];
         dst[j] = x;
@@ -769,65 +769,104 @@ template __global__ void k_outer_eq<true, false>(OUTER_EQ_SIG);
 template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
 #undef OUTER_EQ_SIG
 
-// Eq weight of pair y in round k, read from the factors (the variables
-// above k: eq_low's part times eq_high's part).
-__device__ __forceinline__ g4 tail_weight(const f128* L, int ell, const f128* H, int k, size_t y, uint4* col) {
-    const int lo_bits = ell > k + 1 ? ell - (k + 1) : 0;
-    const int hshift = k + 1 > ell ? k + 1 - ell : 0;
-    const size_t ylo = y & (((size_t)1 << lo_bits) - 1), yhi = y >> lo_bits;
-    g4 w = g_load(H + (yhi << hshift));
-    if (lo_bits) w = g_mul_lane(w, g_load(L + (ylo << (k + 1))), col);
-    return w;
+// ============================== rounds held in LDS: chunk, tail, batch ===
+// A stage (LdsStage) runs rounds k_first .. k_first+s-1 of one eq-factored
+// proof, with the challenges known in advance, on one chunk of the tables
+// held in LDS, folding by r_k between rounds.  Round l of the stage has
+// 2^(pl0-l) pairs per chunk (pl0 = log2 of the chunk's first-round pairs);
+// pair y of chunk g is pair g 2^(pl0-l) + y of the round, so its weight
+// eq(tau_[k+1,n), .) splits into W_l(y) over the variables [k+1, b) and the
+// chunk's W_hi(g) over [b, n), with b = k_first + 1 + pl0 for every round of
+// the stage.  W_l(y) = E(y 2^l) for the one table E(x) over
+// [k_first+1, b): the entries of E whose low l variables are zero.  The
+// factor entries carry constants (products of 1 + tau_i over the variables
+// they do not cover, and E(y 2^l) the factors 1 + tau_i of its zero
+// variables) that the host divides out of the round's sums.
+
+// Product of the factor entries over the variables [a, b) of the weight,
+// at index idx (bit j of idx is variable a + j): eq_low's part times
+// eq_high's part, which is eq(tau_[a,b), idx) times a constant.  The run
+// reads eq_low alone when b <= ell, eq_high alone when a >= ell (an empty run
+// reads that factor's entry 0).
+__device__ __forceinline__ g4 run_weight(const f128* L, int ell, const f128* H, int a, int b, size_t idx,
+                                         uint4* col) {
+    if (b <= ell) return g_load_hbm(L + (idx << a));
+    if (a >= ell) return g_load_hbm(H + (idx << (a - ell)));
+    const int lo_bits = ell - a;
+    return g_mul_lane(g_load_hbm(L + ((idx & (((size_t)1 << lo_bits) - 1)) << a)), g_load_hbm(H + (idx >> lo_bits)), col);
 }
 
-// Round k's sums over the pairs held in LDS (three tables at stride 2^cap_lg,
-// 2^pl pairs; global pair index = base + local), XORed into acc_k.  Products
-// through this lane's LDS table col.
-__device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int pl, size_t base, int k,
-                                               bool q1, const f128* L, int ell, const f128* H,
-                                               uint32_t* acc_k, bool single_block, uint4* col) {
+// Offset of round l's weight table W_l (2^(pl0-l) entries) in the stage's
+// weight area of 2^(pl0+1) entries.
+__device__ __forceinline__ int stage_woff(int pl0, int l) { return (2 << pl0) - (2 << (pl0 - l)); }
+
+// Builds W_0(x) = E(x) W_hi(g) for x < 2^pl0, barrier, then the strided
+// copies W_l(y) = W_0(y 2^l) for l in [1, s).  The caller's barrier ends it.
+__device__ __forceinline__ void build_stage_weights(uint4* wt, int pl0, int s, const f128* L, int ell,
+                                                    const f128* H, int a0, int n, size_t g, uint4* col) {
+    const int b = a0 + pl0;
+    const g4 wh = run_weight(L, ell, H, b, n, g, col);
+    for (int x = threadIdx.x; x < (1 << pl0); x += kStageBlock) {
+        const g4 w = g_mul_lane(run_weight(L, ell, H, a0, b, (size_t)x, col), wh, col);
+        wt[x] = make_uint4(w.w[0], w.w[1], w.w[2], w.w[3]);
+    }
+    __syncthreads();
+    for (int l = 1; l < s; ++l)
+        for (int y = threadIdx.x; y < (1 << (pl0 - l)); y += kStageBlock) wt[stage_woff(pl0, l) + y] = wt[y << l];
+}
+
+// Round sums over the 2^pl pairs of the chunk in LDS (three tables at stride
+// 2^cap_lg) with the round's weights W at wt: sum_y W(y) (a0 b0 + c0),
+// sum_y W(y) (a0+a1)(b0+b1) and, with q1, sum_y W(y) (a1 b1 + c1), one
+// (pair, term) item at a time through this lane's table col.  Stored at
+// acc_k by a stage of one workgroup, XORed into it otherwise.
+__device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int pl, const uint4* wt, bool q1,
+                                               uint32_t* acc_k, bool store, uint4* col) {
     const size_t P = (size_t)1 << pl;
     const size_t items = (q1 ? 3 : 2) * P;
-    auto ld = [&](int t, size_t j) {
-        const uint4 u = lds[((size_t)t << cap_lg) + j];
+    auto ld = [&](const uint4* p) {
+        const uint4 u = *p;
         g4 r;
         r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
         return r;
     };
+    const uint4* A = lds;
+    const uint4* B = lds + ((size_t)1 << cap_lg);
+    const uint4* C = lds + ((size_t)2 << cap_lg);
     g4 s[3] = {g_zero(), g_zero(), g_zero()};
-    for (size_t it = threadIdx.x; it < items; it += blockDim.x) {
+    for (size_t it = threadIdx.x; it < items; it += kStageBlock) {
         const size_t y = it & (P - 1);
         const int which = (int)(it >> pl);
-        const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
-        const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
+        const g4 a0 = ld(A + 2 * y), a1 = ld(A + 2 * y + 1);
+        const g4 b0 = ld(B + 2 * y), b1 = ld(B + 2 * y + 1);
         g4 p;
         if (which == 0)
-            p = g_add(g_mul_lane(a0, b0, col), ld(2, 2 * y));
+            p = g_add(g_mul_lane(a0, b0, col), ld(C + 2 * y));
         else if (which == 1)
             p = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
         else
-            p = g_add(g_mul_lane(a1, b1, col), ld(2, 2 * y + 1));
-        const g4 wp = g_mul_lane(tail_weight(L, ell, H, k, base + y, col), p, col);
+            p = g_add(g_mul_lane(a1, b1, col), ld(C + 2 * y + 1));
+        const g4 wp = g_mul_lane(ld(wt + y), p, col);
         if (which == 0) s[0] = g_add(s[0], wp);
         else if (which == 1) s[1] = g_add(s[1], wp);
         else s[2] = g_add(s[2], wp);
     }
-    if (single_block)
-        block_xor_store<3>(s, acc_k);
+    if (store)
+        block_xor_store<3>(s, acc_k, kStageBlock);
     else
-        block_xor_to_global<3>(s, acc_k);
+        block_xor_to_global<3>(s, acc_k, kStageBlock);
 }
 
 // In-place fold of the three LDS tables (2^(pl+1) entries each at stride
 // 2^cap_lg) through the 4-bit table of r in rt: read everything, barrier,
-// write.  At most kFoldPerThread outputs per thread.
+// write, barrier.  At most kFoldPerThread outputs per thread.
 template <int kFoldPerThread>
 __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const uint4* rt) {
     const size_t half = (size_t)1 << pl;
     g4 nv[kFoldPerThread];
 #pragma unroll
     for (int q = 0; q < kFoldPerThread; ++q) {
-        const size_t e = threadIdx.x + (size_t)q * blockDim.x;
+        const size_t e = threadIdx.x + (size_t)q * kStageBlock;
         if (e < 3 * half) {
             const int t = (int)(e >> pl);
             const size_t j = e & (half - 1);
@@ -841,7 +880,7 @@ __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const u
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kFoldPerThread; ++q) {
-        const size_t e = threadIdx.x + (size_t)q * blockDim.x;
+        const size_t e = threadIdx.x + (size_t)q * kStageBlock;
         if (e < 3 * half) {
             const int t = (int)(e >> pl);
             const size_t j = e & (half - 1);
@@ -851,61 +890,131 @@ __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const u
     __syncthreads();
 }
 
-__device__ __forceinline__ void copy_table4(uint4* lds, const uint4* __restrict__ src) {
-    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) lds[e] = src[e];
+// A challenge's 4-bit table in two steps: the loads into registers, issued
+// early, and the LDS stores once the last reader of the previous table is
+// done.  256 threads.
+constexpr int kRtabPer = kConstTable4Entries / kStageBlock;
+__device__ __forceinline__ void rtab_fetch(g4 (&v)[kRtabPer], const uint4* __restrict__ src) {
+#pragma unroll
+    for (int q = 0; q < kRtabPer; ++q) v[q] = g_load_hbm(reinterpret_cast<const f128*>(src + threadIdx.x + q * kStageBlock));
+}
+__device__ __forceinline__ void rtab_store(uint4* rt, const g4 (&v)[kRtabPer]) {
+#pragma unroll
+    for (int q = 0; q < kRtabPer; ++q) rt[threadIdx.x + q * kStageBlock] = make_uint4(v[q].w[0], v[q].w[1], v[q].w[2], v[q].w[3]);
 }
 
-// Loads 2^lg folded entries per table (chunk of 2^(lg+1) inputs starting at
-// element `off`) into LDS: out[j] = in[2j] + r (in[2j] + in[2j+1]).
-__device__ __forceinline__ void lds_load_folded(uint4* lds, int lg, const f128* a, const f128* b,
-                                                const f128* c, size_t off, const uint4* rt) {
+// Loads entries [off, off + 2^lg) of a, b and c into the LDS tables (stride
+// 2^cap_lg).  Chunks of kPer 2^8 entries per table take one group of loads
+// per thread, all issued before the first LDS store; smaller ones go entry
+// by entry.
+template <int kPer>
+__device__ __forceinline__ void lds_load_rows(uint4* lds, int cap_lg, int lg, const f128* a, const f128* b,
+                                              const f128* c, size_t off) {
     const size_t len = (size_t)1 << lg;
-    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+    if (len == (size_t)kPer * kStageBlock) {
+        g4 v[3][kPer];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                v[t][q] = g_load_hbm((t == 0 ? a : (t == 1 ? b : c)) + off + q * kStageBlock + threadIdx.x);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                lds[((size_t)t << cap_lg) + q * kStageBlock + threadIdx.x] =
+                    make_uint4(v[t][q].w[0], v[t][q].w[1], v[t][q].w[2], v[t][q].w[3]);
+        return;
+    }
+    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
         const int t = (int)(e >> lg);
-        const size_t j = e & (len - 1);
-        const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off;
-        const g4 lo = g_load(src + 2 * j), hi = g_load(src + 2 * j + 1);
-        const g4 v = g_add(lo, g_mul_table4(rt, g_add(lo, hi)));
-        lds[e] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+        const g4 v = g_load_hbm((t == 0 ? a : (t == 1 ? b : c)) + off + (e & (len - 1)));
+        lds[((size_t)t << cap_lg) + (e & (len - 1))] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
     }
 }
 
-// Middle rounds kc .. kc+s-1 of an eq-factored proof with challenges known in
-// advance.  Adjacent-pair folding keeps chunks independent: workgroup g
-// loads inputs [g 2^cs, (g+1) 2^cs) of each table (the tables before the
-// fold by r_{kc-1}), folds them in LDS round after round and adds its pairs'
-// share of every round's sums into the round's slot; it writes its 2^(cs-s)
-// entries of the tables of round kc+s-1 (before the fold by r_{kc+s-1}).
-// Dynamic LDS: the three tables (3 x 2^(cs-1) entries), then 16 KiB of lane
-// tables per wave.
-__global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
+// The same for 2^lg folded entries per table, out[j] = in[2j] + r (in[2j] +
+// in[2j+1]) over the 2^(lg+1) inputs from off, with r's table in rt.
+template <int kPer>
+__device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, const f128* a, const f128* b,
+                                                const f128* c, size_t off, const uint4* rt) {
+    const size_t len = (size_t)1 << lg;
+    if (len == (size_t)kPer * kStageBlock) {
+        g4 lo[3][kPer], hi[3][kPer];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off + 2 * (q * kStageBlock + threadIdx.x);
+                lo[t][q] = g_load_hbm(src);
+                hi[t][q] = g_load_hbm(src + 1);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const g4 v = g_add(lo[t][q], g_mul_table4(rt, g_add(lo[t][q], hi[t][q])));
+                lds[((size_t)t << cap_lg) + q * kStageBlock + threadIdx.x] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+            }
+        return;
+    }
+    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
+        const int t = (int)(e >> lg);
+        const size_t j = e & (len - 1);
+        const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off + 2 * j;
+        const g4 lo = g_load_hbm(src), hi = g_load_hbm(src + 1);
+        const g4 v = g_add(lo, g_mul_table4(rt, g_add(lo, hi)));
+        lds[((size_t)t << cap_lg) + j] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+    }
+}
+
+// One chunk of a stage, 256 threads.  Dynamic LDS (stage_smem): the three
+// tables (2^cap_lg entries each), the weight tables (2^cap_lg entries) and
+// 16 KiB of lane tables per wave.
+__device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
     __shared__ uint4 rt[kConstTable4Entries];
     extern __shared__ uint4 lds_raw[];
-    const size_t g = blockIdx.x;
-    int lg = A.cs - 1;
-    const int cap_lg = lg;
-    uint4* col = lds_raw + ((size_t)3 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    const int cap_lg = A.lg_in - A.fold_first;
+    const int pl0 = cap_lg - 1;
+    uint4* wt = lds_raw + ((size_t)3 << cap_lg);
+    uint4* col = wt + ((size_t)1 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
-    copy_table4(rt, A.rtabs4 + (size_t)(A.kc - 1) * kConstTable4Entries);
+    g4 nx[kRtabPer];
+    if (A.fold_first) {
+        rtab_fetch(nx, A.rtabs4 + (size_t)(A.k_first - 1) * kConstTable4Entries);
+        rtab_store(rt, nx);
+    }
+    build_stage_weights(wt, pl0, A.s, A.L, A.ell, A.H, A.k_first + 1, A.n, g, col);  // barrier inside
+    constexpr int kPer = (1 << kStageMaxLog) / kStageBlock;  // loads per thread and table of a full chunk
+    if (A.fold_first)
+        lds_load_folded<kPer>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in, rt);
+    else
+        lds_load_rows<kPer>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in);
     __syncthreads();
-    lds_load_folded(lds_raw, lg, A.a, A.b, A.c, g << A.cs, rt);
-    __syncthreads();
+    int lg = cap_lg;
     for (int l = 0; l < A.s; ++l) {
-        const int k = A.kc + l;
+        const int k = A.k_first + l;
         const int pl = lg - 1;
-        lds_round_sums(lds_raw, cap_lg, pl, g << pl, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k,
-                       false, col);
-        if (l + 1 < A.s) {
+        const bool fold = l + 1 < A.s || A.final_stage;
+        if (fold) rtab_fetch(nx, A.rtabs4 + (size_t)k * kConstTable4Entries);
+        lds_round_sums(lds_raw, cap_lg, pl, wt + stage_woff(pl0, l), (A.q1mask >> k) & 1, A.acc + 12 * k,
+                       A.store_sums, col);
+        if (fold) {
             __syncthreads();  // every reader of rt is done
-            copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
+            rtab_store(rt, nx);
             __syncthreads();
-            lds_fold<kChunkFoldPerThread>(lds_raw, cap_lg, pl, rt);
+            lds_fold<6>(lds_raw, cap_lg, pl, rt);
             lg = pl;
         }
     }
-    __syncthreads();
+    if (A.final_stage) {
+        if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
+        return;
+    }
     const size_t len = (size_t)1 << lg;
-    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
+    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
         const int t = (int)(e >> lg);
         const size_t j = e & (len - 1);
         f128* dst = (t == 0 ? A.ao : (t == 1 ? A.bo : A.co)) + (g << lg) + j;
@@ -913,47 +1022,12 @@ __global__ __launch_bounds__(256) void k_outer_chunk(ChunkArgs A) {
     }
 }
 
-// The last rounds of an eq-factored proof with challenges known in advance,
-// in one workgroup: the three tables (at most 2^kTailMaxLog elements each
-// after the first fold) stay in LDS, and every round is a fold by r_{k-1}
-// followed by the round sums, separated by barriers instead of kernel
-// launches.  The round sums are stored at acc + 12 k and the final A(r),
-// B(r), C(r) at fin[0..2].  256 threads; dynamic LDS: the three tables, then
-// 16 KiB of lane tables per wave.
-__global__ __launch_bounds__(256) void k_outer_tail(TailArgs A) {
-    __shared__ uint4 rt[kConstTable4Entries];
-    extern __shared__ uint4 lds_raw[];
-    int lg = A.len_in_log;
-    if (A.fold_first) {
-        copy_table4(rt, A.rtabs4 + (size_t)(A.k0 - 1) * kConstTable4Entries);
-        __syncthreads();
-        lg -= 1;
-        lds_load_folded(lds_raw, lg, A.a, A.b, A.c, 0, rt);
-    } else {
-        const size_t len = (size_t)1 << lg;
-        for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
-            const int t = (int)(e >> lg);
-            const f128* src = t == 0 ? A.a : (t == 1 ? A.b : A.c);
-            lds_raw[e] = *reinterpret_cast<const uint4*>(src + (e & (len - 1)));
-        }
-    }
-    __syncthreads();
-    const int cap_lg = lg;
-    uint4* col = lds_raw + ((size_t)3 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
-    lane_table_clear(col);
-    for (int k = A.k0; k < A.n; ++k) {
-        const int pl = lg - 1;
-        lds_round_sums(lds_raw, cap_lg, pl, 0, k, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k, true,
-                       col);
-        __syncthreads();
-        copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
-        __syncthreads();
-        lds_fold<kTailFoldPerThread>(lds_raw, cap_lg, pl, rt);
-        lg = pl;
-    }
-    if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
-}
+// Middle rounds of a single proof: workgroup g runs the stage on chunk g.
+__global__ __launch_bounds__(kStageBlock) void k_outer_chunk(LdsStage A) { lds_stage(A, blockIdx.x); }
 
+// The last rounds of a single proof in one workgroup (a final stage of one
+// chunk): barriers instead of kernel launches between rounds.
+__global__ __launch_bounds__(kStageBlock) void k_outer_tail(LdsStage A) { lds_stage(A, 0); }
 
 // ======================================= batches of independent proofs ===
 __global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* items, int count) {
@@ -981,118 +1055,19 @@ __global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* it
     for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
 }
 
-// Product of the factor entries over the variables [a, b) of the weight,
-// at index idx (bit j of idx is variable a + j): eq_low's part times
-// eq_high's part, which is eq(tau_[a,b), idx) times a constant the host
-// divides out.  The run picks eq_low alone when b <= ell, eq_high alone when
-// a >= ell (an empty run reads that factor's entry 0).
-__device__ __forceinline__ g4 run_weight(const f128* L, int ell, const f128* H, int a, int b, size_t idx,
-                                         uint4* col) {
-    if (b <= ell) return g_load(L + (idx << a));
-    if (a >= ell) return g_load(H + (idx << (a - ell)));
-    const int lo_bits = ell - a;
-    return g_mul_lane(g_load(L + ((idx & (((size_t)1 << lo_bits) - 1)) << a)), g_load(H + (idx >> lo_bits)), col);
-}
-
-// Round k's sums over a chunk's pairs in LDS (2^pl pairs; the chunk is pair
-// block g of the round): sum_yl W_lo(yl) p(yl) per thread, one pair per
-// thread at a time, then times W_hi(g), XORed into acc_k.  W_lo covers the
-// variables [k+1, k+1+pl), W_hi the rest up to n.
-__device__ __forceinline__ void chunk_round_sums(const uint4* lds, int cap_lg, int pl, size_t g, int k, int n,
-                                                 bool q1, const f128* L, int ell, const f128* H,
-                                                 uint32_t* acc_k, uint4* col) {
-    const size_t P = (size_t)1 << pl;
-    const size_t items = (q1 ? 3 : 2) * P;
-    auto ld = [&](int t, size_t j) {
-        const uint4 u = lds[((size_t)t << cap_lg) + j];
-        g4 r;
-        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
-        return r;
-    };
-    g4 s[3] = {g_zero(), g_zero(), g_zero()};
-    for (size_t it = threadIdx.x; it < items; it += blockDim.x) {
-        const size_t y = it & (P - 1);
-        const int which = (int)(it >> pl);
-        const g4 a0 = ld(0, 2 * y), a1 = ld(0, 2 * y + 1);
-        const g4 b0 = ld(1, 2 * y), b1 = ld(1, 2 * y + 1);
-        g4 p;
-        if (which == 0)
-            p = g_add(g_mul_lane(a0, b0, col), ld(2, 2 * y));
-        else if (which == 1)
-            p = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
-        else
-            p = g_add(g_mul_lane(a1, b1, col), ld(2, 2 * y + 1));
-        const g4 wp = g_mul_lane(run_weight(L, ell, H, k + 1, k + 1 + pl, y, col), p, col);
-        if (which == 0) s[0] = g_add(s[0], wp);
-        else if (which == 1) s[1] = g_add(s[1], wp);
-        else s[2] = g_add(s[2], wp);
-    }
-    if (threadIdx.x < items) {
-        const g4 wh = run_weight(L, ell, H, k + 1 + pl, n, g, col);
-        for (int j = 0; j < 3; ++j) s[j] = g_mul_lane(wh, s[j], col);
-    }
-    block_xor_to_global<3>(s, acc_k);
-}
-
-// One workgroup per (instance, chunk) of the current stage: the chunked
-// rounds of k_outer_chunk, for many instances of different sizes at once,
-// from round 0 (no fold first) to the last round (final stage).
-__global__ __launch_bounds__(256) void k_outer_batch(const BatchInst* insts, int count) {
-    __shared__ uint4 rt[kConstTable4Entries];
-    extern __shared__ uint4 lds_raw[];
+// One workgroup per (proof, chunk) of a batch stage: the stage of the proof
+// whose chunk_base is the last one at or below the workgroup index.
+__global__ __launch_bounds__(kStageBlock) void k_outer_batch(const LdsStage* stages, int count) {
     int lo = 0, hi = count - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (insts[mid].chunk_base <= blockIdx.x)
+        if (stages[mid].chunk_base <= blockIdx.x)
             lo = mid;
         else
             hi = mid - 1;
     }
-    const BatchInst& A = insts[lo];
-    const size_t g = blockIdx.x - A.chunk_base;
-    int lg = A.cs_in - A.fold_first;
-    const int cap_lg = lg;
-    uint4* col = lds_raw + ((size_t)3 << kBatchMaxLog) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
-                 (threadIdx.x & 63);
-    lane_table_clear(col);
-    if (A.fold_first) {
-        copy_table4(rt, A.rtabs4 + (size_t)(A.k_first - 1) * kConstTable4Entries);
-        __syncthreads();
-        lds_load_folded(lds_raw, lg, A.a, A.b, A.c, g << A.cs_in, rt);
-    } else {
-        const size_t len = (size_t)1 << lg;
-        for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
-            const int t = (int)(e >> lg);
-            const f128* src = (t == 0 ? A.a : (t == 1 ? A.b : A.c)) + (g << lg);
-            lds_raw[e] = *reinterpret_cast<const uint4*>(src + (e & (len - 1)));
-        }
-    }
-    __syncthreads();
-    for (int l = 0; l < A.s; ++l) {
-        const int k = A.k_first + l;
-        const int pl = lg - 1;
-        chunk_round_sums(lds_raw, cap_lg, pl, g, k, A.n, (A.q1mask >> k) & 1, A.L, A.ell, A.H, A.acc + 12 * k,
-                         col);
-        if (l + 1 < A.s || A.final_stage) {
-            __syncthreads();  // every reader of rt is done
-            copy_table4(rt, A.rtabs4 + (size_t)k * kConstTable4Entries);
-            __syncthreads();
-            lds_fold<kBatchFoldPerThread>(lds_raw, cap_lg, pl, rt);
-            lg = pl;
-        }
-    }
-    __syncthreads();
-    if (A.final_stage) {
-        if (threadIdx.x < 3) *reinterpret_cast<uint4*>(A.fin + threadIdx.x) = lds_raw[(size_t)threadIdx.x << cap_lg];
-        return;
-    }
-    const size_t len = (size_t)1 << lg;
-    for (size_t e = threadIdx.x; e < 3 * len; e += blockDim.x) {
-        const int t = (int)(e >> lg);
-        const size_t j = e & (len - 1);
-        f128* dst = (t == 0 ? A.ao : (t == 1 ? A.bo : A.co)) + (g << lg) + j;
-        *reinterpret_cast<uint4*>(dst) = lds_raw[((size_t)t << cap_lg) + j];
-    }
+    const LdsStage A = stages[lo];
+    lds_stage(A, blockIdx.x - A.chunk_base);
 }
 
 }  // namespace bitz

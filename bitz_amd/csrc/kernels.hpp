@@ -100,73 +100,43 @@ constexpr int kR0Block = 256;
 constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
 constexpr int kThreadBits = 17;  // threads of a multi-iteration round: one wave of lane-table workgroups
 
-// Single-workgroup tail of an eq-factored proof (see k_outer_tail).
-constexpr int kTailMaxLog = 11;        // at most 2^11 input elements per table (2^10 in LDS)
-constexpr int kTailBlock = 256;
-constexpr int kTailFoldPerThread = 6;  // 3 * 2^9 fold outputs over 256 threads
-struct TailArgs {
-    const f128* a;
-    const f128* b;
-    const f128* c;
-    const uint4* rtabs4;  // 4-bit table of r_k at rtabs4 + 512 k
-    const f128* L;
-    const f128* H;
-    int ell, n, k0, fold_first, len_in_log;
-    uint64_t q1mask;
-    uint32_t* acc;  // round k's sums at acc + 12 k
-    f128* fin;      // A(r), B(r), C(r)
-};
-__global__ void k_outer_tail(TailArgs args);
-
-// Chunked middle rounds (see k_outer_chunk): 256-thread workgroups, chunks
-// of 2^cs inputs with cs <= kChunkMaxLog.
-constexpr int kChunkMaxLog = 11;
-constexpr int kChunkBlock = 256;
-constexpr int kChunkFoldPerThread = 6;  // 3 * 2^(kChunkMaxLog - 2) fold outputs over 256 threads
-// Lane tables of a 256-thread workgroup (4 waves x 64 lanes x 16 entries).
-constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
-struct ChunkArgs {
+// ---- rounds held in LDS: chunk, tail and batch stages -------------------
+// One stage of one proof: rounds [k_first, k_first + s) on chunks of 2^lg_in
+// inputs per table (the tables before the fold by r_{k_first - 1} when
+// fold_first), chunk g at inputs [g 2^lg_in, (g + 1) 2^lg_in).  A final stage
+// folds after its last round too and stores A(r), B(r), C(r) at fin; other
+// stages store their 2^(lg_in - fold_first - s + 1) entries per chunk and
+// table at ao/bo/co.  Round k's sums go to acc + 12 k: stored when
+// store_sums (a stage of one chunk), XORed in otherwise.
+struct LdsStage {
     const f128* a;
     const f128* b;
     const f128* c;
     f128* ao;
     f128* bo;
     f128* co;
-    const uint4* rtabs4;
     const f128* L;
     const f128* H;
-    int ell, n, kc, cs, s;
-    uint64_t q1mask;
+    const uint4* rtabs4;  // 4-bit table of r_k at rtabs4 + 512 k
     uint32_t* acc;
-};
-__global__ void k_outer_chunk(ChunkArgs args);
-constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
-
-// ---- batches of independent eq-factored proofs (k_outer_batch) ----------
-// One stage of one instance: rounds [k_first, k_first + s) over chunks of
-// 2^cs_in inputs (the tables before the fold by r_{k_first - 1} when
-// fold_first), chunk g = work item chunk_base + g.  A final stage folds the
-// last round too and stores A(r), B(r), C(r) at fin; other stages store
-// their 2^(cs_in - fold_first - s) entries per chunk and table at ao/bo/co.
-struct BatchInst {
-    const f128* a;
-    const f128* b;
-    const f128* c;
-    f128* ao;
-    f128* bo;
-    f128* co;
-    const f128* L;
-    const f128* H;
-    const uint4* rtabs4;  // 4-bit table of r_k at rtabs4 + 512 k
-    uint32_t* acc;        // round k's sums at acc + 12 k
     f128* fin;
     uint64_t q1mask;
-    int ell, n, k_first, s, cs_in, fold_first, final_stage;
-    uint32_t chunk_base;
+    int ell, n, k_first, s, lg_in, fold_first, final_stage, store_sums;
+    uint32_t chunk_base;  // batches: first workgroup of this stage
 };
-constexpr int kBatchMaxLog = 10;  // at most 2^10 entries per table in LDS
-constexpr int kBatchFoldPerThread = 6;  // 3 * 2^9 fold outputs over 256 threads
-__global__ void k_outer_batch(const BatchInst* insts, int count);
+constexpr int kStageBlock = 256;
+constexpr int kStageMaxLog = 10;  // at most 2^10 entries per table in LDS
+constexpr int kChunkMaxLog = 11;  // chunk stage of a single proof: 2^11 inputs, folded on load
+constexpr int kTailMaxLog = 11;
+// Lane tables of a 256-thread workgroup (4 waves x 64 lanes x 16 entries).
+constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
+// Dynamic LDS of a stage with 2^cap_lg entries per table: the three tables,
+// the weight tables (2^cap_lg entries) and the lane tables.
+constexpr size_t stage_smem(int cap_lg) { return sizeof(uint4) * (((size_t)4 << cap_lg) + kLaneTables256); }
+__global__ void k_outer_chunk(LdsStage args);
+__global__ void k_outer_tail(LdsStage args);
+__global__ void k_outer_batch(const LdsStage* stages, int count);
+constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
 // Prologue of a batch: per instance, clearing of its sums, the factor check
 // (k_eq_check layout at eq_out) and the 4-bit tables of its challenges.
