@@ -570,7 +570,6 @@ struct bitz_outer {
     int ell = 0, m = 0;
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
-    std::vector<f128> scale_batch;            // batched rounds (when set, used for every round)
     std::vector<f128> kpre[2], ksuf[2];       // prefix / suffix products of (1 + tau_i) per factor
     Dev htabs4, rtabs, rtabs4;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
@@ -718,9 +717,9 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batc
     desc.first[desc.rounds] = (uint32_t)htab_total;
     h->hdesc = desc;
     h->htab_total = htab_total;
-    if (h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4))) return false;
     if (!batch_member &&
-        (h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
+        (h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
+         h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
          h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4))))
         return false;
     static bool attr_set = false;
@@ -1459,7 +1458,8 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
 // are (a recovery would divide by tau_k = 0, or the check fails).
 static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 initial_claim,
                               uint64_t q1mask, bool check_claim, std::vector<bitz_f128>& co) {
-    const int n = h->n, kc = h->batch_member ? 0 : pipe_schedule(n).kc;
+    // Grid-wide rounds before kc (a batch member's before k0: no chunk stage).
+    const int n = h->n, kc = h->batch_member ? pipe_schedule(n).k0 : pipe_schedule(n).kc;
     const uint32_t* w = h->host_words();
     for (int k = 0; k < n; ++k)
         if (!((q1mask >> k) & 1) && hostf::is_zero(h->tau[k])) return false;
@@ -1469,7 +1469,7 @@ static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 i
     for (int k = 0; k < n; ++k) {
         bitz_f128 raw[3];
         memcpy(raw, w + 12 * k, 48);
-        const f128& sc = !h->scale_batch.empty() ? h->scale_batch[k] : k >= kc ? h->scale_tail[k] : h->scale_big[k];
+        const f128& sc = k >= kc ? h->scale_tail[k] : h->scale_big[k];
         outer_eq_coeffs(h, k, raw, (q1mask >> k) & 1, sc, &co[3 * (size_t)k]);
         outer_eq_advance(h, k, F(challenges[k]));
     }
@@ -1657,42 +1657,6 @@ struct BigPinned {
     }
 } g_batch_pinned;
 
-struct BatchStage {
-    int k_first, s, lg_in, fold_first, final_stage;
-    int len_log_in, len_log_out;  // log2 table length before / after the stage
-};
-
-// Stages of one instance of n rounds: chunks of at most 2^kStageMaxLog
-// entries per table in LDS, nine rounds per non-final stage.
-std::vector<BatchStage> plan_batch(int n) {
-    std::vector<BatchStage> st;
-    int k = 0, len_log = n, fold_first = 0;
-    while (k < n) {
-        BatchStage b{};
-        b.k_first = k;
-        b.fold_first = fold_first;
-        b.len_log_in = len_log;
-        const int cap = len_log - fold_first;
-        if (cap <= bitz::kStageMaxLog) {
-            b.lg_in = len_log;
-            b.s = n - k;
-            b.final_stage = 1;
-            b.len_log_out = 0;
-            st.push_back(b);
-            break;
-        }
-        b.lg_in = bitz::kStageMaxLog + fold_first;
-        b.s = bitz::kStageMaxLog - 1;
-        b.final_stage = 0;
-        b.len_log_out = (len_log - b.lg_in) + (bitz::kStageMaxLog - b.s + 1);
-        st.push_back(b);
-        k += b.s;
-        len_log = b.len_log_out;
-        fold_first = 1;
-    }
-    return st;
-}
-
 // A batch member: the eq-factored state of one instance (geometry, tau and
 // constants after validation, coefficient recovery) without scratch of its
 // own.
@@ -1713,14 +1677,6 @@ int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in) {
     h->eqmode = outer_setup_eq(h, (int)in.low_vars, (int)in.high_vars, true);
     *out = h;
     return BITZ_OK;
-}
-
-// The constants of every stage's weights.
-void batch_scales(bitz_outer* h, const std::vector<BatchStage>& plan) {
-    std::vector<f128> kap(h->n, hostf::one());
-    for (const auto& st : plan) stage_constants(h, st.k_first, st.s, st.lg_in - st.fold_first - 1, kap.data());
-    batch_inverse(kap);
-    h->scale_batch = kap;
 }
 
 }  // namespace
@@ -1754,41 +1710,46 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         StreamOwner so;
         TRY(so.create());
         hipStream_t s = so.s;
-        // Layout of the shared sums buffer and of the host staging.
-        std::vector<size_t> acc_off(count, 0), ch_off(count, 0);
-        size_t acc_words = 0, ch_total = 0;
-        std::vector<std::vector<BatchStage>> plans(count);
-        size_t n_stages = 0;
+        // Per member: grid-wide rounds [0, k0) (k_batch_r0, then k_batch_fold
+        // writing ping-pong tables P (odd rounds) and Q (even rounds)), then the
+        // LDS tail [k0, n) in one workgroup (k_outer_batch).
+        std::vector<int> k0(count, 0);
+        std::vector<size_t> acc_off(count, 0), ch_off(count, 0), ht_off(count, 0), p_off(count, 0), q_off(count, 0);
+        size_t acc_words = 0, ch_total = 0, ht_total = 0, p_total = 0, q_total = 0;
+        int kmax = 0;
         for (uint32_t i : members) {
             bitz_outer* h = hs[i].get();
+            k0[i] = pipe_schedule(h->n).k0;
+            kmax = std::max(kmax, k0[i]);
             acc_off[i] = acc_words;
             acc_words += (h->acc_words() + 63) & ~(size_t)63;
             ch_off[i] = ch_total;
             ch_total += (size_t)h->n;
-            plans[i] = plan_batch(h->n);
-            n_stages = std::max(n_stages, plans[i].size());
+            ht_off[i] = ht_total;
+            ht_total += h->htab_total;
+            p_off[i] = p_total;
+            if (k0[i] >= 2) p_total += h->N / 2;
+            q_off[i] = q_total;
+            if (k0[i] >= 3) q_total += h->N / 4;
         }
-        // Stage scratch: the largest stage output, ping-ponged.
-        std::vector<size_t> out_len(n_stages, 0);  // elements per table summed over instances
-        std::vector<std::vector<size_t>> out_off(count);
-        for (uint32_t i : members) {
-            out_off[i].assign(plans[i].size(), 0);
-            for (size_t j = 0; j < plans[i].size(); ++j)
-                if (!plans[i][j].final_stage) {
-                    out_off[i][j] = out_len[j];
-                    out_len[j] += (size_t)1 << plans[i][j].len_log_out;
-                }
-        }
-        size_t scratch = 1;
-        for (size_t v : out_len) scratch = std::max(scratch, v);
-        Dev acc, chal, buf[2], desc;
+        Dev acc, chal, rt4, ht4, pbuf, qbuf, desc;
         TRY(acc.alloc(acc_words * 4));
         TRY(chal.alloc(std::max<size_t>(ch_total, 1) * sizeof(f128)));
-        TRY(buf[0].alloc(3 * scratch * sizeof(f128)));
-        TRY(buf[1].alloc(3 * scratch * sizeof(f128)));
-        // Descriptors: prologue items, then every stage's instances.
+        TRY(rt4.alloc(std::max<size_t>(ch_total, 1) * kConstTable4Entries * sizeof(uint4)));
+        TRY(ht4.alloc(std::max<size_t>(ht_total, 1) * kConstTable4Entries * sizeof(uint4)));
+        TRY(pbuf.alloc(std::max<size_t>(3 * p_total, 1) * sizeof(f128)));
+        TRY(qbuf.alloc(std::max<size_t>(3 * q_total, 1) * sizeof(f128)));
+        auto rtab_of = [&](uint32_t i) { return static_cast<uint4*>(rt4.p) + ch_off[i] * kConstTable4Entries; };
+        // Round k's tables of member i: the inputs for k = 0, then the output of
+        // round k's fold by r_{k-1}.
+        auto tables_of = [&](uint32_t i, int k, int t) -> f128* {
+            if (k == 0) return const_cast<f128*>(hs[i]->inputs[t]);
+            if (k & 1) return pbuf.f() + t * p_total + p_off[i];
+            return qbuf.f() + t * q_total + q_off[i];
+        };
+        // Descriptors and block maps of every launch, staged in one pinned block.
         std::vector<bitz::BatchPrologue> pro;
-        uint32_t pro_blocks = 0;
+        std::vector<uint32_t> pro_map;
         for (uint32_t i : members) {
             bitz_outer* h = hs[i].get();
             bitz::BatchPrologue p{};
@@ -1803,117 +1764,162 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
             p.eq_blocks = h->eq_bl + h->eq_bh;
             p.eq_out = acc.u32() + acc_off[i] + h->eq_off();
             p.chal = chal.f() + ch_off[i];
-            p.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
-            p.block_base = pro_blocks;
-            pro_blocks += p.eq_blocks + (unsigned)h->n;
+            p.rtabs4 = rtab_of(i);
+            p.hdesc = h->hdesc;
+            p.htabs4 = static_cast<uint4*>(ht4.p) + ht_off[i] * kConstTable4Entries;
+            p.block_base = (uint32_t)pro_map.size();
+            pro_map.insert(pro_map.end(), p.eq_blocks + (unsigned)h->n + h->htab_total, (uint32_t)pro.size());
             pro.push_back(p);
         }
-        std::vector<std::vector<bitz::LdsStage>> stage_items(n_stages);
-        std::vector<uint32_t> stage_chunks(n_stages, 0);
-        for (size_t j = 0; j < n_stages; ++j) {
-            for (uint32_t i : members) {
-                if (j >= plans[i].size()) continue;
-                bitz_outer* h = hs[i].get();
-                const BatchStage& st = plans[i][j];
-                bitz::LdsStage b{};
-                const f128* src = j == 0 ? nullptr : buf[(j - 1) & 1].f();
-                const size_t src_stride = j == 0 ? 0 : scratch;
-                if (j == 0) {
-                    b.a = h->inputs[0];
-                    b.b = h->inputs[1];
-                    b.c = h->inputs[2];
-                } else {
-                    const size_t o = out_off[i][j - 1];
-                    b.a = src + o;
-                    b.b = src + src_stride + o;
-                    b.c = src + 2 * src_stride + o;
-                }
-                if (!st.final_stage) {
-                    f128* dst = buf[j & 1].f();
-                    const size_t o = out_off[i][j];
-                    b.ao = dst + o;
-                    b.bo = dst + scratch + o;
-                    b.co = dst + 2 * scratch + o;
-                }
-                b.L = h->eq_in[0];
-                b.H = h->eq_in[1];
-                b.rtabs4 = static_cast<const uint4*>(h->rtabs4.p);
-                b.acc = acc.u32() + acc_off[i];
-                b.fin = reinterpret_cast<f128*>(acc.u32() + acc_off[i] + h->fin_off());
-                b.ell = h->ell;
-                b.n = h->n;
-                b.k_first = st.k_first;
-                b.s = st.s;
-                b.lg_in = st.lg_in;
-                b.fold_first = st.fold_first;
-                b.final_stage = st.final_stage;
-                b.chunk_base = stage_chunks[j];
-                stage_chunks[j] += (uint32_t)1 << (st.len_log_in - st.lg_in);
-                stage_items[j].push_back(b);
-            }
-        }
-        // Round 0 skips q(1) unless a challenge is zero (the claim check at
-        // the end catches a wrong claim).
+        // q(1) in round 0 when a challenge is zero (otherwise the claim check
+        // at the end catches a wrong claim).
         std::vector<uint64_t> q1mask(count, 0);
-        for (uint32_t i : members) {
+        for (uint32_t i : members)
             for (uint32_t k = 0; k < inst[i].num_vars; ++k)
                 if (hostf::is_zero(F(inst[i].challenges[k]))) q1mask[i] = 1;
+        std::vector<std::vector<bitz::RoundDesc>> rounds(kmax);
+        std::vector<std::vector<uint32_t>> round_map(kmax);
+        for (int k = 0; k < kmax; ++k) {
+            const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
+            for (uint32_t i : members) {
+                if (k >= k0[i]) continue;
+                bitz_outer* h = hs[i].get();
+                const auto& g = h->geo[k];
+                bitz::RoundDesc d{};
+                // Round k reads round k-1's tables and folds them by r_{k-1}.
+                const int src = k == 0 ? 0 : k - 1;
+                d.a = tables_of(i, src, 0);
+                d.b = tables_of(i, src, 1);
+                d.c = tables_of(i, src, 2);
+                if (k > 0) {
+                    d.ao = tables_of(i, k, 0);
+                    d.bo = tables_of(i, k, 1);
+                    d.co = tables_of(i, k, 2);
+                    d.r_tab4 = rtab_of(i) + (size_t)(k - 1) * kConstTable4Entries;
+                }
+                d.la = h->table_ptr(g.la.table);
+                d.lb = h->table_ptr(g.lb.table);
+                d.la_bits = g.la_bits;
+                d.la_shift = g.la.a - h->table_base(g.la.table);
+                d.lb_shift = g.lb.a - h->table_base(g.lb.table);
+                d.h_tabs4 = static_cast<const uint4*>(ht4.p) + (ht_off[i] + g.htab_off) * kConstTable4Entries;
+                d.acc = acc.u32() + acc_off[i] + 12 * (size_t)k;
+                d.threads = g.threads;
+                d.iters = g.iters;
+                d.q1 = (int)((q1mask[i] >> k) & 1);
+                d.block_base = (uint32_t)round_map[k].size();
+                round_map[k].insert(round_map[k].end(), (g.threads + block - 1) / block, (uint32_t)rounds[k].size());
+                rounds[k].push_back(d);
+            }
         }
-        {
-            // Stage items were appended in member order: the same order here.
-            std::vector<size_t> pos(n_stages, 0);
-            for (uint32_t i : members)
-                for (size_t j = 0; j < plans[i].size(); ++j) stage_items[j][pos[j]++].q1mask = q1mask[i];
+        std::vector<bitz::LdsStage> tails;
+        for (uint32_t i : members) {
+            bitz_outer* h = hs[i].get();
+            bitz::LdsStage t{};
+            const int kk = k0[i];
+            // Round k0-1's tables, folded by r_{k0-1} on load (the inputs when k0 = 0).
+            const int src = kk > 0 ? kk - 1 : 0;
+            t.a = tables_of(i, src, 0);
+            t.b = tables_of(i, src, 1);
+            t.c = tables_of(i, src, 2);
+            t.L = h->eq_in[0];
+            t.H = h->eq_in[1];
+            t.rtabs4 = rtab_of(i);
+            t.acc = acc.u32() + acc_off[i];
+            t.fin = reinterpret_cast<f128*>(acc.u32() + acc_off[i] + h->fin_off());
+            t.q1mask = q1mask[i];
+            t.ell = h->ell;
+            t.n = h->n;
+            t.k_first = kk;
+            t.s = h->n - kk;
+            t.lg_in = kk > 0 ? h->n - kk + 1 : h->n;
+            t.fold_first = kk > 0;
+            t.final_stage = 1;
+            t.store_sums = 1;
+            t.chunk_base = (uint32_t)tails.size();  // one workgroup: chunk 0
+            tails.push_back(t);
         }
-        // Staging: descriptors and challenges up, sums down.
-        std::vector<size_t> stage_desc_off(n_stages);
-        size_t desc_bytes = pro.size() * sizeof(bitz::BatchPrologue);
-        for (size_t j = 0; j < n_stages; ++j) {
-            desc_bytes = (desc_bytes + 255) & ~(size_t)255;
-            stage_desc_off[j] = desc_bytes;
-            desc_bytes += stage_items[j].size() * sizeof(bitz::LdsStage);
+        std::vector<uint32_t> tail_map(tails.size());
+        for (size_t j = 0; j < tails.size(); ++j) tail_map[j] = (uint32_t)j;
+        // Staging: [prologue | map] [round k | map]... [tails | map] [challenges] ... [sums readback]
+        size_t off = 0;
+        auto place = [&](size_t bytes) {
+            off = (off + 255) & ~(size_t)255;
+            const size_t o = off;
+            off += bytes;
+            return o;
+        };
+        const size_t pro_o = place(pro.size() * sizeof(bitz::BatchPrologue));
+        const size_t pro_m = place(pro_map.size() * 4);
+        std::vector<size_t> rd_o(kmax), rd_m(kmax);
+        for (int k = 0; k < kmax; ++k) {
+            rd_o[k] = place(rounds[k].size() * sizeof(bitz::RoundDesc));
+            rd_m[k] = place(round_map[k].size() * 4);
         }
-        const size_t up_bytes = ((desc_bytes + 255) & ~(size_t)255) + ch_total * sizeof(f128);
-        const size_t stage_bytes = up_bytes + acc_words * 4;
+        const size_t tl_o = place(tails.size() * sizeof(bitz::LdsStage));
+        const size_t tl_m = place(tail_map.size() * 4);
+        const size_t desc_bytes = off;
+        const size_t ch_o = place(ch_total * sizeof(f128));
+        const size_t up_bytes = off;
+        const size_t rb_o = place(acc_words * 4);
         std::lock_guard<std::mutex> lock(g_batch_pinned.mu);
-        TRY(g_batch_pinned.reserve(stage_bytes));
+        TRY(g_batch_pinned.reserve(off));
         char* pin = static_cast<char*>(g_batch_pinned.p);
-        memcpy(pin, pro.data(), pro.size() * sizeof(bitz::BatchPrologue));
-        for (size_t j = 0; j < n_stages; ++j)
-            memcpy(pin + stage_desc_off[j], stage_items[j].data(), stage_items[j].size() * sizeof(bitz::LdsStage));
-        const size_t ch_bytes_off = (desc_bytes + 255) & ~(size_t)255;
+        memcpy(pin + pro_o, pro.data(), pro.size() * sizeof(bitz::BatchPrologue));
+        memcpy(pin + pro_m, pro_map.data(), pro_map.size() * 4);
+        for (int k = 0; k < kmax; ++k) {
+            memcpy(pin + rd_o[k], rounds[k].data(), rounds[k].size() * sizeof(bitz::RoundDesc));
+            memcpy(pin + rd_m[k], round_map[k].data(), round_map[k].size() * 4);
+        }
+        memcpy(pin + tl_o, tails.data(), tails.size() * sizeof(bitz::LdsStage));
+        memcpy(pin + tl_m, tail_map.data(), tail_map.size() * 4);
         for (uint32_t i : members)
-            memcpy(pin + ch_bytes_off + ch_off[i] * sizeof(f128), inst[i].challenges,
-                   inst[i].num_vars * sizeof(f128));
+            memcpy(pin + ch_o + ch_off[i] * sizeof(f128), inst[i].challenges, inst[i].num_vars * sizeof(f128));
         TRY(desc.alloc(std::max<size_t>(desc_bytes, 16)));
         HIP_TRY(hipMemcpyAsync(desc.p, pin, desc_bytes, hipMemcpyHostToDevice, s));
         if (ch_total)
-            HIP_TRY(hipMemcpyAsync(chal.p, pin + ch_bytes_off, ch_total * sizeof(f128), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(chal.p, pin + ch_o, ch_total * sizeof(f128), hipMemcpyHostToDevice, s));
+        (void)up_bytes;
         static bool attr = false;
         if (!attr) {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_batch),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)bitz::stage_smem(bitz::kStageMaxLog));
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_fold),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
             attr = true;
         }
+        const char* dp = static_cast<const char*>(desc.p);
         Timed tp;
         tp.begin(s, "outer_batch_prologue", 0.0);
-        hipLaunchKernelGGL(bitz::k_batch_prologue, dim3(pro_blocks), dim3(1024), 0, s,
-                           static_cast<const bitz::BatchPrologue*>(desc.p), (int)pro.size());
+        hipLaunchKernelGGL(bitz::k_batch_prologue, dim3((unsigned)pro_map.size()), dim3(1024), 0, s,
+                           reinterpret_cast<const bitz::BatchPrologue*>(dp + pro_o),
+                           reinterpret_cast<const uint32_t*>(dp + pro_m));
         tp.end(s);
         TRY(check_launch());
-        std::vector<Timed> ts(n_stages);
-        const size_t smem = bitz::stage_smem(bitz::kStageMaxLog);
-        for (size_t j = 0; j < n_stages; ++j) {
-            ts[j].begin(s, "outer_batch_stage", 0.0);
-            hipLaunchKernelGGL(bitz::k_outer_batch, dim3(stage_chunks[j]), dim3(256), smem, s,
-                               reinterpret_cast<const bitz::LdsStage*>(static_cast<char*>(desc.p) +
-                                                                        stage_desc_off[j]),
-                               (int)stage_items[j].size());
-            ts[j].end(s);
+        std::vector<Timed> ts(kmax + 1);
+        for (int k = 0; k < kmax; ++k) {
+            const auto* d = reinterpret_cast<const bitz::RoundDesc*>(dp + rd_o[k]);
+            const auto* m = reinterpret_cast<const uint32_t*>(dp + rd_m[k]);
+            const unsigned grid = (unsigned)round_map[k].size();
+            ts[k].begin(s, k == 0 ? "outer_batch_round0" : "outer_batch_fold", 0.0);
+            if (k == 0)
+                hipLaunchKernelGGL(bitz::k_batch_r0, dim3(grid), dim3(bitz::kR0Block), bitz::kR0Smem, s, d, m);
+            else
+                hipLaunchKernelGGL(bitz::k_batch_fold, dim3(grid), dim3(bitz::kFoldBlock), bitz::kFoldSmem, s, d, m);
+            ts[k].end(s);
             TRY(check_launch());
         }
-        uint32_t* host_acc = reinterpret_cast<uint32_t*>(pin + up_bytes);
+        ts[kmax].begin(s, "outer_batch_tail", 0.0);
+        hipLaunchKernelGGL(bitz::k_outer_batch, dim3((unsigned)tails.size()), dim3(bitz::kStageBlock),
+                           bitz::stage_smem(bitz::kStageMaxLog), s,
+                           reinterpret_cast<const bitz::LdsStage*>(dp + tl_o),
+                           reinterpret_cast<const uint32_t*>(dp + tl_m));
+        ts[kmax].end(s);
+        TRY(check_launch());
+        uint32_t* host_acc = reinterpret_cast<uint32_t*>(pin + rb_o);
         HIP_TRY(hipMemcpyAsync(host_acc, acc.p, acc_words * 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         tp.collect();
@@ -1925,7 +1931,6 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
             const auto& x = inst[i];
             std::vector<bitz_f128> co;
             const bool ok = outer_validate_eq(h, h->host_words() + h->eq_off()) &&
-                            (batch_scales(h, plans[i]), true) &&
                             outer_assemble_eq(h, x.challenges, F(x.initial_claim), q1mask[i], !(q1mask[i] & 1), co);
             if (!ok) {
                 alone.push_back(i);

@@ -55,6 +55,15 @@ __device__ __forceinline__ g4 g_load_hbm(const f128* p) {
     r.w[0] = v.x; r.w[1] = v.y; r.w[2] = v.z; r.w[3] = v.w;
     return r;
 }
+__device__ __forceinline__ uint4 u4_load_hbm(const uint4* p) {
+    const u32x4 v = *(const __attribute__((address_space(1))) u32x4*)(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void g_store_hbm(f128* p, const g4& a) {
+    u32x4 v;
+    v.x = a.w[0]; v.y = a.w[1]; v.z = a.w[2]; v.w = a.w[3];
+    *(__attribute__((address_space(1))) u32x4*)(p) = v;
+}
 __device__ __forceinline__ void g_store(f128* p, const g4& a) {
     *reinterpret_cast<uint4*>(p) = make_uint4(a.w[0], a.w[1], a.w[2], a.w[3]);
 }

@@ -39,7 +39,9 @@ __device__ __forceinline__ void block_xor_to_global(g4 (&v)[K], uint32_t* dst, i
     for (int j = threadIdx.x; j < 4 * K; j += nthreads) {
         uint32_t x = 0;
         for (int w = 0; w < waves; ++w) x ^= red[w][j];
-        if (x) atomicXor(dst + j, x);
+        if (x)
+            __scoped_atomic_fetch_xor((__attribute__((address_space(1))) uint32_t*)(dst + j), x, __ATOMIC_RELAXED,
+                                      __MEMORY_SCOPE_DEVICE);
     }
 }
 
@@ -627,31 +629,31 @@ __global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b,
 // wave owning 16 KiB of lane tables, H(i) through a 4-bit constant table
 // (8 KiB) copied per iteration.  Pair y = i * threads + t as in k_outer_eq.
 template <bool Q1>
-__global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
-                                                  size_t iters, const f128* la, int la_bits, int la_shift,
-                                                  const f128* lb, int lb_shift, const uint4* h_tabs4,
-                                                  uint32_t* acc) {
+__device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, const f128* c, size_t threads,
+                                              size_t iters, const f128* la, int la_bits, int la_shift,
+                                              const f128* lb, int lb_shift, const uint4* h_tabs4, uint32_t* acc,
+                                              size_t block) {
     extern __shared__ uint4 smem_raw[];
     uint4* htab = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t t = block * blockDim.x + threadIdx.x;
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {
         if (iters > 1) {
             __syncthreads();  // the previous iteration's readers are done with htab
             for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
-                htab[e] = h_tabs4[i * kConstTable4Entries + e];
+                htab[e] = u4_load_hbm(h_tabs4 + i * kConstTable4Entries + e);
             __syncthreads();
         }
         if (t < threads) {
             const size_t y = i * threads + t;
-            const g4 a0 = g_load(a + 2 * y), a1 = g_load(a + 2 * y + 1);
-            const g4 b0 = g_load(b + 2 * y), b1 = g_load(b + 2 * y + 1);
-            const g4 p0 = g_add(g_mul_lane(a0, b0, col), g_load(c + 2 * y));
+            const g4 a0 = g_load_hbm(a + 2 * y), a1 = g_load_hbm(a + 2 * y + 1);
+            const g4 b0 = g_load_hbm(b + 2 * y), b1 = g_load_hbm(b + 2 * y + 1);
+            const g4 p0 = g_add(g_mul_lane(a0, b0, col), g_load_hbm(c + 2 * y));
             const g4 pi = g_mul_lane(g_add(a0, a1), g_add(b0, b1), col);
             g4 p1;
-            if (Q1) p1 = g_add(g_mul_lane(a1, b1, col), g_load(c + 2 * y + 1));
+            if (Q1) p1 = g_add(g_mul_lane(a1, b1, col), g_load_hbm(c + 2 * y + 1));
             if (iters > 1) {
                 s0 = g_add(s0, g_mul_table4(htab, p0));
                 si = g_add(si, g_mul_table4(htab, pi));
@@ -666,12 +668,19 @@ __global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, 
     g4 v[3] = {g_zero(), g_zero(), g_zero()};
     if (t < threads) {
         const size_t mask = ((size_t)1 << la_bits) - 1;
-        const g4 L = g_mul_lane(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)), col);
+        const g4 L = g_mul_lane(g_load_hbm(la + ((t & mask) << la_shift)), g_load_hbm(lb + ((t >> la_bits) << lb_shift)), col);
         v[0] = g_mul_lane(L, s0, col);
         v[1] = g_mul_lane(L, si, col);
         if (Q1) v[2] = g_mul_lane(L, s1, col);
     }
     block_xor_to_global<3>(v, acc);
+}
+template <bool Q1>
+__global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
+                                                  size_t iters, const f128* la, int la_bits, int la_shift,
+                                                  const f128* lb, int lb_shift, const uint4* h_tabs4,
+                                                  uint32_t* acc) {
+    outer_r0_body<Q1>(a, b, c, threads, iters, la, la_bits, la_shift, lb, lb_shift, h_tabs4, acc, blockIdx.x);
 }
 
 #define OUTER_R0_SIG                                                                                        \
@@ -686,25 +695,24 @@ template __global__ void k_outer_r0<true>(OUTER_R0_SIG);
 // (g_mul_lane) and H(i) through 4-bit tables.  512-thread workgroups: eight
 // waves x 16 KiB of lane tables plus the two 8 KiB constant tables.
 template <bool Q1>
-__global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao,
-                                                    f128* bo, f128* co, size_t threads, size_t iters,
-                                                    const uint4* r_tab4, const f128* la, int la_bits,
-                                                    int la_shift, const f128* lb, int lb_shift,
-                                                    const uint4* h_tabs4, uint32_t* acc) {
+__device__ __forceinline__ void outer_fold_body(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo,
+                                                f128* co, size_t threads, size_t iters, const uint4* r_tab4,
+                                                const f128* la, int la_bits, int la_shift, const f128* lb,
+                                                int lb_shift, const uint4* h_tabs4, uint32_t* acc, size_t block) {
     extern __shared__ uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* ht = smem_raw + kConstTable4Entries;
     uint4* col = smem_raw + 2 * kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
                  (threadIdx.x & 63);
     lane_table_clear(col);
-    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = r_tab4[e];
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = u4_load_hbm(r_tab4 + e);
+    const size_t t = block * blockDim.x + threadIdx.x;
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {
         if (iters > 1) {
             __syncthreads();  // the previous iteration's readers are done with ht
             for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
-                ht[e] = h_tabs4[i * kConstTable4Entries + e];
+                ht[e] = u4_load_hbm(h_tabs4 + i * kConstTable4Entries + e);
         }
         __syncthreads();
         if (t < threads) {
@@ -712,9 +720,9 @@ __global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b
             g4 in[3][4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                in[0][j] = g_load(a + 4 * y + j);
-                in[1][j] = g_load(b + 4 * y + j);
-                in[2][j] = g_load(c + 4 * y + j);
+                in[0][j] = g_load_hbm(a + 4 * y + j);
+                in[1][j] = g_load_hbm(b + 4 * y + j);
+                in[2][j] = g_load_hbm(c + 4 * y + j);
             }
             g4 f[3][2];
 #pragma unroll
@@ -722,12 +730,12 @@ __global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
                     f[u][j] = g_add(in[u][2 * j], g_mul_table4(rt, g_add(in[u][2 * j], in[u][2 * j + 1])));
-            g_store(ao + 2 * y, f[0][0]);
-            g_store(ao + 2 * y + 1, f[0][1]);
-            g_store(bo + 2 * y, f[1][0]);
-            g_store(bo + 2 * y + 1, f[1][1]);
-            g_store(co + 2 * y, f[2][0]);
-            g_store(co + 2 * y + 1, f[2][1]);
+            g_store_hbm(ao + 2 * y, f[0][0]);
+            g_store_hbm(ao + 2 * y + 1, f[0][1]);
+            g_store_hbm(bo + 2 * y, f[1][0]);
+            g_store_hbm(bo + 2 * y + 1, f[1][1]);
+            g_store_hbm(co + 2 * y, f[2][0]);
+            g_store_hbm(co + 2 * y + 1, f[2][1]);
             const g4 p0 = g_add(g_mul_lane(f[0][0], f[1][0], col), f[2][0]);
             const g4 pi = g_mul_lane(g_add(f[0][0], f[0][1]), g_add(f[1][0], f[1][1]), col);
             g4 p1;
@@ -747,12 +755,21 @@ __global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b
     if (t < threads) {
         const size_t mask = ((size_t)1 << la_bits) - 1;
         const g4 L =
-            g_mul_lane(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)), col);
+            g_mul_lane(g_load_hbm(la + ((t & mask) << la_shift)), g_load_hbm(lb + ((t >> la_bits) << lb_shift)), col);
         v[0] = g_mul_lane(L, s0, col);
         v[1] = g_mul_lane(L, si, col);
         if (Q1) v[2] = g_mul_lane(L, s1, col);
     }
     block_xor_to_global<3>(v, acc);
+}
+template <bool Q1>
+__global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao,
+                                                    f128* bo, f128* co, size_t threads, size_t iters,
+                                                    const uint4* r_tab4, const f128* la, int la_bits,
+                                                    int la_shift, const f128* lb, int lb_shift,
+                                                    const uint4* h_tabs4, uint32_t* acc) {
+    outer_fold_body<Q1>(a, b, c, ao, bo, co, threads, iters, r_tab4, la, la_bits, la_shift, lb, lb_shift, h_tabs4,
+                        acc, blockIdx.x);
 }
 
 #define OUTER_FOLD_SIG                                                                                    \
@@ -1030,18 +1047,10 @@ __global__ __launch_bounds__(kStageBlock) void k_outer_chunk(LdsStage A) { lds_s
 __global__ __launch_bounds__(kStageBlock) void k_outer_tail(LdsStage A) { lds_stage(A, 0); }
 
 // ======================================= batches of independent proofs ===
-__global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* items, int count) {
+__global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* items, const uint32_t* bmap) {
     __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].block_base <= blockIdx.x)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const BatchPrologue& P = items[lo];
+    const BatchPrologue& P = items[bmap[blockIdx.x]];
     const unsigned b = blockIdx.x - P.block_base;
     if (b == 0)
         for (unsigned e = threadIdx.x; e < P.zero_words; e += blockDim.x) P.zero[e] = 0;
@@ -1050,24 +1059,43 @@ __global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* it
         return;
     }
     const unsigned t = b - P.eq_blocks;
+    if (t >= (unsigned)P.n) {
+        h4_block(P.L, P.ell, P.H, P.hdesc, P.htabs4, t - P.n, tab, basis);
+        return;
+    }
     build_const_table4(tab, basis, g_load(P.chal + t));
     uint4* dst = P.rtabs4 + (size_t)t * kConstTable4Entries;
     for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
 }
 
-// One workgroup per (proof, chunk) of a batch stage: the stage of the proof
-// whose chunk_base is the last one at or below the workgroup index.
-__global__ __launch_bounds__(kStageBlock) void k_outer_batch(const LdsStage* stages, int count) {
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (stages[mid].chunk_base <= blockIdx.x)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const LdsStage A = stages[lo];
+// One workgroup per (proof, chunk) of a batch stage; bmap gives each
+// workgroup's stage descriptor.
+__global__ __launch_bounds__(kStageBlock) void k_outer_batch(const LdsStage* stages, const uint32_t* bmap) {
+    const LdsStage A = stages[bmap[blockIdx.x]];
     lds_stage(A, blockIdx.x - A.chunk_base);
+}
+
+// Grid-wide rounds of a batch: one launch per round over every proof still
+// in its grid-wide rounds, workgroup -> descriptor through bmap.
+__global__ __launch_bounds__(kR0Block) void k_batch_r0(const RoundDesc* d, const uint32_t* bmap) {
+    const RoundDesc& A = d[bmap[blockIdx.x]];
+    const size_t block = blockIdx.x - A.block_base;
+    if (A.q1)
+        outer_r0_body<true>(A.a, A.b, A.c, A.threads, A.iters, A.la, A.la_bits, A.la_shift, A.lb, A.lb_shift,
+                            A.h_tabs4, A.acc, block);
+    else
+        outer_r0_body<false>(A.a, A.b, A.c, A.threads, A.iters, A.la, A.la_bits, A.la_shift, A.lb, A.lb_shift,
+                             A.h_tabs4, A.acc, block);
+}
+__global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, const uint32_t* bmap) {
+    const RoundDesc& A = d[bmap[blockIdx.x]];
+    const size_t block = blockIdx.x - A.block_base;
+    if (A.q1)
+        outer_fold_body<true>(A.a, A.b, A.c, A.ao, A.bo, A.co, A.threads, A.iters, A.r_tab4, A.la, A.la_bits,
+                              A.la_shift, A.lb, A.lb_shift, A.h_tabs4, A.acc, block);
+    else
+        outer_fold_body<false>(A.a, A.b, A.c, A.ao, A.bo, A.co, A.threads, A.iters, A.r_tab4, A.la, A.la_bits,
+                               A.la_shift, A.lb, A.lb_shift, A.h_tabs4, A.acc, block);
 }
 
 }  // namespace bitz

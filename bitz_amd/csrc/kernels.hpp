@@ -135,8 +135,30 @@ constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
 constexpr size_t stage_smem(int cap_lg) { return sizeof(uint4) * (((size_t)4 << cap_lg) + kLaneTables256); }
 __global__ void k_outer_chunk(LdsStage args);
 __global__ void k_outer_tail(LdsStage args);
-__global__ void k_outer_batch(const LdsStage* stages, int count);
+__global__ void k_outer_batch(const LdsStage* stages, const uint32_t* bmap);
 constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
+
+// One proof's grid-wide round in a batched launch: the arguments of
+// k_outer_r0 / k_outer_fold, the first workgroup of the proof in the
+// launch, and q1 (compute q(1) too).
+struct RoundDesc {
+    const f128* a;
+    const f128* b;
+    const f128* c;
+    f128* ao;
+    f128* bo;
+    f128* co;
+    const uint4* r_tab4;
+    const f128* la;
+    const f128* lb;
+    const uint4* h_tabs4;
+    uint32_t* acc;
+    size_t threads, iters;
+    int la_bits, la_shift, lb_shift, q1;
+    uint32_t block_base;
+};
+__global__ void k_batch_r0(const RoundDesc* d, const uint32_t* bmap);
+__global__ void k_batch_fold(const RoundDesc* d, const uint32_t* bmap);
 
 // Prologue of a batch: per instance, clearing of its sums, the factor check
 // (k_eq_check layout at eq_out) and the 4-bit tables of its challenges.
@@ -150,8 +172,10 @@ struct BatchPrologue {
     uint32_t* eq_out;
     const f128* chal;  // device array, n entries
     uint4* rtabs4;
-    uint32_t block_base;  // first workgroup: eq check, then n table builds
+    HtabDesc hdesc;  // H(i) tables of the multi-iteration grid-wide rounds
+    uint4* htabs4;
+    uint32_t block_base;  // first workgroup: eq check, then n challenge tables, then H(i) tables
 };
-__global__ void k_batch_prologue(const BatchPrologue* items, int count);
+__global__ void k_batch_prologue(const BatchPrologue* items, const uint32_t* bmap);
 
 }  // namespace bitz
