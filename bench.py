@@ -84,6 +84,9 @@ def main() -> int:
                     help="outer: one 2^log_n outer sumcheck per step (default); batch: 256 outer sumchecks "
                          "of 2^12..2^22 elements per step, batched")
     ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batch: launch the kernels directly instead of replaying the captured graph "
+                         "(for kernel tracers that do not follow graph launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device twice (the known-size reference for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -217,8 +220,10 @@ def main() -> int:
 def bench_batch(G, args) -> int:
     """256 independent outer sumchecks, num_vars uniform in [12, 22] from the
     seed, each with A, B, C, tau and challenges as the single workload.  A
-    step proves the whole batch in one bitz_prove_outer_batch call; the same
-    instances proved one call each give the plain single-instance rate."""
+    step proves the whole batch in one bitz_prove_outer_batch call (a
+    replay of the captured launch graph, tile shapes from the autotuner,
+    which runs during the first warmup call); the same instances proved one
+    call each give the plain single-instance rate."""
     G.init(0)
     rng = np.random.default_rng(args.seed)
     sizes = [int(x) for x in rng.integers(12, 23, size=args.batch)]
@@ -245,19 +250,27 @@ def bench_batch(G, args) -> int:
         ch = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
         insts.append((ch, claim, eq, prods))
     folded = sum(3 * ((1 << n) - 1) for n in sizes)
+    graph = not args.batch_eager
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        G.prove_outer_batch(insts)
+        G.prove_outer_batch(insts, graph=graph)
         log(f"bench: warmup {i + 1}: {(time.perf_counter() - t0) * 1e3:.2f} ms")
     G.synchronize()
     times = []
     t_start = time.perf_counter()
     for i in range(args.steps):
         t0 = time.perf_counter()
-        G.prove_outer_batch(insts)
+        G.prove_outer_batch(insts, graph=graph)
         times.append(time.perf_counter() - t0)
     G.synchronize()
     wall = time.perf_counter() - t_start
+    tile = G.outer_batch_config()
+    # The same batch with the kernels launched directly (no graph).
+    t0 = time.perf_counter()
+    for _ in range(2):
+        G.prove_outer_batch(insts, graph=False)
+    G.synchronize()
+    eager = (time.perf_counter() - t0) / 2
     # The plain path: the same instances, one pipelined proof per call.
     t0 = time.perf_counter()
     for ch, claim, eq, prods in insts:
@@ -282,6 +295,9 @@ def bench_batch(G, args) -> int:
         "config": {"workload": f"{args.batch} outer sumchecks, num_vars uniform in [12, 22] (seed {args.seed}), "
                                "challenges fixed in advance, one batched call per step",
                    "instances": args.batch, "elements_per_table": sum(1 << n for n in sizes)},
+        "tile_shapes": {"thread_bits": tile[0], "tail_log": tile[1], "chosen_by": "autotuner"},
+        "launch": "eager" if args.batch_eager else "captured graph",
+        "eager_launch_ms_per_step": eager * 1e3,
         "plain_single_instance": {"ms_for_all_instances": plain * 1e3,
                                   "elements_folded_per_s": folded / plain},
         "speedup_vs_plain": (folded * args.steps / wall) / (folded / plain),

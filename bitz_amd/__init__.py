@@ -88,6 +88,9 @@ def lib():
             "bitz_profile_enable": (I32, [I32]),
             "bitz_profile_only": (I32, [C.c_char_p]),
             "bitz_prove_outer_batch": (I32, [P, U32, U32]),
+            "bitz_outer_batch_config": (I32, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+            "bitz_outer_batch_release": (None, []),
+            "bitz_outer_batch_set_config": (I32, [U32, U32]),
             "bitz_profile_read": (I32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_double)]),
             "bitz_transcript_callbacks": (P, [P, P, P]),
@@ -469,13 +472,19 @@ class _OuterInstance(C.Structure):
                 ("final_claim", C.c_void_p), ("claims", C.c_void_p)]
 
 
-def prove_outer_batch(instances) -> list:
+BATCH_EAGER = 2
+BATCH_NO_TUNE = 4
+
+
+def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
     """Many independent outer sumchecks with challenges fixed in advance, in
     one batched call (bitz_prove_outer_batch).  instances: iterable of
     (challenges, initial_claim, (eq_low, eq_high), R1csProductMles) with
     DeviceTables for the products and factors.  Returns one
     OuterSumcheckOutput per instance, equal to
-    prove_outer_sumcheck_with_challenges on the same inputs."""
+    prove_outer_sumcheck_with_challenges on the same inputs.  graph=False
+    launches the kernels directly instead of replaying the captured graph;
+    tune=False keeps the default tile shapes."""
     init()
     instances = list(instances)
     arr = (_OuterInstance * max(len(instances), 1))()
@@ -500,9 +509,29 @@ def prove_outer_batch(instances) -> list:
         x.final_claim, x.claims = _p(final).value, _p(claims).value
         keep.append((chs, lo, hi, rounds, points, final, claims, az, bz, cz))
         outs.append((n, rounds, points, final, claims))
-    _check(lib().bitz_prove_outer_batch(C.cast(arr, C.c_void_p), len(instances), DEVICE), SumcheckError)
+    flags = DEVICE | (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
+    _check(lib().bitz_prove_outer_batch(C.cast(arr, C.c_void_p), len(instances), flags), SumcheckError)
     return [OuterSumcheckOutput(r[: 4 * n].reshape(n, 4, 2), p[:n], to_int(f[0]), to_int(c[0]), to_int(c[1]),
                                 to_int(c[2])) for n, r, p, f, c in outs]
+
+
+def outer_batch_config() -> tuple:
+    """(thread_bits, tail_log) of the last batch: the tile shapes the
+    autotuner chose (or the defaults)."""
+    tb, tl = C.c_uint32(), C.c_uint32()
+    _check(lib().bitz_outer_batch_config(C.byref(tb), C.byref(tl)))
+    return tb.value, tl.value
+
+
+def outer_batch_set_config(thread_bits: int = 0, tail_log: int = 0) -> None:
+    """Fixes the tile shapes of later batches; thread_bits=0 returns to the
+    autotuner."""
+    _check(lib().bitz_outer_batch_set_config(thread_bits, tail_log))
+
+
+def outer_batch_release() -> None:
+    """Frees the cached batch plan (device buffers, captured graph)."""
+    lib().bitz_outer_batch_release()
 
 
 def outer_claim(eq_factors, products: R1csProductMles) -> int:

@@ -8,9 +8,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -541,6 +543,11 @@ extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, 
 }
 
 // ======================================================= outer sumcheck ===
+// Rounds whose tables hold at most 2^kTailLog elements run in the
+// single-workgroup tail kernel (pipelined prover only).
+constexpr int kTailLog = 10;
+static_assert(kTailLog + 1 <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
+
 struct bitz_outer {
     int n = 0;
     size_t N = 0, len = 0, low_len = 0, high_len = 0;
@@ -612,6 +619,7 @@ struct bitz_outer {
     }
     // A batch member's sums live in the batch's shared buffers.
     bool batch_member = false;
+    int tail_log = kTailLog;  // a batch member's grid-wide rounds end at n - tail_log
     const uint32_t* host_ext = nullptr;
     const uint32_t* host_words() const {
         return host_ext ? host_ext : static_cast<const uint32_t*>(host.p);
@@ -664,17 +672,14 @@ static f128 f128_at(const uint32_t* w) {
     return x;
 }
 
-// Rounds whose tables hold at most 2^kTailLog elements run in the
-// single-workgroup tail kernel (pipelined prover only).
-constexpr int kTailLog = 10;
-static_assert(kTailLog + 1 <= bitz::kTailMaxLog, "tail tables must fit the kernel's LDS budget");
 
 // Plans the eq-factored mode (round geometry, H(i) tables) and allocates
 // its buffers.  Returns false (general kernels) when the factors are too
 // large to check cheaply.  The check itself is launched by
 // outer_launch_checks or, in the pipelined prover, by its prologue; nothing
 // reads back here: the verdict arrives with the first round sums.
-static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batch_member = false) {
+static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batch_member = false,
+                           int thread_bits = bitz::kThreadBits) {
     const int n = h->n;
     if (n == 0) return false;
     const size_t nl = (size_t)1 << low_vars, nh = (size_t)1 << high_vars;
@@ -689,7 +694,7 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batc
         const int pbits = n - k - 1;  // log2 pairs of round k
         // log2 threads: one wave of lane-table workgroups across the chip;
         // larger rounds iterate.
-        const int tbits = std::min(pbits, bitz::kThreadBits);
+        const int tbits = std::min(pbits, thread_bits);
         auto& g = h->geo[k];
         g.threads = (size_t)1 << tbits;
         g.iters = (size_t)1 << (pbits - tbits);
@@ -767,8 +772,8 @@ static int32_t outer_launch_checks(bitz_outer* h) {
 struct PipeSchedule {
     int kc, k0;
 };
-static PipeSchedule pipe_schedule(int n) {
-    const int k0 = std::max(0, n - kTailLog);
+static PipeSchedule pipe_schedule(int n, int tail_log = kTailLog) {
+    const int k0 = std::max(0, n - tail_log);
     const int kc = std::max(1, n - 1 - bitz::kThreadBits);  // rounds before kc iterate over their pairs
     if (k0 - kc >= 1) return {kc, k0};
     return {k0, k0};
@@ -869,7 +874,7 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     }
     // Rounds of the pipelined prover's LDS stages (chunk and tail).
     h->tau = tau;
-    const PipeSchedule ps = pipe_schedule(n);
+    const PipeSchedule ps = pipe_schedule(n, h->tail_log);
     if (ps.kc < ps.k0) stage_constants(h, ps.kc, ps.k0 - ps.kc, bitz::kChunkMaxLog - 2, kap.data() + n);
     stage_constants(h, ps.k0, n - ps.k0, n - ps.k0 - 1, kap.data() + n);
     batch_inverse(kap);
@@ -1459,7 +1464,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
 static bool outer_assemble_eq(bitz_outer* h, const bitz_f128* challenges, f128 initial_claim,
                               uint64_t q1mask, bool check_claim, std::vector<bitz_f128>& co) {
     // Grid-wide rounds before kc (a batch member's before k0: no chunk stage).
-    const int n = h->n, kc = h->batch_member ? pipe_schedule(n).k0 : pipe_schedule(n).kc;
+    const int n = h->n, kc = h->batch_member ? pipe_schedule(n, h->tail_log).k0 : pipe_schedule(n).kc;
     const uint32_t* w = h->host_words();
     for (int k = 0; k < n; ++k)
         if (!((q1mask >> k) & 1) && hostf::is_zero(h->tau[k])) return false;
@@ -1628,41 +1633,28 @@ done:
 
 // ============================================ batched outer sumchecks ===
 // Many independent eq-factored outer sumchecks with challenges fixed in
-// advance, proved together: one prologue launch for every instance, then
-// stages of k_outer_batch in which every (instance, chunk) is a workgroup
-// (up to nine rounds of every instance per launch), then one read-back.
+// advance, proved together: one prologue launch for every instance (factor
+// checks, challenge and H(i) tables), one launch per grid-wide round over
+// every instance still in its grid-wide rounds (k_batch_r0, k_batch_fold),
+// one launch of one-workgroup tails (k_outer_batch), then one read-back.
+// The launches are captured in a graph and replayed while the shapes repeat.
 // Each instance's result is exactly bitz_prove_outer_sumcheck_with_challenges';
 // an instance whose optimistic pass is rejected (factors that are not eq
 // tables, tau_k = 0, a wrong claim) is proved again on its own.
 
 namespace {
 
-// Host staging that outlives a call: pinned, grown on demand, one batch at a
-// time.
-struct BigPinned {
-    std::mutex mu;
-    void* p = nullptr;
-    size_t bytes = 0;
-    int32_t reserve(size_t n) {
-        if (n <= bytes) return BITZ_OK;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
-        }
-        bytes = n;
-        return BITZ_OK;
-    }
-} g_batch_pinned;
-
 // A batch member: the eq-factored state of one instance (geometry, tau and
 // constants after validation, coefficient recovery) without scratch of its
-// own.
-int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in) {
+// own.  Grid-wide rounds run cfg.thread_bits-bit thread grids and end at
+// n - cfg.tail_log.
+struct BatchConfig {
+    int thread_bits, tail_log;
+};
+int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in, const BatchConfig& cfg) {
     auto* h = new bitz_outer;
     h->batch_member = true;
+    h->tail_log = cfg.tail_log;
     h->n = (int)in.num_vars;
     h->N = h->len = (size_t)1 << in.num_vars;
     h->low_len = (size_t)1 << in.low_vars;
@@ -1674,8 +1666,388 @@ int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in) {
     h->eq_in[1] = reinterpret_cast<const f128*>(in.eq_high);
     h->eq_bl = eq_check_blocks(h->low_len, 1024);
     h->eq_bh = eq_check_blocks(h->high_len, 1024);
-    h->eqmode = outer_setup_eq(h, (int)in.low_vars, (int)in.high_vars, true);
+    h->eqmode = outer_setup_eq(h, (int)in.low_vars, (int)in.high_vars, true, cfg.thread_bits);
     *out = h;
+    return BITZ_OK;
+}
+
+// Tile shapes the autotuner tries for a batch: the thread-grid cap of the
+// grid-wide rounds (larger rounds iterate, weighing each iteration through
+// its H(i) table) and the size of the one-workgroup tail.
+constexpr BatchConfig kBatchDefault = {bitz::kThreadBits, kTailLog};
+constexpr BatchConfig kBatchCandidates[] = {{17, 10}, {16, 10}, {15, 10}, {14, 10}, {17, 9}, {16, 9}, {15, 9}};
+
+// Everything a batch of a given shape needs, built once and reused while
+// the shapes repeat: member handles (geometry), buffer layout, device
+// buffers, pinned staging, the block maps, and the captured launch graph.
+// A call refills the descriptors (pointers, q(1) flags) and the challenges
+// in the pinned staging; the graph's first node copies them to the device.
+struct BatchPlan {
+    std::vector<uint32_t> key;  // per instance: num_vars, low_vars, high_vars
+    BatchConfig cfg{};
+    std::vector<std::unique_ptr<bitz_outer, void (*)(bitz_outer*)>> hs;
+    std::vector<uint32_t> members, alone;
+    std::vector<int> k0;
+    std::vector<size_t> acc_off, ch_off, ht_off, p_off, q_off;
+    size_t acc_words = 0, ch_total = 0, ht_total = 0, p_total = 0, q_total = 0;
+    int kmax = 0;
+    Dev acc, chal, rt4, ht4, pbuf, qbuf, desc;
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    // staging layout: [maps][descriptors][challenges][readback]
+    size_t pro_m = 0, tl_m = 0, pro_o = 0, tl_o = 0, ch_o = 0, rb_o = 0;
+    size_t maps_end = 0, up_end = 0;
+    std::vector<size_t> rd_o, rd_m;
+    unsigned pro_grid = 0, tail_grid = 0;
+    std::vector<unsigned> round_grid;
+    std::vector<size_t> round_count;
+    StreamOwner so;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchPlan() {
+        if (so.s) (void)hipStreamSynchronize(so.s);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (pin) (void)hipHostFree(pin);
+    }
+    uint32_t* host_acc() const { return reinterpret_cast<uint32_t*>(static_cast<char*>(pin) + rb_o); }
+};
+
+struct BatchCache {
+    std::mutex mu;  // one batch at a time
+    std::unique_ptr<BatchPlan> plan;
+    std::map<std::vector<uint32_t>, BatchConfig> tuned;
+    BatchConfig last{0, 0};
+    BatchConfig forced{0, 0};  // set by bitz_outer_batch_set_config
+} g_batch;
+
+std::vector<uint32_t> batch_key(const bitz_outer_instance* inst, uint32_t count) {
+    std::vector<uint32_t> key;
+    key.reserve(3 * (size_t)count);
+    for (uint32_t i = 0; i < count; ++i) {
+        key.push_back(inst[i].num_vars);
+        key.push_back(inst[i].low_vars);
+        key.push_back(inst[i].high_vars);
+    }
+    return key;
+}
+
+// Round k's tables of member i: the inputs for k = 0, then the output of
+// round k's fold by r_{k-1} (odd rounds in P, even rounds in Q).
+const f128* batch_tables(const BatchPlan& P, uint32_t i, int k, int t) {
+    if (k == 0) return P.hs[i]->inputs[t];
+    if (k & 1) return P.pbuf.f() + t * P.p_total + P.p_off[i];
+    return P.qbuf.f() + t * P.q_total + P.q_off[i];
+}
+
+int32_t batch_plan_build(BatchPlan& P, const bitz_outer_instance* inst, uint32_t count, const BatchConfig& cfg) {
+    P.key = batch_key(inst, count);
+    P.cfg = cfg;
+    for (uint32_t i = 0; i < count; ++i) {
+        bitz_outer* h = nullptr;
+        TRY(batch_member(&h, inst[i], cfg));
+        P.hs.emplace_back(h, [](bitz_outer* p) { delete p; });
+        if (h->eqmode && h->n >= 1 && h->n <= bitz::kMaxRounds)
+            P.members.push_back(i);
+        else
+            P.alone.push_back(i);
+    }
+    if (P.members.empty()) return BITZ_OK;
+    TRY(P.so.create());
+    P.k0.assign(count, 0);
+    P.acc_off.assign(count, 0);
+    P.ch_off.assign(count, 0);
+    P.ht_off.assign(count, 0);
+    P.p_off.assign(count, 0);
+    P.q_off.assign(count, 0);
+    for (uint32_t i : P.members) {
+        bitz_outer* h = P.hs[i].get();
+        P.k0[i] = pipe_schedule(h->n, cfg.tail_log).k0;
+        P.kmax = std::max(P.kmax, P.k0[i]);
+        P.acc_off[i] = P.acc_words;
+        P.acc_words += (h->acc_words() + 63) & ~(size_t)63;
+        P.ch_off[i] = P.ch_total;
+        P.ch_total += (size_t)h->n;
+        P.ht_off[i] = P.ht_total;
+        P.ht_total += h->htab_total;
+        P.p_off[i] = P.p_total;
+        if (P.k0[i] >= 2) P.p_total += h->N / 2;
+        P.q_off[i] = P.q_total;
+        if (P.k0[i] >= 3) P.q_total += h->N / 4;
+    }
+    TRY(P.acc.alloc(P.acc_words * 4));
+    TRY(P.chal.alloc(std::max<size_t>(P.ch_total, 1) * sizeof(f128)));
+    TRY(P.rt4.alloc(std::max<size_t>(P.ch_total, 1) * kConstTable4Entries * sizeof(uint4)));
+    TRY(P.ht4.alloc(std::max<size_t>(P.ht_total, 1) * kConstTable4Entries * sizeof(uint4)));
+    TRY(P.pbuf.alloc(std::max<size_t>(3 * P.p_total, 1) * sizeof(f128)));
+    TRY(P.qbuf.alloc(std::max<size_t>(3 * P.q_total, 1) * sizeof(f128)));
+    // Block maps: workgroup -> descriptor of every launch.
+    std::vector<uint32_t> pro_map, tail_map;
+    std::vector<std::vector<uint32_t>> round_map(P.kmax);
+    P.round_count.assign(P.kmax, 0);
+    for (uint32_t j = 0; j < (uint32_t)P.members.size(); ++j) {
+        bitz_outer* h = P.hs[P.members[j]].get();
+        pro_map.insert(pro_map.end(), h->eq_bl + h->eq_bh + (unsigned)h->n + h->htab_total, j);
+        tail_map.push_back(j);
+    }
+    for (int k = 0; k < P.kmax; ++k) {
+        const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
+        for (uint32_t i : P.members) {
+            if (k >= P.k0[i]) continue;
+            const auto& g = P.hs[i]->geo[k];
+            round_map[k].insert(round_map[k].end(), (g.threads + block - 1) / block, (uint32_t)P.round_count[k]);
+            P.round_count[k] += 1;
+        }
+    }
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t o = off;
+        off += bytes;
+        return o;
+    };
+    P.pro_m = place(pro_map.size() * 4);
+    P.rd_m.resize(P.kmax);
+    for (int k = 0; k < P.kmax; ++k) P.rd_m[k] = place(round_map[k].size() * 4);
+    P.tl_m = place(tail_map.size() * 4);
+    P.maps_end = (off + 255) & ~(size_t)255;
+    P.pro_o = place(P.members.size() * sizeof(bitz::BatchPrologue));
+    P.rd_o.resize(P.kmax);
+    for (int k = 0; k < P.kmax; ++k) P.rd_o[k] = place(P.round_count[k] * sizeof(bitz::RoundDesc));
+    P.tl_o = place(P.members.size() * sizeof(bitz::LdsStage));
+    P.ch_o = place(P.ch_total * sizeof(f128));
+    P.up_end = off;
+    P.rb_o = place(P.acc_words * 4);
+    P.pin_bytes = off;
+    if (hipHostMalloc(&P.pin, P.pin_bytes, hipHostMallocDefault) != hipSuccess) {
+        P.pin = nullptr;
+        return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+    }
+    char* pin = static_cast<char*>(P.pin);
+    memcpy(pin + P.pro_m, pro_map.data(), pro_map.size() * 4);
+    for (int k = 0; k < P.kmax; ++k) memcpy(pin + P.rd_m[k], round_map[k].data(), round_map[k].size() * 4);
+    memcpy(pin + P.tl_m, tail_map.data(), tail_map.size() * 4);
+    TRY(P.desc.alloc(P.up_end));
+    HIP_TRY(hipMemcpyAsync(P.desc.p, pin, P.maps_end, hipMemcpyHostToDevice, P.so.s));
+    HIP_TRY(hipStreamSynchronize(P.so.s));
+    P.pro_grid = (unsigned)pro_map.size();
+    P.tail_grid = (unsigned)tail_map.size();
+    P.round_grid.resize(P.kmax);
+    for (int k = 0; k < P.kmax; ++k) P.round_grid[k] = (unsigned)round_map[k].size();
+    return BITZ_OK;
+}
+
+// Writes this call's descriptors and challenges into the plan's staging.
+void batch_plan_fill(BatchPlan& P, const bitz_outer_instance* inst, const std::vector<uint64_t>& q1mask) {
+    char* pin = static_cast<char*>(P.pin);
+    auto rtab_of = [&](uint32_t i) { return static_cast<uint4*>(P.rt4.p) + P.ch_off[i] * kConstTable4Entries; };
+    for (uint32_t i : P.members) {
+        bitz_outer* h = P.hs[i].get();
+        const auto& x = inst[i];
+        h->inputs[0] = reinterpret_cast<const f128*>(x.az);
+        h->inputs[1] = reinterpret_cast<const f128*>(x.bz);
+        h->inputs[2] = reinterpret_cast<const f128*>(x.cz);
+        h->eq_in[0] = reinterpret_cast<const f128*>(x.eq_low);
+        h->eq_in[1] = reinterpret_cast<const f128*>(x.eq_high);
+        memcpy(pin + P.ch_o + P.ch_off[i] * sizeof(f128), x.challenges, x.num_vars * sizeof(f128));
+    }
+    auto* pro = reinterpret_cast<bitz::BatchPrologue*>(pin + P.pro_o);
+    auto* tails = reinterpret_cast<bitz::LdsStage*>(pin + P.tl_o);
+    uint32_t pro_base = 0;
+    for (uint32_t j = 0; j < (uint32_t)P.members.size(); ++j) {
+        const uint32_t i = P.members[j];
+        bitz_outer* h = P.hs[i].get();
+        bitz::BatchPrologue p{};
+        p.L = h->eq_in[0];
+        p.H = h->eq_in[1];
+        p.ell = h->ell;
+        p.m = h->m;
+        p.n = h->n;
+        p.zero = P.acc.u32() + P.acc_off[i];
+        p.zero_words = (unsigned)h->eq_off();
+        p.eq_blocks_l = h->eq_bl;
+        p.eq_blocks = h->eq_bl + h->eq_bh;
+        p.eq_out = P.acc.u32() + P.acc_off[i] + h->eq_off();
+        p.chal = P.chal.f() + P.ch_off[i];
+        p.rtabs4 = rtab_of(i);
+        p.hdesc = h->hdesc;
+        p.htabs4 = static_cast<uint4*>(P.ht4.p) + P.ht_off[i] * kConstTable4Entries;
+        p.block_base = pro_base;
+        pro_base += p.eq_blocks + (unsigned)h->n + (unsigned)h->htab_total;
+        pro[j] = p;
+        // Tail: round k0-1's tables folded by r_{k0-1} on load (the inputs when k0 = 0).
+        const int kk = P.k0[i];
+        const int src = kk > 0 ? kk - 1 : 0;
+        bitz::LdsStage t{};
+        t.a = batch_tables(P, i, src, 0);
+        t.b = batch_tables(P, i, src, 1);
+        t.c = batch_tables(P, i, src, 2);
+        t.L = h->eq_in[0];
+        t.H = h->eq_in[1];
+        t.rtabs4 = rtab_of(i);
+        t.acc = P.acc.u32() + P.acc_off[i];
+        t.fin = reinterpret_cast<f128*>(P.acc.u32() + P.acc_off[i] + h->fin_off());
+        t.q1mask = q1mask[i];
+        t.ell = h->ell;
+        t.n = h->n;
+        t.k_first = kk;
+        t.s = h->n - kk;
+        t.lg_in = kk > 0 ? h->n - kk + 1 : h->n;
+        t.fold_first = kk > 0;
+        t.final_stage = 1;
+        t.store_sums = 1;
+        t.chunk_base = j;  // one workgroup per member: chunk 0
+        tails[j] = t;
+    }
+    for (int k = 0; k < P.kmax; ++k) {
+        auto* rd = reinterpret_cast<bitz::RoundDesc*>(pin + P.rd_o[k]);
+        const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
+        uint32_t base = 0, c = 0;
+        for (uint32_t i : P.members) {
+            if (k >= P.k0[i]) continue;
+            bitz_outer* h = P.hs[i].get();
+            const auto& g = h->geo[k];
+            bitz::RoundDesc d{};
+            // Round k reads round k-1's tables and folds them by r_{k-1}.
+            const int src = k == 0 ? 0 : k - 1;
+            d.a = batch_tables(P, i, src, 0);
+            d.b = batch_tables(P, i, src, 1);
+            d.c = batch_tables(P, i, src, 2);
+            if (k > 0) {
+                d.ao = const_cast<f128*>(batch_tables(P, i, k, 0));
+                d.bo = const_cast<f128*>(batch_tables(P, i, k, 1));
+                d.co = const_cast<f128*>(batch_tables(P, i, k, 2));
+                d.r_tab4 = rtab_of(i) + (size_t)(k - 1) * kConstTable4Entries;
+            }
+            d.la = h->table_ptr(g.la.table);
+            d.lb = h->table_ptr(g.lb.table);
+            d.la_bits = g.la_bits;
+            d.la_shift = g.la.a - h->table_base(g.la.table);
+            d.lb_shift = g.lb.a - h->table_base(g.lb.table);
+            d.h_tabs4 = static_cast<const uint4*>(P.ht4.p) + (P.ht_off[i] + g.htab_off) * kConstTable4Entries;
+            d.acc = P.acc.u32() + P.acc_off[i] + 12 * (size_t)k;
+            d.threads = g.threads;
+            d.iters = g.iters;
+            d.q1 = (int)((q1mask[i] >> k) & 1);
+            d.block_base = base;
+            base += (uint32_t)((g.threads + block - 1) / block);
+            rd[c++] = d;
+        }
+    }
+}
+
+// Enqueues one batch on the plan's stream: descriptors and challenges up,
+// prologue, grid-wide rounds, tails, sums down.  Per-kernel timers only when
+// not captured.
+int32_t batch_plan_enqueue(BatchPlan& P, bool timed) {
+    hipStream_t s = P.so.s;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::stage_smem(bitz::kStageMaxLog));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_fold),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
+        attr = true;
+    }
+    char* pin = static_cast<char*>(P.pin);
+    const char* dp = static_cast<const char*>(P.desc.p);
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(P.desc.p) + P.maps_end, pin + P.maps_end, P.ch_o - P.maps_end,
+                           hipMemcpyHostToDevice, s));
+    if (P.ch_total)
+        HIP_TRY(hipMemcpyAsync(P.chal.p, pin + P.ch_o, P.ch_total * sizeof(f128), hipMemcpyHostToDevice, s));
+    std::vector<Timed> ts(timed ? P.kmax + 2 : 0);
+    if (timed) ts[0].begin(s, "outer_batch_prologue", 0.0);
+    hipLaunchKernelGGL(bitz::k_batch_prologue, dim3(P.pro_grid), dim3(1024), 0, s,
+                       reinterpret_cast<const bitz::BatchPrologue*>(dp + P.pro_o),
+                       reinterpret_cast<const uint32_t*>(dp + P.pro_m));
+    if (timed) ts[0].end(s);
+    TRY(check_launch());
+    for (int k = 0; k < P.kmax; ++k) {
+        const auto* d = reinterpret_cast<const bitz::RoundDesc*>(dp + P.rd_o[k]);
+        const auto* m = reinterpret_cast<const uint32_t*>(dp + P.rd_m[k]);
+        if (timed) ts[k + 1].begin(s, k == 0 ? "outer_batch_round0" : "outer_batch_fold", 0.0);
+        if (k == 0)
+            hipLaunchKernelGGL(bitz::k_batch_r0, dim3(P.round_grid[k]), dim3(bitz::kR0Block), bitz::kR0Smem, s, d, m);
+        else
+            hipLaunchKernelGGL(bitz::k_batch_fold, dim3(P.round_grid[k]), dim3(bitz::kFoldBlock), bitz::kFoldSmem, s,
+                               d, m);
+        if (timed) ts[k + 1].end(s);
+        TRY(check_launch());
+    }
+    if (timed) ts[P.kmax + 1].begin(s, "outer_batch_tail", 0.0);
+    hipLaunchKernelGGL(bitz::k_outer_batch, dim3(P.tail_grid), dim3(bitz::kStageBlock),
+                       bitz::stage_smem(bitz::kStageMaxLog), s, reinterpret_cast<const bitz::LdsStage*>(dp + P.tl_o),
+                       reinterpret_cast<const uint32_t*>(dp + P.tl_m));
+    if (timed) ts[P.kmax + 1].end(s);
+    TRY(check_launch());
+    HIP_TRY(hipMemcpyAsync(P.host_acc(), P.acc.p, P.acc_words * 4, hipMemcpyDeviceToHost, s));
+    if (timed) {
+        HIP_TRY(hipStreamSynchronize(s));
+        for (auto& t : ts) t.collect();
+    }
+    return BITZ_OK;
+}
+
+// Runs the plan once: through its captured graph (captured on first use)
+// unless eager, then waits for the sums.
+int32_t batch_plan_run(BatchPlan& P, bool eager) {
+    hipStream_t s = P.so.s;
+    if (eager || g_prof.on) {
+        TRY(batch_plan_enqueue(P, true));
+        HIP_TRY(hipStreamSynchronize(s));
+        return BITZ_OK;
+    }
+    if (!P.exec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int32_t rc = batch_plan_enqueue(P, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(BITZ_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        P.graph = g;
+        HIP_TRY(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(P.exec, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
+// q(1) in round 0 when a challenge is zero (otherwise the claim check at
+// the end catches a wrong claim).
+std::vector<uint64_t> batch_q1mask(const BatchPlan& P, const bitz_outer_instance* inst, uint32_t count) {
+    std::vector<uint64_t> q1mask(count, 0);
+    for (uint32_t i : P.members)
+        for (uint32_t k = 0; k < inst[i].num_vars; ++k)
+            if (hostf::is_zero(F(inst[i].challenges[k]))) q1mask[i] = 1;
+    return q1mask;
+}
+
+// Times each candidate shape on this batch (one warm run, then one timed
+// run each) and returns the fastest.
+int32_t batch_autotune(const bitz_outer_instance* inst, uint32_t count, bool eager, BatchConfig* best) {
+    float best_ms = 0;
+    *best = kBatchDefault;
+    bool any = false;
+    for (const BatchConfig& c : kBatchCandidates) {
+        BatchPlan P;
+        TRY(batch_plan_build(P, inst, count, c));
+        if (P.members.empty()) return BITZ_OK;
+        batch_plan_fill(P, inst, batch_q1mask(P, inst, count));
+        TRY(batch_plan_run(P, eager));
+        const auto t0 = std::chrono::steady_clock::now();
+        TRY(batch_plan_run(P, eager));
+        const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!any || ms < best_ms) {
+            best_ms = ms;
+            *best = c;
+            any = true;
+        }
+    }
     return BITZ_OK;
 }
 
@@ -1693,249 +2065,47 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
         if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
     }
     if (!count) return BITZ_OK;
-    // Members; instances the batched kernels cannot take go alone.
-    std::vector<std::unique_ptr<bitz_outer, void (*)(bitz_outer*)>> hs;
-    std::vector<uint32_t> members, alone;
-    for (uint32_t i = 0; i < count; ++i) {
-        bitz_outer* h = nullptr;
-        TRY(batch_member(&h, inst[i]));
-        hs.emplace_back(h, [](bitz_outer* p) { delete p; });
-        if (h->eqmode && h->n >= 1 && h->n <= bitz::kMaxRounds)
-            members.push_back(i);
-        else
-            alone.push_back(i);
+    std::lock_guard<std::mutex> lock(g_batch.mu);
+    const bool eager = (flags & BITZ_BATCH_EAGER) != 0;
+    const std::vector<uint32_t> key = batch_key(inst, count);
+    BatchConfig cfg = kBatchDefault;
+    if (g_batch.forced.thread_bits) {
+        cfg = g_batch.forced;
+    } else if (!(flags & BITZ_BATCH_NO_TUNE)) {
+        auto it = g_batch.tuned.find(key);
+        if (it == g_batch.tuned.end()) {
+            g_batch.plan.reset();  // its buffers return to the cache for the trials
+            TRY(batch_autotune(inst, count, eager, &cfg));
+            it = g_batch.tuned.emplace(key, cfg).first;
+        }
+        cfg = it->second;
     }
+    g_batch.last = cfg;
+    if (!g_batch.plan || g_batch.plan->key != key || g_batch.plan->cfg.thread_bits != cfg.thread_bits ||
+        g_batch.plan->cfg.tail_log != cfg.tail_log) {
+        g_batch.plan.reset();
+        auto P = std::make_unique<BatchPlan>();
+        TRY(batch_plan_build(*P, inst, count, cfg));
+        g_batch.plan = std::move(P);
+    }
+    BatchPlan& P = *g_batch.plan;
+    std::vector<uint32_t> alone = P.alone;
     int32_t rc = BITZ_OK;
-    if (!members.empty()) {
-        StreamOwner so;
-        TRY(so.create());
-        hipStream_t s = so.s;
-        // Per member: grid-wide rounds [0, k0) (k_batch_r0, then k_batch_fold
-        // writing ping-pong tables P (odd rounds) and Q (even rounds)), then the
-        // LDS tail [k0, n) in one workgroup (k_outer_batch).
-        std::vector<int> k0(count, 0);
-        std::vector<size_t> acc_off(count, 0), ch_off(count, 0), ht_off(count, 0), p_off(count, 0), q_off(count, 0);
-        size_t acc_words = 0, ch_total = 0, ht_total = 0, p_total = 0, q_total = 0;
-        int kmax = 0;
-        for (uint32_t i : members) {
-            bitz_outer* h = hs[i].get();
-            k0[i] = pipe_schedule(h->n).k0;
-            kmax = std::max(kmax, k0[i]);
-            acc_off[i] = acc_words;
-            acc_words += (h->acc_words() + 63) & ~(size_t)63;
-            ch_off[i] = ch_total;
-            ch_total += (size_t)h->n;
-            ht_off[i] = ht_total;
-            ht_total += h->htab_total;
-            p_off[i] = p_total;
-            if (k0[i] >= 2) p_total += h->N / 2;
-            q_off[i] = q_total;
-            if (k0[i] >= 3) q_total += h->N / 4;
-        }
-        Dev acc, chal, rt4, ht4, pbuf, qbuf, desc;
-        TRY(acc.alloc(acc_words * 4));
-        TRY(chal.alloc(std::max<size_t>(ch_total, 1) * sizeof(f128)));
-        TRY(rt4.alloc(std::max<size_t>(ch_total, 1) * kConstTable4Entries * sizeof(uint4)));
-        TRY(ht4.alloc(std::max<size_t>(ht_total, 1) * kConstTable4Entries * sizeof(uint4)));
-        TRY(pbuf.alloc(std::max<size_t>(3 * p_total, 1) * sizeof(f128)));
-        TRY(qbuf.alloc(std::max<size_t>(3 * q_total, 1) * sizeof(f128)));
-        auto rtab_of = [&](uint32_t i) { return static_cast<uint4*>(rt4.p) + ch_off[i] * kConstTable4Entries; };
-        // Round k's tables of member i: the inputs for k = 0, then the output of
-        // round k's fold by r_{k-1}.
-        auto tables_of = [&](uint32_t i, int k, int t) -> f128* {
-            if (k == 0) return const_cast<f128*>(hs[i]->inputs[t]);
-            if (k & 1) return pbuf.f() + t * p_total + p_off[i];
-            return qbuf.f() + t * q_total + q_off[i];
-        };
-        // Descriptors and block maps of every launch, staged in one pinned block.
-        std::vector<bitz::BatchPrologue> pro;
-        std::vector<uint32_t> pro_map;
-        for (uint32_t i : members) {
-            bitz_outer* h = hs[i].get();
-            bitz::BatchPrologue p{};
-            p.L = h->eq_in[0];
-            p.H = h->eq_in[1];
-            p.ell = h->ell;
-            p.m = h->m;
-            p.n = h->n;
-            p.zero = acc.u32() + acc_off[i];
-            p.zero_words = (unsigned)h->eq_off();
-            p.eq_blocks_l = h->eq_bl;
-            p.eq_blocks = h->eq_bl + h->eq_bh;
-            p.eq_out = acc.u32() + acc_off[i] + h->eq_off();
-            p.chal = chal.f() + ch_off[i];
-            p.rtabs4 = rtab_of(i);
-            p.hdesc = h->hdesc;
-            p.htabs4 = static_cast<uint4*>(ht4.p) + ht_off[i] * kConstTable4Entries;
-            p.block_base = (uint32_t)pro_map.size();
-            pro_map.insert(pro_map.end(), p.eq_blocks + (unsigned)h->n + h->htab_total, (uint32_t)pro.size());
-            pro.push_back(p);
-        }
-        // q(1) in round 0 when a challenge is zero (otherwise the claim check
-        // at the end catches a wrong claim).
-        std::vector<uint64_t> q1mask(count, 0);
-        for (uint32_t i : members)
-            for (uint32_t k = 0; k < inst[i].num_vars; ++k)
-                if (hostf::is_zero(F(inst[i].challenges[k]))) q1mask[i] = 1;
-        std::vector<std::vector<bitz::RoundDesc>> rounds(kmax);
-        std::vector<std::vector<uint32_t>> round_map(kmax);
-        for (int k = 0; k < kmax; ++k) {
-            const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
-            for (uint32_t i : members) {
-                if (k >= k0[i]) continue;
-                bitz_outer* h = hs[i].get();
-                const auto& g = h->geo[k];
-                bitz::RoundDesc d{};
-                // Round k reads round k-1's tables and folds them by r_{k-1}.
-                const int src = k == 0 ? 0 : k - 1;
-                d.a = tables_of(i, src, 0);
-                d.b = tables_of(i, src, 1);
-                d.c = tables_of(i, src, 2);
-                if (k > 0) {
-                    d.ao = tables_of(i, k, 0);
-                    d.bo = tables_of(i, k, 1);
-                    d.co = tables_of(i, k, 2);
-                    d.r_tab4 = rtab_of(i) + (size_t)(k - 1) * kConstTable4Entries;
-                }
-                d.la = h->table_ptr(g.la.table);
-                d.lb = h->table_ptr(g.lb.table);
-                d.la_bits = g.la_bits;
-                d.la_shift = g.la.a - h->table_base(g.la.table);
-                d.lb_shift = g.lb.a - h->table_base(g.lb.table);
-                d.h_tabs4 = static_cast<const uint4*>(ht4.p) + (ht_off[i] + g.htab_off) * kConstTable4Entries;
-                d.acc = acc.u32() + acc_off[i] + 12 * (size_t)k;
-                d.threads = g.threads;
-                d.iters = g.iters;
-                d.q1 = (int)((q1mask[i] >> k) & 1);
-                d.block_base = (uint32_t)round_map[k].size();
-                round_map[k].insert(round_map[k].end(), (g.threads + block - 1) / block, (uint32_t)rounds[k].size());
-                rounds[k].push_back(d);
-            }
-        }
-        std::vector<bitz::LdsStage> tails;
-        for (uint32_t i : members) {
-            bitz_outer* h = hs[i].get();
-            bitz::LdsStage t{};
-            const int kk = k0[i];
-            // Round k0-1's tables, folded by r_{k0-1} on load (the inputs when k0 = 0).
-            const int src = kk > 0 ? kk - 1 : 0;
-            t.a = tables_of(i, src, 0);
-            t.b = tables_of(i, src, 1);
-            t.c = tables_of(i, src, 2);
-            t.L = h->eq_in[0];
-            t.H = h->eq_in[1];
-            t.rtabs4 = rtab_of(i);
-            t.acc = acc.u32() + acc_off[i];
-            t.fin = reinterpret_cast<f128*>(acc.u32() + acc_off[i] + h->fin_off());
-            t.q1mask = q1mask[i];
-            t.ell = h->ell;
-            t.n = h->n;
-            t.k_first = kk;
-            t.s = h->n - kk;
-            t.lg_in = kk > 0 ? h->n - kk + 1 : h->n;
-            t.fold_first = kk > 0;
-            t.final_stage = 1;
-            t.store_sums = 1;
-            t.chunk_base = (uint32_t)tails.size();  // one workgroup: chunk 0
-            tails.push_back(t);
-        }
-        std::vector<uint32_t> tail_map(tails.size());
-        for (size_t j = 0; j < tails.size(); ++j) tail_map[j] = (uint32_t)j;
-        // Staging: [prologue | map] [round k | map]... [tails | map] [challenges] ... [sums readback]
-        size_t off = 0;
-        auto place = [&](size_t bytes) {
-            off = (off + 255) & ~(size_t)255;
-            const size_t o = off;
-            off += bytes;
-            return o;
-        };
-        const size_t pro_o = place(pro.size() * sizeof(bitz::BatchPrologue));
-        const size_t pro_m = place(pro_map.size() * 4);
-        std::vector<size_t> rd_o(kmax), rd_m(kmax);
-        for (int k = 0; k < kmax; ++k) {
-            rd_o[k] = place(rounds[k].size() * sizeof(bitz::RoundDesc));
-            rd_m[k] = place(round_map[k].size() * 4);
-        }
-        const size_t tl_o = place(tails.size() * sizeof(bitz::LdsStage));
-        const size_t tl_m = place(tail_map.size() * 4);
-        const size_t desc_bytes = off;
-        const size_t ch_o = place(ch_total * sizeof(f128));
-        const size_t up_bytes = off;
-        const size_t rb_o = place(acc_words * 4);
-        std::lock_guard<std::mutex> lock(g_batch_pinned.mu);
-        TRY(g_batch_pinned.reserve(off));
-        char* pin = static_cast<char*>(g_batch_pinned.p);
-        memcpy(pin + pro_o, pro.data(), pro.size() * sizeof(bitz::BatchPrologue));
-        memcpy(pin + pro_m, pro_map.data(), pro_map.size() * 4);
-        for (int k = 0; k < kmax; ++k) {
-            memcpy(pin + rd_o[k], rounds[k].data(), rounds[k].size() * sizeof(bitz::RoundDesc));
-            memcpy(pin + rd_m[k], round_map[k].data(), round_map[k].size() * 4);
-        }
-        memcpy(pin + tl_o, tails.data(), tails.size() * sizeof(bitz::LdsStage));
-        memcpy(pin + tl_m, tail_map.data(), tail_map.size() * 4);
-        for (uint32_t i : members)
-            memcpy(pin + ch_o + ch_off[i] * sizeof(f128), inst[i].challenges, inst[i].num_vars * sizeof(f128));
-        TRY(desc.alloc(std::max<size_t>(desc_bytes, 16)));
-        HIP_TRY(hipMemcpyAsync(desc.p, pin, desc_bytes, hipMemcpyHostToDevice, s));
-        if (ch_total)
-            HIP_TRY(hipMemcpyAsync(chal.p, pin + ch_o, ch_total * sizeof(f128), hipMemcpyHostToDevice, s));
-        (void)up_bytes;
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_batch),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)bitz::stage_smem(bitz::kStageMaxLog));
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_fold),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
-            attr = true;
-        }
-        const char* dp = static_cast<const char*>(desc.p);
-        Timed tp;
-        tp.begin(s, "outer_batch_prologue", 0.0);
-        hipLaunchKernelGGL(bitz::k_batch_prologue, dim3((unsigned)pro_map.size()), dim3(1024), 0, s,
-                           reinterpret_cast<const bitz::BatchPrologue*>(dp + pro_o),
-                           reinterpret_cast<const uint32_t*>(dp + pro_m));
-        tp.end(s);
-        TRY(check_launch());
-        std::vector<Timed> ts(kmax + 1);
-        for (int k = 0; k < kmax; ++k) {
-            const auto* d = reinterpret_cast<const bitz::RoundDesc*>(dp + rd_o[k]);
-            const auto* m = reinterpret_cast<const uint32_t*>(dp + rd_m[k]);
-            const unsigned grid = (unsigned)round_map[k].size();
-            ts[k].begin(s, k == 0 ? "outer_batch_round0" : "outer_batch_fold", 0.0);
-            if (k == 0)
-                hipLaunchKernelGGL(bitz::k_batch_r0, dim3(grid), dim3(bitz::kR0Block), bitz::kR0Smem, s, d, m);
-            else
-                hipLaunchKernelGGL(bitz::k_batch_fold, dim3(grid), dim3(bitz::kFoldBlock), bitz::kFoldSmem, s, d, m);
-            ts[k].end(s);
-            TRY(check_launch());
-        }
-        ts[kmax].begin(s, "outer_batch_tail", 0.0);
-        hipLaunchKernelGGL(bitz::k_outer_batch, dim3((unsigned)tails.size()), dim3(bitz::kStageBlock),
-                           bitz::stage_smem(bitz::kStageMaxLog), s,
-                           reinterpret_cast<const bitz::LdsStage*>(dp + tl_o),
-                           reinterpret_cast<const uint32_t*>(dp + tl_m));
-        ts[kmax].end(s);
-        TRY(check_launch());
-        uint32_t* host_acc = reinterpret_cast<uint32_t*>(pin + rb_o);
-        HIP_TRY(hipMemcpyAsync(host_acc, acc.p, acc_words * 4, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        tp.collect();
-        for (auto& t : ts) t.collect();
+    if (!P.members.empty()) {
+        const std::vector<uint64_t> q1mask = batch_q1mask(P, inst, count);
+        batch_plan_fill(P, inst, q1mask);
+        TRY(batch_plan_run(P, eager));
+        const uint32_t* host_acc = P.host_acc();
         // Per instance: validate, recover the coefficients, check the claim.
-        for (uint32_t i : members) {
-            bitz_outer* h = hs[i].get();
-            h->host_ext = host_acc + acc_off[i];
+        // Instances are independent; a few host threads share them.
+        auto finish = [&](uint32_t i) -> bool {
+            bitz_outer* h = P.hs[i].get();
+            h->host_ext = host_acc + P.acc_off[i];
             const auto& x = inst[i];
             std::vector<bitz_f128> co;
             const bool ok = outer_validate_eq(h, h->host_words() + h->eq_off()) &&
                             outer_assemble_eq(h, x.challenges, F(x.initial_claim), q1mask[i], !(q1mask[i] & 1), co);
-            if (!ok) {
-                alone.push_back(i);
-                continue;
-            }
+            if (!ok) return false;
             f128 claim = F(x.initial_claim);
             for (int k = 0; k < h->n; ++k) {
                 f128 c[4] = {F(co[3 * k]), hostf::add(claim, hostf::add(F(co[3 * k + 1]), F(co[3 * k + 2]))),
@@ -1947,17 +2117,51 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
             }
             *x.final_claim = B(claim);
             memcpy(x.claims, h->host_words() + h->fin_off(), 48);
-        }
+            return true;
+        };
+        const auto& members = P.members;
+        std::vector<char> done(members.size(), 0);
+        const size_t workers = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
+                                                 (members.size() + 31) / 32});
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w)
+            pool.emplace_back([&, w] {
+                for (size_t j = w; j < members.size(); j += workers) done[j] = finish(members[j]);
+            });
+        for (size_t j = 0; j < members.size(); j += workers) done[j] = finish(members[j]);
+        for (auto& t : pool) t.join();
+        for (size_t j = 0; j < members.size(); ++j)
+            if (!done[j]) alone.push_back(members[j]);
     }
     for (uint32_t i : alone) {
         const auto& x = inst[i];
         rc = bitz_prove_outer_sumcheck_with_challenges(x.challenges, x.initial_claim, x.eq_low, x.low_vars,
                                                        x.eq_high, x.high_vars, x.az, x.num_vars, x.bz, x.num_vars,
-                                                       x.cz, x.num_vars, flags, x.round_polys, x.eval_points,
-                                                       x.final_claim, x.claims);
+                                                       x.cz, x.num_vars, flags & BITZ_DEVICE, x.round_polys,
+                                                       x.eval_points, x.final_claim, x.claims);
         if (rc) return fail(rc, "instance " + std::to_string(i) + ": " + g_err);
     }
     return rc;
+}
+
+extern "C" int32_t bitz_outer_batch_config(uint32_t* thread_bits, uint32_t* tail_log) {
+    std::lock_guard<std::mutex> lock(g_batch.mu);
+    if (thread_bits) *thread_bits = (uint32_t)g_batch.last.thread_bits;
+    if (tail_log) *tail_log = (uint32_t)g_batch.last.tail_log;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_outer_batch_set_config(uint32_t thread_bits, uint32_t tail_log) {
+    if (thread_bits && (thread_bits < 6 || thread_bits > 20 || tail_log < 1 || (int)tail_log > kTailLog))
+        return fail(BITZ_ERR_ARG, "batch tile shapes out of range");
+    std::lock_guard<std::mutex> lock(g_batch.mu);
+    g_batch.forced = thread_bits ? BatchConfig{(int)thread_bits, (int)tail_log} : BatchConfig{0, 0};
+    return BITZ_OK;
+}
+
+extern "C" void bitz_outer_batch_release(void) {
+    std::lock_guard<std::mutex> lock(g_batch.mu);
+    g_batch.plan.reset();
 }
 
 // ======================================================= inner sumcheck ===

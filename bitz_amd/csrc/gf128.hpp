@@ -210,14 +210,24 @@ __attribute__((target("pclmul,sse2"))) inline f128 mul(f128 a, f128 b) {
 
 inline f128 square(f128 a) { return mul(a, a); }
 
-// a^(2^128 - 2) = a^-1 (a != 0).
+// a^-1 = a^(2^128 - 2) = (a^(2^127 - 1))^2 (a != 0), by Itoh and Tsujii's
+// addition chain: with b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j, walked
+// along 1, 2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127 (127 squarings, 12
+// products).
 inline f128 inv(f128 a) {
-    f128 acc = one(), p = a;
-    for (int i = 1; i < 128; ++i) {
-        p = square(p);
-        acc = mul(acc, p);
+    auto sq_n = [](f128 x, int n) {
+        for (int i = 0; i < n; ++i) x = square(x);
+        return x;
+    };
+    f128 b = a;  // b_1
+    int k = 1;
+    static const int chain[] = {2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127};
+    for (int next : chain) {
+        const int j = next - k;  // j == k (doubling) or j == 1
+        b = mul(sq_n(b, j), j == k ? b : a);
+        k = next;
     }
-    return acc;
+    return square(b);
 }
 
 // Horner from the top coefficient (crates/spartan/src/sumcheck.rs:784-794).

@@ -172,6 +172,20 @@ typedef struct {
     bitz_f128* claims;      /* [Az, Bz, Cz](r_x) */
 } bitz_outer_instance;
 int32_t bitz_prove_outer_batch(const bitz_outer_instance* instances, uint32_t count, uint32_t flags);
+/* Batch flags (with BITZ_DEVICE).  By default a batch runs as a captured
+ * launch graph (one launch per round over every instance), replayed while
+ * the instance shapes repeat, with tile shapes picked by a timing autotuner
+ * on the first batch of each shape. */
+#define BITZ_BATCH_EAGER 2u   /* launch the kernels directly, no graph */
+#define BITZ_BATCH_NO_TUNE 4u /* default tile shapes, no autotuning */
+/* Tile shapes of the last batch: the grid-wide rounds' thread-grid cap
+ * (log2) and the one-workgroup tail's size (log2 table length). */
+int32_t bitz_outer_batch_config(uint32_t* thread_bits, uint32_t* tail_log);
+/* Fixes the tile shapes of later batches (thread_bits in [6, 20], tail_log
+ * in [1, 10]); thread_bits = 0 returns to the autotuner. */
+int32_t bitz_outer_batch_set_config(uint32_t thread_bits, uint32_t tail_log);
+/* Frees the cached batch plan (its device buffers and graph). */
+void bitz_outer_batch_release(void);
 
 /* Per-round form of the same prover for callers that keep the transcript:
  * begin -> round0 ([c0,c2,c3]) -> fold_next(r_k) per round (returns the next

@@ -432,3 +432,54 @@ def test_outer_batch_of_many_small_instances():
     outs = G.prove_outer_batch(insts)
     for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
         assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+
+
+def batch_cases(specs, seed0):
+    cases = []
+    for j, (n, split) in enumerate(specs):
+        a, b, c, tau, lo, hi = outer_case(n, split, seed0 + 17 * j)
+        cases.append((elements(seed0 + 1 + j, n), true_claim(a, b, c, tau), lo, hi, a, b, c))
+    return cases
+
+
+def batch_insts(cases):
+    return [(ch, claim, (G.DeviceTable(lo), G.DeviceTable(hi)),
+             G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c)))
+            for ch, claim, lo, hi, a, b, c in cases]
+
+
+@pytest.mark.parametrize("thread_bits,tail_log", [(17, 10), (14, 10), (12, 9), (8, 4)])
+def test_outer_batch_every_tile_shape(thread_bits, tail_log):
+    """Fixed tile shapes (multi-iteration grid-wide rounds with H(i) tables
+    from thread_bits below num_vars - 1, short tails) against the oracle."""
+    specs = [(3, 1), (9, 4), (12, 6), (15, 7), (16, 16), (18, 9)]
+    cases = batch_cases(specs, 700 + thread_bits)
+    G.outer_batch_set_config(thread_bits, tail_log)
+    try:
+        for graph in (True, False):
+            outs = G.prove_outer_batch(batch_insts(cases), graph=graph)
+            assert G.outer_batch_config() == (thread_bits, tail_log)
+            for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
+                assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+    finally:
+        G.outer_batch_set_config(0, 0)
+
+
+def test_outer_batch_graph_replays_track_new_data():
+    """The captured graph is replayed for batches of the same shapes with new
+    tables, challenges and claims, with eager batches and other device work
+    between the replays; every replay equals the eager result and the oracle."""
+    specs = [(4, 2), (11, 5), (13, 6), (16, 8)]
+    eager_ref = None
+    for rep in range(3):
+        cases = batch_cases(specs, 800 + 100 * rep)
+        outs = G.prove_outer_batch(batch_insts(cases), graph=True)
+        eager = G.prove_outer_batch(batch_insts(cases), graph=False, tune=False)
+        G.mul_batch(elements_fast(rep, 4096), elements_fast(rep + 1, 4096))
+        for (ch, claim, lo, hi, a, b, c), out, e in zip(cases, outs, eager):
+            ref = oracle_with(ch, claim, lo, hi, a, b, c)
+            assert_outer_equal(ref, out)
+            assert_outer_equal(ref, e)
+        eager_ref = outs
+    assert eager_ref is not None
+    G.outer_batch_release()
