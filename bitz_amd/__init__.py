@@ -101,6 +101,7 @@ def lib():
             "bitz_transcript_squeeze": (I32, [P, P]),
             "bitz_transcript_narg": (SZ, [P, P, SZ]),
             "bitz_mul_batch": (I32, [P, P, P, SZ, U32]),
+            "bitz_gkr_prove_with_challenges": (I32, [P, P, U32, P, SZ, P, P, P, P]),
             "bitz_eq_table": (I32, [P, U32, P, U32]),
             "bitz_mle_fold": (I32, [P, C.POINTER(U32), P, U32, U32]),
             "bitz_mle_evaluate": (I32, [P, U32, P, U32, P, U32]),
@@ -706,6 +707,26 @@ def grand_product(leaves, log_groups: int):
     h = C.c_void_p()
     _check(lib().bitz_gkr_build(C.byref(h), _p(lv), log_leaves, log_groups, _p(out), flags))
     return out, LayerWitnesses(h, log_leaves, log_groups)
+
+
+def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
+    """gpgkr_prove with the challenges known in advance
+    (bitz_gkr_prove_with_challenges): per layer from the top, its round
+    challenges in squeeze order, then t.  Returns (messages (rounds, 2, 2),
+    layer_evals (layers, 2, 2), leaf point, claim)."""
+    point = elems(point) if len(point) else np.zeros((0, 2), np.uint64)
+    ch = elems(challenges)
+    n0, nl = witnesses.log_groups, witnesses.log_leaves
+    rounds = sum(range(n0, nl))
+    layers = nl - n0
+    msgs = np.zeros((max(rounds, 1) * 2, 2), dtype=np.uint64)
+    evals = np.zeros((max(layers, 1) * 2, 2), dtype=np.uint64)
+    leaf_point = np.zeros((nl, 2), dtype=np.uint64)
+    claim = F128()
+    _check(lib().bitz_gkr_prove_with_challenges(witnesses.h, _p(point), len(point), _p(ch), len(ch), _p(msgs),
+                                                _p(evals), _p(leaf_point), C.byref(claim)))
+    return (msgs[: 2 * rounds].reshape(rounds, 2, 2), evals[: 2 * layers].reshape(layers, 2, 2), leaf_point,
+            to_int(claim))
 
 
 def gpgkr_prove(transcript: Transcript, point, witnesses: LayerWitnesses):

@@ -2651,6 +2651,133 @@ extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f1
     return BITZ_OK;
 }
 
+// The same proof with the challenges known in advance (a transcript whose
+// challenges do not depend on the messages, such as the seeded stream):
+// every layer's point follows from them, so all eq tables and rounds are
+// queued back to back and read once.  challenges: per layer from the top,
+// the n round challenges in squeeze order (coordinate n-1 first), then t.
+// messages receives [eq_bound sum_b, eq_bound sum_inf] per round in the same
+// order, layer_evals [L(r), R(r)] per layer.
+extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* point, uint32_t point_len,
+                                                  const bitz_f128* challenges, size_t challenge_count,
+                                                  bitz_f128* messages, bitz_f128* layer_evals,
+                                                  bitz_f128* leaf_point, bitz_f128* leaf_claim) {
+    if (!h || !leaf_point || !leaf_claim || (point_len && !point)) return fail(BITZ_ERR_ARG, "null pointer");
+    if ((int)point_len != h->log_groups || point_len == 0)
+        return fail(BITZ_ERR_POINT_LEN, "the output has one value per tree, so the point has a coordinate per tree bit");
+    const int layers = h->log_leaves - h->log_groups;
+    size_t need = 0, total_rounds = 0;
+    for (int n = h->log_groups; n < h->log_leaves; ++n) {
+        need += (size_t)n + 1;
+        total_rounds += (size_t)n;
+    }
+    if (challenge_count != need) return fail(BITZ_ERR_ARG, "challenge count must be sum over layers of (n + 1)");
+    if (need && (!challenges || !layer_evals || (total_rounds && !messages))) return fail(BITZ_ERR_ARG, "null pointer");
+    if (h->consumed) return fail(BITZ_ERR_STATE, "layers already consumed");
+    h->consumed = true;
+    hipStream_t s = h->so.s;
+    // Every layer's point: the top one given, then (r_0 .. r_{n-1}, t).
+    std::vector<std::vector<f128>> pts(layers + 1);
+    pts[0].resize(point_len);
+    for (uint32_t i = 0; i < point_len; ++i) pts[0][i] = F(point[i]);
+    {
+        size_t c = 0;
+        for (int l = 0; l < layers; ++l) {
+            const int n = (int)pts[l].size();
+            std::vector<f128> next(n + 1);
+            for (int i = n - 1; i >= 0; --i) next[i] = F(challenges[c++]);
+            next[n] = F(challenges[c++]);
+            pts[l + 1] = next;
+        }
+    }
+    std::vector<size_t> pt_off(layers);
+    size_t pt_total = 0;
+    for (int l = 0; l < layers; ++l) {
+        pt_off[l] = pt_total;
+        pt_total += pts[l].size();
+    }
+    TRY(h->acc.alloc((total_rounds + 2 * (size_t)layers + 1) * 8 * sizeof(uint32_t)));
+    TRY(h->eq.alloc(((size_t)1 << std::max(h->log_leaves - 2, 0)) * sizeof(f128)));
+    TRY(h->point.alloc(std::max<size_t>(pt_total, 1) * sizeof(f128)));
+    {
+        const int kmax = std::max(h->log_leaves - 2, 0);
+        const size_t sc = ((size_t)1 << (kmax / 2)) + ((size_t)1 << (kmax - kmax / 2));
+        if (h->eq_scratch.bytes < sc * sizeof(f128)) TRY(h->eq_scratch.alloc(sc * sizeof(f128)));
+    }
+    std::vector<f128> flat(std::max<size_t>(pt_total, 1));
+    for (int l = 0; l < layers; ++l) std::copy(pts[l].begin(), pts[l].end(), flat.begin() + pt_off[l]);
+    Pinned up;
+    TRY(up.alloc(flat.size() * sizeof(f128)));
+    memcpy(up.p, flat.data(), flat.size() * sizeof(f128));
+    HIP_TRY(hipMemcpyAsync(h->point.p, up.p, flat.size() * sizeof(f128), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s));
+    uint32_t* evals = h->acc.u32() + 8 * total_rounds;  // [L, R] per layer, 8 words
+    size_t slot = 0;
+    for (int l = 0; l < layers; ++l) {
+        const int j = layers - 1 - l;  // storage layer (0 = leaves)
+        const int n = (int)pts[l].size();
+        f128* left = h->store.f() + h->off[j];
+        f128* right = left + ((size_t)1 << n);
+        TRY(eq_build(h->point.f() + pt_off[l], n - 1, h->eq.f(), h->eq_scratch, s));
+        f128 r_prev{0, 0};
+        for (int i = n - 1; i >= 0; --i) {
+            const int at_one = hostf::is_zero(pts[l][i]);  // sends_one (lib.rs:622-624)
+            uint32_t* acc = h->acc.u32() + 8 * slot;
+            if (i == n - 1) {
+                const size_t half = (size_t)1 << i;
+                h->tm.begin(s, "gkr_sums", 80.0 * (double)half);
+                hipLaunchKernelGGL(bitz::k_gkr_sums, dim3(grid_for(half)), dim3(kBlock), 0, s, left, right,
+                                   h->eq.f(), half, at_one, acc);
+            } else {
+                const size_t hh = (size_t)1 << i;
+                h->tm.begin(s, "gkr_bind_next", 240.0 * (double)hh);
+                hipLaunchKernelGGL(bitz::k_gkr_bind_next, dim3(grid_for(hh)), dim3(kBlock), 0, s, left, right,
+                                   h->eq.f(), hh, r_prev, at_one, acc);
+            }
+            h->tm.end(s);
+            TRY(check_launch());
+            ++slot;
+            r_prev = pts[l + 1][i];
+        }
+        hipLaunchKernelGGL(bitz::k_gkr_bind, dim3(1), dim3(kBlock), 0, s, left, right, (size_t)1, r_prev);
+        TRY(check_launch());
+        HIP_TRY(hipMemcpyAsync(evals + 8 * l, left, 16, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(evals + 8 * l + 4, right, 16, hipMemcpyDeviceToDevice, s));
+    }
+    Pinned down;
+    const size_t words = 8 * (total_rounds + (size_t)layers);
+    TRY(down.alloc(std::max<size_t>(words, 1) * 4));
+    HIP_TRY(hipMemcpyAsync(down.p, h->acc.p, words * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h->tm.collect();
+    const uint32_t* w = static_cast<const uint32_t*>(down.p);
+    slot = 0;
+    f128 claim{0, 0};
+    for (int l = 0; l < layers; ++l) {
+        const int n = (int)pts[l].size();
+        f128 eq_bound = hostf::one();
+        for (int i = n - 1; i >= 0; --i) {
+            f128 sums[2];
+            memcpy(sums, w + 8 * slot, 32);
+            // lib.rs:142: prover_message([eq_bound * sum_b, eq_bound * sum_inf]).
+            messages[2 * slot] = B(hostf::mul(eq_bound, sums[0]));
+            messages[2 * slot + 1] = B(hostf::mul(eq_bound, sums[1]));
+            const f128 r = pts[l + 1][i], z = pts[l][i];
+            eq_bound = hostf::mul(eq_bound, f128{1 ^ r.lo ^ z.lo, r.hi ^ z.hi});  // eq_factor
+            ++slot;
+        }
+        f128 lr[2];
+        memcpy(lr, w + 8 * total_rounds + 8 * l, 32);
+        layer_evals[2 * l] = B(lr[0]);
+        layer_evals[2 * l + 1] = B(lr[1]);
+        const f128 tt = pts[l + 1][n];
+        claim = hostf::add(lr[0], hostf::mul(tt, hostf::add(lr[1], lr[0])));
+    }
+    for (size_t i = 0; i < pts[layers].size(); ++i) leaf_point[i] = B(pts[layers][i]);
+    *leaf_claim = B(claim);
+    return BITZ_OK;
+}
+
 extern "C" void bitz_gkr_free(bitz_gkr* h) {
     if (h) {
         (void)hipStreamSynchronize(h->so.s);

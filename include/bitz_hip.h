@@ -262,6 +262,17 @@ int32_t bitz_gkr_build(bitz_gkr** h, const bitz_f128* leaves, uint32_t log_leave
 int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point, uint32_t point_len,
                        bitz_f128* leaf_point, bitz_f128* leaf_claim);
 /* Copies layer j (0 = leaves) to host; *len receives its length. */
+/* gpgkr_prove with the challenges known in advance (a transcript whose
+ * challenges do not depend on the messages, such as the seeded stream); no
+ * counterpart in the reference, which draws them round by round.  All rounds
+ * are queued back to back and read once.  challenges: per layer from the
+ * top, its n round challenges in squeeze order, then t (sum over layers of
+ * n + 1 values).  messages: the prover_message pairs [eq_bound sum_b,
+ * eq_bound sum_inf] of every round in the same order; layer_evals: [L(r),
+ * R(r)] per layer.  leaf_point / leaf_claim as bitz_gkr_prove. */
+int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* point, uint32_t point_len,
+                                       const bitz_f128* challenges, size_t challenge_count, bitz_f128* messages,
+                                       bitz_f128* layer_evals, bitz_f128* leaf_point, bitz_f128* leaf_claim);
 int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_t cap, size_t* len);
 void bitz_gkr_free(bitz_gkr* h);
 

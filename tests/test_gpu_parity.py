@@ -213,6 +213,35 @@ def test_gkr_matches_oracle(log_leaves, log_groups, zero):
     assert gc == O.mle_evaluate(leaves, gp)
 
 
+@pytest.mark.parametrize("log_leaves,log_groups,zero", [(6, 2, None), (12, 1, 4), (15, 7, None), (18, 1, None)])
+def test_gkr_with_challenges_matches_oracle(log_leaves, log_groups, zero):
+    """The queued GKR prover against the oracle driven by the same challenge
+    list: every prover_message, every [L(r), R(r)], the leaf point and claim."""
+    leaves = elements_fast(90 + log_leaves, 1 << log_leaves)
+    store, layers = O.gkr_fold_layers(leaves, log_groups)
+    point = elements(91, log_groups)
+    count = sum(n + 1 for n in range(log_groups, log_leaves))
+    ch = elements(92 + log_leaves, count)
+    if zero is not None:
+        point[0] = 0
+        ch[zero] = 0  # a zero challenge makes a later round send its at-one sum
+    ot = O.ListTranscript(ints(ch))
+    ref_point, ref_claim = O.gkr_prove(ot, point, store, log_leaves)
+    _, wit = G.grand_product(leaves, log_groups)
+    msgs, evals, gp, gc = G.gpgkr_prove_with_challenges(point, ch, wit)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim
+    sent = [np.frombuffer(b, dtype=np.uint64).reshape(2, 2) for kind, b in ot.messages if kind == 1]
+    expect_msgs, expect_evals, k = [], [], 0
+    for n in range(log_groups, log_leaves):
+        expect_msgs += sent[k: k + n]
+        expect_evals.append(sent[k + n])
+        k += n + 1
+    assert k == len(sent)
+    assert np.array_equal(msgs, np.array(expect_msgs).reshape(msgs.shape))
+    assert np.array_equal(evals, np.array(expect_evals).reshape(evals.shape))
+    assert gc == O.mle_evaluate(leaves, gp)
+
+
 def test_gkr_rejects_a_wrong_point_length():
     leaves = elements_fast(1, 1 << 6)
     _, wit = G.grand_product(leaves, 2)
