@@ -348,14 +348,14 @@ def bench_gkr(G, args) -> int:
         log(f"bench: warmup {i + 1}/{args.warmup}: {step(warm_ch[i]) * 1e3:.2f} ms")
     G.synchronize()
     G.profile_enable(True)
-    G.profile_only("gkr_bind_next")
+    G.profile_only("gkr_round_batch")
     times = []
     t_start = time.perf_counter()
     for i in range(args.steps):
         times.append(step(timed_ch[i]))
     G.synchronize()
     wall = time.perf_counter() - t_start
-    bind = G.profile_read("gkr_bind_next")
+    bind = G.profile_read("gkr_round_batch")
     G.profile_only(None)
     G.profile_enable(False)
     tt = []
@@ -388,7 +388,7 @@ def bench_gkr(G, args) -> int:
         "algorithmic_gb_per_s": 92.0 * children * args.steps / wall / 1e9,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_gkr_bind_next (fused round sums + bind, every layer)",
+            "kernel": "k_gkr_round_batch (grid-wide rounds of every large layer, one launch per round index)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

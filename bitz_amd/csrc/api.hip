@@ -2653,11 +2653,13 @@ extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f1
 
 // The same proof with the challenges known in advance (a transcript whose
 // challenges do not depend on the messages, such as the seeded stream):
-// every layer's point follows from them, so all eq tables and rounds are
-// queued back to back and read once.  challenges: per layer from the top,
-// the n round challenges in squeeze order (coordinate n-1 first), then t.
-// messages receives [eq_bound sum_b, eq_bound sum_inf] per round in the same
-// order, layer_evals [L(r), R(r)] per layer.
+// every layer's point follows from them, so the layers are independent and
+// run side by side: their eq tables in two launches, the grid-wide rounds of
+// the large layers one launch per round index, every layer's last rounds in
+// one launch of LDS tails, then one read-back.  challenges: per layer from
+// the top, the n round challenges in squeeze order (coordinate n-1 first),
+// then t.  messages receives [eq_bound sum_b, eq_bound sum_inf] per round in
+// the same order, layer_evals [L(r), R(r)] per layer.
 extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* point, uint32_t point_len,
                                                   const bitz_f128* challenges, size_t challenge_count,
                                                   bitz_f128* messages, bitz_f128* layer_evals,
@@ -2676,82 +2678,194 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
     if (h->consumed) return fail(BITZ_ERR_STATE, "layers already consumed");
     h->consumed = true;
     hipStream_t s = h->so.s;
+    constexpr int T = bitz::kGkrTailMax;
     // Every layer's point: the top one given, then (r_0 .. r_{n-1}, t).
     std::vector<std::vector<f128>> pts(layers + 1);
     pts[0].resize(point_len);
     for (uint32_t i = 0; i < point_len; ++i) pts[0][i] = F(point[i]);
+    std::vector<size_t> ch_off(layers), slot_off(layers), pt_off(layers), eq_off(layers);
+    size_t pt_total = 0, eq_total = 0, scratch_total = 0;
     {
-        size_t c = 0;
+        size_t c = 0, slot = 0;
         for (int l = 0; l < layers; ++l) {
             const int n = (int)pts[l].size();
+            ch_off[l] = c;
+            slot_off[l] = slot;
+            slot += (size_t)n;
             std::vector<f128> next(n + 1);
             for (int i = n - 1; i >= 0; --i) next[i] = F(challenges[c++]);
             next[n] = F(challenges[c++]);
             pts[l + 1] = next;
+            pt_off[l] = pt_total;
+            pt_total += (size_t)n;
+            eq_off[l] = eq_total;
+            eq_total += (size_t)1 << (n - 1);
+            if (n - 1 > bitz::kGkrEqSmallMax)
+                scratch_total += ((size_t)1 << ((n - 1) / 2)) + ((size_t)1 << (n - 1 - (n - 1) / 2));
         }
     }
-    std::vector<size_t> pt_off(layers);
-    size_t pt_total = 0;
-    for (int l = 0; l < layers; ++l) {
-        pt_off[l] = pt_total;
-        pt_total += pts[l].size();
-    }
-    TRY(h->acc.alloc((total_rounds + 2 * (size_t)layers + 1) * 8 * sizeof(uint32_t)));
-    TRY(h->eq.alloc(((size_t)1 << std::max(h->log_leaves - 2, 0)) * sizeof(f128)));
-    TRY(h->point.alloc(std::max<size_t>(pt_total, 1) * sizeof(f128)));
+    // Descriptors.
+    std::vector<bitz::GkrEqItem> eq_items;
+    std::vector<bitz::GkrCombine> combines;
+    std::vector<uint32_t> comb_map;
+    Dev pdev, chdev, rt4, eqa, eqs, desc;
+    TRY(pdev.alloc(std::max<size_t>(pt_total, 1) * sizeof(f128)));
+    TRY(chdev.alloc(std::max<size_t>(need, 1) * sizeof(f128)));
+    TRY(rt4.alloc(std::max<size_t>(need, 1) * kConstTable4Entries * sizeof(uint4)));
+    TRY(eqa.alloc(std::max<size_t>(eq_total, 1) * sizeof(f128)));
+    TRY(eqs.alloc(std::max<size_t>(scratch_total, 1) * sizeof(f128)));
+    TRY(h->acc.alloc((total_rounds + (size_t)layers) * 8 * sizeof(uint32_t)));
     {
-        const int kmax = std::max(h->log_leaves - 2, 0);
-        const size_t sc = ((size_t)1 << (kmax / 2)) + ((size_t)1 << (kmax - kmax / 2));
-        if (h->eq_scratch.bytes < sc * sizeof(f128)) TRY(h->eq_scratch.alloc(sc * sizeof(f128)));
+        size_t so = 0;
+        for (int l = 0; l < layers; ++l) {
+            const int n = (int)pts[l].size(), k = n - 1;
+            const f128* r = pdev.f() + pt_off[l];
+            if (k <= bitz::kGkrEqSmallMax) {
+                eq_items.push_back({r, k, eqa.f() + eq_off[l]});
+                continue;
+            }
+            const int kl = k / 2, kh = k - kl;
+            f128* low = eqs.f() + so;
+            f128* high = low + ((size_t)1 << kl);
+            so += ((size_t)1 << kl) + ((size_t)1 << kh);
+            eq_items.push_back({r, kl, low});
+            eq_items.push_back({r + kl, kh, high});
+            bitz::GkrCombine c{};
+            c.low = low;
+            c.high = high;
+            c.out = eqa.f() + eq_off[l];
+            c.n = (size_t)1 << k;
+            c.low_bits = kl;
+            c.blocks = grid_for(c.n);
+            c.block_base = (uint32_t)comb_map.size();
+            comb_map.insert(comb_map.end(), c.blocks, (uint32_t)combines.size());
+            combines.push_back(c);
+        }
     }
+    auto layer_ptrs = [&](int l, f128** left, f128** right) {
+        const int j = layers - 1 - l;  // storage layer (0 = leaves)
+        *left = h->store.f() + h->off[j];
+        *right = *left + ((size_t)1 << pts[l].size());
+    };
+    int max_n = 0;
+    for (int l = 0; l < layers; ++l) max_n = std::max(max_n, (int)pts[l].size());
+    std::vector<std::vector<bitz::GkrRound>> rounds;
+    std::vector<std::vector<uint32_t>> round_maps;
+    for (int i = max_n - 1; i > T; --i) {
+        std::vector<bitz::GkrRound> items;
+        std::vector<uint32_t> map;
+        for (int l = 0; l < layers; ++l) {
+            const int n = (int)pts[l].size();
+            if (i > n - 1) continue;
+            bitz::GkrRound g{};
+            layer_ptrs(l, &g.left, &g.right);
+            g.eq = eqa.f() + eq_off[l];
+            g.h = (size_t)1 << i;
+            g.first = i == n - 1;
+            if (!g.first) g.r = pts[l + 1][i + 1];
+            g.at_one = hostf::is_zero(pts[l][i]);
+            g.acc = h->acc.u32() + 8 * (slot_off[l] + (size_t)(n - 1 - i));
+            g.blocks = grid_for(g.h);
+            g.block_base = (uint32_t)map.size();
+            map.insert(map.end(), g.blocks, (uint32_t)items.size());
+            items.push_back(g);
+        }
+        rounds.push_back(std::move(items));
+        round_maps.push_back(std::move(map));
+    }
+    std::vector<bitz::GkrTail> tails(layers);
+    uint32_t* evals = h->acc.u32() + 8 * total_rounds;
+    for (int l = 0; l < layers; ++l) {
+        const int n = (int)pts[l].size();
+        auto& t = tails[l];
+        layer_ptrs(l, &t.left, &t.right);
+        t.eq = eqa.f() + eq_off[l];
+        t.rtabs4 = static_cast<const uint4*>(rt4.p) + ch_off[l] * kConstTable4Entries;
+        t.acc = h->acc.u32() + 8 * slot_off[l];
+        t.evals = evals + 8 * (size_t)l;
+        t.at_one = 0;
+        for (int i = 0; i < n; ++i)
+            if (hostf::is_zero(pts[l][i])) t.at_one |= 1u << i;
+        t.n = n;
+        t.i0 = std::min(n - 1, T);
+    }
+    // Staging: points | challenges | eq items | combines | map | rounds + maps | tails.
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t o = off;
+        off += bytes;
+        return o;
+    };
+    const size_t eqi_o = place(eq_items.size() * sizeof(bitz::GkrEqItem));
+    const size_t cmb_o = place(combines.size() * sizeof(bitz::GkrCombine));
+    const size_t cmap_o = place(comb_map.size() * 4);
+    std::vector<size_t> rd_o(rounds.size()), rm_o(rounds.size());
+    for (size_t g = 0; g < rounds.size(); ++g) {
+        rd_o[g] = place(rounds[g].size() * sizeof(bitz::GkrRound));
+        rm_o[g] = place(round_maps[g].size() * 4);
+    }
+    const size_t tl_o = place(tails.size() * sizeof(bitz::GkrTail));
+    const size_t desc_bytes = off;
+    std::vector<char> stage(desc_bytes);
+    memcpy(stage.data() + eqi_o, eq_items.data(), eq_items.size() * sizeof(bitz::GkrEqItem));
+    memcpy(stage.data() + cmb_o, combines.data(), combines.size() * sizeof(bitz::GkrCombine));
+    memcpy(stage.data() + cmap_o, comb_map.data(), comb_map.size() * 4);
+    for (size_t g = 0; g < rounds.size(); ++g) {
+        memcpy(stage.data() + rd_o[g], rounds[g].data(), rounds[g].size() * sizeof(bitz::GkrRound));
+        memcpy(stage.data() + rm_o[g], round_maps[g].data(), round_maps[g].size() * 4);
+    }
+    memcpy(stage.data() + tl_o, tails.data(), tails.size() * sizeof(bitz::GkrTail));
     std::vector<f128> flat(std::max<size_t>(pt_total, 1));
     for (int l = 0; l < layers; ++l) std::copy(pts[l].begin(), pts[l].end(), flat.begin() + pt_off[l]);
-    Pinned up;
-    TRY(up.alloc(flat.size() * sizeof(f128)));
-    memcpy(up.p, flat.data(), flat.size() * sizeof(f128));
-    HIP_TRY(hipMemcpyAsync(h->point.p, up.p, flat.size() * sizeof(f128), hipMemcpyHostToDevice, s));
+    TRY(desc.alloc(std::max<size_t>(desc_bytes, 16)));
+    HIP_TRY(hipMemcpyAsync(desc.p, stage.data(), desc_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(pdev.p, flat.data(), flat.size() * sizeof(f128), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(chdev.p, challenges, need * sizeof(f128), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s));
-    uint32_t* evals = h->acc.u32() + 8 * total_rounds;  // [L, R] per layer, 8 words
-    size_t slot = 0;
-    for (int l = 0; l < layers; ++l) {
-        const int j = layers - 1 - l;  // storage layer (0 = leaves)
-        const int n = (int)pts[l].size();
-        f128* left = h->store.f() + h->off[j];
-        f128* right = left + ((size_t)1 << n);
-        TRY(eq_build(h->point.f() + pt_off[l], n - 1, h->eq.f(), h->eq_scratch, s));
-        f128 r_prev{0, 0};
-        for (int i = n - 1; i >= 0; --i) {
-            const int at_one = hostf::is_zero(pts[l][i]);  // sends_one (lib.rs:622-624)
-            uint32_t* acc = h->acc.u32() + 8 * slot;
-            if (i == n - 1) {
-                const size_t half = (size_t)1 << i;
-                h->tm.begin(s, "gkr_sums", 80.0 * (double)half);
-                hipLaunchKernelGGL(bitz::k_gkr_sums, dim3(grid_for(half)), dim3(kBlock), 0, s, left, right,
-                                   h->eq.f(), half, at_one, acc);
-            } else {
-                const size_t hh = (size_t)1 << i;
-                h->tm.begin(s, "gkr_bind_next", 240.0 * (double)hh);
-                hipLaunchKernelGGL(bitz::k_gkr_bind_next, dim3(grid_for(hh)), dim3(kBlock), 0, s, left, right,
-                                   h->eq.f(), hh, r_prev, at_one, acc);
-            }
-            h->tm.end(s);
-            TRY(check_launch());
-            ++slot;
-            r_prev = pts[l + 1][i];
-        }
-        hipLaunchKernelGGL(bitz::k_gkr_bind, dim3(1), dim3(kBlock), 0, s, left, right, (size_t)1, r_prev);
-        TRY(check_launch());
-        HIP_TRY(hipMemcpyAsync(evals + 8 * l, left, 16, hipMemcpyDeviceToDevice, s));
-        HIP_TRY(hipMemcpyAsync(evals + 8 * l + 4, right, 16, hipMemcpyDeviceToDevice, s));
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrTailSmem);
+        attr = true;
     }
+    const char* dp = static_cast<const char*>(desc.p);
+    hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)need), dim3(512), 0, s, chdev.f(),
+                       static_cast<uint4*>(rt4.p));
+    TRY(check_launch());
+    hipLaunchKernelGGL(bitz::k_gkr_eq_items, dim3((unsigned)eq_items.size()), dim3(kBlock), 0, s,
+                       reinterpret_cast<const bitz::GkrEqItem*>(dp + eqi_o));
+    TRY(check_launch());
+    if (!combines.empty()) {
+        hipLaunchKernelGGL(bitz::k_gkr_eq_combine_batch, dim3((unsigned)comb_map.size()), dim3(kBlock), 0, s,
+                           reinterpret_cast<const bitz::GkrCombine*>(dp + cmb_o),
+                           reinterpret_cast<const uint32_t*>(dp + cmap_o));
+        TRY(check_launch());
+    }
+    std::vector<Timed> ts(rounds.size() + 1);
+    for (size_t g = 0; g < rounds.size(); ++g) {
+        double bytes = 0;
+        for (const auto& r : rounds[g]) bytes += (r.first ? 80.0 : 240.0) * (double)r.h;
+        ts[g].begin(s, "gkr_round_batch", bytes);
+        hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3((unsigned)round_maps[g].size()), dim3(kBlock), 0, s,
+                           reinterpret_cast<const bitz::GkrRound*>(dp + rd_o[g]),
+                           reinterpret_cast<const uint32_t*>(dp + rm_o[g]));
+        ts[g].end(s);
+        TRY(check_launch());
+    }
+    ts.back().begin(s, "gkr_tail", 0.0);
+    hipLaunchKernelGGL(bitz::k_gkr_tail, dim3((unsigned)layers), dim3(bitz::kStageBlock), bitz::kGkrTailSmem, s,
+                       reinterpret_cast<const bitz::GkrTail*>(dp + tl_o));
+    ts.back().end(s);
+    TRY(check_launch());
     Pinned down;
     const size_t words = 8 * (total_rounds + (size_t)layers);
     TRY(down.alloc(std::max<size_t>(words, 1) * 4));
     HIP_TRY(hipMemcpyAsync(down.p, h->acc.p, words * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    h->tm.collect();
+    for (auto& t : ts) t.collect();
     const uint32_t* w = static_cast<const uint32_t*>(down.p);
-    slot = 0;
+    size_t slot = 0;
     f128 claim{0, 0};
     for (int l = 0; l < layers; ++l) {
         const int n = (int)pts[l].size();

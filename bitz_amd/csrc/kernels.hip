@@ -1098,4 +1098,160 @@ __global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, c
                                A.la_shift, A.lb, A.lb_shift, A.h_tabs4, A.acc, block);
 }
 
+// ================================ GKR layers in parallel (known challenges) ===
+__global__ __launch_bounds__(kBlock) void k_gkr_eq_items(const GkrEqItem* items) {
+    const GkrEqItem& A = items[blockIdx.x];
+    f128* table = A.out;
+    const int k = A.k;
+    if (threadIdx.x == 0) g_store_hbm(table, g_one());
+    __syncthreads();
+    for (int i = 0; i < k; ++i) {
+        const size_t half = (size_t)1 << i;
+        const g4 ri = g_load_hbm(A.r + i);
+        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
+            const g4 parent = g_load_hbm(table + j);
+            const g4 one = g_mul(parent, ri);
+            g_store_hbm(table + j, g_add(parent, one));
+            g_store_hbm(table + j + half, one);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gkr_eq_combine_batch(const GkrCombine* d, const uint32_t* bmap) {
+    const GkrCombine& A = d[bmap[blockIdx.x]];
+    const size_t mask = ((size_t)1 << A.low_bits) - 1;
+    for (size_t j = (blockIdx.x - A.block_base) * (size_t)blockDim.x + threadIdx.x; j < A.n;
+         j += (size_t)A.blocks * blockDim.x)
+        g_store_hbm(A.out + j, g_mul(g_load_hbm(A.low + (j & mask)), g_load_hbm(A.high + (j >> A.low_bits))));
+}
+
+__global__ __launch_bounds__(kBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
+    const GkrRound& A = d[bmap[blockIdx.x]];
+    const size_t start = (blockIdx.x - A.block_base) * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)A.blocks * blockDim.x;
+    const size_t h = A.h;
+    const int at_one = A.at_one;
+    f128* left = A.left;
+    f128* right = A.right;
+    f128* eq = A.eq;
+    g8 sb = w_zero(), si = w_zero();
+    if (A.first) {
+        for (size_t x = start; x < h; x += stride) {
+            const g4 l0 = g_load_hbm(left + x), l1 = g_load_hbm(left + x + h);
+            const g4 r0 = g_load_hbm(right + x), r1 = g_load_hbm(right + x + h);
+            const g4 e = g_load_hbm(eq + x);
+            clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
+            clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
+        }
+    } else {
+        const g4 rr = g_from(A.r);
+        const size_t hh = 2 * h;
+        for (size_t x = start; x < h; x += stride) {
+            const g4 l0 = interp(g_load_hbm(left + x), g_load_hbm(left + x + hh), rr);
+            const g4 l1 = interp(g_load_hbm(left + x + h), g_load_hbm(left + x + h + hh), rr);
+            const g4 r0 = interp(g_load_hbm(right + x), g_load_hbm(right + x + hh), rr);
+            const g4 r1 = interp(g_load_hbm(right + x + h), g_load_hbm(right + x + h + hh), rr);
+            g_store_hbm(left + x, l0);
+            g_store_hbm(left + x + h, l1);
+            g_store_hbm(right + x, r0);
+            g_store_hbm(right + x + h, r1);
+            const g4 e = g_add(g_load_hbm(eq + x), g_load_hbm(eq + x + h));
+            g_store_hbm(eq + x, e);
+            clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
+            clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
+        }
+    }
+    g4 v[2] = {g_reduce(sb), g_reduce(si)};
+    block_xor_to_global<2>(v, A.acc);
+}
+
+// The layer's arrays in LDS: L and R (2^(i0+2) entries at most), E, the
+// current round's r table, then the lane tables.  Round i (h = 2^i) binds
+// the halves [0, 2h) x [2h, 4h) by r_(i+1) in place (thread x owns entries
+// x, x+h, x+2h, x+3h) and halves E, then sums over the bound [0, h) x [h, 2h);
+// the layer's first round sums without binding.
+__global__ __launch_bounds__(kStageBlock) void k_gkr_tail(const GkrTail* d) {
+    const GkrTail A = d[blockIdx.x];
+    extern __shared__ uint4 lds_raw[];
+    constexpr size_t kArr = (size_t)4 << kGkrTailMax;
+    uint4* Ls = lds_raw;
+    uint4* Rs = Ls + kArr;
+    uint4* Es = Rs + kArr;
+    uint4* rt = Es + ((size_t)2 << kGkrTailMax);
+    uint4* col = rt + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    const int n = A.n, i0 = A.i0;
+    const bool whole = i0 == n - 1;
+    const size_t len = whole ? (size_t)1 << n : (size_t)4 << i0;
+    const size_t elen = whole ? (size_t)1 << (n - 1) : (size_t)2 << i0;
+    auto ld = [](const uint4* p) {
+        const uint4 u = *p;
+        g4 r;
+        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
+        return r;
+    };
+    auto st = [](uint4* p, const g4& v) { *p = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]); };
+    for (size_t e = threadIdx.x; e < len; e += kStageBlock) {
+        Ls[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.left + e));
+        Rs[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.right + e));
+    }
+    for (size_t e = threadIdx.x; e < elen; e += kStageBlock) Es[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.eq + e));
+    g4 nx[kRtabPer];
+    if (!whole) rtab_fetch(nx, A.rtabs4 + (size_t)(n - 1 - (i0 + 1)) * kConstTable4Entries);
+    __syncthreads();
+    for (int i = i0; i >= 0; --i) {
+        const size_t h = (size_t)1 << i;
+        const bool bind = !(whole && i == n - 1);
+        if (bind) {
+            rtab_store(rt, nx);
+            __syncthreads();
+        }
+        const int at_one = (A.at_one >> i) & 1;
+        g4 sb = g_zero(), si = g_zero();
+        for (size_t x = threadIdx.x; x < h; x += kStageBlock) {
+            g4 l0, l1, r0, r1, e;
+            if (bind) {
+                const size_t hh = 2 * h;
+                auto ip = [&](const uint4* p, size_t a, size_t b) {
+                    const g4 u = ld(p + a), v = ld(p + b);
+                    return g_add(u, g_mul_table4(rt, g_add(u, v)));
+                };
+                l0 = ip(Ls, x, x + hh);
+                l1 = ip(Ls, x + h, x + h + hh);
+                r0 = ip(Rs, x, x + hh);
+                r1 = ip(Rs, x + h, x + h + hh);
+                st(Ls + x, l0);
+                st(Ls + x + h, l1);
+                st(Rs + x, r0);
+                st(Rs + x + h, r1);
+                e = g_add(ld(Es + x), ld(Es + x + h));
+                st(Es + x, e);
+            } else {
+                l0 = ld(Ls + x);
+                l1 = ld(Ls + x + h);
+                r0 = ld(Rs + x);
+                r1 = ld(Rs + x + h);
+                e = ld(Es + x);
+            }
+            sb = g_add(sb, g_mul_lane(g_mul_lane(e, at_one ? l1 : l0, col), at_one ? r1 : r0, col));
+            si = g_add(si, g_mul_lane(g_mul_lane(e, g_add(l1, l0), col), g_add(r1, r0), col));
+        }
+        // the next table: r_i binds round i-1 (or the final bind after round 0)
+        rtab_fetch(nx, A.rtabs4 + (size_t)(n - 1 - i) * kConstTable4Entries);
+        g4 v[2] = {sb, si};
+        block_xor_store<2>(v, A.acc + 8 * (size_t)(n - 1 - i), kStageBlock);
+        __syncthreads();  // every round-i reader of rt, L, R and E is done
+    }
+    // Final bind of the two remaining entries by r_0.
+    rtab_store(rt, nx);
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint4* P = threadIdx.x == 0 ? Ls : Rs;
+        const g4 u = ld(P), v = ld(P + 1);
+        const g4 w = g_add(u, g_mul_table4(rt, g_add(u, v)));
+        reinterpret_cast<uint4*>(A.evals)[threadIdx.x] = make_uint4(w.w[0], w.w[1], w.w[2], w.w[3]);
+    }
+}
+
 }  // namespace bitz

@@ -178,4 +178,59 @@ struct BatchPrologue {
 };
 __global__ void k_batch_prologue(const BatchPrologue* items, const uint32_t* bmap);
 
+// ---- GKR layers in parallel (challenges known in advance) ----------------
+// With every challenge known, each layer's point is known, so the layers'
+// sumchecks are independent: small eq tables and the low/high halves of
+// large ones in one launch (k_gkr_eq_items, one workgroup each), the large
+// ones combined in one launch, the grid-wide rounds of all large layers one
+// launch per round index, and every layer's last rounds in one launch of
+// LDS tails (one workgroup per layer).
+struct GkrEqItem {
+    const f128* r;  // k coordinates
+    int k;          // at most kGkrEqSmallMax
+    f128* out;      // 2^k entries
+};
+constexpr int kGkrEqSmallMax = 12;
+__global__ void k_gkr_eq_items(const GkrEqItem* items);
+struct GkrCombine {
+    const f128* low;
+    const f128* high;
+    f128* out;
+    size_t n;
+    int low_bits;
+    uint32_t block_base, blocks;
+};
+__global__ void k_gkr_eq_combine_batch(const GkrCombine* d, const uint32_t* bmap);
+// One layer's round in a batched launch: the first round's sums (first) or
+// bind by r then the next round's sums, as k_gkr_sums / k_gkr_bind_next.
+struct GkrRound {
+    f128* left;
+    f128* right;
+    f128* eq;
+    size_t h;
+    f128 r;
+    uint32_t* acc;
+    int first, at_one;
+    uint32_t block_base, blocks;
+};
+__global__ void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap);
+// A layer's rounds i0 .. 0 and the final bind in one workgroup.  Before
+// round i0 the arrays hold 2^(i0+2) entries each and eq 2^(i0+1) (i0 < n-1),
+// or the layer's 2^n children each and eq(z_<n-1) (i0 = n-1).
+constexpr int kGkrTailMax = 8;  // tail rounds have h = 2^i <= 2^8: arrays of at most 2^10 entries
+struct GkrTail {
+    f128* left;
+    f128* right;
+    f128* eq;
+    const uint4* rtabs4;  // 4-bit table of the layer's r_i at rtabs4 + 512 (n - 1 - i)
+    uint32_t* acc;        // round i's sums at acc + 8 (n - 1 - i)
+    uint32_t* evals;      // [L(r), R(r)]
+    uint32_t at_one;      // bit i: the point's coordinate i is zero
+    int n, i0;
+};
+constexpr size_t kGkrTailSmem =
+    sizeof(uint4) * (2 * ((size_t)4 << kGkrTailMax) + ((size_t)2 << kGkrTailMax) + kConstTable4Entries +
+                     4 * 64 * kLaneTableEntries);
+__global__ void k_gkr_tail(const GkrTail* d);
+
 }  // namespace bitz
