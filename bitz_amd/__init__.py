@@ -477,6 +477,17 @@ BATCH_EAGER = 2
 BATCH_NO_TUNE = 4
 
 
+def _instance_dtype() -> np.dtype:
+    """A numpy view of bitz_outer_instance with the C struct's offsets."""
+    fields = {}
+    for name, ctype in _OuterInstance._fields_:
+        off = getattr(_OuterInstance, name).offset
+        fields[name] = (np.dtype((np.uint64, 2)) if ctype is F128 else
+                        np.dtype(np.uint32) if ctype is C.c_uint32 else np.dtype(np.uint64), off)
+    return np.dtype({"names": list(fields), "formats": [f for f, _ in fields.values()],
+                     "offsets": [o for _, o in fields.values()], "itemsize": C.sizeof(_OuterInstance)})
+
+
 def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
     """Many independent outer sumchecks with challenges fixed in advance, in
     one batched call (bitz_prove_outer_batch).  instances: iterable of
@@ -488,32 +499,48 @@ def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
     tune=False keeps the default tile shapes."""
     init()
     instances = list(instances)
-    arr = (_OuterInstance * max(len(instances), 1))()
-    keep, outs = [], []
-    for i, (ch, claim, eq, prods) in enumerate(instances):
-        (az, na, fa), (bz, nb, _), (cz, nc, _) = (_table(t) for t in (prods.az, prods.bz, prods.cz))
-        if not fa & DEVICE:
+    m = len(instances)
+    ns, chs, tabs = [], [], []
+    for ch, claim, eq, prods in instances:
+        az, bz, cz = prods.az, prods.bz, prods.cz
+        if not all(isinstance(t, DeviceTable) for t in (az, bz, cz)):
             raise BitzError(-7, "batched proofs take DeviceTables")
-        lo, hi = (_factor(f, fa) for f in eq)
-        n = _nvars(na)
-        chs = elems(ch) if n else np.zeros((1, 2), np.uint64)
-        rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
-        points = np.zeros((max(n, 1), 2), dtype=np.uint64)
-        final = np.zeros((1, 2), dtype=np.uint64)
-        claims = np.zeros((3, 2), dtype=np.uint64)
-        x = arr[i]
-        x.challenges, x.initial_claim = _p(chs).value, f128(claim)
-        x.eq_low, x.low_vars = _p(lo).value, _nvars(len(lo))
-        x.eq_high, x.high_vars = _p(hi).value, _nvars(len(hi))
-        x.az, x.bz, x.cz, x.num_vars = _p(az).value, _p(bz).value, _p(cz).value, n
-        x.round_polys, x.eval_points = _p(rounds).value, _p(points).value
-        x.final_claim, x.claims = _p(final).value, _p(claims).value
-        keep.append((chs, lo, hi, rounds, points, final, claims, az, bz, cz))
-        outs.append((n, rounds, points, final, claims))
+        lo, hi = (_factor(f, DEVICE) for f in eq)
+        n = _nvars(az.len)
+        _nvars(bz.len), _nvars(cz.len)
+        ns.append(n)
+        chs.append(elems(ch)[:n] if n else np.zeros((0, 2), np.uint64))
+        tabs.append((claim, lo, hi, az, bz, cz))
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    total = int(off[-1])
+    allch = np.ascontiguousarray(np.concatenate(chs) if total else np.zeros((1, 2), np.uint64), dtype=np.uint64)
+    rounds = np.zeros((max(4 * total, 1), 2), dtype=np.uint64)
+    points = np.zeros((max(total, 1), 2), dtype=np.uint64)
+    finals = np.zeros((max(m, 1), 2), dtype=np.uint64)
+    claims = np.zeros((max(m, 1) * 3, 2), dtype=np.uint64)
+    rec = np.zeros(max(m, 1), dtype=_instance_dtype())
+    if m:
+        rec["challenges"][:m] = allch.ctypes.data + 16 * off[:-1]
+        rec["initial_claim"][:m] = [[c & (2**64 - 1), (c >> 64) & (2**64 - 1)] for c, *_ in tabs]
+        rec["eq_low"][:m] = [t[1].ptr for t in tabs]
+        rec["low_vars"][:m] = [_nvars(t[1].len) for t in tabs]
+        rec["eq_high"][:m] = [t[2].ptr for t in tabs]
+        rec["high_vars"][:m] = [_nvars(t[2].len) for t in tabs]
+        rec["az"][:m] = [t[3].ptr for t in tabs]
+        rec["bz"][:m] = [t[4].ptr for t in tabs]
+        rec["cz"][:m] = [t[5].ptr for t in tabs]
+        rec["num_vars"][:m] = ns
+        rec["round_polys"][:m] = rounds.ctypes.data + 64 * off[:-1]
+        rec["eval_points"][:m] = points.ctypes.data + 16 * off[:-1]
+        rec["final_claim"][:m] = finals.ctypes.data + 16 * np.arange(m)
+        rec["claims"][:m] = claims.ctypes.data + 48 * np.arange(m)
     flags = DEVICE | (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
-    _check(lib().bitz_prove_outer_batch(C.cast(arr, C.c_void_p), len(instances), flags), SumcheckError)
-    return [OuterSumcheckOutput(r[: 4 * n].reshape(n, 4, 2), p[:n], to_int(f[0]), to_int(c[0]), to_int(c[1]),
-                                to_int(c[2])) for n, r, p, f, c in outs]
+    _check(lib().bitz_prove_outer_batch(C.c_void_p(rec.ctypes.data), m, flags), SumcheckError)
+    fin = [x[0] | (x[1] << 64) for x in finals[:m].tolist()]
+    cl = [x[0] | (x[1] << 64) for x in claims[: 3 * m].tolist()]
+    return [OuterSumcheckOutput(rounds[4 * off[i]: 4 * off[i + 1]].reshape(ns[i], 4, 2),
+                                points[off[i]: off[i + 1]], fin[i], cl[3 * i], cl[3 * i + 1], cl[3 * i + 2])
+            for i in range(m)]
 
 
 def outer_batch_config() -> tuple:

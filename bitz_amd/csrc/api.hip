@@ -1719,7 +1719,11 @@ struct BatchCache {
     std::map<std::vector<uint32_t>, BatchConfig> tuned;
     BatchConfig last{0, 0};
     BatchConfig forced{0, 0};  // set by bitz_outer_batch_set_config
-} g_batch;
+};
+// Never destroyed: a cached plan's teardown makes HIP calls, which must not
+// run from static destructors after the runtime (or a tool wrapped around it)
+// has shut down.  bitz_outer_batch_release frees it explicitly.
+BatchCache& g_batch = *new BatchCache;
 
 std::vector<uint32_t> batch_key(const bitz_outer_instance* inst, uint32_t count) {
     std::vector<uint32_t> key;
