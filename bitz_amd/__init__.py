@@ -102,6 +102,7 @@ def lib():
             "bitz_transcript_narg": (SZ, [P, P, SZ]),
             "bitz_mul_batch": (I32, [P, P, P, SZ, U32]),
             "bitz_gkr_prove_with_challenges": (I32, [P, P, U32, P, SZ, P, P, P, P]),
+            "bitz_prove_inner_sumcheck_with_challenges": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
             "bitz_eq_table": (I32, [P, U32, P, U32]),
             "bitz_mle_fold": (I32, [P, C.POINTER(U32), P, U32, U32]),
             "bitz_mle_evaluate": (I32, [P, U32, P, U32, P, U32]),
@@ -633,6 +634,26 @@ def prove_inner_sumcheck(transcript: Transcript, initial_claim, batched_matrix, 
     evals = np.zeros((2, 2), dtype=np.uint64)
     rc = lib().bitz_prove_inner_sumcheck(transcript.h, f128(initial_claim), _p(d), n, _p(w), _nvars(nw), fd,
                                          _p(rounds), _p(points), C.byref(final), _p(evals))
+    _check(rc, SumcheckError)
+    return InnerSumcheckOutput(rounds[: 3 * n].reshape(n, 3, 2), points[:n], to_int(final),
+                               to_int(evals[0]), to_int(evals[1]))
+
+
+def prove_inner_sumcheck_with_challenges(challenges, initial_claim, batched_matrix, witness):
+    """prove_inner_sumcheck with the challenges known in advance
+    (bitz_prove_inner_sumcheck_with_challenges): rounds queued back to back."""
+    init()
+    (d, nd, fd), (w, nw, fw) = _table(batched_matrix), _table(witness)
+    if fd != fw:
+        raise BitzError(-7, "matrix and witness must both be host arrays or both DeviceTables")
+    n = _nvars(nd)
+    ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
+    rounds = np.zeros((max(n, 1) * 3, 2), dtype=np.uint64)
+    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    final = F128()
+    evals = np.zeros((2, 2), dtype=np.uint64)
+    rc = lib().bitz_prove_inner_sumcheck_with_challenges(_p(ch), f128(initial_claim), _p(d), n, _p(w), _nvars(nw),
+                                                         fd, _p(rounds), _p(points), C.byref(final), _p(evals))
     _check(rc, SumcheckError)
     return InnerSumcheckOutput(rounds[: 3 * n].reshape(n, 3, 2), points[:n], to_int(final),
                                to_int(evals[0]), to_int(evals[1]))

@@ -2321,6 +2321,117 @@ done:
     return rc;
 }
 
+// The inner sumcheck with the challenges known in advance: round 0 and the
+// grid-wide fold rounds (lane-table products, r's 4-bit table built on the
+// device) queued back to back, the last rounds in one workgroup, one read of
+// every round's [c0, c2] and of D(r), h(r) at the end.
+extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* challenges, bitz_f128 initial_claim,
+                                                             const bitz_f128* matrix, uint32_t matrix_vars,
+                                                             const bitz_f128* witness, uint32_t witness_vars,
+                                                             uint32_t flags, bitz_f128* round_polys,
+                                                             bitz_f128* eval_points, bitz_f128* final_claim,
+                                                             bitz_f128* evals) {
+    if (!final_claim || !evals || (matrix_vars && (!round_polys || !eval_points || !challenges)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    bitz_inner* h = nullptr;
+    TRY(bitz_inner_begin(&h, matrix, matrix_vars, witness, witness_vars, flags));
+    std::unique_ptr<bitz_inner, void (*)(bitz_inner*)> guard(h, bitz_inner_free);
+    const int n = h->n;
+    hipStream_t s = h->so.s;
+    f128 claim = F(initial_claim);
+    if (n == 0) {
+        HIP_TRY(hipMemcpyAsync(h->host.p, h->cur[0], sizeof(f128), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(h->host.p) + 16, h->cur[1], sizeof(f128), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        memcpy(evals, h->host.p, 32);
+        *final_claim = B(claim);
+        return BITZ_OK;
+    }
+    Dev chal, rt4;
+    TRY(chal.alloc((size_t)n * sizeof(f128)));
+    TRY(rt4.alloc((size_t)n * kConstTable4Entries * sizeof(uint4)));
+    Pinned up;
+    TRY(up.alloc((size_t)n * sizeof(f128)));
+    memcpy(up.p, challenges, (size_t)n * sizeof(f128));
+    HIP_TRY(hipMemcpyAsync(chal.p, up.p, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)n), dim3(512), 0, s, chal.f(), static_cast<uint4*>(rt4.p));
+    TRY(check_launch());
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_r0_lane),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_fold_lane),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerTailSmem);
+        attr = true;
+    }
+    const uint4* rt = static_cast<const uint4*>(rt4.p);
+    const int k0 = std::max(0, n - bitz::kInnerTailMaxLog);
+    auto grid = [](size_t work) {
+        return (unsigned)std::max<size_t>(1, std::min<size_t>((work + bitz::kInnerBlock - 1) / bitz::kInnerBlock, 2048));
+    };
+    std::vector<Timed> ts(k0 + 1);
+    size_t len = h->len;
+    for (int k = 0; k < k0; ++k) {
+        if (k == 0) {
+            ts[k].begin(s, "inner_round0_lane", 32.0 * (double)len);
+            hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(grid(len / 2)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
+                               s, h->cur[0], h->cur[1], len / 2, h->acc.u32());
+        } else {
+            Dev& dst = (h->where == 1) ? h->Q : h->P;
+            const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+            f128* out[2] = {dst.f(), dst.f() + cap};
+            const size_t out_pairs = len / 4;
+            ts[k].begin(s, "inner_fold_lane", 96.0 * (double)len / 2);
+            hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
+                               bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], out_pairs,
+                               rt + (size_t)(k - 1) * kConstTable4Entries, h->acc.u32() + 8 * (size_t)k);
+            h->cur[0] = out[0];
+            h->cur[1] = out[1];
+            h->where = (h->where == 1) ? 2 : 1;
+            len /= 2;
+        }
+        ts[k].end(s);
+        TRY(check_launch());
+    }
+    bitz::InnerTail ta{};
+    ta.d = h->cur[0];
+    ta.w = h->cur[1];
+    ta.rtabs4 = rt;
+    ta.acc = h->acc.u32();
+    ta.fin = reinterpret_cast<f128*>(h->acc.u32() + 8 * (size_t)n);
+    ta.n = n;
+    ta.k0 = k0;
+    ta.fold_first = k0 > 0;
+    ta.lg_in = k0 > 0 ? n - k0 + 1 : n;
+    ts[k0].begin(s, "inner_tail", 0.0);
+    hipLaunchKernelGGL(bitz::k_inner_tail, dim3(1), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ta);
+    ts[k0].end(s);
+    TRY(check_launch());
+    Pinned down;
+    TRY(down.alloc(((size_t)n + 1) * 32));
+    HIP_TRY(hipMemcpyAsync(down.p, h->acc.p, ((size_t)n + 1) * 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (auto& t : ts) t.collect();
+    const uint32_t* w = static_cast<const uint32_t*>(down.p);
+    for (int k = 0; k < n; ++k) {
+        f128 c[3];
+        memcpy(&c[0], w + 8 * k, 16);
+        memcpy(&c[2], w + 8 * k + 4, 16);
+        c[1] = hostf::add(claim, c[2]);  // g(0) + g(1) = claim in characteristic 2 (sumcheck.rs:760-781)
+        const f128 r = F(challenges[k]);
+        claim = hostf::horner(c, 3, r);
+        for (int t = 0; t < 3; ++t) round_polys[3 * k + t] = B(c[t]);
+        eval_points[k] = challenges[k];
+    }
+    memcpy(evals, w + 8 * n, 32);
+    *final_claim = B(claim);
+    return BITZ_OK;
+}
+
 // ========================================================= matrices =======
 struct bitz_matrices {
     size_t rows = 0, rows_padded = 0, cols = 0;

@@ -1098,6 +1098,145 @@ __global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, c
                                A.la_shift, A.lb, A.lb_shift, A.h_tabs4, A.acc, block);
 }
 
+// ======================== inner sumcheck with challenges known in advance ===
+// [c0, c2] of round k = [sum m0 w0, sum (m1+m0)(w1+w0)] over the round's pairs
+// (sumcheck.rs:568-609), products through lane tables.
+__global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs,
+                                                              uint32_t* acc) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    g4 s0 = g_zero(), s2 = g_zero();
+#pragma unroll 1
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pairs; p += (size_t)gridDim.x * blockDim.x) {
+        const g4 m0 = g_load_hbm(d + 2 * p), m1 = g_load_hbm(d + 2 * p + 1);
+        const g4 w0 = g_load_hbm(w + 2 * p), w1 = g_load_hbm(w + 2 * p + 1);
+        s0 = g_add(s0, g_mul_lane(m0, w0, col));
+        __builtin_amdgcn_sched_barrier(0);  // one lane product in flight at a time
+        s2 = g_add(s2, g_mul_lane(g_add(m1, m0), g_add(w1, w0), col));
+    }
+    g4 v[2] = {s0, s2};
+    block_xor_to_global<2>(v, acc);
+}
+
+// Fold by r_{k-1} on load (4 constant-table products per output pair) and
+// round k's sums (fold_and_compute_next_inner_round_coefficients_without_linear,
+// sumcheck.rs:641-688).
+__global__ __launch_bounds__(kInnerBlock) void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout,
+                                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* rt = smem_raw;
+    uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = u4_load_hbm(rtab4 + e);
+    __syncthreads();
+    g4 s0 = g_zero(), s2 = g_zero();
+    auto ip = [&](const g4& a, const g4& b) { return g_add(a, g_mul_table4(rt, g_add(a, b))); };
+#pragma unroll 1
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < out_pairs;
+         p += (size_t)gridDim.x * blockDim.x) {
+        g4 in[2][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            in[0][j] = g_load_hbm(d + 4 * p + j);
+            in[1][j] = g_load_hbm(w + 4 * p + j);
+        }
+        const g4 m0 = ip(in[0][0], in[0][1]), m1 = ip(in[0][2], in[0][3]);
+        const g4 w0 = ip(in[1][0], in[1][1]), w1 = ip(in[1][2], in[1][3]);
+        g_store_hbm(dout + 2 * p, m0);
+        g_store_hbm(dout + 2 * p + 1, m1);
+        g_store_hbm(wout + 2 * p, w0);
+        g_store_hbm(wout + 2 * p + 1, w1);
+        s0 = g_add(s0, g_mul_lane(m0, w0, col));
+        __builtin_amdgcn_sched_barrier(0);  // one lane product in flight at a time
+        s2 = g_add(s2, g_mul_lane(g_add(m1, m0), g_add(w1, w0), col));
+    }
+    g4 v[2] = {s0, s2};
+    block_xor_to_global<2>(v, acc);
+}
+
+// Rounds k0 .. n-1 in one workgroup: the two tables (folded by r_{k0-1} on
+// load when fold_first) in LDS, each round's sums stored at acc + 8 k, then a
+// fold by r_k; D(r) and h(r) at fin.
+__global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
+    extern __shared__ uint4 lds_raw[];
+    constexpr size_t kCap = (size_t)1 << kInnerTailMaxLog;
+    uint4* Ds = lds_raw;
+    uint4* Ws = Ds + kCap;
+    uint4* rt = Ws + kCap;
+    uint4* col = rt + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    auto ld = [](const uint4* p) {
+        const uint4 u = *p;
+        g4 r;
+        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
+        return r;
+    };
+    auto st = [](uint4* p, const g4& v) { *p = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]); };
+    int lg = A.lg_in - A.fold_first;
+    const size_t len = (size_t)1 << lg;
+    g4 nx[kRtabPer];
+    if (A.fold_first) {
+        rtab_fetch(nx, A.rtabs4 + (size_t)(A.k0 - 1) * kConstTable4Entries);
+        rtab_store(rt, nx);
+        __syncthreads();
+        for (size_t e = threadIdx.x; e < 2 * len; e += kStageBlock) {
+            const f128* src = e < len ? A.d + 2 * e : A.w + 2 * (e - len);
+            const g4 a = g_load_hbm(src), b = g_load_hbm(src + 1);
+            st((e < len ? Ds + e : Ws + (e - len)), g_add(a, g_mul_table4(rt, g_add(a, b))));
+        }
+    } else {
+        for (size_t e = threadIdx.x; e < len; e += kStageBlock) {
+            Ds[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.d + e));
+            Ws[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.w + e));
+        }
+    }
+    __syncthreads();
+    for (int k = A.k0; k < A.n; ++k) {
+        const size_t P = (size_t)1 << (lg - 1);
+        rtab_fetch(nx, A.rtabs4 + (size_t)k * kConstTable4Entries);
+        g4 s0 = g_zero(), s2 = g_zero();
+        for (size_t p = threadIdx.x; p < P; p += kStageBlock) {
+            const g4 m0 = ld(Ds + 2 * p), m1 = ld(Ds + 2 * p + 1);
+            const g4 w0 = ld(Ws + 2 * p), w1 = ld(Ws + 2 * p + 1);
+            s0 = g_add(s0, g_mul_lane(m0, w0, col));
+            __builtin_amdgcn_sched_barrier(0);
+            s2 = g_add(s2, g_mul_lane(g_add(m1, m0), g_add(w1, w0), col));
+        }
+        g4 v[2] = {s0, s2};
+        block_xor_store<2>(v, A.acc + 8 * (size_t)k, kStageBlock);
+        __syncthreads();  // readers of rt done (and of the tables, before the fold writes)
+        rtab_store(rt, nx);
+        __syncthreads();
+        // fold both tables in place: read, barrier, write (2P <= 2^kInnerTailMaxLog
+        // outputs, at most kFold per thread)
+        constexpr int kFold = (2 << (kInnerTailMaxLog - 1)) / kStageBlock;
+        g4 nv[kFold];
+#pragma unroll
+        for (int q = 0; q < kFold; ++q) {
+            const size_t e = threadIdx.x + (size_t)q * kStageBlock;
+            if (e < 2 * P) {
+                const uint4* T = e < P ? Ds : Ws;
+                const size_t j = e < P ? e : e - P;
+                const g4 a = ld(T + 2 * j), b = ld(T + 2 * j + 1);
+                nv[q] = g_add(a, g_mul_table4(rt, g_add(a, b)));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kFold; ++q) {
+            const size_t e = threadIdx.x + (size_t)q * kStageBlock;
+            if (e < 2 * P) {
+                uint4* T = e < P ? Ds : Ws;
+                st(T + (e < P ? e : e - P), nv[q]);
+            }
+        }
+        __syncthreads();
+        lg -= 1;
+    }
+    if (threadIdx.x < 2) reinterpret_cast<uint4*>(A.fin)[threadIdx.x] = threadIdx.x == 0 ? Ds[0] : Ws[0];
+}
+
 // ================================ GKR layers in parallel (known challenges) ===
 __global__ __launch_bounds__(kBlock) void k_gkr_eq_items(const GkrEqItem* items) {
     const GkrEqItem& A = items[blockIdx.x];

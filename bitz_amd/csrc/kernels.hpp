@@ -178,6 +178,28 @@ struct BatchPrologue {
 };
 __global__ void k_batch_prologue(const BatchPrologue* items, const uint32_t* bmap);
 
+// ---- inner sumcheck with challenges known in advance ----------------------
+// Round 0 (k_inner_r0_lane) and the grid-wide fold rounds (k_inner_fold_lane,
+// fold by r_{k-1} on load through r's 4-bit table) with lane-table products,
+// then the last rounds in one workgroup (k_inner_tail).
+constexpr int kInnerBlock = 256;
+constexpr size_t kInnerGridSmem = sizeof(uint4) * (kConstTable4Entries + 4 * 64 * kLaneTableEntries);
+__global__ void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
+__global__ void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout, size_t out_pairs,
+                                  const uint4* rtab4, uint32_t* acc);
+constexpr int kInnerTailMaxLog = 10;  // at most 2^10 entries per table in LDS
+struct InnerTail {
+    const f128* d;
+    const f128* w;
+    const uint4* rtabs4;  // r_k's table at rtabs4 + 512 k
+    uint32_t* acc;        // round k's [c0, c2] at acc + 8 k
+    f128* fin;            // D(r), h(r)
+    int n, k0, fold_first, lg_in;
+};
+constexpr size_t kInnerTailSmem =
+    sizeof(uint4) * (((size_t)2 << kInnerTailMaxLog) + kConstTable4Entries + 4 * 64 * kLaneTableEntries);
+__global__ void k_inner_tail(InnerTail args);
+
 // ---- GKR layers in parallel (challenges known in advance) ----------------
 // With every challenge known, each layer's point is known, so the layers'
 // sumchecks are independent: small eq tables and the low/high halves of

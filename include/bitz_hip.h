@@ -209,6 +209,17 @@ int32_t bitz_outer_tables(bitz_outer* h, bitz_f128* az, bitz_f128* bz, bitz_f128
 void bitz_outer_free(bitz_outer* h);
 
 /* ---- Spartan inner sumcheck ------------------------------------------------- */
+/* prove_inner_sumcheck with the challenges known in advance (a transcript
+ * whose challenges do not depend on the messages, such as the seeded
+ * stream): all rounds queued back to back, one read-back; the outputs equal
+ * bitz_prove_inner_sumcheck's under a transcript that squeezes the same
+ * challenges.  No counterpart in the reference, which draws them round by
+ * round (crates/spartan/src/sumcheck.rs:442-552). */
+int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* challenges, bitz_f128 initial_claim,
+                                                  const bitz_f128* matrix, uint32_t matrix_vars,
+                                                  const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
+                                                  bitz_f128* round_polys, bitz_f128* eval_points,
+                                                  bitz_f128* final_claim, bitz_f128* evals);
 /* prove_inner_sumcheck (sumcheck.rs:442-552): round_polys[num_vars][3],
  * evals[2] = [D(r_y), h(r_y)]. */
 int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,

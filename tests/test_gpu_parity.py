@@ -149,6 +149,26 @@ def test_inner_sumcheck_matches_oracle(n):
     assert O.squeeze(ot) == gt.squeeze()
 
 
+@pytest.mark.parametrize("n", [0, 1, 2, 9, 10, 11, 12, 16, 20])
+def test_inner_sumcheck_with_challenges_matches_oracle(n):
+    """The queued inner prover (grid-wide rounds, then the LDS tail) against
+    the oracle on the same challenge list, host and device tables."""
+    d, w = elements_fast(500 + n, 1 << n), elements_fast(600 + n, 1 << n)
+    claim = O.inner_product(d, w)
+    ch = elements(700 + n, n) if n else np.zeros((0, 2), np.uint64)
+    if n > 4:
+        ch[2] = 0  # a zero challenge folds to the low half
+    rc, ref = O.inner_sumcheck(O.ListTranscript(ints(ch)), claim, d, w)
+    assert rc == 0
+    for dev in (False, True):
+        args = (G.DeviceTable(d), G.DeviceTable(w)) if dev else (d, w)
+        out = G.prove_inner_sumcheck_with_challenges(ch, claim, *args)
+        assert np.array_equal(out.round_polynomials, ref["rounds"])
+        assert np.array_equal(out.eval_points, ref["points"])
+        assert out.final_claim == ref["final_claim"]
+        assert [out.batched_matrix_evaluation, out.witness_evaluation] == ints(ref["evals"])
+
+
 def test_inner_dimension_error():
     with pytest.raises(G.SumcheckError) as e:
         G.prove_inner_sumcheck(G.Transcript.seeded(1), 0, from_ints([1, 2]), from_ints([1, 2, 3, 4]))
