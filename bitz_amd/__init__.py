@@ -46,6 +46,9 @@ class BitzError(RuntimeError):
         -7: "InvalidArgument",
         -8: "InvalidState",
         -9: "InvalidPointLength",
+        -10: "InvalidAssignmentDimensions",
+        -11: "InvalidAssignmentLength",
+        -12: "InvalidAssignmentConstant",
         -100: "DeviceError",
         -101: "AllocationError",
     }
@@ -103,6 +106,10 @@ def lib():
             "bitz_mul_batch": (I32, [P, P, P, SZ, U32]),
             "bitz_gkr_prove_with_challenges": (I32, [P, P, U32, P, SZ, P, P, P, P]),
             "bitz_prove_inner_sumcheck_with_challenges": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
+            "bitz_r1cs_products": (I32, [P, P, P, P, P, U32]),
+            "bitz_assignment_mle": (I32, [P, SZ, SZ, P, U32]),
+            "bitz_constraint_matrix_digest": (I32, [SZ, SZ, P, P, SZ, SZ, P, P, P, P]),
+            "bitz_prove_spartan_piop": (I32, [P, P, P, P, P, P, U32, P, U32, U32, P, P, P, P, P, P]),
             "bitz_eq_table": (I32, [P, U32, P, U32]),
             "bitz_mle_fold": (I32, [P, C.POINTER(U32), P, U32, U32]),
             "bitz_mle_evaluate": (I32, [P, U32, P, U32, P, U32]),
@@ -685,11 +692,95 @@ class PreparedConstraintMatrices:
         _check(lib().bitz_bind_and_batch(self.h, _p(row_point), len(row_point), f128(rho), _p(out), 0))
         return out
 
+    def r1cs_products(self, assignment):
+        """Az, Bz, Cz = A h, B h, C h padded to 2^num_row_vars (the reference
+        test's multiply + padded_mle, crates/spartan/src/piop.rs:352-372):
+        host arrays for a host assignment, DeviceTables for a DeviceTable."""
+        h, n, fl = _table(assignment)
+        if n != 1 << self.num_column_vars:
+            raise BitzError(-7, "the assignment must span the padded columns")
+        rows = 1 << self.num_row_vars
+        if fl & DEVICE:
+            outs = [DeviceTable(length=rows) for _ in range(3)]
+            ptrs = [C.c_void_p(o.ptr) for o in outs]
+        else:
+            outs = [np.zeros((rows, 2), dtype=np.uint64) for _ in range(3)]
+            ptrs = [_p(o) for o in outs]
+        _check(lib().bitz_r1cs_products(self.h, _p(h), *ptrs, fl))
+        return R1csProductMles(*outs)
+
     def __del__(self):
         try:
             lib().bitz_matrices_free(self.h)
         except Exception:
             pass
+
+
+def assignment_mle(bits, columns: int) -> np.ndarray:
+    """build_assignment_mle (crates/spartan/src/matrix.rs:392-420): a
+    Boolean assignment (array of 0/1) as {0, 1} elements padded to a power of
+    two.  The length and constant-bit checks run before any device work."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint8)
+    words = np.packbits(bits, bitorder="little")
+    words = np.concatenate([words, np.zeros((-len(words)) % 8, np.uint8)]).view(np.uint64) if len(words) else \
+        np.zeros(1, np.uint64)
+    padded = 1 << max(int(columns - 1).bit_length(), 0) if columns > 1 else 1
+    out = np.zeros((padded, 2), dtype=np.uint64)
+    _check(lib().bitz_assignment_mle(_p(words), len(bits), columns, _p(out), 0))
+    return out
+
+
+def constraint_matrix_digest(m_csr, m_columns: int, rows: int, columns: int, csr) -> bytes:
+    """constraint_matrix_digest (crates/spartan/src/matrix.rs:562-627): m_csr =
+    (row_starts, positions) of M; csr = three (row_starts, columns, values).
+    Host computation (no device)."""
+    m_starts = np.ascontiguousarray(m_csr[0], dtype=np.uint64)
+    m_pos = np.ascontiguousarray(m_csr[1], dtype=np.uint32) if len(m_csr[1]) else np.zeros(1, np.uint32)
+    starts = [np.ascontiguousarray(m[0], dtype=np.uint64) for m in csr]
+    cols = [np.ascontiguousarray(m[1], dtype=np.uint32) if len(m[1]) else np.zeros(1, np.uint32) for m in csr]
+    vals = [elems(m[2]) if len(m[2]) else np.zeros((1, 2), np.uint64) for m in csr]
+    arr_s = (C.c_void_p * 3)(*[x.ctypes.data for x in starts])
+    arr_c = (C.c_void_p * 3)(*[x.ctypes.data for x in cols])
+    arr_v = (C.c_void_p * 3)(*[x.ctypes.data for x in vals])
+    out = (C.c_uint8 * 32)()
+    _check(lib().bitz_constraint_matrix_digest(len(m_starts) - 1, m_columns, _p(m_starts), _p(m_pos), rows, columns,
+                                               arr_s, arr_c, arr_v, out))
+    return bytes(out)
+
+
+@dataclass
+class SpartanPiopOutput:
+    """SpartanPiopProof (crates/spartan/src/piop.rs:18-22) and its terminal
+    ScaledMleEvaluationClaim (piop.rs:101-105)."""
+    outer_rounds: np.ndarray     # (row_vars, 4, 2)
+    outer_claims: list           # [Az, Bz, Cz](r_x)
+    inner_rounds: np.ndarray     # (col_vars, 3, 2)
+    inner_point: np.ndarray      # r_y
+    matrix_evaluation: int       # D(r_y)
+    final_claim: int
+
+
+def prove_spartan_piop(transcript: Transcript, matrices: "PreparedConstraintMatrices", digest: bytes,
+                       products: R1csProductMles, assignment) -> SpartanPiopOutput:
+    """prove_spartan_piop (crates/spartan/src/piop.rs:67-112) with the engine's
+    outer sumcheck, bind_and_batch and inner sumcheck."""
+    init()
+    (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
+    hv, nh, fh = _table(assignment)
+    if fh != fa:
+        raise BitzError(-7, "products and assignment must be on the same side")
+    rv, cv = _nvars(na), _nvars(nh)
+    outer = np.zeros((max(rv, 1) * 4, 2), dtype=np.uint64)
+    claims = np.zeros((3, 2), dtype=np.uint64)
+    inner = np.zeros((max(cv, 1) * 3, 2), dtype=np.uint64)
+    point = np.zeros((max(cv, 1), 2), dtype=np.uint64)
+    meval, final = F128(), F128()
+    dg = (C.c_uint8 * 32).from_buffer_copy(digest)
+    _check(lib().bitz_prove_spartan_piop(transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz), rv, _p(hv), cv, fa,
+                                         _p(outer), _p(claims), _p(inner), _p(point), C.byref(meval),
+                                         C.byref(final)), SumcheckError)
+    return SpartanPiopOutput(outer[: 4 * rv].reshape(rv, 4, 2), [to_int(c) for c in claims],
+                             inner[: 3 * cv].reshape(cv, 3, 2), point[:cv], to_int(meval), to_int(final))
 
 
 class MaterializedMTranspose:

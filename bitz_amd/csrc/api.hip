@@ -2437,6 +2437,7 @@ struct bitz_matrices {
     size_t rows = 0, rows_padded = 0, cols = 0;
     int row_vars = 0, col_vars = 0;
     Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
+    Dev rs[3], cs[3], vs[3];  // row-major copies of A, B, C (products)
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
@@ -2499,6 +2500,18 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
         (nnz && hipMemcpy(h->which.p, whichv.data(), nnz, hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(h->values.p, vals.data(), vals.size() * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "matrix upload failed"));
+    for (int m = 0; m < 3; ++m) {
+        const size_t mn = row_starts[m][rows];
+        std::vector<uint32_t> rs32(rows + 1);
+        for (size_t i = 0; i <= rows; ++i) rs32[i] = (uint32_t)row_starts[m][i];
+        if ((rc = h->rs[m].alloc((rows + 1) * 4)) || (rc = h->cs[m].alloc(std::max<size_t>(mn, 1) * 4)) ||
+            (rc = h->vs[m].alloc(std::max<size_t>(mn, 1) * sizeof(f128))))
+            return cleanup(rc);
+        if (hipMemcpy(h->rs[m].p, rs32.data(), (rows + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            (mn && hipMemcpy(h->cs[m].p, columns[m], mn * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+            (mn && hipMemcpy(h->vs[m].p, values[m], mn * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess))
+            return cleanup(fail(BITZ_ERR_DEVICE, "matrix upload failed"));
+    }
     *out = h;
     return BITZ_OK;
 }
@@ -2532,6 +2545,165 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_r1cs_products(bitz_matrices* h, const bitz_f128* assignment, bitz_f128* az, bitz_f128* bz,
+                                      bitz_f128* cz, uint32_t flags) {
+    if (!h || !assignment || !az || !bz || !cz) return fail(BITZ_ERR_ARG, "null pointer");
+    hipStream_t s = h->so.s;
+    Dev din, dout;
+    const f128* hv = reinterpret_cast<const f128*>(assignment);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(upload(din, assignment, h->cols, 0, s));
+        hv = din.f();
+        TRY(dout.alloc(3 * h->rows_padded * sizeof(f128)));
+    }
+    bitz_f128* outs[3] = {az, bz, cz};
+    for (int m = 0; m < 3; ++m) {
+        f128* target = (flags & BITZ_DEVICE) ? reinterpret_cast<f128*>(outs[m]) : dout.f() + m * h->rows_padded;
+        hipLaunchKernelGGL(bitz::k_r1cs_rows, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->rs[m].u32(),
+                           h->cs[m].u32(), h->vs[m].f(), hv, h->rows, target, h->rows_padded);
+        TRY(check_launch());
+    }
+    if (!(flags & BITZ_DEVICE))
+        for (int m = 0; m < 3; ++m)
+            HIP_TRY(hipMemcpyAsync(outs[m], dout.f() + m * h->rows_padded, h->rows_padded * sizeof(f128),
+                                   hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_assignment_mle(const uint64_t* words, size_t bit_len, size_t columns, bitz_f128* out,
+                                       uint32_t flags) {
+    if (!out || (bit_len && !words)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (bit_len != columns) return fail(BITZ_ERR_ASSIGNMENT_LEN, "assignment length differs from the columns");
+    if (columns == 0 || !(words[0] & 1)) return fail(BITZ_ERR_ASSIGNMENT_CONST, "the first assignment bit must be one");
+    size_t padded = 1;
+    while (padded < columns) padded <<= 1;
+    StreamOwner so;
+    TRY(so.create());
+    Dev dw, dout;
+    const size_t nw = (bit_len + 63) / 64;
+    TRY(dw.alloc(nw * 8));
+    HIP_TRY(hipMemcpyAsync(dw.p, words, nw * 8, hipMemcpyHostToDevice, so.s));
+    f128* target = reinterpret_cast<f128*>(out);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(dout.alloc(padded * sizeof(f128)));
+        target = dout.f();
+    }
+    hipLaunchKernelGGL(bitz::k_assignment_bits, dim3(grid_for(padded)), dim3(kBlock), 0, so.s,
+                       static_cast<const uint64_t*>(dw.p), bit_len, target, padded);
+    TRY(check_launch());
+    if (!(flags & BITZ_DEVICE))
+        HIP_TRY(hipMemcpyAsync(out, target, padded * sizeof(f128), hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipStreamSynchronize(so.s));
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_constraint_matrix_digest(size_t m_rows, size_t m_columns, const uint64_t* m_row_starts,
+                                                 const uint32_t* m_positions, size_t rows, size_t columns,
+                                                 const uint64_t* const row_starts[3], const uint32_t* const cols[3],
+                                                 const bitz_f128* const values[3], uint8_t digest[32]) {
+    if (!digest || !m_row_starts || !row_starts || !cols || !values) return fail(BITZ_ERR_ARG, "null pointer");
+    bitz::Sha256 hs;
+    static const char kDomain[] = "bitz/spartan/constraint-matrices/v1";
+    hs.update(kDomain, sizeof kDomain - 1);
+    hs.update("M", 1);
+    hs.update_u64(m_rows);
+    hs.update_u64(m_columns);
+    for (size_t i = 0; i < m_rows; ++i) {
+        hs.update_u64(m_row_starts[i + 1] - m_row_starts[i]);
+        for (uint64_t k = m_row_starts[i]; k < m_row_starts[i + 1]; ++k) hs.update_u64(m_positions[k]);
+    }
+    static const char* kLabel[3] = {"A", "B", "C"};
+    for (int m = 0; m < 3; ++m) {
+        if (!row_starts[m]) return fail(BITZ_ERR_ARG, "null row_starts");
+        hs.update(kLabel[m], 1);
+        hs.update_u64(rows);
+        hs.update_u64(columns);
+        for (size_t i = 0; i < rows; ++i) {
+            hs.update_u64(row_starts[m][i + 1] - row_starts[m][i]);
+            for (uint64_t k = row_starts[m][i]; k < row_starts[m][i + 1]; ++k) {
+                hs.update_u64(cols[m][k]);
+                hs.update_u64(16);  // the coefficient's encoding: F128::to_bytes (crates/field/src/codec.rs:24-28)
+                hs.update(&values[m][k], 16);
+            }
+        }
+    }
+    hs.finish(digest);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
+                                           const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
+                                           uint32_t row_vars, const bitz_f128* assignment, uint32_t col_vars,
+                                           uint32_t flags, bitz_f128* outer_rounds, bitz_f128* outer_claims,
+                                           bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
+                                           bitz_f128* final_claim) {
+    if (!t || !h || !digest || !az || !bz || !cz || !assignment || !outer_claims || !matrix_eval || !final_claim ||
+        (row_vars && !outer_rounds) || (col_vars && (!inner_rounds || !inner_point)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    // piop.rs:74-83: dimensions before any transcript call.
+    if ((int)row_vars != h->row_vars) return fail(BITZ_ERR_PRODUCT_DIMS, "products do not span the row domain");
+    if ((int)col_vars != h->col_vars) return fail(BITZ_ERR_ASSIGNMENT_DIMS, "assignment does not span the columns");
+    hipStream_t s = h->so.s;
+    const bool dev = (flags & BITZ_DEVICE) != 0;
+    // piop.rs:85-90: digest, tau, make_equality_factors (crates/poly/src/eq.rs:107-115).
+    t->t.absorb(0, digest, 32);
+    std::vector<f128> tau(row_vars);
+    for (uint32_t i = 0; i < row_vars; ++i) tau[i] = t->t.squeeze();
+    const uint32_t split = row_vars / 2;
+    auto host_eq = [](const f128* r, uint32_t k) {  // eq_table (eq.rs:63-99)
+        std::vector<f128> tbl((size_t)1 << k, f128{0, 0});
+        tbl[0] = hostf::one();
+        for (uint32_t i = 0; i < k; ++i) {
+            const size_t half = (size_t)1 << i;
+            for (size_t j = 0; j < half; ++j) {
+                const f128 one = hostf::mul(tbl[j], r[i]);
+                tbl[j] = hostf::add(tbl[j], one);
+                tbl[j + half] = one;
+            }
+        }
+        return tbl;
+    };
+    const std::vector<f128> lo = host_eq(tau.data(), split), hi = host_eq(tau.data() + split, row_vars - split);
+    Dev dlo, dhi, drx, dD, dh;
+    const bitz_f128* eq_lo = reinterpret_cast<const bitz_f128*>(lo.data());
+    const bitz_f128* eq_hi = reinterpret_cast<const bitz_f128*>(hi.data());
+    if (dev) {
+        TRY(upload(dlo, eq_lo, lo.size(), 0, s));
+        TRY(upload(dhi, eq_hi, hi.size(), 0, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        eq_lo = reinterpret_cast<const bitz_f128*>(dlo.p);
+        eq_hi = reinterpret_cast<const bitz_f128*>(dhi.p);
+    }
+    // piop.rs:91: the outer sumcheck from claim 0.
+    std::vector<bitz_f128> rx(std::max<uint32_t>(row_vars, 1));
+    bitz_f128 outer_final;
+    TRY(bitz_prove_outer_sumcheck(t, bitz_f128{0, 0}, eq_lo, split, eq_hi, row_vars - split, az, row_vars, bz,
+                                  row_vars, cz, row_vars, flags & BITZ_DEVICE, outer_rounds, rx.data(), &outer_final,
+                                  outer_claims));
+    // piop.rs:94-98: rho, the inner claim, D = bind_and_batch(r_x, rho), on the device.
+    const f128 rho = t->t.squeeze();
+    const f128 a = F(outer_claims[0]), b = F(outer_claims[1]), c = F(outer_claims[2]);
+    const f128 inner_claim = hostf::add(a, hostf::add(hostf::mul(rho, b), hostf::mul(hostf::mul(rho, rho), c)));
+    TRY(upload(drx, rx.data(), std::max<uint32_t>(row_vars, 1), 0, s));
+    TRY(dD.alloc(h->cols * sizeof(f128)));
+    HIP_TRY(hipStreamSynchronize(s));
+    TRY(bitz_bind_and_batch(h, reinterpret_cast<const bitz_f128*>(drx.p), row_vars, B(rho),
+                            reinterpret_cast<bitz_f128*>(dD.p), BITZ_DEVICE));
+    // piop.rs:99: the inner sumcheck against the assignment.
+    const bitz_f128* hv = assignment;
+    if (!dev) {
+        TRY(upload(dh, assignment, h->cols, 0, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        hv = reinterpret_cast<const bitz_f128*>(dh.p);
+    }
+    bitz_f128 evals[2];
+    TRY(bitz_prove_inner_sumcheck(t, B(inner_claim), reinterpret_cast<const bitz_f128*>(dD.p), col_vars, hv, col_vars,
+                                  BITZ_DEVICE, inner_rounds, inner_point, final_claim, evals));
+    *matrix_eval = evals[0];  // ScaledMleEvaluationClaim (piop.rs:101-105): D(r_y) scales h(r_y)
     return BITZ_OK;
 }
 

@@ -341,6 +341,34 @@ __global__ __launch_bounds__(kBlock) void k_bind_columns(const uint32_t* col_sta
     }
 }
 
+// out[i] = sum over row i's nonzeros of value * h[column] for i < rows, zero
+// above (the reference test's multiply + padded_mle, crates/spartan/src/piop.rs:352-372).
+__global__ __launch_bounds__(kBlock) void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols,
+                                                      const f128* vals, const f128* h, size_t rows, f128* out,
+                                                      size_t padded) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < padded; i += (size_t)gridDim.x * blockDim.x) {
+        if (i >= rows) {
+            g_store(out + i, g_zero());
+            continue;
+        }
+        g8 s = w_zero();
+        for (uint32_t k = row_starts[i]; k < row_starts[i + 1]; ++k) clmul_acc(s, g_load(vals + k), g_load(h + cols[k]));
+        g_store(out + i, g_reduce(s));
+    }
+}
+
+// build_assignment_mle (matrix.rs:392-420): bit i of the packed words as 0 / 1,
+// zeros up to the padded length.
+__global__ __launch_bounds__(kBlock) void k_assignment_bits(const uint64_t* words, size_t bit_len, f128* out,
+                                                            size_t padded) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < padded; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t b = i < bit_len ? (uint32_t)((words[i / 64] >> (i % 64)) & 1) : 0u;
+        g4 v = g_zero();
+        v.w[0] = b;
+        g_store(out + i, v);
+    }
+}
+
 // row_scale[m][i] = eq(i, r_x) * scale_m.
 __global__ __launch_bounds__(kBlock) void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
                              size_t rows_padded) {

@@ -38,6 +38,9 @@ __global__ void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
                                size_t rows_padded, f128* out, size_t cols);
 __global__ void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
                              size_t rows_padded);
+__global__ void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols, const f128* vals, const f128* h,
+                            size_t rows, f128* out, size_t padded);
+__global__ void k_assignment_bits(const uint64_t* words, size_t bit_len, f128* out, size_t padded);
 __global__ void k_mt_apply(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
                            f128* out, size_t cols);
 __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* rows,

@@ -53,17 +53,42 @@ void compress(uint32_t h[8], const uint8_t* p) {
 }
 }  // namespace
 
-void sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
-    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
-                     0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
-    size_t full = len / 64;
-    for (size_t i = 0; i < full; ++i) compress(h, msg + 64 * i);
+Sha256::Sha256() {
+    static const uint32_t iv[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                                   0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    memcpy(h, iv, sizeof h);
+}
+
+void Sha256::update(const void* data, size_t len) {
+    const uint8_t* p = static_cast<const uint8_t*>(data);
+    total += len;
+    if (fill) {
+        const size_t take = len < 64 - fill ? len : 64 - fill;
+        memcpy(buf + fill, p, take);
+        fill += take;
+        p += take;
+        len -= take;
+        if (fill < 64) return;
+        compress(h, buf);
+        fill = 0;
+    }
+    for (; len >= 64; p += 64, len -= 64) compress(h, p);
+    memcpy(buf, p, len);
+    fill = len;
+}
+
+void Sha256::update_u64(uint64_t v) {
+    uint8_t b[8];
+    for (int i = 0; i < 8; ++i) b[i] = (uint8_t)(v >> (8 * i));
+    update(b, 8);
+}
+
+void Sha256::finish(uint8_t out[32]) {
+    const uint64_t bits = total * 8;
     uint8_t tail[128] = {0};
-    const size_t rem = len - 64 * full;
-    memcpy(tail, msg + 64 * full, rem);
-    tail[rem] = 0x80;
-    const size_t tl = rem + 9 <= 64 ? 64 : 128;
-    const uint64_t bits = (uint64_t)len * 8;
+    memcpy(tail, buf, fill);
+    tail[fill] = 0x80;
+    const size_t tl = fill + 9 <= 64 ? 64 : 128;
     for (int i = 0; i < 8; ++i) tail[tl - 1 - i] = (uint8_t)(bits >> (8 * i));
     compress(h, tail);
     if (tl == 128) compress(h, tail + 64);
@@ -73,6 +98,12 @@ void sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
         out[4 * i + 2] = (uint8_t)(h[i] >> 8);
         out[4 * i + 3] = (uint8_t)h[i];
     }
+}
+
+void sha256(const uint8_t* msg, size_t len, uint8_t out[32]) {
+    Sha256 s;
+    s.update(msg, len);
+    s.finish(out);
 }
 
 uint64_t splitmix_next(uint64_t& x) {

@@ -11,6 +11,18 @@
 namespace bitz {
 
 void sha256(const uint8_t* msg, size_t len, uint8_t out[32]);
+
+// Streaming SHA-256 (FIPS 180-4).
+struct Sha256 {
+    uint32_t h[8];
+    uint8_t buf[64];
+    size_t fill = 0;
+    uint64_t total = 0;
+    Sha256();
+    void update(const void* data, size_t len);
+    void update_u64(uint64_t v);  // 8 little-endian bytes
+    void finish(uint8_t out[32]);
+};
 uint64_t splitmix_next(uint64_t& state);
 
 struct Transcript {

@@ -46,6 +46,9 @@ enum {
     BITZ_ERR_ARG = -7,           /* bad argument (null pointer, size out of range) */
     BITZ_ERR_STATE = -8,         /* handle used out of order */
     BITZ_ERR_POINT_LEN = -9,     /* GKR point length != tree bits (crates/gkr/src/lib.rs:39-43 panics) */
+    BITZ_ERR_ASSIGNMENT_DIMS = -10,  /* SpartanError::InvalidAssignmentDimensions (crates/spartan/src/piop.rs:80-82) */
+    BITZ_ERR_ASSIGNMENT_LEN = -11,   /* SpartanMatrixError::InvalidAssignmentLength (matrix.rs:23) */
+    BITZ_ERR_ASSIGNMENT_CONST = -12, /* SpartanMatrixError::InvalidAssignmentConstant (matrix.rs:24) */
     BITZ_ERR_DEVICE = -100,      /* HIP runtime error; see bitz_last_error() */
     BITZ_ERR_ALLOC = -101        /* device allocation failed */
 };
@@ -245,6 +248,39 @@ int32_t bitz_matrices_prepare(bitz_matrices** h, size_t rows, uint32_t row_vars,
 /* bind_and_batch (matrix.rs:429-468): out[2^col_vars] = D. */
 int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
                             bitz_f128 rho, bitz_f128* out, uint32_t flags);
+/* Az, Bz, Cz = A h, B h, C h over the prepared matrices, each padded with
+ * zeros to 2^row_vars: the F128 restatement of the reference test's
+ * multiply + padded_mle (crates/spartan/src/piop.rs:352-372), the reference's
+ * own producer build_product_mles (matrix.rs:336-370) being over a prime
+ * field.  assignment: 2^col_vars entries; outputs: 2^row_vars each. */
+int32_t bitz_r1cs_products(bitz_matrices* h, const bitz_f128* assignment, bitz_f128* az, bitz_f128* bz,
+                           bitz_f128* cz, uint32_t flags);
+/* build_assignment_mle (matrix.rs:392-420): the packed Boolean assignment
+ * (bit i = words[i / 64] >> (i % 64) & 1, as PackedWitness::bit) as {0, 1}
+ * elements padded with zeros to 2^ceil(log2 columns).  bit_len != columns ->
+ * BITZ_ERR_ASSIGNMENT_LEN; columns == 0 or bit 0 clear -> BITZ_ERR_ASSIGNMENT_CONST. */
+int32_t bitz_assignment_mle(const uint64_t* words, size_t bit_len, size_t columns, bitz_f128* out, uint32_t flags);
+/* constraint_matrix_digest (matrix.rs:562-627): SHA-256 over the domain
+ * "bitz/spartan/constraint-matrices/v1", M's positions (m_rows rows of
+ * m_columns columns, CSR), then A, B, C row by row (rows x columns, CSR) with
+ * every coefficient as its 16-byte encoding, lengths and indices as u64 LE. */
+int32_t bitz_constraint_matrix_digest(size_t m_rows, size_t m_columns, const uint64_t* m_row_starts,
+                                      const uint32_t* m_positions, size_t rows, size_t columns,
+                                      const uint64_t* const row_starts[3], const uint32_t* const cols[3],
+                                      const bitz_f128* const values[3], uint8_t digest[32]);
+/* prove_spartan_piop (crates/spartan/src/piop.rs:67-112): absorb the digest,
+ * squeeze tau (row_vars challenges), outer sumcheck from claim 0 over
+ * eq(tau) = make_equality_factors(tau), squeeze rho, bind_and_batch at the
+ * outer point, inner sumcheck from az + rho bz + rho^2 cz against the
+ * assignment.  Products (2^row_vars each) and assignment (2^col_vars) on the
+ * side flags names.  Outputs (host): outer_rounds (row_vars x [c0..c3]),
+ * outer_claims [Az, Bz, Cz](r_x), inner_rounds (col_vars x [c0, c1, c2]),
+ * inner_point r_y (col_vars), matrix_eval D(r_y), final_claim. */
+int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
+                                const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz, uint32_t row_vars,
+                                const bitz_f128* assignment, uint32_t col_vars, uint32_t flags,
+                                bitz_f128* outer_rounds, bitz_f128* outer_claims, bitz_f128* inner_rounds,
+                                bitz_f128* inner_point, bitz_f128* matrix_eval, bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
 
 /* ---- M^T (crates/circuit/src/matrix_transpose.rs) ------------------------- */
