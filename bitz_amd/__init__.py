@@ -716,15 +716,20 @@ class PreparedConstraintMatrices:
             pass
 
 
-def assignment_mle(bits, columns: int) -> np.ndarray:
+def assignment_mle(bits, columns: int, device: bool = False):
     """build_assignment_mle (crates/spartan/src/matrix.rs:392-420): a
     Boolean assignment (array of 0/1) as {0, 1} elements padded to a power of
-    two.  The length and constant-bit checks run before any device work."""
+    two, on the host or (device=True) as a DeviceTable.  The length and
+    constant-bit checks run before any device work."""
     bits = np.ascontiguousarray(bits, dtype=np.uint8)
     words = np.packbits(bits, bitorder="little")
     words = np.concatenate([words, np.zeros((-len(words)) % 8, np.uint8)]).view(np.uint64) if len(words) else \
         np.zeros(1, np.uint64)
     padded = 1 << max(int(columns - 1).bit_length(), 0) if columns > 1 else 1
+    if device and len(bits) == columns and columns and bits[0] & 1:
+        out = DeviceTable(length=padded)
+        _check(lib().bitz_assignment_mle(_p(words), len(bits), columns, C.c_void_p(out.ptr), DEVICE))
+        return out
     out = np.zeros((padded, 2), dtype=np.uint64)
     _check(lib().bitz_assignment_mle(_p(words), len(bits), columns, _p(out), 0))
     return out

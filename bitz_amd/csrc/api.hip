@@ -2174,7 +2174,7 @@ struct bitz_inner {
     size_t len = 0;
     int round = 0;
     bool started = false;
-    Dev in_d, in_w, P, Q;
+    Dev in_d, in_w, P, Q, rtab;  // rtab: the current challenge's 4-bit table
     const f128* cur[2] = {nullptr, nullptr};
     int where = 0;
     size_t capP = 0, capQ = 0;
@@ -2183,6 +2183,22 @@ struct bitz_inner {
     Timed tm;
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
+
+// Grid of the inner sumcheck's lane-table kernels (grid-stride loops).
+static unsigned inner_grid(size_t work) {
+    return (unsigned)std::max<size_t>(1, std::min<size_t>((work + bitz::kInnerBlock - 1) / bitz::kInnerBlock, 2048));
+}
+static void inner_set_attributes() {
+    static bool attr = false;
+    if (attr) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_r0_lane),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_fold_lane),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_tail),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerTailSmem);
+    attr = true;
+}
 
 extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
                                     const bitz_f128* witness, uint32_t witness_vars, uint32_t flags) {
@@ -2238,9 +2254,10 @@ extern "C" int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* co) {
         return BITZ_OK;
     }
     const size_t pairs = h->len / 2;
+    inner_set_attributes();
     h->tm.begin(h->so.s, "inner_round0", 32.0 * (double)h->len);
-    hipLaunchKernelGGL(bitz::k_inner_round0, dim3(grid_for(pairs)), dim3(kBlock), 0, h->so.s, h->cur[0],
-                       h->cur[1], pairs, h->acc.u32());
+    hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(inner_grid(pairs)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
+                       h->so.s, h->cur[0], h->cur[1], pairs, h->acc.u32());
     h->tm.end(h->so.s);
     TRY(check_launch());
     return inner_read(h, 0, co);
@@ -2260,10 +2277,13 @@ extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* 
         hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[0], out[0], (size_t)1, r);
         hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[1], out[1], (size_t)1, r);
     } else {
+        // r's 4-bit table, then the fold with lane-table products.
+        if (!h->rtab.p) TRY(h->rtab.alloc(kConstTable4Entries * sizeof(uint4)));
+        hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, static_cast<uint4*>(h->rtab.p));
         h->tm.begin(s, "inner_fold_next", 48.0 * (double)h->len);
-        hipLaunchKernelGGL(bitz::k_inner_fold_next, dim3(grid_for(next_len / 2)), dim3(kBlock), 0, s,
-                           h->cur[0], h->cur[1], out[0], out[1], next_len / 2, r,
-                           h->acc.u32() + 8 * (h->round + 1));
+        hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(inner_grid(next_len / 2)), dim3(bitz::kInnerBlock),
+                           bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], next_len / 2,
+                           static_cast<const uint4*>(h->rtab.p), h->acc.u32() + 8 * (h->round + 1));
         h->tm.end(s);
         has_next = true;
     }
@@ -2358,21 +2378,10 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     HIP_TRY(hipMemcpyAsync(chal.p, up.p, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)n), dim3(512), 0, s, chal.f(), static_cast<uint4*>(rt4.p));
     TRY(check_launch());
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_r0_lane),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_fold_lane),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_tail),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerTailSmem);
-        attr = true;
-    }
+    inner_set_attributes();
     const uint4* rt = static_cast<const uint4*>(rt4.p);
     const int k0 = std::max(0, n - bitz::kInnerTailMaxLog);
-    auto grid = [](size_t work) {
-        return (unsigned)std::max<size_t>(1, std::min<size_t>((work + bitz::kInnerBlock - 1) / bitz::kInnerBlock, 2048));
-    };
+    auto grid = [](size_t work) { return inner_grid(work); };
     std::vector<Timed> ts(k0 + 1);
     size_t len = h->len;
     for (int k = 0; k < k0; ++k) {
@@ -2438,6 +2447,7 @@ struct bitz_matrices {
     int row_vars = 0, col_vars = 0;
     Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
     Dev rs[3], cs[3], vs[3];  // row-major copies of A, B, C (products)
+    std::mutex mu;            // calls on one handle share its scratch and stream: one at a time
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
@@ -2521,6 +2531,7 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
     if (!h || !out || (point_len && !row_point)) return fail(BITZ_ERR_ARG, "null pointer");
     // matrix.rs:435-441: the row point must have num_row_vars coordinates.
     if (point_len != (uint32_t)h->row_vars) return fail(BITZ_ERR_ROW_POINT, "row point length");
+    std::lock_guard<std::mutex> lock(h->mu);
     hipStream_t s = h->so.s;
     if (point_len)
         HIP_TRY(hipMemcpyAsync(h->point.p, row_point, point_len * sizeof(f128),
@@ -2551,6 +2562,7 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
 extern "C" int32_t bitz_r1cs_products(bitz_matrices* h, const bitz_f128* assignment, bitz_f128* az, bitz_f128* bz,
                                       bitz_f128* cz, uint32_t flags) {
     if (!h || !assignment || !az || !bz || !cz) return fail(BITZ_ERR_ARG, "null pointer");
+    std::lock_guard<std::mutex> lock(h->mu);
     hipStream_t s = h->so.s;
     Dev din, dout;
     const f128* hv = reinterpret_cast<const f128*>(assignment);
