@@ -80,7 +80,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["outer", "batch", "gkr", "inner", "serve"], default="outer",
+    ap.add_argument("--workload", choices=["outer", "batch", "gkr", "inner", "serve", "bind"], default="outer",
                     help="outer: one 2^log_n outer sumcheck per step (default); batch: 256 outer sumchecks "
                          "of 2^12..2^22 elements per step, batched; gkr: product tree and layered sumcheck "
                          "over 2^22 leaves per step; inner: one inner sumcheck over 2 tables of 2^inner_log_n")
@@ -127,6 +127,8 @@ def main() -> int:
         return bench_inner(G, args)
     if args.workload == "serve":
         return bench_serve(G, args)
+    if args.workload == "bind":
+        return bench_bind(G, args)
 
     n = args.log_n
     G.init(0)
@@ -550,6 +552,54 @@ def bench_serve(G, args) -> int:
                       "peak_in_flight": sat_peak},
         "one_at_a_time": {"blocks_per_s": blocks / serial, "proofs_per_s": len(reqs) / serial},
         "capacity_vs_one_at_a_time": serial / saturated,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_bind(G, args) -> int:
+    """bind_and_batch over a 256-block SHA-256-chain-shaped circuit (178432
+    rows, 2^23 columns, 8 nonzeros per row in each of A, B, C; synthetic, see
+    bitz_amd/serving.py), the output D left on the device."""
+    from bitz_amd import serving as S
+    G.init(0)
+    c = S.make_circuit(256, args.seed)
+    nnz = c.nnz
+    rng = np.random.default_rng(args.seed)
+    rx = G.DeviceTable(rng.integers(0, 2**64, size=(c.row_vars, 2), dtype=np.uint64))
+    rho = 0x1234567890ABCDEF1234567890ABCDEF
+    out = G.DeviceTable(length=1 << c.col_vars)
+    for _ in range(args.warmup):
+        c.matrices.bind_and_batch(rx, rho, device=True, out=out)
+    G.synchronize()
+    times = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        c.matrices.bind_and_batch(rx, rho, device=True, out=out)
+        times.append(time.perf_counter() - t0)
+    G.synchronize()
+    wall = time.perf_counter() - t_start
+    algo = nnz * (4 + 1 + 16) + 16 * (1 << c.col_vars) + 16 * 4 * (1 << c.row_vars)
+    line = {
+        "metric": "bind_and_batch calls/sec, 1 MI355X",
+        "value": args.steps / wall,
+        "unit": "bind_and_batch/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (random 256-block chain-shaped R1CS)",
+        "config": {"workload": f"bind_and_batch: {c.rows} rows (2^{c.row_vars}), 2^{c.col_vars} columns, "
+                               f"{nnz} nonzeros", "nnz": nnz},
+        "algorithmic_gb_per_s": algo * args.steps / wall / 1e9,
+        "algorithmic_bytes": algo,
     }
     print(json.dumps(line), flush=True)
     return 0

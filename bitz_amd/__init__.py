@@ -685,8 +685,21 @@ class PreparedConstraintMatrices:
         _check(lib().bitz_matrices_prepare(C.byref(h), rows, num_row_vars, num_column_vars, arr_s, arr_c, arr_v))
         self.h = h
 
-    def bind_and_batch(self, row_point, rho) -> np.ndarray:
-        """matrix.rs:429-468: D(j) = sum_i eq(i, r_x)(A + rho B + rho^2 C)[i, j]."""
+    def bind_and_batch(self, row_point, rho, device: bool = False, out: "DeviceTable | None" = None):
+        """matrix.rs:429-468: D(j) = sum_i eq(i, r_x)(A + rho B + rho^2 C)[i, j];
+        a host array, or (device=True) a DeviceTable (out, when given, is
+        reused: 2^num_column_vars entries)."""
+        if device:
+            if not isinstance(row_point, DeviceTable):
+                row_point = DeviceTable(elems(row_point) if len(row_point) else np.zeros((1, 2), np.uint64))
+            if out is None:
+                out = DeviceTable(length=1 << self.num_column_vars)
+            elif out.len < 1 << self.num_column_vars:
+                raise BitzError(-7, "output table too short")
+            n = self.num_row_vars
+            _check(lib().bitz_bind_and_batch(self.h, C.c_void_p(row_point.ptr), n, f128(rho), C.c_void_p(out.ptr),
+                                             DEVICE))
+            return out
         row_point = elems(row_point) if len(row_point) else np.zeros((0, 2), np.uint64)
         out = np.zeros((1 << self.num_column_vars, 2), dtype=np.uint64)
         _check(lib().bitz_bind_and_batch(self.h, _p(row_point), len(row_point), f128(rho), _p(out), 0))

@@ -206,8 +206,7 @@ int32_t eq_build(const f128* r_dev, int k, f128* out, Dev& scratch, hipStream_t 
     if (scratch.bytes < (nl + nh) * sizeof(f128)) TRY(scratch.alloc((nl + nh) * sizeof(f128)));
     f128* low = scratch.f();
     f128* high = low + nl;
-    hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev, kl, low);
-    hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev + kl, kh, high);
+    hipLaunchKernelGGL(bitz::k_eq_small2, dim3(2), dim3(kBlock), 0, s, r_dev, kl, low, kh, high);
     const size_t n = (size_t)1 << k;
     hipLaunchKernelGGL(bitz::k_eq_combine, dim3(grid_for(n)), dim3(kBlock), 0, s, low, kl, high, out, n);
     return check_launch();
@@ -2447,6 +2446,8 @@ struct bitz_matrices {
     int row_vars = 0, col_vars = 0;
     Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
     Dev rs[3], cs[3], vs[3];  // row-major copies of A, B, C (products)
+    Dev prod;                 // bind_and_batch: one product per nonzero
+    size_t nnz = 0;
     std::mutex mu;            // calls on one handle share its scratch and stream: one at a time
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
@@ -2501,6 +2502,8 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
     if ((rc = h->row_idx.alloc(std::max<size_t>(nnz, 1) * 4))) return cleanup(rc);
     if ((rc = h->which.alloc(std::max<size_t>(nnz, 1)))) return cleanup(rc);
     if ((rc = h->values.alloc(vals.size() * sizeof(f128)))) return cleanup(rc);
+    h->nnz = nnz;
+    if ((rc = h->prod.alloc(std::max<size_t>(nnz, 1) * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->eq_rows.alloc(h->rows_padded * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->row_scale.alloc(3 * h->rows_padded * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->scales.alloc(3 * sizeof(f128)))) return cleanup(rc);
@@ -2538,20 +2541,32 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
                                (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     TRY(eq_build(h->point.f(), h->row_vars, h->eq_rows.f(), h->scratch, s));
     const f128 r = F(rho);
-    const f128 sc[3] = {hostf::one(), r, hostf::mul(r, r)};  // matrix.rs:533-545
-    HIP_TRY(hipMemcpyAsync(h->scales.p, sc, sizeof sc, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));  // sc lives on this stack frame
-    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s,
-                       h->eq_rows.f(), h->scales.f(), h->row_scale.f(), h->rows_padded);
+    // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value
+    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(), r,
+                       hostf::mul(r, r), h->row_scale.f(), h->rows_padded);
     Dev dout;
     f128* target = reinterpret_cast<f128*>(out);
     if (!(flags & BITZ_DEVICE)) {
         TRY(dout.alloc(h->cols * sizeof(f128)));
         target = dout.f();
     }
-    hipLaunchKernelGGL(bitz::k_bind_columns, dim3(grid_for(h->cols)), dim3(kBlock), 0, s, h->col_starts.u32(),
-                       h->row_idx.u32(), static_cast<const uint8_t*>(h->which.p), h->values.f(),
-                       h->row_scale.f(), h->rows_padded, target, h->cols);
+    // every nonzero's product (a converged pass), then each column's XOR over its run
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_bind_products),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kBindSmem);
+        attr = true;
+    }
+    if (h->nnz) {
+        const unsigned pgrid = (unsigned)std::max<size_t>(
+            1, std::min<size_t>((h->nnz + bitz::kStageBlock - 1) / bitz::kStageBlock, 4096));
+        hipLaunchKernelGGL(bitz::k_bind_products, dim3(pgrid), dim3(bitz::kStageBlock), bitz::kBindSmem, s,
+                           h->row_idx.u32(), static_cast<const uint8_t*>(h->which.p), h->values.f(), h->row_scale.f(),
+                           h->rows_padded, h->prod.f(), h->nnz);
+    }
+    const unsigned cgrid = (unsigned)std::max<size_t>(1, std::min<size_t>((h->cols + kBlock - 1) / kBlock, 1u << 20));
+    hipLaunchKernelGGL(bitz::k_bind_segments, dim3(cgrid), dim3(kBlock), 0, s, h->col_starts.u32(), h->prod.f(),
+                       target, h->cols);
     TRY(check_launch());
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));
@@ -3061,10 +3076,11 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
             g.eq = eqa.f() + eq_off[l];
             g.h = (size_t)1 << i;
             g.first = i == n - 1;
-            if (!g.first) g.r = pts[l + 1][i + 1];
+            // bind by r_{i+1}: the layer's challenge table of coordinate i+1
+            if (!g.first) g.rt4 = static_cast<const uint4*>(rt4.p) + (ch_off[l] + (size_t)(n - 1 - (i + 1))) * kConstTable4Entries;
             g.at_one = hostf::is_zero(pts[l][i]);
             g.acc = h->acc.u32() + 8 * (slot_off[l] + (size_t)(n - 1 - i));
-            g.blocks = grid_for(g.h);
+            g.blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((g.h + bitz::kStageBlock - 1) / bitz::kStageBlock, 2048));
             g.block_base = (uint32_t)map.size();
             map.insert(map.end(), g.blocks, (uint32_t)items.size());
             items.push_back(g);
@@ -3126,6 +3142,8 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrTailSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_round_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrRoundSmem);
         attr = true;
     }
     const char* dp = static_cast<const char*>(desc.p);
@@ -3146,7 +3164,8 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
         double bytes = 0;
         for (const auto& r : rounds[g]) bytes += (r.first ? 80.0 : 240.0) * (double)r.h;
         ts[g].begin(s, "gkr_round_batch", bytes);
-        hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3((unsigned)round_maps[g].size()), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3((unsigned)round_maps[g].size()), dim3(bitz::kStageBlock),
+                           bitz::kGkrRoundSmem, s,
                            reinterpret_cast<const bitz::GkrRound*>(dp + rd_o[g]),
                            reinterpret_cast<const uint32_t*>(dp + rm_o[g]));
         ts[g].end(s);

@@ -120,6 +120,29 @@ __global__ __launch_bounds__(kBlock) void k_mul(const f128* a, const f128* b, f1
 
 // In-place level doubling of a small eq table by one workgroup
 // (crates/poly/src/eq.rs:63-99).
+// Both halves of a split eq table in one launch: workgroup 0 builds eq(r_<kl)
+// into low, workgroup 1 eq(r_[kl, kl+kh)) into high.
+__global__ __launch_bounds__(kBlock) void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high) {
+    const bool hi = blockIdx.x == 1;
+    const f128* rr = hi ? r + kl : r;
+    const int k = hi ? kh : kl;
+    f128* table = hi ? high : low;
+    if (threadIdx.x == 0) g_store(table, g_one());
+    for (size_t j = 1 + threadIdx.x; j < ((size_t)1 << k); j += blockDim.x) g_store(table + j, g_zero());
+    __syncthreads();
+    for (int i = 0; i < k; ++i) {
+        const size_t half = (size_t)1 << i;
+        const g4 ri = g_load(rr + i);
+        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
+            const g4 parent = g_load(table + j);
+            const g4 one = g_mul(parent, ri);
+            g_store(table + j, g_add(parent, one));
+            g_store(table + j + half, one);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_eq_small(const f128* r, int k, f128* table) {
     if (threadIdx.x == 0) g_store(table, g_one());
     for (size_t j = 1 + threadIdx.x; j < ((size_t)1 << k); j += blockDim.x) g_store(table + j, g_zero());
@@ -369,15 +392,38 @@ __global__ __launch_bounds__(kBlock) void k_assignment_bits(const uint64_t* word
     }
 }
 
+__global__ __launch_bounds__(kStageBlock) void k_bind_products(const uint32_t* rows, const uint8_t* which,
+                                                               const f128* values, const f128* row_scale,
+                                                               size_t rows_padded, f128* prod, size_t nnz) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+#pragma unroll 1
+    for (size_t k = blockIdx.x * (size_t)kStageBlock + threadIdx.x; k < nnz; k += (size_t)gridDim.x * kStageBlock) {
+        const g4 w = g_load_hbm(row_scale + which[k] * rows_padded + rows[k]);
+        g_store_hbm(prod + k, g_mul_lane(w, g_load_hbm(values + k), col));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bind_segments(const uint32_t* col_starts, const f128* prod, f128* out,
+                                                         size_t cols) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < cols; j += (size_t)gridDim.x * blockDim.x) {
+        g4 s = g_zero();
+        for (uint32_t k = col_starts[j]; k < col_starts[j + 1]; ++k) s = g_add(s, g_load_hbm(prod + k));
+        g_store_hbm(out + j, s);
+    }
+}
+
 // row_scale[m][i] = eq(i, r_x) * scale_m.
-__global__ __launch_bounds__(kBlock) void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
+__global__ __launch_bounds__(kBlock) void k_row_scales(const f128* eq_rows, f128 rho, f128 rho2, f128* row_scale,
                              size_t rows_padded) {
+    const g4 s1 = g_from(rho), s2 = g_from(rho2);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < rows_padded;
          i += (size_t)gridDim.x * blockDim.x) {
         const g4 w = g_load(eq_rows + i);
         g_store(row_scale + i, w);
-        g_store(row_scale + rows_padded + i, g_mul(w, g_load(scales + 1)));
-        g_store(row_scale + 2 * rows_padded + i, g_mul(w, g_load(scales + 2)));
+        g_store(row_scale + rows_padded + i, g_mul(w, s1));
+        g_store(row_scale + 2 * rows_padded + i, g_mul(w, s2));
     }
 }
 
@@ -1293,44 +1339,52 @@ __global__ __launch_bounds__(kBlock) void k_gkr_eq_combine_batch(const GkrCombin
         g_store_hbm(A.out + j, g_mul(g_load_hbm(A.low + (j & mask)), g_load_hbm(A.high + (j >> A.low_bits))));
 }
 
-__global__ __launch_bounds__(kBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
-    const GkrRound& A = d[bmap[blockIdx.x]];
-    const size_t start = (blockIdx.x - A.block_base) * (size_t)blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)A.blocks * blockDim.x;
+__global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
+    const GkrRound A = d[bmap[blockIdx.x]];
+    extern __shared__ uint4 smem_raw[];
+    uint4* rt = smem_raw;
+    uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    const size_t start = (blockIdx.x - A.block_base) * (size_t)kStageBlock + threadIdx.x;
+    const size_t stride = (size_t)A.blocks * kStageBlock;
     const size_t h = A.h;
     const int at_one = A.at_one;
     f128* left = A.left;
     f128* right = A.right;
     f128* eq = A.eq;
-    g8 sb = w_zero(), si = w_zero();
+    g4 sb = g_zero(), si = g_zero();
+    auto sums = [&](const g4& l0, const g4& l1, const g4& r0, const g4& r1, const g4& e) {
+        sb = g_add(sb, g_mul_lane(g_mul_lane(e, at_one ? l1 : l0, col), at_one ? r1 : r0, col));
+        __builtin_amdgcn_sched_barrier(0);  // one lane product chain in flight at a time
+        si = g_add(si, g_mul_lane(g_mul_lane(e, g_add(l1, l0), col), g_add(r1, r0), col));
+    };
     if (A.first) {
-        for (size_t x = start; x < h; x += stride) {
-            const g4 l0 = g_load_hbm(left + x), l1 = g_load_hbm(left + x + h);
-            const g4 r0 = g_load_hbm(right + x), r1 = g_load_hbm(right + x + h);
-            const g4 e = g_load_hbm(eq + x);
-            clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
-            clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
-        }
+#pragma unroll 1
+        for (size_t x = start; x < h; x += stride)
+            sums(g_load_hbm(left + x), g_load_hbm(left + x + h), g_load_hbm(right + x), g_load_hbm(right + x + h),
+                 g_load_hbm(eq + x));
     } else {
-        const g4 rr = g_from(A.r);
+        for (int e = threadIdx.x; e < kConstTable4Entries; e += kStageBlock) rt[e] = u4_load_hbm(A.rt4 + e);
+        __syncthreads();
         const size_t hh = 2 * h;
+        auto ip = [&](const g4& u, const g4& v) { return g_add(u, g_mul_table4(rt, g_add(u, v))); };
+#pragma unroll 1
         for (size_t x = start; x < h; x += stride) {
-            const g4 l0 = interp(g_load_hbm(left + x), g_load_hbm(left + x + hh), rr);
-            const g4 l1 = interp(g_load_hbm(left + x + h), g_load_hbm(left + x + h + hh), rr);
-            const g4 r0 = interp(g_load_hbm(right + x), g_load_hbm(right + x + hh), rr);
-            const g4 r1 = interp(g_load_hbm(right + x + h), g_load_hbm(right + x + h + hh), rr);
+            const g4 l0 = ip(g_load_hbm(left + x), g_load_hbm(left + x + hh));
+            const g4 l1 = ip(g_load_hbm(left + x + h), g_load_hbm(left + x + h + hh));
+            const g4 r0 = ip(g_load_hbm(right + x), g_load_hbm(right + x + hh));
+            const g4 r1 = ip(g_load_hbm(right + x + h), g_load_hbm(right + x + h + hh));
             g_store_hbm(left + x, l0);
             g_store_hbm(left + x + h, l1);
             g_store_hbm(right + x, r0);
             g_store_hbm(right + x + h, r1);
             const g4 e = g_add(g_load_hbm(eq + x), g_load_hbm(eq + x + h));
             g_store_hbm(eq + x, e);
-            clmul_acc(sb, g_mul(e, at_one ? l1 : l0), at_one ? r1 : r0);
-            clmul_acc(si, g_mul(e, g_add(l1, l0)), g_add(r1, r0));
+            sums(l0, l1, r0, r1, e);
         }
     }
-    g4 v[2] = {g_reduce(sb), g_reduce(si)};
-    block_xor_to_global<2>(v, A.acc);
+    g4 v[2] = {sb, si};
+    block_xor_to_global<2>(v, A.acc, kStageBlock);
 }
 
 // The layer's arrays in LDS: L and R (2^(i0+2) entries at most), E, the

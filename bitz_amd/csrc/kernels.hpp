@@ -13,6 +13,7 @@ constexpr int kBlock = 256;
 
 __global__ void k_mul(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_eq_small(const f128* r, int k, f128* table);
+__global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
 __global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);
 __global__ void k_fold(const f128* in, f128* out, size_t out_len, f128 r);
 __global__ void k_fold3(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
@@ -36,8 +37,13 @@ __global__ void k_gkr_bind(f128* left, f128* right, size_t half, f128 r);
 __global__ void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
                                const uint8_t* which, const f128* values, const f128* row_scale,
                                size_t rows_padded, f128* out, size_t cols);
-__global__ void k_row_scales(const f128* eq_rows, const f128* scales, f128* row_scale,
-                             size_t rows_padded);
+// bind_and_batch in two passes: every nonzero's product W[row] * value
+// (lane tables, 256 threads), then each column's XOR over its contiguous run.
+__global__ void k_bind_products(const uint32_t* rows, const uint8_t* which, const f128* values, const f128* row_scale,
+                                size_t rows_padded, f128* prod, size_t nnz);
+__global__ void k_bind_segments(const uint32_t* col_starts, const f128* prod, f128* out, size_t cols);
+constexpr size_t kBindSmem = sizeof(uint4) * (4 * 64 * kLaneTableEntries);
+__global__ void k_row_scales(const f128* eq_rows, f128 rho, f128 rho2, f128* row_scale, size_t rows_padded);
 __global__ void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols, const f128* vals, const f128* h,
                             size_t rows, f128* out, size_t padded);
 __global__ void k_assignment_bits(const uint64_t* words, size_t bit_len, f128* out, size_t padded);
@@ -233,11 +239,13 @@ struct GkrRound {
     f128* right;
     f128* eq;
     size_t h;
-    f128 r;
+    const uint4* rt4;  // bind rounds: r's 4-bit table
     uint32_t* acc;
     int first, at_one;
     uint32_t block_base, blocks;
 };
+// 256 threads: r's table (bind rounds) and the lane tables in dynamic LDS.
+constexpr size_t kGkrRoundSmem = sizeof(uint4) * (kConstTable4Entries + 4 * 64 * kLaneTableEntries);
 __global__ void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap);
 // A layer's rounds i0 .. 0 and the final bind in one workgroup.  Before
 // round i0 the arrays hold 2^(i0+2) entries each and eq 2^(i0+1) (i0 < n-1),

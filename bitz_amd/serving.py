@@ -65,6 +65,7 @@ class Circuit:
     col_vars: int
     matrices: PreparedConstraintMatrices
     digest: bytes
+    nnz: int = 0
 
     def footprint(self) -> int:
         """Device bytes a request holds at its peak: the assignment and D
@@ -79,7 +80,8 @@ def make_circuit(blocks: int, seed: int = 0) -> Circuit:
     csr = [_random_csr(rng, rows, cols, NNZ_PER_ROW) for _ in range(3)]
     m = csr[0][:2]  # the positions of M (a stand-in with A's pattern)
     digest = constraint_matrix_digest(m, cols, rows, cols, csr)
-    return Circuit(blocks, rows, cols, rv, cv, PreparedConstraintMatrices(rows, rv, cv, csr), digest)
+    return Circuit(blocks, rows, cols, rv, cv, PreparedConstraintMatrices(rows, rv, cv, csr), digest,
+                   sum(len(m[1]) for m in csr))
 
 
 @dataclass
