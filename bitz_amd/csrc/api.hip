@@ -346,12 +346,41 @@ extern "C" int32_t bitz_init(int32_t device) {
 
 extern "C" const char* bitz_last_error(void) { return g_err.c_str(); }
 
+// Caller allocations come from the same size-keyed cache as the engine's
+// scratch: hipFree synchronizes the whole device, which would stall every
+// other stream (concurrent requests) at each release.
+struct Outstanding {
+    std::mutex mu;
+    std::map<void*, size_t> sizes;
+} g_outstanding;
+
 extern "C" int32_t bitz_device_alloc(void** ptr, size_t bytes) {
     if (!ptr) return fail(BITZ_ERR_ARG, "null pointer");
-    if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) return fail(BITZ_ERR_ALLOC, "hipMalloc failed");
+    const size_t n = bytes ? bytes : 16;
+    void* p = g_cache.get(n);
+    if (!p) return fail(BITZ_ERR_ALLOC, "hipMalloc failed");
+    {
+        std::lock_guard<std::mutex> lock(g_outstanding.mu);
+        g_outstanding.sizes[p] = n;
+    }
+    *ptr = p;
     return BITZ_OK;
 }
 extern "C" int32_t bitz_device_free(void* ptr) {
+    if (!ptr) return BITZ_OK;
+    size_t n = 0;
+    {
+        std::lock_guard<std::mutex> lock(g_outstanding.mu);
+        auto it = g_outstanding.sizes.find(ptr);
+        if (it != g_outstanding.sizes.end()) {
+            n = it->second;
+            g_outstanding.sizes.erase(it);
+        }
+    }
+    if (n) {
+        g_cache.put(ptr, n);
+        return BITZ_OK;
+    }
     HIP_TRY(hipFree(ptr));
     return BITZ_OK;
 }
