@@ -2862,6 +2862,8 @@ struct bitz_gkr {
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
+constexpr size_t kSmallTreeSmem = sizeof(uint4) * ((size_t)1 << bitz::kGkrSmallTreeLog) + bitz::kLaneSmem256;
+
 extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint32_t log_leaves,
                                   uint32_t log_groups, bitz_f128* output, uint32_t flags) {
     if (!out || !leaves || !output) return fail(BITZ_ERR_ARG, "null pointer");
@@ -2882,14 +2884,35 @@ extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint3
         total += (size_t)1 << (log_leaves - j);
     }
     if ((rc = h->store.alloc(total * sizeof(f128)))) return cleanup(rc);
-    if (hipMemcpyAsync(h->store.p, leaves, ((size_t)1 << log_leaves) * sizeof(f128),
-                       (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s) != hipSuccess)
-        return cleanup(fail(BITZ_ERR_DEVICE, "leaf upload failed"));
-    for (int j = 0; j < (int)(log_leaves - log_groups); ++j) {
-        const size_t half = (size_t)1 << (log_leaves - j - 1);
-        hipLaunchKernelGGL(bitz::k_gkr_layer, dim3(grid_for(half)), dim3(kBlock), 0, s,
-                           h->store.f() + h->off[j], h->store.f() + h->off[j + 1], half);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_layer_lane),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_layers_small),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallTreeSmem);
+        attr = true;
     }
+    // Layers are contiguous: layer j+1 starts where layer j ends. Large layers
+    // take one grid-wide launch each (the first also copies device leaves into
+    // the store); from 2^kGkrSmallTreeLog elements down one workgroup builds the rest.
+    const int depth = (int)(log_leaves - log_groups);
+    const int j_small = std::max(0, (int)log_leaves - bitz::kGkrSmallTreeLog);
+    const bool fused_copy = (flags & BITZ_DEVICE) && depth > 0 && j_small > 0;
+    if (!fused_copy && hipMemcpyAsync(h->store.p, leaves, ((size_t)1 << log_leaves) * sizeof(f128),
+                                      (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                      s) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "leaf upload failed"));
+    for (int j = 0; j < std::min(depth, j_small); ++j) {
+        const size_t half = (size_t)1 << (log_leaves - j - 1);
+        const bool copy = fused_copy && j == 0;
+        hipLaunchKernelGGL(bitz::k_gkr_layer_lane, dim3(grid_for(half)), dim3(bitz::kStageBlock),
+                           bitz::kLaneSmem256, s,
+                           copy ? reinterpret_cast<const f128*>(leaves) : h->store.f() + h->off[j],
+                           copy ? h->store.f() : nullptr, h->store.f() + h->off[j + 1], half);
+    }
+    if (depth > j_small)
+        hipLaunchKernelGGL(bitz::k_gkr_layers_small, dim3(1), dim3(bitz::kStageBlock), kSmallTreeSmem, s,
+                           h->store.f() + h->off[j_small], (int)log_leaves - j_small, depth - j_small);
     if ((rc = check_launch())) return cleanup(rc);
     if ((rc = download(output, h->store.f() + h->off.back(), (size_t)1 << log_groups, flags, s)))
         return cleanup(rc);
@@ -3173,17 +3196,23 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrTailSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_round_batch),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrRoundSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_eq_items),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_eq_combine_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
         attr = true;
     }
     const char* dp = static_cast<const char*>(desc.p);
     hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)need), dim3(512), 0, s, chdev.f(),
                        static_cast<uint4*>(rt4.p));
     TRY(check_launch());
-    hipLaunchKernelGGL(bitz::k_gkr_eq_items, dim3((unsigned)eq_items.size()), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(bitz::k_gkr_eq_items, dim3((unsigned)eq_items.size()), dim3(bitz::kStageBlock),
+                       bitz::kLaneSmem256, s,
                        reinterpret_cast<const bitz::GkrEqItem*>(dp + eqi_o));
     TRY(check_launch());
     if (!combines.empty()) {
-        hipLaunchKernelGGL(bitz::k_gkr_eq_combine_batch, dim3((unsigned)comb_map.size()), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(bitz::k_gkr_eq_combine_batch, dim3((unsigned)comb_map.size()), dim3(bitz::kStageBlock),
+                           bitz::kLaneSmem256, s,
                            reinterpret_cast<const bitz::GkrCombine*>(dp + cmb_o),
                            reinterpret_cast<const uint32_t*>(dp + cmap_o));
         TRY(check_launch());

@@ -286,13 +286,6 @@ __global__ __launch_bounds__(kBlock) void k_inner_fold_next(const f128* d, const
 }
 
 // ------------------------------------------------------------------ gkr --
-// Product tree layer: next[k] = cur[k] * cur[k + half] (crates/gkr/src/lib.rs:874-894).
-__global__ __launch_bounds__(kBlock) void k_gkr_layer(const f128* cur, f128* next, size_t half) {
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < half;
-         k += (size_t)gridDim.x * blockDim.x)
-        g_store(next + k, g_mul(g_load(cur + k), g_load(cur + k + half)));
-}
-
 // Round sums of a dense layer (round_sums, lib.rs:164-219) for the round
 // whose halves are [0, half) and [half, 2 half):
 //   sum_b   = sum_x eq[x] L_b(x) R_b(x)     (b = 1 when at_one)
@@ -1312,8 +1305,11 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
 }
 
 // ================================ GKR layers in parallel (known challenges) ===
-__global__ __launch_bounds__(kBlock) void k_gkr_eq_items(const GkrEqItem* items) {
+__global__ __launch_bounds__(kStageBlock) void k_gkr_eq_items(const GkrEqItem* items) {
     const GkrEqItem& A = items[blockIdx.x];
+    extern __shared__ uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
     f128* table = A.out;
     const int k = A.k;
     if (threadIdx.x == 0) g_store_hbm(table, g_one());
@@ -1321,9 +1317,9 @@ __global__ __launch_bounds__(kBlock) void k_gkr_eq_items(const GkrEqItem* items)
     for (int i = 0; i < k; ++i) {
         const size_t half = (size_t)1 << i;
         const g4 ri = g_load_hbm(A.r + i);
-        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
+        for (size_t j = threadIdx.x; j < half; j += kStageBlock) {
             const g4 parent = g_load_hbm(table + j);
-            const g4 one = g_mul(parent, ri);
+            const g4 one = g_mul_lane(parent, ri, col);
             g_store_hbm(table + j, g_add(parent, one));
             g_store_hbm(table + j + half, one);
         }
@@ -1331,12 +1327,71 @@ __global__ __launch_bounds__(kBlock) void k_gkr_eq_items(const GkrEqItem* items)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_gkr_eq_combine_batch(const GkrCombine* d, const uint32_t* bmap) {
-    const GkrCombine& A = d[bmap[blockIdx.x]];
+__global__ __launch_bounds__(kStageBlock) void k_gkr_eq_combine_batch(const GkrCombine* d, const uint32_t* bmap) {
+    const GkrCombine A = d[bmap[blockIdx.x]];
+    extern __shared__ uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
     const size_t mask = ((size_t)1 << A.low_bits) - 1;
-    for (size_t j = (blockIdx.x - A.block_base) * (size_t)blockDim.x + threadIdx.x; j < A.n;
-         j += (size_t)A.blocks * blockDim.x)
-        g_store_hbm(A.out + j, g_mul(g_load_hbm(A.low + (j & mask)), g_load_hbm(A.high + (j >> A.low_bits))));
+#pragma unroll 1
+    for (size_t j = (blockIdx.x - A.block_base) * (size_t)kStageBlock + threadIdx.x; j < A.n;
+         j += (size_t)A.blocks * kStageBlock)
+        g_store_hbm(A.out + j,
+                    g_mul_lane(g_load_hbm(A.low + (j & mask)), g_load_hbm(A.high + (j >> A.low_bits)), col));
+}
+
+// next[k] = cur[k] * cur[k + half] (fold_layers, crates/gkr/src/lib.rs:874-894).
+__global__ __launch_bounds__(kStageBlock) void k_gkr_layer_lane(const f128* cur, f128* cur_copy, f128* next,
+                                                                size_t half) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+#pragma unroll 1
+    for (size_t k = blockIdx.x * (size_t)kStageBlock + threadIdx.x; k < half; k += (size_t)gridDim.x * kStageBlock) {
+        const g4 x = g_load_hbm(cur + k), y = g_load_hbm(cur + k + half);
+        if (cur_copy) {
+            g_store_hbm(cur_copy + k, x);
+            g_store_hbm(cur_copy + k + half, y);
+        }
+        g_store_hbm(next + k, g_mul_lane(x, y, col));
+    }
+}
+
+// The `count` layers after `layer` (2^log_len elements, at most
+// 2^kGkrSmallTreeLog) in one workgroup: the current layer sits in LDS and is
+// halved in place (thread k reads k and k + half, writes k); each new layer is
+// also written right after the previous one in the store.
+__global__ __launch_bounds__(kStageBlock) void k_gkr_layers_small(f128* layer, int log_len, int count) {
+    extern __shared__ uint4 smem_raw[];
+    uint4* T = smem_raw;
+    uint4* col = smem_raw + ((size_t)1 << kGkrSmallTreeLog) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
+                 (threadIdx.x & 63);
+    lane_table_clear(col);
+    auto ld = [](const uint4* p) {
+        const uint4 u = *p;
+        g4 r;
+        r.w[0] = u.x;
+        r.w[1] = u.y;
+        r.w[2] = u.z;
+        r.w[3] = u.w;
+        return r;
+    };
+    size_t cur = (size_t)1 << log_len;
+    for (size_t e = threadIdx.x; e < cur; e += kStageBlock)
+        T[e] = u4_load_hbm(reinterpret_cast<const uint4*>(layer + e));
+    __syncthreads();
+    f128* dst = layer + cur;
+    for (int j = 0; j < count; ++j) {
+        const size_t half = cur / 2;
+        for (size_t k = threadIdx.x; k < half; k += kStageBlock) {
+            const g4 v = g_mul_lane(ld(T + k), ld(T + k + half), col);
+            T[k] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+            g_store_hbm(dst + k, v);
+        }
+        __syncthreads();
+        dst += half;
+        cur = half;
+    }
 }
 
 __global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {

@@ -28,7 +28,6 @@ __global__ void k_outer_claim(const f128* a, const f128* b, const f128* c, size_
 __global__ void k_inner_round0(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
 __global__ void k_inner_fold_next(const f128* d, const f128* w, f128* dout, f128* wout,
                                   size_t out_pairs, f128 r, uint32_t* acc);
-__global__ void k_gkr_layer(const f128* cur, f128* next, size_t half);
 __global__ void k_gkr_sums(const f128* left, const f128* right, const f128* eq, size_t half,
                            int at_one, uint32_t* acc);
 __global__ void k_gkr_bind_next(f128* left, f128* right, f128* eq, size_t h, f128 r,
@@ -222,7 +221,16 @@ struct GkrEqItem {
     f128* out;      // 2^k entries
 };
 constexpr int kGkrEqSmallMax = 12;
-__global__ void k_gkr_eq_items(const GkrEqItem* items);
+__global__ void k_gkr_eq_items(const GkrEqItem* items);  // 256 threads, dynamic LDS: kLaneSmem256
+constexpr size_t kLaneSmem256 = sizeof(uint4) * 4 * 64 * kLaneTableEntries;
+// Product tree with lane-table products: one grid-wide launch per large
+// layer, then every layer from 2^kGkrSmallTreeLog elements down in one
+// workgroup (LDS), each layer written to its place in the store.
+constexpr int kGkrSmallTreeLog = 12;
+// cur_copy, when not null, receives a copy of cur (the leaf layer read once).
+__global__ void k_gkr_layer_lane(const f128* cur, f128* cur_copy, f128* next, size_t half);
+// layer holds 2^log_len elements; the next `count` layers follow it in memory.
+__global__ void k_gkr_layers_small(f128* layer, int log_len, int count);
 struct GkrCombine {
     const f128* low;
     const f128* high;

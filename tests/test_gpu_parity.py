@@ -233,6 +233,19 @@ def test_gkr_matches_oracle(log_leaves, log_groups, zero):
     assert gc == O.mle_evaluate(leaves, gp)
 
 
+@pytest.mark.parametrize("log_leaves,log_groups", [(12, 0), (13, 0), (16, 1), (20, 3), (9, 9)])
+def test_gkr_layers_from_device_leaves(log_leaves, log_groups):
+    """Every layer of the product tree, with the leaves already in device
+    memory (the first layer also copies them into the store) and across the
+    switch from grid-wide layers to the one-workgroup small tree."""
+    leaves = elements_fast(60 + log_leaves, 1 << log_leaves)
+    _, layers = O.gkr_fold_layers(leaves, log_groups)
+    out, wit = G.grand_product(G.DeviceTable(leaves), log_groups)
+    assert np.array_equal(out, layers[-1])
+    for j in range(len(layers)):
+        assert np.array_equal(wit.layer(j), layers[j]), j
+
+
 @pytest.mark.parametrize("log_leaves,log_groups,zero", [(6, 2, None), (12, 1, 4), (15, 7, None), (18, 1, None)])
 def test_gkr_with_challenges_matches_oracle(log_leaves, log_groups, zero):
     """The queued GKR prover against the oracle driven by the same challenge
