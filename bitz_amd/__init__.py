@@ -142,6 +142,12 @@ def lib():
             "bitz_gkr_prove": (I32, [P, P, P, U32, P, P]),
             "bitz_gkr_layer": (I32, [P, U32, P, SZ, C.POINTER(SZ)]),
             "bitz_gkr_free": (None, [P]),
+            "bitz_sha256_circuit": (I32, [C.POINTER(C.c_void_p), U32, SZ, P]),
+            "bitz_sha256_shape": (I32, [P, C.POINTER(SZ), C.POINTER(SZ), C.POINTER(SZ), P, C.POINTER(SZ)]),
+            "bitz_sha256_export": (I32, [P, P, P, P, P, P, P]),
+            "bitz_sha256_witness": (I32, [P, P, P, P, P, P, C.POINTER(I32)]),
+            "bitz_sha256_prepare": (I32, [P, C.POINTER(C.c_void_p), P]),
+            "bitz_sha256_free": (None, [P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -685,6 +691,15 @@ class PreparedConstraintMatrices:
         _check(lib().bitz_matrices_prepare(C.byref(h), rows, num_row_vars, num_column_vars, arr_s, arr_c, arr_v))
         self.h = h
 
+    @classmethod
+    def _adopt(cls, h, rows: int, num_row_vars: int, num_column_vars: int) -> "PreparedConstraintMatrices":
+        """Wraps matrices the library prepared itself (bitz_sha256_prepare)."""
+        self = cls.__new__(cls)
+        self.rows, self.num_row_vars, self.num_column_vars = rows, num_row_vars, num_column_vars
+        self._keep = ()
+        self.h = h
+        return self
+
     def bind_and_batch(self, row_point, rho, device: bool = False, out: "DeviceTable | None" = None):
         """matrix.rs:429-468: D(j) = sum_i eq(i, r_x)(A + rho B + rho^2 C)[i, j];
         a host array, or (device=True) a DeviceTable (out, when given, is
@@ -745,6 +760,22 @@ def assignment_mle(bits, columns: int, device: bool = False):
         return out
     out = np.zeros((padded, 2), dtype=np.uint64)
     _check(lib().bitz_assignment_mle(_p(words), len(bits), columns, _p(out), 0))
+    return out
+
+
+def assignment_mle_words(words, bit_len: int, columns: int, device: bool = False):
+    """assignment_mle over an already packed assignment (bit i = words[i // 64]
+    >> (i % 64) & 1, as PackedWitness::bit), e.g. a SHA-256 witness's h."""
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    if not len(words):
+        words = np.zeros(1, np.uint64)
+    padded = 1 << max(int(columns - 1).bit_length(), 0) if columns > 1 else 1
+    if device:
+        out = DeviceTable(length=padded)
+        _check(lib().bitz_assignment_mle(_p(words), bit_len, columns, C.c_void_p(out.ptr), DEVICE))
+        return out
+    out = np.zeros((padded, 2), dtype=np.uint64)
+    _check(lib().bitz_assignment_mle(_p(words), bit_len, columns, _p(out), 0))
     return out
 
 

@@ -346,6 +346,11 @@ extern "C" int32_t bitz_init(int32_t device) {
 
 extern "C" const char* bitz_last_error(void) { return g_err.c_str(); }
 
+namespace bitz {
+// Error reporting for the host-only translation units (sha256_r1cs.cpp).
+int32_t report_error(int32_t code, const std::string& msg) { return fail(code, msg); }
+}  // namespace bitz
+
 // Caller allocations come from the same size-keyed cache as the engine's
 // scratch: hipFree synchronizes the whole device, which would stall every
 // other stream (concurrent requests) at each release.
@@ -2662,31 +2667,48 @@ extern "C" int32_t bitz_constraint_matrix_digest(size_t m_rows, size_t m_columns
                                                  const uint64_t* const row_starts[3], const uint32_t* const cols[3],
                                                  const bitz_f128* const values[3], uint8_t digest[32]) {
     if (!digest || !m_row_starts || !row_starts || !cols || !values) return fail(BITZ_ERR_ARG, "null pointer");
+    for (int m = 0; m < 3; ++m)
+        if (!row_starts[m]) return fail(BITZ_ERR_ARG, "null row_starts");
+    // The byte stream is staged through a local buffer and hashed in large
+    // pieces; u64 values are little-endian (x86 stores them so).
     bitz::Sha256 hs;
+    std::vector<uint8_t> buf(1 << 16);
+    size_t fill = 0;
+    auto flush = [&] {
+        hs.update(buf.data(), fill);
+        fill = 0;
+    };
+    auto put = [&](const void* p, size_t n) {
+        if (fill + n > buf.size()) flush();
+        memcpy(buf.data() + fill, p, n);
+        fill += n;
+    };
+    auto put_u64 = [&](uint64_t v) { put(&v, 8); };
     static const char kDomain[] = "bitz/spartan/constraint-matrices/v1";
-    hs.update(kDomain, sizeof kDomain - 1);
-    hs.update("M", 1);
-    hs.update_u64(m_rows);
-    hs.update_u64(m_columns);
+    put(kDomain, sizeof kDomain - 1);
+    put("M", 1);
+    put_u64(m_rows);
+    put_u64(m_columns);
     for (size_t i = 0; i < m_rows; ++i) {
-        hs.update_u64(m_row_starts[i + 1] - m_row_starts[i]);
-        for (uint64_t k = m_row_starts[i]; k < m_row_starts[i + 1]; ++k) hs.update_u64(m_positions[k]);
+        put_u64(m_row_starts[i + 1] - m_row_starts[i]);
+        for (uint64_t k = m_row_starts[i]; k < m_row_starts[i + 1]; ++k) put_u64(m_positions[k]);
     }
     static const char* kLabel[3] = {"A", "B", "C"};
     for (int m = 0; m < 3; ++m) {
-        if (!row_starts[m]) return fail(BITZ_ERR_ARG, "null row_starts");
-        hs.update(kLabel[m], 1);
-        hs.update_u64(rows);
-        hs.update_u64(columns);
+        put(kLabel[m], 1);
+        put_u64(rows);
+        put_u64(columns);
         for (size_t i = 0; i < rows; ++i) {
-            hs.update_u64(row_starts[m][i + 1] - row_starts[m][i]);
+            put_u64(row_starts[m][i + 1] - row_starts[m][i]);
             for (uint64_t k = row_starts[m][i]; k < row_starts[m][i + 1]; ++k) {
-                hs.update_u64(cols[m][k]);
-                hs.update_u64(16);  // the coefficient's encoding: F128::to_bytes (crates/field/src/codec.rs:24-28)
-                hs.update(&values[m][k], 16);
+                uint64_t entry[4] = {cols[m][k], 16, values[m][k].lo, values[m][k].hi};
+                // column, then the coefficient's encoding: length 16 and
+                // F128::to_bytes (crates/field/src/codec.rs:24-28)
+                put(entry, sizeof entry);
             }
         }
     }
+    flush();
     hs.finish(digest);
     return BITZ_OK;
 }

@@ -6,6 +6,8 @@
 //  - seeded: splitmix64(seed) challenges, F128::from([w0, w1]), the stream
 //    the reference's GKR tests draw test data from (crates/gkr/src/lib.rs:986-999);
 //  - duplex: a SHA-256 ratchet binding every challenge to every earlier message.
+#include <cpuid.h>
+#include <immintrin.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -51,6 +53,55 @@ void compress(uint32_t h[8], const uint8_t* p) {
     }
     for (int i = 0; i < 8; ++i) h[i] += v[i];
 }
+
+// The same compression with the x86 SHA extensions (SHA256RNDS2 runs two
+// rounds on the state split as ABEF / CDGH; SHA256MSG1/MSG2 extend the
+// message schedule four words at a time), as Intel documents them.
+__attribute__((target("sha,sse4.1,ssse3"))) void compress_shani(uint32_t h[8], const uint8_t* p, size_t blocks) {
+    const __m128i bswap = _mm_set_epi64x(0x0c0d0e0f08090a0bLL, 0x0405060700010203LL);
+    __m128i tmp = _mm_shuffle_epi32(_mm_loadu_si128(reinterpret_cast<const __m128i*>(h)), 0xB1);
+    __m128i cdgh = _mm_shuffle_epi32(_mm_loadu_si128(reinterpret_cast<const __m128i*>(h + 4)), 0x1B);
+    __m128i abef = _mm_alignr_epi8(tmp, cdgh, 8);
+    cdgh = _mm_blend_epi16(cdgh, tmp, 0xF0);
+    for (; blocks; --blocks, p += 64) {
+        const __m128i abef0 = abef, cdgh0 = cdgh;
+        __m128i w[4];
+        for (int q = 0; q < 16; ++q) {
+            if (q < 4) {
+                w[q] = _mm_shuffle_epi8(_mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 16 * q)), bswap);
+            } else {
+                __m128i x = _mm_sha256msg1_epu32(w[q & 3], w[(q + 1) & 3]);
+                x = _mm_add_epi32(x, _mm_alignr_epi8(w[(q + 3) & 3], w[(q + 2) & 3], 4));
+                w[q & 3] = _mm_sha256msg2_epu32(x, w[(q + 3) & 3]);
+            }
+            __m128i m = _mm_add_epi32(w[q & 3], _mm_loadu_si128(reinterpret_cast<const __m128i*>(K256 + 4 * q)));
+            cdgh = _mm_sha256rnds2_epu32(cdgh, abef, m);
+            abef = _mm_sha256rnds2_epu32(abef, cdgh, _mm_shuffle_epi32(m, 0x0E));
+        }
+        abef = _mm_add_epi32(abef, abef0);
+        cdgh = _mm_add_epi32(cdgh, cdgh0);
+    }
+    tmp = _mm_shuffle_epi32(abef, 0x1B);
+    cdgh = _mm_shuffle_epi32(cdgh, 0xB1);
+    _mm_storeu_si128(reinterpret_cast<__m128i*>(h), _mm_blend_epi16(tmp, cdgh, 0xF0));
+    _mm_storeu_si128(reinterpret_cast<__m128i*>(h + 4), _mm_alignr_epi8(cdgh, tmp, 8));
+}
+
+bool cpu_has_sha() {
+    unsigned a, b, c, d;
+    if (!__get_cpuid(1, &a, &b, &c, &d) || !(c & (1u << 19)) || !(c & (1u << 9))) return false;  // SSE4.1, SSSE3
+    return __get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & (1u << 29));                          // SHA
+}
+
+const bool kShaNi = cpu_has_sha() && !getenv("BITZ_NO_SHANI");
+
+void compress_blocks(uint32_t h[8], const uint8_t* p, size_t blocks) {
+    if (kShaNi) {
+        compress_shani(h, p, blocks);
+        return;
+    }
+    for (; blocks; --blocks, p += 64) compress(h, p);
+}
 }  // namespace
 
 Sha256::Sha256() {
@@ -69,10 +120,14 @@ void Sha256::update(const void* data, size_t len) {
         p += take;
         len -= take;
         if (fill < 64) return;
-        compress(h, buf);
+        compress_blocks(h, buf, 1);
         fill = 0;
     }
-    for (; len >= 64; p += 64, len -= 64) compress(h, p);
+    if (len >= 64) {
+        compress_blocks(h, p, len / 64);
+        p += len & ~(size_t)63;
+        len &= 63;
+    }
     memcpy(buf, p, len);
     fill = len;
 }
@@ -90,8 +145,7 @@ void Sha256::finish(uint8_t out[32]) {
     tail[fill] = 0x80;
     const size_t tl = fill + 9 <= 64 ? 64 : 128;
     for (int i = 0; i < 8; ++i) tail[tl - 1 - i] = (uint8_t)(bits >> (8 * i));
-    compress(h, tail);
-    if (tl == 128) compress(h, tail + 64);
+    compress_blocks(h, tail, tl / 64);
     for (int i = 0; i < 8; ++i) {
         out[4 * i] = (uint8_t)(h[i] >> 24);
         out[4 * i + 1] = (uint8_t)(h[i] >> 16);

@@ -283,6 +283,55 @@ int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint
                                 bitz_f128* inner_point, bitz_f128* matrix_eval, bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
 
+/* ---- SHA-256 R1CS instances (crates/circuit/src/sha256.rs) ---------------- */
+/* The constraint system (M, A, B, C) and witness (f, h) of BitZ's SHA-256
+ * circuits, produced on the host by one gadget description run symbolically
+ * (ConstraintGenerator, crates/circuit/src/constraints.rs:578-790) and
+ * concretely (Witgen, crates/circuit/src/witgen.rs:425-500).
+ *   COMPRESSION_CHAIN: inputs are the message words (bit j of a word is bit
+ *     j of the big-endian u32) then the 256 chaining-value bits; block b
+ *     starts from block b-1's output.  One block is compression_circuit
+ *     (sha256.rs:476-490; 20457 x 7145 M, 184 rows, tooling/cli/tests/
+ *     sha256_spartan.rs:44-49).  Matrices depend on the block count only.
+ *   CHAIN: the CLI's sha256-chain statement (tooling/cli/src/circuits.rs:
+ *     167-217): every message bit and every output bit is constrained to the
+ *     statement (1 * bit = expected), compression from the FIPS IV.  The rows
+ *     depend on the statement's message.
+ *   BLOCK_ALIGNED: sha256_block_aligned_circuit (sha256.rs:503-550), the
+ *     reference benchmark's circuit (tooling/cli/benches/sha256_spartan.rs:
+ *     60-90): the message as a bit stream, then the padding block of
+ *     constants; no statement rows. */
+#define BITZ_SHA256_COMPRESSION_CHAIN 0u
+#define BITZ_SHA256_CHAIN 1u
+#define BITZ_SHA256_BLOCK_ALIGNED 2u
+typedef struct bitz_sha256 bitz_sha256;
+/* statement: 64 * blocks message bytes, for CHAIN only (NULL otherwise). */
+int32_t bitz_sha256_circuit(bitz_sha256** out, uint32_t kind, size_t blocks, const uint8_t* statement);
+/* rows of A/B/C; columns = |h| with the constant (= rows of M); witness_bits
+ * = |f| (M has witness_bits + 1 columns); nonzeros of A, B, C; positions of M. */
+int32_t bitz_sha256_shape(const bitz_sha256* c, size_t* rows, size_t* columns, size_t* witness_bits,
+                          size_t nnz[3], size_t* m_positions);
+/* CSR of A, B, C (rows + 1 starts, columns, integer coefficients, and their
+ * images under the parity projection Z -> F2 in GF(2^128)), CSR of M
+ * (columns + 1 starts, positions).  Any pointer may be NULL. */
+int32_t bitz_sha256_export(const bitz_sha256* c, uint64_t* const row_starts[3], uint32_t* const cols[3],
+                           int64_t* const coefficients[3], bitz_f128* const parity[3],
+                           uint64_t* m_row_starts, uint32_t* m_positions);
+/* The witness for `message` (64 * blocks bytes; block-aligned: the stream).
+ * iv: the chaining value (COMPRESSION_CHAIN only; NULL = FIPS IV).  f_words
+ * (ceil(witness_bits / 64)) and h_words (ceil(columns / 64)) are packed
+ * least significant bit first (PackedWitness, witgen.rs:207-305); h bit 0 is
+ * the constant one.  digest: the final state as 32 big-endian bytes
+ * (block-aligned: the SHA-256 digest of the message).  satisfied: 1 when
+ * every row holds over the integers for this witness.  Any output may be NULL. */
+int32_t bitz_sha256_witness(const bitz_sha256* c, const uint8_t* message, const uint32_t iv[8], uint64_t* f_words,
+                            uint64_t* h_words, uint8_t digest[32], int32_t* satisfied);
+/* The parity projection of A, B, C prepared for the prover (as
+ * bitz_matrices_prepare) and/or the statement digest over it (as
+ * bitz_constraint_matrix_digest).  Either output may be NULL. */
+int32_t bitz_sha256_prepare(const bitz_sha256* c, bitz_matrices** out, uint8_t digest[32]);
+void bitz_sha256_free(bitz_sha256* c);
+
 /* ---- M^T (crates/circuit/src/matrix_transpose.rs) ------------------------- */
 typedef struct bitz_mt bitz_mt;
 /* MaterializedMTranspose: CSC of M (column_offsets[columns+1], row_indices). */
