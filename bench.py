@@ -495,17 +495,19 @@ def bench_inner(G, args) -> int:
 
 
 def bench_serve(G, args) -> int:
-    """Serving: Poisson arrivals of SHA-256-chain-shaped proving requests
-    (blocks = 2^U, U uniform in 0..8; synthetic circuits of the chain's
-    shape, see bitz_amd/serving.py), admitted by HBM budget into concurrent
-    slots.  The same requests proved one after another give the plain rate."""
+    """Serving: Poisson arrivals of SHA-256 proving requests (a compression
+    chain of blocks = 2^U blocks, U uniform in 0..8, message and chaining
+    value uniform from the seed; see bitz_amd/serving.py), admitted by HBM
+    budget into concurrent slots.  Each request: witness (host), assignment,
+    Az/Bz/Cz and the PIOP on the device.  The same requests proved one after
+    another give the plain rate."""
     from bitz_amd import serving as S
-    srv = S.ProvingServer(hbm_budget=int(args.serve_budget_gib * (1 << 30)), slots=args.serve_slots, seed=args.seed)
-    log("bench: preparing the nine chain circuits (1 .. 256 blocks)")
+    srv = S.ProvingServer(hbm_budget=int(args.serve_budget_gib * (1 << 30)), slots=args.serve_slots)
+    log("bench: preparing the nine SHA-256 circuits (1 .. 256 blocks)")
     rng = np.random.default_rng(args.seed + 1)
     for u in range(9):  # circuits and kernels warm, like a deployed service
         b = 1 << u
-        srv.prove(10**6 + u, b, S.request_bits(rng, srv.circuit(b).columns))
+        srv.prove(10**6 + u, b, *S.random_input(rng, b))
     G.synchronize()
     reqs = S.poisson_requests(args.serve_rate, args.serve_requests, args.seed)
     log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, {sum(r.blocks for r in reqs)} blocks")
@@ -516,20 +518,22 @@ def bench_serve(G, args) -> int:
     lat = np.array([r.latency for r in reqs]) * 1e3
     blocks = sum(r.blocks for r in reqs)
     # Capacity: the same requests all queued at once (the server saturated).
-    sat = [S.Request(r.rid, 0.0, r.blocks, r.bits) for r in reqs]
+    sat = [S.Request(r.rid, 0.0, r.blocks, r.message, r.iv) for r in reqs]
     t0 = time.perf_counter()
     srv.serve(sat)
     saturated = time.perf_counter() - t0
+    sat_lat = np.array([r.latency for r in sat]) * 1e3
     sat_peak = srv.peak_in_flight
     t0 = time.perf_counter()
     for r in reqs:
-        srv.prove(r.rid, r.blocks, r.bits)
+        srv.prove(r.rid, r.blocks, r.message, r.iv)
     G.synchronize()
     serial = time.perf_counter() - t0
+    c256 = srv.circuit(256)
     line = {
         "metric": METRIC,
         "value": blocks / span,
-        "unit": "SHA-256 blocks/s (synthetic chain-shaped circuits)",
+        "unit": "SHA-256 blocks/s",
         "n_gpus": 1,
         "steps": len(reqs),
         "warmup": 9,
@@ -540,46 +544,117 @@ def bench_serve(G, args) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "gf2_128",
-        "data": "synthetic (random chain-shaped R1CS per block count, random assignment bits per request)",
-        "config": {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.serve_rate}/s, blocks 2^U with U "
-                               f"uniform in 0..8, {args.serve_slots} slots, HBM budget {args.serve_budget_gib} GiB; "
-                               "each request: assignment MLE, Az/Bz/Cz, prove_spartan_piop (duplex transcript)",
-                   "requests": len(reqs), "blocks": blocks},
+        "data": "synthetic (message bytes and chaining value per request uniform from the seed)",
+        "config": {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.serve_rate}/s of SHA-256 "
+                               "compression chains, blocks 2^U with U uniform in 0..8, "
+                               f"{args.serve_slots} slots, HBM budget {args.serve_budget_gib} GiB; each request: "
+                               "witness, assignment MLE, Az/Bz/Cz, prove_spartan_piop (duplex transcript)",
+                   "requests": len(reqs), "blocks": blocks,
+                   "circuit_256_blocks": {"rows": c256.rows, "columns": c256.columns, "nnz": c256.nnz}},
         "proofs_per_s": len(reqs) / span,
         "peak_in_flight": peak_in_flight,
         "peak_hbm_gib": peak_bytes / (1 << 30),
         "saturated": {"blocks_per_s": blocks / saturated, "proofs_per_s": len(reqs) / saturated,
-                      "peak_in_flight": sat_peak},
+                      "peak_in_flight": sat_peak, "p50_ms": float(np.percentile(sat_lat, 50)),
+                      "p99_ms": float(np.percentile(sat_lat, 99))},
         "one_at_a_time": {"blocks_per_s": blocks / serial, "proofs_per_s": len(reqs) / serial},
         "capacity_vs_one_at_a_time": serial / saturated,
     }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_sha256(args.seed)
     print(json.dumps(line), flush=True)
     return 0
 
 
+def cpu_baseline_sha256(seed: int, budget_s: float = 10.0) -> dict:
+    """The oracle proving one-block SHA-256 compressions (random message and
+    chaining value) on one host core with the bit-serial multiply: Az/Bz/Cz,
+    outer sumcheck, bind_and_batch, inner sumcheck, repeated for about
+    budget_s seconds."""
+    from bitz_amd import sha256 as SH
+    from oracle import oracle as O
+    O.set_kernel("portable")
+    c = SH.Sha256Circuit(SH.COMPRESSION_CHAIN, 1)
+    _, digest = c.prepare(matrices=False)
+    ex = c.export()
+    mats = [O.CsrMatrix(ex[k][0], ex[k][1], ex[k][3]) for k in "abc"]
+    rv, cv = c.num_row_vars, c.num_column_vars
+    rng = np.random.default_rng(seed)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s or done == 0:
+        msg = rng.integers(0, 256, 64, dtype=np.uint8).tobytes()
+        iv = [int(x) for x in rng.integers(0, 2**32, 8, dtype=np.uint64)]
+        w = c.witness(msg, iv=iv)
+        bits = np.unpackbits(w.h.view(np.uint8), bitorder="little")[: c.columns]
+        h = np.zeros((1 << cv, 2), np.uint64)
+        h[: c.columns, 0] = bits
+        prods = []
+        for k in "abc":  # A h, B h, C h: one multiply per nonzero, XOR per row
+            st, co, _, par = ex[k]
+            terms = O.mul_batch(par, h[co.astype(np.int64)])
+            p = np.zeros((1 << rv, 2), np.uint64)
+            rows_of = np.repeat(np.arange(c.rows), np.diff(st.astype(np.int64)))
+            np.bitwise_xor.at(p, rows_of, terms)
+            prods.append(p)
+        t = O.DuplexTranscript(b"cpu/sha256")
+        O.absorb(t, 0, digest)
+        tau = np.array([[(x := O.squeeze(t)) & (2**64 - 1), x >> 64] for _ in range(rv)], np.uint64)
+        lo, hi = O.eq_table(tau[: rv // 2]), O.eq_table(tau[rv // 2:])
+        rc, outer = O.outer_sumcheck(t, 0, lo, hi, *prods)
+        rho = O.squeeze(t)
+        a_c, b_c, c_c = (O.to_int(x) for x in outer["claims"])
+        claim = a_c ^ O.to_int(O.mul(rho, b_c)) ^ O.to_int(O.mul(O.to_int(O.mul(rho, rho)), c_c))
+        d = O.bind_and_batch(mats, c.rows, rv, cv, outer["points"], rho)
+        rc2, _ = O.inner_sumcheck(t, claim, d, h)
+        assert rc == 0 and rc2 == 0
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "SHA-256 blocks/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, bit-serial multiply, 1 thread: {done} one-block compression proofs "
+                      f"(witness, Az/Bz/Cz, outer 2^{rv}, bind_and_batch over {sum(c.nnz)} nonzeros, "
+                      f"inner 2^{cv}) in {dt:.1f} s"}
+
+
 def bench_bind(G, args) -> int:
-    """bind_and_batch over a 256-block SHA-256-chain-shaped circuit (178432
-    rows, 2^23 columns, 8 nonzeros per row in each of A, B, C; synthetic, see
-    bitz_amd/serving.py), the output D left on the device."""
+    """bind_and_batch over the positions of the 256-block SHA-256 compression
+    chain (47104 rows, 2^23 columns, 13.9 M positions from the producer) with
+    every value a uniform nonzero element (as a projection onto a large field
+    leaves them), the output D left on the device.  The same call on the
+    parity projection itself (where even coefficients map to zero and the
+    device keeps only the odd ones) is reported beside it."""
     from bitz_amd import serving as S
     G.init(0)
-    c = S.make_circuit(256, args.seed)
-    nnz = c.nnz
+    c = S.make_circuit(256)
+    ex = c.circuit.export()
     rng = np.random.default_rng(args.seed)
+    csr = []
+    for k in "abc":
+        st, co = ex[k][0], ex[k][1]
+        vals = rng.integers(0, 2**64, size=(len(co), 2), dtype=np.uint64)
+        vals[(vals[:, 0] == 0) & (vals[:, 1] == 0), 0] = 1
+        csr.append((st, co, vals))
+    generic = G.PreparedConstraintMatrices(c.rows, c.row_vars, c.col_vars, csr)
+    nnz = c.nnz
     rx = G.DeviceTable(rng.integers(0, 2**64, size=(c.row_vars, 2), dtype=np.uint64))
     rho = 0x1234567890ABCDEF1234567890ABCDEF
     out = G.DeviceTable(length=1 << c.col_vars)
-    for _ in range(args.warmup):
-        c.matrices.bind_and_batch(rx, rho, device=True, out=out)
-    G.synchronize()
-    times = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        c.matrices.bind_and_batch(rx, rho, device=True, out=out)
-        times.append(time.perf_counter() - t0)
-    G.synchronize()
-    wall = time.perf_counter() - t_start
+
+    def timed(m):
+        for _ in range(args.warmup):
+            m.bind_and_batch(rx, rho, device=True, out=out)
+        G.synchronize()
+        times = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            m.bind_and_batch(rx, rho, device=True, out=out)
+            times.append(time.perf_counter() - t0)
+        G.synchronize()
+        return time.perf_counter() - t_start, np.array(times) * 1e3
+
+    wall, times = timed(generic)
+    pwall, ptimes = timed(c.matrices)
+    live = int(sum(int((ex[k][2] & 1).sum()) for k in "abc"))
     algo = nnz * (4 + 1 + 16) + 16 * (1 << c.col_vars) + 16 * 4 * (1 << c.row_vars)
     line = {
         "metric": "bind_and_batch calls/sec, 1 MI355X",
@@ -589,17 +664,20 @@ def bench_bind(G, args) -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
-        "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
-        "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+        "p50_ms": float(np.percentile(times, 50)),
+        "p99_ms": float(np.percentile(times, 99)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "gf2_128",
-        "data": "synthetic (random 256-block chain-shaped R1CS)",
+        "data": "positions of the 256-block SHA-256 compression chain (producer), uniform nonzero values, "
+                "random row point",
         "config": {"workload": f"bind_and_batch: {c.rows} rows (2^{c.row_vars}), 2^{c.col_vars} columns, "
                                f"{nnz} nonzeros", "nnz": nnz},
         "algorithmic_gb_per_s": algo * args.steps / wall / 1e9,
         "algorithmic_bytes": algo,
+        "parity_projection": {"ms_per_step": pwall / args.steps * 1e3, "p50_ms": float(np.percentile(ptimes, 50)),
+                              "nonzero_values": live},
     }
     print(json.dumps(line), flush=True)
     return 0

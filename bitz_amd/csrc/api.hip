@@ -2481,7 +2481,8 @@ struct bitz_matrices {
     Dev col_starts, row_idx, which, values, eq_rows, row_scale, scales, point, scratch;
     Dev rs[3], cs[3], vs[3];  // row-major copies of A, B, C (products)
     Dev prod;                 // bind_and_batch: one product per nonzero
-    size_t nnz = 0;
+    Dev long_chunks;          // {column, first, end} runs of the long columns
+    size_t nnz = 0, n_long_chunks = 0;
     std::mutex mu;            // calls on one handle share its scratch and stream: one at a time
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
@@ -2494,6 +2495,10 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
     if (row_vars > 30 || col_vars > 31 || rows > ((size_t)1 << row_vars))
         return fail(BITZ_ERR_ARG, "matrix shape out of range");
     const size_t cols = (size_t)1 << col_vars;
+    // The device keeps only entries whose value is nonzero: a zero entry adds
+    // nothing to A h, B h, C h or to D (a projection can map coefficients to
+    // zero, and the layout keeps their positions).
+    auto live = [&](int m, uint64_t k) { return values[m][k].lo != 0 || values[m][k].hi != 0; };
     // Column-major copy of all three matrices: counting sort by column.
     std::vector<uint32_t> count(cols + 1, 0);
     size_t nnz = 0;
@@ -2502,9 +2507,11 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
         const size_t mn = row_starts[m][rows];
         for (size_t k = 0; k < mn; ++k) {
             if (columns[m][k] >= cols) return fail(BITZ_ERR_ARG, "column index out of range");
-            count[columns[m][k] + 1]++;
+            if (live(m, k)) {
+                count[columns[m][k] + 1]++;
+                ++nnz;
+            }
         }
-        nnz += mn;
     }
     if (nnz >= 0xFFFFFFFFull) return fail(BITZ_ERR_ARG, "too many nonzeros");
     for (size_t j = 0; j < cols; ++j) count[j + 1] += count[j];
@@ -2515,10 +2522,21 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
     for (int m = 0; m < 3; ++m)
         for (size_t i = 0; i < rows; ++i)
             for (uint64_t k = row_starts[m][i]; k < row_starts[m][i + 1]; ++k) {
+                if (!live(m, k)) continue;
                 const uint32_t pos = fill[columns[m][k]]++;
                 rowv[pos] = (uint32_t)i;
                 whichv[pos] = (uint8_t)m;
                 vals[pos] = values[m][k];
+            }
+    // Columns longer than kBindLongColumn entries are summed by workgroups
+    // over runs of at most kBindLongRun entries instead of by one thread.
+    std::vector<uint32_t> chunks;
+    for (size_t j = 0; j < cols; ++j)
+        if (starts[j + 1] - starts[j] > bitz::kBindLongColumn)
+            for (uint32_t lo = starts[j]; lo < starts[j + 1]; lo += bitz::kBindLongRun) {
+                chunks.push_back((uint32_t)j);
+                chunks.push_back(lo);
+                chunks.push_back(std::min<uint32_t>(lo + bitz::kBindLongRun, starts[j + 1]));
             }
     auto* h = new bitz_matrices;
     auto cleanup = [&](int32_t c) {
@@ -2542,21 +2560,34 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
     if ((rc = h->row_scale.alloc(3 * h->rows_padded * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->scales.alloc(3 * sizeof(f128)))) return cleanup(rc);
     if ((rc = h->point.alloc(std::max<uint32_t>(row_vars, 1) * sizeof(f128)))) return cleanup(rc);
-    if (hipMemcpy(h->col_starts.p, starts.data(), starts.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+    h->n_long_chunks = chunks.size() / 3;
+    if ((rc = h->long_chunks.alloc(std::max<size_t>(chunks.size(), 1) * 4))) return cleanup(rc);
+    if ((!chunks.empty() &&
+         hipMemcpy(h->long_chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(h->col_starts.p, starts.data(), starts.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         (nnz && hipMemcpy(h->row_idx.p, rowv.data(), nnz * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (nnz && hipMemcpy(h->which.p, whichv.data(), nnz, hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(h->values.p, vals.data(), vals.size() * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "matrix upload failed"));
     for (int m = 0; m < 3; ++m) {
-        const size_t mn = row_starts[m][rows];
-        std::vector<uint32_t> rs32(rows + 1);
-        for (size_t i = 0; i <= rows; ++i) rs32[i] = (uint32_t)row_starts[m][i];
+        std::vector<uint32_t> rs32(rows + 1), cs32;
+        std::vector<bitz_f128> vs;
+        rs32[0] = 0;
+        for (size_t i = 0; i < rows; ++i) {
+            for (uint64_t k = row_starts[m][i]; k < row_starts[m][i + 1]; ++k)
+                if (live(m, k)) {
+                    cs32.push_back(columns[m][k]);
+                    vs.push_back(values[m][k]);
+                }
+            rs32[i + 1] = (uint32_t)cs32.size();
+        }
+        const size_t mn = cs32.size();
         if ((rc = h->rs[m].alloc((rows + 1) * 4)) || (rc = h->cs[m].alloc(std::max<size_t>(mn, 1) * 4)) ||
             (rc = h->vs[m].alloc(std::max<size_t>(mn, 1) * sizeof(f128))))
             return cleanup(rc);
         if (hipMemcpy(h->rs[m].p, rs32.data(), (rows + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            (mn && hipMemcpy(h->cs[m].p, columns[m], mn * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-            (mn && hipMemcpy(h->vs[m].p, values[m], mn * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess))
+            (mn && hipMemcpy(h->cs[m].p, cs32.data(), mn * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+            (mn && hipMemcpy(h->vs[m].p, vs.data(), mn * sizeof(f128), hipMemcpyHostToDevice) != hipSuccess))
             return cleanup(fail(BITZ_ERR_DEVICE, "matrix upload failed"));
     }
     *out = h;
@@ -2601,6 +2632,9 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
     const unsigned cgrid = (unsigned)std::max<size_t>(1, std::min<size_t>((h->cols + kBlock - 1) / kBlock, 1u << 20));
     hipLaunchKernelGGL(bitz::k_bind_segments, dim3(cgrid), dim3(kBlock), 0, s, h->col_starts.u32(), h->prod.f(),
                        target, h->cols);
+    if (h->n_long_chunks)  // the long columns, left zero above: XOR of each run's sum
+        hipLaunchKernelGGL(bitz::k_bind_long, dim3((unsigned)h->n_long_chunks), dim3(kBlock), 0, s,
+                           h->long_chunks.u32(), h->prod.f(), target);
     TRY(check_launch());
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));

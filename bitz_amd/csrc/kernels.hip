@@ -402,9 +402,20 @@ __global__ __launch_bounds__(kBlock) void k_bind_segments(const uint32_t* col_st
                                                          size_t cols) {
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < cols; j += (size_t)gridDim.x * blockDim.x) {
         g4 s = g_zero();
-        for (uint32_t k = col_starts[j]; k < col_starts[j + 1]; ++k) s = g_add(s, g_load_hbm(prod + k));
+        const uint32_t lo = col_starts[j], hi = col_starts[j + 1];
+        if (hi - lo <= kBindLongColumn)
+            for (uint32_t k = lo; k < hi; ++k) s = g_add(s, g_load_hbm(prod + k));
         g_store_hbm(out + j, s);
     }
+}
+
+// One run {column, first, end} of a long column per workgroup: the run's XOR
+// into out[column] (left zero by k_bind_segments).
+__global__ __launch_bounds__(kBlock) void k_bind_long(const uint32_t* chunks, const f128* prod, f128* out) {
+    const uint32_t col = chunks[3 * blockIdx.x], lo = chunks[3 * blockIdx.x + 1], hi = chunks[3 * blockIdx.x + 2];
+    g4 v[1] = {g_zero()};
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += kBlock) v[0] = g_add(v[0], g_load_hbm(prod + k));
+    block_xor_to_global<1>(v, reinterpret_cast<uint32_t*>(out + col));
 }
 
 // row_scale[m][i] = eq(i, r_x) * scale_m.

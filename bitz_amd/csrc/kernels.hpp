@@ -42,6 +42,11 @@ __global__ void k_bind_products(const uint32_t* rows, const uint8_t* which, cons
                                 size_t rows_padded, f128* prod, size_t nnz);
 __global__ void k_bind_segments(const uint32_t* col_starts, const f128* prod, f128* out, size_t cols);
 constexpr size_t kBindSmem = sizeof(uint4) * (4 * 64 * kLaneTableEntries);
+// bind_and_batch: columns with more nonzeros than this are summed in runs of
+// kBindLongRun entries, one workgroup per run (k_bind_long).
+constexpr uint32_t kBindLongColumn = 64;
+constexpr uint32_t kBindLongRun = 2048;
+__global__ void k_bind_long(const uint32_t* chunks, const f128* prod, f128* out);
 __global__ void k_row_scales(const f128* eq_rows, f128 rho, f128 rho2, f128* row_scale, size_t rows_padded);
 __global__ void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols, const f128* vals, const f128* h,
                             size_t rows, f128* out, size_t padded);

@@ -1,23 +1,27 @@
-"""Serving: a proving loop over Spartan PIOP requests of uneven size with
+"""Serving: a proving loop over SHA-256 proving requests of uneven size with
 admission by an HBM budget and concurrent requests overlapping on the GPU.
 
-A request is the sumcheck stage of a SHA-256-chain proof of B blocks:
-assignment MLE, Az/Bz/Cz, and prove_spartan_piop (outer sumcheck,
-bind_and_batch, inner sumcheck; crates/spartan/src/piop.rs:67-112).  The
-reference has no SHA-256 R1CS producer over GF(2^128) (its product builder
-works over a prime field, crates/spartan/src/matrix.rs:336-370), so each
-block count gets a synthetic circuit of the chain's shape: 696 B + 256
-constraint rows and 20457 B assignment columns (tooling/cli/src/circuits.rs:
-167-209, tooling/cli/tests/sha256_spartan.rs:44-49), eight nonzeros per row
-in each of A, B and C.  Circuits are prepared once per block count; a
-request brings its own assignment bits.
+A request is the sumcheck stage of a proof that a chain of B SHA-256
+compressions maps its chaining value through its message blocks: the
+witness (host, C++: bitz_sha256_witness), the assignment MLE, Az/Bz/Cz and
+prove_spartan_piop (outer sumcheck, bind_and_batch, inner sumcheck;
+crates/spartan/src/piop.rs:67-112), every table on the device.  The circuit
+is the compression chain of the SHA-256 producer: one block is the
+reference's compression_circuit (crates/circuit/src/sha256.rs:476-490), B
+blocks chain it, and the message and chaining value are witness inputs, so
+one prepared circuit (matrices in the parity projection, statement digest)
+per block count serves every request of that size, as the reference's
+benchmark prepares one instance per size (tooling/cli/benches/
+sha256_spartan.rs:45-90).
 
 Scheduling: requests wait in arrival order; one is admitted when a slot is
 free and its HBM footprint fits the budget beside the requests in flight.
 Every admitted request runs on its own host thread with its own HIP streams,
-so the requests' kernels overlap on the device; a request that arrives while
+so the requests' kernels overlap on the device (and one request's witness
+generation overlaps the others' device work); a request that arrives while
 others run starts as soon as a slot and memory free up.  Each result depends
-only on its request (its circuit, its bits, its transcript), not on the mix.
+only on its request (its circuit, its message and chaining value, its
+transcript), not on the mix.
 """
 from __future__ import annotations
 
@@ -29,43 +33,36 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (PreparedConstraintMatrices, SpartanPiopOutput, Transcript, assignment_mle,
-               constraint_matrix_digest, init, prove_spartan_piop)
-
-ROWS_PER_BLOCK, ROWS_FIXED = 696, 256
-COLUMNS_PER_BLOCK = 20457
-NNZ_PER_ROW = 8
-
-
-def chain_shape(blocks: int) -> tuple[int, int, int, int]:
-    """(rows, columns, row_vars, col_vars) of a SHA-256 chain of `blocks` blocks."""
-    rows = ROWS_PER_BLOCK * blocks + ROWS_FIXED
-    cols = COLUMNS_PER_BLOCK * blocks
-    return rows, cols, max(rows - 1, 1).bit_length(), max(cols - 1, 1).bit_length()
-
-
-def _random_csr(rng, rows: int, cols: int, k: int):
-    c = np.sort(rng.integers(0, cols, size=(rows, k), dtype=np.int64), axis=1)
-    keep = np.ones_like(c, dtype=bool)
-    keep[:, 1:] = c[:, 1:] != c[:, :-1]
-    counts = keep.sum(axis=1)
-    starts = np.zeros(rows + 1, np.uint64)
-    np.cumsum(counts, out=starts[1:])
-    columns = c[keep].astype(np.uint32)
-    values = rng.integers(0, 2**64, size=(len(columns), 2), dtype=np.uint64)
-    return starts, columns, values
+from . import PreparedConstraintMatrices, SpartanPiopOutput, Transcript, assignment_mle_words, init, prove_spartan_piop
+from . import sha256 as sha
 
 
 @dataclass
 class Circuit:
     blocks: int
-    rows: int
-    columns: int
-    row_vars: int
-    col_vars: int
+    circuit: sha.Sha256Circuit
     matrices: PreparedConstraintMatrices
     digest: bytes
-    nnz: int = 0
+
+    @property
+    def rows(self) -> int:
+        return self.circuit.rows
+
+    @property
+    def columns(self) -> int:
+        return self.circuit.columns
+
+    @property
+    def row_vars(self) -> int:
+        return self.circuit.num_row_vars
+
+    @property
+    def col_vars(self) -> int:
+        return self.circuit.num_column_vars
+
+    @property
+    def nnz(self) -> int:
+        return sum(self.circuit.nnz)
 
     def footprint(self) -> int:
         """Device bytes a request holds at its peak: the assignment and D
@@ -74,14 +71,10 @@ class Circuit:
         return 16 * int((1 << self.col_vars) * 3.5 + (1 << self.row_vars) * 5.25)
 
 
-def make_circuit(blocks: int, seed: int = 0) -> Circuit:
-    rows, cols, rv, cv = chain_shape(blocks)
-    rng = np.random.default_rng(seed * 1000 + blocks)
-    csr = [_random_csr(rng, rows, cols, NNZ_PER_ROW) for _ in range(3)]
-    m = csr[0][:2]  # the positions of M (a stand-in with A's pattern)
-    digest = constraint_matrix_digest(m, cols, rows, cols, csr)
-    return Circuit(blocks, rows, cols, rv, cv, PreparedConstraintMatrices(rows, rv, cv, csr), digest,
-                   sum(len(m[1]) for m in csr))
+def make_circuit(blocks: int) -> Circuit:
+    c = sha.Sha256Circuit(sha.COMPRESSION_CHAIN, blocks)
+    matrices, digest = c.prepare()
+    return Circuit(blocks, c, matrices, digest)
 
 
 @dataclass
@@ -89,28 +82,30 @@ class Request:
     rid: int
     arrival: float  # seconds from the start of serving
     blocks: int
-    bits: np.ndarray
+    message: bytes  # 64 bytes per block
+    iv: tuple       # the chaining value, 8 words
     admitted: float = 0.0
     done: float = 0.0
     result: SpartanPiopOutput | None = field(default=None, repr=False)
+    digest: bytes = b""  # the chain's output state
 
     @property
     def latency(self) -> float:
         return self.done - self.arrival
 
 
-def request_bits(rng, columns: int) -> np.ndarray:
-    bits = rng.integers(0, 2, size=columns, dtype=np.uint8)
-    bits[0] = 1  # the R1CS constant one
-    return bits
+def random_input(rng, blocks: int) -> tuple[bytes, tuple]:
+    """Message bytes and chaining value, uniform from the generator."""
+    return (rng.integers(0, 256, 64 * blocks, dtype=np.uint8).tobytes(),
+            tuple(int(x) for x in rng.integers(0, 2**32, 8, dtype=np.uint64)))
 
 
 class ProvingServer:
     """Admits requests by HBM budget and slots; see the module notes."""
 
-    def __init__(self, hbm_budget: int = 32 << 30, slots: int = 8, seed: int = 0):
+    def __init__(self, hbm_budget: int = 32 << 30, slots: int = 8):
         init()
-        self.hbm_budget, self.slots, self.seed = hbm_budget, slots, seed
+        self.hbm_budget, self.slots = hbm_budget, slots
         self.circuits: dict[int, Circuit] = {}
         self._lock = threading.Lock()
         self.peak_bytes = 0      # largest footprint in flight during the last serve()
@@ -119,16 +114,18 @@ class ProvingServer:
     def circuit(self, blocks: int) -> Circuit:
         with self._lock:
             if blocks not in self.circuits:
-                self.circuits[blocks] = make_circuit(blocks, self.seed)
+                self.circuits[blocks] = make_circuit(blocks)
             return self.circuits[blocks]
 
-    def prove(self, rid: int, blocks: int, bits: np.ndarray) -> SpartanPiopOutput:
-        """One request, every table on the device: assignment, products, PIOP."""
+    def prove(self, rid: int, blocks: int, message: bytes, iv) -> tuple[SpartanPiopOutput, bytes]:
+        """One request: the witness on the host, then assignment, products and
+        PIOP with every table on the device.  Returns (proof, output state)."""
         c = self.circuit(blocks)
-        h = assignment_mle(bits, c.columns, device=True)
+        w = c.circuit.witness(message, iv=iv)
+        h = assignment_mle_words(w.h, c.columns, c.columns, device=True)
         products = c.matrices.r1cs_products(h)
         t = Transcript.duplex(b"bitz/serve/%d" % rid)
-        return prove_spartan_piop(t, c.matrices, c.digest, products, h)
+        return prove_spartan_piop(t, c.matrices, c.digest, products, h), w.digest
 
     def serve(self, requests: list[Request]) -> list[Request]:
         """Runs requests at their arrival times (seconds from now); returns
@@ -150,7 +147,7 @@ class ProvingServer:
                     r = pending.popleft()
                     r.admitted = now
                     used += fp
-                    in_flight[pool.submit(self.prove, r.rid, r.blocks, r.bits)] = (r, fp)
+                    in_flight[pool.submit(self.prove, r.rid, r.blocks, r.message, r.iv)] = (r, fp)
                     self.peak_bytes = max(self.peak_bytes, used)
                     self.peak_in_flight = max(self.peak_in_flight, len(in_flight))
                 timeout = None
@@ -162,7 +159,7 @@ class ProvingServer:
                 done, _ = wait(list(in_flight), timeout=timeout, return_when=FIRST_COMPLETED)
                 for f in done:
                     r, fp = in_flight.pop(f)
-                    r.result = f.result()
+                    r.result, r.digest = f.result()
                     r.done = time.perf_counter() - t0
                     used -= fp
         return requests
@@ -176,5 +173,5 @@ def poisson_requests(rate: float, count: int, seed: int, max_log_blocks: int = 8
     for i in range(count):
         t += float(rng.exponential(1.0 / rate))
         blocks = 1 << int(rng.integers(0, max_log_blocks + 1))
-        out.append(Request(i, t, blocks, request_bits(rng, chain_shape(blocks)[1])))
+        out.append(Request(i, t, blocks, *random_input(rng, blocks)))
     return out

@@ -199,6 +199,37 @@ def test_bind_and_batch_matches_oracle(rows, row_vars, col_vars, per_row):
     assert e.value.variant == "InvalidRowPointLength"
 
 
+@pytest.mark.parametrize("rows,row_vars,col_vars", [(5000, 13, 10), (300, 9, 8)])
+def test_bind_and_products_with_long_columns_and_zero_values(rows, row_vars, col_vars):
+    """Every row holds columns 0 and 5 (long columns, summed in runs by
+    workgroups) and a third of the values are zero (entries the device
+    drops); D and A h, B h, C h against the oracle and the definition."""
+    rng = np.random.default_rng(rows + 1)
+    csr = []
+    for m in range(3):
+        starts, cl = [0], []
+        for _ in range(rows):
+            cs = np.unique(np.concatenate([[0, 5], rng.integers(0, 1 << col_vars, size=4)]))
+            cl += cs.tolist()
+            starts.append(len(cl))
+        vals = elements_fast(900 + m, len(cl))
+        vals[rng.integers(0, 3, size=len(cl)) == 0] = 0
+        csr.append((np.array(starts, np.uint64), np.array(cl, np.uint32), vals))
+    rx = elements(17, row_vars)
+    rho = 0x0123456789ABCDEFFEDCBA9876543210
+    prepared = G.PreparedConstraintMatrices(rows, row_vars, col_vars, csr)
+    mats = [O.CsrMatrix(*m) for m in csr]
+    assert np.array_equal(prepared.bind_and_batch(rx, rho), O.bind_and_batch(mats, rows, row_vars, col_vars, rx, rho))
+    h = elements_fast(31, 1 << col_vars)
+    out = prepared.r1cs_products(h)
+    for m, t in enumerate((out.az, out.bz, out.cz)):
+        st, co, va = csr[m]
+        ref = np.zeros((1 << row_vars, 2), np.uint64)
+        terms = O.mul_batch(va, h[co.astype(np.int64)])
+        np.bitwise_xor.at(ref, np.repeat(np.arange(rows), np.diff(st.astype(np.int64))), terms)
+        assert np.array_equal(t, ref)
+
+
 def test_mt_apply_and_transpose_inner_match_oracle():
     rng = np.random.default_rng(3)
     rows, cols = 5000, 3000

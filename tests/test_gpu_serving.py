@@ -1,11 +1,15 @@
 """Serving: every result is the same alone and in the mix, admission keeps
-the footprint in flight within the HBM budget, and the served proofs are
-the PIOP the oracle composes (checked for one request)."""
+the footprint in flight within the HBM budget, each request proves its own
+SHA-256 chain (output state = the compressions of its message from its
+chaining value), and a served proof is the PIOP the oracle composes."""
+import struct
+
 import numpy as np
 import pytest
 
-import bitz_amd as G
 from bitz_amd import serving as S
+from common import compress_py
+from test_gpu_sha256 import prove_and_check
 
 pytestmark = pytest.mark.gpu
 
@@ -16,22 +20,39 @@ def same(a, b):
             a.matrix_evaluation == b.matrix_evaluation and a.final_claim == b.final_claim)
 
 
+def chain(iv, message):
+    st = list(iv)
+    for b in range(len(message) // 64):
+        st = compress_py(st, message[64 * b: 64 * b + 64])
+    return struct.pack(">8I", *st)
+
+
 def test_results_are_the_same_alone_and_in_the_mix():
-    srv = S.ProvingServer(hbm_budget=1 << 30, slots=4, seed=1)
+    srv = S.ProvingServer(hbm_budget=1 << 30, slots=4)
     reqs = S.poisson_requests(rate=5000.0, count=12, seed=3, max_log_blocks=3)
-    alone = [srv.prove(r.rid, r.blocks, r.bits) for r in reqs]
+    alone = [srv.prove(r.rid, r.blocks, r.message, r.iv) for r in reqs]
     served = srv.serve(reqs)
     assert srv.peak_in_flight > 1  # the requests did overlap
-    for a, r in zip(alone, served):
-        assert r.result is not None and same(a, r.result)
+    for (proof, digest), r in zip(alone, served):
+        assert r.result is not None and same(proof, r.result)
+        assert r.digest == digest == chain(r.iv, r.message)
         assert r.done >= r.admitted >= r.arrival - 1e-6
 
 
 def test_admission_keeps_the_budget():
-    srv = S.ProvingServer(hbm_budget=1 << 30, slots=8, seed=1)
+    srv = S.ProvingServer(hbm_budget=1 << 30, slots=8)
     reqs = S.poisson_requests(rate=5000.0, count=10, seed=4, max_log_blocks=3)
     fps = [srv.circuit(r.blocks).footprint() for r in reqs]
     srv.hbm_budget = 2 * max(fps)  # at most two of the largest at once
     srv.serve(reqs)
     assert srv.peak_bytes <= srv.hbm_budget
     assert all(r.result is not None for r in reqs)
+
+
+def test_a_served_proof_matches_the_oracle():
+    srv = S.ProvingServer(hbm_budget=1 << 30, slots=2)
+    reqs = [r for r in S.poisson_requests(rate=5000.0, count=8, seed=5, max_log_blocks=2) if r.blocks == 2][:1]
+    assert reqs
+    r = srv.serve(reqs)[0]
+    _, ref = prove_and_check(srv.circuit(2).circuit, r.message, list(r.iv), label=b"bitz/serve/%d" % r.rid)
+    assert same(r.result, ref)
