@@ -611,7 +611,7 @@ struct bitz_outer {
     std::vector<f128> tau, tau_inv;
     std::vector<f128> scale_big, scale_tail;  // divide out the factors' constants
     std::vector<f128> kpre[2], ksuf[2];       // prefix / suffix products of (1 + tau_i) per factor
-    Dev htabs4, rtabs, rtabs4;
+    Dev htabs4, rtabs4;
     struct Run {  // variables [a, b) read from factor `table` (0 = eq_low, 1 = eq_high)
         int table, a, b;
     };
@@ -757,15 +757,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batc
     h->htab_total = htab_total;
     if (!batch_member &&
         (h->rtabs4.alloc((size_t)std::max(n, 1) * kConstTable4Entries * sizeof(uint4)) ||
-         h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4)) ||
-         h->rtabs.alloc((size_t)std::max(n, 1) * kConstTableEntries * sizeof(uint4))))
+         h->htabs4.alloc(std::max<size_t>(htab_total, 1) * kConstTable4Entries * sizeof(uint4))))
         return false;
     static bool attr_set = false;
     if (!attr_set) {
-        const void* fns[2] = {reinterpret_cast<const void*>(&bitz::k_outer_eq<true, false>),
-                              reinterpret_cast<const void*>(&bitz::k_outer_eq<true, true>)};
-        for (const void* f : fns)
-            (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kOuterEqSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_r0<false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_r0<true>),
@@ -919,11 +914,10 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     return true;
 }
 
-// Launches the eq-factored round-k kernel into slot k: round 0 and the
-// multi-iteration fold rounds through lane tables (k_outer_r0,
-// k_outer_fold, folding with the 4-bit table rtab4 of r_{k-1}), the
-// single-iteration fold rounds through k_outer_eq (8-bit table rtab8).
-static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab8, const uint4* rtab4, bool q1,
+// Launches the eq-factored round-k kernel into slot k: round 0 through
+// k_outer_r0, the fold rounds through k_outer_fold (lane-table products,
+// folding with the 4-bit table rtab4 of r_{k-1}).
+static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab4, bool q1,
                                f128* const out[3], Timed* tm = nullptr) {
     if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
@@ -959,19 +953,19 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab8, const u
                                OUTER_FOLD_ARGS);
 #undef OUTER_FOLD_ARGS
     } else {
-        // Single iteration: 256-thread workgroups (whole waves) so small
-        // rounds spread over every CU.
-        const unsigned block = (unsigned)std::max<size_t>(64, std::min<size_t>(g.threads, 256));
+        // Single iteration: the lane-table fold kernel in workgroups of at
+        // most 256 threads (whole waves) so small rounds spread over the CUs.
+        const unsigned block = (unsigned)std::min<size_t>(256, (g.threads + 63) / 64 * 64);
         const unsigned grid = (unsigned)((g.threads + block - 1) / block);
-        const size_t smem = 2 * kConstTableEntries * sizeof(uint4);
-#define OUTER_EQ_ARGS                                                                                     \
-    h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], g.threads, g.iters, rtab8, la, g.la_bits, la_shift, \
-        lb, lb_shift, nullptr, slot
+        const size_t smem = sizeof(uint4) * (2 * kConstTable4Entries + (size_t)(block / 64) * 64 * kLaneTableEntries);
+#define OUTER_FOLD1_ARGS                                                                                   \
+    h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], g.threads, g.iters, rtab4, la, g.la_bits, la_shift, \
+        lb, lb_shift, h4, slot
         if (q1)
-            hipLaunchKernelGGL((bitz::k_outer_eq<true, true>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
+            hipLaunchKernelGGL((bitz::k_outer_fold<true>), dim3(grid), dim3(block), smem, s, OUTER_FOLD1_ARGS);
         else
-            hipLaunchKernelGGL((bitz::k_outer_eq<true, false>), dim3(grid), dim3(block), smem, s, OUTER_EQ_ARGS);
-#undef OUTER_EQ_ARGS
+            hipLaunchKernelGGL((bitz::k_outer_fold<false>), dim3(grid), dim3(block), smem, s, OUTER_FOLD1_ARGS);
+#undef OUTER_FOLD1_ARGS
     }
     tm->end(s);
     return check_launch();
@@ -1134,7 +1128,7 @@ extern "C" int32_t bitz_outer_round0(bitz_outer* h, bitz_f128* co) {
     if (h->eqmode) {
         // q(1) is computed: the claim the caller holds is not trusted here.
         // The factor check is read with the first sums.
-        TRY(outer_eq_launch(h, 0, nullptr, nullptr, true, nullptr));
+        TRY(outer_eq_launch(h, 0, nullptr, true, nullptr));
         bitz_f128 raw[3];
         if (h->eq_valid) {
             TRY(h->read_slot(0, raw));
@@ -1171,14 +1165,10 @@ extern "C" int32_t bitz_outer_fold_next(bitz_outer* h, bitz_f128 rb, bitz_f128* 
         const int k = h->round;  // r is round k's challenge
         outer_eq_advance(h, k, r);
         if (k + 1 < h->n) {
-            uint4* rtab8 = static_cast<uint4*>(h->rtabs.p);
             uint4* rtab4 = static_cast<uint4*>(h->rtabs4.p);
-            if (h->geo[k + 1].iters > 1)
-                hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, rtab4);
-            else
-                hipLaunchKernelGGL(bitz::k_const_table1, dim3(1), dim3(1024), 0, s, r, rtab8);
+            hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, rtab4);
             const bool q1 = hostf::is_zero(h->tau[k + 1]);
-            TRY(outer_eq_launch(h, k + 1, rtab8, rtab4, q1, out));
+            TRY(outer_eq_launch(h, k + 1, rtab4, q1, out));
             for (int t = 0; t < 3; ++t) h->cur[t] = out[t];
             h->where = (h->where == 1) ? 2 : 1;
             h->len = next_len;
@@ -1388,13 +1378,7 @@ static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenge
     pa.htabs4 = static_cast<uint4*>(h->htabs4.p);
     pa.t4_blocks = (unsigned)n;
     pa.rtabs4 = static_cast<uint4*>(h->rtabs4.p);
-    // 8-bit tables only for grid-wide rounds that run single-iteration.
-    unsigned t8 = 0;
-    for (int k = 1; k < ps.kc; ++k)
-        if (h->geo[k].iters == 1) t8 = (unsigned)k;
-    pa.t8_blocks = t8;
-    pa.rtabs = static_cast<uint4*>(h->rtabs.p);
-    const unsigned grid = pa.eq_blocks + pa.h4_blocks + pa.t4_blocks + pa.t8_blocks;
+    const unsigned grid = pa.eq_blocks + pa.h4_blocks + pa.t4_blocks;
     hipLaunchKernelGGL(bitz::k_prologue, dim3(grid), dim3(1024), 0, s, pa);
     return check_launch();
 }
@@ -1407,7 +1391,6 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     hipStream_t s = h->so.s;
     const int n = h->n;
     const PipeSchedule ps = pipe_schedule(n);
-    uint4* rtabs = static_cast<uint4*>(h->rtabs.p);
     const uint4* rtabs4 = static_cast<const uint4*>(h->rtabs4.p);
     auto next_out = [&](f128* (&out)[3]) {
         Dev& dst = (h->where == 1) ? h->Q : h->P;
@@ -1424,9 +1407,8 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     for (int k = 0; k < ps.kc; ++k) {
         f128* out[3] = {nullptr, nullptr, nullptr};
         if (k > 0) next_out(out);
-        TRY(outer_eq_launch(h, k, k > 0 ? rtabs + (size_t)(k - 1) * kConstTableEntries : nullptr,
-                            k > 0 ? rtabs4 + (size_t)(k - 1) * kConstTable4Entries : nullptr, (q1mask >> k) & 1,
-                            k > 0 ? out : nullptr, &timers[k]));
+        TRY(outer_eq_launch(h, k, k > 0 ? rtabs4 + (size_t)(k - 1) * kConstTable4Entries : nullptr,
+                            (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k]));
         if (k > 0) advance(out, h->len / 2);
     }
     if (ps.kc < ps.k0) {

@@ -241,10 +241,8 @@ inline f128 horner(const f128* c, int n, f128 x) {
 
 #if defined(__HIPCC__)
 // ------------------------------------------------ multiply by a constant ---
-// Multiplying by a fixed element c is F2-linear: c * x = XOR over the 16 bytes
-// x_k of T_k[x_k], with T_k[v] = c * v * X^(8k) reduced.  The 16 x 256 table
-// (64 KiB) lives in LDS; every workgroup builds its own copy.
-constexpr int kConstTableEntries = 16 * 256;
+// Multiplying by a fixed element c is F2-linear: c * x = XOR over the
+// 4-bit windows x_w of T_w[x_w], with T_w[v] = c * v * X^(4w) reduced.
 
 // c * X^j (j < 128): shift into 256 bits, then reduce.
 __device__ __forceinline__ g4 g_mul_xpow(const g4& c, int j) {
@@ -265,56 +263,9 @@ __device__ __forceinline__ g4 g_mul_xpow(const g4& c, int j) {
     return g_reduce(t);
 }
 
-// Builds the 64 KiB table of c in `tab` (uint4[4096]); uses `basis`
-// (uint4[128]) as scratch.  All threads of the block must call it; it ends
-// with a barrier.
-__device__ __forceinline__ void build_const_table(uint4* tab, uint4* basis, const g4& c) {
-    for (int j = threadIdx.x; j < 128; j += blockDim.x) {
-        const g4 b = g_mul_xpow(c, j);
-        basis[j] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
-    }
-    __syncthreads();
-    // Entry (k, v): XOR of basis[8k + bit] over the set bits of v.  Each work
-    // item writes four entries v = 4q .. 4q + 3 of one window.
-    for (int item = threadIdx.x; item < 16 * 64; item += blockDim.x) {
-        const int k = item >> 6, q = item & 63;
-        uint4 base = make_uint4(0, 0, 0, 0);
-#pragma unroll
-        for (int bit = 2; bit < 8; ++bit)
-            if ((q >> (bit - 2)) & 1) {
-                const uint4 e = basis[8 * k + bit];
-                base.x ^= e.x; base.y ^= e.y; base.z ^= e.z; base.w ^= e.w;
-            }
-        const uint4 b0 = basis[8 * k], b1 = basis[8 * k + 1];
-        uint4* dst = tab + 256 * k + 4 * q;
-        dst[0] = base;
-        dst[1] = make_uint4(base.x ^ b0.x, base.y ^ b0.y, base.z ^ b0.z, base.w ^ b0.w);
-        dst[2] = make_uint4(base.x ^ b1.x, base.y ^ b1.y, base.z ^ b1.z, base.w ^ b1.w);
-        dst[3] = make_uint4(base.x ^ b0.x ^ b1.x, base.y ^ b0.y ^ b1.y, base.z ^ b0.z ^ b1.z,
-                            base.w ^ b0.w ^ b1.w);
-    }
-    __syncthreads();
-}
-
-// c * x through the LDS table of c.
-__device__ __forceinline__ g4 g_mul_table(const uint4* tab, const g4& x) {
-    g4 r = g_zero();
-#pragma unroll
-    for (int k = 0; k < 16; k += 2) {
-        const uint32_t b0 = __builtin_amdgcn_ubfe(x.w[k >> 2], 8 * (k & 3), 8);
-        const uint32_t b1 = __builtin_amdgcn_ubfe(x.w[(k + 1) >> 2], 8 * ((k + 1) & 3), 8);
-        const uint4 e0 = tab[256 * k + b0], e1 = tab[256 * (k + 1) + b1];
-        r.w[0] = xor3(r.w[0], e0.x, e1.x);
-        r.w[1] = xor3(r.w[1], e0.y, e1.y);
-        r.w[2] = xor3(r.w[2], e0.z, e1.z);
-        r.w[3] = xor3(r.w[3], e0.w, e1.w);
-    }
-    return r;
-}
-
-// 4-bit windows: 32 x 16 entries (8 KiB), entry (w, v) = c * v * X^(4w).
-// Half the lookups' reach of the 8-bit table, an eighth of its size: cheap
-// to copy into LDS once per round.
+// 4-bit windows: 32 x 16 entries (8 KiB), entry (w, v) = c * v * X^(4w):
+// cheap to copy into LDS once per round, and conflict-free (a window's 16
+// entries cover the 64 banks once).
 constexpr int kConstTable4Entries = 32 * 16;
 
 // Builds c's 4-bit table in `tab` (uint4[512]), `basis` (uint4[128]) as

@@ -460,33 +460,14 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
     }
 }
 
-// Constant tables are built once into global memory (one workgroup per
-// constant) and copied into LDS by every workgroup that multiplies by them.
-__global__ __launch_bounds__(1024) void k_const_tables(const f128* consts, uint4* out) {
-    __shared__ uint4 tab[kConstTableEntries];
-    __shared__ uint4 basis[128];
-    build_const_table(tab, basis, g_load(consts + blockIdx.x));
-    uint4* dst = out + (size_t)blockIdx.x * kConstTableEntries;
-    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
-}
-
-__global__ __launch_bounds__(1024) void k_const_table1(f128 c, uint4* out) {
-    __shared__ uint4 tab[kConstTableEntries];
-    __shared__ uint4 basis[128];
-    build_const_table(tab, basis, g_from(c));
-    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) out[e] = tab[e];
-}
-
+// 4-bit constant tables, one workgroup per constant, into global memory;
+// workgroups that multiply by a constant copy its table into LDS.
 __global__ __launch_bounds__(512) void k_const_tables4(const f128* consts, uint4* out) {
     __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
     build_const_table4(tab, basis, g_load(consts + blockIdx.x));
     uint4* dst = out + (size_t)blockIdx.x * kConstTable4Entries;
     for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
-}
-
-__device__ __forceinline__ void copy_table(uint4* lds, const uint4* __restrict__ src) {
-    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) lds[e] = src[e];
 }
 
 // ============================================ outer sumcheck, eq-factored ===
@@ -583,10 +564,9 @@ __global__ __launch_bounds__(1024) void k_const_table4_1(f128 c, uint4* out) {
 
 // Everything the pipelined prover needs before round 0, in one launch of
 // 1024-thread workgroups: clearing the round-sum slots, the factor check,
-// the H(i) tables, the 4-bit tables of every challenge and the 8-bit tables
-// of the challenges the single-iteration grid-wide rounds fold with.
+// the H(i) tables and the 4-bit tables of every challenge.
 __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
-    __shared__ uint4 tab[kConstTableEntries];
+    __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
     unsigned b = blockIdx.x;
     if (b == 0)
@@ -601,111 +581,17 @@ __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
         return;
     }
     b -= A.h4_blocks;
-    if (b < A.t4_blocks) {
-        build_const_table4(tab, basis, g_from(A.chal[b]));
-        uint4* dst = A.rtabs4 + (size_t)b * kConstTable4Entries;
-        for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
-        return;
-    }
-    b -= A.t4_blocks;
-    build_const_table(tab, basis, g_from(A.chal[b]));
-    uint4* dst = A.rtabs + (size_t)b * kConstTableEntries;
-    for (int e = threadIdx.x; e < kConstTableEntries; e += blockDim.x) dst[e] = tab[e];
-}
-
-namespace {
-
-// h_tab first: a kernel that does not fold needs only the first 64 KiB.
-struct Smem {
-    uint4 h_tab[kConstTableEntries];
-    uint4 r_tab[kConstTableEntries];
-};
-
-template <bool FOLD>
-__device__ __forceinline__ void load_pair(const f128* a, const f128* b, const f128* c, f128* ao,
-                                          f128* bo, f128* co, size_t y, const uint4* r_tab,
-                                          g4& a0, g4& a1, g4& b0, g4& b1, g4& c0, g4& c1) {
-    if (!FOLD) {
-        a0 = g_load(a + 2 * y); a1 = g_load(a + 2 * y + 1);
-        b0 = g_load(b + 2 * y); b1 = g_load(b + 2 * y + 1);
-        c0 = g_load(c + 2 * y); c1 = g_load(c + 2 * y + 1);
-        return;
-    }
-    g4 in[3][4];
-    const f128* src[3] = {a, b, c};
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) in[t][j] = g_load(src[t] + 4 * y + j);
-    g4 out[3][2];
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const g4 d = g_add(in[t][2 * j + 1], in[t][2 * j]);
-            out[t][j] = g_add(in[t][2 * j], g_mul_table(r_tab, d));
-        }
-    g_store(ao + 2 * y, out[0][0]); g_store(ao + 2 * y + 1, out[0][1]);
-    g_store(bo + 2 * y, out[1][0]); g_store(bo + 2 * y + 1, out[1][1]);
-    g_store(co + 2 * y, out[2][0]); g_store(co + 2 * y + 1, out[2][1]);
-    a0 = out[0][0]; a1 = out[0][1];
-    b0 = out[1][0]; b1 = out[1][1];
-    c0 = out[2][0]; c1 = out[2][1];
-}
-
-}  // namespace
-
-template <bool FOLD, bool Q1>
-__global__ __launch_bounds__(1024) void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao,
-                                                   f128* bo, f128* co, size_t threads, size_t iters,
-                                                   const uint4* r_tab, const f128* la, int la_bits, int la_shift,
-                                                   const f128* lb, int lb_shift, const uint4* h_tabs,
-                                                   uint32_t* acc) {
-    extern __shared__ uint4 smem_raw[];
-    Smem& sm = *reinterpret_cast<Smem*>(smem_raw);
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (FOLD) copy_table(sm.r_tab, r_tab);
-    g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
-    for (size_t i = 0; i < iters; ++i) {
-        if (iters > 1) {
-            __syncthreads();  // the previous iteration's readers are done with h_tab
-            copy_table(sm.h_tab, h_tabs + i * kConstTableEntries);
-        }
-        __syncthreads();
-        if (t < threads) {
-            const size_t y = i * threads + t;
-            g4 a0, a1, b0, b1, c0, c1;
-            load_pair<FOLD>(a, b, c, ao, bo, co, y, sm.r_tab, a0, a1, b0, b1, c0, c1);
-            const g4 p0 = g_add(g_mul(a0, b0), c0);
-            const g4 pi = g_mul(g_add(a0, a1), g_add(b0, b1));
-            g4 p1;
-            if (Q1) p1 = g_add(g_mul(a1, b1), c1);
-            if (iters > 1) {
-                s0 = g_add(s0, g_mul_table(sm.h_tab, p0));
-                si = g_add(si, g_mul_table(sm.h_tab, pi));
-                if (Q1) s1 = g_add(s1, g_mul_table(sm.h_tab, p1));
-            } else {  // no iteration variables: H = 1
-                s0 = p0;
-                si = pi;
-                if (Q1) s1 = p1;
-            }
-        }
-    }
-    g4 v[3] = {g_zero(), g_zero(), g_zero()};
-    if (t < threads) {
-        const size_t mask = ((size_t)1 << la_bits) - 1;
-        const g4 L = g_mul(g_load(la + ((t & mask) << la_shift)), g_load(lb + ((t >> la_bits) << lb_shift)));
-        v[0] = g_mul(L, s0);
-        v[1] = g_mul(L, si);
-        if (Q1) v[2] = g_mul(L, s1);
-    }
-    block_xor_to_global<3>(v, acc);
+    build_const_table4(tab, basis, g_from(A.chal[b]));
+    uint4* dst = A.rtabs4 + (size_t)b * kConstTable4Entries;
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) dst[e] = tab[e];
 }
 
 // Round 0 of the eq-factored outer sumcheck with the general products
 // through per-lane LDS tables (g_mul_lane): 256-thread workgroups, each
 // wave owning 16 KiB of lane tables, H(i) through a 4-bit constant table
-// (8 KiB) copied per iteration.  Pair y = i * threads + t as in k_outer_eq.
+// (8 KiB) copied per iteration.  Thread t handles pairs y = i * threads + t
+// for iterations i (the iteration index i is the high part of y, so H(i) is
+// one constant per iteration and the thread's low part t fixes L(t)).
 template <bool Q1>
 __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, const f128* c, size_t threads,
                                               size_t iters, const f128* la, int la_bits, int la_shift,
@@ -856,13 +742,6 @@ __global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b
 template __global__ void k_outer_fold<false>(OUTER_FOLD_SIG);
 template __global__ void k_outer_fold<true>(OUTER_FOLD_SIG);
 #undef OUTER_FOLD_SIG
-
-#define OUTER_EQ_SIG                                                                                       \
-    const f128*, const f128*, const f128*, f128*, f128*, f128*, size_t, size_t, const uint4*, const f128*, int, \
-        int, const f128*, int, const uint4*, uint32_t*
-template __global__ void k_outer_eq<true, false>(OUTER_EQ_SIG);
-template __global__ void k_outer_eq<true, true>(OUTER_EQ_SIG);
-#undef OUTER_EQ_SIG
 
 // ============================== rounds held in LDS: chunk, tail, batch ===
 // A stage (LdsStage) runs rounds k_first .. k_first+s-1 of one eq-factored

@@ -57,12 +57,6 @@ __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* ro
                                      const f128* row_w, int log_rows, const f128* col_w,
                                      f128* out, size_t cols);
 
-template <bool FOLD, bool Q1>
-__global__ void k_outer_eq(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
-                           size_t threads, size_t iters, const uint4* r_tab, const f128* la, int la_bits,
-                           int la_shift, const f128* lb, int lb_shift, const uint4* h_tabs, uint32_t* acc);
-__global__ void k_const_tables(const f128* consts, uint4* out);
-__global__ void k_const_table1(f128 c, uint4* out);
 __global__ void k_eq_check(const f128* L, int ell, const f128* H, int m, uint32_t* out, unsigned blocks_l);
 
 // H(i) tables of the multi-iteration rounds (see k_htabs4).
@@ -89,8 +83,6 @@ struct PrologueArgs {
     f128 chal[kMaxRounds];  // the challenges, by value
     unsigned t4_blocks;  // 4-bit tables of chal[0 ..)
     uint4* rtabs4;
-    unsigned t8_blocks;  // 8-bit tables of chal[0 ..)
-    uint4* rtabs;
 };
 __global__ void k_prologue(PrologueArgs args);
 __global__ void k_htabs4(const f128* L, int ell, const f128* H, HtabDesc d, uint4* out);
@@ -149,7 +141,6 @@ constexpr size_t stage_smem(int cap_lg) { return sizeof(uint4) * (((size_t)4 << 
 __global__ void k_outer_chunk(LdsStage args);
 __global__ void k_outer_tail(LdsStage args);
 __global__ void k_outer_batch(const LdsStage* stages, const uint32_t* bmap);
-constexpr size_t kOuterEqSmem = sizeof(uint4) * (2 * kConstTableEntries);
 
 // One proof's grid-wide round in a batched launch: the arguments of
 // k_outer_r0 / k_outer_fold, the first workgroup of the proof in the
