@@ -139,6 +139,7 @@ def lib():
             "bitz_mt_transpose_inner": (I32, [P, P, SZ, P, SZ, P, U32]),
             "bitz_mt_free": (None, [P]),
             "bitz_gkr_build": (I32, [C.POINTER(C.c_void_p), P, U32, U32, P, U32]),
+            "bitz_gkr_build_bittable": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32]),
             "bitz_gkr_prove": (I32, [P, P, P, U32, P, P]),
             "bitz_gkr_layer": (I32, [P, U32, P, SZ, C.POINTER(SZ)]),
             "bitz_gkr_free": (None, [P]),
@@ -895,6 +896,24 @@ def grand_product(leaves, log_groups: int):
     h = C.c_void_p()
     _check(lib().bitz_gkr_build(C.byref(h), _p(lv), log_leaves, log_groups, _p(out), flags))
     return out, LayerWitnesses(h, log_leaves, log_groups)
+
+
+def grand_product_bittable(row_images, packed, log_rows: int, log_columns: int):
+    """grand_product over a BitTable (crates/gkr/src/lib.rs:785-802): tree c
+    multiplies row_images[b] over the rows b whose bit (column c, row b) is set,
+    bit (c, b) being bit (c << log_rows) | b of `packed` (u64 words).  Returns
+    (output, LayerWitnesses) like grand_product."""
+    init()
+    images = elems(row_images)
+    words = np.ascontiguousarray(packed, dtype=np.uint64)
+    if len(images) != 1 << log_rows:
+        raise BitzError(-7, "one image per row of the table")
+    if len(words) * 64 != 1 << (log_rows + log_columns):
+        raise BitzError(-7, "the table has 2^(log_rows + log_columns) bits")
+    out = np.zeros((1 << log_columns, 2), dtype=np.uint64)
+    h = C.c_void_p()
+    _check(lib().bitz_gkr_build_bittable(C.byref(h), _p(images), log_rows, _p(words), log_columns, _p(out), 0))
+    return out, LayerWitnesses(h, log_rows + log_columns, log_columns)
 
 
 def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):

@@ -2902,10 +2902,12 @@ struct bitz_gkr {
 
 constexpr size_t kSmallTreeSmem = sizeof(uint4) * ((size_t)1 << bitz::kGkrSmallTreeLog) + bitz::kLaneSmem256;
 
-extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint32_t log_leaves,
-                                  uint32_t log_groups, bitz_f128* output, uint32_t flags) {
-    if (!out || !leaves || !output) return fail(BITZ_ERR_ARG, "null pointer");
-    if (log_groups > log_leaves || log_leaves > 32) return fail(BITZ_ERR_ARG, "bad tree shape");
+// The product tree over layer 0.  `fill` writes the leaves into the store;
+// with `src` a device table instead, the first layer reads it and copies it
+// into the store itself (the leaves are read once).
+template <class Fill>
+static int32_t gkr_build(bitz_gkr** out, uint32_t log_leaves, uint32_t log_groups, const f128* src, Fill fill,
+                         bitz_f128* output, uint32_t flags) {
     auto* h = new bitz_gkr;
     auto cleanup = [&](int32_t c) {
         delete h;
@@ -2931,22 +2933,18 @@ extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint3
         attr = true;
     }
     // Layers are contiguous: layer j+1 starts where layer j ends. Large layers
-    // take one grid-wide launch each (the first also copies device leaves into
-    // the store); from 2^kGkrSmallTreeLog elements down one workgroup builds the rest.
+    // take one grid-wide launch each; from 2^kGkrSmallTreeLog elements down
+    // one workgroup builds the rest.
     const int depth = (int)(log_leaves - log_groups);
     const int j_small = std::max(0, (int)log_leaves - bitz::kGkrSmallTreeLog);
-    const bool fused_copy = (flags & BITZ_DEVICE) && depth > 0 && j_small > 0;
-    if (!fused_copy && hipMemcpyAsync(h->store.p, leaves, ((size_t)1 << log_leaves) * sizeof(f128),
-                                      (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                      s) != hipSuccess)
-        return cleanup(fail(BITZ_ERR_DEVICE, "leaf upload failed"));
+    const bool fused_copy = src && depth > 0 && j_small > 0;
+    if (!fused_copy && (rc = fill(h, s))) return cleanup(rc);
     for (int j = 0; j < std::min(depth, j_small); ++j) {
         const size_t half = (size_t)1 << (log_leaves - j - 1);
         const bool copy = fused_copy && j == 0;
         hipLaunchKernelGGL(bitz::k_gkr_layer_lane, dim3(grid_for(half)), dim3(bitz::kStageBlock),
-                           bitz::kLaneSmem256, s,
-                           copy ? reinterpret_cast<const f128*>(leaves) : h->store.f() + h->off[j],
-                           copy ? h->store.f() : nullptr, h->store.f() + h->off[j + 1], half);
+                           bitz::kLaneSmem256, s, copy ? src : h->store.f() + h->off[j], copy ? h->store.f() : nullptr,
+                           h->store.f() + h->off[j + 1], half);
     }
     if (depth > j_small)
         hipLaunchKernelGGL(bitz::k_gkr_layers_small, dim3(1), dim3(bitz::kStageBlock), kSmallTreeSmem, s,
@@ -2957,6 +2955,51 @@ extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint3
     if ((rc = h->host.alloc(64))) return cleanup(rc);
     *out = h;
     return BITZ_OK;
+}
+
+extern "C" int32_t bitz_gkr_build(bitz_gkr** out, const bitz_f128* leaves, uint32_t log_leaves,
+                                  uint32_t log_groups, bitz_f128* output, uint32_t flags) {
+    if (!out || !leaves || !output) return fail(BITZ_ERR_ARG, "null pointer");
+    if (log_groups > log_leaves || log_leaves > 32) return fail(BITZ_ERR_ARG, "bad tree shape");
+    const bool dev = (flags & BITZ_DEVICE) != 0;
+    auto fill = [&](bitz_gkr* h, hipStream_t s) -> int32_t {
+        if (hipMemcpyAsync(h->store.p, leaves, ((size_t)1 << log_leaves) * sizeof(f128),
+                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(BITZ_ERR_DEVICE, "leaf upload failed");
+        return BITZ_OK;
+    };
+    return gkr_build(out, log_leaves, log_groups, dev ? reinterpret_cast<const f128*>(leaves) : nullptr, fill,
+                     output, flags);
+}
+
+extern "C" int32_t bitz_gkr_build_bittable(bitz_gkr** out, const bitz_f128* images, uint32_t log_rows,
+                                           const uint64_t* packed, uint32_t log_columns, bitz_f128* output,
+                                           uint32_t flags) {
+    if (!out || !images || !packed || !output) return fail(BITZ_ERR_ARG, "null pointer");
+    // LutLayers::new (crates/gkr/src/lib.rs:274-302) needs a word of nibbles per lookup table.
+    if (log_columns < 4) return fail(BITZ_ERR_ARG, "the table needs at least 16 trees (s >= 4)");
+    if (log_rows + log_columns > 32) return fail(BITZ_ERR_ARG, "bad table shape");
+    const bool dev = (flags & BITZ_DEVICE) != 0;
+    const size_t rows = (size_t)1 << log_rows, words = std::max<size_t>(((size_t)1 << (log_rows + log_columns)) / 64, 1);
+    Dev dimg, dbits;
+    auto fill = [&](bitz_gkr* h, hipStream_t s) -> int32_t {
+        const f128* img = reinterpret_cast<const f128*>(images);
+        const uint64_t* bits = packed;
+        if (!dev) {
+            TRY(upload(dimg, images, rows, 0, s));
+            TRY(dbits.alloc(words * 8));
+            HIP_TRY(hipMemcpyAsync(dbits.p, packed, words * 8, hipMemcpyHostToDevice, s));
+            img = dimg.f();
+            bits = static_cast<const uint64_t*>(dbits.p);
+        }
+        hipLaunchKernelGGL(bitz::k_bittable_leaves, dim3(grid_for(rows << log_columns)), dim3(kBlock), 0, s, bits,
+                           img, (int)log_rows, (int)log_columns, h->store.f());
+        return check_launch();
+    };
+    const int32_t rc = gkr_build(out, log_rows + log_columns, log_columns, nullptr, fill, output, flags);
+    // the staged table and images must outlive the expansion kernel
+    if (rc == BITZ_OK && !dev) HIP_TRY(hipStreamSynchronize((*out)->so.s));
+    return rc;
 }
 
 extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_t cap, size_t* len) {

@@ -1230,6 +1230,22 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_eq_combine_batch(const GkrC
                     g_mul_lane(g_load_hbm(A.low + (j & mask)), g_load_hbm(A.high + (j >> A.low_bits)), col));
 }
 
+// Dense leaves of a BitTable (grand_product, crates/gkr/src/lib.rs:785-802;
+// the reference's own dense construction, lib.rs:1023-1037): leaf
+// (b << s) | c is images[b] where bit (c << log_rows) | b of the packed table
+// is set, one elsewhere.
+__global__ __launch_bounds__(kBlock) void k_bittable_leaves(const uint64_t* packed, const f128* images, int log_rows,
+                                                            int log_columns, f128* leaves) {
+    const size_t n = (size_t)1 << (log_rows + log_columns);
+    const size_t cmask = ((size_t)1 << log_columns) - 1;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = i >> log_columns, c = i & cmask;
+        const size_t bit = (c << log_rows) | b;
+        const bool set = (packed[bit >> 6] >> (bit & 63)) & 1;
+        g_store_hbm(leaves + i, set ? g_load_hbm(images + b) : g_one());
+    }
+}
+
 // next[k] = cur[k] * cur[k + half] (fold_layers, crates/gkr/src/lib.rs:874-894).
 __global__ __launch_bounds__(kStageBlock) void k_gkr_layer_lane(const f128* cur, f128* cur_copy, f128* next,
                                                                 size_t half) {

@@ -223,6 +223,8 @@ constexpr size_t kLaneSmem256 = sizeof(uint4) * 4 * 64 * kLaneTableEntries;
 // layer, then every layer from 2^kGkrSmallTreeLog elements down in one
 // workgroup (LDS), each layer written to its place in the store.
 constexpr int kGkrSmallTreeLog = 12;
+__global__ void k_bittable_leaves(const uint64_t* packed, const f128* images, int log_rows, int log_columns,
+                                  f128* leaves);
 // cur_copy, when not null, receives a copy of cur (the leaf layer read once).
 __global__ void k_gkr_layer_lane(const f128* cur, f128* cur_copy, f128* next, size_t half);
 // layer holds 2^log_len elements; the next `count` layers follow it in memory.

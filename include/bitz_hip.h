@@ -352,6 +352,22 @@ typedef struct bitz_gkr bitz_gkr;
  * (tree c = leaves c, c + 2^log_groups, ...). */
 int32_t bitz_gkr_build(bitz_gkr** h, const bitz_f128* leaves, uint32_t log_leaves,
                        uint32_t log_groups, bitz_f128* output, uint32_t flags);
+/* grand_product over a BitTable (crates/gkr/src/lib.rs:785-802): 2^log_columns
+ * trees of 2^log_rows leaves; leaf (b << s) | c (s = log_columns) is images[b]
+ * where the table's bit (column c, row b) is set and one elsewhere, bit (c, b)
+ * being bit (c << log_rows) | b of `packed` (BitTable::bit, crates/common/src/
+ * table.rs; u64 words, least significant bit first).  The reference keeps
+ * layers 0-2 as 16-entry lookup tables (LutLayers, lib.rs:274-302) and proves
+ * their first rounds from them (prove_lut_layer, lib.rs:397-602), with the
+ * same messages as the dense tree over these leaves (its tests build the
+ * leaves so, lib.rs:1023-1037, 1173-1226); here the device expands the table
+ * into the dense leaves and the dense tree follows, so bitz_gkr_prove and
+ * bitz_gkr_prove_with_challenges take the handle as they take
+ * bitz_gkr_build's.  log_columns < 4 -> BITZ_ERR_ARG (the reference panics).
+ * images: 2^log_rows; packed: 2^(log_rows + log_columns) bits; output: the
+ * 2^log_columns tree products.  BITZ_DEVICE: all three on the device. */
+int32_t bitz_gkr_build_bittable(bitz_gkr** h, const bitz_f128* images, uint32_t log_rows, const uint64_t* packed,
+                                uint32_t log_columns, bitz_f128* output, uint32_t flags);
 /* gpgkr_prove (lib.rs:24-79) over the dense layers: point has log_groups
  * coordinates; leaf_point receives log_leaves coordinates and *leaf_claim
  * the claim on the leaves.  Consumes the layers (binds them in place). */

@@ -306,6 +306,56 @@ def test_gkr_with_challenges_matches_oracle(log_leaves, log_groups, zero):
     assert gc == O.mle_evaluate(leaves, gp)
 
 
+def bittable_instance(t, s, bits, seed):
+    """The reference's test instance (crates/gkr/src/lib.rs:1002-1014): packed
+    words (u128, as lo/hi u64 pairs) then the row images, from splitmix(seed)."""
+    n_words = (1 << (t + s)) // 128
+    els = elements(seed, n_words + (1 << t))
+    packed = els[:n_words].copy()
+    if bits == "clear":
+        packed[:] = 0
+    elif bits == "set":
+        packed[:] = np.uint64(2**64 - 1)
+    return packed.reshape(-1), els[n_words:]
+
+
+def bittable_leaves(packed, images, t, s):
+    """leaves() of the reference's tests (lib.rs:1023-1037): leaf (b << s) | c
+    is images[b] where bit (c, b) is set, one elsewhere."""
+    bits = np.unpackbits(packed.view(np.uint8), bitorder="little")
+    leaves = np.zeros((1 << (t + s), 2), np.uint64)
+    leaves[:, 0] = 1
+    for b in range(1 << t):
+        cols = np.arange(1 << s)
+        on = bits[(cols << t) | b].astype(bool)
+        leaves[(b << s) | cols[on]] = images[b]
+    return leaves
+
+
+@pytest.mark.parametrize("t,s,bits", [(7, 4, "drawn"), (9, 6, "drawn"), (8, 5, "clear"), (7, 5, "set"),
+                                      (11, 6, "drawn")])
+def test_gkr_bittable_matches_the_dense_tree(t, s, bits):
+    packed, images = bittable_instance(t, s, bits, 4 + t)
+    leaves = bittable_leaves(packed, images, t, s)
+    store, layers = O.gkr_fold_layers(leaves, s)
+    out, wit = G.grand_product_bittable(images, packed, t, s)
+    assert np.array_equal(out, layers[-1])
+    for j in (0, 1, 2, 3, len(layers) - 1):
+        assert np.array_equal(wit.layer(j), layers[j])
+    point = elements(60 + t, s)
+    ot = O.DuplexTranscript(b"gkr/bittable")
+    ref_point, ref_claim = O.gkr_prove(ot, point, store, t + s)
+    gt = G.Transcript.duplex(b"gkr/bittable")
+    gp, gc = G.gpgkr_prove(gt, point, wit)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim and gt.narg() == ot.narg()
+
+
+def test_gkr_bittable_needs_sixteen_trees():
+    packed, images = bittable_instance(7, 3, "drawn", 1)
+    with pytest.raises(G.BitzError):
+        G.grand_product_bittable(images, packed, 7, 3)
+
+
 def test_gkr_rejects_a_wrong_point_length():
     leaves = elements_fast(1, 1 << 6)
     _, wit = G.grand_product(leaves, 2)
