@@ -143,6 +143,12 @@ def lib():
             "bitz_gkr_prove": (I32, [P, P, P, U32, P, P]),
             "bitz_gkr_layer": (I32, [P, U32, P, SZ, C.POINTER(SZ)]),
             "bitz_gkr_free": (None, [P]),
+            "bitz_piop_engine_create": (I32, [C.POINTER(C.c_void_p)]),
+            "bitz_piop_engine_submit": (I32, [P, C.c_uint64, P, P, P, P, P, P, U32, P, U32]),
+            "bitz_piop_engine_step": (I32, [P, C.POINTER(SZ)]),
+            "bitz_piop_engine_done": (I32, [P, P, SZ, C.POINTER(SZ)]),
+            "bitz_piop_engine_take": (I32, [P, C.c_uint64, P, P, P, P, P, P]),
+            "bitz_piop_engine_free": (None, [P]),
             "bitz_sha256_circuit": (I32, [C.POINTER(C.c_void_p), U32, SZ, P]),
             "bitz_sha256_shape": (I32, [P, C.POINTER(SZ), C.POINTER(SZ), C.POINTER(SZ), P, C.POINTER(SZ)]),
             "bitz_sha256_export": (I32, [P, P, P, P, P, P, P]),
@@ -831,6 +837,64 @@ def prove_spartan_piop(transcript: Transcript, matrices: "PreparedConstraintMatr
                                          C.byref(final)), SumcheckError)
     return SpartanPiopOutput(outer[: 4 * rv].reshape(rv, 4, 2), [to_int(c) for c in claims],
                              inner[: 3 * cv].reshape(cv, 3, 2), point[:cv], to_int(meval), to_int(final))
+
+
+class PiopEngine:
+    """Many prove_spartan_piop proofs with their rounds batched
+    (bitz_piop_engine_*): submit proofs (device products and assignment),
+    step until they finish, take each one's SpartanPiopOutput.  The inputs of
+    a proof are kept alive here until it is taken."""
+
+    def __init__(self):
+        init()
+        h = C.c_void_p()
+        _check(lib().bitz_piop_engine_create(C.byref(h)))
+        self.h = h
+        self._held: dict[int, tuple] = {}
+
+    def submit(self, tag: int, transcript: Transcript, matrices: "PreparedConstraintMatrices", digest: bytes,
+               products: R1csProductMles, assignment: DeviceTable) -> None:
+        (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
+        hv, nh, fh = _table(assignment)
+        if not (fa & DEVICE and fh & DEVICE):
+            raise BitzError(-7, "the engine takes device tables")
+        dg = (C.c_uint8 * 32).from_buffer_copy(digest)
+        _check(lib().bitz_piop_engine_submit(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz),
+                                             _nvars(na), _p(hv), _nvars(nh)), SumcheckError)
+        self._held[tag] = (transcript, matrices, products, assignment, _nvars(na), _nvars(nh))
+
+    def step(self) -> int:
+        n = C.c_size_t()
+        _check(lib().bitz_piop_engine_step(self.h, C.byref(n)))
+        return n.value
+
+    def done(self) -> list[int]:
+        n = C.c_size_t()
+        _check(lib().bitz_piop_engine_done(self.h, None, 0, C.byref(n)))
+        tags = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(lib().bitz_piop_engine_done(self.h, _p(tags), len(tags), C.byref(n)))
+        return [int(t) for t in tags[: n.value]]
+
+    def take(self, tag: int) -> SpartanPiopOutput:
+        rv, cv = self._held[tag][4], self._held[tag][5]
+        outer = np.zeros((rv * 4, 2), dtype=np.uint64)
+        claims = np.zeros((3, 2), dtype=np.uint64)
+        inner = np.zeros((cv * 3, 2), dtype=np.uint64)
+        point = np.zeros((cv, 2), dtype=np.uint64)
+        meval, final = F128(), F128()
+        rc = lib().bitz_piop_engine_take(self.h, tag, _p(outer), _p(claims), _p(inner), _p(point), C.byref(meval),
+                                         C.byref(final))
+        if rc != -8:  # not still in flight: the engine has released the proof and its inputs
+            self._held.pop(tag, None)
+        _check(rc, SumcheckError)
+        return SpartanPiopOutput(outer.reshape(rv, 4, 2), [to_int(c) for c in claims], inner.reshape(cv, 3, 2),
+                                 point, to_int(meval), to_int(final))
+
+    def __del__(self):
+        try:
+            lib().bitz_piop_engine_free(self.h)
+        except Exception:
+            pass
 
 
 class MaterializedMTranspose:

@@ -840,6 +840,8 @@ static void stage_constants(const bitz_outer* h, int k_first, int s, int pl0, f1
 // carry.  For a factor T with variables V and a run R of them, T[x << s]
 // = c_R eq(tau_R, x) with c_R = prod_{i in V \ R} (1 + tau_i), since
 // T[0] = prod_V (1 + tau_i) and T[2^i] / T[0] = tau_i / (1 + tau_i).
+static bool outer_eq_constants(bitz_outer* h, const std::vector<f128>& tau);
+
 static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
     const int n = h->n, ell = h->ell;
     const uint32_t* probes = region;
@@ -864,10 +866,20 @@ static bool outer_validate_eq(bitz_outer* h, const uint32_t* region) {
         }
     }
     batch_inverse(den);
-    std::vector<f128> tau(n), one_tau(n);
+    std::vector<f128> tau(n);
+    for (int i = 0; i < n; ++i) tau[i] = hostf::mul(num[i], den[i]);
+    return outer_eq_constants(h, tau);
+}
+
+// The eq-factored prover's constants from the point tau: the run constants of
+// every round's weights, their inverses (the round scales) and 1 / tau_k.
+// False when some 1 + tau_i is zero (the factors then vanish somewhere).
+static bool outer_eq_constants(bitz_outer* h, const std::vector<f128>& tau) {
+    const int n = h->n, ell = h->ell;
+    std::vector<f128> one_tau(n);
     for (int i = 0; i < n; ++i) {
-        tau[i] = hostf::mul(num[i], den[i]);
         one_tau[i] = hostf::add(hostf::one(), tau[i]);
+        if (hostf::is_zero(one_tau[i])) return false;
     }
     // Per factor t with variables [lo, hi): pre[t][j] = prod_{lo <= i < lo + j}
     // (1 + tau_i), suf[t][j] = prod_{lo + j <= i < hi}.  For a run [a, b)
@@ -2576,27 +2588,14 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
     return BITZ_OK;
 }
 
-extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
-                                       bitz_f128 rho, bitz_f128* out, uint32_t flags) {
-    if (!h || !out || (point_len && !row_point)) return fail(BITZ_ERR_ARG, "null pointer");
-    // matrix.rs:435-441: the row point must have num_row_vars coordinates.
-    if (point_len != (uint32_t)h->row_vars) return fail(BITZ_ERR_ROW_POINT, "row point length");
-    std::lock_guard<std::mutex> lock(h->mu);
-    hipStream_t s = h->so.s;
-    if (point_len)
-        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, point_len * sizeof(f128),
-                               (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+// Enqueues D = bind_and_batch(r_x, rho) into target (2^col_vars, device) on
+// stream s, the row point already in h->point.  Uses the handle's scratch:
+// calls on one handle must run one after another.
+static int32_t bind_enqueue(bitz_matrices* h, f128 rho, f128* target, hipStream_t s) {
     TRY(eq_build(h->point.f(), h->row_vars, h->eq_rows.f(), h->scratch, s));
-    const f128 r = F(rho);
     // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value
-    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(), r,
-                       hostf::mul(r, r), h->row_scale.f(), h->rows_padded);
-    Dev dout;
-    f128* target = reinterpret_cast<f128*>(out);
-    if (!(flags & BITZ_DEVICE)) {
-        TRY(dout.alloc(h->cols * sizeof(f128)));
-        target = dout.f();
-    }
+    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(), rho,
+                       hostf::mul(rho, rho), h->row_scale.f(), h->rows_padded);
     // every nonzero's product (a converged pass), then each column's XOR over its run
     static bool attr = false;
     if (!attr) {
@@ -2617,7 +2616,26 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
     if (h->n_long_chunks)  // the long columns, left zero above: XOR of each run's sum
         hipLaunchKernelGGL(bitz::k_bind_long, dim3((unsigned)h->n_long_chunks), dim3(kBlock), 0, s,
                            h->long_chunks.u32(), h->prod.f(), target);
-    TRY(check_launch());
+    return check_launch();
+}
+
+extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
+                                       bitz_f128 rho, bitz_f128* out, uint32_t flags) {
+    if (!h || !out || (point_len && !row_point)) return fail(BITZ_ERR_ARG, "null pointer");
+    // matrix.rs:435-441: the row point must have num_row_vars coordinates.
+    if (point_len != (uint32_t)h->row_vars) return fail(BITZ_ERR_ROW_POINT, "row point length");
+    std::lock_guard<std::mutex> lock(h->mu);
+    hipStream_t s = h->so.s;
+    if (point_len)
+        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, point_len * sizeof(f128),
+                               (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    Dev dout;
+    f128* target = reinterpret_cast<f128*>(out);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(dout.alloc(h->cols * sizeof(f128)));
+        target = dout.f();
+    }
+    TRY(bind_enqueue(h, F(rho), target, s));
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -2802,6 +2820,467 @@ extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h,
 }
 
 extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
+
+// ============================================================ PIOP engine ==
+// Many prove_spartan_piop proofs advanced together, one round each per step.
+// Every proof keeps its own transcript, tables and state; a step gathers the
+// round each proof is at, launches every round of a kind in one kernel
+// (outer round 0, outer fold rounds, inner round 0, inner fold rounds; the
+// final folds and bind_and_batch per proof), reads all sums back at once,
+// then each proof's transcript absorbs its round polynomial and squeezes its
+// next challenge on the host.  A proof submitted between steps starts at the
+// next step.  The transcript order is bitz_prove_spartan_piop's.
+namespace {
+
+enum class JobPhase { kOuter, kOuterFinal, kBind, kInner, kInnerFinal, kDone };
+
+struct PiopJob {
+    uint64_t tag = 0;
+    bitz_transcript* t = nullptr;
+    bitz_matrices* m = nullptr;
+    const f128* hv = nullptr;  // the assignment (device)
+    int n = 0, nc = 0;
+    JobPhase phase = JobPhase::kOuter;
+    int round = 0;  // next round to launch in the current sumcheck
+    bool q1 = true;
+    f128 claim{0, 0}, r{0, 0}, rho{0, 0};
+    std::vector<f128> lo, hi;
+    Dev dlo, dhi, dD, iP, iQ;
+    bitz_outer* outer = nullptr;
+    const f128* icur[2] = {nullptr, nullptr};
+    int iwhere = 0;
+    size_t ilen = 0, icapP = 0, icapQ = 0;
+    f128* pend[3] = {nullptr, nullptr, nullptr};  // outputs of the fold launched this step
+    std::vector<bitz_f128> outer_rounds, rx, inner_rounds, inner_point;
+    bitz_f128 outer_claims[3] = {}, matrix_eval{}, final_claim{};
+    int32_t status = BITZ_OK;
+    ~PiopJob() { bitz_outer_free(outer); }
+};
+
+}  // namespace
+
+struct bitz_piop_engine {
+    std::vector<std::unique_ptr<PiopJob>> jobs;
+    Dev slots, stage, rt4;
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    StreamOwner so;
+    ~bitz_piop_engine() {
+        (void)hipStreamSynchronize(so.s);
+        if (pin) (void)hipHostFree(pin);
+    }
+};
+
+extern "C" int32_t bitz_piop_engine_create(bitz_piop_engine** out) {
+    if (!out) return fail(BITZ_ERR_ARG, "null pointer");
+    auto* e = new bitz_piop_engine;
+    if (int32_t rc = e->so.create()) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return BITZ_OK;
+}
+
+extern "C" void bitz_piop_engine_free(bitz_piop_engine* e) { delete e; }
+
+extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                                           const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
+                                           const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
+                                           uint32_t col_vars) {
+    if (!e || !t || !m || !digest || !az || !bz || !cz || !assignment) return fail(BITZ_ERR_ARG, "null pointer");
+    // piop.rs:74-83: dimensions before any transcript call.
+    if ((int)row_vars != m->row_vars) return fail(BITZ_ERR_PRODUCT_DIMS, "products do not span the row domain");
+    if ((int)col_vars != m->col_vars) return fail(BITZ_ERR_ASSIGNMENT_DIMS, "assignment does not span the columns");
+    if (row_vars == 0 || col_vars == 0 || row_vars > 32)
+        return fail(BITZ_ERR_ARG, "the engine proves 1 <= row_vars <= 32 and col_vars >= 1");
+    for (const auto& j : e->jobs)
+        if (j->tag == tag) return fail(BITZ_ERR_ARG, "tag in use");
+    auto j = std::make_unique<PiopJob>();
+    j->tag = tag;
+    j->t = t;
+    j->m = m;
+    j->hv = reinterpret_cast<const f128*>(assignment);
+    j->n = (int)row_vars;
+    j->nc = (int)col_vars;
+    hipStream_t s = e->so.s;
+    // piop.rs:85-90: digest, tau, make_equality_factors (crates/poly/src/eq.rs:107-115).
+    t->t.absorb(0, digest, 32);
+    std::vector<f128> tau(row_vars);
+    for (uint32_t i = 0; i < row_vars; ++i) tau[i] = t->t.squeeze();
+    const uint32_t split = row_vars / 2;
+    auto host_eq = [](const f128* r, uint32_t k) {  // eq_table (eq.rs:63-99)
+        std::vector<f128> tbl((size_t)1 << k, f128{0, 0});
+        tbl[0] = hostf::one();
+        for (uint32_t i = 0; i < k; ++i) {
+            const size_t half = (size_t)1 << i;
+            for (size_t x = 0; x < half; ++x) {
+                const f128 one = hostf::mul(tbl[x], r[i]);
+                tbl[x] = hostf::add(tbl[x], one);
+                tbl[x + half] = one;
+            }
+        }
+        return tbl;
+    };
+    j->lo = host_eq(tau.data(), split);
+    j->hi = host_eq(tau.data() + split, row_vars - split);
+    TRY(upload(j->dlo, reinterpret_cast<const bitz_f128*>(j->lo.data()), j->lo.size(), 0, s));
+    TRY(upload(j->dhi, reinterpret_cast<const bitz_f128*>(j->hi.data()), j->hi.size(), 0, s));
+    // The outer sumcheck from claim 0 over the eq-factored kernels; the
+    // factors are eq tables of tau by construction, so no device check.
+    TRY(outer_begin_impl(&j->outer, reinterpret_cast<const bitz_f128*>(j->dlo.p), split,
+                         reinterpret_cast<const bitz_f128*>(j->dhi.p), row_vars - split, az, row_vars, bz, row_vars,
+                         cz, row_vars, BITZ_DEVICE, false));
+    bitz_outer* h = j->outer;
+    if (!h->eqmode || !outer_eq_constants(h, tau))
+        return fail(BITZ_ERR_ARG, "eq factors outside the eq-factored kernels' range");
+    h->started = true;
+    if (h->htab_total)
+        hipLaunchKernelGGL(bitz::k_htabs4, dim3((unsigned)h->htab_total), dim3(1024), 0, s, h->eq_in[0], h->ell,
+                           h->eq_in[1], h->hdesc, static_cast<uint4*>(h->htabs4.p));
+    TRY(check_launch());
+    j->outer_rounds.resize(4 * (size_t)row_vars);
+    j->rx.resize(row_vars);
+    j->inner_rounds.resize(3 * (size_t)col_vars);
+    j->inner_point.resize(col_vars);
+    e->jobs.push_back(std::move(j));
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight) {
+    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
+    hipStream_t s = e->so.s;
+    std::vector<PiopJob*> act;
+    for (auto& j : e->jobs)
+        if (j->phase != JobPhase::kDone) act.push_back(j.get());
+    if (in_flight) *in_flight = act.size();
+    if (act.empty()) return BITZ_OK;
+    constexpr size_t kSlotWords = 16;
+    const size_t slot_bytes = act.size() * kSlotWords * 4;
+    if (e->slots.bytes < slot_bytes) TRY(e->slots.alloc(slot_bytes));
+    auto slot = [&](size_t i) { return e->slots.u32() + kSlotWords * i; };
+    // Challenge tables to build: the r each fold round folds by.
+    std::vector<f128> consts;
+    std::vector<int> table_of(act.size(), -1);
+    for (size_t i = 0; i < act.size(); ++i) {
+        const PiopJob* j = act[i];
+        const bool outer_fold = j->phase == JobPhase::kOuter && j->round > 0;
+        const bool inner_fold = j->phase == JobPhase::kInner && j->round > 0;
+        if (outer_fold || inner_fold) {
+            table_of[i] = (int)consts.size();
+            consts.push_back(j->r);
+        }
+    }
+    if (e->rt4.bytes < std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4))
+        TRY(e->rt4.alloc(std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4)));
+    const uint4* rt4 = static_cast<const uint4*>(e->rt4.p);
+    // Descriptors of the batched launches.
+    std::vector<bitz::RoundDesc> o_r0, o_fold;
+    std::vector<uint32_t> o_r0_map, o_fold_map;
+    std::vector<bitz::InnerRoundDesc> i_r0, i_fold;
+    std::vector<uint32_t> i_r0_map, i_fold_map;
+    std::vector<size_t> bind_jobs;
+    for (size_t i = 0; i < act.size(); ++i) {
+        PiopJob* j = act[i];
+        if (j->phase == JobPhase::kOuter) {
+            bitz_outer* h = j->outer;
+            const int k = j->round;
+            bitz::RoundDesc d{};
+            if (k > 0) {
+                outer_eq_advance(h, k - 1, j->r);
+                Dev& dst = (h->where == 1) ? h->Q : h->P;
+                const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+                j->pend[0] = dst.f();
+                j->pend[1] = dst.f() + cap;
+                j->pend[2] = dst.f() + 2 * cap;
+                d.ao = j->pend[0];
+                d.bo = j->pend[1];
+                d.co = j->pend[2];
+                d.r_tab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
+            }
+            j->q1 = k == 0 || hostf::is_zero(h->tau[k]);
+            const auto& g = h->geo[k];
+            d.a = h->cur[0];
+            d.b = h->cur[1];
+            d.c = h->cur[2];
+            d.la = h->table_ptr(g.la.table);
+            d.lb = h->table_ptr(g.lb.table);
+            d.h_tabs4 = static_cast<const uint4*>(h->htabs4.p) + g.htab_off * kConstTable4Entries;
+            d.acc = slot(i);
+            d.threads = g.threads;
+            d.iters = g.iters;
+            d.la_bits = g.la_bits;
+            d.la_shift = g.la.a - h->table_base(g.la.table);
+            d.lb_shift = g.lb.a - h->table_base(g.lb.table);
+            d.q1 = j->q1 ? 1 : 0;
+            auto& descs = k == 0 ? o_r0 : o_fold;
+            auto& map = k == 0 ? o_r0_map : o_fold_map;
+            const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
+            d.block_base = (uint32_t)map.size();
+            map.insert(map.end(), (g.threads + block - 1) / block, (uint32_t)descs.size());
+            descs.push_back(d);
+        } else if (j->phase == JobPhase::kBind) {
+            bind_jobs.push_back(i);
+            bitz::InnerRoundDesc d{};
+            d.d = j->dD.f();
+            d.w = j->hv;
+            d.pairs = j->ilen / 2;
+            d.acc = slot(i);
+            d.blocks = inner_grid(d.pairs);
+            d.block_base = (uint32_t)i_r0_map.size();
+            i_r0_map.insert(i_r0_map.end(), d.blocks, (uint32_t)i_r0.size());
+            i_r0.push_back(d);
+        } else if (j->phase == JobPhase::kInner) {
+            const size_t next_len = j->ilen / 2;
+            Dev& dst = (j->iwhere == 1) ? j->iQ : j->iP;
+            const size_t cap = (j->iwhere == 1) ? j->icapQ : j->icapP;
+            j->pend[0] = dst.f();
+            j->pend[1] = dst.f() + cap;
+            bitz::InnerRoundDesc d{};
+            d.d = j->icur[0];
+            d.w = j->icur[1];
+            d.dout = j->pend[0];
+            d.wout = j->pend[1];
+            d.pairs = next_len / 2;
+            d.rtab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
+            d.acc = slot(i);
+            d.blocks = inner_grid(d.pairs);
+            d.block_base = (uint32_t)i_fold_map.size();
+            i_fold_map.insert(i_fold_map.end(), d.blocks, (uint32_t)i_fold.size());
+            i_fold.push_back(d);
+        }
+    }
+    // Staging: challenges | row points of the binds | descriptors and maps.
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t o = off;
+        off += bytes;
+        return o;
+    };
+    const size_t c_o = place(consts.size() * sizeof(f128));
+    std::vector<size_t> rx_o(bind_jobs.size());
+    for (size_t b = 0; b < bind_jobs.size(); ++b) rx_o[b] = place((size_t)act[bind_jobs[b]]->n * sizeof(f128));
+    const size_t or_o = place(o_r0.size() * sizeof(bitz::RoundDesc)), orm_o = place(o_r0_map.size() * 4);
+    const size_t of_o = place(o_fold.size() * sizeof(bitz::RoundDesc)), ofm_o = place(o_fold_map.size() * 4);
+    const size_t ir_o = place(i_r0.size() * sizeof(bitz::InnerRoundDesc)), irm_o = place(i_r0_map.size() * 4);
+    const size_t if_o = place(i_fold.size() * sizeof(bitz::InnerRoundDesc)), ifm_o = place(i_fold_map.size() * 4);
+    const size_t total = std::max<size_t>(off, 256);
+    if (e->pin_bytes < total) {
+        if (e->pin) (void)hipHostFree(e->pin);
+        e->pin = nullptr;
+        e->pin_bytes = 0;
+        if (hipHostMalloc(&e->pin, std::max(total, slot_bytes) * 2, hipHostMallocDefault) != hipSuccess) {
+            e->pin = nullptr;
+            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+        }
+        e->pin_bytes = std::max(total, slot_bytes) * 2;
+    }
+    if (e->pin_bytes < slot_bytes) {
+        (void)hipHostFree(e->pin);
+        e->pin = nullptr;
+        e->pin_bytes = 0;
+        if (hipHostMalloc(&e->pin, slot_bytes * 2, hipHostMallocDefault) != hipSuccess) {
+            e->pin = nullptr;
+            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+        }
+        e->pin_bytes = slot_bytes * 2;
+    }
+    if (e->stage.bytes < total) TRY(e->stage.alloc(total));
+    char* pin = static_cast<char*>(e->pin);
+    char* dst = static_cast<char*>(e->stage.p);
+    memcpy(pin + c_o, consts.data(), consts.size() * sizeof(f128));
+    for (size_t b = 0; b < bind_jobs.size(); ++b) {
+        const PiopJob* j = act[bind_jobs[b]];
+        memcpy(pin + rx_o[b], j->rx.data(), (size_t)j->n * sizeof(f128));
+    }
+    memcpy(pin + or_o, o_r0.data(), o_r0.size() * sizeof(bitz::RoundDesc));
+    memcpy(pin + orm_o, o_r0_map.data(), o_r0_map.size() * 4);
+    memcpy(pin + of_o, o_fold.data(), o_fold.size() * sizeof(bitz::RoundDesc));
+    memcpy(pin + ofm_o, o_fold_map.data(), o_fold_map.size() * 4);
+    memcpy(pin + ir_o, i_r0.data(), i_r0.size() * sizeof(bitz::InnerRoundDesc));
+    memcpy(pin + irm_o, i_r0_map.data(), i_r0_map.size() * 4);
+    memcpy(pin + if_o, i_fold.data(), i_fold.size() * sizeof(bitz::InnerRoundDesc));
+    memcpy(pin + ifm_o, i_fold_map.data(), i_fold_map.size() * 4);
+    HIP_TRY(hipMemcpyAsync(dst, pin, off, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(e->slots.p, 0, slot_bytes, s));
+    if (!consts.empty())
+        hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)consts.size()), dim3(512), 0, s,
+                           reinterpret_cast<const f128*>(dst + c_o), static_cast<uint4*>(e->rt4.p));
+    // Per proof: binds (then inner round 0 reads D) and the final folds.
+    for (size_t b = 0; b < bind_jobs.size(); ++b) {
+        PiopJob* j = act[bind_jobs[b]];
+        std::lock_guard<std::mutex> lock(j->m->mu);
+        HIP_TRY(hipMemcpyAsync(j->m->point.p, dst + rx_o[b], (size_t)j->n * sizeof(f128), hipMemcpyDeviceToDevice,
+                               s));
+        TRY(bind_enqueue(j->m, j->rho, j->dD.f(), s));
+    }
+    for (size_t i = 0; i < act.size(); ++i) {
+        PiopJob* j = act[i];
+        f128* out = reinterpret_cast<f128*>(slot(i));
+        if (j->phase == JobPhase::kOuterFinal) {
+            bitz_outer* h = j->outer;
+            hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2], out,
+                               out + 1, out + 2, (size_t)1, j->r);
+        } else if (j->phase == JobPhase::kInnerFinal) {
+            hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[0], out, (size_t)1, j->r);
+            hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[1], out + 1, (size_t)1, j->r);
+        }
+    }
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_fold),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_r0_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_fold_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
+        attr = true;
+    }
+    if (!o_r0.empty())
+        hipLaunchKernelGGL(bitz::k_batch_r0, dim3((unsigned)o_r0_map.size()), dim3(bitz::kR0Block), bitz::kR0Smem,
+                           s, reinterpret_cast<const bitz::RoundDesc*>(dst + or_o),
+                           reinterpret_cast<const uint32_t*>(dst + orm_o));
+    if (!o_fold.empty())
+        hipLaunchKernelGGL(bitz::k_batch_fold, dim3((unsigned)o_fold_map.size()), dim3(bitz::kFoldBlock),
+                           bitz::kFoldSmem, s, reinterpret_cast<const bitz::RoundDesc*>(dst + of_o),
+                           reinterpret_cast<const uint32_t*>(dst + ofm_o));
+    if (!i_r0.empty())
+        hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)i_r0_map.size()), dim3(bitz::kInnerBlock),
+                           bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + ir_o),
+                           reinterpret_cast<const uint32_t*>(dst + irm_o));
+    if (!i_fold.empty())
+        hipLaunchKernelGGL(bitz::k_inner_fold_batch, dim3((unsigned)i_fold_map.size()), dim3(bitz::kInnerBlock),
+                           bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + if_o),
+                           reinterpret_cast<const uint32_t*>(dst + ifm_o));
+    TRY(check_launch());
+    HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // Host: every proof's round polynomial into its transcript, its next challenge.
+    for (size_t i = 0; i < act.size(); ++i) {
+        PiopJob* j = act[i];
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(pin) + kSlotWords * i;
+        Transcript& tr = j->t->t;
+        switch (j->phase) {
+            case JobPhase::kOuter: {
+                bitz_outer* h = j->outer;
+                const int k = j->round;
+                if (k > 0) {
+                    for (int u = 0; u < 3; ++u) h->cur[u] = j->pend[u];
+                    h->where = (h->where == 1) ? 2 : 1;
+                    h->len /= 2;
+                    h->round += 1;
+                }
+                bitz_f128 raw[3], co[3];
+                memcpy(raw, w, 48);
+                outer_eq_coeffs(h, k, raw, j->q1, h->scale_big[k], co);
+                j->r = recover_round(tr, j->claim, co, 3, j->outer_rounds.data() + 4 * k, j->rx.data() + k);
+                j->round = k + 1;
+                if (j->round == j->n) j->phase = JobPhase::kOuterFinal;
+                break;
+            }
+            case JobPhase::kOuterFinal: {
+                memcpy(j->outer_claims, w, 48);
+                f128 cl[3] = {F(j->outer_claims[0]), F(j->outer_claims[1]), F(j->outer_claims[2])};
+                tr.absorb_elems(0, cl, 3);  // sumcheck.rs:372
+                // piop.rs:94-98: rho, the inner claim; D and the inner sumcheck follow.
+                j->rho = tr.squeeze();
+                j->claim = hostf::add(cl[0], hostf::add(hostf::mul(j->rho, cl[1]),
+                                                        hostf::mul(hostf::mul(j->rho, j->rho), cl[2])));
+                bitz_outer_free(j->outer);
+                j->outer = nullptr;
+                j->dlo.release();
+                j->dhi.release();
+                const size_t N = (size_t)1 << j->nc;
+                j->ilen = N;
+                j->icapP = std::max<size_t>(N / 2, 1);
+                j->icapQ = std::max<size_t>(N / 4, 1);
+                if ((j->status = j->dD.alloc(N * sizeof(f128))) || (j->status = j->iP.alloc(2 * j->icapP * sizeof(f128))) ||
+                    (j->status = j->iQ.alloc(2 * j->icapQ * sizeof(f128)))) {
+                    j->phase = JobPhase::kDone;
+                    break;
+                }
+                j->icur[0] = j->dD.f();
+                j->icur[1] = j->hv;
+                j->round = 0;
+                j->phase = JobPhase::kBind;
+                break;
+            }
+            case JobPhase::kBind:
+            case JobPhase::kInner: {
+                const int k = j->round;
+                if (k > 0) {
+                    j->icur[0] = j->pend[0];
+                    j->icur[1] = j->pend[1];
+                    j->iwhere = (j->iwhere == 1) ? 2 : 1;
+                    j->ilen /= 2;
+                }
+                bitz_f128 co[2];
+                memcpy(co, w, 32);
+                j->r = recover_round(tr, j->claim, co, 2, j->inner_rounds.data() + 3 * k, j->inner_point.data() + k);
+                j->round = k + 1;
+                j->phase = j->round == j->nc ? JobPhase::kInnerFinal : JobPhase::kInner;
+                break;
+            }
+            case JobPhase::kInnerFinal: {
+                memcpy(&j->matrix_eval, w, 16);  // D(r_y); h(r_y) follows at w + 4
+                j->final_claim = B(j->claim);
+                j->iP.release();
+                j->iQ.release();
+                j->dD.release();
+                j->phase = JobPhase::kDone;
+                break;
+            }
+            case JobPhase::kDone:
+                break;
+        }
+    }
+    if (in_flight) {
+        size_t n = 0;
+        for (auto& j : e->jobs) n += j->phase != JobPhase::kDone;
+        *in_flight = n;
+    }
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count) {
+    if (!e || !count) return fail(BITZ_ERR_ARG, "null pointer");
+    size_t n = 0;
+    for (auto& j : e->jobs)
+        if (j->phase == JobPhase::kDone) {
+            if (tags && n < cap) tags[n] = j->tag;
+            ++n;
+        }
+    *count = n;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds,
+                                         bitz_f128* outer_claims, bitz_f128* inner_rounds, bitz_f128* inner_point,
+                                         bitz_f128* matrix_eval, bitz_f128* final_claim) {
+    if (!e || !outer_rounds || !outer_claims || !inner_rounds || !inner_point || !matrix_eval || !final_claim)
+        return fail(BITZ_ERR_ARG, "null pointer");
+    for (size_t i = 0; i < e->jobs.size(); ++i) {
+        PiopJob* j = e->jobs[i].get();
+        if (j->tag != tag) continue;
+        if (j->phase != JobPhase::kDone) return fail(BITZ_ERR_STATE, "proof still in flight");
+        const int32_t st = j->status;
+        if (st == BITZ_OK) {
+            memcpy(outer_rounds, j->outer_rounds.data(), j->outer_rounds.size() * sizeof(bitz_f128));
+            memcpy(outer_claims, j->outer_claims, 3 * sizeof(bitz_f128));
+            memcpy(inner_rounds, j->inner_rounds.data(), j->inner_rounds.size() * sizeof(bitz_f128));
+            memcpy(inner_point, j->inner_point.data(), j->inner_point.size() * sizeof(bitz_f128));
+            *matrix_eval = j->matrix_eval;
+            *final_claim = j->final_claim;
+        }
+        e->jobs.erase(e->jobs.begin() + (std::ptrdiff_t)i);
+        return st == BITZ_OK ? BITZ_OK : fail(st, "proof failed");
+    }
+    return fail(BITZ_ERR_ARG, "no such tag");
+}
+
 
 // ================================================================ M^T =====
 struct bitz_mt {

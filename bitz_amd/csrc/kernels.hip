@@ -1058,14 +1058,14 @@ __global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, c
 // ======================== inner sumcheck with challenges known in advance ===
 // [c0, c2] of round k = [sum m0 w0, sum (m1+m0)(w1+w0)] over the round's pairs
 // (sumcheck.rs:568-609), products through lane tables.
-__global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs,
-                                                              uint32_t* acc) {
+__device__ __forceinline__ void inner_r0_body(const f128* d, const f128* w, size_t pairs, uint32_t* acc,
+                                              size_t block, size_t blocks) {
     extern __shared__ uint4 smem_raw[];
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     g4 s0 = g_zero(), s2 = g_zero();
 #pragma unroll 1
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pairs; p += (size_t)gridDim.x * blockDim.x) {
+    for (size_t p = block * kInnerBlock + threadIdx.x; p < pairs; p += blocks * kInnerBlock) {
         const g4 m0 = g_load_hbm(d + 2 * p), m1 = g_load_hbm(d + 2 * p + 1);
         const g4 w0 = g_load_hbm(w + 2 * p), w1 = g_load_hbm(w + 2 * p + 1);
         s0 = g_add(s0, g_mul_lane(m0, w0, col));
@@ -1076,22 +1076,27 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, co
     block_xor_to_global<2>(v, acc);
 }
 
+__global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs,
+                                                              uint32_t* acc) {
+    inner_r0_body(d, w, pairs, acc, blockIdx.x, gridDim.x);
+}
+
 // Fold by r_{k-1} on load (4 constant-table products per output pair) and
 // round k's sums (fold_and_compute_next_inner_round_coefficients_without_linear,
 // sumcheck.rs:641-688).
-__global__ __launch_bounds__(kInnerBlock) void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout,
-                                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc) {
+__device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f128* dout, f128* wout,
+                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc, size_t block,
+                                                size_t blocks) {
     extern __shared__ uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
-    for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = u4_load_hbm(rtab4 + e);
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += kInnerBlock) rt[e] = u4_load_hbm(rtab4 + e);
     __syncthreads();
     g4 s0 = g_zero(), s2 = g_zero();
     auto ip = [&](const g4& a, const g4& b) { return g_add(a, g_mul_table4(rt, g_add(a, b))); };
 #pragma unroll 1
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < out_pairs;
-         p += (size_t)gridDim.x * blockDim.x) {
+    for (size_t p = block * kInnerBlock + threadIdx.x; p < out_pairs; p += blocks * kInnerBlock) {
         g4 in[2][4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1110,6 +1115,22 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_fold_lane(const f128* d, 
     }
     g4 v[2] = {s0, s2};
     block_xor_to_global<2>(v, acc);
+}
+
+__global__ __launch_bounds__(kInnerBlock) void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout,
+                                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc) {
+    inner_fold_body(d, w, dout, wout, out_pairs, rtab4, acc, blockIdx.x, gridDim.x);
+}
+
+// The same rounds of many inner sumchecks in one launch: workgroup -> round
+// descriptor through bmap, each descriptor's workgroups striding its pairs.
+__global__ __launch_bounds__(kInnerBlock) void k_inner_r0_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
+    const InnerRoundDesc A = d[bmap[blockIdx.x]];
+    inner_r0_body(A.d, A.w, A.pairs, A.acc, blockIdx.x - A.block_base, A.blocks);
+}
+__global__ __launch_bounds__(kInnerBlock) void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
+    const InnerRoundDesc A = d[bmap[blockIdx.x]];
+    inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks);
 }
 
 // Rounds k0 .. n-1 in one workgroup: the two tables (folded by r_{k0-1} on

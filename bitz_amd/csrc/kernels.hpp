@@ -189,6 +189,20 @@ __global__ void k_batch_prologue(const BatchPrologue* items, const uint32_t* bma
 constexpr int kInnerBlock = 256;
 constexpr size_t kInnerGridSmem = sizeof(uint4) * (kConstTable4Entries + 4 * 64 * kLaneTableEntries);
 __global__ void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
+// One inner sumcheck's round in a batched launch (k_inner_r0_batch: pairs of
+// round 0; k_inner_fold_batch: output pairs, folding d, w by rtab4's challenge).
+struct InnerRoundDesc {
+    const f128* d;
+    const f128* w;
+    f128* dout;
+    f128* wout;
+    size_t pairs;
+    const uint4* rtab4;
+    uint32_t* acc;
+    uint32_t blocks, block_base;
+};
+__global__ void k_inner_r0_batch(const InnerRoundDesc* d, const uint32_t* bmap);
+__global__ void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap);
 __global__ void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout, size_t out_pairs,
                                   const uint4* rtab4, uint32_t* acc);
 constexpr int kInnerTailMaxLog = 10;  // at most 2^10 entries per table in LDS

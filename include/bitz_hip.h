@@ -283,6 +283,34 @@ int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint
                                 bitz_f128* inner_point, bitz_f128* matrix_eval, bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
 
+/* ---- many PIOP proofs, rounds batched (no reference counterpart) -------- */
+/* An engine advances every submitted prove_spartan_piop (piop.rs:67-112) by
+ * one round per step: the rounds of one kind across all proofs run as one
+ * launch, every proof's sums come back in one read, and each proof's own
+ * transcript absorbs its round polynomial and squeezes its next challenge on
+ * the host, in bitz_prove_spartan_piop's order.  A proof's outputs equal
+ * bitz_prove_spartan_piop's for the same transcript and inputs; proofs
+ * submitted between steps join at the next step (continuous batching of a
+ * proving server).  Products, assignment (device) and the transcript must
+ * live until the proof is taken; a bitz_matrices handle in use by an engine
+ * must not be used elsewhere meanwhile.  1 <= row_vars <= 32, col_vars >= 1.
+ * Not thread-safe: one thread drives an engine. */
+typedef struct bitz_piop_engine bitz_piop_engine;
+int32_t bitz_piop_engine_create(bitz_piop_engine** e);
+int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                                const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
+                                const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
+                                uint32_t col_vars);
+/* One round of every proof in flight; *in_flight receives how many remain. */
+int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight);
+/* Tags of the finished proofs not yet taken (up to cap; *count = all). */
+int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count);
+/* A finished proof's outputs (as bitz_prove_spartan_piop's); releases it. */
+int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds, bitz_f128* outer_claims,
+                              bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
+                              bitz_f128* final_claim);
+void bitz_piop_engine_free(bitz_piop_engine* e);
+
 /* ---- SHA-256 R1CS instances (crates/circuit/src/sha256.rs) ---------------- */
 /* The constraint system (M, A, B, C) and witness (f, h) of BitZ's SHA-256
  * circuits, produced on the host by one gadget description run symbolically
