@@ -86,7 +86,7 @@ def main() -> int:
                          "over 2^22 leaves per step; inner: one inner sumcheck over 2 tables of 2^inner_log_n")
     ap.add_argument("--serve-rate", type=float, default=300.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,
                     help="inner: log2 table length (23: the column count of a 256-block SHA-256 chain)")
@@ -498,9 +498,13 @@ def bench_serve(G, args) -> int:
     """Serving: Poisson arrivals of SHA-256 proving requests (a compression
     chain of blocks = 2^U blocks, U uniform in 0..8, message and chaining
     value uniform from the seed; see bitz_amd/serving.py), admitted by HBM
-    budget into concurrent slots.  Each request: witness (host), assignment,
-    Az/Bz/Cz and the PIOP on the device.  The same requests proved one after
-    another give the plain rate."""
+    budget into slots.  Each request: witness (host), assignment, Az/Bz/Cz
+    and the PIOP on the device.  The requests are served with continuous
+    batching (one engine steps every proof in flight, each round kind in one
+    launch) and with concurrent slots (a thread and streams per request); the
+    same requests all queued at once give each mode's capacity, and proved
+    one after another the plain rate.  value = the better capacity;
+    p50/p99 = latency at the Poisson rate in that mode."""
     from bitz_amd import serving as S
     srv = S.ProvingServer(hbm_budget=int(args.serve_budget_gib * (1 << 30)), slots=args.serve_slots)
     log("bench: preparing the nine SHA-256 circuits (1 .. 256 blocks)")
@@ -508,57 +512,57 @@ def bench_serve(G, args) -> int:
     for u in range(9):  # circuits and kernels warm, like a deployed service
         b = 1 << u
         srv.prove(10**6 + u, b, *S.random_input(rng, b))
+    srv.serve_batched([S.Request(10**6 + 10 + u, 0.0, 1 << u, *S.random_input(rng, 1 << u)) for u in range(9)])
     G.synchronize()
     reqs = S.poisson_requests(args.serve_rate, args.serve_requests, args.seed)
-    log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, {sum(r.blocks for r in reqs)} blocks")
-    srv.serve(reqs)
-    peak_in_flight, peak_bytes = srv.peak_in_flight, srv.peak_bytes
-    start = min(r.arrival for r in reqs)
-    span = max(r.done for r in reqs) - start
-    lat = np.array([r.latency for r in reqs]) * 1e3
     blocks = sum(r.blocks for r in reqs)
-    # Capacity: the same requests all queued at once (the server saturated).
-    sat = [S.Request(r.rid, 0.0, r.blocks, r.message, r.iv) for r in reqs]
-    t0 = time.perf_counter()
-    srv.serve(sat)
-    saturated = time.perf_counter() - t0
-    sat_lat = np.array([r.latency for r in sat]) * 1e3
-    sat_peak = srv.peak_in_flight
+    log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, {blocks} blocks")
+
+    def run(mode: str, arrivals: bool) -> dict:
+        rs = [S.Request(r.rid, r.arrival if arrivals else 0.0, r.blocks, r.message, r.iv) for r in reqs]
+        t0 = time.perf_counter()
+        (srv.serve_batched if mode == "batched" else srv.serve)(rs)
+        wall = time.perf_counter() - t0
+        span = max(r.done for r in rs) - min(r.arrival for r in rs)
+        lat = np.array([r.latency for r in rs]) * 1e3
+        return {"blocks_per_s": blocks / (span if arrivals else wall), "proofs_per_s": len(rs) / (span if arrivals else wall),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "peak_in_flight": srv.peak_in_flight, "peak_hbm_gib": srv.peak_bytes / (1 << 30)}
+
+    res = {mode: {"poisson": run(mode, True), "saturated": run(mode, False)} for mode in ("batched", "slots")}
     t0 = time.perf_counter()
     for r in reqs:
         srv.prove(r.rid, r.blocks, r.message, r.iv)
     G.synchronize()
     serial = time.perf_counter() - t0
+    best = max(res, key=lambda m: res[m]["saturated"]["blocks_per_s"])
     c256 = srv.circuit(256)
     line = {
         "metric": METRIC,
-        "value": blocks / span,
+        "value": res[best]["saturated"]["blocks_per_s"],
         "unit": "SHA-256 blocks/s",
         "n_gpus": 1,
         "steps": len(reqs),
-        "warmup": 9,
-        "ms_per_step": span / len(reqs) * 1e3,
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
+        "warmup": 18,
+        "ms_per_step": 1e3 * blocks / res[best]["saturated"]["blocks_per_s"] / len(reqs),
+        "p50_ms": res[best]["poisson"]["p50_ms"],
+        "p99_ms": res[best]["poisson"]["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "gf2_128",
         "data": "synthetic (message bytes and chaining value per request uniform from the seed)",
-        "config": {"workload": f"serving: {len(reqs)} Poisson arrivals at {args.serve_rate}/s of SHA-256 "
-                               "compression chains, blocks 2^U with U uniform in 0..8, "
-                               f"{args.serve_slots} slots, HBM budget {args.serve_budget_gib} GiB; each request: "
-                               "witness, assignment MLE, Az/Bz/Cz, prove_spartan_piop (duplex transcript)",
-                   "requests": len(reqs), "blocks": blocks,
+        "config": {"workload": f"serving SHA-256 compression chains: {len(reqs)} requests, blocks 2^U with U uniform "
+                               f"in 0..8, {args.serve_slots} slots (batched: {max(1, args.serve_slots // 8)} engines), "
+                               f"HBM budget {args.serve_budget_gib} GiB; each "
+                               "request: witness, assignment MLE, Az/Bz/Cz, prove_spartan_piop (duplex transcript). "
+                               f"value: capacity with every request queued ({best} mode); p50/p99: latency under "
+                               f"Poisson arrivals at {args.serve_rate}/s in that mode",
+                   "requests": len(reqs), "blocks": blocks, "mode": best,
                    "circuit_256_blocks": {"rows": c256.rows, "columns": c256.columns, "nnz": c256.nnz}},
-        "proofs_per_s": len(reqs) / span,
-        "peak_in_flight": peak_in_flight,
-        "peak_hbm_gib": peak_bytes / (1 << 30),
-        "saturated": {"blocks_per_s": blocks / saturated, "proofs_per_s": len(reqs) / saturated,
-                      "peak_in_flight": sat_peak, "p50_ms": float(np.percentile(sat_lat, 50)),
-                      "p99_ms": float(np.percentile(sat_lat, 99))},
+        "modes": res,
         "one_at_a_time": {"blocks_per_s": blocks / serial, "proofs_per_s": len(reqs) / serial},
-        "capacity_vs_one_at_a_time": serial / saturated,
+        "capacity_vs_one_at_a_time": res[best]["saturated"]["blocks_per_s"] / (blocks / serial),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_sha256(args.seed)

@@ -146,6 +146,10 @@ def lib():
             "bitz_piop_engine_create": (I32, [C.POINTER(C.c_void_p)]),
             "bitz_piop_engine_submit": (I32, [P, C.c_uint64, P, P, P, P, P, P, U32, P, U32]),
             "bitz_piop_engine_step": (I32, [P, C.POINTER(SZ)]),
+            "bitz_piop_engine_launch": (I32, [P, C.POINTER(SZ)]),
+            "bitz_piop_engine_run": (I32, [P]),
+            "bitz_piop_engine_wait": (I32, [P, U32, C.POINTER(SZ)]),
+            "bitz_piop_engine_finish": (I32, [P, C.POINTER(SZ)]),
             "bitz_piop_engine_done": (I32, [P, P, SZ, C.POINTER(SZ)]),
             "bitz_piop_engine_take": (I32, [P, C.c_uint64, P, P, P, P, P, P]),
             "bitz_piop_engine_free": (None, [P]),
@@ -845,12 +849,24 @@ class PiopEngine:
     step until they finish, take each one's SpartanPiopOutput.  The inputs of
     a proof are kept alive here until it is taken."""
 
-    def __init__(self):
+    def __init__(self, background: bool = False):
+        """background: a thread of the engine steps while proofs are in
+        flight (then use wait/done/take, not step/launch/finish)."""
         init()
         h = C.c_void_p()
         _check(lib().bitz_piop_engine_create(C.byref(h)))
         self.h = h
         self._held: dict[int, tuple] = {}
+        self.launched = False
+        if background:
+            _check(lib().bitz_piop_engine_run(h))
+
+    def wait(self, timeout_s: float) -> int:
+        """Blocks until a finished proof waits to be taken (or the timeout);
+        returns how many wait."""
+        n = C.c_size_t()
+        _check(lib().bitz_piop_engine_wait(self.h, int(timeout_s * 1e6), C.byref(n)))
+        return n.value
 
     def submit(self, tag: int, transcript: Transcript, matrices: "PreparedConstraintMatrices", digest: bytes,
                products: R1csProductMles, assignment: DeviceTable) -> None:
@@ -867,6 +883,24 @@ class PiopEngine:
         n = C.c_size_t()
         _check(lib().bitz_piop_engine_step(self.h, C.byref(n)))
         return n.value
+
+    def launch(self) -> int:
+        """Enqueues one step without waiting; returns the proofs in it."""
+        n = C.c_size_t()
+        _check(lib().bitz_piop_engine_launch(self.h, C.byref(n)))
+        self.launched = n.value > 0
+        return n.value
+
+    def finish(self) -> int:
+        """Completes the launched step; returns the proofs still in flight."""
+        n = C.c_size_t()
+        self.launched = False
+        _check(lib().bitz_piop_engine_finish(self.h, C.byref(n)))
+        return n.value
+
+    @property
+    def in_flight(self) -> int:
+        return len(self._held) - len(self.done())
 
     def done(self) -> list[int]:
         n = C.c_size_t()

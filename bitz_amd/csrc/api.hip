@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -2861,11 +2862,30 @@ struct PiopJob {
 
 struct bitz_piop_engine {
     std::vector<std::unique_ptr<PiopJob>> jobs;
+    std::vector<PiopJob*> launched;  // the proofs of the step in flight on the device
     Dev slots, stage, rt4;
     void* pin = nullptr;
     size_t pin_bytes = 0;
+    // Background stepping (bitz_piop_engine_run): a thread of the engine's
+    // own steps while proofs are in flight; submit hands it proofs through
+    // `incoming`, finished ones wait in `finished` for take.
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::unique_ptr<PiopJob>> incoming, finished;
+    std::thread worker;
+    bool running = false, stop = false;
+    int32_t worker_rc = BITZ_OK;
+    std::string worker_err;
     StreamOwner so;
     ~bitz_piop_engine() {
+        if (running) {
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                stop = true;
+            }
+            cv.notify_all();
+            worker.join();
+        }
         (void)hipStreamSynchronize(so.s);
         if (pin) (void)hipHostFree(pin);
     }
@@ -2894,8 +2914,12 @@ extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bi
     if ((int)col_vars != m->col_vars) return fail(BITZ_ERR_ASSIGNMENT_DIMS, "assignment does not span the columns");
     if (row_vars == 0 || col_vars == 0 || row_vars > 32)
         return fail(BITZ_ERR_ARG, "the engine proves 1 <= row_vars <= 32 and col_vars >= 1");
-    for (const auto& j : e->jobs)
-        if (j->tag == tag) return fail(BITZ_ERR_ARG, "tag in use");
+    {
+        std::lock_guard<std::mutex> lock(e->mu);
+        for (const auto* list : {&e->jobs, &e->incoming, &e->finished})
+            for (const auto& j : *list)
+                if (j->tag == tag) return fail(BITZ_ERR_ARG, "tag in use");
+    }
     auto j = std::make_unique<PiopJob>();
     j->tag = tag;
     j->t = t;
@@ -2943,17 +2967,28 @@ extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bi
     j->rx.resize(row_vars);
     j->inner_rounds.resize(3 * (size_t)col_vars);
     j->inner_point.resize(col_vars);
-    e->jobs.push_back(std::move(j));
+    if (e->running) {
+        {
+            std::lock_guard<std::mutex> lock(e->mu);
+            e->incoming.push_back(std::move(j));
+        }
+        e->cv.notify_all();
+    } else {
+        e->jobs.push_back(std::move(j));
+    }
     return BITZ_OK;
 }
 
-extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight) {
+extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched) {
     if (!e) return fail(BITZ_ERR_ARG, "null pointer");
+    if (e->running && std::this_thread::get_id() != e->worker.get_id())
+        return fail(BITZ_ERR_STATE, "the engine steps in the background");
+    if (!e->launched.empty()) return fail(BITZ_ERR_STATE, "a step is already in flight");
     hipStream_t s = e->so.s;
     std::vector<PiopJob*> act;
     for (auto& j : e->jobs)
         if (j->phase != JobPhase::kDone) act.push_back(j.get());
-    if (in_flight) *in_flight = act.size();
+    if (launched) *launched = act.size();
     if (act.empty()) return BITZ_OK;
     constexpr size_t kSlotWords = 16;
     const size_t slot_bytes = act.size() * kSlotWords * 4;
@@ -3157,7 +3192,17 @@ extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight)
                            reinterpret_cast<const uint32_t*>(dst + ifm_o));
     TRY(check_launch());
     HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_bytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    e->launched = std::move(act);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_flight) {
+    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
+    constexpr size_t kSlotWords = 16;
+    std::vector<PiopJob*> act = std::move(e->launched);
+    e->launched.clear();
+    if (!act.empty()) HIP_TRY(hipStreamSynchronize(e->so.s));
+    const char* pin = static_cast<const char*>(e->pin);
     // Host: every proof's round polynomial into its transcript, its next challenge.
     for (size_t i = 0; i < act.size(); ++i) {
         PiopJob* j = act[i];
@@ -3245,10 +3290,18 @@ extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight)
     return BITZ_OK;
 }
 
+extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight) {
+    TRY(bitz_piop_engine_launch(e, nullptr));
+    return bitz_piop_engine_finish(e, in_flight);
+}
+
 extern "C" int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count) {
     if (!e || !count) return fail(BITZ_ERR_ARG, "null pointer");
+    std::lock_guard<std::mutex> lock(e->mu);
+    if (e->worker_rc) return fail(e->worker_rc, e->worker_err);
     size_t n = 0;
-    for (auto& j : e->jobs)
+    // While the worker runs, only `finished` is the caller's to read.
+    for (auto& j : e->running ? e->finished : e->jobs)
         if (j->phase == JobPhase::kDone) {
             if (tags && n < cap) tags[n] = j->tag;
             ++n;
@@ -3257,13 +3310,68 @@ extern "C" int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, si
     return BITZ_OK;
 }
 
+// The background thread: steps while proofs are in flight, sleeps otherwise.
+static void piop_engine_worker(bitz_piop_engine* e) {
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(e->mu);
+            auto active = [&] {
+                for (auto& j : e->jobs)
+                    if (j->phase != JobPhase::kDone) return true;
+                return false;
+            };
+            e->cv.wait(lock, [&] { return e->stop || !e->incoming.empty() || active(); });
+            for (auto& j : e->incoming) e->jobs.push_back(std::move(j));
+            e->incoming.clear();
+            if (e->stop && !active()) return;
+        }
+        size_t left = 0;
+        int32_t rc = bitz_piop_engine_launch(e, nullptr);
+        if (!rc) rc = bitz_piop_engine_finish(e, &left);
+        {
+            std::lock_guard<std::mutex> lock(e->mu);
+            if (rc) {
+                e->worker_rc = rc;
+                e->worker_err = bitz_last_error();
+                for (auto& j : e->jobs) {
+                    j->status = rc;
+                    j->phase = JobPhase::kDone;
+                }
+            }
+            for (auto& j : e->jobs)
+                if (j && j->phase == JobPhase::kDone) e->finished.push_back(std::move(j));
+            e->jobs.erase(std::remove(e->jobs.begin(), e->jobs.end(), nullptr), e->jobs.end());
+        }
+        e->cv.notify_all();
+    }
+}
+
+extern "C" int32_t bitz_piop_engine_run(bitz_piop_engine* e) {
+    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
+    if (e->running) return BITZ_OK;
+    if (!e->launched.empty()) return fail(BITZ_ERR_STATE, "a step is in flight");
+    e->running = true;
+    e->worker = std::thread(piop_engine_worker, e);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_us, size_t* finished) {
+    if (!e || !finished) return fail(BITZ_ERR_ARG, "null pointer");
+    std::unique_lock<std::mutex> lock(e->mu);
+    e->cv.wait_for(lock, std::chrono::microseconds(timeout_us), [&] { return !e->finished.empty() || e->worker_rc; });
+    *finished = e->finished.size();
+    return BITZ_OK;
+}
+
 extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds,
                                          bitz_f128* outer_claims, bitz_f128* inner_rounds, bitz_f128* inner_point,
                                          bitz_f128* matrix_eval, bitz_f128* final_claim) {
     if (!e || !outer_rounds || !outer_claims || !inner_rounds || !inner_point || !matrix_eval || !final_claim)
         return fail(BITZ_ERR_ARG, "null pointer");
-    for (size_t i = 0; i < e->jobs.size(); ++i) {
-        PiopJob* j = e->jobs[i].get();
+    std::unique_lock<std::mutex> lock(e->mu);
+    auto& list = e->running ? e->finished : e->jobs;
+    for (size_t i = 0; i < list.size(); ++i) {
+        PiopJob* j = list[i].get();
         if (j->tag != tag) continue;
         if (j->phase != JobPhase::kDone) return fail(BITZ_ERR_STATE, "proof still in flight");
         const int32_t st = j->status;
@@ -3275,9 +3383,12 @@ extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz
             *matrix_eval = j->matrix_eval;
             *final_claim = j->final_claim;
         }
-        e->jobs.erase(e->jobs.begin() + (std::ptrdiff_t)i);
+        list.erase(list.begin() + (std::ptrdiff_t)i);
         return st == BITZ_OK ? BITZ_OK : fail(st, "proof failed");
     }
+    for (const auto* other : {&e->jobs, &e->incoming})
+        for (const auto& j : *other)
+            if (j && j->tag == tag) return fail(BITZ_ERR_STATE, "proof still in flight");  // tags only: set at submit
     return fail(BITZ_ERR_ARG, "no such tag");
 }
 

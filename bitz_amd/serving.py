@@ -16,12 +16,16 @@ sha256_spartan.rs:45-90).
 
 Scheduling: requests wait in arrival order; one is admitted when a slot is
 free and its HBM footprint fits the budget beside the requests in flight.
-Every admitted request runs on its own host thread with its own HIP streams,
-so the requests' kernels overlap on the device (and one request's witness
-generation overlaps the others' device work); a request that arrives while
-others run starts as soon as a slot and memory free up.  Each result depends
-only on its request (its circuit, its message and chaining value, its
-transcript), not on the mix.
+Two ways to run the admitted ones:
+  - serve_batched (continuous batching): a few threads prepare each request
+    (witness, assignment, products), then it joins a PiopEngine that advances
+    every proof in flight by one round per step, the rounds of one kind
+    across all proofs in one launch; a request joins at the next step and
+    leaves when done.
+  - serve: every admitted request runs on its own host thread with its own
+    HIP streams, so the requests' kernels overlap on the device.
+Each result depends only on its request (its circuit, its message and
+chaining value, its transcript), not on the mix or the mode.
 """
 from __future__ import annotations
 
@@ -33,7 +37,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import PreparedConstraintMatrices, SpartanPiopOutput, Transcript, assignment_mle_words, init, prove_spartan_piop
+from . import (PiopEngine, PreparedConstraintMatrices, SpartanPiopOutput, Transcript, assignment_mle_words, init,
+               prove_spartan_piop)
 from . import sha256 as sha
 
 
@@ -126,6 +131,80 @@ class ProvingServer:
         products = c.matrices.r1cs_products(h)
         t = Transcript.duplex(b"bitz/serve/%d" % rid)
         return prove_spartan_piop(t, c.matrices, c.digest, products, h), w.digest
+
+    def prepare(self, r: "Request"):
+        """A request's device inputs: witness (host), assignment, Az/Bz/Cz."""
+        c = self.circuit(r.blocks)
+        w = c.circuit.witness(r.message, iv=r.iv)
+        h = assignment_mle_words(w.h, c.columns, c.columns, device=True)
+        return c, h, c.matrices.r1cs_products(h), w.digest
+
+    def serve_batched(self, requests: list[Request], preparers: int = 4, engines: int | None = None) -> list[Request]:
+        """Continuous batching: admitted requests are prepared (witness,
+        assignment, products) by a few threads, then join a PiopEngine (the
+        least loaded of `engines`, each with its own stream and thread) that
+        advances every proof it holds by one round per step, each round kind
+        in one launch across the proofs.  A request joins at its engine's next
+        step and leaves when its proof is done; admission is by slots and HBM
+        budget as in serve().  engines defaults to one per 8 slots."""
+        engines = engines or max(1, self.slots // 8)
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        for r in pending:
+            self.circuit(r.blocks)
+        pool_engines = [PiopEngine(background=True) for _ in range(engines)]
+        load = [0] * engines
+        by_tag: dict[int, tuple] = {}
+        used = 0
+        self.peak_bytes = self.peak_in_flight = 0
+        t0 = time.perf_counter()
+        lock = threading.Lock()
+
+        def admit_one(r: Request, fp: int) -> None:
+            c, h, products, digest = self.prepare(r)
+            r.digest = digest
+            with lock:
+                k = min(range(engines), key=lambda i: load[i])
+                load[k] += 1
+                by_tag[r.rid] = (r, fp, h, products, k)
+            pool_engines[k].submit(r.rid, Transcript.duplex(b"bitz/serve/%d" % r.rid), c.matrices, c.digest,
+                                   products, h)
+
+        in_flight = 0
+        with ThreadPoolExecutor(max_workers=preparers) as pool:
+            futures = []
+            while pending or in_flight:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now and in_flight < self.slots:
+                    fp = self.circuits[pending[0].blocks].footprint()
+                    if in_flight and used + fp > self.hbm_budget:
+                        break
+                    r = pending.popleft()
+                    r.admitted = now
+                    used += fp
+                    in_flight += 1
+                    futures.append(pool.submit(admit_one, r, fp))
+                    self.peak_bytes = max(self.peak_bytes, used)
+                    self.peak_in_flight = max(self.peak_in_flight, in_flight)
+                for f in [f for f in futures if f.done()]:
+                    f.result()  # a failed preparation raises here
+                    futures.remove(f)
+                timeout = 0.0005
+                if pending and in_flight < self.slots:
+                    timeout = min(timeout, max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                if in_flight and (pool_engines[0].wait(timeout / engines) if engines == 1 else
+                                  sum(e.wait(timeout / engines) for e in pool_engines)):
+                    for k, e in enumerate(pool_engines):
+                        for tag in e.done():
+                            with lock:
+                                r, fp, _, _, _ = by_tag.pop(tag)
+                                load[k] -= 1
+                            r.result = e.take(tag)
+                            r.done = time.perf_counter() - t0
+                            used -= fp
+                            in_flight -= 1
+                elif not in_flight and pending:
+                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+        return requests
 
     def serve(self, requests: list[Request]) -> list[Request]:
         """Runs requests at their arrival times (seconds from now); returns

@@ -301,8 +301,20 @@ int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcri
                                 const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
                                 const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
                                 uint32_t col_vars);
-/* One round of every proof in flight; *in_flight receives how many remain. */
+/* One round of every proof in flight; *in_flight receives how many remain.
+ * step = launch (enqueue the step's work on the engine's stream, no wait;
+ * *launched = proofs in it) then finish (wait, then the host's part).  With
+ * several engines, launching one while finishing another overlaps one's
+ * host work with the other's device work. */
 int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight);
+int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched);
+int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_flight);
+/* Background stepping: a thread of the engine steps while proofs are in
+ * flight (submit, done, take and wait may then be called from other
+ * threads; step/launch/finish may not).  wait blocks until a finished proof
+ * is waiting to be taken or timeout_us passes; *finished = how many wait. */
+int32_t bitz_piop_engine_run(bitz_piop_engine* e);
+int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_us, size_t* finished);
 /* Tags of the finished proofs not yet taken (up to cap; *count = all). */
 int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count);
 /* A finished proof's outputs (as bitz_prove_spartan_piop's); releases it. */

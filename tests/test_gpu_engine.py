@@ -78,6 +78,25 @@ def test_engine_admits_proofs_while_others_run():
         assert same(results[tag], ref[tag]), tag
 
 
+def test_background_engine_with_submitters_on_threads():
+    from concurrent.futures import ThreadPoolExecutor
+    reqs = {tag: request(b, 300 + tag) for tag, b in enumerate([4, 1, 16, 2, 8, 1, 2])}
+    ref = {tag: alone(tag, *r) for tag, r in reqs.items()}
+    eng = G.PiopEngine(background=True)
+    with ThreadPoolExecutor(3) as pool:
+        list(pool.map(lambda t: eng.submit(t, G.Transcript.duplex(b"engine/%d" % t), *reqs[t]), reqs))
+    results = {}
+    for _ in range(2000):
+        if eng.wait(0.01):
+            for tag in eng.done():
+                results[tag] = eng.take(tag)
+        if len(results) == len(reqs):
+            break
+    assert sorted(results) == sorted(reqs)
+    for tag in reqs:
+        assert same(results[tag], ref[tag]), tag
+
+
 def test_engine_errors():
     m, dg, pr, h = request(1, 7)
     eng = G.PiopEngine()

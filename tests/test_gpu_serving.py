@@ -39,6 +39,28 @@ def test_results_are_the_same_alone_and_in_the_mix():
         assert r.done >= r.admitted >= r.arrival - 1e-6
 
 
+def test_batched_serving_gives_the_same_proofs():
+    srv = S.ProvingServer(hbm_budget=1 << 30, slots=6)
+    reqs = S.poisson_requests(rate=5000.0, count=12, seed=6, max_log_blocks=4)
+    alone = [srv.prove(r.rid, r.blocks, r.message, r.iv) for r in reqs]
+    served = srv.serve_batched(reqs)
+    assert srv.peak_in_flight > 1
+    for (proof, digest), r in zip(alone, served):
+        assert r.result is not None and same(proof, r.result)
+        assert r.digest == digest == chain(r.iv, r.message)
+        assert r.done >= r.admitted >= r.arrival - 1e-6
+
+
+def test_batched_admission_keeps_the_budget():
+    srv = S.ProvingServer(hbm_budget=1 << 30, slots=8)
+    reqs = S.poisson_requests(rate=5000.0, count=10, seed=7, max_log_blocks=3)
+    fps = [srv.circuit(r.blocks).footprint() for r in reqs]
+    srv.hbm_budget = 2 * max(fps)
+    srv.serve_batched(reqs)
+    assert srv.peak_bytes <= srv.hbm_budget
+    assert all(r.result is not None for r in reqs)
+
+
 def test_admission_keeps_the_budget():
     srv = S.ProvingServer(hbm_budget=1 << 30, slots=8)
     reqs = S.poisson_requests(rate=5000.0, count=10, seed=4, max_log_blocks=3)
