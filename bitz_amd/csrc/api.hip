@@ -3100,26 +3100,17 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     const size_t of_o = place(o_fold.size() * sizeof(bitz::RoundDesc)), ofm_o = place(o_fold_map.size() * 4);
     const size_t ir_o = place(i_r0.size() * sizeof(bitz::InnerRoundDesc)), irm_o = place(i_r0_map.size() * 4);
     const size_t if_o = place(i_fold.size() * sizeof(bitz::InnerRoundDesc)), ifm_o = place(i_fold_map.size() * 4);
+    // One pinned buffer stages the step's uploads, then receives its sums.
     const size_t total = std::max<size_t>(off, 256);
-    if (e->pin_bytes < total) {
+    const size_t need = std::max(total, slot_bytes);
+    if (e->pin_bytes < need) {
         if (e->pin) (void)hipHostFree(e->pin);
-        e->pin = nullptr;
         e->pin_bytes = 0;
-        if (hipHostMalloc(&e->pin, std::max(total, slot_bytes) * 2, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&e->pin, 2 * need, hipHostMallocDefault) != hipSuccess) {
             e->pin = nullptr;
             return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
         }
-        e->pin_bytes = std::max(total, slot_bytes) * 2;
-    }
-    if (e->pin_bytes < slot_bytes) {
-        (void)hipHostFree(e->pin);
-        e->pin = nullptr;
-        e->pin_bytes = 0;
-        if (hipHostMalloc(&e->pin, slot_bytes * 2, hipHostMallocDefault) != hipSuccess) {
-            e->pin = nullptr;
-            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
-        }
-        e->pin_bytes = slot_bytes * 2;
+        e->pin_bytes = 2 * need;
     }
     if (e->stage.bytes < total) TRY(e->stage.alloc(total));
     char* pin = static_cast<char*>(e->pin);
