@@ -80,12 +80,14 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["outer", "batch", "gkr", "inner", "serve", "bind"], default="outer",
+    ap.add_argument("--workload", choices=["outer", "batch", "gkr", "inner", "serve", "bind", "chain"],
+                    default="outer",
                     help="outer: one 2^log_n outer sumcheck per step (default); batch: 256 outer sumchecks "
                          "of 2^12..2^22 elements per step, batched; gkr: product tree and layered sumcheck "
                          "over 2^22 leaves per step; inner: one inner sumcheck over 2 tables of 2^inner_log_n")
     ap.add_argument("--serve-rate", type=float, default=300.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--chain-blocks", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,
@@ -127,6 +129,8 @@ def main() -> int:
         return bench_inner(G, args)
     if args.workload == "serve":
         return bench_serve(G, args)
+    if args.workload == "chain":
+        return bench_chain(G, args)
     if args.workload == "bind":
         return bench_bind(G, args)
 
@@ -563,6 +567,70 @@ def bench_serve(G, args) -> int:
         "modes": res,
         "one_at_a_time": {"blocks_per_s": blocks / serial, "proofs_per_s": len(reqs) / serial},
         "capacity_vs_one_at_a_time": res[best]["saturated"]["blocks_per_s"] / (blocks / serial),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_sha256(args.seed)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_chain(G, args) -> int:
+    """The sha256-chain statement of the CLI (tooling/cli/src/circuits.rs:
+    167-217) at --chain-blocks blocks (default 64): a padded message from the
+    seed, its rows built and digested once (setup, not timed, as the
+    reference's benchmark prepares its instance); a step is the proof's
+    sumcheck stage: witness (host), assignment MLE, Az/Bz/Cz and
+    prove_spartan_piop under the duplex transcript (one host round trip per
+    round)."""
+    import struct
+    from bitz_amd import sha256 as SH
+    G.init(0)
+    B = args.chain_blocks
+    rng = np.random.default_rng(args.seed)
+    raw = rng.integers(0, 256, 64 * B - 9, dtype=np.uint8).tobytes()
+    msg = raw + b"\x80" + struct.pack(">Q", 8 * len(raw))
+    c = SH.Sha256Circuit(SH.CHAIN, B, statement=msg)
+    t0 = time.perf_counter()
+    matrices, digest = c.prepare()
+    setup = time.perf_counter() - t0
+
+    def step(i: int) -> float:
+        t0 = time.perf_counter()
+        w = c.witness(msg)
+        h = G.assignment_mle_words(w.h, c.columns, c.columns, device=True)
+        products = matrices.r1cs_products(h)
+        G.prove_spartan_piop(G.Transcript.duplex(b"chain/%d" % i), matrices, digest, products, h)
+        return time.perf_counter() - t0
+
+    for i in range(args.warmup):
+        step(i)
+    G.synchronize()
+    times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        times.append(step(1000 + i))
+    G.synchronize()
+    wall = time.perf_counter() - t_start
+    lat = np.array(times) * 1e3
+    line = {
+        "metric": METRIC,
+        "value": B * args.steps / wall,
+        "unit": "SHA-256 blocks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (message bytes from the seed, padded as SHA-256 pads)",
+        "config": {"workload": f"sha256-chain statement, {B} blocks, one proof at a time: witness, assignment MLE, "
+                               "Az/Bz/Cz, prove_spartan_piop (duplex transcript)",
+                   "blocks": B, "rows": c.rows, "columns": c.columns, "nnz": sum(c.nnz),
+                   "row_vars": c.num_row_vars, "col_vars": c.num_column_vars, "setup_s": setup},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_sha256(args.seed)

@@ -97,6 +97,47 @@ def test_background_engine_with_submitters_on_threads():
         assert same(results[tag], ref[tag]), tag
 
 
+def test_engine_with_callback_transcripts():
+    """The caller's own transcript (callbacks, as the Rust side's spongefish
+    would be) called from the engine's thread: the same messages and
+    challenges as prove_spartan_piop under the same callbacks."""
+    import hashlib
+
+    class Sponge:
+        def __init__(self, label):
+            self.state = hashlib.sha256(label).digest()
+            self.log = []
+
+        def absorb(self, kind, data):
+            self.log.append((kind, data))
+            self.state = hashlib.sha256(self.state + bytes([kind]) + data).digest()
+
+        def squeeze(self):
+            self.state = hashlib.sha256(self.state + b"sq").digest()
+            return self.state[:16]
+
+    reqs = {tag: request(b, 400 + tag) for tag, b in enumerate([2, 8, 1])}
+    ref, ref_logs = {}, {}
+    for tag, r in reqs.items():
+        sp = Sponge(b"cb/%d" % tag)
+        ref[tag] = G.prove_spartan_piop(G.Transcript.callbacks(sp.absorb, sp.squeeze), *r)
+        ref_logs[tag] = sp.log
+    eng = G.PiopEngine(background=True)
+    sponges = {tag: Sponge(b"cb/%d" % tag) for tag in reqs}
+    for tag, r in reqs.items():
+        eng.submit(tag, G.Transcript.callbacks(sponges[tag].absorb, sponges[tag].squeeze), *r)
+    results = {}
+    for _ in range(2000):
+        if eng.wait(0.01):
+            for tag in eng.done():
+                results[tag] = eng.take(tag)
+        if len(results) == len(reqs):
+            break
+    for tag in reqs:
+        assert same(results[tag], ref[tag]), tag
+        assert sponges[tag].log == ref_logs[tag]
+
+
 def test_engine_errors():
     m, dg, pr, h = request(1, 7)
     eng = G.PiopEngine()
