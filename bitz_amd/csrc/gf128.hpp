@@ -357,15 +357,17 @@ __device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
             for (int i = 7; i >= 1; --i) acc.w[i] = shl_pair(acc.w[i], acc.w[i - 1], 4);
             acc.w[0] <<= 4;
         }
+        // Entry w lands on words w .. w+3: 16 words, XORed in 10 ops.
+        uint4 e[4];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t nib = __builtin_amdgcn_ubfe(b.w[w], 4 * k, 4);
-            const uint4 e = col[64 * nib];
-            acc.w[w] ^= e.x;
-            acc.w[w + 1] ^= e.y;
-            acc.w[w + 2] ^= e.z;
-            acc.w[w + 3] ^= e.w;
-        }
+        for (int w = 0; w < 4; ++w) e[w] = col[64 * __builtin_amdgcn_ubfe(b.w[w], 4 * k, 4)];
+        acc.w[0] ^= e[0].x;
+        acc.w[1] = xor3(acc.w[1], e[0].y, e[1].x);
+        acc.w[2] = xor3(xor3(acc.w[2], e[0].z, e[1].y), e[2].x, 0u);
+        acc.w[3] = xor3(xor3(acc.w[3], e[0].w, e[1].z), e[2].y, e[3].x);
+        acc.w[4] = xor3(xor3(acc.w[4], e[1].w, e[2].z), e[3].y, 0u);
+        acc.w[5] = xor3(acc.w[5], e[2].w, e[3].z);
+        acc.w[6] ^= e[3].w;
     }
     const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 29, 1);
     const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 30, 1);
