@@ -803,7 +803,7 @@ struct PipeSchedule {
 };
 static PipeSchedule pipe_schedule(int n, int tail_log = kTailLog) {
     const int k0 = std::max(0, n - tail_log);
-    const int kc = std::max(1, n - 1 - bitz::kThreadBits);  // rounds before kc iterate over their pairs
+    const int kc = std::max(1, n - 1 - bitz::kChunkFromBits);  // rounds before kc run grid-wide
     if (k0 - kc >= 1) return {kc, k0};
     return {k0, k0};
 }
@@ -970,7 +970,7 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab4, bool q1
         // most 256 threads (whole waves) so small rounds spread over the CUs.
         const unsigned block = (unsigned)std::min<size_t>(256, (g.threads + 63) / 64 * 64);
         const unsigned grid = (unsigned)((g.threads + block - 1) / block);
-        const size_t smem = sizeof(uint4) * (2 * kConstTable4Entries + (size_t)(block / 64) * 64 * kLaneTableEntries);
+        const size_t smem = bitz::fold_smem(block);
 #define OUTER_FOLD1_ARGS                                                                                   \
     h->cur[0], h->cur[1], h->cur[2], out[0], out[1], out[2], g.threads, g.iters, rtab4, la, g.la_bits, la_shift, \
         lb, lb_shift, h4, slot
@@ -1703,7 +1703,7 @@ int32_t batch_member(bitz_outer** out, const bitz_outer_instance& in, const Batc
 // grid-wide rounds (larger rounds iterate, weighing each iteration through
 // its H(i) table) and the size of the one-workgroup tail.
 constexpr BatchConfig kBatchDefault = {bitz::kThreadBits, kTailLog};
-constexpr BatchConfig kBatchCandidates[] = {{17, 10}, {16, 10}, {15, 10}, {14, 10}, {17, 9}, {16, 9}, {15, 9}};
+constexpr BatchConfig kBatchCandidates[] = {{18, 10}, {17, 10}, {16, 10}, {15, 10}, {18, 9}, {17, 9}, {16, 9}};
 
 // Everything a batch of a given shape needs, built once and reused while
 // the shapes repeat: member handles (geometry), buffer layout, device

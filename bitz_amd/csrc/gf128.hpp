@@ -290,77 +290,96 @@ __device__ __forceinline__ void build_const_table4(uint4* tab, uint4* basis, con
     __syncthreads();
 }
 
-// c * x through c's 4-bit table.
+// LDS accesses through 32-bit LDS byte addresses, so lookups can build
+// their addresses with one VALU instruction each.
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(size_t)(const __attribute__((address_space(3))) char*)p;
+}
+__device__ __forceinline__ uint4 lds_ld16(uint32_t addr) {
+    const u32x4 v = *(const lds_u32x4*)(size_t)addr;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void lds_st16(uint32_t addr, const g4& x) {
+    u32x4 v;
+    v.x = x.w[0]; v.y = x.w[1]; v.z = x.w[2]; v.w = x.w[3];
+    *(lds_u32x4*)(size_t)addr = v;
+}
+
+// c * x through c's 4-bit table at `tab` (LDS, 256-byte aligned).  Window k
+// (nibble k of x) reads tab + 256 k + 16 n: the masked words hold 16 n in
+// byte j (even nibbles of a word from x << 4, odd ones from x), and one
+// v_perm_b32 puts that byte under the table's upper address bytes; 256 k
+// goes into the instruction's offset.
 __device__ __forceinline__ g4 g_mul_table4(const uint4* tab, const g4& x) {
+    const uint32_t tb = lds_addr(tab);
     g4 r = g_zero();
 #pragma unroll
-    for (int k = 0; k < 32; k += 2) {
-        const uint32_t n0 = __builtin_amdgcn_ubfe(x.w[k >> 3], 4 * (k & 7), 4);
-        const uint32_t n1 = __builtin_amdgcn_ubfe(x.w[(k + 1) >> 3], 4 * ((k + 1) & 7), 4);
-        const uint4 e0 = tab[16 * k + n0], e1 = tab[16 * (k + 1) + n1];
-        r.w[0] = xor3(r.w[0], e0.x, e1.x);
-        r.w[1] = xor3(r.w[1], e0.y, e1.y);
-        r.w[2] = xor3(r.w[2], e0.z, e1.z);
-        r.w[3] = xor3(r.w[3], e0.w, e1.w);
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t hi = x.w[w] & 0xF0F0F0F0u, lo = (x.w[w] << 4) & 0xF0F0F0F0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k0 = 8 * w + 2 * j;
+            const uint4 e0 = lds_ld16(__builtin_amdgcn_perm(tb, lo, 0x07060500u | j) + 256 * k0);
+            const uint4 e1 = lds_ld16(__builtin_amdgcn_perm(tb, hi, 0x07060500u | j) + 256 * (k0 + 1));
+            r.w[0] = xor3(r.w[0], e0.x, e1.x);
+            r.w[1] = xor3(r.w[1], e0.y, e1.y);
+            r.w[2] = xor3(r.w[2], e0.z, e1.z);
+            r.w[3] = xor3(r.w[3], e0.w, e1.w);
+        }
     }
     return r;
 }
 
 // ------------------------------------- general product through LDS ---
-// a * b with a per-lane 4-bit window table of a in LDS: the table holds
-// T[v] = a' v (v < 16) for a' = a without its top three bits (so every entry
-// fits 128 bits), and b is consumed one nibble per lookup, most significant
-// nibble position first, shifting the 256-bit accumulator by 4 between
-// positions.  a's top bits (125..127) add (top * b) X^125.  `col` is this
-// lane's column: entry v at col[64 v] (entry-major, one wave of 64 lanes per
-// row), so a wave's lookups of any entries are one conflict-free
-// ds_read_b128 each.  Entry 0 must hold zero (written once by the caller).
-constexpr int kLaneTableEntries = 16;  // per lane; a wave's table is 16 KiB
+// a * b with a per-lane 3-bit window table of a in LDS: T[v] = a' v (v < 8)
+// for a' = a without its top two bits (so every entry fits 128 bits); b is
+// consumed three bits per lookup (eleven positions per word, the top one
+// two bits wide), most significant position first, shifting the 256-bit
+// accumulator by 3 between positions.  a's bits 126, 127 add
+// (top * b) X^126.  `col` is this lane's column: entry v at col[64 v]
+// (entry-major, one wave of 64 lanes per row), so a wave's lookups are one
+// conflict-free ds_read_b128 each.  Entry 0 must hold zero (written once by
+// the caller).  Three-bit windows keep a wave's table at 8 KiB, so twice as
+// many waves fit in LDS as with four-bit windows, for 7 table writes (a
+// ds_write_b128 costs three reads) and 44 lookups per product.
+constexpr int kLaneTableEntries = 8;  // per lane; a wave's table is 8 KiB
 
 __device__ __forceinline__ void lane_table_clear(uint4* col) { col[0] = make_uint4(0, 0, 0, 0); }
 
 __device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
+    const uint32_t ca = lds_addr(col);
     g4 al = a;
-    al.w[3] &= 0x1FFFFFFFu;
-    g4 s1, s2, s3;
+    al.w[3] &= 0x3FFFFFFFu;
+    g4 s1, s2;
     s1.w[0] = al.w[0] << 1;
     s2.w[0] = al.w[0] << 2;
-    s3.w[0] = al.w[0] << 3;
 #pragma unroll
     for (int i = 1; i < 4; ++i) {
         s1.w[i] = shl_pair(al.w[i], al.w[i - 1], 1);
         s2.w[i] = shl_pair(al.w[i], al.w[i - 1], 2);
-        s3.w[i] = shl_pair(al.w[i], al.w[i - 1], 3);
     }
-    auto put = [&](int v, const g4& x) { col[64 * v] = make_uint4(x.w[0], x.w[1], x.w[2], x.w[3]); };
-    put(1, al);
-    put(2, s1);
-    put(4, s2);
-    put(8, s3);
-    const g4 t3 = g_add(al, s1), t5 = g_add(al, s2), t6 = g_add(s1, s2), t7 = g_add(t3, s2);
-    put(3, t3);
-    put(5, t5);
-    put(6, t6);
-    put(7, t7);
-    put(9, g_add(al, s3));
-    put(10, g_add(s1, s3));
-    put(11, g_add(t3, s3));
-    put(12, g_add(s2, s3));
-    put(13, g_add(t5, s3));
-    put(14, g_add(t6, s3));
-    put(15, g_add(t7, s3));
+    const g4 t3 = g_add(al, s1);
+    lds_st16(ca + 1024 * 1, al);
+    lds_st16(ca + 1024 * 2, s1);
+    lds_st16(ca + 1024 * 3, t3);
+    lds_st16(ca + 1024 * 4, s2);
+    lds_st16(ca + 1024 * 5, g_add(al, s2));
+    lds_st16(ca + 1024 * 6, g_add(s1, s2));
+    lds_st16(ca + 1024 * 7, g_add(t3, s2));
     g8 acc = w_zero();
 #pragma unroll
-    for (int k = 7; k >= 0; --k) {
-        if (k != 7) {
+    for (int k = 10; k >= 0; --k) {
+        if (k != 10) {
 #pragma unroll
-            for (int i = 7; i >= 1; --i) acc.w[i] = shl_pair(acc.w[i], acc.w[i - 1], 4);
-            acc.w[0] <<= 4;
+            for (int i = 7; i >= 1; --i) acc.w[i] = shl_pair(acc.w[i], acc.w[i - 1], 3);
+            acc.w[0] <<= 3;
         }
         // Entry w lands on words w .. w+3: 16 words, XORed in 10 ops.
         uint4 e[4];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) e[w] = col[64 * __builtin_amdgcn_ubfe(b.w[w], 4 * k, 4)];
+        for (int w = 0; w < 4; ++w)
+            e[w] = lds_ld16((__builtin_amdgcn_ubfe(b.w[w], 3 * k, k == 10 ? 2 : 3) << 10) + ca);
         acc.w[0] ^= e[0].x;
         acc.w[1] = xor3(acc.w[1], e[0].y, e[1].x);
         acc.w[2] = xor3(xor3(acc.w[2], e[0].z, e[1].y), e[2].x, 0u);
@@ -369,17 +388,16 @@ __device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
         acc.w[5] = xor3(acc.w[5], e[2].w, e[3].z);
         acc.w[6] ^= e[3].w;
     }
-    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 29, 1);
-    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 30, 1);
-    const uint32_t m2 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 31, 1);
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 30, 1);
+    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 31, 1);
     uint32_t u[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         const uint32_t bi = i < 4 ? b.w[i] : 0u, bp = i > 0 ? b.w[i - 1] : 0u;
-        u[i] = xor_and(xor_and(bi & m0, shl_pair(bi, bp, 1), m1), shl_pair(bi, bp, 2), m2);
+        u[i] = xor_and(bi & m0, shl_pair(bi, bp, 1), m1);
     }
 #pragma unroll
-    for (int i = 0; i < 5; ++i) acc.w[3 + i] ^= __builtin_amdgcn_alignbit(u[i], i ? u[i - 1] : 0u, 3);
+    for (int i = 0; i < 5; ++i) acc.w[3 + i] ^= __builtin_amdgcn_alignbit(u[i], i ? u[i - 1] : 0u, 2);
     return g_reduce(acc);
 }
 #endif  // __HIPCC__

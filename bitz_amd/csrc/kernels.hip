@@ -388,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_assignment_bits(const uint64_t* word
 __global__ __launch_bounds__(kStageBlock) void k_bind_products(const uint32_t* rows, const uint8_t* which,
                                                                const f128* values, const f128* row_scale,
                                                                size_t rows_padded, f128* prod, size_t nnz) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
 #pragma unroll 1
@@ -597,7 +597,7 @@ __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, cons
                                               size_t iters, const f128* la, int la_bits, int la_shift,
                                               const f128* lb, int lb_shift, const uint4* h_tabs4, uint32_t* acc,
                                               size_t block) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* htab = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
@@ -640,7 +640,7 @@ __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, cons
     block_xor_to_global<3>(v, acc);
 }
 template <bool Q1>
-__global__ __launch_bounds__(256) void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
+__global__ __launch_bounds__(256) BITZ_FOUR_WAVES void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
                                                   size_t iters, const f128* la, int la_bits, int la_shift,
                                                   const f128* lb, int lb_shift, const uint4* h_tabs4,
                                                   uint32_t* acc) {
@@ -656,28 +656,29 @@ template __global__ void k_outer_r0<true>(OUTER_R0_SIG);
 
 // Rounds k >= 1 with more pairs than threads: the fold by r_{k-1} through
 // r's 4-bit table, the general products through per-lane LDS tables
-// (g_mul_lane) and H(i) through 4-bit tables.  512-thread workgroups: eight
-// waves x 16 KiB of lane tables plus the two 8 KiB constant tables.
+// (g_mul_lane) and H(i) through 4-bit tables.  Workgroups of up to 1024
+// threads: each wave's 8 KiB of lane tables first, then r's table and two
+// H(i) tables, iteration i filling buffer i & 1 (its readers of iteration
+// i - 2 have all passed the barrier of iteration i - 1), so one barrier per
+// iteration suffices.
 template <bool Q1>
 __device__ __forceinline__ void outer_fold_body(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo,
                                                 f128* co, size_t threads, size_t iters, const uint4* r_tab4,
                                                 const f128* la, int la_bits, int la_shift, const f128* lb,
                                                 int lb_shift, const uint4* h_tabs4, uint32_t* acc, size_t block) {
-    extern __shared__ uint4 smem_raw[];
-    uint4* rt = smem_raw;
-    uint4* ht = smem_raw + kConstTable4Entries;
-    uint4* col = smem_raw + 2 * kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
-                 (threadIdx.x & 63);
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    const int waves = blockDim.x >> 6;
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    uint4* rt = smem_raw + waves * (64 * kLaneTableEntries);
     lane_table_clear(col);
     for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x) rt[e] = u4_load_hbm(r_tab4 + e);
     const size_t t = block * blockDim.x + threadIdx.x;
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {
-        if (iters > 1) {
-            __syncthreads();  // the previous iteration's readers are done with ht
+        uint4* ht = rt + ((i & 1) + 1) * kConstTable4Entries;
+        if (iters > 1)
             for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
                 ht[e] = u4_load_hbm(h_tabs4 + i * kConstTable4Entries + e);
-        }
         __syncthreads();
         if (t < threads) {
             const size_t y = i * threads + t;
@@ -727,7 +728,7 @@ __device__ __forceinline__ void outer_fold_body(const f128* a, const f128* b, co
     block_xor_to_global<3>(v, acc);
 }
 template <bool Q1>
-__global__ __launch_bounds__(512) void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao,
+__global__ __launch_bounds__(kFoldBlock) void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao,
                                                     f128* bo, f128* co, size_t threads, size_t iters,
                                                     const uint4* r_tab4, const f128* la, int la_bits,
                                                     int la_shift, const f128* lb, int lb_shift,
@@ -948,8 +949,8 @@ __device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, 
 // tables (2^cap_lg entries each), the weight tables (2^cap_lg entries) and
 // 16 KiB of lane tables per wave.
 __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
-    __shared__ uint4 rt[kConstTable4Entries];
-    extern __shared__ uint4 lds_raw[];
+    __shared__ __attribute__((aligned(256))) uint4 rt[kConstTable4Entries];
+    extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];
     const int cap_lg = A.lg_in - A.fold_first;
     const int pl0 = cap_lg - 1;
     uint4* wt = lds_raw + ((size_t)3 << cap_lg);
@@ -1060,7 +1061,7 @@ __global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, c
 // (sumcheck.rs:568-609), products through lane tables.
 __device__ __forceinline__ void inner_r0_body(const f128* d, const f128* w, size_t pairs, uint32_t* acc,
                                               size_t block, size_t blocks) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     g4 s0 = g_zero(), s2 = g_zero();
@@ -1087,7 +1088,7 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, co
 __device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f128* dout, f128* wout,
                                                 size_t out_pairs, const uint4* rtab4, uint32_t* acc, size_t block,
                                                 size_t blocks) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
@@ -1128,7 +1129,7 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_batch(const InnerRound
     const InnerRoundDesc A = d[bmap[blockIdx.x]];
     inner_r0_body(A.d, A.w, A.pairs, A.acc, blockIdx.x - A.block_base, A.blocks);
 }
-__global__ __launch_bounds__(kInnerBlock) void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
+__global__ __launch_bounds__(kInnerBlock) BITZ_FOUR_WAVES void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
     const InnerRoundDesc A = d[bmap[blockIdx.x]];
     inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks);
 }
@@ -1137,7 +1138,7 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_fold_batch(const InnerRou
 // load when fold_first) in LDS, each round's sums stored at acc + 8 k, then a
 // fold by r_k; D(r) and h(r) at fin.
 __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
-    extern __shared__ uint4 lds_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];
     constexpr size_t kCap = (size_t)1 << kInnerTailMaxLog;
     uint4* Ds = lds_raw;
     uint4* Ws = Ds + kCap;
@@ -1218,7 +1219,7 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
 // ================================ GKR layers in parallel (known challenges) ===
 __global__ __launch_bounds__(kStageBlock) void k_gkr_eq_items(const GkrEqItem* items) {
     const GkrEqItem& A = items[blockIdx.x];
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     f128* table = A.out;
@@ -1240,7 +1241,7 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_eq_items(const GkrEqItem* i
 
 __global__ __launch_bounds__(kStageBlock) void k_gkr_eq_combine_batch(const GkrCombine* d, const uint32_t* bmap) {
     const GkrCombine A = d[bmap[blockIdx.x]];
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     const size_t mask = ((size_t)1 << A.low_bits) - 1;
@@ -1270,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_bittable_leaves(const uint64_t* pack
 // next[k] = cur[k] * cur[k + half] (fold_layers, crates/gkr/src/lib.rs:874-894).
 __global__ __launch_bounds__(kStageBlock) void k_gkr_layer_lane(const f128* cur, f128* cur_copy, f128* next,
                                                                 size_t half) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
 #pragma unroll 1
@@ -1289,7 +1290,7 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_layer_lane(const f128* cur,
 // halved in place (thread k reads k and k + half, writes k); each new layer is
 // also written right after the previous one in the store.
 __global__ __launch_bounds__(kStageBlock) void k_gkr_layers_small(f128* layer, int log_len, int count) {
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* T = smem_raw;
     uint4* col = smem_raw + ((size_t)1 << kGkrSmallTreeLog) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) +
                  (threadIdx.x & 63);
@@ -1323,7 +1324,7 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_layers_small(f128* layer, i
 
 __global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
     const GkrRound A = d[bmap[blockIdx.x]];
-    extern __shared__ uint4 smem_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
@@ -1376,7 +1377,7 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound*
 // the layer's first round sums without binding.
 __global__ __launch_bounds__(kStageBlock) void k_gkr_tail(const GkrTail* d) {
     const GkrTail A = d[blockIdx.x];
-    extern __shared__ uint4 lds_raw[];
+    extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];
     constexpr size_t kArr = (size_t)4 << kGkrTailMax;
     uint4* Ls = lds_raw;
     uint4* Rs = Ls + kArr;

@@ -92,9 +92,13 @@ template <bool Q1>
 __global__ void k_outer_fold(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
                              size_t threads, size_t iters, const uint4* r_tab4, const f128* la, int la_bits,
                              int la_shift, const f128* lb, int lb_shift, const uint4* h_tabs4, uint32_t* acc);
-constexpr int kFoldBlock = 512;
-constexpr size_t kFoldSmem =
-    sizeof(uint4) * (2 * kConstTable4Entries + (kFoldBlock / 64) * 64 * kLaneTableEntries);
+constexpr int kFoldBlock = 1024;
+// LDS of k_outer_fold / k_batch_fold with `block` threads: the lane tables,
+// r's 4-bit table and two H(i) tables.
+constexpr size_t fold_smem(size_t block) {
+    return sizeof(uint4) * (3 * kConstTable4Entries + (block / 64) * 64 * kLaneTableEntries);
+}
+constexpr size_t kFoldSmem = fold_smem(kFoldBlock);
 
 template <bool Q1>
 __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads, size_t iters,
@@ -103,7 +107,15 @@ __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t t
 // Round 0 with lane tables: 256 threads, 4 waves x 16 KiB + one 4-bit table.
 constexpr int kR0Block = 256;
 constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
-constexpr int kThreadBits = 17;  // threads of a multi-iteration round: one wave of lane-table workgroups
+// Threads of a multi-iteration round: two 512-thread workgroups per CU
+// (each 80 KiB of LDS) across the chip.
+constexpr int kThreadBits = 18;
+// Rounds whose pair count exceeds 2^kChunkFromBits run grid-wide; the
+// following ones go to the chunked LDS stage.
+constexpr int kChunkFromBits = 17;
+// Occupancy target of the lane-table kernels: four waves per SIMD (at most
+// 128 VGPRs), which their LDS use also allows.
+#define BITZ_FOUR_WAVES __attribute__((amdgpu_waves_per_eu(4)))
 
 // ---- rounds held in LDS: chunk, tail and batch stages -------------------
 // One stage of one proof: rounds [k_first, k_first + s) on chunks of 2^lg_in
