@@ -587,25 +587,27 @@ __global__ __launch_bounds__(1024) void k_prologue(PrologueArgs A) {
 }
 
 // Round 0 of the eq-factored outer sumcheck with the general products
-// through per-lane LDS tables (g_mul_lane): 256-thread workgroups, each
-// wave owning 16 KiB of lane tables, H(i) through a 4-bit constant table
-// (8 KiB) copied per iteration.  Thread t handles pairs y = i * threads + t
-// for iterations i (the iteration index i is the high part of y, so H(i) is
-// one constant per iteration and the thread's low part t fixes L(t)).
+// through per-lane LDS tables (g_mul_lane): workgroups of up to 1024
+// threads, each wave's 8 KiB of lane tables first, then two H(i) 4-bit
+// tables that iterations fill alternately (one barrier per iteration, as in
+// outer_fold_body).  Thread t handles pairs y = i * threads + t for
+// iterations i (the iteration index i is the high part of y, so H(i) is one
+// constant per iteration and the thread's low part t fixes L(t)).
 template <bool Q1>
 __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, const f128* c, size_t threads,
                                               size_t iters, const f128* la, int la_bits, int la_shift,
                                               const f128* lb, int lb_shift, const uint4* h_tabs4, uint32_t* acc,
                                               size_t block) {
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
-    uint4* htab = smem_raw;
-    uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    const int waves = blockDim.x >> 6;
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    uint4* hbase = smem_raw + waves * (64 * kLaneTableEntries);
     lane_table_clear(col);
     const size_t t = block * blockDim.x + threadIdx.x;
     g4 s0 = g_zero(), si = g_zero(), s1 = g_zero();
     for (size_t i = 0; i < iters; ++i) {
+        uint4* htab = hbase + (i & 1) * kConstTable4Entries;
         if (iters > 1) {
-            __syncthreads();  // the previous iteration's readers are done with htab
             for (int e = threadIdx.x; e < kConstTable4Entries; e += blockDim.x)
                 htab[e] = u4_load_hbm(h_tabs4 + i * kConstTable4Entries + e);
             __syncthreads();
@@ -640,7 +642,7 @@ __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, cons
     block_xor_to_global<3>(v, acc);
 }
 template <bool Q1>
-__global__ __launch_bounds__(256) BITZ_FOUR_WAVES void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
+__global__ __launch_bounds__(kR0Block) void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads,
                                                   size_t iters, const f128* la, int la_bits, int la_shift,
                                                   const f128* lb, int lb_shift, const uint4* h_tabs4,
                                                   uint32_t* acc) {

@@ -104,9 +104,9 @@ template <bool Q1>
 __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t threads, size_t iters,
                            const f128* la, int la_bits, int la_shift, const f128* lb, int lb_shift,
                            const uint4* h_tabs4, uint32_t* acc);
-// Round 0 with lane tables: 256 threads, 4 waves x 16 KiB + one 4-bit table.
-constexpr int kR0Block = 256;
-constexpr size_t kR0Smem = sizeof(uint4) * (kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
+// Round 0 with lane tables: 1024 threads, 16 waves x 8 KiB + two 4-bit tables.
+constexpr int kR0Block = 1024;
+constexpr size_t kR0Smem = sizeof(uint4) * (2 * kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
 // Threads of a multi-iteration round: two 512-thread workgroups per CU
 // (each 80 KiB of LDS) across the chip.
 constexpr int kThreadBits = 18;
