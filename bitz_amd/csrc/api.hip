@@ -299,6 +299,7 @@ struct Timed {
     hipEvent_t start = nullptr, stop = nullptr;
     const char* name = nullptr;
     double bytes = 0;
+    uint64_t launches = 1;  // back-to-back launches between the two events
     Timed() = default;
     Timed(const Timed&) = delete;
     Timed& operator=(const Timed&) = delete;
@@ -330,7 +331,7 @@ struct Timed {
             KernelStat& k = g_prof.stats[name];
             k.ms += ms;
             k.bytes += bytes;
-            k.launches += 1;
+            k.launches += launches;
         }
         name = nullptr;
     }
@@ -930,14 +931,15 @@ static bool outer_eq_constants(bitz_outer* h, const std::vector<f128>& tau) {
 // Launches the eq-factored round-k kernel into slot k: round 0 through
 // k_outer_r0, the fold rounds through k_outer_fold (lane-table products,
 // folding with the 4-bit table rtab4 of r_{k-1}).
+// With `timed` false the launch is left to a span timer of the caller.
 static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab4, bool q1,
-                               f128* const out[3], Timed* tm = nullptr) {
+                               f128* const out[3], Timed* tm = nullptr, bool timed = true) {
     if (!tm) tm = &h->tm;
     const auto& g = h->geo[k];
     uint32_t* slot = h->acc.u32() + 12 * k;
     hipStream_t s = h->so.s;
     const size_t pairs = g.threads * g.iters;
-    tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 96.0 * pairs : 288.0 * pairs);
+    if (timed) tm->begin(s, k == 0 ? "outer_round0" : "outer_fold_next", k == 0 ? 96.0 * pairs : 288.0 * pairs);
     const uint4* h4 = static_cast<const uint4*>(h->htabs4.p) + g.htab_off * kConstTable4Entries;
     const f128* la = h->table_ptr(g.la.table);
     const f128* lb = h->table_ptr(g.lb.table);
@@ -980,7 +982,7 @@ static int32_t outer_eq_launch(bitz_outer* h, int k, const uint4* rtab4, bool q1
             hipLaunchKernelGGL((bitz::k_outer_fold<false>), dim3(grid), dim3(block), smem, s, OUTER_FOLD1_ARGS);
 #undef OUTER_FOLD1_ARGS
     }
-    tm->end(s);
+    if (timed) tm->end(s);
     return check_launch();
 }
 
@@ -1417,11 +1419,20 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         h->where = (h->where == 1) ? 2 : 1;
         h->len = new_len;
     };
+    // The grid-wide fold rounds run back to back: one event pair spans them
+    // (two event records per proof instead of two per launch).
+    double fold_bytes = 0;
+    for (int k = 1; k < ps.kc; ++k) fold_bytes += 288.0 * h->geo[k].threads * h->geo[k].iters;
     for (int k = 0; k < ps.kc; ++k) {
         f128* out[3] = {nullptr, nullptr, nullptr};
         if (k > 0) next_out(out);
+        if (k == 1) {
+            timers[1].begin(s, "outer_fold_next", fold_bytes);
+            timers[1].launches = (uint64_t)(ps.kc - 1);
+        }
         TRY(outer_eq_launch(h, k, k > 0 ? rtabs4 + (size_t)(k - 1) * kConstTable4Entries : nullptr,
-                            (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k]));
+                            (q1mask >> k) & 1, k > 0 ? out : nullptr, &timers[k], k == 0));
+        if (k == ps.kc - 1 && k > 0) timers[1].end(s);
         if (k > 0) advance(out, h->len / 2);
     }
     if (ps.kc < ps.k0) {
