@@ -400,4 +400,69 @@ __device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
     for (int i = 0; i < 5; ++i) acc.w[3 + i] ^= __builtin_amdgcn_alignbit(u[i], i ? u[i - 1] : 0u, 2);
     return g_reduce(acc);
 }
+
+// a * b and a * c with one table of a: the two Horner chains run
+// interleaved over the shared table.
+__device__ __forceinline__ void g_mul_lane2(const g4& a, const g4& b, const g4& c, uint4* col, g4& ab, g4& ac) {
+    const uint32_t ca = lds_addr(col);
+    g4 al = a;
+    al.w[3] &= 0x3FFFFFFFu;
+    g4 s1, s2;
+    s1.w[0] = al.w[0] << 1;
+    s2.w[0] = al.w[0] << 2;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        s1.w[i] = shl_pair(al.w[i], al.w[i - 1], 1);
+        s2.w[i] = shl_pair(al.w[i], al.w[i - 1], 2);
+    }
+    const g4 t3 = g_add(al, s1);
+    lds_st16(ca + 1024 * 1, al);
+    lds_st16(ca + 1024 * 2, s1);
+    lds_st16(ca + 1024 * 3, t3);
+    lds_st16(ca + 1024 * 4, s2);
+    lds_st16(ca + 1024 * 5, g_add(al, s2));
+    lds_st16(ca + 1024 * 6, g_add(s1, s2));
+    lds_st16(ca + 1024 * 7, g_add(t3, s2));
+    g8 acc[2] = {w_zero(), w_zero()};
+    const g4* ops[2] = {&b, &c};
+#pragma unroll
+    for (int k = 10; k >= 0; --k) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            g8& A = acc[q];
+            if (k != 10) {
+#pragma unroll
+                for (int i = 7; i >= 1; --i) A.w[i] = shl_pair(A.w[i], A.w[i - 1], 3);
+                A.w[0] <<= 3;
+            }
+            uint4 e[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+                e[w] = lds_ld16((__builtin_amdgcn_ubfe(ops[q]->w[w], 3 * k, k == 10 ? 2 : 3) << 10) + ca);
+            A.w[0] ^= e[0].x;
+            A.w[1] = xor3(A.w[1], e[0].y, e[1].x);
+            A.w[2] = xor3(xor3(A.w[2], e[0].z, e[1].y), e[2].x, 0u);
+            A.w[3] = xor3(xor3(A.w[3], e[0].w, e[1].z), e[2].y, e[3].x);
+            A.w[4] = xor3(xor3(A.w[4], e[1].w, e[2].z), e[3].y, 0u);
+            A.w[5] = xor3(A.w[5], e[2].w, e[3].z);
+            A.w[6] ^= e[3].w;
+        }
+    }
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 30, 1);
+    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)a.w[3], 31, 1);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const g4& bb = *ops[q];
+        uint32_t u[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const uint32_t bi = i < 4 ? bb.w[i] : 0u, bp = i > 0 ? bb.w[i - 1] : 0u;
+            u[i] = xor_and(bi & m0, shl_pair(bi, bp, 1), m1);
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) acc[q].w[3 + i] ^= __builtin_amdgcn_alignbit(u[i], i ? u[i - 1] : 0u, 2);
+    }
+    ab = g_reduce(acc[0]);
+    ac = g_reduce(acc[1]);
+}
 #endif  // __HIPCC__

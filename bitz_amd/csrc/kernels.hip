@@ -635,8 +635,7 @@ __device__ __forceinline__ void outer_r0_body(const f128* a, const f128* b, cons
     if (t < threads) {
         const size_t mask = ((size_t)1 << la_bits) - 1;
         const g4 L = g_mul_lane(g_load_hbm(la + ((t & mask) << la_shift)), g_load_hbm(lb + ((t >> la_bits) << lb_shift)), col);
-        v[0] = g_mul_lane(L, s0, col);
-        v[1] = g_mul_lane(L, si, col);
+        g_mul_lane2(L, s0, si, col, v[0], v[1]);
         if (Q1) v[2] = g_mul_lane(L, s1, col);
     }
     block_xor_to_global<3>(v, acc);
@@ -723,8 +722,7 @@ __device__ __forceinline__ void outer_fold_body(const f128* a, const f128* b, co
         const size_t mask = ((size_t)1 << la_bits) - 1;
         const g4 L =
             g_mul_lane(g_load_hbm(la + ((t & mask) << la_shift)), g_load_hbm(lb + ((t >> la_bits) << lb_shift)), col);
-        v[0] = g_mul_lane(L, s0, col);
-        v[1] = g_mul_lane(L, si, col);
+        g_mul_lane2(L, s0, si, col, v[0], v[1]);
         if (Q1) v[2] = g_mul_lane(L, s1, col);
     }
     block_xor_to_global<3>(v, acc);
@@ -1440,8 +1438,10 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail(const GkrTail* d) {
                 r1 = ld(Rs + x + h);
                 e = ld(Es + x);
             }
-            sb = g_add(sb, g_mul_lane(g_mul_lane(e, at_one ? l1 : l0, col), at_one ? r1 : r0, col));
-            si = g_add(si, g_mul_lane(g_mul_lane(e, g_add(l1, l0), col), g_add(r1, r0), col));
+            g4 el, ed;
+            g_mul_lane2(e, at_one ? l1 : l0, g_add(l1, l0), col, el, ed);
+            sb = g_add(sb, g_mul_lane(el, at_one ? r1 : r0, col));
+            si = g_add(si, g_mul_lane(ed, g_add(r1, r0), col));
         }
         // the next table: r_i binds round i-1 (or the final bind after round 0)
         rtab_fetch(nx, A.rtabs4 + (size_t)(n - 1 - i) * kConstTable4Entries);
