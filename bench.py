@@ -173,8 +173,8 @@ def main() -> int:
     for i in range(args.warmup):
         log(f"bench: warmup {i + 1}/{args.warmup}: {step(warm_ch[i]) * 1e3:.2f} ms")
     G.synchronize()
-    # HIP events bracket only the grid-wide fold launches (each timed launch
-    # adds two event records to the stream).
+    # HIP events bracket only the grid-wide fold launches: one event pair per
+    # proof spans its back-to-back fold launches.
     G.profile_enable(True)
     G.profile_only("outer_fold_next")
     times = []

@@ -2423,6 +2423,9 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     auto grid = [](size_t work) { return inner_grid(work); };
     std::vector<Timed> ts(k0 + 1);
     size_t len = h->len;
+    // The grid-wide fold rounds run back to back: one event pair spans them.
+    double fold_bytes = 0;
+    for (int k = 1; k < k0; ++k) fold_bytes += 96.0 * (double)(h->len >> k);
     for (int k = 0; k < k0; ++k) {
         if (k == 0) {
             ts[k].begin(s, "inner_round0_lane", 32.0 * (double)len);
@@ -2433,7 +2436,10 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
             const size_t cap = (h->where == 1) ? h->capQ : h->capP;
             f128* out[2] = {dst.f(), dst.f() + cap};
             const size_t out_pairs = len / 4;
-            ts[k].begin(s, "inner_fold_lane", 96.0 * (double)len / 2);
+            if (k == 1) {
+                ts[1].begin(s, "inner_fold_lane", fold_bytes);
+                ts[1].launches = (uint64_t)(k0 - 1);
+            }
             hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
                                bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], out_pairs,
                                rt + (size_t)(k - 1) * kConstTable4Entries, h->acc.u32() + 8 * (size_t)k);
@@ -2442,7 +2448,7 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
             h->where = (h->where == 1) ? 2 : 1;
             len /= 2;
         }
-        ts[k].end(s);
+        if (k == 0 || k == k0 - 1) ts[k == 0 ? 0 : 1].end(s);
         TRY(check_launch());
     }
     bitz::InnerTail ta{};
@@ -3890,16 +3896,21 @@ extern "C" int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* 
                            reinterpret_cast<const uint32_t*>(dp + cmap_o));
         TRY(check_launch());
     }
-    std::vector<Timed> ts(rounds.size() + 1);
+    // The round launches run back to back: one event pair spans them.
+    std::vector<Timed> ts(2);
+    double round_bytes = 0;
+    for (const auto& rg : rounds)
+        for (const auto& r : rg) round_bytes += (r.first ? 80.0 : 240.0) * (double)r.h;
     for (size_t g = 0; g < rounds.size(); ++g) {
-        double bytes = 0;
-        for (const auto& r : rounds[g]) bytes += (r.first ? 80.0 : 240.0) * (double)r.h;
-        ts[g].begin(s, "gkr_round_batch", bytes);
+        if (g == 0) {
+            ts[0].begin(s, "gkr_round_batch", round_bytes);
+            ts[0].launches = rounds.size();
+        }
         hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3((unsigned)round_maps[g].size()), dim3(bitz::kStageBlock),
                            bitz::kGkrRoundSmem, s,
                            reinterpret_cast<const bitz::GkrRound*>(dp + rd_o[g]),
                            reinterpret_cast<const uint32_t*>(dp + rm_o[g]));
-        ts[g].end(s);
+        if (g + 1 == rounds.size()) ts[0].end(s);
         TRY(check_launch());
     }
     ts.back().begin(s, "gkr_tail", 0.0);
