@@ -2420,13 +2420,17 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     inner_set_attributes();
     const uint4* rt = static_cast<const uint4*>(rt4.p);
     const int k0 = std::max(0, n - bitz::kInnerTailMaxLog);
+    // Rounds [kc, k0) run in the chunk stage (each workgroup folds its chunk
+    // of 2^kInnerChunkLog inputs round after round in LDS), the grid-wide
+    // kernels take [0, kc).
+    const int kc = k0 - std::max(0, std::min(bitz::kInnerChunkRounds, k0 - 1));
     auto grid = [](size_t work) { return inner_grid(work); };
-    std::vector<Timed> ts(k0 + 1);
+    std::vector<Timed> ts(k0 + 2);
     size_t len = h->len;
     // The grid-wide fold rounds run back to back: one event pair spans them.
     double fold_bytes = 0;
-    for (int k = 1; k < k0; ++k) fold_bytes += 96.0 * (double)(h->len >> k);
-    for (int k = 0; k < k0; ++k) {
+    for (int k = 1; k < kc; ++k) fold_bytes += 96.0 * (double)(h->len >> k);
+    for (int k = 0; k < kc; ++k) {
         if (k == 0) {
             ts[k].begin(s, "inner_round0_lane", 32.0 * (double)len);
             hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(grid(len / 2)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
@@ -2438,7 +2442,7 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
             const size_t out_pairs = len / 4;
             if (k == 1) {
                 ts[1].begin(s, "inner_fold_lane", fold_bytes);
-                ts[1].launches = (uint64_t)(k0 - 1);
+                ts[1].launches = (uint64_t)(kc - 1);
             }
             hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
                                bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], out_pairs,
@@ -2448,8 +2452,34 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
             h->where = (h->where == 1) ? 2 : 1;
             len /= 2;
         }
-        if (k == 0 || k == k0 - 1) ts[k == 0 ? 0 : 1].end(s);
+        if (k == 0 || k == kc - 1) ts[k == 0 ? 0 : 1].end(s);
         TRY(check_launch());
+    }
+    if (kc < k0) {
+        // len = 2^(n - kc + 1) entries per table before the fold by r_{kc-1}
+        Dev& dst = (h->where == 1) ? h->Q : h->P;
+        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+        bitz::InnerTail ca{};
+        ca.d = h->cur[0];
+        ca.w = h->cur[1];
+        ca.rtabs4 = rt;
+        ca.acc = h->acc.u32();
+        ca.dout = dst.f();
+        ca.wout = dst.f() + cap;
+        ca.n = n;
+        ca.k0 = kc;
+        ca.fold_first = 1;
+        ca.lg_in = bitz::kInnerChunkLog;
+        ca.s = k0 - kc;
+        const unsigned chunks = (unsigned)(len >> bitz::kInnerChunkLog);
+        ts[k0 + 1].begin(s, "inner_chunk", 0.0);
+        hipLaunchKernelGGL(bitz::k_inner_tail, dim3(chunks), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ca);
+        ts[k0 + 1].end(s);
+        TRY(check_launch());
+        h->cur[0] = ca.dout;
+        h->cur[1] = ca.wout;
+        h->where = (h->where == 1) ? 2 : 1;
+        len >>= (k0 - kc + 1);
     }
     bitz::InnerTail ta{};
     ta.d = h->cur[0];
@@ -2459,8 +2489,9 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     ta.fin = reinterpret_cast<f128*>(h->acc.u32() + 8 * (size_t)n);
     ta.n = n;
     ta.k0 = k0;
-    ta.fold_first = k0 > 0;
-    ta.lg_in = k0 > 0 ? n - k0 + 1 : n;
+    ta.fold_first = kc == k0 && k0 > 0;
+    ta.lg_in = ta.fold_first ? n - k0 + 1 : n - k0;
+    ta.s = n - k0;
     ts[k0].begin(s, "inner_tail", 0.0);
     hipLaunchKernelGGL(bitz::k_inner_tail, dim3(1), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ta);
     ts[k0].end(s);

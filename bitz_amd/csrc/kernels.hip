@@ -1154,24 +1154,25 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
     auto st = [](uint4* p, const g4& v) { *p = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]); };
     int lg = A.lg_in - A.fold_first;
     const size_t len = (size_t)1 << lg;
+    const size_t base = (size_t)blockIdx.x << A.lg_in;  // this chunk's first input
     g4 nx[kRtabPer];
     if (A.fold_first) {
         rtab_fetch(nx, A.rtabs4 + (size_t)(A.k0 - 1) * kConstTable4Entries);
         rtab_store(rt, nx);
         __syncthreads();
         for (size_t e = threadIdx.x; e < 2 * len; e += kStageBlock) {
-            const f128* src = e < len ? A.d + 2 * e : A.w + 2 * (e - len);
+            const f128* src = e < len ? A.d + base + 2 * e : A.w + base + 2 * (e - len);
             const g4 a = g_load_hbm(src), b = g_load_hbm(src + 1);
             st((e < len ? Ds + e : Ws + (e - len)), g_add(a, g_mul_table4(rt, g_add(a, b))));
         }
     } else {
         for (size_t e = threadIdx.x; e < len; e += kStageBlock) {
-            Ds[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.d + e));
-            Ws[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.w + e));
+            Ds[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.d + base + e));
+            Ws[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.w + base + e));
         }
     }
     __syncthreads();
-    for (int k = A.k0; k < A.n; ++k) {
+    for (int k = A.k0; k < A.k0 + A.s; ++k) {
         const size_t P = (size_t)1 << (lg - 1);
         rtab_fetch(nx, A.rtabs4 + (size_t)k * kConstTable4Entries);
         g4 s0 = g_zero(), s2 = g_zero();
@@ -1183,7 +1184,10 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
             s2 = g_add(s2, g_mul_lane(g_add(m1, m0), g_add(w1, w0), col));
         }
         g4 v[2] = {s0, s2};
-        block_xor_store<2>(v, A.acc + 8 * (size_t)k, kStageBlock);
+        if (A.dout)
+            block_xor_to_global<2>(v, A.acc + 8 * (size_t)k, kStageBlock);
+        else
+            block_xor_store<2>(v, A.acc + 8 * (size_t)k, kStageBlock);
         __syncthreads();  // readers of rt done (and of the tables, before the fold writes)
         rtab_store(rt, nx);
         __syncthreads();
@@ -1212,6 +1216,14 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
         }
         __syncthreads();
         lg -= 1;
+    }
+    if (A.dout) {
+        const size_t out = (size_t)1 << lg;
+        for (size_t e = threadIdx.x; e < out; e += kStageBlock) {
+            *reinterpret_cast<uint4*>(A.dout + ((size_t)blockIdx.x << lg) + e) = Ds[e];
+            *reinterpret_cast<uint4*>(A.wout + ((size_t)blockIdx.x << lg) + e) = Ws[e];
+        }
+        return;
     }
     if (threadIdx.x < 2) reinterpret_cast<uint4*>(A.fin)[threadIdx.x] = threadIdx.x == 0 ? Ds[0] : Ws[0];
 }

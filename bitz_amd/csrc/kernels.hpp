@@ -218,14 +218,24 @@ __global__ void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap
 __global__ void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout, size_t out_pairs,
                                   const uint4* rtab4, uint32_t* acc);
 constexpr int kInnerTailMaxLog = 10;  // at most 2^10 entries per table in LDS
+// Rounds [k0, k0 + s) on chunks of 2^lg_in inputs (folded by r_{k0-1} on
+// load when fold_first), chunk g = workgroup g at inputs [g 2^lg_in, ...).
+// A final stage (dout null, one chunk) runs to the last round and stores
+// D(r), h(r) at fin and its sums at acc; a chunk stage XORs its sums into
+// acc and stores each chunk's folded entries at dout / wout.
 struct InnerTail {
     const f128* d;
     const f128* w;
     const uint4* rtabs4;  // r_k's table at rtabs4 + 512 k
     uint32_t* acc;        // round k's [c0, c2] at acc + 8 k
     f128* fin;            // D(r), h(r)
-    int n, k0, fold_first, lg_in;
+    f128* dout;
+    f128* wout;
+    int n, k0, fold_first, lg_in, s;
 };
+// Rounds of the inner sumcheck's chunk stage: chunks of 2^11 inputs.
+constexpr int kInnerChunkLog = 11;
+constexpr int kInnerChunkRounds = 8;
 constexpr size_t kInnerTailSmem =
     sizeof(uint4) * (((size_t)2 << kInnerTailMaxLog) + kConstTable4Entries + 4 * 64 * kLaneTableEntries);
 __global__ void k_inner_tail(InnerTail args);
