@@ -947,7 +947,7 @@ __device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, 
 
 // One chunk of a stage, 256 threads.  Dynamic LDS (stage_smem): the three
 // tables (2^cap_lg entries each), the weight tables (2^cap_lg entries) and
-// 16 KiB of lane tables per wave.
+// 8 KiB of lane tables per wave.
 __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
     __shared__ __attribute__((aligned(256))) uint4 rt[kConstTable4Entries];
     extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];

@@ -107,8 +107,8 @@ __global__ void k_outer_r0(const f128* a, const f128* b, const f128* c, size_t t
 // Round 0 with lane tables: 1024 threads, 16 waves x 8 KiB + two 4-bit tables.
 constexpr int kR0Block = 1024;
 constexpr size_t kR0Smem = sizeof(uint4) * (2 * kConstTable4Entries + (kR0Block / 64) * 64 * kLaneTableEntries);
-// Threads of a multi-iteration round: two 512-thread workgroups per CU
-// (each 80 KiB of LDS) across the chip.
+// Threads of a multi-iteration round: one 1024-thread workgroup per CU
+// across the chip.
 constexpr int kThreadBits = 18;
 // Rounds whose pair count exceeds 2^kChunkFromBits run grid-wide; the
 // following ones go to the chunked LDS stage.
@@ -145,7 +145,7 @@ constexpr int kStageBlock = 256;
 constexpr int kStageMaxLog = 10;  // at most 2^10 entries per table in LDS
 constexpr int kChunkMaxLog = 11;  // chunk stage of a single proof: 2^11 inputs, folded on load
 constexpr int kTailMaxLog = 11;
-// Lane tables of a 256-thread workgroup (4 waves x 64 lanes x 16 entries).
+// Lane tables of a 256-thread workgroup (4 waves x 64 lanes x kLaneTableEntries entries).
 constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
 // Dynamic LDS of a stage with 2^cap_lg entries per table: the three tables,
 // the weight tables (2^cap_lg entries) and the lane tables.
