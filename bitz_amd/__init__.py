@@ -359,11 +359,16 @@ class Transcript:
 
 
 # ------------------------------------------------------------------- poly ---
-def mul_batch(a, b) -> np.ndarray:
+MUL_LANE, MUL_LANE2, MUL_CONST = 0x100, 0x200, 0x400  # BITZ_MUL_* (product primitive)
+
+
+def mul_batch(a, b, mode: int = 0) -> np.ndarray:
+    """a[i] * b[i] on the device; mode selects the product primitive
+    (MUL_CONST: a[i] * b[0] through b[0]'s constant table)."""
     init()
     a, b = elems(a), elems(b)
     out = np.zeros_like(a)
-    _check(lib().bitz_mul_batch(_p(a), _p(b), _p(out), len(a), 0))
+    _check(lib().bitz_mul_batch(_p(a), _p(b), _p(out), len(a), mode))
     return out
 
 

@@ -495,7 +495,23 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
     TRY(upload(da, a, n, flags, so.s));
     TRY(upload(db, b, n, flags, so.s));
     TRY(dout.alloc(n * sizeof(f128)));
-    hipLaunchKernelGGL(bitz::k_mul, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
+    const uint32_t mode = flags & (BITZ_MUL_LANE | BITZ_MUL_LANE2 | BITZ_MUL_CONST);
+    if (mode == 0) {
+        hipLaunchKernelGGL(bitz::k_mul, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
+    } else if (mode == BITZ_MUL_LANE || mode == BITZ_MUL_LANE2) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_mul_lane),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+            attr = true;
+        }
+        hipLaunchKernelGGL(bitz::k_mul_lane, dim3(grid_for(n)), dim3(kBlock), bitz::kLaneSmem256, so.s, da.f(),
+                           db.f(), dout.f(), n, mode == BITZ_MUL_LANE2 ? 1 : 0);
+    } else if (mode == BITZ_MUL_CONST) {
+        hipLaunchKernelGGL(bitz::k_mul_const, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
+    } else {
+        return fail(BITZ_ERR_ARG, "one product mode at a time");
+    }
     TRY(check_launch());
     return download(out, dout.f(), n, flags, so.s);
 }

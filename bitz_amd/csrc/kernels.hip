@@ -118,6 +118,38 @@ __global__ __launch_bounds__(kBlock) void k_mul(const f128* a, const f128* b, f1
         g_store(out + i, g_mul(g_load(a + i), g_load(b + i)));
 }
 
+// The LDS product primitives on their own (bitz_mul_batch's modes): a[i] b[i]
+// through this lane's 3-bit table (two: g_mul_lane2, pairing element i with
+// i ^ 1 so that both chains are checked), or a[i] b[0] through b[0]'s 4-bit
+// table.  256 threads; k_mul_lane takes kLaneSmem256 of dynamic LDS.
+__global__ __launch_bounds__(kBlock) void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, int two) {
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const g4 x = g_load(a + i), y = g_load(b + i);
+        if (two) {
+            const size_t j = (i ^ 1) < n ? (i ^ 1) : i;
+            const g4 z = g_load(b + j);
+            g4 p, q;
+            if (i & 1)
+                g_mul_lane2(x, z, y, col, q, p);
+            else
+                g_mul_lane2(x, y, z, col, p, q);
+            g_store(out + i, p);
+        } else {
+            g_store(out + i, g_mul_lane(x, y, col));
+        }
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_mul_const(const f128* a, const f128* b, f128* out, size_t n) {
+    __shared__ __attribute__((aligned(256))) uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    build_const_table4(tab, basis, g_load(b));  // ends with a barrier
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        g_store(out + i, g_mul_table4(tab, g_load(a + i)));
+}
+
 // In-place level doubling of a small eq table by one workgroup
 // (crates/poly/src/eq.rs:63-99).
 // Both halves of a split eq table in one launch: workgroup 0 builds eq(r_<kl)

@@ -12,6 +12,8 @@ namespace bitz {
 constexpr int kBlock = 256;
 
 __global__ void k_mul(const f128* a, const f128* b, f128* out, size_t n);
+__global__ void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, int two);
+__global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_eq_small(const f128* r, int k, f128* table);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
 __global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);

@@ -106,7 +106,15 @@ int32_t bitz_transcript_squeeze(bitz_transcript* t, bitz_f128* out);
 size_t bitz_transcript_narg(const bitz_transcript* t, uint8_t* out, size_t cap);
 
 /* ---- field and multilinear primitives -------------------------------------- */
-/* out[i] = a[i] * b[i] (F128 Mul, crates/field/src/gf128.rs:275-281). */
+/* out[i] = a[i] * b[i] (F128 Mul, crates/field/src/gf128.rs:275-281).  The
+ * product primitive is selectable so each can be checked on its own (no
+ * counterpart in the reference): the default is the comb; BITZ_MUL_LANE the
+ * per-lane LDS table product of the round kernels, BITZ_MUL_LANE2 its
+ * two-operand form (one table of a[i], the second chain taking a neighbour's
+ * b), BITZ_MUL_CONST out[i] = a[i] * b[0] through b[0]'s 4-bit table. */
+#define BITZ_MUL_LANE 0x100u
+#define BITZ_MUL_LANE2 0x200u
+#define BITZ_MUL_CONST 0x400u
 int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f128* out, size_t n, uint32_t flags);
 /* poly::eq_table (crates/poly/src/eq.rs:63-99): out has 2^k entries. */
 int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags);

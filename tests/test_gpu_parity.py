@@ -29,6 +29,28 @@ def test_mul_batch_matches_oracle():
     assert np.array_equal(G.mul_batch(A, B), O.mul_batch(A, B))
 
 
+@pytest.mark.parametrize("mode", [G.MUL_LANE, G.MUL_LANE2])
+def test_lane_table_products_match_oracle(mode):
+    """The per-lane LDS table product (3-bit windows, a's top two bits added
+    separately) and its two-operand form on every pair of edge values (zero,
+    one, the reduction constant, the top bits) and on random elements, odd
+    length included."""
+    edges = EDGES + [1 << 126, 3 << 126, (1 << 126) - 1]
+    a = [x for x in edges for _ in edges] + ints(elements(11, 4097))
+    b = [y for _ in edges for y in edges] + ints(elements(12, 4097))
+    A, B = from_ints(a), from_ints(b)
+    assert np.array_equal(G.mul_batch(A, B, mode), O.mul_batch(A, B))
+
+
+@pytest.mark.parametrize("c", EDGES + [0xDEADBEEF << 70 | 12345])
+def test_constant_table_product_matches_oracle(c):
+    """a[i] * c through c's 4-bit constant table (the perm-addressed lookups of
+    the fold and weight products)."""
+    a = from_ints(EDGES + ints(elements(13, 3000)))
+    b = from_ints([c] * len(a))
+    assert np.array_equal(G.mul_batch(a, b, G.MUL_CONST), O.mul_batch(a, b))
+
+
 @pytest.mark.parametrize("k", [0, 1, 5, 12, 13, 20])
 def test_eq_table_matches_oracle(k):
     r = elements(10 + k, k) if k else np.zeros((0, 2), np.uint64)
