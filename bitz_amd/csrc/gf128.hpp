@@ -402,7 +402,10 @@ __device__ __forceinline__ g4 g_mul_lane(const g4& a, const g4& b, uint4* col) {
 }
 
 // a * b and a * c with one table of a: the two Horner chains run
-// interleaved over the shared table.
+// interleaved over the shared table.  Kept in this shape (the chains picked
+// through a pointer array inside one position loop): written as two calls
+// of shared helpers, the compiler scheduled further ahead and the round
+// kernels spilled (up to 324 bytes of scratch, 60 more registers).
 __device__ __forceinline__ void g_mul_lane2(const g4& a, const g4& b, const g4& c, uint4* col, g4& ab, g4& ac) {
     const uint32_t ca = lds_addr(col);
     g4 al = a;
