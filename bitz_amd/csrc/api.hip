@@ -789,10 +789,10 @@ static bool outer_setup_eq(bitz_outer* h, int low_vars, int high_vars, bool batc
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kFoldSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)bitz::stage_smem(bitz::kTailMaxLog - 1));
+                                  (int)bitz::outer_stage_smem(bitz::kTailMaxLog - 1));
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_outer_chunk),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)bitz::stage_smem(bitz::kChunkMaxLog - 1));
+                                  (int)bitz::outer_stage_smem(bitz::kChunkMaxLog - 1));
         attr_set = true;
     }
     return true;
@@ -1475,7 +1475,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
         ca.acc = h->acc.u32();
         const unsigned chunks = (unsigned)(h->len >> cs);
         timers[ps.kc].begin(s, "outer_chunk", 0.0);
-        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(bitz::kStageBlock), bitz::stage_smem(cs - 1), s,
+        hipLaunchKernelGGL(bitz::k_outer_chunk, dim3(chunks), dim3(bitz::kOuterStageBlock), bitz::outer_stage_smem(cs - 1), s,
                            ca);
         timers[ps.kc].end(s);
         TRY(check_launch());
@@ -1500,7 +1500,7 @@ static int32_t outer_pipeline_eq(bitz_outer* h, uint64_t q1mask, std::vector<Tim
     ta.acc = h->acc.u32();
     ta.fin = reinterpret_cast<f128*>(h->acc.u32() + h->fin_off());
     timers[n].begin(s, "outer_tail", 0.0);
-    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(bitz::kStageBlock), bitz::stage_smem(n - ps.k0), s, ta);
+    hipLaunchKernelGGL(bitz::k_outer_tail, dim3(1), dim3(bitz::kOuterStageBlock), bitz::outer_stage_smem(n - ps.k0), s, ta);
     timers[n].end(s);
     TRY(check_launch());
     h->round = n;

@@ -809,17 +809,18 @@ __device__ __forceinline__ int stage_woff(int pl0, int l) { return (2 << pl0) - 
 
 // Builds W_0(x) = E(x) W_hi(g) for x < 2^pl0, barrier, then the strided
 // copies W_l(y) = W_0(y 2^l) for l in [1, s).  The caller's barrier ends it.
+template <int B = kStageBlock>
 __device__ __forceinline__ void build_stage_weights(uint4* wt, int pl0, int s, const f128* L, int ell,
                                                     const f128* H, int a0, int n, size_t g, uint4* col) {
     const int b = a0 + pl0;
     const g4 wh = run_weight(L, ell, H, b, n, g, col);
-    for (int x = threadIdx.x; x < (1 << pl0); x += kStageBlock) {
+    for (int x = threadIdx.x; x < (1 << pl0); x += B) {
         const g4 w = g_mul_lane(run_weight(L, ell, H, a0, b, (size_t)x, col), wh, col);
         wt[x] = make_uint4(w.w[0], w.w[1], w.w[2], w.w[3]);
     }
     __syncthreads();
     for (int l = 1; l < s; ++l)
-        for (int y = threadIdx.x; y < (1 << (pl0 - l)); y += kStageBlock) wt[stage_woff(pl0, l) + y] = wt[y << l];
+        for (int y = threadIdx.x; y < (1 << (pl0 - l)); y += B) wt[stage_woff(pl0, l) + y] = wt[y << l];
 }
 
 // Round sums over the 2^pl pairs of the chunk in LDS (three tables at stride
@@ -827,6 +828,7 @@ __device__ __forceinline__ void build_stage_weights(uint4* wt, int pl0, int s, c
 // sum_y W(y) (a0+a1)(b0+b1) and, with q1, sum_y W(y) (a1 b1 + c1), one
 // (pair, term) item at a time through this lane's table col.  Stored at
 // acc_k by a stage of one workgroup, XORed into it otherwise.
+template <int NT = kStageBlock>
 __device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int pl, const uint4* wt, bool q1,
                                                uint32_t* acc_k, bool store, uint4* col) {
     const size_t P = (size_t)1 << pl;
@@ -841,7 +843,7 @@ __device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int
     const uint4* B = lds + ((size_t)1 << cap_lg);
     const uint4* C = lds + ((size_t)2 << cap_lg);
     g4 s[3] = {g_zero(), g_zero(), g_zero()};
-    for (size_t it = threadIdx.x; it < items; it += kStageBlock) {
+    for (size_t it = threadIdx.x; it < items; it += NT) {
         const size_t y = it & (P - 1);
         const int which = (int)(it >> pl);
         const g4 a0 = ld(A + 2 * y), a1 = ld(A + 2 * y + 1);
@@ -859,21 +861,21 @@ __device__ __forceinline__ void lds_round_sums(const uint4* lds, int cap_lg, int
         else s[2] = g_add(s[2], wp);
     }
     if (store)
-        block_xor_store<3>(s, acc_k, kStageBlock);
+        block_xor_store<3>(s, acc_k, NT);
     else
-        block_xor_to_global<3>(s, acc_k, kStageBlock);
+        block_xor_to_global<3>(s, acc_k, NT);
 }
 
 // In-place fold of the three LDS tables (2^(pl+1) entries each at stride
 // 2^cap_lg) through the 4-bit table of r in rt: read everything, barrier,
 // write, barrier.  At most kFoldPerThread outputs per thread.
-template <int kFoldPerThread>
+template <int kFoldPerThread, int B = kStageBlock>
 __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const uint4* rt) {
     const size_t half = (size_t)1 << pl;
     g4 nv[kFoldPerThread];
 #pragma unroll
     for (int q = 0; q < kFoldPerThread; ++q) {
-        const size_t e = threadIdx.x + (size_t)q * kStageBlock;
+        const size_t e = threadIdx.x + (size_t)q * B;
         if (e < 3 * half) {
             const int t = (int)(e >> pl);
             const size_t j = e & (half - 1);
@@ -887,7 +889,7 @@ __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const u
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kFoldPerThread; ++q) {
-        const size_t e = threadIdx.x + (size_t)q * kStageBlock;
+        const size_t e = threadIdx.x + (size_t)q * B;
         if (e < 3 * half) {
             const int t = (int)(e >> pl);
             const size_t j = e & (half - 1);
@@ -901,40 +903,44 @@ __device__ __forceinline__ void lds_fold(uint4* lds, int cap_lg, int pl, const u
 // early, and the LDS stores once the last reader of the previous table is
 // done.  256 threads.
 constexpr int kRtabPer = kConstTable4Entries / kStageBlock;
-__device__ __forceinline__ void rtab_fetch(g4 (&v)[kRtabPer], const uint4* __restrict__ src) {
+template <int B = kStageBlock>
+__device__ __forceinline__ void rtab_fetch(g4 (&v)[kConstTable4Entries / B], const uint4* __restrict__ src) {
 #pragma unroll
-    for (int q = 0; q < kRtabPer; ++q) v[q] = g_load_hbm(reinterpret_cast<const f128*>(src + threadIdx.x + q * kStageBlock));
+    for (int q = 0; q < kConstTable4Entries / B; ++q)
+        v[q] = g_load_hbm(reinterpret_cast<const f128*>(src + threadIdx.x + q * B));
 }
-__device__ __forceinline__ void rtab_store(uint4* rt, const g4 (&v)[kRtabPer]) {
+template <int B = kStageBlock>
+__device__ __forceinline__ void rtab_store(uint4* rt, const g4 (&v)[kConstTable4Entries / B]) {
 #pragma unroll
-    for (int q = 0; q < kRtabPer; ++q) rt[threadIdx.x + q * kStageBlock] = make_uint4(v[q].w[0], v[q].w[1], v[q].w[2], v[q].w[3]);
+    for (int q = 0; q < kConstTable4Entries / B; ++q)
+        rt[threadIdx.x + q * B] = make_uint4(v[q].w[0], v[q].w[1], v[q].w[2], v[q].w[3]);
 }
 
 // Loads entries [off, off + 2^lg) of a, b and c into the LDS tables (stride
 // 2^cap_lg).  Chunks of kPer 2^8 entries per table take one group of loads
 // per thread, all issued before the first LDS store; smaller ones go entry
 // by entry.
-template <int kPer>
+template <int kPer, int B = kStageBlock>
 __device__ __forceinline__ void lds_load_rows(uint4* lds, int cap_lg, int lg, const f128* a, const f128* b,
                                               const f128* c, size_t off) {
     const size_t len = (size_t)1 << lg;
-    if (len == (size_t)kPer * kStageBlock) {
+    if (len == (size_t)kPer * B) {
         g4 v[3][kPer];
 #pragma unroll
         for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int q = 0; q < kPer; ++q)
-                v[t][q] = g_load_hbm((t == 0 ? a : (t == 1 ? b : c)) + off + q * kStageBlock + threadIdx.x);
+                v[t][q] = g_load_hbm((t == 0 ? a : (t == 1 ? b : c)) + off + q * B + threadIdx.x);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int q = 0; q < kPer; ++q)
-                lds[((size_t)t << cap_lg) + q * kStageBlock + threadIdx.x] =
+                lds[((size_t)t << cap_lg) + q * B + threadIdx.x] =
                     make_uint4(v[t][q].w[0], v[t][q].w[1], v[t][q].w[2], v[t][q].w[3]);
         return;
     }
-    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
+    for (size_t e = threadIdx.x; e < 3 * len; e += B) {
         const int t = (int)(e >> lg);
         const g4 v = g_load_hbm((t == 0 ? a : (t == 1 ? b : c)) + off + (e & (len - 1)));
         lds[((size_t)t << cap_lg) + (e & (len - 1))] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
@@ -943,17 +949,17 @@ __device__ __forceinline__ void lds_load_rows(uint4* lds, int cap_lg, int lg, co
 
 // The same for 2^lg folded entries per table, out[j] = in[2j] + r (in[2j] +
 // in[2j+1]) over the 2^(lg+1) inputs from off, with r's table in rt.
-template <int kPer>
+template <int kPer, int B = kStageBlock>
 __device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, const f128* a, const f128* b,
                                                 const f128* c, size_t off, const uint4* rt) {
     const size_t len = (size_t)1 << lg;
-    if (len == (size_t)kPer * kStageBlock) {
+    if (len == (size_t)kPer * B) {
         g4 lo[3][kPer], hi[3][kPer];
 #pragma unroll
         for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
-                const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off + 2 * (q * kStageBlock + threadIdx.x);
+                const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off + 2 * (q * B + threadIdx.x);
                 lo[t][q] = g_load_hbm(src);
                 hi[t][q] = g_load_hbm(src + 1);
             }
@@ -963,11 +969,11 @@ __device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, 
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
                 const g4 v = g_add(lo[t][q], g_mul_table4(rt, g_add(lo[t][q], hi[t][q])));
-                lds[((size_t)t << cap_lg) + q * kStageBlock + threadIdx.x] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+                lds[((size_t)t << cap_lg) + q * B + threadIdx.x] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
             }
         return;
     }
-    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
+    for (size_t e = threadIdx.x; e < 3 * len; e += B) {
         const int t = (int)(e >> lg);
         const size_t j = e & (len - 1);
         const f128* src = (t == 0 ? a : (t == 1 ? b : c)) + off + 2 * j;
@@ -980,6 +986,7 @@ __device__ __forceinline__ void lds_load_folded(uint4* lds, int cap_lg, int lg, 
 // One chunk of a stage, 256 threads.  Dynamic LDS (stage_smem): the three
 // tables (2^cap_lg entries each), the weight tables (2^cap_lg entries) and
 // 8 KiB of lane tables per wave.
+template <int B>
 __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
     __shared__ __attribute__((aligned(256))) uint4 rt[kConstTable4Entries];
     extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];
@@ -988,31 +995,31 @@ __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
     uint4* wt = lds_raw + ((size_t)3 << cap_lg);
     uint4* col = wt + ((size_t)1 << cap_lg) + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
-    g4 nx[kRtabPer];
+    g4 nx[kConstTable4Entries / B];
     if (A.fold_first) {
-        rtab_fetch(nx, A.rtabs4 + (size_t)(A.k_first - 1) * kConstTable4Entries);
-        rtab_store(rt, nx);
+        rtab_fetch<B>(nx, A.rtabs4 + (size_t)(A.k_first - 1) * kConstTable4Entries);
+        rtab_store<B>(rt, nx);
     }
-    build_stage_weights(wt, pl0, A.s, A.L, A.ell, A.H, A.k_first + 1, A.n, g, col);  // barrier inside
-    constexpr int kPer = (1 << kStageMaxLog) / kStageBlock;  // loads per thread and table of a full chunk
+    build_stage_weights<B>(wt, pl0, A.s, A.L, A.ell, A.H, A.k_first + 1, A.n, g, col);  // barrier inside
+    constexpr int kPer = (1 << kStageMaxLog) / B;  // loads per thread and table of a full chunk
     if (A.fold_first)
-        lds_load_folded<kPer>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in, rt);
+        lds_load_folded<kPer, B>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in, rt);
     else
-        lds_load_rows<kPer>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in);
+        lds_load_rows<kPer, B>(lds_raw, cap_lg, cap_lg, A.a, A.b, A.c, g << A.lg_in);
     __syncthreads();
     int lg = cap_lg;
     for (int l = 0; l < A.s; ++l) {
         const int k = A.k_first + l;
         const int pl = lg - 1;
         const bool fold = l + 1 < A.s || A.final_stage;
-        if (fold) rtab_fetch(nx, A.rtabs4 + (size_t)k * kConstTable4Entries);
-        lds_round_sums(lds_raw, cap_lg, pl, wt + stage_woff(pl0, l), (A.q1mask >> k) & 1, A.acc + 12 * k,
+        if (fold) rtab_fetch<B>(nx, A.rtabs4 + (size_t)k * kConstTable4Entries);
+        lds_round_sums<B>(lds_raw, cap_lg, pl, wt + stage_woff(pl0, l), (A.q1mask >> k) & 1, A.acc + 12 * k,
                        A.store_sums, col);
         if (fold) {
             __syncthreads();  // every reader of rt is done
-            rtab_store(rt, nx);
+            rtab_store<B>(rt, nx);
             __syncthreads();
-            lds_fold<6>(lds_raw, cap_lg, pl, rt);
+            lds_fold<(3 << (kStageMaxLog - 1)) / B, B>(lds_raw, cap_lg, pl, rt);
             lg = pl;
         }
     }
@@ -1021,7 +1028,7 @@ __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
         return;
     }
     const size_t len = (size_t)1 << lg;
-    for (size_t e = threadIdx.x; e < 3 * len; e += kStageBlock) {
+    for (size_t e = threadIdx.x; e < 3 * len; e += B) {
         const int t = (int)(e >> lg);
         const size_t j = e & (len - 1);
         f128* dst = (t == 0 ? A.ao : (t == 1 ? A.bo : A.co)) + (g << lg) + j;
@@ -1030,11 +1037,11 @@ __device__ __forceinline__ void lds_stage(const LdsStage& A, size_t g) {
 }
 
 // Middle rounds of a single proof: workgroup g runs the stage on chunk g.
-__global__ __launch_bounds__(kStageBlock) void k_outer_chunk(LdsStage A) { lds_stage(A, blockIdx.x); }
+__global__ __launch_bounds__(kOuterStageBlock) void k_outer_chunk(LdsStage A) { lds_stage<kOuterStageBlock>(A, blockIdx.x); }
 
 // The last rounds of a single proof in one workgroup (a final stage of one
 // chunk): barriers instead of kernel launches between rounds.
-__global__ __launch_bounds__(kStageBlock) void k_outer_tail(LdsStage A) { lds_stage(A, 0); }
+__global__ __launch_bounds__(kOuterStageBlock) void k_outer_tail(LdsStage A) { lds_stage<kOuterStageBlock>(A, 0); }
 
 // ======================================= batches of independent proofs ===
 __global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* items, const uint32_t* bmap) {
@@ -1062,7 +1069,7 @@ __global__ __launch_bounds__(1024) void k_batch_prologue(const BatchPrologue* it
 // workgroup's stage descriptor.
 __global__ __launch_bounds__(kStageBlock) void k_outer_batch(const LdsStage* stages, const uint32_t* bmap) {
     const LdsStage A = stages[bmap[blockIdx.x]];
-    lds_stage(A, blockIdx.x - A.chunk_base);
+    lds_stage<kStageBlock>(A, blockIdx.x - A.chunk_base);
 }
 
 // Grid-wide rounds of a batch: one launch per round over every proof still

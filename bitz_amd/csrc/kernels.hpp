@@ -152,6 +152,13 @@ constexpr size_t kLaneTables256 = 4 * 64 * kLaneTableEntries;
 // Dynamic LDS of a stage with 2^cap_lg entries per table: the three tables,
 // the weight tables (2^cap_lg entries) and the lane tables.
 constexpr size_t stage_smem(int cap_lg) { return sizeof(uint4) * (((size_t)4 << cap_lg) + kLaneTables256); }
+// The single proof's chunk and tail stages run 512-thread workgroups (eight
+// waves of lane tables): their rounds with more pairs than 256 threads take
+// fewer sequential items per thread.
+constexpr int kOuterStageBlock = 512;
+constexpr size_t outer_stage_smem(int cap_lg) {
+    return sizeof(uint4) * (((size_t)4 << cap_lg) + (size_t)(kOuterStageBlock / 64) * 64 * kLaneTableEntries);
+}
 __global__ void k_outer_chunk(LdsStage args);
 __global__ void k_outer_tail(LdsStage args);
 __global__ void k_outer_batch(const LdsStage* stages, const uint32_t* bmap);
