@@ -191,9 +191,11 @@ def elems(a) -> np.ndarray:
 
 
 def _p(a):
+    """An address for a c_void_p parameter (plain ints convert without the
+    cost of a ctypes cast; None is NULL)."""
     if isinstance(a, DeviceTable):
-        return C.c_void_p(a.ptr)
-    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+        return a.ptr
+    return a.ctypes.data if a is not None and a.size else None
 
 
 def f128(x) -> F128:
@@ -480,17 +482,19 @@ def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, 
     eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
     n = _nvars(na)
     ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
-    rounds = np.zeros((max(n, 1) * 4, 2), dtype=np.uint64)
-    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    m = max(n, 1)
+    out = np.zeros((5 * m + 3, 2), dtype=np.uint64)  # rounds | points | claims, one allocation
+    rounds, points, claims = out[: 4 * m], out[4 * m: 5 * m], out[5 * m:]
     final = F128()
-    claims = np.zeros((3, 2), dtype=np.uint64)
+    base = out.ctypes.data
     rc = lib().bitz_prove_outer_sumcheck_with_challenges(
         _p(ch), f128(initial_claim), _p(eq_low), _nvars(len(eq_low)), _p(eq_high), _nvars(len(eq_high)),
-        _p(az), n, _p(bz), _nvars(nb), _p(cz), _nvars(nc), fa, _p(rounds), _p(points), C.byref(final),
-        _p(claims))
+        _p(az), n, _p(bz), _nvars(nb), _p(cz), _nvars(nc), fa, base, base + 64 * m, C.byref(final),
+        base + 80 * m)
     _check(rc, SumcheckError)
+    cl = claims.tolist()
     return OuterSumcheckOutput(rounds[: 4 * n].reshape(n, 4, 2), points[:n], to_int(final),
-                               to_int(claims[0]), to_int(claims[1]), to_int(claims[2]))
+                               cl[0][0] | (cl[0][1] << 64), cl[1][0] | (cl[1][1] << 64), cl[2][0] | (cl[2][1] << 64))
 
 
 class _OuterInstance(C.Structure):
