@@ -239,6 +239,24 @@ def main() -> int:
     return 0
 
 
+def batch_workload(seed: int, count: int):
+    """The batched workload's instances: num_vars uniform in [12, 22] from the
+    seed; A, B, C of instance i are windows (at a seeded offset) of three
+    uniform pools of 2^(max num_vars + 1) elements; tau and the challenges
+    uniform.  Returns (pools, [(num_vars, offset, tau, challenges)])."""
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.integers(12, 23, size=count)]
+    pool_log = max(sizes) + 1
+    pools = [rng.integers(0, 2**64, size=(1 << pool_log, 2), dtype=np.uint64) for _ in range(3)]
+    insts = []
+    for n in sizes:
+        off = int(rng.integers(0, (1 << pool_log) - (1 << n) + 1))
+        tau = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
+        ch = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
+        insts.append((n, off, tau, ch))
+    return pools, insts
+
+
 def bench_batch(G, args) -> int:
     """256 independent outer sumchecks, num_vars uniform in [12, 22] from the
     seed, each with A, B, C, tau and challenges as the single workload.  A
@@ -247,12 +265,12 @@ def bench_batch(G, args) -> int:
     which runs during the first warmup call); the same instances proved one
     call each give the plain single-instance rate."""
     G.init(0)
-    rng = np.random.default_rng(args.seed)
-    sizes = [int(x) for x in rng.integers(12, 23, size=args.batch)]
-    pool_log = max(sizes) + 1
+    host_pools, spec = batch_workload(args.seed, args.batch)
+    sizes = [n for n, *_ in spec]
     log(f"bench: batch of {args.batch} outer sumchecks, n in [12, 22], total {sum(1 << n for n in sizes)} "
         f"elements per table")
-    pools = [G.DeviceTable(rng.integers(0, 2**64, size=(1 << pool_log, 2), dtype=np.uint64)) for _ in range(3)]
+    pools = [G.DeviceTable(p) for p in host_pools]
+    del host_pools
 
     class View(G.DeviceTable):  # a window of a device pool (owns nothing)
         def __init__(self, pool, off, n):
@@ -262,14 +280,11 @@ def bench_batch(G, args) -> int:
             pass
 
     insts = []
-    for i, n in enumerate(sizes):
-        off = int(rng.integers(0, (1 << pool_log) - (1 << n) + 1))
+    for n, off, tau, ch in spec:
         prods = G.R1csProductMles(*(View(p, off, 1 << n) for p in pools))
-        tau = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
         lo, hi = G.make_equality_factors(tau)
         eq = (G.DeviceTable(lo), G.DeviceTable(hi))
         claim = G.outer_claim(eq, prods)
-        ch = rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64)
         insts.append((ch, claim, eq, prods))
     folded = sum(3 * ((1 << n) - 1) for n in sizes)
     graph = not args.batch_eager
