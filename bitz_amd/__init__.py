@@ -219,6 +219,17 @@ def _nvars(n: int) -> int:
     return n.bit_length() - 1
 
 
+def _challenges(challenges, n: int) -> np.ndarray:
+    """The first n challenges as a contiguous (n, 2) array (one zero row when
+    n = 0, so the C entry point gets a valid pointer).  The C entry points
+    read exactly num_vars challenges; fewer is an error here, before any
+    call, so a short array never reads past its end."""
+    ch = elems(challenges) if len(challenges) else np.zeros((0, 2), np.uint64)
+    if len(ch) < n:
+        raise BitzError(-7, f"{len(ch)} challenges for {n} rounds")
+    return np.ascontiguousarray(ch[:n]) if n else np.zeros((1, 2), np.uint64)
+
+
 class DeviceTable:
     """A table of field elements resident in HBM (bitz_device_alloc)."""
 
@@ -475,13 +486,13 @@ def prove_outer_sumcheck(transcript: Transcript, initial_claim, eq_factors, prod
 def prove_outer_sumcheck_with_challenges(challenges, initial_claim, eq_factors, products: R1csProductMles):
     """prove_outer_sumcheck against challenges fixed in advance: every round
     is queued back to back, the round sums are read once."""
-    init()
     (az, na, fa), (bz, nb, fb), (cz, nc, fc) = (_table(t) for t in (products.az, products.bz, products.cz))
     if not (fa == fb == fc):
         raise BitzError(-7, "products must all be host arrays or all DeviceTables")
-    eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
     n = _nvars(na)
-    ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
+    ch = _challenges(challenges, n)
+    init()
+    eq_low, eq_high = (_factor(f, fa) for f in eq_factors)
     m = max(n, 1)
     out = np.zeros((5 * m + 3, 2), dtype=np.uint64)  # rounds | points | claims, one allocation
     rounds, points, claims = out[: 4 * m], out[4 * m: 5 * m], out[5 * m:]
@@ -543,7 +554,7 @@ def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
         n = _nvars(az.len)
         _nvars(bz.len), _nvars(cz.len)
         ns.append(n)
-        chs.append(elems(ch)[:n] if n else np.zeros((0, 2), np.uint64))
+        chs.append(_challenges(ch, n)[:n])
         tabs.append((claim, lo, hi, az, bz, cz))
     off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
     total = int(off[-1])
@@ -675,12 +686,12 @@ def prove_inner_sumcheck(transcript: Transcript, initial_claim, batched_matrix, 
 def prove_inner_sumcheck_with_challenges(challenges, initial_claim, batched_matrix, witness):
     """prove_inner_sumcheck with the challenges known in advance
     (bitz_prove_inner_sumcheck_with_challenges): rounds queued back to back."""
-    init()
     (d, nd, fd), (w, nw, fw) = _table(batched_matrix), _table(witness)
     if fd != fw:
         raise BitzError(-7, "matrix and witness must both be host arrays or both DeviceTables")
     n = _nvars(nd)
-    ch = elems(challenges) if n else np.zeros((1, 2), np.uint64)
+    ch = _challenges(challenges, n)
+    init()
     rounds = np.zeros((max(n, 1) * 3, 2), dtype=np.uint64)
     points = np.zeros((max(n, 1), 2), dtype=np.uint64)
     final = F128()
@@ -884,9 +895,15 @@ class PiopEngine:
         if not (fa & DEVICE and fh & DEVICE):
             raise BitzError(-7, "the engine takes device tables")
         dg = (C.c_uint8 * 32).from_buffer_copy(digest)
-        _check(lib().bitz_piop_engine_submit(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz),
-                                             _nvars(na), _p(hv), _nvars(nh)), SumcheckError)
+        # Held before the native submit: a background engine may finish the
+        # proof (and another thread take it) before submit returns.
         self._held[tag] = (transcript, matrices, products, assignment, _nvars(na), _nvars(nh))
+        try:
+            _check(lib().bitz_piop_engine_submit(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz),
+                                                 _p(cz), _nvars(na), _p(hv), _nvars(nh)), SumcheckError)
+        except BaseException:
+            self._held.pop(tag, None)
+            raise
 
     def step(self) -> int:
         n = C.c_size_t()

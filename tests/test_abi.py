@@ -39,3 +39,21 @@ def test_dimension_errors_need_no_device():
     rc = L.bitz_outer_begin(C.byref(C.c_void_p()), one, 1, one, 0, one, 0, one, 0, one, 0, 0)
     assert rc == -2
     assert b"eq factor" in L.bitz_last_error()
+
+
+def test_short_challenge_arrays_are_rejected_before_any_call():
+    """The queued provers read exactly num_vars challenges; a shorter array
+    raises in the wrapper instead of reading past its end."""
+    import numpy as np
+
+    import bitz_amd as G
+    n = 3
+    t = np.zeros((1 << n, 2), np.uint64)
+    lo, hi = np.zeros((2, 2), np.uint64), np.zeros((4, 2), np.uint64)
+    for short in (np.zeros((2, 2), np.uint64), []):
+        with pytest.raises(G.BitzError) as e:
+            G.prove_outer_sumcheck_with_challenges(short, 0, (lo, hi), G.R1csProductMles(t, t, t))
+        assert e.value.variant == "InvalidArgument" and "challenges" in str(e.value)
+        with pytest.raises(G.BitzError) as e:
+            G.prove_inner_sumcheck_with_challenges(short, 0, t, t)
+        assert e.value.variant == "InvalidArgument" and "challenges" in str(e.value)

@@ -648,3 +648,18 @@ def test_outer_batch_graph_replays_track_new_data():
         eager_ref = outs
     assert eager_ref is not None
     G.outer_batch_release()
+
+
+def test_outer_batch_rejects_a_short_challenge_array():
+    """Each instance needs num_vars challenges; one short array fails the
+    whole call before anything is concatenated or launched."""
+    cases = batch_cases([(5, 2), (7, 3)], 990)
+    insts = batch_insts(cases)
+    ch, claim, eq, prods = insts[1]
+    insts[1] = (ch[:-1], claim, eq, prods)
+    with pytest.raises(G.BitzError) as e:
+        G.prove_outer_batch(insts)
+    assert "challenges" in str(e.value)
+    outs = G.prove_outer_batch(batch_insts(cases))
+    for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
+        assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
