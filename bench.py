@@ -1,14 +1,27 @@
-"""Benchmark: the Spartan outer sumcheck over GF(2^128) on one MI355X.
+"""Benchmark: SHA-256 blocks proved per second (sumcheck stage) with p50/p99
+proof latency on one MI355X -- the serving workload, by default.
 
-Workload: three multilinears A, B, C of 2^n GF(2^128) elements (default
-n = 24), uniform from the seed, an eq polynomial at a uniform point, and the
-degree-3 round polynomial of eq * (A * B + C) over all n rounds with
-challenges drawn from the seed.  A step is one complete proof through the C
-ABI (bitz_prove_outer_sumcheck) with the tables already resident in HBM.
+Default workload (serve): proving requests for SHA-256 compression chains of
+2^U blocks (U uniform in 0..8; message bytes and chaining value uniform from
+the seed).  Each request is its witness (host), the assignment MLE,
+Az/Bz/Cz and prove_spartan_piop (outer sumcheck, bind_and_batch, inner
+sumcheck) with every table on the device, served with continuous batching
+(bitz_amd/serving.py).  A step is one set of --serve-requests requests, all
+queued at once, served to completion; value = blocks proved per second over
+the timed steps.  p50/p99 are per-request latencies (arrival to final claim)
+under Poisson arrivals at --serve-rate requests/s; the latency at other
+rates, the concurrent-slots mode and the plain one-request-at-a-time path
+are reported beside it, as are the isolated outer sumcheck at 2^24 and
+the roofline of the serving run's dominant kernel (the batched inner fold,
+timed live with HIP events on the engines' streams).
 
-Prints one JSON line: throughput, p50/p99 proof latency, the roofline of the
-dominant kernel (timed live with HIP events on its stream), and the CPU
-oracle's rate on a bounded sample of the same workload.
+Other workloads (--workload): outer (one 2^log_n outer sumcheck per step),
+batch (256 outer sumchecks of 2^12..2^22), gkr (grand product over 2^22
+leaves), inner, bind, chain (the sha256-chain statement).
+
+Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
+the reference with the bit-serial multiply the reference selects on x86),
+timed on a bounded sample before the GPU is used.
 """
 from __future__ import annotations
 
@@ -80,15 +93,28 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["outer", "batch", "gkr", "inner", "serve", "bind", "chain"],
-                    default="outer",
-                    help="outer: one 2^log_n outer sumcheck per step (default); batch: 256 outer sumchecks "
+    ap.add_argument("--workload", choices=["serve", "outer", "batch", "gkr", "inner", "bind", "chain"],
+                    default="serve",
+                    help="serve: SHA-256 proving requests served with continuous batching (default); "
+                         "outer: one 2^log_n outer sumcheck per step; batch: 256 outer sumchecks "
                          "of 2^12..2^22 elements per step, batched; gkr: product tree and layered sumcheck "
                          "over 2^22 leaves per step; inner: one inner sumcheck over 2 tables of 2^inner_log_n")
-    ap.add_argument("--serve-rate", type=float, default=300.0, help="serve: Poisson arrivals per second")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=1000.0,
+                    help="serve: Poisson arrivals per second for the reported p50/p99")
+    ap.add_argument("--serve-rates", default="300,1000,2000,3000",
+                    help="serve: the Poisson rates of the latency curve")
+    ap.add_argument("--serve-requests", type=int, default=200, help="serve: requests per step")
+    ap.add_argument("--serve-latency-requests", type=int, default=400,
+                    help="serve: requests per Poisson run of the latency curve")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="serve: worker processes of the full-core CPU baseline (0: min(16, cpu count))")
+    ap.add_argument("--no-outer", action="store_true", help="serve: skip the outer 2^24 extra figures")
+    ap.add_argument("--capacity-only", action="store_true",
+                    help="serve: only the timed capacity steps (no latency curve, slots or plain runs; "
+                         "for kernel profiles of the batched mode)")
     ap.add_argument("--chain-blocks", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-engines", type=int, default=0, help="serve: PIOP engines (0: one per 8 slots)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,
                     help="inner: log2 table length (23: the column count of a 256-block SHA-256 chain)")
@@ -105,8 +131,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(a proof shards by high index bits; see the design notes). Use --gpus 1.",
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (a proof would shard "
+              "its tables by high index bits, one exchange of 48 bytes per GPU and round). Use --gpus 1.",
               file=sys.stderr)
         return 2
 
@@ -134,10 +160,15 @@ def main() -> int:
     if args.workload == "bind":
         return bench_bind(G, args)
 
-    n = args.log_n
+    return bench_outer(G, args)
+
+
+def run_outer(G, n: int, steps: int, warmup: int, seed: int) -> dict:
+    """The isolated outer sumcheck: 2^n tables, rounds queued with the
+    seeded challenges; returns the JSON line without its CPU baseline."""
     G.init(0)
     log(f"bench: outer sumcheck n={n}, generating inputs")
-    a, b, c, tau = make_inputs(n, args.seed)
+    a, b, c, tau = make_inputs(n, seed)
     lo, hi = G.make_equality_factors(tau)
     eq = (G.DeviceTable(lo), G.DeviceTable(hi))
     prods = G.R1csProductMles(G.DeviceTable(a), G.DeviceTable(b), G.DeviceTable(c))
@@ -146,12 +177,12 @@ def main() -> int:
     log("bench: inputs resident in HBM; initial claim computed on device")
 
     def seeded_challenges(i: int) -> np.ndarray:
-        t = G.Transcript.seeded(args.seed + i)
+        t = G.Transcript.seeded(seed + i)
         return np.array([[(x := t.squeeze()) & (2**64 - 1), x >> 64] for _ in range(n)], dtype=np.uint64)
 
     # Challenges are inputs: drawn before any timing.
-    warm_ch = [seeded_challenges(i) for i in range(args.warmup)]
-    timed_ch = [seeded_challenges(1000 + i) for i in range(args.steps)]
+    warm_ch = [seeded_challenges(i) for i in range(warmup)]
+    timed_ch = [seeded_challenges(1000 + i) for i in range(steps)]
 
     def step(ch: np.ndarray) -> float:
         """One proof with the seeded challenges: the rounds are queued back to
@@ -170,8 +201,8 @@ def main() -> int:
         G.prove_outer_sumcheck(t, claim, eq, prods)
         return time.perf_counter() - t0
 
-    for i in range(args.warmup):
-        log(f"bench: warmup {i + 1}/{args.warmup}: {step(warm_ch[i]) * 1e3:.2f} ms")
+    for i in range(warmup):
+        log(f"bench: warmup {i + 1}/{warmup}: {step(warm_ch[i]) * 1e3:.2f} ms")
     G.synchronize()
     # HIP events bracket only the grid-wide fold launches: one event pair per
     # proof spans its back-to-back fold launches.
@@ -179,29 +210,29 @@ def main() -> int:
     G.profile_only("outer_fold_next")
     times = []
     t_start = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         times.append(step(timed_ch[i]))
     G.synchronize()
     wall = time.perf_counter() - t_start
     for i, dt in enumerate(times):
-        log(f"bench: step {i + 1}/{args.steps}: {dt * 1e3:.2f} ms")
+        log(f"bench: step {i + 1}/{steps}: {dt * 1e3:.2f} ms")
     fold = G.profile_read("outer_fold_next")
     G.profile_only(None)
     G.profile_enable(False)
-    tt = sorted(step_transcript(i) for i in range(max(3, args.steps // 2)))
+    tt = sorted(step_transcript(i) for i in range(max(3, steps // 2)))
     log(f"bench: transcript-bound proofs: {[round(x * 1e3, 2) for x in tt]} ms")
 
     N = 1 << n
-    ms_per_step = wall / args.steps * 1e3
+    ms_per_step = wall / steps * 1e3
     achieved = fold["bytes"] / (fold["ms"] * 1e-3) / 1e9 if fold["ms"] else 0.0
     total_algo = 192.0 * N  # bytes the whole proof must move (read 3N, then fold rounds)
     line = {
-        "metric": METRIC,
-        "value": args.steps / wall,
+        "metric": f"outer sumchecks proved/sec (eq*(A*B+C) over 2^{n} GF(2^128) elements per table), 1 MI355X",
+        "value": steps / wall,
         "unit": "outer-sumcheck proofs/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
         "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
@@ -218,8 +249,8 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "transcript_mode_p50_ms": float(np.percentile(np.array(tt) * 1e3, 50)),
-        "elements_folded_per_s": 3 * (N - 1) * args.steps / wall,
-        "proof_hbm_gbs": total_algo * args.steps / wall / 1e9,
+        "elements_folded_per_s": 3 * (N - 1) * steps / wall,
+        "proof_hbm_gbs": total_algo * steps / wall / 1e9,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_outer_fold (grid-wide fold rounds 1-5)",
@@ -229,12 +260,18 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(n),
             "launches": fold["launches"],
-            "kernel_ms_per_step": fold["ms"] / args.steps,
+            "kernel_ms_per_step": fold["ms"] / steps,
         },
     }
+    del eq, prods
+    return line
+
+
+def bench_outer(G, args) -> int:
+    line = run_outer(G, args.log_n, args.steps, args.warmup, args.seed)
     if not args.no_cpu:
         log(f"bench: CPU oracle sample at 2^{args.cpu_sample_log_n}")
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log_n, n)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log_n, args.log_n)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -513,17 +550,22 @@ def bench_inner(G, args) -> int:
     return 0
 
 
+def serve_mix(seed: int, count: int, rate: float):
+    """count requests of the serving workload (blocks 2^U, U uniform in 0..8;
+    message and chaining value uniform), Poisson arrivals at `rate`/s."""
+    from bitz_amd import serving as S
+    return S.poisson_requests(rate, count, seed)
+
+
 def bench_serve(G, args) -> int:
-    """Serving: Poisson arrivals of SHA-256 proving requests (a compression
-    chain of blocks = 2^U blocks, U uniform in 0..8, message and chaining
-    value uniform from the seed; see bitz_amd/serving.py), admitted by HBM
-    budget into slots.  Each request: witness (host), assignment, Az/Bz/Cz
-    and the PIOP on the device.  The requests are served with continuous
-    batching (one engine steps every proof in flight, each round kind in one
-    launch) and with concurrent slots (a thread and streams per request); the
-    same requests all queued at once give each mode's capacity, and proved
-    one after another the plain rate.  value = the better capacity;
-    p50/p99 = latency at the Poisson rate in that mode."""
+    """The serving workload (see the module notes).  The CPU baseline runs
+    first, before anything touches the GPU; then the circuits are prepared
+    and the kernels warmed, --warmup untimed steps run, and --steps timed
+    steps each serve a fresh set of --serve-requests requests, all queued at
+    once, with continuous batching."""
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_serve(args)
     from bitz_amd import serving as S
     srv = S.ProvingServer(hbm_budget=int(args.serve_budget_gib * (1 << 30)), slots=args.serve_slots)
     log("bench: preparing the nine SHA-256 circuits (1 .. 256 blocks)")
@@ -531,60 +573,138 @@ def bench_serve(G, args) -> int:
     for u in range(9):  # circuits and kernels warm, like a deployed service
         b = 1 << u
         srv.prove(10**6 + u, b, *S.random_input(rng, b))
-    srv.serve_batched([S.Request(10**6 + 10 + u, 0.0, 1 << u, *S.random_input(rng, 1 << u)) for u in range(9)])
     G.synchronize()
-    reqs = S.poisson_requests(args.serve_rate, args.serve_requests, args.seed)
-    blocks = sum(r.blocks for r in reqs)
-    log(f"bench: serving {len(reqs)} requests at {args.serve_rate}/s, {blocks} blocks")
 
-    def run(mode: str, arrivals: bool) -> dict:
-        rs = [S.Request(r.rid, r.arrival if arrivals else 0.0, r.blocks, r.message, r.iv) for r in reqs]
+    engines = args.serve_engines or max(1, args.serve_slots // 8)
+
+    def queued(seed: int):
+        rs = serve_mix(seed, args.serve_requests, 1.0)
+        for r in rs:
+            r.arrival = 0.0
+        return rs
+
+    for i in range(args.warmup):
         t0 = time.perf_counter()
-        (srv.serve_batched if mode == "batched" else srv.serve)(rs)
-        wall = time.perf_counter() - t0
+        srv.serve_batched(queued(args.seed + 500 + i), engines=engines)
+        log(f"bench: warmup {i + 1}/{args.warmup}: {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    # Steps: inputs drawn before the clock starts; HIP events on the engines'
+    # streams time every batched inner fold launch.
+    step_reqs = [queued(args.seed + i) for i in range(args.steps)]
+    blocks = sum(r.blocks for rs in step_reqs for r in rs)
+    G.synchronize()
+    # Under rocprofv3 (which exports ROCPROFILER_LIBRARY_CTOR to the program)
+    # the kernel tracer times the launches and the events stay off: a run
+    # with both ended in a host fault inside the tracer's hipStreamSynchronize
+    # on an engine thread.
+    events = "ROCPROFILER_LIBRARY_CTOR" not in os.environ
+    G.profile_enable(events)
+    G.profile_only("inner_fold_batch")
+    step_ms = []
+    t_start = time.perf_counter()
+    for rs in step_reqs:
+        t0 = time.perf_counter()
+        srv.serve_batched(rs, engines=engines)
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    G.synchronize()
+    wall = time.perf_counter() - t_start
+    fold = G.profile_read("inner_fold_batch")
+    G.profile_only(None)
+    G.profile_enable(False)
+    queued_lat = np.array([r.latency for rs in step_reqs for r in rs]) * 1e3
+    value = blocks / wall
+    log(f"bench: {args.steps} steps, {blocks} blocks in {wall:.3f} s: {value:.0f} blocks/s")
+
+    def poisson(mode: str, rate: float, seed: int) -> dict:
+        rs = serve_mix(seed, args.serve_latency_requests, rate)
+        t0 = time.perf_counter()
+        if mode == "batched":
+            srv.serve_batched(rs, engines=engines)
+        else:
+            srv.serve(rs)
         span = max(r.done for r in rs) - min(r.arrival for r in rs)
         lat = np.array([r.latency for r in rs]) * 1e3
-        return {"blocks_per_s": blocks / (span if arrivals else wall), "proofs_per_s": len(rs) / (span if arrivals else wall),
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "peak_in_flight": srv.peak_in_flight, "peak_hbm_gib": srv.peak_bytes / (1 << 30)}
+        b = sum(r.blocks for r in rs)
+        return {"rate_per_s": rate, "offered_blocks_per_s": b / max(r.arrival for r in rs),
+                "blocks_per_s": b / span, "p50_ms": float(np.percentile(lat, 50)),
+                "p90_ms": float(np.percentile(lat, 90)), "p99_ms": float(np.percentile(lat, 99)),
+                "peak_in_flight": srv.peak_in_flight, "peak_hbm_gib": srv.peak_bytes / (1 << 30),
+                "wall_s": time.perf_counter() - t0}
 
-    res = {mode: {"poisson": run(mode, True), "saturated": run(mode, False)} for mode in ("batched", "slots")}
+    if args.capacity_only:
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "SHA-256 blocks/s", "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+                          "inner_fold_batch": fold}), flush=True)
+        return 0
+    rates = sorted({float(x) for x in args.serve_rates.split(",") if x} | {args.serve_rate})
+    curve = [poisson("batched", rate, args.seed + 100 + k) for k, rate in enumerate(rates)]
+    for c in curve:
+        log(f"bench: poisson {c['rate_per_s']:.0f}/s: p50 {c['p50_ms']:.2f} ms, p99 {c['p99_ms']:.2f} ms")
+    head = next(c for c in curve if c["rate_per_s"] == args.serve_rate)
+    # The concurrent-slots mode and the plain path on the first step's requests.
+    rs = queued(args.seed)
     t0 = time.perf_counter()
-    for r in reqs:
+    srv.serve(rs)
+    slots_sat = sum(r.blocks for r in rs) / (time.perf_counter() - t0)
+    slots_poisson = poisson("slots", args.serve_rate, args.seed + 100 + rates.index(args.serve_rate))
+    rs = queued(args.seed)
+    t0 = time.perf_counter()
+    for r in rs:
         srv.prove(r.rid, r.blocks, r.message, r.iv)
     G.synchronize()
-    serial = time.perf_counter() - t0
-    best = max(res, key=lambda m: res[m]["saturated"]["blocks_per_s"])
+    plain = sum(r.blocks for r in rs) / (time.perf_counter() - t0)
     c256 = srv.circuit(256)
+    achieved = fold["bytes"] / (fold["ms"] * 1e-3) / 1e9 if fold["ms"] else 0.0
     line = {
         "metric": METRIC,
-        "value": res[best]["saturated"]["blocks_per_s"],
+        "value": value,
         "unit": "SHA-256 blocks/s",
         "n_gpus": 1,
-        "steps": len(reqs),
-        "warmup": 18,
-        "ms_per_step": 1e3 * blocks / res[best]["saturated"]["blocks_per_s"] / len(reqs),
-        "p50_ms": res[best]["poisson"]["p50_ms"],
-        "p99_ms": res[best]["poisson"]["p99_ms"],
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "gf2_128",
         "data": "synthetic (message bytes and chaining value per request uniform from the seed)",
-        "config": {"workload": f"serving SHA-256 compression chains: {len(reqs)} requests, blocks 2^U with U uniform "
-                               f"in 0..8, {args.serve_slots} slots (batched: {max(1, args.serve_slots // 8)} engines), "
-                               f"HBM budget {args.serve_budget_gib} GiB; each "
-                               "request: witness, assignment MLE, Az/Bz/Cz, prove_spartan_piop (duplex transcript). "
-                               f"value: capacity with every request queued ({best} mode); p50/p99: latency under "
-                               f"Poisson arrivals at {args.serve_rate}/s in that mode",
-                   "requests": len(reqs), "blocks": blocks, "mode": best,
+        "config": {"workload": f"serving SHA-256 compression chains (config C5): a step serves {args.serve_requests} "
+                               "requests of 2^U blocks (U uniform in 0..8) queued at once, with continuous batching "
+                               f"({engines} PIOP engines, {args.serve_slots} slots, HBM budget "
+                               f"{args.serve_budget_gib} GiB); each request: witness (host), assignment MLE, "
+                               "Az/Bz/Cz, prove_spartan_piop (duplex transcript), every table on the device. "
+                               f"p50/p99: per-request latency under Poisson arrivals at {args.serve_rate:g}/s",
+                   "requests_per_step": args.serve_requests, "blocks_timed": blocks,
+                   "poisson_rate_per_s": args.serve_rate,
                    "circuit_256_blocks": {"rows": c256.rows, "columns": c256.columns, "nnz": c256.nnz}},
-        "modes": res,
-        "one_at_a_time": {"blocks_per_s": blocks / serial, "proofs_per_s": len(reqs) / serial},
-        "capacity_vs_one_at_a_time": res[best]["saturated"]["blocks_per_s"] / (blocks / serial),
+        "step_ms": {"p50": float(np.percentile(step_ms, 50)), "max": float(max(step_ms))},
+        "saturated_latency_ms": {"p50": float(np.percentile(queued_lat, 50)),
+                                 "p99": float(np.percentile(queued_lat, 99))},
+        "latency_curve": curve,
+        "slots_mode": {"blocks_per_s": slots_sat, "poisson": slots_poisson},
+        "plain_one_at_a_time": {"blocks_per_s": plain},
+        "speedup_vs_plain": value / plain,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_inner_fold_batch (the engines' batched inner-sumcheck fold rounds)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "launches": fold["launches"],
+            "avg_launch_us": fold["ms"] * 1e3 / max(fold["launches"], 1),
+            "algorithmic_bytes_per_launch": fold["bytes"] / max(fold["launches"], 1),
+        },
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_sha256(args.seed)
+    if not args.no_outer:
+        del srv
+        o = run_outer(G, 24, 10, 3, args.seed)
+        line["outer_2e24"] = {k: o[k] for k in ("value", "unit", "p50_ms", "p99_ms", "transcript_mode_p50_ms",
+                                                "elements_folded_per_s", "proof_hbm_gbs", "roofline")}
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
     return 0
 
@@ -648,58 +768,130 @@ def bench_chain(G, args) -> int:
                    "row_vars": c.num_row_vars, "col_vars": c.num_column_vars, "setup_s": setup},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_sha256(args.seed)
+        line["cpu_baseline"] = cpu_baseline_sha256(SH.CHAIN, B, msg, args.seed)
     print(json.dumps(line), flush=True)
     return 0
 
 
-def cpu_baseline_sha256(seed: int, budget_s: float = 10.0) -> dict:
-    """The oracle proving one-block SHA-256 compressions (random message and
-    chaining value) on one host core with the bit-serial multiply: Az/Bz/Cz,
-    outer sumcheck, bind_and_batch, inner sumcheck, repeated for about
-    budget_s seconds."""
-    from bitz_amd import sha256 as SH
-    from oracle import oracle as O
-    O.set_kernel("portable")
-    c = SH.Sha256Circuit(SH.COMPRESSION_CHAIN, 1)
-    _, digest = c.prepare(matrices=False)
-    ex = c.export()
-    mats = [O.CsrMatrix(ex[k][0], ex[k][1], ex[k][3]) for k in "abc"]
-    rv, cv = c.num_row_vars, c.num_column_vars
-    rng = np.random.default_rng(seed)
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or done == 0:
-        msg = rng.integers(0, 256, 64, dtype=np.uint8).tobytes()
-        iv = [int(x) for x in rng.integers(0, 2**32, 8, dtype=np.uint64)]
-        w = c.witness(msg, iv=iv)
+class OracleCircuit:
+    """A SHA-256 circuit for the oracle: the producer's matrices in the
+    parity projection as oracle CSR matrices, and its statement digest."""
+
+    def __init__(self, kind: int, blocks: int, statement: bytes | None = None):
+        from bitz_amd import sha256 as SH
+        from oracle import oracle as O
+        self.c = SH.Sha256Circuit(kind, blocks, statement=statement)
+        _, self.digest = self.c.prepare(matrices=False)
+        ex = self.c.export()
+        self.mats = [O.CsrMatrix(ex[k][0], ex[k][1], ex[k][3]) for k in "abc"]
+
+    def prove(self, message: bytes, iv=None, label: bytes = b"cpu/sha256") -> None:
+        """One request on the oracle: witness (the producer, host), the
+        assignment, Az/Bz/Cz (the reference tests' multiply), and
+        prove_spartan_piop's sequence (crates/spartan/src/piop.rs:67-112):
+        digest, tau, outer sumcheck, rho, bind_and_batch, inner sumcheck."""
+        from oracle import oracle as O
+        c = self.c
+        rv, cv = c.num_row_vars, c.num_column_vars
+        w = c.witness(message, iv=iv)
         bits = np.unpackbits(w.h.view(np.uint8), bitorder="little")[: c.columns]
         h = np.zeros((1 << cv, 2), np.uint64)
         h[: c.columns, 0] = bits
-        prods = []
-        for k in "abc":  # A h, B h, C h: one multiply per nonzero, XOR per row
-            st, co, _, par = ex[k]
-            terms = O.mul_batch(par, h[co.astype(np.int64)])
-            p = np.zeros((1 << rv, 2), np.uint64)
-            rows_of = np.repeat(np.arange(c.rows), np.diff(st.astype(np.int64)))
-            np.bitwise_xor.at(p, rows_of, terms)
-            prods.append(p)
-        t = O.DuplexTranscript(b"cpu/sha256")
-        O.absorb(t, 0, digest)
+        prods = [O.csr_multiply(m, c.rows, h, 1 << rv) for m in self.mats]
+        t = O.DuplexTranscript(label)
+        O.absorb(t, 0, self.digest)
         tau = np.array([[(x := O.squeeze(t)) & (2**64 - 1), x >> 64] for _ in range(rv)], np.uint64)
         lo, hi = O.eq_table(tau[: rv // 2]), O.eq_table(tau[rv // 2:])
         rc, outer = O.outer_sumcheck(t, 0, lo, hi, *prods)
         rho = O.squeeze(t)
         a_c, b_c, c_c = (O.to_int(x) for x in outer["claims"])
         claim = a_c ^ O.to_int(O.mul(rho, b_c)) ^ O.to_int(O.mul(O.to_int(O.mul(rho, rho)), c_c))
-        d = O.bind_and_batch(mats, c.rows, rv, cv, outer["points"], rho)
+        d = O.bind_and_batch(self.mats, c.rows, rv, cv, outer["points"], rho)
         rc2, _ = O.inner_sumcheck(t, claim, d, h)
         assert rc == 0 and rc2 == 0
-        done += 1
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "SHA-256 blocks/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, bit-serial multiply, 1 thread: {done} one-block compression proofs "
-                      f"(witness, Az/Bz/Cz, outer 2^{rv}, bind_and_batch over {sum(c.nnz)} nonzeros, "
-                      f"inner 2^{cv}) in {dt:.1f} s"}
+
+
+def _cpu_time_requests(blocks_list, seed: int, kernel: str) -> list:
+    """Seconds per request for each block count in blocks_list (compression
+    chains, message and chaining value from the seed), on this process."""
+    from bitz_amd import sha256 as SH
+    from oracle import oracle as O
+    O.set_kernel(kernel)
+    rng = np.random.default_rng(seed)
+    circuits, out = {}, []
+    for b in blocks_list:
+        if b not in circuits:
+            circuits[b] = OracleCircuit(SH.COMPRESSION_CHAIN, b)
+        msg = rng.integers(0, 256, 64 * b, dtype=np.uint8).tobytes()
+        iv = [int(x) for x in rng.integers(0, 2**32, 8, dtype=np.uint64)]
+        t0 = time.perf_counter()
+        circuits[b].prove(msg, iv)
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def cpu_baseline_serve(args) -> dict:
+    """The oracle proving the serving workload's requests on the host, with
+    the bit-serial multiply the reference selects on x86
+    (crates/field/src/gf128.rs:40-49), before the GPU is touched.
+
+    1 thread: one request of each block count 1, 2, ..., 256 (the mix draws
+    them with equal probability), timed one after another; requests of 1
+    block, the configs' CPU case, 11 times (median).  The mix rate is the
+    blocks of the first timed step's requests over the sum of their sizes'
+    times.  Full cores: P worker processes each prove the same list of
+    requests (16 blocks, 4 times) at once; their aggregate rate over the
+    1-thread rate of the same list scales the mix rate."""
+    import multiprocessing as mp
+    kernel = "portable"
+    log("bench: CPU baseline (oracle, bit-serial multiply), 1 thread")
+    one = _cpu_time_requests([1] * 11, args.seed + 7, kernel)
+    sizes = [1 << u for u in range(9)]
+    per = dict(zip(sizes, _cpu_time_requests(sizes, args.seed + 8, kernel)))
+    per[1] = float(np.median(one))
+    mix = serve_mix(args.seed, args.serve_requests, 1.0)
+    mix_blocks = sum(r.blocks for r in mix)
+    mix_s = sum(per[r.blocks] for r in mix)
+    rate1 = mix_blocks / mix_s
+    workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+    sample = [16] * 4
+    single = sum(_cpu_time_requests(sample, args.seed + 9, kernel))
+    log(f"bench: CPU baseline, {workers} worker processes")
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        pool.starmap(_cpu_time_requests, [(sample, args.seed + 10 + w, kernel) for w in range(workers)])
+    par = time.perf_counter() - t0
+    agg = workers * sum(sample) / par
+    scale = agg / (sum(sample) / single)
+    return {"value": rate1, "unit": "SHA-256 blocks/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C restatement of the reference, bit-serial multiply), 1 thread: one compression "
+                       f"chain request of each size 1..256 blocks ({sum(per[b] for b in sizes):.1f} s), "
+                       f"weighted by the {args.serve_requests} requests ({mix_blocks} blocks) of the first timed "
+                       "step; each request: witness, assignment, Az/Bz/Cz, outer sumcheck, bind_and_batch, "
+                       "inner sumcheck under the duplex transcript"),
+            "seconds_per_request": {str(b): per[b] for b in sizes},
+            "c1_one_block_median_ms": 1e3 * per[1], "c1_runs": len(one),
+            "full_cores": {"value": rate1 * scale, "cores": workers,
+                           "sample": (f"{workers} processes each proving 4 requests of 16 blocks at once: "
+                                      f"{agg:.1f} blocks/s aggregate, {scale:.2f}x the 1-thread rate of the same "
+                                      "requests, applied to the mix rate (process start-up included)")}}
+
+
+def cpu_baseline_sha256(kind: int, blocks: int, statement: bytes, seed: int, runs: int = 3) -> dict:
+    """The oracle proving the given SHA-256 statement on one host thread with
+    the bit-serial multiply, median of `runs`."""
+    from oracle import oracle as O
+    O.set_kernel("portable")
+    oc = OracleCircuit(kind, blocks, statement=statement)
+    times = []
+    for i in range(runs):
+        t0 = time.perf_counter()
+        oc.prove(statement, label=b"cpu/%d" % i)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": blocks / med, "unit": "SHA-256 blocks/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, bit-serial multiply, 1 thread: median of {runs} proofs of the statement, {med:.2f} s"}
 
 
 def bench_bind(G, args) -> int:

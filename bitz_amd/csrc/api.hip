@@ -2940,6 +2940,9 @@ struct bitz_piop_engine {
     bool running = false, stop = false;
     int32_t worker_rc = BITZ_OK;
     std::string worker_err;
+    // HIP events around the step's batched inner fold (when profiling), read
+    // once the step has synchronised.
+    Timed t_inner_fold;
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -3241,10 +3244,17 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
         hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)i_r0_map.size()), dim3(bitz::kInnerBlock),
                            bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + ir_o),
                            reinterpret_cast<const uint32_t*>(dst + irm_o));
-    if (!i_fold.empty())
+    if (!i_fold.empty()) {
+        // Algorithmic bytes: per output pair, 4 + 4 inputs of D and h read and
+        // 2 + 2 folded entries written.
+        double fold_bytes = 0;
+        for (const auto& d : i_fold) fold_bytes += 192.0 * (double)d.pairs;
+        e->t_inner_fold.begin(s, "inner_fold_batch", fold_bytes);
         hipLaunchKernelGGL(bitz::k_inner_fold_batch, dim3((unsigned)i_fold_map.size()), dim3(bitz::kInnerBlock),
                            bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + if_o),
                            reinterpret_cast<const uint32_t*>(dst + ifm_o));
+        e->t_inner_fold.end(s);
+    }
     TRY(check_launch());
     HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_bytes, hipMemcpyDeviceToHost, s));
     e->launched = std::move(act);
@@ -3257,6 +3267,7 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
     std::vector<PiopJob*> act = std::move(e->launched);
     e->launched.clear();
     if (!act.empty()) HIP_TRY(hipStreamSynchronize(e->so.s));
+    e->t_inner_fold.collect();
     const char* pin = static_cast<const char*>(e->pin);
     // Host: every proof's round polynomial into its transcript, its next challenge.
     for (size_t i = 0; i < act.size(); ++i) {

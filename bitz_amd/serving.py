@@ -115,6 +115,14 @@ class ProvingServer:
         self._lock = threading.Lock()
         self.peak_bytes = 0      # largest footprint in flight during the last serve()
         self.peak_in_flight = 0  # most requests in flight during the last serve()
+        self._engines: list[PiopEngine] = []  # kept across serve_batched calls
+
+    def engines(self, count: int) -> list[PiopEngine]:
+        """count background PIOP engines, created on first use and kept (each
+        holds a stream, a thread and its staging buffers)."""
+        while len(self._engines) < count:
+            self._engines.append(PiopEngine(background=True))
+        return self._engines[:count]
 
     def circuit(self, blocks: int) -> Circuit:
         with self._lock:
@@ -151,7 +159,7 @@ class ProvingServer:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         for r in pending:
             self.circuit(r.blocks)
-        pool_engines = [PiopEngine(background=True) for _ in range(engines)]
+        pool_engines = self.engines(engines)
         load = [0] * engines
         by_tag: dict[int, tuple] = {}
         used = 0

@@ -132,6 +132,10 @@ typedef struct {
 int oracle_bind_and_batch(const oracle_csr mats[3], size_t rows, int row_vars,
                           int col_vars, const of128* row_point, of128 rho,
                           of128* out);
+/* The reference tests' `multiply` (crates/spartan/src/piop.rs:352-364):
+ * out[i] = sum over row i's entries of coefficient * assignment[column], for
+ * i < rows; out has `rows` entries (padding is the caller's). */
+void oracle_csr_multiply(const oracle_csr* m, size_t rows, const of128* assignment, of128* out);
 /* evaluate_batched, matrix.rs:477-529 (direct, not via the bound table). */
 int oracle_evaluate_batched(const oracle_csr mats[3], size_t rows, int row_vars,
                             int col_vars, const of128* row_point, of128 rho,

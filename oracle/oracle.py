@@ -104,6 +104,7 @@ def lib():
         L.oracle_outer_verify.argtypes = [P, OF128, P, C.c_int, P, P, P]
         L.oracle_bind_and_batch.restype = C.c_int
         L.oracle_bind_and_batch.argtypes = [P, C.c_size_t, C.c_int, C.c_int, P, OF128, P]
+        L.oracle_csr_multiply.argtypes = [P, C.c_size_t, P, P]
         L.oracle_evaluate_batched.restype = C.c_int
         L.oracle_evaluate_batched.argtypes = [P, C.c_size_t, C.c_int, C.c_int, P, OF128, P, P]
         L.oracle_mt_apply.argtypes = [P, C.c_size_t, P, P, P]
@@ -360,6 +361,16 @@ def bind_and_batch(mats, rows, row_vars, col_vars, row_point, rho):
     rc = lib().oracle_bind_and_batch(cs, rows, row_vars, col_vars, ptr(row_point), f(rho), ptr(out))
     assert rc == 0
     return out
+
+
+def csr_multiply(m: "CsrMatrix", rows: int, assignment, padded: int) -> np.ndarray:
+    """The reference tests' multiply + padded_mle (piop.rs:352-372): A h per
+    row, zero-padded to `padded` entries."""
+    assignment = as_elems(assignment)
+    out = arr(max(padded, rows, 1))
+    cs = m.c()
+    lib().oracle_csr_multiply(C.byref(cs), rows, ptr(assignment), ptr(out))
+    return out[:padded]
 
 
 def evaluate_batched(mats, rows, row_vars, col_vars, row_point, rho, col_point):

@@ -317,6 +317,16 @@ int oracle_bind_and_batch(const oracle_csr mats[3], size_t rows, int row_vars, i
     return ORACLE_OK;
 }
 
+/* piop.rs:352-364 multiply: one product per entry, summed per row. */
+void oracle_csr_multiply(const oracle_csr* m, size_t rows, const of128* assignment, of128* out) {
+    for (size_t i = 0; i < rows; ++i) {
+        of128 sum = ZERO;
+        for (uint64_t k = m->row_starts[i]; k < m->row_starts[i + 1]; ++k)
+            sum = fadd(sum, oracle_mul(m->values[k], assignment[m->columns[k]]));
+        out[i] = sum;
+    }
+}
+
 /* matrix.rs:477-529 evaluate_batched, independent of bind_and_batch:
  * sum over nonzeros of scale * eq(i, r_x) * eq(j, r_y) * value. */
 int oracle_evaluate_batched(const oracle_csr mats[3], size_t rows, int row_vars, int col_vars,

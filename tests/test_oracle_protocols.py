@@ -249,6 +249,22 @@ def test_bind_and_batch_matches_dense_brute_force_and_direct_evaluation():
     assert O.mle_evaluate(d, ry) == direct
 
 
+def test_csr_multiply_matches_the_dense_product():
+    """A h per row (the reference tests' multiply), zero-padded: against the
+    dense matrix-vector product on the big-int model."""
+    rng = random.Random(12)
+    rows, cols = 11, 16
+    M = random_csr(rng, rows, cols, 5, 200)
+    h = elements(13, cols)
+    out = O.csr_multiply(M, rows, h, 16)
+    dense = [0] * 16
+    hi = ints(h)
+    for i in range(rows):
+        for k in range(int(M.row_starts[i]), int(M.row_starts[i + 1])):
+            dense[i] ^= gf_mul(to_int(M.values[k]), hi[int(M.columns[k])])
+    assert ints(out) == dense
+
+
 def test_mt_apply_and_transpose_inner_match_dense():
     rng = random.Random(3)
     rows, cols = 40, 17
