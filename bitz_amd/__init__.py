@@ -896,13 +896,18 @@ class PiopEngine:
             raise BitzError(-7, "the engine takes device tables")
         dg = (C.c_uint8 * 32).from_buffer_copy(digest)
         # Held before the native submit: a background engine may finish the
-        # proof (and another thread take it) before submit returns.
+        # proof (and another thread take it) before submit returns.  A
+        # rejected submit (a tag in use, say) restores what was held before.
+        before = self._held.get(tag)
         self._held[tag] = (transcript, matrices, products, assignment, _nvars(na), _nvars(nh))
         try:
             _check(lib().bitz_piop_engine_submit(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz),
                                                  _p(cz), _nvars(na), _p(hv), _nvars(nh)), SumcheckError)
         except BaseException:
-            self._held.pop(tag, None)
+            if before is None:
+                self._held.pop(tag, None)
+            else:
+                self._held[tag] = before
             raise
 
     def step(self) -> int:

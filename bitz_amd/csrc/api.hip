@@ -2396,10 +2396,138 @@ done:
     return rc;
 }
 
-// The inner sumcheck with the challenges known in advance: round 0 and the
-// grid-wide fold rounds (lane-table products, r's 4-bit table built on the
-// device) queued back to back, the last rounds in one workgroup, one read of
-// every round's [c0, c2] and of D(r), h(r) at the end.
+// Device buffers of the queued inner prover: the inputs, the ping-pong
+// tables P (2 x capP entries: D then h) and Q (2 x capQ), n challenge tables,
+// the n challenges and the round sums (8 words per round, then D(r), h(r);
+// (n + 2) x 8 words, zero on entry).
+struct InnerQueued {
+    const f128* d = nullptr;
+    const f128* w = nullptr;
+    f128* P = nullptr;
+    f128* Q = nullptr;
+    size_t capP = 0, capQ = 0;
+    uint4* rt4 = nullptr;
+    const f128* chal = nullptr;
+    uint32_t* acc = nullptr;
+};
+
+// Enqueues every round of the queued inner sumcheck over 2^n entries (n >= 1)
+// on stream s: the challenge tables, round 0 and the grid-wide fold rounds
+// (lane-table products, r's 4-bit table) back to back, the middle rounds in
+// the chunk stage, the last rounds in one workgroup.  ts (k0 + 2 entries,
+// or null) times the launches when profiling is on.
+static int32_t inner_queued_enqueue(const InnerQueued& B, int n, hipStream_t s, std::vector<Timed>* ts) {
+    hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)n), dim3(512), 0, s, B.chal, B.rt4);
+    TRY(check_launch());
+    inner_set_attributes();
+    const uint4* rt = B.rt4;
+    const int k0 = std::max(0, n - bitz::kInnerTailMaxLog);
+    // Rounds [kc, k0) run in the chunk stage (each workgroup folds its chunk
+    // of 2^kInnerChunkLog inputs round after round in LDS), the grid-wide
+    // kernels take [0, kc).
+    const int kc = k0 - std::max(0, std::min(bitz::kInnerChunkRounds, k0 - 1));
+    auto grid = [](size_t work) { return inner_grid(work); };
+    auto timed = [&](int i) -> Timed* { return ts ? &(*ts)[i] : nullptr; };
+    const size_t N = (size_t)1 << n;
+    size_t len = N;
+    const f128* cur[2] = {B.d, B.w};
+    int where = 0;
+    // The grid-wide fold rounds run back to back: one event pair spans them.
+    double fold_bytes = 0;
+    for (int k = 1; k < kc; ++k) fold_bytes += 96.0 * (double)(N >> k);
+    for (int k = 0; k < kc; ++k) {
+        if (k == 0) {
+            if (Timed* t = timed(0)) t->begin(s, "inner_round0_lane", 32.0 * (double)len);
+            hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(grid(len / 2)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
+                               s, cur[0], cur[1], len / 2, B.acc);
+        } else {
+            f128* dst = (where == 1) ? B.Q : B.P;
+            const size_t cap = (where == 1) ? B.capQ : B.capP;
+            f128* out[2] = {dst, dst + cap};
+            const size_t out_pairs = len / 4;
+            if (k == 1)
+                if (Timed* t = timed(1)) {
+                    t->begin(s, "inner_fold_lane", fold_bytes);
+                    t->launches = (uint64_t)(kc - 1);
+                }
+            hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
+                               bitz::kInnerGridSmem, s, cur[0], cur[1], out[0], out[1], out_pairs,
+                               rt + (size_t)(k - 1) * kConstTable4Entries, B.acc + 8 * (size_t)k);
+            cur[0] = out[0];
+            cur[1] = out[1];
+            where = (where == 1) ? 2 : 1;
+            len /= 2;
+        }
+        if (k == 0 || k == kc - 1)
+            if (Timed* t = timed(k == 0 ? 0 : 1)) t->end(s);
+        TRY(check_launch());
+    }
+    if (kc < k0) {
+        // len = 2^(n - kc + 1) entries per table before the fold by r_{kc-1}
+        f128* dst = (where == 1) ? B.Q : B.P;
+        const size_t cap = (where == 1) ? B.capQ : B.capP;
+        bitz::InnerTail ca{};
+        ca.d = cur[0];
+        ca.w = cur[1];
+        ca.rtabs4 = rt;
+        ca.acc = B.acc;
+        ca.dout = dst;
+        ca.wout = dst + cap;
+        ca.n = n;
+        ca.k0 = kc;
+        ca.fold_first = 1;
+        ca.lg_in = bitz::kInnerChunkLog;
+        ca.s = k0 - kc;
+        const unsigned chunks = (unsigned)(len >> bitz::kInnerChunkLog);
+        if (Timed* t = timed(k0 + 1)) t->begin(s, "inner_chunk", 0.0);
+        hipLaunchKernelGGL(bitz::k_inner_tail, dim3(chunks), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ca);
+        if (Timed* t = timed(k0 + 1)) t->end(s);
+        TRY(check_launch());
+        cur[0] = ca.dout;
+        cur[1] = ca.wout;
+        where = (where == 1) ? 2 : 1;
+        len >>= (k0 - kc + 1);
+    }
+    bitz::InnerTail ta{};
+    ta.d = cur[0];
+    ta.w = cur[1];
+    ta.rtabs4 = rt;
+    ta.acc = B.acc;
+    ta.fin = reinterpret_cast<f128*>(B.acc + 8 * (size_t)n);
+    ta.n = n;
+    ta.k0 = k0;
+    ta.fold_first = kc == k0 && k0 > 0;
+    ta.lg_in = ta.fold_first ? n - k0 + 1 : n - k0;
+    ta.s = n - k0;
+    if (Timed* t = timed(k0)) t->begin(s, "inner_tail", 0.0);
+    hipLaunchKernelGGL(bitz::k_inner_tail, dim3(1), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ta);
+    if (Timed* t = timed(k0)) t->end(s);
+    return check_launch();
+}
+
+// The round polynomials from the queued prover's sums w ((n + 1) x 8 words):
+// c1 = claim + c2 (g(0) + g(1) = claim in characteristic 2,
+// sumcheck.rs:760-781), claim <- g(r_k), as the transcript-driven prover.
+static void inner_queued_recover(const uint32_t* w, int n, const bitz_f128* challenges, f128 claim,
+                                 bitz_f128* round_polys, bitz_f128* eval_points, bitz_f128* final_claim,
+                                 bitz_f128* evals) {
+    for (int k = 0; k < n; ++k) {
+        f128 c[3];
+        memcpy(&c[0], w + 8 * k, 16);
+        memcpy(&c[2], w + 8 * k + 4, 16);
+        c[1] = hostf::add(claim, c[2]);
+        const f128 r = F(challenges[k]);
+        claim = hostf::horner(c, 3, r);
+        for (int t = 0; t < 3; ++t) round_polys[3 * k + t] = B(c[t]);
+        eval_points[k] = challenges[k];
+    }
+    memcpy(evals, w + 8 * n, 32);
+    *final_claim = B(claim);
+}
+
+// The inner sumcheck with the challenges known in advance: every round
+// queued back to back (inner_queued_enqueue), one read of every round's
+// [c0, c2] and of D(r), h(r) at the end.
 extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* challenges, bitz_f128 initial_claim,
                                                              const bitz_f128* matrix, uint32_t matrix_vars,
                                                              const bitz_f128* witness, uint32_t witness_vars,
@@ -2431,105 +2559,25 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     TRY(up.alloc((size_t)n * sizeof(f128)));
     memcpy(up.p, challenges, (size_t)n * sizeof(f128));
     HIP_TRY(hipMemcpyAsync(chal.p, up.p, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)n), dim3(512), 0, s, chal.f(), static_cast<uint4*>(rt4.p));
-    TRY(check_launch());
-    inner_set_attributes();
-    const uint4* rt = static_cast<const uint4*>(rt4.p);
-    const int k0 = std::max(0, n - bitz::kInnerTailMaxLog);
-    // Rounds [kc, k0) run in the chunk stage (each workgroup folds its chunk
-    // of 2^kInnerChunkLog inputs round after round in LDS), the grid-wide
-    // kernels take [0, kc).
-    const int kc = k0 - std::max(0, std::min(bitz::kInnerChunkRounds, k0 - 1));
-    auto grid = [](size_t work) { return inner_grid(work); };
-    std::vector<Timed> ts(k0 + 2);
-    size_t len = h->len;
-    // The grid-wide fold rounds run back to back: one event pair spans them.
-    double fold_bytes = 0;
-    for (int k = 1; k < kc; ++k) fold_bytes += 96.0 * (double)(h->len >> k);
-    for (int k = 0; k < kc; ++k) {
-        if (k == 0) {
-            ts[k].begin(s, "inner_round0_lane", 32.0 * (double)len);
-            hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(grid(len / 2)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
-                               s, h->cur[0], h->cur[1], len / 2, h->acc.u32());
-        } else {
-            Dev& dst = (h->where == 1) ? h->Q : h->P;
-            const size_t cap = (h->where == 1) ? h->capQ : h->capP;
-            f128* out[2] = {dst.f(), dst.f() + cap};
-            const size_t out_pairs = len / 4;
-            if (k == 1) {
-                ts[1].begin(s, "inner_fold_lane", fold_bytes);
-                ts[1].launches = (uint64_t)(kc - 1);
-            }
-            hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
-                               bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], out_pairs,
-                               rt + (size_t)(k - 1) * kConstTable4Entries, h->acc.u32() + 8 * (size_t)k);
-            h->cur[0] = out[0];
-            h->cur[1] = out[1];
-            h->where = (h->where == 1) ? 2 : 1;
-            len /= 2;
-        }
-        if (k == 0 || k == kc - 1) ts[k == 0 ? 0 : 1].end(s);
-        TRY(check_launch());
-    }
-    if (kc < k0) {
-        // len = 2^(n - kc + 1) entries per table before the fold by r_{kc-1}
-        Dev& dst = (h->where == 1) ? h->Q : h->P;
-        const size_t cap = (h->where == 1) ? h->capQ : h->capP;
-        bitz::InnerTail ca{};
-        ca.d = h->cur[0];
-        ca.w = h->cur[1];
-        ca.rtabs4 = rt;
-        ca.acc = h->acc.u32();
-        ca.dout = dst.f();
-        ca.wout = dst.f() + cap;
-        ca.n = n;
-        ca.k0 = kc;
-        ca.fold_first = 1;
-        ca.lg_in = bitz::kInnerChunkLog;
-        ca.s = k0 - kc;
-        const unsigned chunks = (unsigned)(len >> bitz::kInnerChunkLog);
-        ts[k0 + 1].begin(s, "inner_chunk", 0.0);
-        hipLaunchKernelGGL(bitz::k_inner_tail, dim3(chunks), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ca);
-        ts[k0 + 1].end(s);
-        TRY(check_launch());
-        h->cur[0] = ca.dout;
-        h->cur[1] = ca.wout;
-        h->where = (h->where == 1) ? 2 : 1;
-        len >>= (k0 - kc + 1);
-    }
-    bitz::InnerTail ta{};
-    ta.d = h->cur[0];
-    ta.w = h->cur[1];
-    ta.rtabs4 = rt;
-    ta.acc = h->acc.u32();
-    ta.fin = reinterpret_cast<f128*>(h->acc.u32() + 8 * (size_t)n);
-    ta.n = n;
-    ta.k0 = k0;
-    ta.fold_first = kc == k0 && k0 > 0;
-    ta.lg_in = ta.fold_first ? n - k0 + 1 : n - k0;
-    ta.s = n - k0;
-    ts[k0].begin(s, "inner_tail", 0.0);
-    hipLaunchKernelGGL(bitz::k_inner_tail, dim3(1), dim3(bitz::kStageBlock), bitz::kInnerTailSmem, s, ta);
-    ts[k0].end(s);
-    TRY(check_launch());
+    InnerQueued q;
+    q.d = h->cur[0];
+    q.w = h->cur[1];
+    q.P = h->P.f();
+    q.Q = h->Q.f();
+    q.capP = h->capP;
+    q.capQ = h->capQ;
+    q.rt4 = static_cast<uint4*>(rt4.p);
+    q.chal = chal.f();
+    q.acc = h->acc.u32();
+    std::vector<Timed> ts(std::max(0, n - bitz::kInnerTailMaxLog) + 2);
+    TRY(inner_queued_enqueue(q, n, s, &ts));
     Pinned down;
     TRY(down.alloc(((size_t)n + 1) * 32));
     HIP_TRY(hipMemcpyAsync(down.p, h->acc.p, ((size_t)n + 1) * 32, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (auto& t : ts) t.collect();
-    const uint32_t* w = static_cast<const uint32_t*>(down.p);
-    for (int k = 0; k < n; ++k) {
-        f128 c[3];
-        memcpy(&c[0], w + 8 * k, 16);
-        memcpy(&c[2], w + 8 * k + 4, 16);
-        c[1] = hostf::add(claim, c[2]);  // g(0) + g(1) = claim in characteristic 2 (sumcheck.rs:760-781)
-        const f128 r = F(challenges[k]);
-        claim = hostf::horner(c, 3, r);
-        for (int t = 0; t < 3; ++t) round_polys[3 * k + t] = B(c[t]);
-        eval_points[k] = challenges[k];
-    }
-    memcpy(evals, w + 8 * n, 32);
-    *final_claim = B(claim);
+    inner_queued_recover(static_cast<const uint32_t*>(down.p), n, challenges, claim, round_polys, eval_points,
+                         final_claim, evals);
     return BITZ_OK;
 }
 
