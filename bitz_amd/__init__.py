@@ -106,6 +106,7 @@ def lib():
             "bitz_mul_batch": (I32, [P, P, P, SZ, U32]),
             "bitz_gkr_prove_with_challenges": (I32, [P, P, U32, P, SZ, P, P, P, P]),
             "bitz_prove_inner_sumcheck_with_challenges": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
+            "bitz_prove_inner_sumcheck_with_challenges_noalloc": (I32, [P, F128, P, U32, P, U32, P, P, P, P, P]),
             "bitz_r1cs_products": (I32, [P, P, P, P, P, U32]),
             "bitz_assignment_mle": (I32, [P, SZ, SZ, P, U32]),
             "bitz_constraint_matrix_digest": (I32, [SZ, SZ, P, P, SZ, SZ, P, P, P, P]),
@@ -701,6 +702,34 @@ def prove_inner_sumcheck_with_challenges(challenges, initial_claim, batched_matr
     _check(rc, SumcheckError)
     return InnerSumcheckOutput(rounds[: 3 * n].reshape(n, 3, 2), points[:n], to_int(final),
                                to_int(evals[0]), to_int(evals[1]))
+
+
+NOALLOC_MAX_VARS = 20
+
+
+def prove_inner_sumcheck_with_challenges_noalloc(challenges, initial_claim, batched_matrix: DeviceTable,
+                                                 witness: DeviceTable):
+    """prove_inner_sumcheck_with_challenges through the allocation-free entry
+    point (bitz_prove_inner_sumcheck_with_challenges_noalloc): device tables
+    of at most 2^NOALLOC_MAX_VARS entries, scratch in the library's static
+    device array (concurrent calls on one GPU race)."""
+    if not (isinstance(batched_matrix, DeviceTable) and isinstance(witness, DeviceTable)):
+        raise BitzError(-7, "the allocation-free prover takes DeviceTables")
+    n = _nvars(batched_matrix.len)
+    nw = _nvars(witness.len)
+    ch = _challenges(challenges, n)
+    init()
+    m = max(n, 1)
+    out = np.zeros((5 * m + 2, 2), dtype=np.uint64)  # rounds | points | evals
+    base = out.ctypes.data
+    final = F128()
+    rc = lib().bitz_prove_inner_sumcheck_with_challenges_noalloc(
+        _p(ch), f128(initial_claim), batched_matrix.ptr, n, witness.ptr, nw, None, base, base + 48 * m,
+        C.byref(final), base + 80 * m)
+    _check(rc, SumcheckError)
+    ev = out[5 * m:]
+    return InnerSumcheckOutput(out[: 3 * n].reshape(n, 3, 2), out[3 * m: 3 * m + n], to_int(final),
+                               to_int(ev[0]), to_int(ev[1]))
 
 
 class PreparedConstraintMatrices:

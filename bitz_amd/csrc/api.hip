@@ -339,11 +339,57 @@ struct Timed {
 
 }  // namespace
 
+// ====================================================== static scratch ===
+// The allocation-free queued inner prover
+// (bitz_prove_inner_sumcheck_with_challenges_noalloc) works in this array:
+// every GPU the library runs on holds its own copy from module load until the
+// process ends.  Host code never uses the array's name as an address: each
+// device's address comes from hipGetSymbolAddress, resolved once per device
+// with that device current (bitz_init, or the entry point's first call on
+// the device).  Layout for 2^n entries, n <= kNoAllocMaxVars: P (D, h folded
+// once: 2 x 2^(n-1) entries) | Q (2 x 2^(n-2)) | n challenge tables | the n
+// challenges | the round sums ((n + 2) x 8 words).
+namespace {
+constexpr int kNoAllocMaxVars = 20;
+constexpr size_t kNoAllocQOff = ((size_t)1 << kNoAllocMaxVars) * sizeof(f128);   // 16 MiB
+constexpr size_t kNoAllocTabOff = kNoAllocQOff + (kNoAllocQOff >> 1);             // 24 MiB
+constexpr size_t kNoAllocChalOff = kNoAllocTabOff + (size_t)kNoAllocMaxVars * kConstTable4Entries * sizeof(uint4);
+constexpr size_t kNoAllocAccOff = kNoAllocChalOff + 512;
+constexpr size_t kNoAllocScratchBytes = kNoAllocAccOff + 1024;
+static_assert(kNoAllocMaxVars * sizeof(f128) <= 512, "challenges fit their slot");
+static_assert((kNoAllocMaxVars + 2) * 32 <= 1024, "round sums fit their slot");
+constexpr int kMaxDevices = 64;
+struct ScratchAddrs {
+    std::mutex mu;
+    void* ptr[kMaxDevices] = {};
+} g_scratch;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// This device's copy of `scratch` (the current device's), resolved on first use.
+static int32_t scratch_address(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(BITZ_ERR_ARG, "device index out of range for the static scratch");
+    std::lock_guard<std::mutex> lock(g_scratch.mu);
+    if (!g_scratch.ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BITZ_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch.ptr[dev] = p;
+    }
+    *out = g_scratch.ptr[dev];
+    return BITZ_OK;
+}
+
 // =============================================================== runtime ===
 extern "C" int32_t bitz_init(int32_t device) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipFree(nullptr));
-    return BITZ_OK;
+    void* p = nullptr;
+    return scratch_address(&p);
 }
 
 extern "C" const char* bitz_last_error(void) { return g_err.c_str(); }
@@ -2578,6 +2624,62 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* ch
     for (auto& t : ts) t.collect();
     inner_queued_recover(static_cast<const uint32_t*>(down.p), n, challenges, claim, round_polys, eval_points,
                          final_claim, evals);
+    return BITZ_OK;
+}
+
+// The queued inner prover without any allocation: the ping-pong tables,
+// challenge tables, challenges and round sums live in the device's static
+// scratch (2^20 entries per table at most).  Calls that overlap on one GPU
+// share that scratch and race; the round sums are cleared by a kernel at the
+// start of every call.
+extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges_noalloc(
+    const bitz_f128* challenges, bitz_f128 initial_claim, const bitz_f128* matrix, uint32_t matrix_vars,
+    const bitz_f128* witness, uint32_t witness_vars, void* stream, bitz_f128* round_polys, bitz_f128* eval_points,
+    bitz_f128* final_claim, bitz_f128* evals) {
+    if (!matrix || !witness || !final_claim || !evals ||
+        (matrix_vars && (!round_polys || !eval_points || !challenges)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    if (matrix_vars > (uint32_t)kNoAllocMaxVars)
+        return fail(BITZ_ERR_ARG, "num_vars " + std::to_string(matrix_vars) + " exceeds the static scratch's maximum of " +
+                                      std::to_string(kNoAllocMaxVars));
+    void* base = nullptr;
+    TRY(scratch_address(&base));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n = (int)matrix_vars;
+    const f128* d = reinterpret_cast<const f128*>(matrix);
+    const f128* w = reinterpret_cast<const f128*>(witness);
+    f128 claim = F(initial_claim);
+    if (n == 0) {
+        HIP_TRY(hipMemcpyAsync(evals, d, sizeof(f128), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(evals + 1, w, sizeof(f128), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        *final_claim = B(claim);
+        return BITZ_OK;
+    }
+    char* sc = static_cast<char*>(base);
+    const size_t N = (size_t)1 << n;
+    InnerQueued q;
+    q.d = d;
+    q.w = w;
+    q.capP = std::max<size_t>(N / 2, 1);
+    q.capQ = std::max<size_t>(N / 4, 1);
+    q.P = reinterpret_cast<f128*>(sc);
+    q.Q = reinterpret_cast<f128*>(sc + kNoAllocQOff);
+    q.rt4 = reinterpret_cast<uint4*>(sc + kNoAllocTabOff);
+    f128* chal = reinterpret_cast<f128*>(sc + kNoAllocChalOff);
+    q.chal = chal;
+    q.acc = reinterpret_cast<uint32_t*>(sc + kNoAllocAccOff);
+    const size_t acc_words = ((size_t)n + 2) * 8;
+    hipLaunchKernelGGL(bitz::k_zero_words, dim3(1), dim3(256), 0, s, q.acc, acc_words);
+    TRY(check_launch());
+    HIP_TRY(hipMemcpyAsync(chal, challenges, (size_t)n * sizeof(f128), hipMemcpyHostToDevice, s));
+    TRY(inner_queued_enqueue(q, n, s, nullptr));
+    alignas(16) uint32_t sums[(kNoAllocMaxVars + 1) * 8];
+    HIP_TRY(hipMemcpyAsync(sums, q.acc, ((size_t)n + 1) * 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    inner_queued_recover(sums, n, challenges, claim, round_polys, eval_points, final_claim, evals);
     return BITZ_OK;
 }
 

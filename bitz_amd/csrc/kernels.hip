@@ -494,6 +494,10 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
 
 // 4-bit constant tables, one workgroup per constant, into global memory;
 // workgroups that multiply by a constant copy its table into LDS.
+__global__ void k_zero_words(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
 __global__ __launch_bounds__(512) void k_const_tables4(const f128* consts, uint4* out) {
     __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];

@@ -70,6 +70,8 @@ struct HtabDesc {
 };
 
 __global__ void k_const_tables4(const f128* consts, uint4* out);
+// p[0 .. n) = 0 (a kernel, where a memset node would not do: captured graphs).
+__global__ void k_zero_words(uint32_t* p, size_t n);
 
 struct PrologueArgs {
     const f128* L;

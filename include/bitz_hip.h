@@ -231,6 +231,23 @@ int32_t bitz_prove_inner_sumcheck_with_challenges(const bitz_f128* challenges, b
                                                   const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
                                                   bitz_f128* round_polys, bitz_f128* eval_points,
                                                   bitz_f128* final_claim, bitz_f128* evals);
+/* bitz_prove_inner_sumcheck_with_challenges without any allocation: no
+ * hipMalloc, hipFree, pinned buffer or other allocating call.  No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  matrix and
+ * witness are device tables of at most 2^20 entries (num_vars <= 20, else
+ * BITZ_ERR_ARG); the ping-pong tables, challenge tables and round sums live in
+ * a static device array of 24.2 MiB that every GPU the library runs on holds
+ * until the process ends.  The work runs on `stream` (a hipStream_t; null =
+ * the default stream) and the call returns once it is done.  Calls that
+ * overlap on one GPU share that array and race.  Outputs as
+ * bitz_prove_inner_sumcheck_with_challenges. */
+int32_t bitz_prove_inner_sumcheck_with_challenges_noalloc(const bitz_f128* challenges, bitz_f128 initial_claim,
+                                                          const bitz_f128* matrix, uint32_t matrix_vars,
+                                                          const bitz_f128* witness, uint32_t witness_vars,
+                                                          void* stream, bitz_f128* round_polys,
+                                                          bitz_f128* eval_points, bitz_f128* final_claim,
+                                                          bitz_f128* evals);
 /* prove_inner_sumcheck (sumcheck.rs:442-552): round_polys[num_vars][3],
  * evals[2] = [D(r_y), h(r_y)]. */
 int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
