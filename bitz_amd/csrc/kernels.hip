@@ -281,42 +281,6 @@ __global__ __launch_bounds__(kBlock) void k_outer_claim(const f128* a, const f12
     block_xor_to_global<1>(v, acc);
 }
 
-// ------------------------------------------------------- inner sumcheck --
-// [c0, c2] = [sum m0 w0, sum (m1 - m0)(w1 - w0)] (sumcheck.rs:555-609).
-__global__ __launch_bounds__(kBlock) void k_inner_round0(const f128* d, const f128* w, size_t pairs, uint32_t* acc) {
-    g8 s0 = w_zero(), s2 = w_zero();
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pairs;
-         p += (size_t)gridDim.x * blockDim.x) {
-        const g4 m0 = g_load(d + 2 * p), m1 = g_load(d + 2 * p + 1);
-        const g4 w0 = g_load(w + 2 * p), w1 = g_load(w + 2 * p + 1);
-        clmul_acc(s0, m0, w0);
-        clmul_acc(s2, g_add(m1, m0), g_add(w1, w0));
-    }
-    g4 v[2] = {g_reduce(s0), g_reduce(s2)};
-    block_xor_to_global<2>(v, acc);
-}
-
-// fold_and_compute_next_inner_round_coefficients_without_linear
-// (sumcheck.rs:641-688).
-__global__ __launch_bounds__(kBlock) void k_inner_fold_next(const f128* d, const f128* w, f128* dout, f128* wout,
-                                  size_t out_pairs, f128 r, uint32_t* acc) {
-    const g4 rr = g_from(r);
-    g8 s0 = w_zero(), s2 = w_zero();
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < out_pairs;
-         p += (size_t)gridDim.x * blockDim.x) {
-        const g4 m0 = interp(g_load(d + 4 * p), g_load(d + 4 * p + 1), rr);
-        const g4 m1 = interp(g_load(d + 4 * p + 2), g_load(d + 4 * p + 3), rr);
-        const g4 w0 = interp(g_load(w + 4 * p), g_load(w + 4 * p + 1), rr);
-        const g4 w1 = interp(g_load(w + 4 * p + 2), g_load(w + 4 * p + 3), rr);
-        g_store(dout + 2 * p, m0); g_store(dout + 2 * p + 1, m1);
-        g_store(wout + 2 * p, w0); g_store(wout + 2 * p + 1, w1);
-        clmul_acc(s0, m0, w0);
-        clmul_acc(s2, g_add(m1, m0), g_add(w1, w0));
-    }
-    g4 v[2] = {g_reduce(s0), g_reduce(s2)};
-    block_xor_to_global<2>(v, acc);
-}
-
 // ------------------------------------------------------------------ gkr --
 // Round sums of a dense layer (round_sums, lib.rs:164-219) for the round
 // whose halves are [0, half) and [half, 2 half):
@@ -374,21 +338,6 @@ __global__ __launch_bounds__(kBlock) void k_gkr_bind(f128* left, f128* right, si
 }
 
 // --------------------------------------------------------- sparse gathers --
-// D(j) = sum over column j's nonzeros of W[row] * value, W[row] the row's eq
-// weight times the matrix's batching scale (bind_and_batch, matrix.rs:429-468),
-// over a column-major copy of the three matrices.
-__global__ __launch_bounds__(kBlock) void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
-                               const uint8_t* which, const f128* values, const f128* row_scale,
-                               size_t rows_padded, f128* out, size_t cols) {
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < cols;
-         j += (size_t)gridDim.x * blockDim.x) {
-        g8 s = w_zero();
-        for (uint32_t k = col_starts[j]; k < col_starts[j + 1]; ++k)
-            clmul_acc(s, g_load(row_scale + which[k] * rows_padded + rows[k]), g_load(values + k));
-        g_store(out + j, g_reduce(s));
-    }
-}
-
 // out[i] = sum over row i's nonzeros of value * h[column] for i < rows, zero
 // above (the reference test's multiply + padded_mle, crates/spartan/src/piop.rs:352-372).
 __global__ __launch_bounds__(kBlock) void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols,

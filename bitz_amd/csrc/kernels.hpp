@@ -27,17 +27,11 @@ __global__ void k_outer_fold_next(const f128* a, const f128* b, const f128* c, f
                                   int low_bits, const f128* high, uint32_t* acc);
 __global__ void k_outer_claim(const f128* a, const f128* b, const f128* c, size_t n, const f128* low,
                               int low_bits, const f128* high, uint32_t* acc);
-__global__ void k_inner_round0(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
-__global__ void k_inner_fold_next(const f128* d, const f128* w, f128* dout, f128* wout,
-                                  size_t out_pairs, f128 r, uint32_t* acc);
 __global__ void k_gkr_sums(const f128* left, const f128* right, const f128* eq, size_t half,
                            int at_one, uint32_t* acc);
 __global__ void k_gkr_bind_next(f128* left, f128* right, f128* eq, size_t h, f128 r,
                                 int at_one, uint32_t* acc);
 __global__ void k_gkr_bind(f128* left, f128* right, size_t half, f128 r);
-__global__ void k_bind_columns(const uint32_t* col_starts, const uint32_t* rows,
-                               const uint8_t* which, const f128* values, const f128* row_scale,
-                               size_t rows_padded, f128* out, size_t cols);
 // bind_and_batch in two passes: every nonzero's product W[row] * value
 // (lane tables, 256 threads), then each column's XOR over its contiguous run.
 __global__ void k_bind_products(const uint32_t* rows, const uint8_t* which, const f128* values, const f128* row_scale,

@@ -46,7 +46,7 @@ enum {
     BITZ_ERR_ARG = -7,           /* bad argument (null pointer, size out of range) */
     BITZ_ERR_STATE = -8,         /* handle used out of order */
     BITZ_ERR_POINT_LEN = -9,     /* GKR point length != tree bits (crates/gkr/src/lib.rs:39-43 panics) */
-    BITZ_ERR_ASSIGNMENT_DIMS = -10,  /* SpartanError::InvalidAssignmentDimensions (crates/spartan/src/piop.rs:80-82) */
+    BITZ_ERR_ASSIGNMENT_DIMS = -10,  /* SpartanError::InvalidAssignmentDimensions (crates/spartan/src/piop.rs:30, checked :80-82) */
     BITZ_ERR_ASSIGNMENT_LEN = -11,   /* SpartanMatrixError::InvalidAssignmentLength (matrix.rs:23) */
     BITZ_ERR_ASSIGNMENT_CONST = -12, /* SpartanMatrixError::InvalidAssignmentConstant (matrix.rs:24) */
     BITZ_ERR_DEVICE = -100,      /* HIP runtime error; see bitz_last_error() */
