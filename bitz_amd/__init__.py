@@ -141,7 +141,12 @@ def lib():
             "bitz_mt_free": (None, [P]),
             "bitz_gkr_build": (I32, [C.POINTER(C.c_void_p), P, U32, U32, P, U32]),
             "bitz_gkr_build_bittable": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32]),
-            "bitz_gkr_prove": (I32, [P, P, P, U32, P, P]),
+            "bitz_gkr_prove": (I32, [P, P, P, U32, U32, P, P]),
+            "bitz_transcript_set_grind": (I32, [P, P]),
+            "bitz_transcript_grind": (I32, [P, P, SZ, U32]),
+            "bitz_pow_find": (I32, [P, U32, C.POINTER(C.c_uint64)]),
+            "bitz_pow_valid": (I32, [P, C.c_uint64, U32]),
+            "bitz_grinding_bits": (U32, [U32, C.c_uint64]),
             "bitz_gkr_layer": (I32, [P, U32, P, SZ, C.POINTER(SZ)]),
             "bitz_gkr_free": (None, [P]),
             "bitz_piop_engine_create": (I32, [C.POINTER(C.c_void_p)]),
@@ -306,6 +311,7 @@ def synchronize() -> None:
 # ------------------------------------------------------------- transcripts --
 ABSORB_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.POINTER(C.c_uint8), C.c_size_t)
 SQUEEZE_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_uint8))
+GRIND_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_uint8), C.c_size_t, C.c_uint32)
 
 
 class Transcript:
@@ -316,21 +322,21 @@ class Transcript:
             raise BitzError(-7, "transcript creation failed")
         self.h = C.c_void_p(handle)
 
+    # Transcripts are host objects: creating one does not touch the device.
     @classmethod
     def seeded(cls, seed: int) -> "Transcript":
-        init()
         return cls(lib().bitz_transcript_seeded(seed & (2**64 - 1)))
 
     @classmethod
     def duplex(cls, domain: bytes) -> "Transcript":
-        init()
         buf = C.create_string_buffer(domain, len(domain))
         return cls(lib().bitz_transcript_duplex(buf, len(domain)))
 
     @classmethod
-    def callbacks(cls, absorb, squeeze) -> "Transcript":
-        """absorb(kind: int, data: bytes) -> None; squeeze() -> 16 bytes."""
-        init()
+    def callbacks(cls, absorb, squeeze, grind=None) -> "Transcript":
+        """absorb(kind: int, data: bytes) -> None; squeeze() -> 16 bytes;
+        grind(label: bytes, bits: int) -> None, the caller's own grinding
+        (ProverState::grind), or None to frame grinding over absorb/squeeze."""
 
         def _a(ctx, kind, ptr, n):
             absorb(kind, C.string_at(ptr, n) if n else b"")
@@ -342,7 +348,19 @@ class Transcript:
         a, s = ABSORB_FN(_a), SQUEEZE_FN(_s)
         t = cls(lib().bitz_transcript_callbacks(None, C.cast(a, C.c_void_p), C.cast(s, C.c_void_p)))
         t._keep = (a, s)
+        if grind is not None:
+            def _g(ctx, label, n, bits):
+                grind(C.string_at(label, n) if n else b"", int(bits))
+
+            g = GRIND_FN(_g)
+            _check(lib().bitz_transcript_set_grind(t.h, C.cast(g, C.c_void_p)))
+            t._keep = (a, s, g)
         return t
+
+    def grind(self, label: bytes, bits: int) -> None:
+        """ProverState::grind (crates/transcript/src/prover.rs:29-41)."""
+        buf = C.create_string_buffer(label, max(len(label), 1))
+        _check(lib().bitz_transcript_grind(self.h, buf, len(label), bits))
 
     def absorb(self, kind: int, data: bytes) -> None:
         buf = C.create_string_buffer(data, max(len(data), 1))
@@ -1094,11 +1112,33 @@ def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
             to_int(claim))
 
 
-def gpgkr_prove(transcript: Transcript, point, witnesses: LayerWitnesses):
-    """gpgkr_prove (lib.rs:24-79) over dense layers: returns (leaf point, claim)."""
+BITS100, BITS128 = 100, 128  # SecurityLevel (crates/common/src/security.rs:7-10)
+
+
+def grinding_bits(security: int, coefficient: int) -> int:
+    """SecurityLevel::grinding_bits (crates/common/src/security.rs:25-32)."""
+    return int(lib().bitz_grinding_bits(security, coefficient))
+
+
+def pow_find(seed: bytes, bits: int) -> int:
+    """crates/transcript/src/pow.rs:21-33: the first valid nonce."""
+    nonce = C.c_uint64()
+    _check(lib().bitz_pow_find(C.create_string_buffer(bytes(seed), 16), bits, C.byref(nonce)))
+    return nonce.value
+
+
+def pow_valid(seed: bytes, nonce: int, bits: int) -> bool:
+    """crates/transcript/src/pow.rs:38-52."""
+    return bool(lib().bitz_pow_valid(C.create_string_buffer(bytes(seed), 16), nonce, bits))
+
+
+def gpgkr_prove(transcript: Transcript, point, witnesses: LayerWitnesses, security: int = BITS100):
+    """gpgkr_prove (lib.rs:24-79) over dense layers: returns (leaf point,
+    claim).  security: BITS100 (no grinding) or BITS128 (2 bits of grinding
+    before every cubic round's challenge, through the transcript's grind)."""
     point = elems(point) if len(point) else np.zeros((0, 2), np.uint64)
     leaf_point = np.zeros((witnesses.log_leaves, 2), dtype=np.uint64)
     claim = F128()
-    _check(lib().bitz_gkr_prove(witnesses.h, transcript.h, _p(point), len(point), _p(leaf_point),
+    _check(lib().bitz_gkr_prove(witnesses.h, transcript.h, _p(point), len(point), security, _p(leaf_point),
                                 C.byref(claim)))
     return leaf_point, to_int(claim)

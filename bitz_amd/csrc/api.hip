@@ -500,6 +500,30 @@ extern "C" bitz_transcript* bitz_transcript_callbacks(void* ctx, bitz_absorb_fn 
     t->t.cb_squeeze = squeeze;
     return t;
 }
+extern "C" int32_t bitz_transcript_set_grind(bitz_transcript* t, bitz_grind_fn grind) {
+    if (!t) return fail(BITZ_ERR_ARG, "null pointer");
+    if (t->t.type != Transcript::Type::kCallbacks) return fail(BITZ_ERR_ARG, "only callback transcripts take a grind hook");
+    t->t.cb_grind = grind;
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_transcript_grind(bitz_transcript* t, const uint8_t* label, size_t len, uint32_t bits) {
+    if (!t || (len && !label)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (bits > bitz::kMaxGrindingBits) return fail(BITZ_ERR_ARG, "grinding difficulty exceeds 32 bits");
+    t->t.grind(label, len, bits);
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_pow_find(const uint8_t seed[16], uint32_t bits, uint64_t* nonce) {
+    if (!seed || !nonce) return fail(BITZ_ERR_ARG, "null pointer");
+    if (bits > bitz::kMaxGrindingBits) return fail(BITZ_ERR_ARG, "grinding difficulty exceeds 32 bits");
+    *nonce = bitz::pow_find(seed, bits);
+    return BITZ_OK;
+}
+extern "C" int32_t bitz_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits) {
+    return seed && bitz::pow_valid(seed, nonce, bits) ? 1 : 0;
+}
+extern "C" uint32_t bitz_grinding_bits(uint32_t security_bits, uint64_t coefficient) {
+    return bitz::grinding_bits(security_bits, coefficient);
+}
 extern "C" bitz_transcript* bitz_transcript_seeded(uint64_t seed) {
     auto* t = new bitz_transcript;
     t->t.type = Transcript::Type::kSeeded;
@@ -3822,9 +3846,15 @@ extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_
 }
 
 extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point,
-                                  uint32_t point_len, bitz_f128* leaf_point, bitz_f128* leaf_claim) {
+                                  uint32_t point_len, uint32_t security_bits, bitz_f128* leaf_point,
+                                  bitz_f128* leaf_claim) {
     if (!h || !t || !leaf_point || !leaf_claim || (point_len && !point))
         return fail(BITZ_ERR_ARG, "null pointer");
+    if (security_bits != 100 && security_bits != 128)
+        return fail(BITZ_ERR_ARG, "security_bits is a SecurityLevel: 100 or 128");
+    // lib.rs:144: grinding_bits(3) before every cubic round's challenge.
+    static const uint8_t kCubicLabel[] = "gkr/cubic/v1";  // lib.rs:10
+    const uint32_t grind_bits = bitz::grinding_bits(security_bits, 3);
     // lib.rs:39-43: one coordinate per tree bit; EqTables needs at least one.
     if ((int)point_len != h->log_groups || point_len == 0)
         return fail(BITZ_ERR_POINT_LEN, "the output has one value per tree, so the point has a coordinate per tree bit");
@@ -3879,8 +3909,7 @@ extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f1
             // lib.rs:142: prover_message([eq_bound * sum_b, eq_bound * sum_inf]).
             f128 msg[2] = {hostf::mul(eq_bound, sums[0]), hostf::mul(eq_bound, sums[1])};
             t->t.absorb_elems(1, msg, 2);
-            // Grinding is zero bits for these rounds at 100-bit security
-            // (crates/common/src/security.rs:25-32).
+            t->t.grind(kCubicLabel, sizeof kCubicLabel - 1, grind_bits);
             const f128 r = t->t.squeeze();
             next[i] = r;
             eq_bound = hostf::mul(eq_bound, f128{1 ^ r.lo ^ z.lo, r.hi ^ z.hi});  // eq_factor

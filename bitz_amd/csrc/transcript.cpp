@@ -219,4 +219,105 @@ f128 Transcript::squeeze() {
     return r;
 }
 
+namespace {
+// BLAKE3 (the published specification) for inputs of one block: compressed
+// once from the IV, counter 0, block length = input length, flags
+// CHUNK_START | CHUNK_END | ROOT.
+const uint32_t kB3Iv[8] = {0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A,
+                           0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19};
+const uint8_t kB3Perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+inline void b3_mix(uint32_t* s, int a, int b, int c, int d, uint32_t x, uint32_t y) {
+    s[a] += s[b] + x;
+    s[d] = rotr(s[d] ^ s[a], 16);
+    s[c] += s[d];
+    s[b] = rotr(s[b] ^ s[c], 12);
+    s[a] += s[b] + y;
+    s[d] = rotr(s[d] ^ s[a], 8);
+    s[c] += s[d];
+    s[b] = rotr(s[b] ^ s[c], 7);
+}
+const uint8_t kPowHashTag[] = "bitz-pcs-pow-v1";              // pow.rs:5
+const uint8_t kPowTranscriptTag[] = "bitz-transcript-pow-v1"; // pow.rs:6
+}  // namespace
+
+bool blake3_small(const uint8_t* in, size_t len, uint8_t out[32]) {
+    if (len > 64) return false;
+    uint8_t block[64] = {0};
+    if (len) memcpy(block, in, len);
+    uint32_t m[16], st[16];
+    for (int i = 0; i < 16; ++i) memcpy(&m[i], block + 4 * i, 4);  // little-endian words
+    memcpy(st, kB3Iv, 32);
+    memcpy(st + 8, kB3Iv, 16);
+    st[12] = 0;
+    st[13] = 0;
+    st[14] = (uint32_t)len;
+    st[15] = 1u | 2u | 8u;
+    for (int round = 0; round < 7; ++round) {
+        b3_mix(st, 0, 4, 8, 12, m[0], m[1]);
+        b3_mix(st, 1, 5, 9, 13, m[2], m[3]);
+        b3_mix(st, 2, 6, 10, 14, m[4], m[5]);
+        b3_mix(st, 3, 7, 11, 15, m[6], m[7]);
+        b3_mix(st, 0, 5, 10, 15, m[8], m[9]);
+        b3_mix(st, 1, 6, 11, 12, m[10], m[11]);
+        b3_mix(st, 2, 7, 8, 13, m[12], m[13]);
+        b3_mix(st, 3, 4, 9, 14, m[14], m[15]);
+        uint32_t p[16];
+        for (int i = 0; i < 16; ++i) p[i] = m[kB3Perm[i]];
+        memcpy(m, p, sizeof m);
+    }
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t w = st[i] ^ st[i + 8];
+        memcpy(out + 4 * i, &w, 4);
+    }
+    return true;
+}
+
+bool pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits) {
+    if (bits > kMaxGrindingBits) return false;
+    if (bits == 0) return nonce == 0;
+    uint8_t msg[15 + 16 + 8], h[32];
+    memcpy(msg, kPowHashTag, 15);
+    memcpy(msg + 15, seed, 16);
+    memcpy(msg + 31, &nonce, 8);  // nonce.to_le_bytes()
+    blake3_small(msg, sizeof msg, h);
+    const uint32_t first = (uint32_t)h[0] << 24 | (uint32_t)h[1] << 16 | (uint32_t)h[2] << 8 | h[3];
+    return (first == 0 ? 32u : (uint32_t)__builtin_clz(first)) >= bits;
+}
+
+uint64_t pow_find(const uint8_t seed[16], uint32_t bits) {
+    uint64_t nonce = 0;
+    while (!pow_valid(seed, nonce, bits)) ++nonce;
+    return nonce;
+}
+
+uint32_t grinding_bits(uint32_t security_bits, uint64_t coefficient) {
+    if (coefficient == 0) return 0;
+    uint32_t log_c = 0;
+    while (((uint64_t)1 << log_c) < coefficient) ++log_c;
+    return security_bits + log_c > 128 ? security_bits + log_c - 128 : 0;
+}
+
+void Transcript::grind(const uint8_t* label, size_t len, uint32_t bits) {
+    if (bits == 0) return;
+    if (type == Type::kCallbacks && cb_grind) {
+        cb_grind(cb_ctx, label, len, bits);
+        return;
+    }
+    uint8_t le[8];
+    absorb(0, kPowTranscriptTag, sizeof kPowTranscriptTag - 1);
+    const uint64_t l = len;
+    memcpy(le, &l, 8);
+    absorb(0, le, 8);
+    absorb(0, label, len);
+    memcpy(le, &bits, 4);
+    absorb(0, le, 4);
+    const f128 seed = squeeze();
+    uint8_t sb[16];
+    memcpy(sb, &seed.lo, 8);
+    memcpy(sb + 8, &seed.hi, 8);
+    const uint64_t nonce = pow_find(sb, bits);
+    memcpy(le, &nonce, 8);
+    absorb(1, le, 8);
+}
+
 }  // namespace bitz

@@ -25,11 +25,23 @@ struct Sha256 {
 };
 uint64_t splitmix_next(uint64_t& state);
 
+// BLAKE3 of at most 64 bytes (false beyond); the reference's proof-of-work
+// check and search (crates/transcript/src/pow.rs:21-52); grinding_bits
+// (crates/common/src/security.rs:25-32).
+bool blake3_small(const uint8_t* in, size_t len, uint8_t out[32]);
+bool pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits);
+uint64_t pow_find(const uint8_t seed[16], uint32_t bits);
+uint32_t grinding_bits(uint32_t security_bits, uint64_t coefficient);
+constexpr uint32_t kMaxGrindingBits = 32;
+
 struct Transcript {
     enum class Type { kCallbacks, kSeeded, kDuplex } type;
     void* cb_ctx = nullptr;
     void (*cb_absorb)(void*, int32_t, const uint8_t*, size_t) = nullptr;
     void (*cb_squeeze)(void*, uint8_t*) = nullptr;
+    // ProverState::grind handed over whole (the caller's own transcript), or
+    // null: grind() frames it over absorb and squeeze.
+    void (*cb_grind)(void*, const uint8_t*, size_t, uint32_t) = nullptr;
     uint64_t seed = 0;
     uint8_t state[32] = {0};
     std::vector<uint8_t> narg;
@@ -39,6 +51,11 @@ struct Transcript {
         absorb(kind, reinterpret_cast<const uint8_t*>(e), n * sizeof(f128));
     }
     f128 squeeze();
+    // ProverState::grind (crates/transcript/src/prover.rs:29-41): nothing at
+    // zero bits; else the pow.rs:11-16 header as public messages (the tag,
+    // le64(label length), the label, le32(bits)), a squeezed seed, and the
+    // first valid nonce as an 8-byte prover message.  bits <= 32.
+    void grind(const uint8_t* label, size_t len, uint32_t bits);
 };
 
 }  // namespace bitz

@@ -104,6 +104,22 @@ int32_t bitz_transcript_absorb(bitz_transcript* t, int32_t kind, const uint8_t* 
 int32_t bitz_transcript_squeeze(bitz_transcript* t, bitz_f128* out);
 /* Copies up to cap narg bytes; returns the total narg length. */
 size_t bitz_transcript_narg(const bitz_transcript* t, uint8_t* out, size_t cap);
+/* Grinding, ProverState::grind (crates/transcript/src/prover.rs:29-41): at
+ * zero bits nothing; else, for a callback transcript with a grind hook, the
+ * hook (the caller's own grind, e.g. ProverState::grind); otherwise over the
+ * transcript's absorb and squeeze: public messages "bitz-transcript-pow-v1",
+ * le64(label length), the label, le32(bits) (pow.rs:11-16), a squeezed seed,
+ * and the first nonce n with BLAKE3("bitz-pcs-pow-v1" || seed || le64(n))
+ * starting with `bits` zero bits (pow.rs:21-52) as an 8-byte prover message.
+ * bits > 32 -> BITZ_ERR_ARG. */
+typedef void (*bitz_grind_fn)(void* ctx, const uint8_t* label, size_t len, uint32_t bits);
+int32_t bitz_transcript_set_grind(bitz_transcript* t, bitz_grind_fn grind);
+int32_t bitz_transcript_grind(bitz_transcript* t, const uint8_t* label, size_t len, uint32_t bits);
+/* pow.rs:21-52 find / valid, and SecurityLevel::grinding_bits
+ * (crates/common/src/security.rs:25-32) for security_bits 100 or 128. */
+int32_t bitz_pow_find(const uint8_t seed[16], uint32_t bits, uint64_t* nonce);
+int32_t bitz_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits);
+uint32_t bitz_grinding_bits(uint32_t security_bits, uint64_t coefficient);
 
 /* ---- field and multilinear primitives -------------------------------------- */
 /* out[i] = a[i] * b[i] (F128 Mul, crates/field/src/gf128.rs:275-281).  The
@@ -435,9 +451,13 @@ int32_t bitz_gkr_build_bittable(bitz_gkr** h, const bitz_f128* images, uint32_t 
                                 uint32_t log_columns, bitz_f128* output, uint32_t flags);
 /* gpgkr_prove (lib.rs:24-79) over the dense layers: point has log_groups
  * coordinates; leaf_point receives log_leaves coordinates and *leaf_claim
- * the claim on the leaves.  Consumes the layers (binds them in place). */
+ * the claim on the leaves.  Consumes the layers (binds them in place).
+ * security_bits is the SecurityLevel (100 = Bits100, 128 = Bits128; else
+ * BITZ_ERR_ARG): every cubic round grinds grinding_bits(3) bits (0 at
+ * Bits100, 2 at Bits128) under the label "gkr/cubic/v1" before its
+ * challenge (lib.rs:144, 416), through bitz_transcript_grind. */
 int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point, uint32_t point_len,
-                       bitz_f128* leaf_point, bitz_f128* leaf_claim);
+                       uint32_t security_bits, bitz_f128* leaf_point, bitz_f128* leaf_claim);
 /* Copies layer j (0 = leaves) to host; *len receives its length. */
 /* gpgkr_prove with the challenges known in advance (a transcript whose
  * challenges do not depend on the messages, such as the seeded stream); no
@@ -446,7 +466,9 @@ int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point, 
  * top, its n round challenges in squeeze order, then t (sum over layers of
  * n + 1 values).  messages: the prover_message pairs [eq_bound sum_b,
  * eq_bound sum_inf] of every round in the same order; layer_evals: [L(r),
- * R(r)] per layer.  leaf_point / leaf_claim as bitz_gkr_prove. */
+ * R(r)] per layer.  leaf_point / leaf_claim as bitz_gkr_prove.  Grinding is
+ * the transcript's: the list holds the round challenges a transcript
+ * squeezed after any grinding (at Bits100 there is none). */
 int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* point, uint32_t point_len,
                                        const bitz_f128* challenges, size_t challenge_count, bitz_f128* messages,
                                        bitz_f128* layer_evals, bitz_f128* leaf_point, bitz_f128* leaf_claim);

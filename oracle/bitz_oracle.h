@@ -47,7 +47,17 @@ typedef struct {
     void* ctx;
     void (*absorb)(void* ctx, int kind, const uint8_t* bytes, size_t len);
     void (*squeeze)(void* ctx, uint8_t out[16]);
+    /* ProverState::grind (prover.rs:29-41) handed over whole, or NULL: then
+     * oracle_transcript_grind frames it over absorb and squeeze. */
+    void (*grind)(void* ctx, const uint8_t* label, size_t len, uint32_t bits);
 } oracle_transcript;
+/* Grinding (crates/transcript/src/{prover.rs:29-41,pow.rs:11-52}); BLAKE3 of
+ * at most 64 bytes; security.rs:25-32 grinding_bits. */
+void oracle_transcript_grind(oracle_transcript* t, const uint8_t* label, size_t len, uint32_t bits);
+int oracle_blake3_small(const uint8_t* in, size_t len, uint8_t out[32]);
+int oracle_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits);
+uint64_t oracle_pow_find(const uint8_t seed[16], uint32_t bits);
+uint32_t oracle_grinding_bits(uint32_t security_bits, uint64_t coefficient);
 
 /* Seeded source: challenges are splitmix64 words (gkr/src/lib.rs:986-999),
  * F128::from([w0, w1]); absorbed messages are recorded but do not steer it. */
@@ -168,10 +178,14 @@ void oracle_gkr_eq_tables(const of128* point, int n, of128* storage);
  * coordinates; returns the claim W(r, t). */
 of128 oracle_gkr_prove_layer(oracle_transcript* t, const of128* point, int n,
                              const of128* children, of128* next_point,
-                             of128* messages /* (n + 1) x 2 */);
-/* gpgkr_prove over dense layers: from the output point down to the leaves. */
+                             of128* messages /* (n + 1) x 2 */, uint32_t grind_bits);
+/* gpgkr_prove over dense layers: from the output point down to the leaves;
+ * security_bits 100 or 128 (SecurityLevel, crates/common/src/security.rs):
+ * each cubic round grinds grinding_bits(3) before its challenge
+ * (gkr/src/lib.rs:144). */
 of128 oracle_gkr_prove(oracle_transcript* t, const of128* point, int n_out,
-                       const of128* store, int log_leaves, of128* leaf_point);
+                       const of128* store, int log_leaves, of128* leaf_point,
+                       uint32_t security_bits);
 /* gpgkr_verify (lib.rs:717-772); returns 0 on success. */
 int oracle_gkr_verify(oracle_transcript* t, of128 claim, const of128* point,
                       int n_out, int layers, const of128* messages,

@@ -104,8 +104,10 @@ static inline of128 eq_factor(of128 r, of128 z) {
 static inline int is_zero(of128 a) { return a.lo == 0 && a.hi == 0; }
 
 /* lib.rs:88-159 prove_layer / prove_rounds, dense. */
+static const uint8_t CUBIC_GRINDING_LABEL[] = "gkr/cubic/v1"; /* lib.rs:10 */
+
 of128 oracle_gkr_prove_layer(oracle_transcript* t, const of128* point, int n,
-                             const of128* children, of128* next_point, of128* messages) {
+                             const of128* children, of128* next_point, of128* messages, uint32_t grind_bits) {
     size_t mid = (size_t)1 << n;
     of128* left = malloc(mid * sizeof(of128));
     of128* right = malloc(mid * sizeof(of128));
@@ -139,6 +141,8 @@ of128 oracle_gkr_prove_layer(oracle_transcript* t, const of128* point, int n,
             messages[2 * msg + 1] = m[1];
         }
         ++msg;
+        /* lib.rs:144: the grinding boundary before the round's challenge. */
+        oracle_transcript_grind(t, CUBIC_GRINDING_LABEL, sizeof CUBIC_GRINDING_LABEL - 1, grind_bits);
         of128 r = squeeze(t);
         /* lib.rs:223-243 bind: l0 += r (l1 - l0), r0 += r (r1 - r0). */
         for (size_t x = 0; x < half; ++x) {
@@ -166,7 +170,8 @@ of128 oracle_gkr_prove_layer(oracle_transcript* t, const of128* point, int n,
 /* gpgkr_prove (lib.rs:24-79) over dense layers: store holds every layer
  * from the leaves (2^log_leaves) up to the output (2^n_out). */
 of128 oracle_gkr_prove(oracle_transcript* t, const of128* point, int n_out, const of128* store,
-                       int log_leaves, of128* leaf_point) {
+                       int log_leaves, of128* leaf_point, uint32_t security_bits) {
+    const uint32_t grind_bits = oracle_grinding_bits(security_bits, 3);
     int layers = log_leaves - n_out; /* sumchecks: one per layer below the output */
     of128* pt = malloc((size_t)(log_leaves + 1) * sizeof(of128));
     of128* nxt = malloc((size_t)(log_leaves + 1) * sizeof(of128));
@@ -179,7 +184,7 @@ of128 oracle_gkr_prove(oracle_transcript* t, const of128* point, int n_out, cons
     int n = n_out;
     for (int j = layers - 1; j >= 0; --j) {
         /* children = layer j (2^(n+1) nodes), parent point has n coordinates */
-        claim = oracle_gkr_prove_layer(t, pt, n, store + off[j], nxt, NULL);
+        claim = oracle_gkr_prove_layer(t, pt, n, store + off[j], nxt, NULL, grind_bits);
         memcpy(pt, nxt, (size_t)(n + 1) * sizeof(of128));
         ++n;
     }

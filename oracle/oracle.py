@@ -27,8 +27,11 @@ ABSORB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_uint8), C.c_size_t
 SQUEEZE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_uint8))
 
 
+GRIND = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_uint8), C.c_size_t, C.c_uint32)
+
+
 class Transcript(C.Structure):
-    _fields_ = [("ctx", C.c_void_p), ("absorb", C.c_void_p), ("squeeze", C.c_void_p)]
+    _fields_ = [("ctx", C.c_void_p), ("absorb", C.c_void_p), ("squeeze", C.c_void_p), ("grind", C.c_void_p)]
 
 
 class Seeded(C.Structure):
@@ -115,9 +118,18 @@ def lib():
         L.oracle_gkr_fold_layers.argtypes = [P, C.c_int, C.c_int, P]
         L.oracle_gkr_eq_tables.argtypes = [P, C.c_int, P]
         L.oracle_gkr_prove_layer.restype = OF128
-        L.oracle_gkr_prove_layer.argtypes = [P, P, C.c_int, P, P, P]
+        L.oracle_gkr_prove_layer.argtypes = [P, P, C.c_int, P, P, P, C.c_uint32]
         L.oracle_gkr_prove.restype = OF128
-        L.oracle_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, P]
+        L.oracle_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, P, C.c_uint32]
+        L.oracle_transcript_grind.argtypes = [P, P, C.c_size_t, C.c_uint32]
+        L.oracle_blake3_small.restype = C.c_int
+        L.oracle_blake3_small.argtypes = [P, C.c_size_t, P]
+        L.oracle_pow_valid.restype = C.c_int
+        L.oracle_pow_valid.argtypes = [P, C.c_uint64, C.c_uint32]
+        L.oracle_pow_find.restype = C.c_uint64
+        L.oracle_pow_find.argtypes = [P, C.c_uint32]
+        L.oracle_grinding_bits.restype = C.c_uint32
+        L.oracle_grinding_bits.argtypes = [C.c_uint32, C.c_uint64]
         L.oracle_gkr_verify.restype = C.c_int
         L.oracle_gkr_verify.argtypes = [P, OF128, P, C.c_int, C.c_int, P, P, P]
         _lib = L
@@ -221,21 +233,32 @@ class DuplexTranscript:
 
 class ListTranscript:
     """Challenges from a fixed list (tests that need chosen challenges);
-    messages are recorded."""
+    messages are recorded.  With grind_hook, grinding is handed over whole
+    and recorded as ("grind", label, bits) in `events` (beside ("absorb",
+    kind, bytes) and ("squeeze",)); without it, grinding runs over absorb and
+    squeeze (pow.rs framing) and its seed takes a challenge from the list."""
 
-    def __init__(self, challenges):
+    def __init__(self, challenges, grind_hook: bool = False):
         self.challenges = [int(c) for c in challenges]
         self.messages = []
+        self.events = []
 
         def _a(ctx, kind, ptr, n):
-            self.messages.append((kind, C.string_at(ptr, n) if n else b""))
+            data = C.string_at(ptr, n) if n else b""
+            self.messages.append((kind, data))
+            self.events.append(("absorb", kind, data))
 
         def _s(ctx, out):
             v = self.challenges.pop(0)
+            self.events.append(("squeeze",))
             C.memmove(out, v.to_bytes(16, "little"), 16)
 
-        self._a, self._s = ABSORB(_a), SQUEEZE(_s)
-        self.t = Transcript(None, C.cast(self._a, C.c_void_p), C.cast(self._s, C.c_void_p))
+        def _g(ctx, label, n, bits):
+            self.events.append(("grind", C.string_at(label, n), int(bits)))
+
+        self._a, self._s, self._g = ABSORB(_a), SQUEEZE(_s), GRIND(_g)
+        self.t = Transcript(None, C.cast(self._a, C.c_void_p), C.cast(self._s, C.c_void_p),
+                            C.cast(self._g, C.c_void_p) if grind_hook else None)
 
     @property
     def ref(self):
@@ -433,22 +456,48 @@ def gkr_eq_tables(point):
     return out
 
 
-def gkr_prove_layer(tr, point, children):
+def gkr_prove_layer(tr, point, children, grind_bits: int = 0):
     point, children = as_elems(point), as_elems(children)
     n = len(point)
     nxt = arr(n + 1)
     msgs = arr(2 * (n + 1))
     claim = lib().oracle_gkr_prove_layer(tr.ref, ptr(point) if n else ptr(arr(1)), n,
-                                         ptr(children), ptr(nxt), ptr(msgs))
+                                         ptr(children), ptr(nxt), ptr(msgs), grind_bits)
     return nxt, to_int(claim), msgs
 
 
-def gkr_prove(tr, point, store, log_leaves):
+def gkr_prove(tr, point, store, log_leaves, security_bits: int = 100):
+    """gpgkr_prove; security_bits is the SecurityLevel (100 or 128)."""
     point = as_elems(point)
     leaf_point = arr(log_leaves)
     claim = lib().oracle_gkr_prove(tr.ref, ptr(point), len(point), ptr(store), log_leaves,
-                                   ptr(leaf_point))
+                                   ptr(leaf_point), security_bits)
     return leaf_point, to_int(claim)
+
+
+def blake3(data: bytes) -> bytes:
+    """BLAKE3 of at most 64 bytes (one block of the first chunk)."""
+    out = (C.c_uint8 * 32)()
+    buf = C.create_string_buffer(data, max(len(data), 1))
+    assert lib().oracle_blake3_small(buf, len(data), out) == 0
+    return bytes(out)
+
+
+def pow_valid(seed: bytes, nonce: int, bits: int) -> bool:
+    return bool(lib().oracle_pow_valid(C.create_string_buffer(seed, 16), nonce, bits))
+
+
+def pow_find(seed: bytes, bits: int) -> int:
+    return int(lib().oracle_pow_find(C.create_string_buffer(seed, 16), bits))
+
+
+def grinding_bits(security_bits: int, coefficient: int) -> int:
+    return int(lib().oracle_grinding_bits(security_bits, coefficient))
+
+
+def grind(tr, label: bytes, bits: int) -> None:
+    buf = C.create_string_buffer(label, max(len(label), 1))
+    lib().oracle_transcript_grind(tr.ref, buf, len(label), bits)
 
 
 def gkr_verify(tr, claim, point, layers, messages):

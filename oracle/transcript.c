@@ -116,7 +116,7 @@ void oracle_seeded_free(oracle_seeded* s) {
 }
 
 oracle_transcript oracle_seeded_transcript(oracle_seeded* s) {
-    oracle_transcript t = {s, seeded_absorb, seeded_squeeze};
+    oracle_transcript t = {s, seeded_absorb, seeded_squeeze, NULL};
     return t;
 }
 
@@ -161,6 +161,122 @@ void oracle_duplex_free(oracle_duplex* d) {
 }
 
 oracle_transcript oracle_duplex_transcript(oracle_duplex* d) {
-    oracle_transcript t = {d, duplex_absorb, duplex_squeeze};
+    oracle_transcript t = {d, duplex_absorb, duplex_squeeze, NULL};
     return t;
+}
+
+/* ---- BLAKE3 (one chunk, one block) and proof-of-work grinding -------------
+ * The reference grinds with crates/transcript/src/pow.rs:21-52: nonce = the
+ * first n in 0, 1, ... with BLAKE3(POW_HASH_TAG || seed || le64(n)) whose
+ * first four bytes, read big-endian, have at least `bits` leading zeros.
+ * BLAKE3 (the published specification): an input of at most 64 bytes is one
+ * block of the first chunk, compressed once from the IV with counter 0,
+ * block length = input length, flags CHUNK_START | CHUNK_END | ROOT. */
+static const uint32_t B3_IV[8] = {0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A,
+                                  0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19};
+static const uint8_t B3_PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+static void b3_g(uint32_t* s, int a, int b, int c, int d, uint32_t x, uint32_t y) {
+    s[a] = s[a] + s[b] + x;
+    s[d] = rotr(s[d] ^ s[a], 16);
+    s[c] = s[c] + s[d];
+    s[b] = rotr(s[b] ^ s[c], 12);
+    s[a] = s[a] + s[b] + y;
+    s[d] = rotr(s[d] ^ s[a], 8);
+    s[c] = s[c] + s[d];
+    s[b] = rotr(s[b] ^ s[c], 7);
+}
+int oracle_blake3_small(const uint8_t* in, size_t len, uint8_t out[32]) {
+    if (len > 64) return ORACLE_ERR_ARG;
+    uint8_t block[64];
+    memset(block, 0, sizeof block);
+    if (len) memcpy(block, in, len);
+    uint32_t m[16], s[16];
+    for (int i = 0; i < 16; ++i)
+        m[i] = (uint32_t)block[4 * i] | ((uint32_t)block[4 * i + 1] << 8) | ((uint32_t)block[4 * i + 2] << 16) |
+               ((uint32_t)block[4 * i + 3] << 24);
+    for (int i = 0; i < 8; ++i) s[i] = B3_IV[i];
+    for (int i = 0; i < 4; ++i) s[8 + i] = B3_IV[i];
+    s[12] = 0;
+    s[13] = 0;
+    s[14] = (uint32_t)len;
+    s[15] = 1u | 2u | 8u;
+    for (int round = 0; round < 7; ++round) {
+        b3_g(s, 0, 4, 8, 12, m[0], m[1]);
+        b3_g(s, 1, 5, 9, 13, m[2], m[3]);
+        b3_g(s, 2, 6, 10, 14, m[4], m[5]);
+        b3_g(s, 3, 7, 11, 15, m[6], m[7]);
+        b3_g(s, 0, 5, 10, 15, m[8], m[9]);
+        b3_g(s, 1, 6, 11, 12, m[10], m[11]);
+        b3_g(s, 2, 7, 8, 13, m[12], m[13]);
+        b3_g(s, 3, 4, 9, 14, m[14], m[15]);
+        uint32_t p[16];
+        for (int i = 0; i < 16; ++i) p[i] = m[B3_PERM[i]];
+        memcpy(m, p, sizeof m);
+    }
+    for (int i = 0; i < 8; ++i) {
+        uint32_t w = s[i] ^ s[i + 8];
+        out[4 * i] = (uint8_t)w;
+        out[4 * i + 1] = (uint8_t)(w >> 8);
+        out[4 * i + 2] = (uint8_t)(w >> 16);
+        out[4 * i + 3] = (uint8_t)(w >> 24);
+    }
+    return ORACLE_OK;
+}
+
+static const char POW_HASH_TAG[] = "bitz-pcs-pow-v1";              /* pow.rs:5 */
+static const char POW_TRANSCRIPT_TAG[] = "bitz-transcript-pow-v1"; /* pow.rs:6 */
+
+/* pow.rs:38-52 valid. */
+int oracle_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits) {
+    if (bits > 32) return 0;
+    if (bits == 0) return nonce == 0;
+    uint8_t msg[15 + 16 + 8], h[32];
+    memcpy(msg, POW_HASH_TAG, 15);
+    memcpy(msg + 15, seed, 16);
+    for (int i = 0; i < 8; ++i) msg[31 + i] = (uint8_t)(nonce >> (8 * i));
+    oracle_blake3_small(msg, sizeof msg, h);
+    uint32_t first = ((uint32_t)h[0] << 24) | ((uint32_t)h[1] << 16) | ((uint32_t)h[2] << 8) | h[3];
+    uint32_t lz = 0;
+    while (lz < 32 && !(first & (0x80000000u >> lz))) ++lz;
+    return lz >= bits;
+}
+
+/* pow.rs:21-33 find. */
+uint64_t oracle_pow_find(const uint8_t seed[16], uint32_t bits) {
+    uint64_t nonce = 0;
+    while (!oracle_pow_valid(seed, nonce, bits)) ++nonce;
+    return nonce;
+}
+
+/* crates/common/src/security.rs:25-32 grinding_bits: ceil(log2(c)) + target - 128, at least 0. */
+uint32_t oracle_grinding_bits(uint32_t security_bits, uint64_t coefficient) {
+    if (coefficient == 0) return 0;
+    uint32_t log_c = 0;
+    while (((uint64_t)1 << log_c) < coefficient) ++log_c;
+    return security_bits + log_c > 128 ? security_bits + log_c - 128 : 0;
+}
+
+/* ProverState::grind (crates/transcript/src/prover.rs:29-41) over a
+ * transcript's own calls: the header of pow.rs:11-16 as public messages
+ * (the tag, le64(label length), the label, le32(bits)), a squeezed seed, the
+ * nonce as an 8-byte prover message.  A transcript with a grind hook hands
+ * the whole step to it instead.  Zero bits leave the transcript untouched. */
+void oracle_transcript_grind(oracle_transcript* t, const uint8_t* label, size_t len, uint32_t bits) {
+    if (bits == 0) return;
+    if (t->grind) {
+        t->grind(t->ctx, label, len, bits);
+        return;
+    }
+    uint8_t le[8];
+    t->absorb(t->ctx, 0, (const uint8_t*)POW_TRANSCRIPT_TAG, sizeof POW_TRANSCRIPT_TAG - 1);
+    for (int i = 0; i < 8; ++i) le[i] = (uint8_t)((uint64_t)len >> (8 * i));
+    t->absorb(t->ctx, 0, le, 8);
+    t->absorb(t->ctx, 0, label, len);
+    for (int i = 0; i < 4; ++i) le[i] = (uint8_t)(bits >> (8 * i));
+    t->absorb(t->ctx, 0, le, 4);
+    uint8_t seed[16];
+    t->squeeze(t->ctx, seed);
+    uint64_t nonce = oracle_pow_find(seed, bits);
+    for (int i = 0; i < 8; ++i) le[i] = (uint8_t)(nonce >> (8 * i));
+    t->absorb(t->ctx, 1, le, 8);
 }

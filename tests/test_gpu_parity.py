@@ -663,3 +663,60 @@ def test_outer_batch_rejects_a_short_challenge_array():
     outs = G.prove_outer_batch(batch_insts(cases))
     for (ch, claim, lo, hi, a, b, c), out in zip(cases, outs):
         assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
+
+
+# ---- GKR at Bits128: grinding before every cubic round's challenge ---------------
+@pytest.mark.parametrize("log_leaves,log_groups", [(8, 2), (13, 3)])
+def test_gkr_bits128_grinds_like_the_oracle(log_leaves, log_groups):
+    """Every nonce in the narg string and every challenge after it equal the
+    oracle's (crates/gkr/src/lib.rs:144: grinding_bits(3) = 2 at Bits128);
+    Bits100 adds none."""
+    leaves = elements_fast(140 + log_leaves, 1 << log_leaves)
+    store, _ = O.gkr_fold_layers(leaves, log_groups)
+    point = elements(141, log_groups)
+    ot = O.DuplexTranscript(b"gkr/bits128")
+    ref_point, ref_claim = O.gkr_prove(ot, point, store, log_leaves, 128)
+    _, wit = G.grand_product(leaves, log_groups)
+    gt = G.Transcript.duplex(b"gkr/bits128")
+    gp, gc = G.gpgkr_prove(gt, point, wit, G.BITS128)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim and gt.narg() == ot.narg()
+    assert gc == O.mle_evaluate(leaves, gp)
+    rounds = sum(range(log_groups, log_leaves))
+    layers = log_leaves - log_groups
+    assert len(gt.narg()) == 32 * rounds + 32 * layers + 8 * rounds  # messages, [L, R], one nonce per round
+    _, wit = G.grand_product(leaves, log_groups)
+    g100 = G.Transcript.duplex(b"gkr/bits128")
+    G.gpgkr_prove(g100, point, wit, G.BITS100)
+    assert len(g100.narg()) == 32 * rounds + 32 * layers
+
+
+def test_gkr_bits128_hands_grinding_to_the_callers_transcript():
+    """With a grind hook the prover's calls, in order, are the oracle's:
+    each round's message, grind("gkr/cubic/v1", 2), its challenge; no grind
+    before t."""
+    log_leaves, log_groups = 7, 2
+    leaves = elements_fast(150, 1 << log_leaves)
+    store, _ = O.gkr_fold_layers(leaves, log_groups)
+    point = elements(151, log_groups)
+    count = sum(n + 1 for n in range(log_groups, log_leaves))
+    ch = ints(elements(152, count))
+    ot = O.ListTranscript(ch, grind_hook=True)
+    ref_point, ref_claim = O.gkr_prove(ot, point, store, log_leaves, 128)
+    events, it = [], iter(ch)
+    gt = G.Transcript.callbacks(lambda kind, data: events.append(("absorb", kind, data)),
+                                lambda: (events.append(("squeeze",)), next(it).to_bytes(16, "little"))[1],
+                                grind=lambda label, bits: events.append(("grind", label, bits)))
+    _, wit = G.grand_product(leaves, log_groups)
+    gp, gc = G.gpgkr_prove(gt, point, wit, G.BITS128)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim
+    assert events == ot.events
+    assert sum(e[0] == "grind" for e in events) == sum(range(log_groups, log_leaves))
+    assert all(e == ("grind", b"gkr/cubic/v1", 2) for e in events if e[0] == "grind")
+
+
+def test_gkr_rejects_an_unknown_security_level():
+    leaves = elements_fast(1, 1 << 6)
+    _, wit = G.grand_product(leaves, 2)
+    with pytest.raises(G.BitzError) as e:
+        G.gpgkr_prove(G.Transcript.seeded(1), elements(2, 2), wit, 120)
+    assert e.value.variant == "InvalidArgument"
