@@ -49,6 +49,13 @@ class BitzError(RuntimeError):
         -10: "InvalidAssignmentDimensions",
         -11: "InvalidAssignmentLength",
         -12: "InvalidAssignmentConstant",
+        -20: "OutOfInput",
+        -21: "BadMagic",
+        -22: "UnsupportedVersion",
+        -23: "BadHeaderLen",
+        -24: "ReservedFlagsSet",
+        -25: "LengthOverflow",
+        -26: "LengthMismatch",
         -100: "DeviceError",
         -101: "AllocationError",
     }
@@ -61,6 +68,15 @@ class BitzError(RuntimeError):
 
 class SumcheckError(BitzError):
     """crates/spartan/src/sumcheck.rs:39-47 SumcheckError."""
+
+
+class ProofDecodingError(BitzError):
+    """crates/host/src/wire_proof.rs:43-60 ProofDecodingError; `detail` is the
+    variant's value (the version, header length, flags or declared length)."""
+
+    def __init__(self, code: int, message: str, detail: int = 0):
+        super().__init__(code, message)
+        self.detail = detail
 
 
 class F128(C.Structure):
@@ -143,6 +159,12 @@ def lib():
             "bitz_gkr_build_bittable": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32]),
             "bitz_gkr_prove": (I32, [P, P, P, U32, U32, P, P]),
             "bitz_transcript_set_grind": (I32, [P, P]),
+            "bitz_transcript_hint": (I32, [P, P, SZ]),
+            "bitz_transcript_hints": (SZ, [P, P, SZ]),
+            "bitz_transcript_proof": (I32, [P, P, SZ, C.POINTER(SZ)]),
+            "bitz_proof_encode": (I32, [P, SZ, P, SZ, P, SZ, C.POINTER(SZ)]),
+            "bitz_proof_decode": (I32, [P, SZ, C.POINTER(SZ), C.POINTER(SZ), C.POINTER(SZ), C.POINTER(SZ),
+                                        C.POINTER(C.c_uint64)]),
             "bitz_transcript_grind": (I32, [P, P, SZ, U32]),
             "bitz_pow_find": (I32, [P, U32, C.POINTER(C.c_uint64)]),
             "bitz_pow_valid": (I32, [P, C.c_uint64, U32]),
@@ -356,6 +378,25 @@ class Transcript:
             _check(lib().bitz_transcript_set_grind(t.h, C.cast(g, C.c_void_p)))
             t._keep = (a, s, g)
         return t
+
+    def hint(self, data: bytes) -> None:
+        """ProverState::hint: bytes beside the narg string, never absorbed."""
+        buf = C.create_string_buffer(bytes(data), max(len(data), 1))
+        _check(lib().bitz_transcript_hint(self.h, buf, len(data)))
+
+    def hints(self) -> bytes:
+        n = lib().bitz_transcript_hints(self.h, None, 0)
+        buf = (C.c_uint8 * max(n, 1))()
+        lib().bitz_transcript_hints(self.h, buf, n)
+        return bytes(buf[:n])
+
+    def proof(self) -> bytes:
+        """ProverState::finish, then the wire container (wire_proof::encode)."""
+        n = C.c_size_t()
+        _check(lib().bitz_transcript_proof(self.h, None, 0, C.byref(n)))
+        buf = (C.c_uint8 * max(n.value, 1))()
+        _check(lib().bitz_transcript_proof(self.h, buf, n.value, C.byref(n)))
+        return bytes(buf[: n.value])
 
     def grind(self, label: bytes, bits: int) -> None:
         """ProverState::grind (crates/transcript/src/prover.rs:29-41)."""
@@ -1110,6 +1151,30 @@ def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
                                                 _p(evals), _p(leaf_point), C.byref(claim)))
     return (msgs[: 2 * rounds].reshape(rounds, 2, 2), evals[: 2 * layers].reshape(layers, 2, 2), leaf_point,
             to_int(claim))
+
+
+def encode_proof(narg: bytes, hints: bytes = b"") -> bytes:
+    """crates/host/src/wire_proof.rs encode: the 32-byte header, narg, hints."""
+    narg, hints = bytes(narg), bytes(hints)
+    n = C.c_size_t()
+    a = C.create_string_buffer(narg, max(len(narg), 1))
+    b = C.create_string_buffer(hints, max(len(hints), 1))
+    _check(lib().bitz_proof_encode(a, len(narg), b, len(hints), None, 0, C.byref(n)))
+    out = (C.c_uint8 * n.value)()
+    _check(lib().bitz_proof_encode(a, len(narg), b, len(hints), out, n.value, C.byref(n)))
+    return bytes(out)
+
+
+def decode_proof(data: bytes) -> tuple[bytes, bytes]:
+    """wire_proof.rs decode: (narg, hints) of the exact encoding; anything
+    else raises ProofDecodingError with the reference's variant."""
+    data = bytes(data)
+    buf = C.create_string_buffer(data, max(len(data), 1))
+    no, nl, ho, hl, detail = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
+    rc = lib().bitz_proof_decode(buf, len(data), C.byref(no), C.byref(nl), C.byref(ho), C.byref(hl), C.byref(detail))
+    if rc:
+        raise ProofDecodingError(rc, (lib().bitz_last_error() or b"").decode(), detail.value)
+    return data[no.value: no.value + nl.value], data[ho.value: ho.value + hl.value]
 
 
 BITS100, BITS128 = 100, 128  # SecurityLevel (crates/common/src/security.rs:7-10)

@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbitz_hip.so")
-SOURCES = ["kernels.hip", "api.hip", "transcript.cpp", "sha256_r1cs.cpp"]
+SOURCES = ["kernels.hip", "api.hip", "transcript.cpp", "sha256_r1cs.cpp", "proof.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -25,9 +25,6 @@
 using bitz::kBlock;
 using bitz::Transcript;
 
-struct bitz_transcript {
-    Transcript t;
-};
 
 namespace {
 

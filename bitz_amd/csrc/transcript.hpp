@@ -45,6 +45,9 @@ struct Transcript {
     uint64_t seed = 0;
     uint8_t state[32] = {0};
     std::vector<uint8_t> narg;
+    // The hint stream (ProverState::hint, crates/transcript/src/prover.rs:64-68):
+    // prover data beside the narg string that the sponge never sees.
+    std::vector<uint8_t> hints;
 
     void absorb(int kind, const uint8_t* bytes, size_t len);
     void absorb_elems(int kind, const f128* e, size_t n) {
@@ -59,3 +62,8 @@ struct Transcript {
 };
 
 }  // namespace bitz
+
+// The C ABI's opaque transcript handle.
+struct bitz_transcript {
+    bitz::Transcript t;
+};

@@ -6,7 +6,7 @@
 
 use std::{env, path::PathBuf, process::Command};
 
-const SOURCES: [&str; 4] = ["kernels.hip", "api.hip", "transcript.cpp", "sha256_r1cs.cpp"];
+const SOURCES: [&str; 5] = ["kernels.hip", "api.hip", "transcript.cpp", "sha256_r1cs.cpp", "proof.cpp"];
 
 fn main() {
     let manifest = PathBuf::from(env::var("CARGO_MANIFEST_DIR").expect("cargo sets CARGO_MANIFEST_DIR"));

@@ -49,6 +49,14 @@ enum {
     BITZ_ERR_ASSIGNMENT_DIMS = -10,  /* SpartanError::InvalidAssignmentDimensions (crates/spartan/src/piop.rs:30, checked :80-82) */
     BITZ_ERR_ASSIGNMENT_LEN = -11,   /* SpartanMatrixError::InvalidAssignmentLength (matrix.rs:23) */
     BITZ_ERR_ASSIGNMENT_CONST = -12, /* SpartanMatrixError::InvalidAssignmentConstant (matrix.rs:24) */
+    /* ProofDecodingError (crates/host/src/wire_proof.rs:43-60) */
+    BITZ_ERR_PROOF_SHORT = -20,      /* OutOfInput: the input ends inside the header */
+    BITZ_ERR_PROOF_MAGIC = -21,      /* BadMagic */
+    BITZ_ERR_PROOF_VERSION = -22,    /* UnsupportedVersion(version) */
+    BITZ_ERR_PROOF_HEADER_LEN = -23, /* BadHeaderLen(len) */
+    BITZ_ERR_PROOF_FLAGS = -24,      /* ReservedFlagsSet(flags) */
+    BITZ_ERR_PROOF_OVERFLOW = -25,   /* LengthOverflow */
+    BITZ_ERR_PROOF_LENGTH = -26,     /* LengthMismatch { declared, actual } */
     BITZ_ERR_DEVICE = -100,      /* HIP runtime error; see bitz_last_error() */
     BITZ_ERR_ALLOC = -101        /* device allocation failed */
 };
@@ -120,6 +128,29 @@ int32_t bitz_transcript_grind(bitz_transcript* t, const uint8_t* label, size_t l
 int32_t bitz_pow_find(const uint8_t seed[16], uint32_t bits, uint64_t* nonce);
 int32_t bitz_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32_t bits);
 uint32_t bitz_grinding_bits(uint32_t security_bits, uint64_t coefficient);
+
+/* ---- proof bytes ------------------------------------------------------------ */
+/* Proof { narg_string, hints } (crates/transcript/src/proof.rs:13-16): the
+ * hint stream rides beside the narg string and is never absorbed
+ * (ProverState::hint, prover.rs:64-68); built-in transcripts keep it, a
+ * callback transcript keeps its own (BITZ_ERR_ARG). */
+int32_t bitz_transcript_hint(bitz_transcript* t, const uint8_t* bytes, size_t len);
+/* Copies up to cap hint bytes; returns the total hint length. */
+size_t bitz_transcript_hints(const bitz_transcript* t, uint8_t* out, size_t cap);
+/* The proof container (crates/host/src/wire_proof.rs): a 32-byte header
+ * (magic "bitzPCS\0", wire_version 1, header_len 32, flags 0, le64 narg
+ * length, le64 hint length, all little-endian), the narg bytes, the hint
+ * bytes.  encode: *len = 32 + narg_len + hints_len (out NULL: length only;
+ * cap too small: BITZ_ERR_ARG).  decode: the exact encoding or the first
+ * failing check in the header's order (BITZ_ERR_PROOF_*; *detail = the
+ * version, header length, flags or declared length of the error); on
+ * success the two streams' offsets and lengths within `bytes`. */
+int32_t bitz_proof_encode(const uint8_t* narg, size_t narg_len, const uint8_t* hints, size_t hints_len,
+                          uint8_t* out, size_t cap, size_t* len);
+int32_t bitz_proof_decode(const uint8_t* bytes, size_t len, size_t* narg_offset, size_t* narg_len,
+                          size_t* hints_offset, size_t* hints_len, uint64_t* detail);
+/* ProverState::finish then wire_proof::encode for a built-in transcript. */
+int32_t bitz_transcript_proof(const bitz_transcript* t, uint8_t* out, size_t cap, size_t* len);
 
 /* ---- field and multilinear primitives -------------------------------------- */
 /* out[i] = a[i] * b[i] (F128 Mul, crates/field/src/gf128.rs:275-281).  The

@@ -681,6 +681,9 @@ def test_gkr_bits128_grinds_like_the_oracle(log_leaves, log_groups):
     gp, gc = G.gpgkr_prove(gt, point, wit, G.BITS128)
     assert np.array_equal(gp, ref_point) and gc == ref_claim and gt.narg() == ot.narg()
     assert gc == O.mle_evaluate(leaves, gp)
+    # the serialized proof: the narg string (nonces included) in the wire container
+    assert G.decode_proof(gt.proof()) == (ot.narg(), b"")
+    assert gt.proof() == G.encode_proof(ot.narg())
     rounds = sum(range(log_groups, log_leaves))
     layers = log_leaves - log_groups
     assert len(gt.narg()) == 32 * rounds + 32 * layers + 8 * rounds  # messages, [L, R], one nonce per round
