@@ -49,6 +49,8 @@ class BitzError(RuntimeError):
         -10: "InvalidAssignmentDimensions",
         -11: "InvalidAssignmentLength",
         -12: "InvalidAssignmentConstant",
+        -13: "WitnessLengthMismatch",
+        -14: "ClaimDoesNotHold",
         -20: "OutOfInput",
         -21: "BadMagic",
         -22: "UnsupportedVersion",
@@ -159,6 +161,7 @@ def lib():
             "bitz_gkr_build_bittable": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32]),
             "bitz_gkr_prove": (I32, [P, P, P, U32, U32, P, P]),
             "bitz_transcript_set_grind": (I32, [P, P]),
+            "bitz_post_gkr_prove": (I32, [P, P, U32, P, U32, F128, P, SZ, U32, U32, P, P]),
             "bitz_transcript_hint": (I32, [P, P, SZ]),
             "bitz_transcript_hints": (SZ, [P, P, SZ]),
             "bitz_transcript_proof": (I32, [P, P, SZ, C.POINTER(SZ)]),
@@ -1207,3 +1210,22 @@ def gpgkr_prove(transcript: Transcript, point, witnesses: LayerWitnesses, securi
     _check(lib().bitz_gkr_prove(witnesses.h, transcript.h, _p(point), len(point), security, _p(leaf_point),
                                 C.byref(claim)))
     return leaf_point, to_int(claim)
+
+
+# --------------------------------------------------------------- post_gkr --
+def post_gkr_prove(transcript: Transcript, rows, columns, target, packed, security: int = BITS100):
+    """post_gkr::prove (crates/post_gkr/src/lib.rs:85-148) on the GPU: the
+    claim <rows (x) columns, f> = target over the bits f (2^t x 2^s, t >= 7,
+    `packed` 128 bits per element column after column, or DeviceTables)
+    reduced to MLE[f](point) = evaluation.  Returns (point (t + s, 2),
+    evaluation)."""
+    init()
+    (r, nr, fr), (c, nc, fc), (p, npk, fp) = _table(rows), _table(columns), _table(packed)
+    if not (fr == fc == fp):
+        raise BitzError(-7, "rows, columns and packed must all be host arrays or all DeviceTables")
+    t, s = _nvars(nr), _nvars(nc)
+    point = np.zeros((t + s, 2), dtype=np.uint64)
+    ev = F128()
+    _check(lib().bitz_post_gkr_prove(transcript.h, _p(r), t, _p(c), s, f128(target), _p(p), npk, security, fr,
+                                     _p(point), C.byref(ev)))
+    return point, to_int(ev)

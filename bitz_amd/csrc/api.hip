@@ -2704,6 +2704,107 @@ extern "C" int32_t bitz_prove_inner_sumcheck_with_challenges_noalloc(
     return BITZ_OK;
 }
 
+// ============================================================ post-GKR =====
+// prove_factors (crates/post_gkr/src/lib.rs:125-148) with the degree-two
+// rounds of crates/post_gkr/src/sumcheck.rs:99-178: the claim
+// <rows (x) columns, f> = target over the committed bits f (2^t x 2^s, bit
+// (c << t) | b, packed 128 to an element, t >= 7) becomes MLE[f](point) =
+// evaluation.  Round 0 runs off the packed bits (k_pg_first: XOR sums of
+// row weights, one product per element); the device then writes W = the rows
+// folded at r_0 tensored with the columns and V = the bits bound at r_0
+// (k_pg_expand), and the other t + s - 1 rounds are the inner sumcheck's
+// rounds over (W, V): the same [sum w0 v0, sum (w0 + w1)(v0 + v1)] and fold.
+// Each round: its [a0, a2] as a prover message, the grind, the challenge;
+// the closing evaluation as a prover message.
+extern "C" int32_t bitz_post_gkr_prove(bitz_transcript* t, const bitz_f128* rows, uint32_t log_rows,
+                                       const bitz_f128* columns, uint32_t log_columns, bitz_f128 target,
+                                       const bitz_f128* packed, size_t packed_len, uint32_t security_bits,
+                                       uint32_t flags, bitz_f128* point, bitz_f128* evaluation) {
+    if (!t || !rows || !columns || !packed || !point || !evaluation) return fail(BITZ_ERR_ARG, "null pointer");
+    if (log_rows < 7 || log_rows + log_columns > 40)
+        return fail(BITZ_ERR_ARG, "the claim is over 2^t x 2^s bits with 7 <= t, t + s <= 40");
+    if (security_bits != 100 && security_bits != 128)
+        return fail(BITZ_ERR_ARG, "security_bits is a SecurityLevel: 100 or 128");
+    const size_t per_column = (size_t)1 << (log_rows - 7), ncols = (size_t)1 << log_columns;
+    if (packed_len != per_column * ncols)  // lib.rs:133-135
+        return fail(BITZ_ERR_WITNESS_LENGTH, "the packed witness is not one element per 128 bits of the claim");
+    StreamOwner so;
+    TRY(so.create());
+    hipStream_t s = so.s;
+    const size_t nrows = (size_t)1 << log_rows;
+    Dev drows, dcols, dpacked;
+    const f128* R = reinterpret_cast<const f128*>(rows);
+    const f128* Cw = reinterpret_cast<const f128*>(columns);
+    const f128* Pk = reinterpret_cast<const f128*>(packed);
+    if (!(flags & BITZ_DEVICE)) {
+        TRY(upload(drows, rows, nrows, flags, s));
+        TRY(upload(dcols, columns, ncols, flags, s));
+        TRY(upload(dpacked, packed, packed_len, flags, s));
+        R = drows.f();
+        Cw = dcols.f();
+        Pk = dpacked.f();
+    }
+    Dev acc;
+    TRY(acc.alloc(64));
+    HIP_TRY(hipMemsetAsync(acc.p, 0, 64, s));
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((packed_len + 3) / 4, 2048));
+    hipLaunchKernelGGL(bitz::k_pg_first, dim3(blocks), dim3(256), 0, s, R, (int)log_rows, Cw, Pk, packed_len,
+                       acc.u32());
+    TRY(check_launch());
+    Pinned host;
+    TRY(host.alloc(64));
+    HIP_TRY(hipMemcpyAsync(host.p, acc.p, 48, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    f128 sums[3];
+    memcpy(sums, host.p, 48);
+    if (!hostf::eq(sums[0], F(target)))  // lib.rs:136-138
+        return fail(BITZ_ERR_CLAIM, "the claim's weights against the witness do not give the target");
+    static const uint8_t kLabel[] = "bitz/pcs/sumcheck/v1";  // lib.rs:71
+    const uint32_t gbits = bitz::grinding_bits(security_bits, 2);
+    // lib.rs:286-289 advance: h' = a0 + r (h + a2 + r a2).
+    auto advance = [](f128 h, f128 a0, f128 a2, f128 r) {
+        return hostf::add(a0, hostf::mul(r, hostf::add(hostf::add(h, a2), hostf::mul(r, a2))));
+    };
+    f128 msg[2] = {sums[1], sums[2]};
+    t->t.absorb_elems(1, msg, 2);
+    t->t.grind(kLabel, sizeof kLabel - 1, gbits);
+    f128 r = t->t.squeeze();
+    point[0] = B(r);
+    f128 h = advance(F(target), msg[0], msg[1], r);
+    const int m1 = (int)(log_rows + log_columns) - 1;
+    const size_t n = (size_t)1 << m1, half_rows = nrows / 2;
+    Dev rf, W, V;
+    TRY(rf.alloc(half_rows * sizeof(f128)));
+    TRY(W.alloc(n * sizeof(f128)));
+    TRY(V.alloc(n * sizeof(f128)));
+    hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(half_rows)), dim3(kBlock), 0, s, R, rf.f(), half_rows, r);
+    hipLaunchKernelGGL(bitz::k_pg_expand, dim3(grid_for(n)), dim3(kBlock), 0, s, rf.f(), (int)log_rows, Cw, Pk, r,
+                       W.f(), V.f(), n);
+    TRY(check_launch());
+    HIP_TRY(hipStreamSynchronize(s));  // the inner handle works on a stream of its own
+    bitz_inner* ih = nullptr;
+    TRY(bitz_inner_begin(&ih, reinterpret_cast<const bitz_f128*>(W.p), (uint32_t)m1,
+                         reinterpret_cast<const bitz_f128*>(V.p), (uint32_t)m1, BITZ_DEVICE));
+    std::unique_ptr<bitz_inner, void (*)(bitz_inner*)> guard(ih, bitz_inner_free);
+    bitz_f128 co[2];
+    TRY(bitz_inner_round0(ih, co));
+    for (int k = 1; k <= m1; ++k) {
+        const f128 m[2] = {F(co[0]), F(co[1])};
+        t->t.absorb_elems(1, m, 2);
+        t->t.grind(kLabel, sizeof kLabel - 1, gbits);
+        r = t->t.squeeze();
+        point[k] = B(r);
+        h = advance(h, m[0], m[1], r);
+        TRY(bitz_inner_fold_next(ih, B(r), co));
+    }
+    bitz_f128 ev[2];
+    TRY(bitz_inner_finish(ih, ev));
+    const f128 v = F(ev[1]);
+    t->t.absorb_elems(1, &v, 1);  // sumcheck.rs:172-177
+    *evaluation = ev[1];
+    return BITZ_OK;
+}
+
 // ========================================================= matrices =======
 struct bitz_matrices {
     size_t rows = 0, rows_padded = 0, cols = 0;

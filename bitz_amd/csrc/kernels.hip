@@ -443,6 +443,69 @@ __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* o
 
 // 4-bit constant tables, one workgroup per constant, into global memory;
 // workgroups that multiply by a constant copy its table into LDS.
+// ------------------------------------------------- post-GKR sumcheck --
+// The first round off the packed bits (crates/post_gkr/src/lib.rs:189-232):
+// element e carries bits 128 (e mod 2^(t-7)) .. of column e >> (t-7); one
+// wave per element, lane l the bit pair (2l, 2l + 1).  Per element the wave
+// XORs the row weights of the set bits (the claim's weighted sum), of the set
+// even bits (a0) and w0 + w1 where the pair's bits differ (a2); lane 0 weights
+// the three sums by the column's weight.  acc: 3 elements.
+__global__ __launch_bounds__(256) void k_pg_first(const f128* rows, int t, const f128* cols, const f128* packed,
+                                                  size_t n_elems, uint32_t* acc) {
+    const int lane = threadIdx.x & 63;
+    const size_t wave = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
+    const size_t row_mask = ((size_t)1 << (t - 7)) - 1;
+    g4 sums[3] = {g_zero(), g_zero(), g_zero()};
+    for (size_t e = wave; e < n_elems; e += waves) {
+        const size_t base = (e & row_mask) << 7;
+        const f128 el = packed[e];
+        const uint64_t word = lane < 32 ? el.lo : el.hi;
+        const int sh = (2 * lane) & 63;
+        const bool f0 = (word >> sh) & 1, f1 = (word >> (sh + 1)) & 1;
+        const g4 r0 = g_load(rows + base + 2 * lane), r1 = g_load(rows + base + 2 * lane + 1);
+        g4 v[3];
+        v[0] = g_add(f0 ? r0 : g_zero(), f1 ? r1 : g_zero());
+        v[1] = f0 ? r0 : g_zero();
+        v[2] = (f0 != f1) ? g_add(r0, r1) : g_zero();
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                uint32_t x = v[k].w[i];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
+                v[k].w[i] = x;
+            }
+        if (lane == 0) {
+            const g4 c = g_load(cols + (e >> (t - 7)));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sums[k] = g_add(sums[k], g_mul(c, v[k]));
+        }
+    }
+    block_xor_to_global<3>(sums, acc);
+}
+
+// The tables after the first round (lib.rs:236-244, 265-284): W[j] =
+// columns[j >> (t-1)] rf[j mod 2^(t-1)] (rf: the rows folded at r) and V[j]
+// = the bit pair j bound at r: 0, 1 + r, r or 1.  Pair j is bits 2j, 2j + 1,
+// in element j >> 6.
+__global__ __launch_bounds__(kBlock) void k_pg_expand(const f128* rf, int t, const f128* cols, const f128* packed,
+                                                      f128 r, f128* W, f128* V, size_t n) {
+    const g4 rr = g_from(r);
+    const g4 one = g_from(f128{1, 0});
+    const g4 rr1 = g_add(rr, one);
+    const size_t row_mask = ((size_t)1 << (t - 1)) - 1;
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) {
+        g_store(W + j, g_mul(g_load(cols + (j >> (t - 1))), g_load(rf + (j & row_mask))));
+        const f128 el = packed[j >> 6];
+        const int p = (int)(j & 63);
+        const uint64_t word = p < 32 ? el.lo : el.hi;
+        const unsigned q = (unsigned)(word >> ((2 * p) & 63)) & 3u;  // f0 | f1 << 1
+        g_store(V + j, q == 0 ? g_zero() : q == 1 ? rr1 : q == 2 ? rr : one);
+    }
+}
+
 __global__ void k_zero_words(uint32_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }

@@ -64,6 +64,12 @@ struct HtabDesc {
 };
 
 __global__ void k_const_tables4(const f128* consts, uint4* out);
+// post-GKR sumcheck (crates/post_gkr): the first round off the packed bits
+// (acc: [weighted sum, a0, a2]) and the tables W, V after it.
+__global__ void k_pg_first(const f128* rows, int t, const f128* cols, const f128* packed, size_t n_elems,
+                           uint32_t* acc);
+__global__ void k_pg_expand(const f128* rf, int t, const f128* cols, const f128* packed, f128 r, f128* W, f128* V,
+                            size_t n);
 // p[0 .. n) = 0 (a kernel, where a memset node would not do: captured graphs).
 __global__ void k_zero_words(uint32_t* p, size_t n);
 

@@ -15,6 +15,8 @@ pub struct BitzF128 {
 pub const BITZ_OK: i32 = 0;
 pub const BITZ_ERR_PRODUCT_DIMS: i32 = -1;
 pub const BITZ_ERR_EQUALITY_DIMS: i32 = -2;
+pub const BITZ_ERR_WITNESS_LENGTH: i32 = -13;
+pub const BITZ_ERR_CLAIM: i32 = -14;
 /// Element arrays are device pointers (from `bitz_device_alloc`).
 pub const BITZ_DEVICE: u32 = 1;
 
@@ -143,4 +145,20 @@ unsafe extern "C" {
         leaf_claim: *mut BitzF128,
     ) -> i32;
     pub fn bitz_gkr_free(h: *mut Gkr);
+
+    // crates/post_gkr/src/lib.rs:85-148
+    pub fn bitz_post_gkr_prove(
+        t: *mut Transcript,
+        rows: *const BitzF128,
+        log_rows: u32,
+        columns: *const BitzF128,
+        log_columns: u32,
+        target: BitzF128,
+        packed: *const BitzF128,
+        packed_len: usize,
+        security_bits: u32,
+        flags: u32,
+        point: *mut BitzF128,
+        evaluation: *mut BitzF128,
+    ) -> i32;
 }

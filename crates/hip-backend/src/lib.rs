@@ -10,7 +10,8 @@
 //! - [`HipConstraintMatrices::bind_and_batch`] for
 //!   `PreparedConstraintMatrices::bind_and_batch` (crates/spartan/src/matrix.rs:429-468);
 //! - [`grand_product`] and [`gpgkr_prove`] for `gkr::grand_product` and
-//!   `gkr::gpgkr_prove` (crates/gkr/src/lib.rs:24-79, 785-802).
+//!   `gkr::gpgkr_prove` (crates/gkr/src/lib.rs:24-79, 785-802);
+//! - [`post_gkr_prove`] for `post_gkr::prove` (crates/post_gkr/src/lib.rs:85-148).
 //!
 //! The transcript stays here, in `transcript::ProverState`: the sumchecks run
 //! through the library's per-round handles and this crate completes every
@@ -31,7 +32,7 @@ use std::ffi::{CStr, c_void};
 use std::ptr;
 use std::sync::OnceLock;
 
-use common::{BitTable, SecurityLevel};
+use common::{BitTable, LinearClaim, OpeningQuery, SecurityLevel};
 use field::F128;
 use poly::DenseMultilinearExtension;
 use spartan::{
@@ -611,18 +612,21 @@ pub fn grand_product_dense(leaves: &[F128], log_trees: usize) -> (Vec<F128>, Hip
     (output, HipLayerWitnesses { h, log_leaves, log_trees })
 }
 
-// The callback transcript of gpgkr_prove: every absorb is a prover message
-// [F128; 2] (lib.rs:142, 154), every squeeze a verifier message, every
-// grind ProverState::grind (lib.rs:144).  A panic here aborts the process.
+// The callback transcript of gpgkr_prove and post_gkr_prove: every absorb is
+// a prover message, [F128; 2] (gkr lib.rs:142, 154; post_gkr lib.rs:141,
+// sumcheck.rs:129) or one F128 (post_gkr sumcheck.rs:175), every squeeze a
+// verifier message, every grind ProverState::grind (gkr lib.rs:144, post_gkr
+// lib.rs:142).  A panic here aborts the process.
 extern "C" fn gkr_absorb(ctx: *mut c_void, kind: i32, bytes: *const u8, len: usize) {
-    // SAFETY: ctx is the &mut ProverState gpgkr_prove lent; bytes holds len bytes.
+    // SAFETY: ctx is the &mut ProverState the prover lent; bytes holds len bytes.
     let (ps, data) = unsafe { (&mut *ctx.cast::<ProverState>(), std::slice::from_raw_parts(bytes, len)) };
-    assert!(kind == 1 && len == 32, "the GKR prover absorbs [F128; 2] prover messages");
-    let message = [
-        F128::from_bytes(data[..16].try_into().expect("16 bytes")),
-        F128::from_bytes(data[16..].try_into().expect("16 bytes")),
-    ];
-    ps.prover_message(&message);
+    assert!(kind == 1 && (len == 32 || len == 16), "the provers absorb F128 prover messages");
+    let element = |i: usize| F128::from_bytes(data[16 * i..16 * i + 16].try_into().expect("16 bytes"));
+    if len == 32 {
+        ps.prover_message(&[element(0), element(1)]);
+    } else {
+        ps.prover_message(&element(0));
+    }
 }
 
 extern "C" fn gkr_squeeze(ctx: *mut c_void, out: *mut u8) {
@@ -677,6 +681,66 @@ pub fn gpgkr_prove(
     unsafe { ffi::bitz_transcript_free(t) };
     expect_ok(rc);
     (leaf_point.into_iter().map(from_c).collect(), from_c(claim))
+}
+
+/// Why `post_gkr::prove` cannot run (crates/post_gkr/src/lib.rs:50-56).
+#[derive(Clone, Copy, Debug, PartialEq, Eq)]
+pub enum PostGkrProveError {
+    WitnessLengthMismatch,
+    ClaimDoesNotHold,
+}
+
+/// `post_gkr::prove` (crates/post_gkr/src/lib.rs:85-148) on the GPU: the
+/// claim's inner product against the committed bits (`packed`, 128 bits an
+/// element, column after column) reduced to `MLE[f](point) = target`, the
+/// transcript calls and grinding on `transcript` in the reference's order.
+pub fn post_gkr_prove(
+    claim: &LinearClaim<F128>,
+    packed: &[F128],
+    security: SecurityLevel,
+    transcript: &mut ProverState,
+) -> Result<OpeningQuery, PostGkrProveError> {
+    init();
+    let (rows, columns) = (claim.row_weights(), claim.column_weights());
+    let (log_rows, log_columns) = (rows.len().trailing_zeros(), columns.len().trailing_zeros());
+    let ctx: *mut ProverState = transcript;
+    // SAFETY: the callbacks run only inside bitz_post_gkr_prove below.
+    let t = unsafe { ffi::bitz_transcript_callbacks(ctx.cast(), gkr_absorb, gkr_squeeze) };
+    assert!(!t.is_null(), "bitz_transcript_callbacks failed");
+    // SAFETY: t is a callback transcript.
+    expect_ok(unsafe { ffi::bitz_transcript_set_grind(t, gkr_grind) });
+    let mut point = vec![BitzF128::default(); (log_rows + log_columns) as usize];
+    let mut evaluation = BitzF128::default();
+    // SAFETY: 2^t rows, 2^s columns, packed_len elements; point has t + s slots.
+    let rc = unsafe {
+        ffi::bitz_post_gkr_prove(
+            t,
+            c_ptr(rows),
+            log_rows,
+            c_ptr(columns),
+            log_columns,
+            to_c(claim.target()),
+            c_ptr(packed),
+            packed.len(),
+            security.bits(),
+            0,
+            point.as_mut_ptr(),
+            &mut evaluation,
+        )
+    };
+    // SAFETY: created above, freed once.
+    unsafe { ffi::bitz_transcript_free(t) };
+    match rc {
+        ffi::BITZ_ERR_WITNESS_LENGTH => Err(PostGkrProveError::WitnessLengthMismatch),
+        ffi::BITZ_ERR_CLAIM => Err(PostGkrProveError::ClaimDoesNotHold),
+        other => {
+            expect_ok(other);
+            Ok(OpeningQuery::Mle {
+                point: point.into_iter().map(from_c).collect(),
+                target: from_c(evaluation),
+            })
+        }
+    }
 }
 
 #[cfg(test)]

@@ -49,6 +49,8 @@ enum {
     BITZ_ERR_ASSIGNMENT_DIMS = -10,  /* SpartanError::InvalidAssignmentDimensions (crates/spartan/src/piop.rs:30, checked :80-82) */
     BITZ_ERR_ASSIGNMENT_LEN = -11,   /* SpartanMatrixError::InvalidAssignmentLength (matrix.rs:23) */
     BITZ_ERR_ASSIGNMENT_CONST = -12, /* SpartanMatrixError::InvalidAssignmentConstant (matrix.rs:24) */
+    BITZ_ERR_WITNESS_LENGTH = -13,   /* post_gkr ProveError::WitnessLengthMismatch (crates/post_gkr/src/lib.rs:52-53) */
+    BITZ_ERR_CLAIM = -14,            /* post_gkr ProveError::ClaimDoesNotHold (lib.rs:54-55) */
     /* ProofDecodingError (crates/host/src/wire_proof.rs:43-60) */
     BITZ_ERR_PROOF_SHORT = -20,      /* OutOfInput: the input ends inside the header */
     BITZ_ERR_PROOF_MAGIC = -21,      /* BadMagic */
@@ -354,6 +356,21 @@ int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint
                                 bitz_f128* outer_rounds, bitz_f128* outer_claims, bitz_f128* inner_rounds,
                                 bitz_f128* inner_point, bitz_f128* matrix_eval, bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
+
+/* ---- post-GKR inner-product sumcheck ----------------------------------------- */
+/* post_gkr::prove (crates/post_gkr/src/lib.rs:85-148, sumcheck.rs:99-178):
+ * the claim <rows (x) columns, f> = target over the committed bits f, 2^t x
+ * 2^s (t = log_rows >= 7, bit (c << t) | b weighs rows[b] columns[c]),
+ * packed 128 bits to an element column after column (packed_len =
+ * 2^(t-7+s)); returns MLE[f](point) = *evaluation, point = t + s challenges
+ * low coordinate first.  Per round: prover message [a0, a2], grind
+ * grinding_bits(2) bits ("bitz/pcs/sumcheck/v1"; 1 at Bits128), challenge;
+ * then the evaluation as a prover message.  Errors before the transcript:
+ * BITZ_ERR_WITNESS_LENGTH, BITZ_ERR_CLAIM.  BITZ_DEVICE: rows, columns and
+ * packed are device arrays. */
+int32_t bitz_post_gkr_prove(bitz_transcript* t, const bitz_f128* rows, uint32_t log_rows, const bitz_f128* columns,
+                            uint32_t log_columns, bitz_f128 target, const bitz_f128* packed, size_t packed_len,
+                            uint32_t security_bits, uint32_t flags, bitz_f128* point, bitz_f128* evaluation);
 
 /* ---- many PIOP proofs, rounds batched (no reference counterpart) -------- */
 /* An engine advances every submitted prove_spartan_piop (piop.rs:67-112) by

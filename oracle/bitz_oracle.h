@@ -101,8 +101,23 @@ enum {
     ORACLE_ERR_TERMINAL = -4,      /* SumcheckError::InvalidTerminalClaim */
     ORACLE_ERR_ROW_POINT = -5,     /* SpartanMatrixError::InvalidRowPointLength */
     ORACLE_ERR_ROUND_COUNT = -6,   /* SumcheckError::InvalidRoundCount */
-    ORACLE_ERR_ARG = -7
+    ORACLE_ERR_ARG = -7,
+    ORACLE_POST_GKR_WITNESS_LENGTH = -13, /* post_gkr ProveError::WitnessLengthMismatch */
+    ORACLE_POST_GKR_CLAIM = -14,          /* post_gkr ProveError::ClaimDoesNotHold */
+    ORACLE_POST_GKR_MISMATCH = -15        /* post_gkr VerifyError::EvaluationMismatch */
 };
+
+/* ---- post_gkr: crates/post_gkr/src/{lib.rs,sumcheck.rs} ------------------- */
+/* prove_factors (lib.rs:125-148): rows 2^t (t >= 7), columns 2^s, packed
+ * 2^(t-7+s) elements of 128 bits; point receives t + s challenges (the
+ * first off the packed bits), *evaluation = MLE[f](point). */
+int oracle_post_gkr_prove(oracle_transcript* tr, const of128* rows, int t, const of128* columns, int s,
+                          of128 target, const of128* packed, size_t packed_len, uint32_t security_bits,
+                          of128* point, of128* evaluation);
+/* verify_factors (lib.rs:151-166) over the prover's (t + s) x [a0, a2]. */
+int oracle_post_gkr_verify(oracle_transcript* tr, const of128* rows, int t, const of128* columns, int s,
+                           of128 target, uint32_t security_bits, const of128* messages, of128 evaluation,
+                           of128* point);
 
 /* prove_outer_sumcheck, sumcheck.rs:252-384.  round_polys receives
  * num_vars x [c0,c1,c2,c3]; claims = [az, bz, cz](r_x). */

@@ -122,6 +122,10 @@ def lib():
         L.oracle_gkr_prove.restype = OF128
         L.oracle_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, P, C.c_uint32]
         L.oracle_transcript_grind.argtypes = [P, P, C.c_size_t, C.c_uint32]
+        L.oracle_post_gkr_prove.restype = C.c_int
+        L.oracle_post_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, OF128, P, C.c_size_t, C.c_uint32, P, P]
+        L.oracle_post_gkr_verify.restype = C.c_int
+        L.oracle_post_gkr_verify.argtypes = [P, P, C.c_int, P, C.c_int, OF128, C.c_uint32, P, OF128, P]
         L.oracle_blake3_small.restype = C.c_int
         L.oracle_blake3_small.argtypes = [P, C.c_size_t, P]
         L.oracle_pow_valid.restype = C.c_int
@@ -509,3 +513,25 @@ def gkr_verify(tr, claim, point, layers, messages):
     rc = lib().oracle_gkr_verify(tr.ref, f(claim), ptr(point), len(point), layers, ptr(messages),
                                  ptr(lp), ptr(lc))
     return rc, lp[:total], to_int(lc[0])
+
+
+# ---- post_gkr ------------------------------------------------------------------------
+def post_gkr_prove(tr, rows, columns, target, packed, security_bits: int = 100):
+    """crates/post_gkr prove_factors: returns (rc, point, evaluation)."""
+    rows, columns, packed = as_elems(rows), as_elems(columns), as_elems(packed)
+    t, s = len(rows).bit_length() - 1, len(columns).bit_length() - 1
+    point = arr(t + s)
+    ev = arr(1)
+    rc = lib().oracle_post_gkr_prove(tr.ref, ptr(rows), t, ptr(columns), s, f(target), ptr(packed), len(packed),
+                                     security_bits, ptr(point), ptr(ev))
+    return rc, point, to_int(ev[0])
+
+
+def post_gkr_verify(tr, rows, columns, target, messages, evaluation, security_bits: int = 100):
+    """crates/post_gkr verify_factors over the prover's messages: (rc, point)."""
+    rows, columns, messages = as_elems(rows), as_elems(columns), as_elems(messages)
+    t, s = len(rows).bit_length() - 1, len(columns).bit_length() - 1
+    point = arr(t + s)
+    rc = lib().oracle_post_gkr_verify(tr.ref, ptr(rows), t, ptr(columns), s, f(target), security_bits,
+                                      ptr(messages), f(evaluation), ptr(point))
+    return rc, point
