@@ -115,6 +115,8 @@ def main() -> int:
     ap.add_argument("--chain-blocks", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-engines", type=int, default=0, help="serve: PIOP engines (0: one per 8 slots)")
+    ap.add_argument("--serve-preparers", type=int, default=4,
+                    help="serve: host threads preparing requests (witness, assignment, Az/Bz/Cz)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,
                     help="inner: log2 table length (23: the column count of a 256-block SHA-256 chain)")
@@ -585,7 +587,7 @@ def bench_serve(G, args) -> int:
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
-        srv.serve_batched(queued(args.seed + 500 + i), engines=engines)
+        srv.serve_batched(queued(args.seed + 500 + i), engines=engines, preparers=args.serve_preparers)
         log(f"bench: warmup {i + 1}/{args.warmup}: {(time.perf_counter() - t0) * 1e3:.1f} ms")
     # Steps: inputs drawn before the clock starts; HIP events on the engines'
     # streams time every batched inner fold launch.
@@ -603,7 +605,7 @@ def bench_serve(G, args) -> int:
     t_start = time.perf_counter()
     for rs in step_reqs:
         t0 = time.perf_counter()
-        srv.serve_batched(rs, engines=engines)
+        srv.serve_batched(rs, engines=engines, preparers=args.serve_preparers)
         step_ms.append((time.perf_counter() - t0) * 1e3)
     G.synchronize()
     wall = time.perf_counter() - t_start
@@ -618,7 +620,7 @@ def bench_serve(G, args) -> int:
         rs = serve_mix(seed, args.serve_latency_requests, rate)
         t0 = time.perf_counter()
         if mode == "batched":
-            srv.serve_batched(rs, engines=engines)
+            srv.serve_batched(rs, engines=engines, preparers=args.serve_preparers)
         else:
             srv.serve(rs)
         span = max(r.done for r in rs) - min(r.arrival for r in rs)

@@ -161,6 +161,8 @@ def lib():
             "bitz_gkr_build_bittable": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, P, U32]),
             "bitz_gkr_prove": (I32, [P, P, P, U32, U32, P, P]),
             "bitz_transcript_set_grind": (I32, [P, P]),
+            "bitz_transcript_keccak": (P, [P, SZ]),
+            "bitz_keccak_hash": (I32, [P, SZ, I32, C.c_uint8, P, SZ, U32]),
             "bitz_post_gkr_prove": (I32, [P, P, U32, P, U32, F128, P, SZ, U32, U32, P, P]),
             "bitz_transcript_hint": (I32, [P, P, SZ]),
             "bitz_transcript_hints": (SZ, [P, P, SZ]),
@@ -356,6 +358,13 @@ class Transcript:
     def duplex(cls, domain: bytes) -> "Transcript":
         buf = C.create_string_buffer(domain, len(domain))
         return cls(lib().bitz_transcript_duplex(buf, len(domain)))
+
+    @classmethod
+    def keccak(cls, domain: bytes) -> "Transcript":
+        """The Keccak duplex (bitz_transcript_keccak); its challenges can be
+        drawn on the device."""
+        buf = C.create_string_buffer(domain, max(len(domain), 1))
+        return cls(lib().bitz_transcript_keccak(buf, len(domain)))
 
     @classmethod
     def callbacks(cls, absorb, squeeze, grind=None) -> "Transcript":
@@ -1154,6 +1163,17 @@ def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
                                                 _p(evals), _p(leaf_point), C.byref(claim)))
     return (msgs[: 2 * rounds].reshape(rounds, 2, 2), evals[: 2 * layers].reshape(layers, 2, 2), leaf_point,
             to_int(claim))
+
+
+def keccak_hash(msg: bytes, rate: int, pad: int, out_len: int, device: bool = False) -> bytes:
+    """A one-message Keccak sponge (rate in bytes; pad 0x06 SHA3, 0x1F SHAKE),
+    on the host or by one device thread."""
+    if device:
+        init()
+    out = (C.c_uint8 * max(out_len, 1))()
+    buf = C.create_string_buffer(bytes(msg), max(len(msg), 1))
+    _check(lib().bitz_keccak_hash(buf, len(msg), rate, pad, out, out_len, DEVICE if device else 0))
+    return bytes(out[:out_len])
 
 
 def encode_proof(narg: bytes, hints: bytes = b"") -> bytes:

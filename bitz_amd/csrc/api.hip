@@ -521,6 +521,35 @@ extern "C" int32_t bitz_pow_valid(const uint8_t seed[16], uint64_t nonce, uint32
 extern "C" uint32_t bitz_grinding_bits(uint32_t security_bits, uint64_t coefficient) {
     return bitz::grinding_bits(security_bits, coefficient);
 }
+extern "C" bitz_transcript* bitz_transcript_keccak(const uint8_t* domain, size_t len) {
+    if (len && !domain) return nullptr;
+    auto* t = new bitz_transcript;
+    t->t.type = Transcript::Type::kKeccak;
+    bitz::keccak_start(t->t.keccak, domain, len);
+    return t;
+}
+extern "C" int32_t bitz_keccak_hash(const uint8_t* msg, size_t len, int32_t rate, uint8_t pad, uint8_t* out,
+                                    size_t out_len, uint32_t flags) {
+    if ((len && !msg) || (out_len && !out)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (rate <= 0 || rate > 200 || rate % 8) return fail(BITZ_ERR_ARG, "the rate is a multiple of 8 bytes below 200");
+    if (!(flags & BITZ_DEVICE)) {
+        bitz::keccak_hash(msg, len, rate, pad, out, out_len);
+        return BITZ_OK;
+    }
+    // The same permutation run by one device thread (the device code's check).
+    StreamOwner so;
+    TRY(so.create());
+    Dev dm, dout;
+    TRY(dm.alloc(std::max<size_t>(len, 1)));
+    TRY(dout.alloc(std::max<size_t>(out_len, 1)));
+    if (len) HIP_TRY(hipMemcpyAsync(dm.p, msg, len, hipMemcpyHostToDevice, so.s));
+    hipLaunchKernelGGL(bitz::k_keccak_hash, dim3(1), dim3(64), 0, so.s, static_cast<const uint8_t*>(dm.p), len,
+                       (int)rate, pad, static_cast<uint8_t*>(dout.p), out_len);
+    TRY(check_launch());
+    if (out_len) HIP_TRY(hipMemcpyAsync(out, dout.p, out_len, hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipStreamSynchronize(so.s));
+    return BITZ_OK;
+}
 extern "C" bitz_transcript* bitz_transcript_seeded(uint64_t seed) {
     auto* t = new bitz_transcript;
     t->t.type = Transcript::Type::kSeeded;
@@ -2294,6 +2323,10 @@ extern "C" void bitz_outer_batch_release(void) {
 struct bitz_inner {
     int n = 0;
     size_t len = 0;
+    // Entries of the current tables that may be nonzero: from round 0 on,
+    // 1 + the last index where D or h is nonzero, halved (rounded up) by
+    // each fold; the fold rounds read nothing past it.
+    size_t live = 0;
     int round = 0;
     bool started = false;
     Dev in_d, in_w, P, Q, rtab;  // rtab: the current challenge's 4-bit table
@@ -2338,6 +2371,7 @@ extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, u
     h->n = (int)matrix_vars;
     const size_t N = (size_t)1 << matrix_vars;
     h->len = N;
+    h->live = N;
     if (flags & BITZ_DEVICE) {
         h->cur[0] = reinterpret_cast<const f128*>(matrix);
         h->cur[1] = reinterpret_cast<const f128*>(witness);
@@ -2375,14 +2409,23 @@ extern "C" int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* co) {
         memset(co, 0, 32);
         return BITZ_OK;
     }
+    // Round 0 reads both tables in full and records their live length in
+    // the spare slot n + 1 (slot k holds round k's sums, k < n): the fold
+    // rounds then skip the tail where D and h are both zero.
     const size_t pairs = h->len / 2;
+    auto* live_word = reinterpret_cast<unsigned long long*>(h->acc.u32() + 8 * (h->n + 1));
     inner_set_attributes();
     h->tm.begin(h->so.s, "inner_round0", 32.0 * (double)h->len);
     hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(inner_grid(pairs)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
-                       h->so.s, h->cur[0], h->cur[1], pairs, h->acc.u32());
+                       h->so.s, h->cur[0], h->cur[1], pairs, h->acc.u32(), live_word);
     h->tm.end(h->so.s);
     TRY(check_launch());
-    return inner_read(h, 0, co);
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(h->host.p) + 32, live_word, 8, hipMemcpyDeviceToHost, h->so.s));
+    TRY(inner_read(h, 0, co));
+    unsigned long long live;
+    memcpy(&live, static_cast<const char*>(h->host.p) + 32, 8);
+    h->live = std::max<size_t>(1, std::min<size_t>((size_t)live, h->len));
+    return BITZ_OK;
 }
 
 extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* co) {
@@ -2402,9 +2445,10 @@ extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* 
         // r's 4-bit table, then the fold with lane-table products.
         if (!h->rtab.p) TRY(h->rtab.alloc(kConstTable4Entries * sizeof(uint4)));
         hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, static_cast<uint4*>(h->rtab.p));
+        const size_t out_pairs = ((h->live + 1) / 2 + 1) / 2;  // over the live prefix
         h->tm.begin(s, "inner_fold_next", 48.0 * (double)h->len);
-        hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(inner_grid(next_len / 2)), dim3(bitz::kInnerBlock),
-                           bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], next_len / 2,
+        hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(inner_grid(out_pairs)), dim3(bitz::kInnerBlock),
+                           bitz::kInnerGridSmem, s, h->cur[0], h->cur[1], out[0], out[1], out_pairs, h->live,
                            static_cast<const uint4*>(h->rtab.p), h->acc.u32() + 8 * (h->round + 1));
         h->tm.end(s);
         has_next = true;
@@ -2414,6 +2458,7 @@ extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* 
     h->cur[1] = out[1];
     h->where = (h->where == 1) ? 2 : 1;
     h->len = next_len;
+    h->live = (h->live + 1) / 2;
     h->round += 1;
     if (has_next) return inner_read(h, h->round, co);
     memset(co, 0, 32);
@@ -2436,12 +2481,11 @@ extern "C" void bitz_inner_free(bitz_inner* h) {
     }
 }
 
-extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
-                                             const bitz_f128* matrix, uint32_t matrix_vars,
-                                             const bitz_f128* witness, uint32_t witness_vars,
-                                             uint32_t flags, bitz_f128* round_polys,
-                                             bitz_f128* eval_points, bitz_f128* final_claim,
-                                             bitz_f128* evals) {
+// The transcript-driven inner sumcheck through the per-round handle.
+static int32_t prove_inner_impl(bitz_transcript* t, bitz_f128 initial_claim, const bitz_f128* matrix,
+                                uint32_t matrix_vars, const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
+                                bitz_f128* round_polys, bitz_f128* eval_points, bitz_f128* final_claim,
+                                bitz_f128* evals) {
     if (!t || !final_claim || !evals || (matrix_vars && (!round_polys || !eval_points)))
         return fail(BITZ_ERR_ARG, "null pointer");
     if (witness_vars != matrix_vars)
@@ -2461,6 +2505,16 @@ extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initi
 done:
     bitz_inner_free(h);
     return rc;
+}
+
+extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
+                                             const bitz_f128* matrix, uint32_t matrix_vars,
+                                             const bitz_f128* witness, uint32_t witness_vars,
+                                             uint32_t flags, bitz_f128* round_polys,
+                                             bitz_f128* eval_points, bitz_f128* final_claim,
+                                             bitz_f128* evals) {
+    return prove_inner_impl(t, initial_claim, matrix, matrix_vars, witness, witness_vars, flags, round_polys,
+                            eval_points, final_claim, evals);
 }
 
 // Device buffers of the queued inner prover: the inputs, the ping-pong
@@ -2506,7 +2560,7 @@ static int32_t inner_queued_enqueue(const InnerQueued& B, int n, hipStream_t s, 
         if (k == 0) {
             if (Timed* t = timed(0)) t->begin(s, "inner_round0_lane", 32.0 * (double)len);
             hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(grid(len / 2)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
-                               s, cur[0], cur[1], len / 2, B.acc);
+                               s, cur[0], cur[1], len / 2, B.acc, nullptr);
         } else {
             f128* dst = (where == 1) ? B.Q : B.P;
             const size_t cap = (where == 1) ? B.capQ : B.capP;
@@ -2518,7 +2572,7 @@ static int32_t inner_queued_enqueue(const InnerQueued& B, int n, hipStream_t s, 
                     t->launches = (uint64_t)(kc - 1);
                 }
             hipLaunchKernelGGL(bitz::k_inner_fold_lane, dim3(grid(out_pairs)), dim3(bitz::kInnerBlock),
-                               bitz::kInnerGridSmem, s, cur[0], cur[1], out[0], out[1], out_pairs,
+                               bitz::kInnerGridSmem, s, cur[0], cur[1], out[0], out[1], out_pairs, len,
                                rt + (size_t)(k - 1) * kConstTable4Entries, B.acc + 8 * (size_t)k);
             cur[0] = out[0];
             cur[1] = out[1];
@@ -3150,8 +3204,8 @@ extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h,
         hv = reinterpret_cast<const bitz_f128*>(dh.p);
     }
     bitz_f128 evals[2];
-    TRY(bitz_prove_inner_sumcheck(t, B(inner_claim), reinterpret_cast<const bitz_f128*>(dD.p), col_vars, hv, col_vars,
-                                  BITZ_DEVICE, inner_rounds, inner_point, final_claim, evals));
+    TRY(prove_inner_impl(t, B(inner_claim), reinterpret_cast<const bitz_f128*>(dD.p), col_vars, hv, col_vars,
+                         BITZ_DEVICE, inner_rounds, inner_point, final_claim, evals));
     *matrix_eval = evals[0];  // ScaledMleEvaluationClaim (piop.rs:101-105): D(r_y) scales h(r_y)
     return BITZ_OK;
 }
@@ -3187,6 +3241,7 @@ struct PiopJob {
     const f128* icur[2] = {nullptr, nullptr};
     int iwhere = 0;
     size_t ilen = 0, icapP = 0, icapQ = 0;
+    size_t ilive = 1;  // entries of the inner tables that may be nonzero (round 0 measures it)
     f128* pend[3] = {nullptr, nullptr, nullptr};  // outputs of the fold launched this step
     std::vector<bitz_f128> outer_rounds, rx, inner_rounds, inner_point;
     bitz_f128 outer_claims[3] = {}, matrix_eval{}, final_claim{};
@@ -3399,6 +3454,8 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
             d.d = j->dD.f();
             d.w = j->hv;
             d.pairs = j->ilen / 2;
+            d.live = j->ilen;
+            d.live_out = reinterpret_cast<unsigned long long*>(slot(i) + 8);  // the tables' live length
             d.acc = slot(i);
             d.blocks = inner_grid(d.pairs);
             d.block_base = (uint32_t)i_r0_map.size();
@@ -3415,7 +3472,8 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
             d.w = j->icur[1];
             d.dout = j->pend[0];
             d.wout = j->pend[1];
-            d.pairs = next_len / 2;
+            d.pairs = std::min(next_len / 2, ((j->ilive + 1) / 2 + 1) / 2);  // the folded live prefix, in pairs
+            d.live = j->ilive;
             d.rtab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
             d.acc = slot(i);
             d.blocks = inner_grid(d.pairs);
@@ -3578,6 +3636,7 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
                 j->dhi.release();
                 const size_t N = (size_t)1 << j->nc;
                 j->ilen = N;
+                j->ilive = N;
                 j->icapP = std::max<size_t>(N / 2, 1);
                 j->icapQ = std::max<size_t>(N / 4, 1);
                 if ((j->status = j->dD.alloc(N * sizeof(f128))) || (j->status = j->iP.alloc(2 * j->icapP * sizeof(f128))) ||
@@ -3599,6 +3658,13 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
                     j->icur[1] = j->pend[1];
                     j->iwhere = (j->iwhere == 1) ? 2 : 1;
                     j->ilen /= 2;
+                    j->ilive = (j->ilive + 1) / 2;
+                } else {
+                    // Round 0's live length (1 + the last index where D or h is
+                    // nonzero): the fold rounds skip the zero tail.
+                    unsigned long long live;
+                    memcpy(&live, w + 8, 8);
+                    j->ilive = std::max<size_t>(1, std::min<size_t>((size_t)live, j->ilen));
                 }
                 bitz_f128 co[2];
                 memcpy(co, w, 32);

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gf128.hpp"
+#include "keccak.hpp"
 #include "kernels.hpp"
 
 namespace bitz {
@@ -504,6 +505,10 @@ __global__ __launch_bounds__(kBlock) void k_pg_expand(const f128* rf, int t, con
         const unsigned q = (unsigned)(word >> ((2 * p) & 63)) & 3u;  // f0 | f1 << 1
         g_store(V + j, q == 0 ? g_zero() : q == 1 ? rr1 : q == 2 ? rr : one);
     }
+}
+
+__global__ void k_keccak_hash(const uint8_t* msg, size_t n, int rate, uint8_t pad, uint8_t* out, size_t out_len) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) keccak_hash(msg, n, rate, pad, out, out_len);
 }
 
 __global__ void k_zero_words(uint32_t* p, size_t n) {
@@ -1114,35 +1119,49 @@ __global__ __launch_bounds__(kFoldBlock) void k_batch_fold(const RoundDesc* d, c
 // ======================== inner sumcheck with challenges known in advance ===
 // [c0, c2] of round k = [sum m0 w0, sum (m1+m0)(w1+w0)] over the round's pairs
 // (sumcheck.rs:568-609), products through lane tables.
+// With live_out, also the tables' live length: 1 + the last index where D or
+// h is nonzero (an atomic max), so later rounds can skip a zero tail.
 __device__ __forceinline__ void inner_r0_body(const f128* d, const f128* w, size_t pairs, uint32_t* acc,
-                                              size_t block, size_t blocks) {
+                                              unsigned long long* live_out, size_t block, size_t blocks) {
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     g4 s0 = g_zero(), s2 = g_zero();
+    unsigned long long live = 0;
+    auto nonzero = [](const g4& a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) != 0; };
 #pragma unroll 1
     for (size_t p = block * kInnerBlock + threadIdx.x; p < pairs; p += blocks * kInnerBlock) {
         const g4 m0 = g_load_hbm(d + 2 * p), m1 = g_load_hbm(d + 2 * p + 1);
         const g4 w0 = g_load_hbm(w + 2 * p), w1 = g_load_hbm(w + 2 * p + 1);
+        if (nonzero(m1) || nonzero(w1))
+            live = 2 * p + 2;
+        else if (nonzero(m0) || nonzero(w0))
+            live = 2 * p + 1;
         s0 = g_add(s0, g_mul_lane(m0, w0, col));
         __builtin_amdgcn_sched_barrier(0);  // one lane product in flight at a time
         s2 = g_add(s2, g_mul_lane(g_add(m1, m0), g_add(w1, w0), col));
     }
     g4 v[2] = {s0, s2};
     block_xor_to_global<2>(v, acc);
+    if (live_out) {
+        for (int off = 32; off >= 1; off >>= 1) live = max(live, (unsigned long long)__shfl_xor(live, off, 64));
+        if ((threadIdx.x & 63) == 0 && live) atomicMax(live_out, live);
+    }
 }
 
 __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs,
-                                                              uint32_t* acc) {
-    inner_r0_body(d, w, pairs, acc, blockIdx.x, gridDim.x);
+                                                              uint32_t* acc, unsigned long long* live_out) {
+    inner_r0_body(d, w, pairs, acc, live_out, blockIdx.x, gridDim.x);
 }
 
 // Fold by r_{k-1} on load (4 constant-table products per output pair) and
 // round k's sums (fold_and_compute_next_inner_round_coefficients_without_linear,
 // sumcheck.rs:641-688).
+// Inputs at index live and above are zero (not read): a table whose tail is
+// known to be zero is folded over its live prefix only.
 __device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f128* dout, f128* wout,
-                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc, size_t block,
-                                                size_t blocks) {
+                                                size_t out_pairs, size_t live, const uint4* rtab4, uint32_t* acc,
+                                                size_t block, size_t blocks) {
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
@@ -1154,10 +1173,19 @@ __device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f1
 #pragma unroll 1
     for (size_t p = block * kInnerBlock + threadIdx.x; p < out_pairs; p += blocks * kInnerBlock) {
         g4 in[2][4];
+        if (4 * p + 3 < live) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            in[0][j] = g_load_hbm(d + 4 * p + j);
-            in[1][j] = g_load_hbm(w + 4 * p + j);
+            for (int j = 0; j < 4; ++j) {
+                in[0][j] = g_load_hbm(d + 4 * p + j);
+                in[1][j] = g_load_hbm(w + 4 * p + j);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool in_live = 4 * p + j < live;
+                in[0][j] = in_live ? g_load_hbm(d + 4 * p + j) : g_zero();
+                in[1][j] = in_live ? g_load_hbm(w + 4 * p + j) : g_zero();
+            }
         }
         const g4 m0 = ip(in[0][0], in[0][1]), m1 = ip(in[0][2], in[0][3]);
         const g4 w0 = ip(in[1][0], in[1][1]), w1 = ip(in[1][2], in[1][3]);
@@ -1174,19 +1202,20 @@ __device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f1
 }
 
 __global__ __launch_bounds__(kInnerBlock) void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout,
-                                                                size_t out_pairs, const uint4* rtab4, uint32_t* acc) {
-    inner_fold_body(d, w, dout, wout, out_pairs, rtab4, acc, blockIdx.x, gridDim.x);
+                                                                size_t out_pairs, size_t live, const uint4* rtab4,
+                                                                uint32_t* acc) {
+    inner_fold_body(d, w, dout, wout, out_pairs, live, rtab4, acc, blockIdx.x, gridDim.x);
 }
 
 // The same rounds of many inner sumchecks in one launch: workgroup -> round
 // descriptor through bmap, each descriptor's workgroups striding its pairs.
 __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
     const InnerRoundDesc A = d[bmap[blockIdx.x]];
-    inner_r0_body(A.d, A.w, A.pairs, A.acc, blockIdx.x - A.block_base, A.blocks);
+    inner_r0_body(A.d, A.w, A.pairs, A.acc, A.live_out, blockIdx.x - A.block_base, A.blocks);
 }
 __global__ __launch_bounds__(kInnerBlock) BITZ_FOUR_WAVES void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
     const InnerRoundDesc A = d[bmap[blockIdx.x]];
-    inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks);
+    inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.live, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks);
 }
 
 // Rounds k0 .. n-1 in one workgroup: the two tables (folded by r_{k0-1} on

@@ -70,6 +70,8 @@ __global__ void k_pg_first(const f128* rows, int t, const f128* cols, const f128
                            uint32_t* acc);
 __global__ void k_pg_expand(const f128* rf, int t, const f128* cols, const f128* packed, f128 r, f128* W, f128* V,
                             size_t n);
+// keccak_hash (keccak.hpp) by one device thread: the device permutation's check.
+__global__ void k_keccak_hash(const uint8_t* msg, size_t n, int rate, uint8_t pad, uint8_t* out, size_t out_len);
 // p[0 .. n) = 0 (a kernel, where a memset node would not do: captured graphs).
 __global__ void k_zero_words(uint32_t* p, size_t n);
 
@@ -211,7 +213,8 @@ __global__ void k_batch_prologue(const BatchPrologue* items, const uint32_t* bma
 // then the last rounds in one workgroup (k_inner_tail).
 constexpr int kInnerBlock = 256;
 constexpr size_t kInnerGridSmem = sizeof(uint4) * (kConstTable4Entries + 4 * 64 * kLaneTableEntries);
-__global__ void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs, uint32_t* acc);
+__global__ void k_inner_r0_lane(const f128* d, const f128* w, size_t pairs, uint32_t* acc,
+                                unsigned long long* live_out);
 // One inner sumcheck's round in a batched launch (k_inner_r0_batch: pairs of
 // round 0; k_inner_fold_batch: output pairs, folding d, w by rtab4's challenge).
 struct InnerRoundDesc {
@@ -220,6 +223,8 @@ struct InnerRoundDesc {
     f128* dout;
     f128* wout;
     size_t pairs;
+    size_t live;         // fold rounds: input entries below live (the rest is zero)
+    unsigned long long* live_out;  // round 0: atomic max of 1 + the last nonzero index of D or h (or null)
     const uint4* rtab4;
     uint32_t* acc;
     uint32_t blocks, block_base;
@@ -227,7 +232,7 @@ struct InnerRoundDesc {
 __global__ void k_inner_r0_batch(const InnerRoundDesc* d, const uint32_t* bmap);
 __global__ void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap);
 __global__ void k_inner_fold_lane(const f128* d, const f128* w, f128* dout, f128* wout, size_t out_pairs,
-                                  const uint4* rtab4, uint32_t* acc);
+                                  size_t live, const uint4* rtab4, uint32_t* acc);
 constexpr int kInnerTailMaxLog = 10;  // at most 2^10 entries per table in LDS
 // Rounds [k0, k0 + s) on chunks of 2^lg_in inputs (folded by r_{k0-1} on
 // load when fold_first), chunk g = workgroup g at inputs [g 2^lg_in, ...).

@@ -176,6 +176,9 @@ void Transcript::absorb(int kind, const uint8_t* bytes, size_t len) {
             break;
         case Type::kSeeded:
             break;
+        case Type::kKeccak:
+            keccak_message(keccak, kind, bytes, len);
+            break;
         case Type::kDuplex: {
             std::vector<uint8_t> buf(32 + 2 + 8 + len);
             memcpy(buf.data(), state, 32);
@@ -200,6 +203,12 @@ f128 Transcript::squeeze() {
             const uint64_t w0 = splitmix_next(seed), w1 = splitmix_next(seed);
             memcpy(out, &w0, 8);
             memcpy(out + 8, &w1, 8);
+            break;
+        }
+        case Type::kKeccak: {
+            uint64_t w[2];
+            keccak_squeeze(keccak, w);
+            memcpy(out, w, 16);
             break;
         }
         case Type::kDuplex: {

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "gf128.hpp"
+#include "keccak.hpp"
 
 namespace bitz {
 
@@ -35,7 +36,7 @@ uint32_t grinding_bits(uint32_t security_bits, uint64_t coefficient);
 constexpr uint32_t kMaxGrindingBits = 32;
 
 struct Transcript {
-    enum class Type { kCallbacks, kSeeded, kDuplex } type;
+    enum class Type { kCallbacks, kSeeded, kDuplex, kKeccak } type;
     void* cb_ctx = nullptr;
     void (*cb_absorb)(void*, int32_t, const uint8_t*, size_t) = nullptr;
     void (*cb_squeeze)(void*, uint8_t*) = nullptr;
@@ -44,6 +45,7 @@ struct Transcript {
     void (*cb_grind)(void*, const uint8_t*, size_t, uint32_t) = nullptr;
     uint64_t seed = 0;
     uint8_t state[32] = {0};
+    KeccakState keccak{};  // kKeccak: the duplex state (keccak.hpp)
     std::vector<uint8_t> narg;
     // The hint stream (ProverState::hint, crates/transcript/src/prover.rs:64-68):
     // prover data beside the narg string that the sponge never sees.

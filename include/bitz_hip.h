@@ -109,6 +109,19 @@ bitz_transcript* bitz_transcript_seeded(uint64_t seed);
 /* SHA-256 ratchet over every message: state = SHA256(state||'A'||kind||le64(len)||m),
  * challenge = SHA256(state||'S')[0..16], state = SHA256(state||'R'). */
 bitz_transcript* bitz_transcript_duplex(const uint8_t* domain, size_t len);
+/* Keccak duplex over Keccak-f[1600] (FIPS 202), rate 136 bytes: start
+ * 'D' || le64(len) || domain; message ('P' public | 'N' prover) ||
+ * le64(len) || bytes; squeeze: 'S', pad 0x1F at the position and 0x80 at
+ * byte 135, permute, the state's first 16 bytes.  Shaped like spongefish's
+ * duplex sponge over Keccak (crates/transcript/src/domain.rs:13-42), whose
+ * own framing is not reproduced; the same code runs on the device, so
+ * provers can draw these challenges there (BITZ_DEVICE_TRANSCRIPT). */
+bitz_transcript* bitz_transcript_keccak(const uint8_t* domain, size_t len);
+/* One-message Keccak sponge (pad 0x06: SHA3-*, 0x1F: SHAKE*) on the host,
+ * or with BITZ_DEVICE by one device thread: the permutation's check against
+ * FIPS 202 answers. */
+int32_t bitz_keccak_hash(const uint8_t* msg, size_t len, int32_t rate, uint8_t pad, uint8_t* out, size_t out_len,
+                         uint32_t flags);
 void bitz_transcript_free(bitz_transcript* t);
 int32_t bitz_transcript_absorb(bitz_transcript* t, int32_t kind, const uint8_t* bytes, size_t len);
 int32_t bitz_transcript_squeeze(bitz_transcript* t, bitz_f128* out);

@@ -85,6 +85,24 @@ void oracle_duplex_free(oracle_duplex* d);
 oracle_transcript oracle_duplex_transcript(oracle_duplex* d);
 void oracle_sha256(const uint8_t* msg, size_t len, uint8_t out[32]);
 
+/* Keccak-f[1600] (FIPS 202), a one-message sponge for the known answers,
+ * and the Keccak duplex transcript (rate 136 bytes): start 'D' || le64(len)
+ * || domain; message ('P' public | 'N' narg) || le64(len) || bytes; squeeze
+ * 'S', pad 0x1F at the position and 0x80 at byte 135, permute, output the
+ * first 16 bytes.  A stand-in shaped like spongefish's duplex; it is not
+ * spongefish. */
+void oracle_keccak_f1600(uint64_t a[25]);
+void oracle_keccak_hash(const uint8_t* msg, size_t n, int rate, uint8_t pad, uint8_t* out, size_t out_len);
+typedef struct {
+    uint64_t a[25];
+    size_t pos;
+    uint8_t* narg;
+    size_t narg_len, narg_cap;
+} oracle_keccak_duplex;
+void oracle_keccak_duplex_init(oracle_keccak_duplex* d, const uint8_t* domain, size_t len);
+void oracle_keccak_duplex_free(oracle_keccak_duplex* d);
+oracle_transcript oracle_keccak_duplex_transcript(oracle_keccak_duplex* d);
+
 /* ---- poly: crates/poly/src/{eq.rs,mle.rs,f128.rs} ------------------------ */
 void oracle_eq_table(const of128* r, int k, of128* out);        /* eq.rs:63-99 */
 of128 oracle_eq_eval(const of128* x, const of128* y, int n);    /* eq.rs:22-41 */

@@ -40,6 +40,11 @@ class Seeded(C.Structure):
                 ("absorbed_bytes", C.c_uint64)]
 
 
+class KeccakDuplex(C.Structure):
+    _fields_ = [("a", C.c_uint64 * 25), ("pos", C.c_size_t), ("narg", C.POINTER(C.c_uint8)),
+                ("narg_len", C.c_size_t), ("narg_cap", C.c_size_t)]
+
+
 class Duplex(C.Structure):
     _fields_ = [("state", C.c_uint8 * 32), ("narg", C.POINTER(C.c_uint8)),
                 ("narg_len", C.c_size_t), ("narg_cap", C.c_size_t)]
@@ -122,6 +127,12 @@ def lib():
         L.oracle_gkr_prove.restype = OF128
         L.oracle_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, P, C.c_uint32]
         L.oracle_transcript_grind.argtypes = [P, P, C.c_size_t, C.c_uint32]
+        L.oracle_keccak_f1600.argtypes = [P]
+        L.oracle_keccak_hash.argtypes = [P, C.c_size_t, C.c_int, C.c_uint8, P, C.c_size_t]
+        L.oracle_keccak_duplex_init.argtypes = [P, P, C.c_size_t]
+        L.oracle_keccak_duplex_free.argtypes = [P]
+        L.oracle_keccak_duplex_transcript.restype = Transcript
+        L.oracle_keccak_duplex_transcript.argtypes = [P]
         L.oracle_post_gkr_prove.restype = C.c_int
         L.oracle_post_gkr_prove.argtypes = [P, P, C.c_int, P, C.c_int, OF128, P, C.c_size_t, C.c_uint32, P, P]
         L.oracle_post_gkr_verify.restype = C.c_int
@@ -233,6 +244,42 @@ class DuplexTranscript:
 
     def narg(self) -> bytes:
         return C.string_at(self._d.narg, self._d.narg_len) if self._d.narg_len else b""
+
+
+class KeccakTranscript:
+    """The Keccak duplex (rate 136 bytes; framing in oracle/bitz_oracle.h)."""
+
+    def __init__(self, domain: bytes):
+        self._d = KeccakDuplex()
+        buf = C.create_string_buffer(domain, max(len(domain), 1))
+        lib().oracle_keccak_duplex_init(C.byref(self._d), buf, len(domain))
+        self.t = lib().oracle_keccak_duplex_transcript(C.byref(self._d))
+
+    def __del__(self):
+        try:
+            lib().oracle_keccak_duplex_free(C.byref(self._d))
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return C.byref(self.t)
+
+    def narg(self) -> bytes:
+        return C.string_at(self._d.narg, self._d.narg_len) if self._d.narg_len else b""
+
+
+def keccak_hash(msg: bytes, rate: int, pad: int, out_len: int) -> bytes:
+    out = (C.c_uint8 * max(out_len, 1))()
+    buf = C.create_string_buffer(msg, max(len(msg), 1))
+    lib().oracle_keccak_hash(buf, len(msg), rate, pad, out, out_len)
+    return bytes(out[:out_len])
+
+
+def keccak_f1600(lanes) -> list:
+    a = (C.c_uint64 * 25)(*lanes)
+    lib().oracle_keccak_f1600(a)
+    return list(a)
 
 
 class ListTranscript:

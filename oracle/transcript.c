@@ -280,3 +280,125 @@ void oracle_transcript_grind(oracle_transcript* t, const uint8_t* label, size_t 
     for (int i = 0; i < 8; ++i) le[i] = (uint8_t)(nonce >> (8 * i));
     t->absorb(t->ctx, 1, le, 8);
 }
+
+/* ---- Keccak-f[1600] and the Keccak duplex ---------------------------------
+ * FIPS 202 section 3: the round constants from the LFSR rc(t) (x^8 + x^6 +
+ * x^5 + x^4 + 1), RC_i bit 2^j - 1 = rc(j + 7 i); the rho offsets from the
+ * walk (x, y) = (1, 0), offset (t + 1)(t + 2) / 2, (x, y) <- (y, 2x + 3y). */
+static uint64_t K_RC[24];
+static int K_RHO[25];
+static int k_ready = 0;
+static int k_rc_bit(int t) {
+    if (t % 255 == 0) return 1;
+    unsigned r = 1; /* R[i] at bit i; R = 10000000 */
+    for (int i = 1; i <= t % 255; ++i) {
+        r <<= 1; /* R = 0 || R; then R[0], R[4], R[5], R[6] ^= R[8]; Trunc8 */
+        if (r & 0x100) r ^= 0x171;
+    }
+    return r & 1;
+}
+static void k_init(void) {
+    if (k_ready) return;
+    for (int i = 0; i < 24; ++i) {
+        uint64_t v = 0;
+        for (int j = 0; j <= 6; ++j)
+            if (k_rc_bit(j + 7 * i)) v |= (uint64_t)1 << ((1 << j) - 1);
+        K_RC[i] = v;
+    }
+    K_RHO[0] = 0;
+    int x = 1, y = 0;
+    for (int t = 0; t < 24; ++t) {
+        K_RHO[x + 5 * y] = ((t + 1) * (t + 2) / 2) % 64;
+        int nx = y, ny = (2 * x + 3 * y) % 5;
+        x = nx;
+        y = ny;
+    }
+    k_ready = 1;
+}
+static uint64_t k_rot(uint64_t v, int n) { return n ? (v << n) | (v >> (64 - n)) : v; }
+void oracle_keccak_f1600(uint64_t a[25]) {
+    k_init();
+    for (int round = 0; round < 24; ++round) {
+        uint64_t c[5], d[5], b[25];
+        for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
+        for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ k_rot(c[(x + 1) % 5], 1);
+        for (int i = 0; i < 25; ++i) a[i] ^= d[i % 5];
+        for (int x = 0; x < 5; ++x)
+            for (int y = 0; y < 5; ++y) b[y + 5 * ((2 * x + 3 * y) % 5)] = k_rot(a[x + 5 * y], K_RHO[x + 5 * y]);
+        for (int i = 0; i < 25; ++i) {
+            int x = i % 5, y = i / 5;
+            a[i] = b[i] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]);
+        }
+        a[0] ^= K_RC[round];
+    }
+}
+/* A one-message sponge (pad 0x06: SHA3, 0x1F: SHAKE), for the known answers. */
+void oracle_keccak_hash(const uint8_t* msg, size_t n, int rate, uint8_t pad, uint8_t* out, size_t out_len) {
+    uint64_t a[25];
+    memset(a, 0, sizeof a);
+    uint8_t* s = (uint8_t*)a; /* little-endian lanes */
+    size_t pos = 0;
+    for (size_t i = 0; i < n; ++i) {
+        s[pos] ^= msg[i];
+        if (++pos == (size_t)rate) {
+            oracle_keccak_f1600(a);
+            pos = 0;
+        }
+    }
+    s[pos] ^= pad;
+    s[rate - 1] ^= 0x80;
+    oracle_keccak_f1600(a);
+    size_t o = 0;
+    while (o < out_len) {
+        for (int i = 0; i < rate && o < out_len; ++i, ++o) out[o] = s[i];
+        if (o < out_len) oracle_keccak_f1600(a);
+    }
+}
+/* The Keccak duplex transcript (rate 136): start 'D' || le64(len) || domain;
+ * messages ('P' | 'N') || le64(len) || m; squeeze 'S', pad 0x1F .. 0x80,
+ * permute, the first 16 bytes. */
+static void kd_bytes(oracle_keccak_duplex* d, const uint8_t* p, size_t n) {
+    uint8_t* s = (uint8_t*)d->a;
+    for (size_t i = 0; i < n; ++i) {
+        s[d->pos] ^= p[i];
+        if (++d->pos == 136) {
+            oracle_keccak_f1600(d->a);
+            d->pos = 0;
+        }
+    }
+}
+static void kd_framed(oracle_keccak_duplex* d, uint8_t tag, const uint8_t* p, size_t n) {
+    uint8_t head[9];
+    head[0] = tag;
+    for (int i = 0; i < 8; ++i) head[1 + i] = (uint8_t)((uint64_t)n >> (8 * i));
+    kd_bytes(d, head, 9);
+    kd_bytes(d, p, n);
+}
+static void kd_absorb(void* ctx, int kind, const uint8_t* bytes, size_t len) {
+    oracle_keccak_duplex* d = (oracle_keccak_duplex*)ctx;
+    kd_framed(d, kind == 0 ? 'P' : 'N', bytes, len);
+    if (kind == 1) narg_append(&d->narg, &d->narg_len, &d->narg_cap, bytes, len);
+}
+static void kd_squeeze(void* ctx, uint8_t out[16]) {
+    oracle_keccak_duplex* d = (oracle_keccak_duplex*)ctx;
+    const uint8_t tag = 'S';
+    kd_bytes(d, &tag, 1);
+    uint8_t* s = (uint8_t*)d->a;
+    s[d->pos] ^= 0x1F;
+    s[135] ^= 0x80;
+    oracle_keccak_f1600(d->a);
+    d->pos = 0;
+    memcpy(out, s, 16);
+}
+void oracle_keccak_duplex_init(oracle_keccak_duplex* d, const uint8_t* domain, size_t len) {
+    memset(d, 0, sizeof *d);
+    kd_framed(d, 'D', domain, len);
+}
+void oracle_keccak_duplex_free(oracle_keccak_duplex* d) {
+    free(d->narg);
+    d->narg = NULL;
+}
+oracle_transcript oracle_keccak_duplex_transcript(oracle_keccak_duplex* d) {
+    oracle_transcript t = {d, kd_absorb, kd_squeeze, NULL};
+    return t;
+}

@@ -71,14 +71,24 @@ def test_assignment_mle_on_device():
     assert np.array_equal(out[:5000, 0], bits.astype(np.uint64)) and not out[5000:].any() and not out[:, 1].any()
 
 
-def satisfied_r1cs(rng, rows, row_vars, col_vars, per_row):
-    """A, B random; a Boolean h with h[0] = 1; C with one entry per row at
-    the constant column, valued (A h)_i (B h)_i, so A h * B h = C h."""
+def prefix(cols, used):
+    """The length of a nonzero prefix: all columns, or (used < 1) an odd
+    count, so the prefix ends inside a pair."""
+    return cols if used >= 1 else min(cols - 1, max(3, int(cols * used) | 1))
+
+
+def satisfied_r1cs(rng, rows, row_vars, col_vars, per_row, used=1.0, h_used=1.0):
+    """A, B random over the first `used` fraction of the columns; a Boolean h
+    with h[0] = 1, zero past its first `h_used` fraction; C with one entry per
+    row at the constant column, valued (A h)_i (B h)_i, so A h * B h = C h."""
     cols = 1 << col_vars
-    a = random_csr(rng, rows, cols, per_row, 80)
-    b = random_csr(rng, rows, cols, per_row, 81)
+    span = prefix(cols, used)
+    a = random_csr(rng, rows, span, per_row, 80)
+    b = random_csr(rng, rows, span, per_row, 81)
     bits = rng.integers(0, 2, size=cols).astype(np.uint8)
+    bits[prefix(cols, h_used):] = 0
     bits[0] = 1
+    bits[prefix(cols, h_used) - 1] = 1
     h = np.zeros((cols, 2), np.uint64)
     h[:, 0] = bits
     ah, bh = multiply(a, h, rows, rows), multiply(b, h, rows, rows)
@@ -87,11 +97,17 @@ def satisfied_r1cs(rng, rows, row_vars, col_vars, per_row):
     return [a, b, c], bits, h
 
 
-@pytest.mark.parametrize("rows,row_vars,col_vars,per_row,dev", [(13, 4, 6, 3, False), (300, 9, 11, 6, True),
-                                                                (1000, 10, 13, 8, True)])
-def test_spartan_piop_matches_oracle(rows, row_vars, col_vars, per_row, dev):
+@pytest.mark.parametrize("rows,row_vars,col_vars,per_row,dev,used,h_used",
+                         [(13, 4, 6, 3, False, 1.0, 1.0), (300, 9, 11, 6, True, 1.0, 1.0),
+                          (1000, 10, 13, 8, True, 1.0, 1.0), (700, 10, 12, 5, True, 0.6, 1.0),
+                          (64, 6, 9, 4, False, 0.3, 0.5), (200, 8, 10, 4, True, 0.01, 0.004),
+                          (500, 9, 13, 5, True, 0.2, 0.7)])
+def test_spartan_piop_matches_oracle(rows, row_vars, col_vars, per_row, dev, used, h_used):
+    """used, h_used < 1: the last columns of the matrices (so of D) or of h
+    are zero, so the inner sumcheck's folds skip a zero tail that round 0
+    measures (D longer than h, h longer than D)."""
     rng = np.random.default_rng(rows + col_vars)
-    csr, bits, h = satisfied_r1cs(rng, rows, row_vars, col_vars, per_row)
+    csr, bits, h = satisfied_r1cs(rng, rows, row_vars, col_vars, per_row, used, h_used)
     m_csr = random_csr(rng, 7, 1 << col_vars, 3, 90)[:2]
     digest = G.constraint_matrix_digest(m_csr, 1 << col_vars, rows, 1 << col_vars, csr)
     prepared = G.PreparedConstraintMatrices(rows, row_vars, col_vars, csr)
