@@ -112,6 +112,9 @@ def main() -> int:
     ap.add_argument("--capacity-only", action="store_true",
                     help="serve: only the timed capacity steps (no latency curve, slots or plain runs; "
                          "for kernel profiles of the batched mode)")
+    ap.add_argument("--engine-stats", action="store_true",
+                    help="serve: also time every engine step (device span by HIP events, host launch and "
+                         "finish, wait) and report per-step averages")
     ap.add_argument("--chain-blocks", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-engines", type=int, default=0, help="serve: PIOP engines (0: one per 8 slots)")
@@ -434,6 +437,22 @@ def bench_gkr(G, args) -> int:
         _, wit = G.grand_product(dev, s)
         G.gpgkr_prove(G.Transcript.seeded(args.seed + 2000 + i), point, wit)
         tt.append(time.perf_counter() - t0)
+
+    def keccak_p50(device: bool) -> float:
+        """A Keccak transcript: challenges drawn on the device, or by the
+        host transcript after each round's read-back."""
+        G.set_device_fiat_shamir(device)
+        ts = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            _, wit = G.grand_product(dev, s)
+            G.gpgkr_prove(G.Transcript.keccak(b"bench/gkr/%d" % i), point, wit)
+            ts.append(time.perf_counter() - t0)
+        G.set_device_fiat_shamir(True)
+        return float(np.percentile(np.array(ts[1:]) * 1e3, 50))
+
+    keccak_device, keccak_host = keccak_p50(True), keccak_p50(False)
+    log(f"bench: Keccak transcript: {keccak_device:.2f} ms with device Fiat-Shamir, {keccak_host:.2f} ms through the host")
     achieved = bind["bytes"] / (bind["ms"] * 1e-3) / 1e9 if bind["ms"] else 0.0
     children = sum(1 << (n + 1) for n in range(s, L))
     line = {
@@ -455,6 +474,7 @@ def bench_gkr(G, args) -> int:
                                "seeded challenges queued", "log_leaves": L, "log_groups": s},
         "leaves_per_s": (1 << L) * args.steps / wall,
         "transcript_mode_p50_ms": float(np.percentile(np.array(tt) * 1e3, 50)),
+        "keccak_transcript_p50_ms": {"device_fiat_shamir": keccak_device, "host_round_trips": keccak_host},
         "algorithmic_gb_per_s": 92.0 * children * args.steps / wall / 1e9,
         "roofline": {
             "bound": "hbm",
@@ -600,7 +620,7 @@ def bench_serve(G, args) -> int:
     # on an engine thread.
     events = "ROCPROFILER_LIBRARY_CTOR" not in os.environ
     G.profile_enable(events)
-    G.profile_only("inner_fold_batch")
+    G.profile_only(None if args.engine_stats else "inner_fold_batch")
     step_ms = []
     t_start = time.perf_counter()
     for rs in step_reqs:
@@ -610,6 +630,13 @@ def bench_serve(G, args) -> int:
     G.synchronize()
     wall = time.perf_counter() - t_start
     fold = G.profile_read("inner_fold_batch")
+    engine = None
+    if args.engine_stats:
+        st = {k: G.profile_read("piop_engine_" + k) for k in ("step", "host_launch", "wait", "host_finish")}
+        n = max(st["step"]["launches"], 1)
+        engine = {"steps": st["step"]["launches"], "proofs_per_step": st["step"]["bytes"] / n,
+                  **{k + "_us_per_step": v["ms"] * 1e3 / n for k, v in st.items()},
+                  "engine_streams": engines, "wall_us_per_step_per_engine": wall * 1e6 * engines / n}
     G.profile_only(None)
     G.profile_enable(False)
     queued_lat = np.array([r.latency for rs in step_reqs for r in rs]) * 1e3
@@ -635,7 +662,7 @@ def bench_serve(G, args) -> int:
     if args.capacity_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "SHA-256 blocks/s", "steps": args.steps,
                           "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
-                          "inner_fold_batch": fold}), flush=True)
+                          "inner_fold_batch": fold, "engine": engine}), flush=True)
         return 0
     rates = sorted({float(x) for x in args.serve_rates.split(",") if x} | {args.serve_rate})
     curve = [poisson("batched", rate, args.seed + 100 + k) for k, rate in enumerate(rates)]

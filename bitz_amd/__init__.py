@@ -162,6 +162,7 @@ def lib():
             "bitz_gkr_prove": (I32, [P, P, P, U32, U32, P, P]),
             "bitz_transcript_set_grind": (I32, [P, P]),
             "bitz_transcript_keccak": (P, [P, SZ]),
+            "bitz_set_device_fiat_shamir": (I32, [I32]),
             "bitz_keccak_hash": (I32, [P, SZ, I32, C.c_uint8, P, SZ, U32]),
             "bitz_post_gkr_prove": (I32, [P, P, U32, P, U32, F128, P, SZ, U32, U32, P, P]),
             "bitz_transcript_hint": (I32, [P, P, SZ]),
@@ -1163,6 +1164,12 @@ def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
                                                 _p(evals), _p(leaf_point), C.byref(claim)))
     return (msgs[: 2 * rounds].reshape(rounds, 2, 2), evals[: 2 * layers].reshape(layers, 2, 2), leaf_point,
             to_int(claim))
+
+
+def set_device_fiat_shamir(on: bool) -> None:
+    """Whether provers draw a Keccak transcript's challenges on the device
+    (the default) or through the host transcript, round by round."""
+    _check(lib().bitz_set_device_fiat_shamir(1 if on else 0))
 
 
 def keccak_hash(msg: bytes, rate: int, pad: int, out_len: int, device: bool = False) -> bytes:

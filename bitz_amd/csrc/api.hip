@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <map>
@@ -268,6 +269,17 @@ struct Profiler {
     std::mutex mu;
     std::map<std::string, KernelStat> stats;
 } g_prof;
+
+// Adds host time to a profiling entry (the engine's host phases).
+static void prof_add_host(const char* name, std::chrono::steady_clock::time_point since) {
+    if (!g_prof.on) return;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - since).count();
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    if (!g_prof.only.empty() && g_prof.only != name) return;
+    KernelStat& k = g_prof.stats[name];
+    k.ms += ms;
+    k.launches += 1;
+}
 
 // Events come from a pool so that a profiled proof creates none.
 struct EventPool {
@@ -3267,9 +3279,9 @@ struct bitz_piop_engine {
     bool running = false, stop = false;
     int32_t worker_rc = BITZ_OK;
     std::string worker_err;
-    // HIP events around the step's batched inner fold (when profiling), read
-    // once the step has synchronised.
-    Timed t_inner_fold;
+    // HIP events around the step's batched inner fold and around the whole
+    // step's device work (when profiling), read once the step has synchronised.
+    Timed t_inner_fold, t_step;
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -3378,6 +3390,7 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     if (e->running && std::this_thread::get_id() != e->worker.get_id())
         return fail(BITZ_ERR_STATE, "the engine steps in the background");
     if (!e->launched.empty()) return fail(BITZ_ERR_STATE, "a step is already in flight");
+    const auto host_t0 = std::chrono::steady_clock::now();
     hipStream_t s = e->so.s;
     std::vector<PiopJob*> act;
     for (auto& j : e->jobs)
@@ -3525,6 +3538,7 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     memcpy(pin + irm_o, i_r0_map.data(), i_r0_map.size() * 4);
     memcpy(pin + if_o, i_fold.data(), i_fold.size() * sizeof(bitz::InnerRoundDesc));
     memcpy(pin + ifm_o, i_fold_map.data(), i_fold_map.size() * 4);
+    e->t_step.begin(s, "piop_engine_step", (double)act.size());  // "bytes": proofs in the step
     HIP_TRY(hipMemcpyAsync(dst, pin, off, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(e->slots.p, 0, slot_bytes, s));
     if (!consts.empty())
@@ -3587,7 +3601,9 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     }
     TRY(check_launch());
     HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_bytes, hipMemcpyDeviceToHost, s));
+    e->t_step.end(s);
     e->launched = std::move(act);
+    prof_add_host("piop_engine_host_launch", host_t0);
     return BITZ_OK;
 }
 
@@ -3596,8 +3612,12 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
     constexpr size_t kSlotWords = 16;
     std::vector<PiopJob*> act = std::move(e->launched);
     e->launched.clear();
+    const auto wait_t0 = std::chrono::steady_clock::now();
     if (!act.empty()) HIP_TRY(hipStreamSynchronize(e->so.s));
+    const auto host_t0 = std::chrono::steady_clock::now();
+    if (!act.empty()) prof_add_host("piop_engine_wait", wait_t0);
     e->t_inner_fold.collect();
+    e->t_step.collect();
     const char* pin = static_cast<const char*>(e->pin);
     // Host: every proof's round polynomial into its transcript, its next challenge.
     for (size_t i = 0; i < act.size(); ++i) {
@@ -3691,6 +3711,7 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
         for (auto& j : e->jobs) n += j->phase != JobPhase::kDone;
         *in_flight = n;
     }
+    if (!act.empty()) prof_add_host("piop_engine_host_finish", host_t0);
     return BITZ_OK;
 }
 
@@ -4009,6 +4030,217 @@ extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_
     return BITZ_OK;
 }
 
+// Whether provers with a Keccak transcript draw their challenges on the
+// device (bitz_set_device_fiat_shamir); on by default.
+static std::atomic<bool> g_device_fs{true};
+
+extern "C" int32_t bitz_set_device_fiat_shamir(int32_t on) {
+    g_device_fs = on != 0;
+    return BITZ_OK;
+}
+
+// gpgkr_prove with the transcript on the device (a Keccak transcript, no
+// grinding): the layers in the reference's order, each layer's eq table from
+// its point on the device, the grid-wide rounds each followed by
+// k_gkr_fs_round, the last rounds and the layer's end in k_gkr_tail_fs.
+// Nothing waits for the host until the end, when the transcript state, the
+// leaf point, the claim and the prover messages (appended to the narg string
+// in their order) come back.
+static int32_t gkr_prove_device_fs(bitz_gkr* h, bitz_transcript* t, const std::vector<f128>& top,
+                                   bitz_f128* leaf_point, bitz_f128* leaf_claim) {
+    hipStream_t s = h->so.s;
+    constexpr int T = bitz::kGkrTailMax;
+    const int layers = h->log_leaves - h->log_groups;
+    // Layer l (0 = the top) has n = log_groups + l coordinates; its point is
+    // at pt_off[l], the leaf point at pt_off[layers].
+    std::vector<size_t> pt_off(layers + 1), msg_off(layers), slot_off(layers);
+    size_t n_pts = 0, n_msgs = 0, n_slots = 0, eq_scratch = 0;
+    for (int l = 0; l <= layers; ++l) {
+        pt_off[l] = n_pts;
+        n_pts += (size_t)(h->log_groups + l);
+    }
+    for (int l = 0; l < layers; ++l) {
+        const int n = h->log_groups + l, k = n - 1;
+        msg_off[l] = n_msgs;
+        n_msgs += 2 * ((size_t)n + 1);
+        slot_off[l] = n_slots;
+        n_slots += (size_t)n;
+        if (k > bitz::kGkrEqSmallMax) eq_scratch = std::max(eq_scratch, ((size_t)1 << (k / 2)) + ((size_t)1 << (k - k / 2)));
+    }
+    Dev P, M, st, rtab, eqs, desc;
+    TRY(P.alloc(n_pts * sizeof(f128)));
+    TRY(M.alloc(n_msgs * sizeof(f128)));
+    TRY(st.alloc(sizeof(bitz::GkrFsState)));
+    TRY(rtab.alloc(kConstTable4Entries * sizeof(uint4)));
+    TRY(eqs.alloc(std::max<size_t>(eq_scratch, 1) * sizeof(f128)));
+    TRY(h->acc.alloc(std::max<size_t>(n_slots, 1) * 8 * sizeof(uint32_t)));
+    TRY(h->eq.alloc(((size_t)1 << std::max(h->log_leaves - 2, 0)) * sizeof(f128)));
+    // Descriptors of every launch: pointers only, so all are known up front.
+    struct LayerPlan {
+        size_t eq_items = 0, eq_items_n = 0, combine = 0;
+        bool split = false;
+        uint32_t combine_blocks = 0;
+        size_t rounds = 0;  // first grid-wide round descriptor
+        int grid_rounds = 0;
+        size_t tail = 0;
+    };
+    std::vector<LayerPlan> plan(layers);
+    std::vector<bitz::GkrEqItem> eq_items;
+    std::vector<bitz::GkrCombine> combines;
+    std::vector<bitz::GkrRound> rounds;
+    std::vector<bitz::GkrTail> tails;
+    uint32_t max_blocks = 1;
+    for (int l = 0; l < layers; ++l) {
+        const int n = h->log_groups + l, k = n - 1;
+        const int j = layers - 1 - l;  // storage layer (0 = leaves): the children of layer l's nodes
+        f128* left = h->store.f() + h->off[j];
+        f128* right = left + ((size_t)1 << n);
+        const f128* pt = P.f() + pt_off[l];
+        LayerPlan& lp = plan[l];
+        lp.eq_items = eq_items.size();
+        if (k <= bitz::kGkrEqSmallMax) {
+            eq_items.push_back({pt, k, h->eq.f()});
+        } else {
+            const int kl = k / 2, kh = k - kl;
+            f128* low = eqs.f();
+            f128* high = low + ((size_t)1 << kl);
+            eq_items.push_back({pt, kl, low});
+            eq_items.push_back({pt + kl, kh, high});
+            bitz::GkrCombine c{};
+            c.low = low;
+            c.high = high;
+            c.out = h->eq.f();
+            c.n = (size_t)1 << k;
+            c.low_bits = kl;
+            c.blocks = grid_for(c.n);
+            c.block_base = 0;
+            lp.split = true;
+            lp.combine = combines.size();
+            lp.combine_blocks = c.blocks;
+            max_blocks = std::max(max_blocks, c.blocks);
+            combines.push_back(c);
+        }
+        lp.eq_items_n = eq_items.size() - lp.eq_items;
+        lp.rounds = rounds.size();
+        for (int i = n - 1; i > T; --i) {
+            bitz::GkrRound g{};
+            g.left = left;
+            g.right = right;
+            g.eq = h->eq.f();
+            g.h = (size_t)1 << i;
+            g.first = i == n - 1;
+            if (!g.first) g.rt4 = static_cast<const uint4*>(rtab.p);
+            g.acc = h->acc.u32() + 8 * (slot_off[l] + (size_t)(n - 1 - i));
+            g.z = pt + i;
+            g.blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((g.h + bitz::kStageBlock - 1) / bitz::kStageBlock, 2048));
+            g.block_base = 0;
+            max_blocks = std::max(max_blocks, g.blocks);
+            rounds.push_back(g);
+            ++lp.grid_rounds;
+        }
+        bitz::GkrTail tl{};
+        tl.left = left;
+        tl.right = right;
+        tl.eq = h->eq.f();
+        tl.n = n;
+        tl.i0 = std::min(n - 1, T);
+        tl.rtabs4 = tl.i0 < n - 1 ? static_cast<const uint4*>(rtab.p) : nullptr;
+        lp.tail = tails.size();
+        tails.push_back(tl);
+    }
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t o = off;
+        off += bytes;
+        return o;
+    };
+    const size_t eqi_o = place(eq_items.size() * sizeof(bitz::GkrEqItem));
+    const size_t cmb_o = place(combines.size() * sizeof(bitz::GkrCombine));
+    const size_t rd_o = place(rounds.size() * sizeof(bitz::GkrRound));
+    const size_t tl_o = place(tails.size() * sizeof(bitz::GkrTail));
+    const size_t map_o = place((size_t)max_blocks * 4);  // all zeros: every launch has one descriptor
+    std::vector<char> stage(off, 0);
+    memcpy(stage.data() + eqi_o, eq_items.data(), eq_items.size() * sizeof(bitz::GkrEqItem));
+    memcpy(stage.data() + cmb_o, combines.data(), combines.size() * sizeof(bitz::GkrCombine));
+    memcpy(stage.data() + rd_o, rounds.data(), rounds.size() * sizeof(bitz::GkrRound));
+    memcpy(stage.data() + tl_o, tails.data(), tails.size() * sizeof(bitz::GkrTail));
+    TRY(desc.alloc(std::max<size_t>(off, 16)));
+    bitz::GkrFsState init{};
+    init.ts = t->t.keccak;
+    init.eq_bound = hostf::one();
+    HIP_TRY(hipMemcpyAsync(desc.p, stage.data(), off, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.p, top.data(), top.size() * sizeof(f128), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(st.p, &init, sizeof init, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s));
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_tail_fs),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrTailSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_round_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrRoundSmem);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_eq_items),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_eq_combine_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        attr = true;
+    }
+    const char* dp = static_cast<const char*>(desc.p);
+    const uint32_t* zmap = reinterpret_cast<const uint32_t*>(dp + map_o);
+    auto* state = static_cast<bitz::GkrFsState*>(st.p);
+    h->tm.begin(s, "gkr_prove_device_fs", 0);
+    for (int l = 0; l < layers; ++l) {
+        const int n = h->log_groups + l;
+        const LayerPlan& lp = plan[l];
+        hipLaunchKernelGGL(bitz::k_gkr_eq_items, dim3((unsigned)lp.eq_items_n), dim3(bitz::kStageBlock),
+                           bitz::kLaneSmem256, s,
+                           reinterpret_cast<const bitz::GkrEqItem*>(dp + eqi_o) + lp.eq_items);
+        if (lp.split)
+            hipLaunchKernelGGL(bitz::k_gkr_eq_combine_batch, dim3(lp.combine_blocks), dim3(bitz::kStageBlock),
+                               bitz::kLaneSmem256, s,
+                               reinterpret_cast<const bitz::GkrCombine*>(dp + cmb_o) + lp.combine, zmap);
+        for (int g = 0; g < lp.grid_rounds; ++g) {
+            const bitz::GkrRound& R = rounds[lp.rounds + g];
+            const int i = n - 1 - g;
+            hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3(R.blocks), dim3(bitz::kStageBlock),
+                               bitz::kGkrRoundSmem, s,
+                               reinterpret_cast<const bitz::GkrRound*>(dp + rd_o) + lp.rounds + g, zmap);
+            bitz::GkrFsRound fr{};
+            fr.st = state;
+            fr.sums = R.acc;
+            fr.z = P.f() + pt_off[l] + i;
+            fr.r_out = P.f() + pt_off[l + 1] + i;
+            fr.rtab_out = static_cast<uint4*>(rtab.p);
+            fr.msg_out = M.f() + msg_off[l] + 2 * (size_t)(n - 1 - i);
+            hipLaunchKernelGGL(bitz::k_gkr_fs_round, dim3(1), dim3(bitz::kStageBlock), 0, s, fr);
+        }
+        bitz::GkrFsTail ft{};
+        ft.st = state;
+        ft.point = P.f() + pt_off[l];
+        ft.next = P.f() + pt_off[l + 1];
+        ft.msgs = M.f() + msg_off[l];
+        hipLaunchKernelGGL(bitz::k_gkr_tail_fs, dim3(1), dim3(bitz::kStageBlock), bitz::kGkrTailSmem, s,
+                           reinterpret_cast<const bitz::GkrTail*>(dp + tl_o) + lp.tail, ft);
+        TRY(check_launch());
+    }
+    h->tm.end(s);
+    bitz::GkrFsState fin{};
+    std::vector<f128> msgs(n_msgs), leaf((size_t)h->log_leaves);
+    HIP_TRY(hipMemcpyAsync(&fin, st.p, sizeof fin, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(msgs.data(), M.p, n_msgs * sizeof(f128), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(leaf.data(), P.f() + pt_off[layers], leaf.size() * sizeof(f128), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h->tm.collect();
+    // The host transcript continues from the device's state; the prover
+    // messages join its narg string in the order they were absorbed.
+    t->t.keccak = fin.ts;
+    const uint8_t* mb = reinterpret_cast<const uint8_t*>(msgs.data());
+    t->t.narg.insert(t->t.narg.end(), mb, mb + n_msgs * sizeof(f128));
+    for (size_t i = 0; i < leaf.size(); ++i) leaf_point[i] = B(leaf[i]);
+    *leaf_claim = B(fin.claim);
+    return BITZ_OK;
+}
+
 extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point,
                                   uint32_t point_len, uint32_t security_bits, bitz_f128* leaf_point,
                                   bitz_f128* leaf_claim) {
@@ -4024,6 +4256,11 @@ extern "C" int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f1
         return fail(BITZ_ERR_POINT_LEN, "the output has one value per tree, so the point has a coordinate per tree bit");
     if (h->consumed) return fail(BITZ_ERR_STATE, "layers already consumed");
     h->consumed = true;
+    if (t->t.type == Transcript::Type::kKeccak && grind_bits == 0 && g_device_fs) {
+        std::vector<f128> top(point_len);
+        for (uint32_t i = 0; i < point_len; ++i) top[i] = F(point[i]);
+        return gkr_prove_device_fs(h, t, top, leaf_point, leaf_claim);
+    }
     hipStream_t s = h->so.s;
     const int layers = h->log_leaves - h->log_groups;
     int total_rounds = 0;

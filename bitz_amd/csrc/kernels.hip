@@ -10,6 +10,7 @@
 
 #include "gf128.hpp"
 #include "keccak.hpp"
+#include "fs.hpp"
 #include "kernels.hpp"
 
 namespace bitz {
@@ -1427,7 +1428,7 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound*
     const size_t start = (blockIdx.x - A.block_base) * (size_t)kStageBlock + threadIdx.x;
     const size_t stride = (size_t)A.blocks * kStageBlock;
     const size_t h = A.h;
-    const int at_one = A.at_one;
+    const int at_one = A.z ? (int)g_is_zero(g_load_hbm(A.z)) : A.at_one;
     f128* left = A.left;
     f128* right = A.right;
     f128* eq = A.eq;
@@ -1554,6 +1555,179 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail(const GkrTail* d) {
         const g4 w = g_add(u, g_mul_table4(rt, g_add(u, v)));
         reinterpret_cast<uint4*>(A.evals)[threadIdx.x] = make_uint4(w.w[0], w.w[1], w.w[2], w.w[3]);
     }
+}
+
+// eq(z, r) for one coordinate: 1 + z + r in characteristic 2.
+__device__ __forceinline__ g4 eq_factor(const g4& r, const g4& z) {
+    g4 e = g_add(r, z);
+    e.w[0] ^= 1u;
+    return e;
+}
+
+// lib.rs:142-150 after a grid-wide round: the message [eq_bound sum_b,
+// eq_bound sum_inf] as a prover message, the challenge, eq_bound times
+// eq(z, r), and r's table for the next round's bind.
+__global__ __launch_bounds__(kStageBlock) void k_gkr_fs_round(GkrFsRound A) {
+    __shared__ FsLds f;
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    fs_load(f, &A.st->ts);
+    const g4 z = g_load_hbm(A.z);
+    const g4 eb = g_load_hbm(&A.st->eq_bound);
+    g4 m[2] = {g_zero(), g_zero()};
+    if (threadIdx.x < 64) {  // wave 0: the products
+        g4 sb, si;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            sb.w[i] = A.sums[i];
+            si.w[i] = A.sums[4 + i];
+        }
+        m[0] = g_mul_wave(eb, sb);
+        m[1] = g_mul_wave(eb, si);
+        if (threadIdx.x == 0) {
+            g_store_hbm(A.msg_out, m[0]);
+            g_store_hbm(A.msg_out + 1, m[1]);
+        }
+    }
+    const g4 r = fs_message2_squeeze(f, 1, m[0], m[1]);
+    if (threadIdx.x < 64) {
+        const g4 e = g_mul_wave(eb, eq_factor(r, z));
+        if (threadIdx.x == 0) {
+            g_store_hbm(A.r_out, r);
+            g_store_hbm(&A.st->eq_bound, e);
+        }
+    }
+    fs_save(&A.st->ts, f);
+    build_const_table4(tab, basis, r);
+    for (int e = threadIdx.x; e < kConstTable4Entries; e += kStageBlock) A.rtab_out[e] = tab[e];
+}
+
+// k_gkr_tail with the transcript on the device: after each round's sums, the
+// message, the challenge (written to the next point) and r's table, built in
+// LDS for the next round's bind; after round 0 the final bind, [L(r), R(r)]
+// as a prover message, the challenge t and the layer's claim
+// L + t (R + L) (lib.rs:142-158).  The layer's coordinates come from its
+// point on the device, so a zero coordinate takes the at-one sums.
+__global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, GkrFsTail fs) {
+    const GkrTail A = d[0];
+    extern __shared__ __attribute__((aligned(256))) uint4 lds_raw[];
+    __shared__ FsLds f;
+    __shared__ uint4 basis[128];
+    __shared__ uint32_t red[8];
+    __shared__ uint4 lr[2];
+    constexpr size_t kArr = (size_t)4 << kGkrTailMax;
+    uint4* Ls = lds_raw;
+    uint4* Rs = Ls + kArr;
+    uint4* Es = Rs + kArr;
+    uint4* rt = Es + ((size_t)2 << kGkrTailMax);
+    uint4* col = rt + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    const int n = A.n, i0 = A.i0;
+    const bool whole = i0 == n - 1;
+    const size_t len = whole ? (size_t)1 << n : (size_t)4 << i0;
+    const size_t elen = whole ? (size_t)1 << (n - 1) : (size_t)2 << i0;
+    auto ld = [](const uint4* p) {
+        const uint4 u = *p;
+        g4 r;
+        r.w[0] = u.x; r.w[1] = u.y; r.w[2] = u.z; r.w[3] = u.w;
+        return r;
+    };
+    auto st = [](uint4* p, const g4& v) { *p = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]); };
+    for (size_t e = threadIdx.x; e < len; e += kStageBlock) {
+        Ls[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.left + e));
+        Rs[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.right + e));
+    }
+    for (size_t e = threadIdx.x; e < elen; e += kStageBlock) Es[e] = u4_load_hbm(reinterpret_cast<const uint4*>(A.eq + e));
+    if (!whole) {
+        g4 nx[kRtabPer];
+        rtab_fetch(nx, A.rtabs4);
+        rtab_store(rt, nx);
+    }
+    fs_load(f, &fs.st->ts);  // ends with a barrier: the arrays and rt are in place
+    g4 eb = g_load_hbm(&fs.st->eq_bound);  // kept by wave 0
+    for (int i = i0; i >= 0; --i) {
+        const size_t h = (size_t)1 << i;
+        const bool bind = !(whole && i == n - 1);
+        const g4 z = g_load_hbm(fs.point + i);
+        const bool at_one = g_is_zero(z);
+        g4 sb = g_zero(), si = g_zero();
+        for (size_t x = threadIdx.x; x < h; x += kStageBlock) {
+            g4 l0, l1, r0, r1, e;
+            if (bind) {
+                const size_t hh = 2 * h;
+                auto ip = [&](const uint4* p, size_t a, size_t b) {
+                    const g4 u = ld(p + a), v = ld(p + b);
+                    return g_add(u, g_mul_table4(rt, g_add(u, v)));
+                };
+                l0 = ip(Ls, x, x + hh);
+                l1 = ip(Ls, x + h, x + h + hh);
+                r0 = ip(Rs, x, x + hh);
+                r1 = ip(Rs, x + h, x + h + hh);
+                st(Ls + x, l0);
+                st(Ls + x + h, l1);
+                st(Rs + x, r0);
+                st(Rs + x + h, r1);
+                e = g_add(ld(Es + x), ld(Es + x + h));
+                st(Es + x, e);
+            } else {
+                l0 = ld(Ls + x);
+                l1 = ld(Ls + x + h);
+                r0 = ld(Rs + x);
+                r1 = ld(Rs + x + h);
+                e = ld(Es + x);
+            }
+            g4 el, ed;
+            g_mul_lane2(e, at_one ? l1 : l0, g_add(l1, l0), col, el, ed);
+            sb = g_add(sb, g_mul_lane(el, at_one ? r1 : r0, col));
+            si = g_add(si, g_mul_lane(ed, g_add(r1, r0), col));
+        }
+        g4 v[2] = {sb, si};
+        block_xor_store<2>(v, red, kStageBlock);  // its barrier: every round-i reader of rt is done
+        __syncthreads();
+        g4 m[2] = {g_zero(), g_zero()};
+        if (threadIdx.x < 64) {  // wave 0: the products
+            g4 s0, s1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                s0.w[k] = red[k];
+                s1.w[k] = red[4 + k];
+            }
+            m[0] = g_mul_wave(eb, s0);
+            m[1] = g_mul_wave(eb, s1);
+            if (threadIdx.x == 0) {
+                g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i), m[0]);
+                g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i) + 1, m[1]);
+            }
+        }
+        const g4 r = fs_message2_squeeze(f, 1, m[0], m[1]);
+        if (threadIdx.x < 64) {
+            eb = g_mul_wave(eb, eq_factor(r, z));
+            if (threadIdx.x == 0) g_store_hbm(fs.next + i, r);
+        }
+        build_const_table4(rt, basis, r);  // r_i binds round i - 1 (or the final bind)
+    }
+    // Final bind of the two remaining entries by r_0.
+    if (threadIdx.x < 2) {
+        uint4* P = threadIdx.x == 0 ? Ls : Rs;
+        const g4 u = ld(P), v = ld(P + 1);
+        st(lr + threadIdx.x, g_add(u, g_mul_table4(rt, g_add(u, v))));
+    }
+    __syncthreads();
+    g4 e2[2] = {ld(lr), ld(lr + 1)};
+    if (threadIdx.x == 0) {
+        g_store_hbm(fs.msgs + 2 * (size_t)n, e2[0]);
+        g_store_hbm(fs.msgs + 2 * (size_t)n + 1, e2[1]);
+    }
+    const g4 t = fs_message2_squeeze(f, 1, e2[0], e2[1]);
+    if (threadIdx.x < 64) {
+        const g4 claim = g_add(e2[0], g_mul_wave(t, g_add(e2[1], e2[0])));
+        if (threadIdx.x == 0) {
+            g_store_hbm(fs.next + n, t);
+            g_store_hbm(&fs.st->claim, claim);
+            g_store_hbm(&fs.st->eq_bound, g_one());
+        }
+    }
+    fs_save(&fs.st->ts, f);
 }
 
 }  // namespace bitz

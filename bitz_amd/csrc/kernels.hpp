@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "gf128.hpp"
+#include "keccak.hpp"
 
 namespace bitz {
 
@@ -300,6 +301,7 @@ struct GkrRound {
     const uint4* rt4;  // bind rounds: r's 4-bit table
     uint32_t* acc;
     int first, at_one;
+    const f128* z;     // when set, at_one is whether the coordinate *z is zero (read on the device)
     uint32_t block_base, blocks;
 };
 // 256 threads: r's table (bind rounds) and the lane tables in dynamic LDS.
@@ -323,5 +325,33 @@ constexpr size_t kGkrTailSmem =
     sizeof(uint4) * (2 * ((size_t)4 << kGkrTailMax) + ((size_t)2 << kGkrTailMax) + kConstTable4Entries +
                      4 * 64 * kLaneTableEntries);
 __global__ void k_gkr_tail(const GkrTail* d);
+
+// ---- GKR with the transcript on the device (the Keccak duplex, fs.hpp) ----
+// Each round's message and challenge are drawn by the device right after the
+// round's sums: k_gkr_fs_round after a grid-wide round, and k_gkr_tail_fs
+// draws them between the LDS rounds of a layer's tail and ends the layer.
+struct GkrFsState {
+    KeccakState ts;  // the transcript
+    f128 eq_bound;   // eq(z, r) over the layer's coordinates bound so far
+    f128 claim;      // the claim of the last finished layer
+};
+struct GkrFsRound {
+    GkrFsState* st;
+    const uint32_t* sums;  // the round's [sum_b, sum_inf]
+    const f128* z;         // the point's coordinate of the round
+    f128* r_out;           // the next point's coordinate
+    uint4* rtab_out;       // r's 4-bit table (the next round binds by it)
+    f128* msg_out;         // [eq_bound sum_b, eq_bound sum_inf]
+};
+__global__ void k_gkr_fs_round(GkrFsRound a);
+// d's rtabs4 is r_(i0+1)'s table when the tail does not start the layer;
+// acc, evals and at_one are unused.
+struct GkrFsTail {
+    GkrFsState* st;
+    const f128* point;  // the layer's point (n coordinates)
+    f128* next;         // the next layer's point (n + 1 coordinates)
+    f128* msgs;         // round i's message at msgs + 2 (n - 1 - i), then [L(r), R(r)] at msgs + 2 n
+};
+__global__ void k_gkr_tail_fs(const GkrTail* d, GkrFsTail fs);
 
 }  // namespace bitz

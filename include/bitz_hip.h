@@ -114,9 +114,16 @@ bitz_transcript* bitz_transcript_duplex(const uint8_t* domain, size_t len);
  * le64(len) || bytes; squeeze: 'S', pad 0x1F at the position and 0x80 at
  * byte 135, permute, the state's first 16 bytes.  Shaped like spongefish's
  * duplex sponge over Keccak (crates/transcript/src/domain.rs:13-42), whose
- * own framing is not reproduced; the same code runs on the device, so
- * provers can draw these challenges there (BITZ_DEVICE_TRANSCRIPT). */
+ * own framing is not reproduced.  The same duplex runs on the device: a
+ * prover given a Keccak transcript draws its challenges there (see
+ * bitz_set_device_fiat_shamir) and hands the state back at the end. */
 bitz_transcript* bitz_transcript_keccak(const uint8_t* domain, size_t len);
+/* Whether provers draw a Keccak transcript's challenges on the device (on by
+ * default; off: every round's message goes to the host transcript and the
+ * challenge comes back).  Same transcript, messages and results either way;
+ * no counterpart in the reference, whose transcript is host code.  Today:
+ * bitz_gkr_prove without grinding (Bits100). */
+int32_t bitz_set_device_fiat_shamir(int32_t on);
 /* One-message Keccak sponge (pad 0x06: SHA3-*, 0x1F: SHAKE*) on the host,
  * or with BITZ_DEVICE by one device thread: the permutation's check against
  * FIPS 202 answers. */
@@ -516,7 +523,9 @@ int32_t bitz_gkr_build_bittable(bitz_gkr** h, const bitz_f128* images, uint32_t 
  * security_bits is the SecurityLevel (100 = Bits100, 128 = Bits128; else
  * BITZ_ERR_ARG): every cubic round grinds grinding_bits(3) bits (0 at
  * Bits100, 2 at Bits128) under the label "gkr/cubic/v1" before its
- * challenge (lib.rs:144, 416), through bitz_transcript_grind. */
+ * challenge (lib.rs:144, 416), through bitz_transcript_grind.  With a Keccak
+ * transcript and no grinding, every message and challenge is drawn on the
+ * device (bitz_set_device_fiat_shamir). */
 int32_t bitz_gkr_prove(bitz_gkr* h, bitz_transcript* t, const bitz_f128* point, uint32_t point_len,
                        uint32_t security_bits, bitz_f128* leaf_point, bitz_f128* leaf_claim);
 /* Copies layer j (0 = leaves) to host; *len receives its length. */

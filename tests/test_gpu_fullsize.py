@@ -7,7 +7,8 @@ the same seeded inputs, bit for bit.
   host round trip per round): every round polynomial, point and claim;
 - the grand product over 2^22 leaves, full depth (2 trees) and the
   end-to-end shape (2^15 trees, 7 layers): every layer message, the leaf
-  point and claim, queued and transcript-driven;
+  point and claim, queued, transcript-driven, and with a Keccak transcript
+  whose challenges the device draws;
 - the batch of 256 outer sumchecks with num_vars uniform in [12, 22]
   (seed 2024, the batched workload's instances), each instance against the
   oracle;
@@ -112,6 +113,15 @@ def test_gkr_full_size_queued_and_transcript(log_groups):
     gp, gc = G.gpgkr_prove(gt, point, wit)
     assert np.array_equal(gp, ref_point) and gc == ref_claim
     assert gt.narg() == dt.narg()
+    say("gkr full size: Keccak transcript, challenges drawn on the device")
+    _, wit = G.grand_product(G.DeviceTable(leaves), log_groups)
+    kt = O.KeccakTranscript(b"gkr/full/keccak")
+    ref_point, ref_claim = O.gkr_prove(kt, point, store, L)
+    gt = G.Transcript.keccak(b"gkr/full/keccak")
+    gp, gc = G.gpgkr_prove(gt, point, wit)
+    assert np.array_equal(gp, ref_point) and gc == ref_claim
+    assert gt.narg() == kt.narg()
+    assert gt.squeeze() == O.squeeze(kt)
 
 
 def test_batch_of_256_instances_at_full_size():
