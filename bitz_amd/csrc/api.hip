@@ -4041,8 +4041,10 @@ extern "C" int32_t bitz_set_device_fiat_shamir(int32_t on) {
 
 // gpgkr_prove with the transcript on the device (a Keccak transcript, no
 // grinding): the layers in the reference's order, each layer's eq table from
-// its point on the device, the grid-wide rounds each followed by
-// k_gkr_fs_round, the last rounds and the layer's end in k_gkr_tail_fs.
+// its point on the device, the grid-wide rounds each with its challenge
+// (k_gkr_round_fs, whose last workgroup draws it, or k_gkr_round_batch then
+// k_gkr_fs_round for the large ones), the last rounds and the layer's end in
+// k_gkr_tail_fs.
 // Nothing waits for the host until the end, when the transcript state, the
 // leaf point, the claim and the prover messages (appended to the narg string
 // in their order) come back.
@@ -4067,8 +4069,9 @@ static int32_t gkr_prove_device_fs(bitz_gkr* h, bitz_transcript* t, const std::v
         n_slots += (size_t)n;
         if (k > bitz::kGkrEqSmallMax) eq_scratch = std::max(eq_scratch, ((size_t)1 << (k / 2)) + ((size_t)1 << (k - k / 2)));
     }
-    Dev P, M, st, rtab, eqs, desc;
+    Dev P, M, st, rtab, eqs, desc, done;
     TRY(P.alloc(n_pts * sizeof(f128)));
+    TRY(done.alloc(std::max<size_t>(n_slots, 1) * sizeof(uint32_t)));
     TRY(M.alloc(n_msgs * sizeof(f128)));
     TRY(st.alloc(sizeof(bitz::GkrFsState)));
     TRY(rtab.alloc(kConstTable4Entries * sizeof(uint4)));
@@ -4173,8 +4176,11 @@ static int32_t gkr_prove_device_fs(bitz_gkr* h, bitz_transcript* t, const std::v
     HIP_TRY(hipMemcpyAsync(P.p, top.data(), top.size() * sizeof(f128), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(st.p, &init, sizeof init, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(h->acc.p, 0, h->acc.bytes, s));
+    HIP_TRY(hipMemsetAsync(done.p, 0, done.bytes, s));
     static bool attr = false;
     if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_round_fs),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrRoundSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_tail_fs),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kGkrTailSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_gkr_round_batch),
@@ -4202,17 +4208,21 @@ static int32_t gkr_prove_device_fs(bitz_gkr* h, bitz_transcript* t, const std::v
         for (int g = 0; g < lp.grid_rounds; ++g) {
             const bitz::GkrRound& R = rounds[lp.rounds + g];
             const int i = n - 1 - g;
-            hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3(R.blocks), dim3(bitz::kStageBlock),
-                               bitz::kGkrRoundSmem, s,
-                               reinterpret_cast<const bitz::GkrRound*>(dp + rd_o) + lp.rounds + g, zmap);
             bitz::GkrFsRound fr{};
             fr.st = state;
-            fr.sums = R.acc;
             fr.z = P.f() + pt_off[l] + i;
             fr.r_out = P.f() + pt_off[l + 1] + i;
             fr.rtab_out = static_cast<uint4*>(rtab.p);
             fr.msg_out = M.f() + msg_off[l] + 2 * (size_t)(n - 1 - i);
-            hipLaunchKernelGGL(bitz::k_gkr_fs_round, dim3(1), dim3(bitz::kStageBlock), 0, s, fr);
+            const bitz::GkrRound* rd = reinterpret_cast<const bitz::GkrRound*>(dp + rd_o) + lp.rounds + g;
+            if (R.blocks <= bitz::kGkrFusedMaxBlocks) {
+                hipLaunchKernelGGL(bitz::k_gkr_round_fs, dim3(R.blocks), dim3(bitz::kStageBlock), bitz::kGkrRoundSmem,
+                                   s, rd, fr, done.u32() + slot_off[l] + (size_t)(n - 1 - i));
+            } else {
+                hipLaunchKernelGGL(bitz::k_gkr_round_batch, dim3(R.blocks), dim3(bitz::kStageBlock),
+                                   bitz::kGkrRoundSmem, s, rd, zmap);
+                hipLaunchKernelGGL(bitz::k_gkr_fs_round, dim3(1), dim3(bitz::kStageBlock), 0, s, rd, fr);
+            }
         }
         bitz::GkrFsTail ft{};
         ft.st = state;

@@ -29,70 +29,51 @@ static __constant__ uint64_t kKeccakRoundConstants[24] = {
     0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
     0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
 
-// rho and pi together as one walk: pi moves lane p = x + 5 y to
-// y + 5 ((2 x + 3 y) mod 5) and, apart from lane 0, is a single cycle of 24
-// lanes, so the step can run in place carrying one lane at a time (each
-// rotated by rho of the lane it comes from).  The walk from lane 1, computed
-// at compile time from the two step mappings.
-struct KeccakPiWalk {
-    int lane[24];  // the lane written at step t
-    int rot[24];   // rho of the lane its value comes from
+// Keccak-f[1600] (FIPS 202 section 3.2) spread over a wave: lane l < 25
+// holds lane word l (x = l mod 5, y = l / 5); per round, theta's column
+// parities and D, pi and chi's row neighbours come through cross-lane reads
+// (ds_bpermute), rho is each lane's own rotation, iota lane 0's.  Lanes
+// 25..63 compute values no lane reads.  A wave issues one instruction per 4
+// clocks, so the 24 rounds' ~45 instructions each (against ~210 for the
+// rounds on scalars) set the time: about 6 us.
+struct KeccakLaneConsts {
+    int rho[25];
+    int pi_src[25];  // pi moves lane pi_src[l] to lane l
 };
-constexpr KeccakPiWalk keccak_pi_walk() {
+constexpr KeccakLaneConsts keccak_lane_consts() {
     constexpr int rho[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
-    KeccakPiWalk w{};
-    int p = 1;
-    for (int t = 0; t < 24; ++t) {
+    KeccakLaneConsts k{};
+    for (int p = 0; p < 25; ++p) {
         const int x = p % 5, y = p / 5;
-        const int q = y + 5 * ((2 * x + 3 * y) % 5);
-        w.lane[t] = q;
-        w.rot[t] = rho[p];
-        p = q;
+        k.rho[p] = rho[p];
+        k.pi_src[y + 5 * ((2 * x + 3 * y) % 5)] = p;
     }
-    return w;
+    return k;
+}
+static __constant__ KeccakLaneConsts kKeccakLane = keccak_lane_consts();
+
+__device__ __forceinline__ uint64_t fs_shfl64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
 }
 
-// Keccak-f[1600] (FIPS 202 section 3.2) in place: theta, the rho-pi walk,
-// chi row by row, iota; 32 live words.  Called by a whole wave on the same
-// (uniform) state, it runs on the scalar unit: 64-bit scalar XOR, AND-NOT
-// and shifts at one instruction per clock.
-__device__ __forceinline__ void fs_permute_regs(uint64_t (&a)[25]) {
-    constexpr KeccakPiWalk walk = keccak_pi_walk();
+__device__ __forceinline__ uint64_t fs_permute_wave(uint64_t a) {
+    const int l = min((int)(threadIdx.x & 63), 24);
+    const int x = l % 5, y = l / 5;
+    const int rot = kKeccakLane.rho[l], src = kKeccakLane.pi_src[l];
+    const int xm1 = (x + 4) % 5, xp1 = (x + 1) % 5, xp2 = (x + 2) % 5;
 #pragma unroll 1
     for (int round = 0; round < 24; ++round) {
-        uint64_t c[5];
-#pragma unroll
-        for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
-#pragma unroll
-        for (int x = 0; x < 5; ++x) {
-            const uint64_t d = c[(x + 4) % 5] ^ keccak_rotl(c[(x + 1) % 5], 1);
-#pragma unroll
-            for (int y = 0; y < 5; ++y) a[x + 5 * y] ^= d;
-        }
-        uint64_t carry = a[1];
-#pragma unroll
-        for (int t = 0; t < 24; ++t) {
-            const uint64_t next = a[walk.lane[t]];
-            a[walk.lane[t]] = keccak_rotl(carry, walk.rot[t]);
-            carry = next;
-        }
-#pragma unroll
-        for (int y = 0; y < 5; ++y) {
-            uint64_t row[5];
-#pragma unroll
-            for (int x = 0; x < 5; ++x) row[x] = a[x + 5 * y];
-#pragma unroll
-            for (int x = 0; x < 5; ++x) a[x + 5 * y] = row[x] ^ (~row[(x + 1) % 5] & row[(x + 2) % 5]);
-        }
-        a[0] ^= kKeccakRoundConstants[round];
+        const uint64_t c = fs_shfl64(a, x) ^ fs_shfl64(a, x + 5) ^ fs_shfl64(a, x + 10) ^ fs_shfl64(a, x + 15) ^
+                           fs_shfl64(a, x + 20);
+        a ^= fs_shfl64(c, xm1) ^ keccak_rotl(fs_shfl64(c, xp1), 1);
+        const uint64_t r = rot ? (a << rot) | (a >> (64 - rot)) : a;
+        const uint64_t b = fs_shfl64(r, src);
+        a = b ^ (~fs_shfl64(b, xp1 + 5 * y) & fs_shfl64(b, xp2 + 5 * y));
+        if (l == 0) a ^= kKeccakRoundConstants[round];
     }
-}
-
-// A 64-bit value every lane of the wave holds, as a scalar.
-__device__ __forceinline__ uint64_t fs_uniform(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
+    return a;
 }
 
 __device__ __forceinline__ void fs_load(FsLds& f, const KeccakState* g) {
@@ -110,17 +91,12 @@ __device__ __forceinline__ void fs_save(KeccakState* g, const FsLds& f) {
     }
 }
 
-// The permutation by wave 0 on scalars; the wave writes the state back
-// (every lane the same words).
+// The permutation by wave 0, lane l on word l.
 __device__ __forceinline__ void fs_permute(FsLds& f) {
     if (threadIdx.x < 64) {
-        uint64_t a[25];
-#pragma unroll
-        for (int i = 0; i < 25; ++i) a[i] = fs_uniform(f.a[i]);
-        fs_permute_regs(a);
-        asm volatile("" ::: "memory");  // the stores stay after the rounds
-#pragma unroll
-        for (int i = 0; i < 25; ++i) f.a[i] = a[i];
+        const int l = min((int)threadIdx.x, 24);
+        const uint64_t w = fs_permute_wave(f.a[l]);
+        if (threadIdx.x < 25) f.a[threadIdx.x] = w;
     }
     __syncthreads();
 }
@@ -205,7 +181,8 @@ __device__ __forceinline__ g4 fs_squeeze(FsLds& f) {
 // pad 0x1F at 42 and 0x80 at 135, so wave 0 XORs seven words into the
 // state on scalars and permutes.  Elsewhere every thread takes the byte
 // path.  Called by all threads; m0, m1 are read from wave 0 (from thread 0
-// on the byte path).
+// on the byte path).  Lanes of wave 0 read their state word before any
+// writes it back: a wave's LDS reads and writes complete in order.
 __device__ __forceinline__ g4 fs_message2_squeeze(FsLds& f, int kind, const g4& m0, const g4& m1) {
     if (f.pos != 0) {
         const g4 m[2] = {m0, m1};
@@ -214,26 +191,25 @@ __device__ __forceinline__ g4 fs_message2_squeeze(FsLds& f, int kind, const g4& 
     }
     if (threadIdx.x < 64) {
         auto word = [](const g4& x, int k) { return (uint64_t)x.w[2 * k] | ((uint64_t)x.w[2 * k + 1] << 32); };
-        const uint64_t lo0 = fs_uniform(word(m0, 0)), hi0 = fs_uniform(word(m0, 1));
-        const uint64_t lo1 = fs_uniform(word(m1, 0)), hi1 = fs_uniform(word(m1, 1));
-        uint64_t a[25];
-#pragma unroll
-        for (int i = 0; i < 25; ++i) a[i] = fs_uniform(f.a[i]);
-        a[0] ^= (uint64_t)(kind == 0 ? 'P' : 'N') | (32ull << 8);
-        a[1] ^= lo0 << 8;
-        a[2] ^= (lo0 >> 56) | (hi0 << 8);
-        a[3] ^= (hi0 >> 56) | (lo1 << 8);
-        a[4] ^= (lo1 >> 56) | (hi1 << 8);
-        a[5] ^= (hi1 >> 56) | ((uint64_t)'S' << 8) | (0x1Full << 16);
-        a[(kKeccakRate - 1) >> 3] ^= 0x80ull << 56;
-        fs_permute_regs(a);
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int i = 0; i < 25; ++i) f.a[i] = a[i];
-        f.out[0] = (uint32_t)a[0];
-        f.out[1] = (uint32_t)(a[0] >> 32);
-        f.out[2] = (uint32_t)a[1];
-        f.out[3] = (uint32_t)(a[1] >> 32);
+        const uint64_t lo0 = word(m0, 0), hi0 = word(m0, 1), lo1 = word(m1, 0), hi1 = word(m1, 1);
+        const int l = min((int)threadIdx.x, 24);
+        uint64_t m = 0;  // this lane's word of the framed message, 'S' and the pad
+        switch (l) {
+            case 0: m = (uint64_t)(kind == 0 ? 'P' : 'N') | (32ull << 8); break;
+            case 1: m = lo0 << 8; break;
+            case 2: m = (lo0 >> 56) | (hi0 << 8); break;
+            case 3: m = (hi0 >> 56) | (lo1 << 8); break;
+            case 4: m = (lo1 >> 56) | (hi1 << 8); break;
+            case 5: m = (hi1 >> 56) | ((uint64_t)'S' << 8) | (0x1Full << 16); break;
+            case (kKeccakRate - 1) >> 3: m = 0x80ull << 56; break;
+            default: break;
+        }
+        const uint64_t w = fs_permute_wave(f.a[l] ^ m);
+        if (threadIdx.x < 25) f.a[threadIdx.x] = w;
+        if (threadIdx.x < 2) {
+            f.out[2 * threadIdx.x] = (uint32_t)w;
+            f.out[2 * threadIdx.x + 1] = (uint32_t)(w >> 32);
+        }
     }
     __syncthreads();
     g4 r;

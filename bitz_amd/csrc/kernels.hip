@@ -1419,13 +1419,15 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_layers_small(f128* layer, i
     }
 }
 
-__global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
-    const GkrRound A = d[bmap[blockIdx.x]];
+// One layer round's share of workgroup `block` of A.blocks: the sums
+// (binding by r_(i+1) first unless it is the layer's first round), XORed
+// into A.acc.
+__device__ __forceinline__ void gkr_round_body(const GkrRound& A, size_t block) {
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
-    const size_t start = (blockIdx.x - A.block_base) * (size_t)kStageBlock + threadIdx.x;
+    const size_t start = block * (size_t)kStageBlock + threadIdx.x;
     const size_t stride = (size_t)A.blocks * kStageBlock;
     const size_t h = A.h;
     const int at_one = A.z ? (int)g_is_zero(g_load_hbm(A.z)) : A.at_one;
@@ -1465,6 +1467,11 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound*
     }
     g4 v[2] = {sb, si};
     block_xor_to_global<2>(v, A.acc, kStageBlock);
+}
+
+__global__ __launch_bounds__(kStageBlock) void k_gkr_round_batch(const GkrRound* d, const uint32_t* bmap) {
+    const GkrRound A = d[bmap[blockIdx.x]];
+    gkr_round_body(A, blockIdx.x - A.block_base);
 }
 
 // The layer's arrays in LDS: L and R (2^(i0+2) entries at most), E, the
@@ -1564,30 +1571,38 @@ __device__ __forceinline__ g4 eq_factor(const g4& r, const g4& z) {
     return e;
 }
 
-// lib.rs:142-150 after a grid-wide round: the message [eq_bound sum_b,
-// eq_bound sum_inf] as a prover message, the challenge, eq_bound times
-// eq(z, r), and r's table for the next round's bind.
-__global__ __launch_bounds__(kStageBlock) void k_gkr_fs_round(GkrFsRound A) {
+// The round message [eq_bound sum_b, eq_bound sum_inf], one product per wave
+// (waves 0 and 1), for every thread.
+__device__ __forceinline__ void gkr_fs_messages(const g4& eb, const g4& s0, const g4& s1, uint4* mbuf, g4 (&m)[2]) {
+    const int wave = (int)(threadIdx.x >> 6);
+    if (wave < 2) {
+        const g4 v = g_mul_wave(eb, wave == 0 ? s0 : s1);
+        if ((threadIdx.x & 63) == 0) mbuf[wave] = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint4 u = mbuf[k];
+        m[k].w[0] = u.x; m[k].w[1] = u.y; m[k].w[2] = u.z; m[k].w[3] = u.w;
+    }
+}
+
+// lib.rs:142-150 after a grid-wide round: the message as a prover message,
+// the challenge, eq_bound times eq(z, r), and r's table for the next round's
+// bind.  Called by every thread of one workgroup with the round's sums.
+__device__ __forceinline__ void gkr_fs_step(const GkrFsRound& A, const g4& s0, const g4& s1) {
     __shared__ FsLds f;
     __shared__ uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
+    __shared__ uint4 mbuf[2];
     fs_load(f, &A.st->ts);
     const g4 z = g_load_hbm(A.z);
     const g4 eb = g_load_hbm(&A.st->eq_bound);
-    g4 m[2] = {g_zero(), g_zero()};
-    if (threadIdx.x < 64) {  // wave 0: the products
-        g4 sb, si;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            sb.w[i] = A.sums[i];
-            si.w[i] = A.sums[4 + i];
-        }
-        m[0] = g_mul_wave(eb, sb);
-        m[1] = g_mul_wave(eb, si);
-        if (threadIdx.x == 0) {
-            g_store_hbm(A.msg_out, m[0]);
-            g_store_hbm(A.msg_out + 1, m[1]);
-        }
+    g4 m[2];
+    gkr_fs_messages(eb, s0, s1, mbuf, m);
+    if (threadIdx.x == 0) {
+        g_store_hbm(A.msg_out, m[0]);
+        g_store_hbm(A.msg_out + 1, m[1]);
     }
     const g4 r = fs_message2_squeeze(f, 1, m[0], m[1]);
     if (threadIdx.x < 64) {
@@ -1600,6 +1615,35 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_fs_round(GkrFsRound A) {
     fs_save(&A.st->ts, f);
     build_const_table4(tab, basis, r);
     for (int e = threadIdx.x; e < kConstTable4Entries; e += kStageBlock) A.rtab_out[e] = tab[e];
+}
+
+// The challenge after a grid-wide round launched on its own (d[0]'s sums).
+__global__ __launch_bounds__(kStageBlock) void k_gkr_fs_round(const GkrRound* d, GkrFsRound fs) {
+    g4 s[2];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k >> 2].w[k & 3] = d[0].acc[k];
+    gkr_fs_step(fs, s[0], s[1]);
+}
+
+// A grid-wide round and its challenge in one launch: every workgroup adds
+// its sums, then counts itself done; the last one to finish (its count
+// comes after every other workgroup's sums, fenced device-wide) reads the
+// sums and draws the challenge.  `done` starts at zero.
+__global__ __launch_bounds__(kStageBlock) void k_gkr_round_fs(const GkrRound* d, GkrFsRound fs, uint32_t* done) {
+    const GkrRound A = d[0];
+    gkr_round_body(A, blockIdx.x);
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1u) == A.blocks - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    g4 s[2];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        s[k >> 2].w[k & 3] = __hip_atomic_load(A.acc + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    gkr_fs_step(fs, s[0], s[1]);
 }
 
 // k_gkr_tail with the transcript on the device: after each round's sums, the
@@ -1615,6 +1659,8 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, G
     __shared__ uint4 basis[128];
     __shared__ uint32_t red[8];
     __shared__ uint4 lr[2];
+    __shared__ uint4 mbuf[2];
+    __shared__ uint4 zs[64];  // the layer's point
     constexpr size_t kArr = (size_t)4 << kGkrTailMax;
     uint4* Ls = lds_raw;
     uint4* Rs = Ls + kArr;
@@ -1643,12 +1689,13 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, G
         rtab_fetch(nx, A.rtabs4);
         rtab_store(rt, nx);
     }
-    fs_load(f, &fs.st->ts);  // ends with a barrier: the arrays and rt are in place
-    g4 eb = g_load_hbm(&fs.st->eq_bound);  // kept by wave 0
+    if (threadIdx.x < (unsigned)n) zs[threadIdx.x] = u4_load_hbm(reinterpret_cast<const uint4*>(fs.point + threadIdx.x));
+    fs_load(f, &fs.st->ts);  // ends with a barrier: the arrays, rt and the point are in place
+    g4 eb = g_load_hbm(&fs.st->eq_bound);
     for (int i = i0; i >= 0; --i) {
         const size_t h = (size_t)1 << i;
         const bool bind = !(whole && i == n - 1);
-        const g4 z = g_load_hbm(fs.point + i);
+        const g4 z = ld(zs + i);
         const bool at_one = g_is_zero(z);
         g4 sb = g_zero(), si = g_zero();
         for (size_t x = threadIdx.x; x < h; x += kStageBlock) {
@@ -1684,26 +1731,20 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, G
         g4 v[2] = {sb, si};
         block_xor_store<2>(v, red, kStageBlock);  // its barrier: every round-i reader of rt is done
         __syncthreads();
-        g4 m[2] = {g_zero(), g_zero()};
-        if (threadIdx.x < 64) {  // wave 0: the products
-            g4 s0, s1;
+        g4 s0, s1, m[2];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                s0.w[k] = red[k];
-                s1.w[k] = red[4 + k];
-            }
-            m[0] = g_mul_wave(eb, s0);
-            m[1] = g_mul_wave(eb, s1);
-            if (threadIdx.x == 0) {
-                g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i), m[0]);
-                g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i) + 1, m[1]);
-            }
+        for (int k = 0; k < 4; ++k) {
+            s0.w[k] = red[k];
+            s1.w[k] = red[4 + k];
+        }
+        gkr_fs_messages(eb, s0, s1, mbuf, m);
+        if (threadIdx.x == 0) {
+            g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i), m[0]);
+            g_store_hbm(fs.msgs + 2 * (size_t)(n - 1 - i) + 1, m[1]);
         }
         const g4 r = fs_message2_squeeze(f, 1, m[0], m[1]);
-        if (threadIdx.x < 64) {
-            eb = g_mul_wave(eb, eq_factor(r, z));
-            if (threadIdx.x == 0) g_store_hbm(fs.next + i, r);
-        }
+        eb = g_mul_wave(eb, eq_factor(r, z));  // every wave keeps eq_bound
+        if (threadIdx.x == 0) g_store_hbm(fs.next + i, r);
         build_const_table4(rt, basis, r);  // r_i binds round i - 1 (or the final bind)
     }
     // Final bind of the two remaining entries by r_0.

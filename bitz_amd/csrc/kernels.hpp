@@ -328,8 +328,9 @@ __global__ void k_gkr_tail(const GkrTail* d);
 
 // ---- GKR with the transcript on the device (the Keccak duplex, fs.hpp) ----
 // Each round's message and challenge are drawn by the device right after the
-// round's sums: k_gkr_fs_round after a grid-wide round, and k_gkr_tail_fs
-// draws them between the LDS rounds of a layer's tail and ends the layer.
+// round's sums: by the last workgroup of a grid-wide round (k_gkr_round_fs),
+// and k_gkr_tail_fs draws them between the LDS rounds of a layer's tail and
+// ends the layer.
 struct GkrFsState {
     KeccakState ts;  // the transcript
     f128 eq_bound;   // eq(z, r) over the layer's coordinates bound so far
@@ -337,13 +338,19 @@ struct GkrFsState {
 };
 struct GkrFsRound {
     GkrFsState* st;
-    const uint32_t* sums;  // the round's [sum_b, sum_inf]
     const f128* z;         // the point's coordinate of the round
     f128* r_out;           // the next point's coordinate
     uint4* rtab_out;       // r's 4-bit table (the next round binds by it)
     f128* msg_out;         // [eq_bound sum_b, eq_bound sum_inf]
 };
-__global__ void k_gkr_fs_round(GkrFsRound a);
+// A grid-wide round (d[0], launched with d[0].blocks workgroups) whose last
+// workgroup to finish draws the challenge; `done` starts at zero.  Each
+// workgroup's device-scope fence costs more than a second launch once the
+// round spans many workgroups: those rounds run k_gkr_round_batch, then
+// k_gkr_fs_round (one workgroup) over d[0]'s sums.
+__global__ void k_gkr_round_fs(const GkrRound* d, GkrFsRound fs, uint32_t* done);
+__global__ void k_gkr_fs_round(const GkrRound* d, GkrFsRound fs);
+constexpr uint32_t kGkrFusedMaxBlocks = 64;
 // d's rtabs4 is r_(i0+1)'s table when the tail does not start the layer;
 // acc, evals and at_one are unused.
 struct GkrFsTail {
