@@ -3,8 +3,8 @@ proof latency on one MI355X -- the serving workload, by default.
 
 Default workload (serve): proving requests for SHA-256 compression chains of
 2^U blocks (U uniform in 0..8; message bytes and chaining value uniform from
-the seed).  Each request is its witness (host), the assignment MLE,
-Az/Bz/Cz and prove_spartan_piop (outer sumcheck, bind_and_batch, inner
+the seed).  Each request is its witness (host), the assignment (packed
+bits), Az/Bz/Cz and prove_spartan_piop (outer sumcheck, bind_and_batch, inner
 sumcheck) with every table on the device, served with continuous batching
 (bitz_amd/serving.py).  A step is one set of --serve-requests requests, all
 queued at once, served to completion; value = blocks proved per second over
@@ -701,7 +701,7 @@ def bench_serve(G, args) -> int:
         "config": {"workload": f"serving SHA-256 compression chains (config C5): a step serves {args.serve_requests} "
                                "requests of 2^U blocks (U uniform in 0..8) queued at once, with continuous batching "
                                f"({engines} PIOP engines, {args.serve_slots} slots, HBM budget "
-                               f"{args.serve_budget_gib} GiB); each request: witness (host), assignment MLE, "
+                               f"{args.serve_budget_gib} GiB); each request: witness (host), packed assignment, "
                                "Az/Bz/Cz, prove_spartan_piop (duplex transcript), every table on the device. "
                                f"p50/p99: per-request latency under Poisson arrivals at {args.serve_rate:g}/s",
                    "requests_per_step": args.serve_requests, "blocks_timed": blocks,
@@ -743,7 +743,7 @@ def bench_chain(G, args) -> int:
     167-217) at --chain-blocks blocks (default 64): a padded message from the
     seed, its rows built and digested once (setup, not timed, as the
     reference's benchmark prepares its instance); a step is the proof's
-    sumcheck stage: witness (host), assignment MLE, Az/Bz/Cz and
+    sumcheck stage: witness (host), the packed assignment, Az/Bz/Cz and
     prove_spartan_piop under the duplex transcript (one host round trip per
     round)."""
     import struct
@@ -761,7 +761,7 @@ def bench_chain(G, args) -> int:
     def step(i: int) -> float:
         t0 = time.perf_counter()
         w = c.witness(msg)
-        h = G.assignment_mle_words(w.h, c.columns, c.columns, device=True)
+        h = G.PackedAssignment(w.h, c.columns, c.columns, device=True)
         products = matrices.r1cs_products(h)
         G.prove_spartan_piop(G.Transcript.duplex(b"chain/%d" % i), matrices, digest, products, h)
         return time.perf_counter() - t0
@@ -791,7 +791,7 @@ def bench_chain(G, args) -> int:
         "vs_baseline": None,
         "dtype": "gf2_128",
         "data": "synthetic (message bytes from the seed, padded as SHA-256 pads)",
-        "config": {"workload": f"sha256-chain statement, {B} blocks, one proof at a time: witness, assignment MLE, "
+        "config": {"workload": f"sha256-chain statement, {B} blocks, one proof at a time: witness, packed assignment, "
                                "Az/Bz/Cz, prove_spartan_piop (duplex transcript)",
                    "blocks": B, "rows": c.rows, "columns": c.columns, "nnz": sum(c.nnz),
                    "row_vars": c.num_row_vars, "col_vars": c.num_column_vars, "setup_s": setup},

@@ -127,6 +127,12 @@ def lib():
             "bitz_prove_inner_sumcheck_with_challenges_noalloc": (I32, [P, F128, P, U32, P, U32, P, P, P, P, P]),
             "bitz_r1cs_products": (I32, [P, P, P, P, P, U32]),
             "bitz_assignment_mle": (I32, [P, SZ, SZ, P, U32]),
+            "bitz_assignment_pack": (I32, [P, SZ, SZ, P, U32]),
+            "bitz_r1cs_products_packed": (I32, [P, P, P, P, P, U32]),
+            "bitz_prove_spartan_piop_packed": (I32, [P, P, P, P, P, P, U32, P, U32, U32, P, P, P, P, P, P]),
+            "bitz_prove_inner_sumcheck_packed": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
+            "bitz_inner_begin_packed": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, U32]),
+            "bitz_piop_engine_submit_packed": (I32, [P, C.c_uint64, P, P, P, P, P, P, U32, P, U32]),
             "bitz_constraint_matrix_digest": (I32, [SZ, SZ, P, P, SZ, SZ, P, P, P, P]),
             "bitz_prove_spartan_piop": (I32, [P, P, P, P, P, P, U32, P, U32, U32, P, P, P, P, P, P]),
             "bitz_eq_table": (I32, [P, U32, P, U32]),
@@ -227,7 +233,7 @@ def elems(a) -> np.ndarray:
 def _p(a):
     """An address for a c_void_p parameter (plain ints convert without the
     cost of a ctypes cast; None is NULL)."""
-    if isinstance(a, DeviceTable):
+    if isinstance(a, (DeviceTable, PackedAssignment)):
         return a.ptr
     return a.ctypes.data if a is not None and a.size else None
 
@@ -298,6 +304,73 @@ class DeviceTable:
         if self.ptr:
             lib().bitz_device_free(C.c_void_p(self.ptr))
             self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PackedAssignment:
+    """A Boolean assignment kept as its packed bits (bitz_assignment_pack):
+    2^num_vars bits, bit j = words[j // 64] >> (j % 64) & 1, zero past the
+    assignment; on the host (numpy words) or in HBM (device=True).  What the
+    *_packed entry points take in place of build_assignment_mle's table
+    (crates/spartan/src/matrix.rs:392-420); the checks are that function's."""
+
+    def __init__(self, words, bit_len: int, columns: int, device: bool = False):
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        if not len(words):
+            words = np.zeros(1, np.uint64)
+        if bit_len > 64 * len(words):
+            raise BitzError(-7, "fewer words than bits")
+        padded = 1 << max(int(columns - 1).bit_length(), 0) if columns > 1 else 1
+        self.num_vars = padded.bit_length() - 1
+        self.len = padded
+        self.nwords = (padded + 63) // 64
+        self.device = device
+        self.ptr = 0
+        if device:
+            init()
+            ptr = C.c_void_p()
+            _check(lib().bitz_device_alloc(C.byref(ptr), self.nwords * 8))
+            self.ptr = ptr.value
+            try:
+                _check(lib().bitz_assignment_pack(_p(words), bit_len, columns, C.c_void_p(self.ptr), DEVICE))
+            except BaseException:
+                self.free()
+                raise
+        else:
+            self.words = np.zeros(self.nwords, np.uint64)
+            _check(lib().bitz_assignment_pack(_p(words), bit_len, columns, _p(self.words), 0))
+            self.ptr = self.words.ctypes.data
+
+    @classmethod
+    def from_bits(cls, bits, columns: int, device: bool = False) -> "PackedAssignment":
+        """From an array of 0/1 values (bit j of the assignment at index j)."""
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        b = np.packbits(bits, bitorder="little")
+        b = np.concatenate([b, np.zeros((-len(b)) % 8, np.uint8)])
+        return cls(b.view(np.uint64) if len(b) else np.zeros(1, np.uint64), len(bits), columns, device)
+
+    def to_table(self) -> np.ndarray:
+        """The {0, 1} elements build_assignment_mle would give (host)."""
+        words = self.words if not self.device else self._download()
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")[: self.len]
+        out = np.zeros((self.len, 2), np.uint64)
+        out[:, 0] = bits
+        return out
+
+    def _download(self) -> np.ndarray:
+        out = np.zeros(self.nwords, np.uint64)
+        _check(lib().bitz_memcpy_d2h(_p(out), C.c_void_p(self.ptr), self.nwords * 8))
+        return out
+
+    def free(self):
+        if self.device and self.ptr:
+            lib().bitz_device_free(C.c_void_p(self.ptr))
+        self.ptr = 0
 
     def __del__(self):
         try:
@@ -756,6 +829,28 @@ def prove_inner_sumcheck(transcript: Transcript, initial_claim, batched_matrix, 
                                to_int(evals[0]), to_int(evals[1]))
 
 
+def prove_inner_sumcheck_packed(transcript: Transcript, initial_claim, batched_matrix,
+                                assignment: PackedAssignment):
+    """prove_inner_sumcheck against a Boolean assignment given as packed bits
+    (bitz_prove_inner_sumcheck_packed): the same outputs as
+    prove_inner_sumcheck over assignment.to_table()."""
+    init()
+    d, nd, fd = _table(batched_matrix)
+    if bool(fd & DEVICE) != assignment.device:
+        raise BitzError(-7, "matrix and assignment must both be on the host or both on the device")
+    n = _nvars(nd)
+    rounds = np.zeros((max(n, 1) * 3, 2), dtype=np.uint64)
+    points = np.zeros((max(n, 1), 2), dtype=np.uint64)
+    final = F128()
+    evals = np.zeros((2, 2), dtype=np.uint64)
+    rc = lib().bitz_prove_inner_sumcheck_packed(transcript.h, f128(initial_claim), _p(d), n, assignment.ptr,
+                                                assignment.num_vars, fd, _p(rounds), _p(points), C.byref(final),
+                                                _p(evals))
+    _check(rc, SumcheckError)
+    return InnerSumcheckOutput(rounds[: 3 * n].reshape(n, 3, 2), points[:n], to_int(final),
+                               to_int(evals[0]), to_int(evals[1]))
+
+
 def prove_inner_sumcheck_with_challenges(challenges, initial_claim, batched_matrix, witness):
     """prove_inner_sumcheck with the challenges known in advance
     (bitz_prove_inner_sumcheck_with_challenges): rounds queued back to back."""
@@ -856,7 +951,10 @@ class PreparedConstraintMatrices:
         """Az, Bz, Cz = A h, B h, C h padded to 2^num_row_vars (the reference
         test's multiply + padded_mle, crates/spartan/src/piop.rs:352-372):
         host arrays for a host assignment, DeviceTables for a DeviceTable."""
-        h, n, fl = _table(assignment)
+        if isinstance(assignment, PackedAssignment):
+            h, n, fl = assignment, assignment.len, DEVICE if assignment.device else 0
+        else:
+            h, n, fl = _table(assignment)
         if n != 1 << self.num_column_vars:
             raise BitzError(-7, "the assignment must span the padded columns")
         rows = 1 << self.num_row_vars
@@ -866,7 +964,8 @@ class PreparedConstraintMatrices:
         else:
             outs = [np.zeros((rows, 2), dtype=np.uint64) for _ in range(3)]
             ptrs = [_p(o) for o in outs]
-        _check(lib().bitz_r1cs_products(self.h, _p(h), *ptrs, fl))
+        fn = lib().bitz_r1cs_products_packed if isinstance(h, PackedAssignment) else lib().bitz_r1cs_products
+        _check(fn(self.h, _p(h), *ptrs, fl))
         return R1csProductMles(*outs)
 
     def __del__(self):
@@ -947,19 +1046,23 @@ def prove_spartan_piop(transcript: Transcript, matrices: "PreparedConstraintMatr
     outer sumcheck, bind_and_batch and inner sumcheck."""
     init()
     (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
-    hv, nh, fh = _table(assignment)
+    packed = isinstance(assignment, PackedAssignment)
+    if packed:
+        hv, nh, fh = assignment, assignment.len, DEVICE if assignment.device else 0
+    else:
+        hv, nh, fh = _table(assignment)
     if fh != fa:
         raise BitzError(-7, "products and assignment must be on the same side")
     rv, cv = _nvars(na), _nvars(nh)
+    fn = lib().bitz_prove_spartan_piop_packed if packed else lib().bitz_prove_spartan_piop
     outer = np.zeros((max(rv, 1) * 4, 2), dtype=np.uint64)
     claims = np.zeros((3, 2), dtype=np.uint64)
     inner = np.zeros((max(cv, 1) * 3, 2), dtype=np.uint64)
     point = np.zeros((max(cv, 1), 2), dtype=np.uint64)
     meval, final = F128(), F128()
     dg = (C.c_uint8 * 32).from_buffer_copy(digest)
-    _check(lib().bitz_prove_spartan_piop(transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz), rv, _p(hv), cv, fa,
-                                         _p(outer), _p(claims), _p(inner), _p(point), C.byref(meval),
-                                         C.byref(final)), SumcheckError)
+    _check(fn(transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz), rv, _p(hv), cv, fa, _p(outer), _p(claims),
+              _p(inner), _p(point), C.byref(meval), C.byref(final)), SumcheckError)
     return SpartanPiopOutput(outer[: 4 * rv].reshape(rv, 4, 2), [to_int(c) for c in claims],
                              inner[: 3 * cv].reshape(cv, 3, 2), point[:cv], to_int(meval), to_int(final))
 
@@ -990,11 +1093,17 @@ class PiopEngine:
         return n.value
 
     def submit(self, tag: int, transcript: Transcript, matrices: "PreparedConstraintMatrices", digest: bytes,
-               products: R1csProductMles, assignment: DeviceTable) -> None:
+               products: R1csProductMles, assignment) -> None:
+        """assignment: a DeviceTable, or a PackedAssignment on the device."""
         (az, na, fa), (bz, _, _), (cz, _, _) = (_table(t) for t in (products.az, products.bz, products.cz))
-        hv, nh, fh = _table(assignment)
+        packed = isinstance(assignment, PackedAssignment)
+        if packed:
+            hv, nh, fh = assignment, assignment.len, DEVICE if assignment.device else 0
+        else:
+            hv, nh, fh = _table(assignment)
         if not (fa & DEVICE and fh & DEVICE):
             raise BitzError(-7, "the engine takes device tables")
+        fn = lib().bitz_piop_engine_submit_packed if packed else lib().bitz_piop_engine_submit
         dg = (C.c_uint8 * 32).from_buffer_copy(digest)
         # Held before the native submit: a background engine may finish the
         # proof (and another thread take it) before submit returns.  A
@@ -1002,8 +1111,8 @@ class PiopEngine:
         before = self._held.get(tag)
         self._held[tag] = (transcript, matrices, products, assignment, _nvars(na), _nvars(nh))
         try:
-            _check(lib().bitz_piop_engine_submit(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz),
-                                                 _p(cz), _nvars(na), _p(hv), _nvars(nh)), SumcheckError)
+            _check(fn(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz), _nvars(na), _p(hv),
+                      _nvars(nh)), SumcheckError)
         except BaseException:
             if before is None:
                 self._held.pop(tag, None)

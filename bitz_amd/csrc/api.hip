@@ -2348,6 +2348,15 @@ struct bitz_inner {
     Dev acc;
     Pinned host;
     Timed tm;
+    // Packed mode (bitz_inner_begin_packed): h is a Boolean assignment given
+    // as its packed bits.  Rounds 0 .. kb run the bit kernels (k_inner_bits:
+    // h_K in the monomial basis of r_0 .. r_{K-1}, sums multiplied by the
+    // monomials here), round kb also writes h_kb as elements, and the
+    // general kernels take the later rounds.
+    const uint64_t* bits = nullptr;
+    Dev in_bits, bacc;  // bacc: round K's sums (kInnerBitsAccWords words) at bacc + kInnerBitsAccWords K
+    int kb = -1;
+    std::vector<f128> r;  // the challenges so far
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
@@ -2367,12 +2376,74 @@ static void inner_set_attributes() {
     attr = true;
 }
 
-extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
-                                    const bitz_f128* witness, uint32_t witness_vars, uint32_t flags) {
-    if (!out || !matrix || !witness) return fail(BITZ_ERR_ARG, "null pointer");
-    if (witness_vars != matrix_vars)
-        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
-    if (matrix_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
+// r_S = prod_{i in S} r_i for every S < 2^K (bit i of S selects r_i).
+static void monomials(const f128* r, int K, f128* mono) {
+    mono[0] = hostf::one();
+    for (int i = 0; i < K; ++i)
+        for (int S = 0; S < (1 << i); ++S) mono[S | (1 << i)] = hostf::mul(mono[S], r[i]);
+}
+
+// Round K's [c0, c2] from a bit kernel's sums (the XOR of their copies): 2^K
+// sums for c0, then 2^K for c2, each weighed by its monomial r_S.  *live
+// (round 0) receives the largest live length the copies hold.
+static void bits_round_coeffs(const uint32_t* acc, int K, const f128* r, bitz_f128* co,
+                              unsigned long long* live = nullptr) {
+    const int S = 1 << K;
+    uint32_t w[8 << bitz::kInnerBitsMaxRound] = {};
+    unsigned long long lv = 0;
+    for (int c = 0; c < bitz::kInnerBitsReplicas; ++c) {
+        const uint32_t* rep = acc + (size_t)c * bitz::kInnerBitsReplicaWords;
+        for (int i = 0; i < 8 * S; ++i) w[i] ^= rep[i];
+        unsigned long long x;
+        memcpy(&x, rep + bitz::kInnerBitsLiveWord, 8);
+        lv = std::max(lv, x);
+    }
+    if (live) *live = lv;
+    f128 mono[1 << bitz::kInnerBitsMaxRound];
+    monomials(r, K, mono);
+    f128 c0{0, 0}, c2{0, 0};
+    for (int s = 0; s < S; ++s) {
+        c0 = hostf::add(c0, hostf::mul(mono[s], f128_at(w + 4 * s)));
+        c2 = hostf::add(c2, hostf::mul(mono[s], f128_at(w + 4 * (S + s))));
+    }
+    co[0] = B(c0);
+    co[1] = B(c2);
+}
+
+// Workgroups of a bit round over `pairs` output pairs.
+static unsigned bits_grid(size_t pairs) {
+    return (unsigned)std::max<size_t>(
+        1, std::min<size_t>((pairs + bitz::kInnerBlock - 1) / bitz::kInnerBlock, bitz::kInnerBitsMaxBlocks));
+}
+
+// Launches round K's bit kernel (one inner sumcheck, grid-stride).
+static int32_t launch_inner_bits(int K, const bitz::InnerBitsDesc& A, unsigned grid, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        const void* fns[] = {reinterpret_cast<const void*>(&bitz::k_inner_bits<1>),
+                             reinterpret_cast<const void*>(&bitz::k_inner_bits<2>),
+                             reinterpret_cast<const void*>(&bitz::k_inner_bits<3>),
+                             reinterpret_cast<const void*>(&bitz::k_inner_bits_batch<1>),
+                             reinterpret_cast<const void*>(&bitz::k_inner_bits_batch<2>),
+                             reinterpret_cast<const void*>(&bitz::k_inner_bits_batch<3>)};
+        for (const void* f : fns)
+            (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerBitsSmem);
+        attr = true;
+    }
+    const size_t smem = K ? bitz::kInnerBitsSmem : 0;
+    switch (K) {
+        case 0: hipLaunchKernelGGL(bitz::k_inner_bits<0>, dim3(grid), dim3(bitz::kInnerBlock), smem, s, A); break;
+        case 1: hipLaunchKernelGGL(bitz::k_inner_bits<1>, dim3(grid), dim3(bitz::kInnerBlock), smem, s, A); break;
+        case 2: hipLaunchKernelGGL(bitz::k_inner_bits<2>, dim3(grid), dim3(bitz::kInnerBlock), smem, s, A); break;
+        case 3: hipLaunchKernelGGL(bitz::k_inner_bits<3>, dim3(grid), dim3(bitz::kInnerBlock), smem, s, A); break;
+        default: return fail(BITZ_ERR_ARG, "no bit kernel for this round");
+    }
+    return check_launch();
+}
+
+// The handle over D and h (witness null: the packed mode sets h up).
+static int32_t inner_begin_impl(bitz_inner** out, const bitz_f128* matrix, const bitz_f128* witness,
+                                uint32_t matrix_vars, uint32_t flags) {
     auto* h = new bitz_inner;
     int32_t rc;
     auto cleanup = [&](int32_t c) {
@@ -2389,9 +2460,11 @@ extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, u
         h->cur[1] = reinterpret_cast<const f128*>(witness);
     } else {
         if ((rc = upload(h->in_d, matrix, N, flags, h->so.s))) return cleanup(rc);
-        if ((rc = upload(h->in_w, witness, N, flags, h->so.s))) return cleanup(rc);
         h->cur[0] = h->in_d.f();
-        h->cur[1] = h->in_w.f();
+        if (witness) {
+            if ((rc = upload(h->in_w, witness, N, flags, h->so.s))) return cleanup(rc);
+            h->cur[1] = h->in_w.f();
+        }
     }
     h->capP = std::max<size_t>(N / 2, 1);
     h->capQ = std::max<size_t>(N / 4, 1);
@@ -2400,8 +2473,57 @@ extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, u
     if ((rc = h->acc.alloc((size_t)(h->n + 2) * 8 * sizeof(uint32_t)))) return cleanup(rc);
     if (hipMemsetAsync(h->acc.p, 0, h->acc.bytes, h->so.s) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
-    if ((rc = h->host.alloc(64))) return cleanup(rc);
+    if ((rc = h->host.alloc(bitz::kInnerBitsAccWords * sizeof(uint32_t)))) return cleanup(rc);
     *out = h;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_inner_begin(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
+                                    const bitz_f128* witness, uint32_t witness_vars, uint32_t flags) {
+    if (!out || !matrix || !witness) return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    if (matrix_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
+    return inner_begin_impl(out, matrix, witness, matrix_vars, flags);
+}
+
+extern "C" int32_t bitz_inner_begin_packed(bitz_inner** out, const bitz_f128* matrix, uint32_t matrix_vars,
+                                           const uint64_t* bits, uint32_t assignment_vars, uint32_t flags) {
+    if (!out || !matrix || !bits) return fail(BITZ_ERR_ARG, "null pointer");
+    if (assignment_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and assignment have different num_vars");
+    if (matrix_vars > 34) return fail(BITZ_ERR_ARG, "num_vars too large");
+    TRY(inner_begin_impl(out, matrix, nullptr, matrix_vars, flags));
+    bitz_inner* h = *out;
+    auto cleanup = [&](int32_t c) {
+        bitz_inner_free(h);
+        *out = nullptr;
+        return c;
+    };
+    const size_t N = (size_t)1 << matrix_vars;
+    const size_t words = (N + 63) / 64;
+    int32_t rc;
+    h->bits = bits;
+    if (!(flags & BITZ_DEVICE)) {
+        if ((rc = h->in_bits.alloc(words * 8))) return cleanup(rc);
+        if (hipMemcpyAsync(h->in_bits.p, bits, words * 8, hipMemcpyHostToDevice, h->so.s) != hipSuccess)
+            return cleanup(fail(BITZ_ERR_DEVICE, "copy of the assignment bits failed"));
+        h->bits = static_cast<const uint64_t*>(h->in_bits.p);
+    }
+    if (matrix_vars <= 1) {
+        // One round at most: the assignment as elements and the general kernels.
+        if ((rc = h->in_w.alloc(N * sizeof(f128)))) return cleanup(rc);
+        hipLaunchKernelGGL(bitz::k_assignment_bits, dim3(1), dim3(kBlock), 0, h->so.s, h->bits, N, h->in_w.f(), N);
+        if ((rc = check_launch())) return cleanup(rc);
+        h->cur[1] = h->in_w.f();
+        h->bits = nullptr;
+        return BITZ_OK;
+    }
+    h->kb = std::min<int>(bitz::kInnerBitsMaxRound, (int)matrix_vars - 1);
+    if ((rc = h->bacc.alloc((size_t)(bitz::kInnerBitsMaxRound + 1) * bitz::kInnerBitsAccWords * sizeof(uint32_t))))
+        return cleanup(rc);
+    if (hipMemsetAsync(h->bacc.p, 0, h->bacc.bytes, h->so.s) != hipSuccess)
+        return cleanup(fail(BITZ_ERR_DEVICE, "memset failed"));
     return BITZ_OK;
 }
 
@@ -2427,9 +2549,29 @@ extern "C" int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* co) {
     const size_t pairs = h->len / 2;
     auto* live_word = reinterpret_cast<unsigned long long*>(h->acc.u32() + 8 * (h->n + 1));
     inner_set_attributes();
+    if (h->bits) {
+        // Round 0 over the bits: D read in full, [c0, c2] by masked XORs.
+        bitz::InnerBitsDesc A{};
+        A.d = h->cur[0];
+        A.bits = h->bits;
+        A.pairs = pairs;
+        A.live = h->len;
+        A.acc = h->bacc.u32();
+        A.blocks = bits_grid(pairs);
+        h->tm.begin(h->so.s, "inner_round0", 16.0 * (double)h->len + (double)h->len / 8);
+        TRY(launch_inner_bits(0, A, A.blocks, h->so.s));
+        h->tm.end(h->so.s);
+        HIP_TRY(hipMemcpyAsync(h->host.p, h->bacc.p, bitz::kInnerBitsAccWords * 4, hipMemcpyDeviceToHost, h->so.s));
+        HIP_TRY(hipStreamSynchronize(h->so.s));
+        h->tm.collect();
+        unsigned long long live;
+        bits_round_coeffs(static_cast<const uint32_t*>(h->host.p), 0, nullptr, co, &live);
+        h->live = std::max<size_t>(1, std::min<size_t>((size_t)live, h->len));
+        return BITZ_OK;
+    }
     h->tm.begin(h->so.s, "inner_round0", 32.0 * (double)h->len);
-    hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(inner_grid(pairs)), dim3(bitz::kInnerBlock), bitz::kInnerGridSmem,
-                       h->so.s, h->cur[0], h->cur[1], pairs, h->acc.u32(), live_word);
+    hipLaunchKernelGGL(bitz::k_inner_r0_lane, dim3(inner_grid(pairs)), dim3(bitz::kInnerBlock),
+                       bitz::kInnerGridSmem, h->so.s, h->cur[0], h->cur[1], pairs, h->acc.u32(), live_word);
     h->tm.end(h->so.s);
     TRY(check_launch());
     HIP_TRY(hipMemcpyAsync(static_cast<char*>(h->host.p) + 32, live_word, 8, hipMemcpyDeviceToHost, h->so.s));
@@ -2450,6 +2592,40 @@ extern "C" int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 rb, bitz_f128* 
     const size_t cap = (h->where == 1) ? h->capQ : h->capP;
     f128* out[2] = {dst.f(), dst.f() + cap};
     bool has_next = false;
+    const int K = h->round + 1;  // the round whose sums this call computes
+    if (h->bits && K <= h->kb) {
+        // Round K over the bits: D folded by r into out[0]; the last such
+        // round also writes h_K as elements into out[1].
+        h->r.push_back(r);
+        if (!h->rtab.p) TRY(h->rtab.alloc(kConstTable4Entries * sizeof(uint4)));
+        hipLaunchKernelGGL(bitz::k_const_table4_1, dim3(1), dim3(1024), 0, s, r, static_cast<uint4*>(h->rtab.p));
+        bitz::InnerBitsDesc A{};
+        A.d = h->cur[0];
+        A.bits = h->bits;
+        A.dout = out[0];
+        A.wout = K == h->kb ? out[1] : nullptr;
+        A.pairs = ((h->live + 1) / 2 + 1) / 2;  // over the live prefix
+        A.live = h->live;
+        A.rtab4 = static_cast<const uint4*>(h->rtab.p);
+        A.acc = h->bacc.u32() + bitz::kInnerBitsAccWords * K;
+        A.blocks = bits_grid(A.pairs);
+        if (A.wout) monomials(h->r.data(), K, A.mono);
+        h->tm.begin(s, "inner_fold_next", 24.0 * (double)h->len + (A.wout ? 8.0 * (double)h->len : 0.0));
+        TRY(launch_inner_bits(K, A, A.blocks, s));
+        h->tm.end(s);
+        HIP_TRY(hipMemcpyAsync(h->host.p, A.acc, bitz::kInnerBitsAccWords * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        h->tm.collect();
+        bits_round_coeffs(static_cast<const uint32_t*>(h->host.p), K, h->r.data(), co);
+        h->cur[0] = out[0];
+        h->cur[1] = A.wout ? out[1] : nullptr;
+        if (A.wout) h->bits = nullptr;  // h_kb is in cur[1]: the general kernels from here
+        h->where = (h->where == 1) ? 2 : 1;
+        h->len = next_len;
+        h->live = (h->live + 1) / 2;
+        h->round += 1;
+        return BITZ_OK;
+    }
     if (next_len == 1) {
         hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[0], out[0], (size_t)1, r);
         hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, h->cur[1], out[1], (size_t)1, r);
@@ -2493,22 +2669,16 @@ extern "C" void bitz_inner_free(bitz_inner* h) {
     }
 }
 
-// The transcript-driven inner sumcheck through the per-round handle.
-static int32_t prove_inner_impl(bitz_transcript* t, bitz_f128 initial_claim, const bitz_f128* matrix,
-                                uint32_t matrix_vars, const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
-                                bitz_f128* round_polys, bitz_f128* eval_points, bitz_f128* final_claim,
-                                bitz_f128* evals) {
-    if (!t || !final_claim || !evals || (matrix_vars && (!round_polys || !eval_points)))
-        return fail(BITZ_ERR_ARG, "null pointer");
-    if (witness_vars != matrix_vars)
-        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
-    bitz_inner* h = nullptr;
-    TRY(bitz_inner_begin(&h, matrix, matrix_vars, witness, witness_vars, flags));
+// The transcript-driven inner sumcheck through the per-round handle (begun
+// by the caller, freed here).
+static int32_t prove_inner_rounds(bitz_inner* h, bitz_transcript* t, bitz_f128 initial_claim, uint32_t n,
+                                  bitz_f128* round_polys, bitz_f128* eval_points, bitz_f128* final_claim,
+                                  bitz_f128* evals) {
     int32_t rc = BITZ_OK;
     bitz_f128 co[2];
     f128 claim = F(initial_claim);
     if ((rc = bitz_inner_round0(h, co))) goto done;
-    for (uint32_t k = 0; k < matrix_vars; ++k) {
+    for (uint32_t k = 0; k < n; ++k) {
         const f128 r = recover_round(t->t, claim, co, 2, round_polys + 3 * k, eval_points + k);
         if ((rc = bitz_inner_fold_next(h, B(r), co))) goto done;
     }
@@ -2519,6 +2689,19 @@ done:
     return rc;
 }
 
+static int32_t prove_inner_impl(bitz_transcript* t, bitz_f128 initial_claim, const bitz_f128* matrix,
+                                uint32_t matrix_vars, const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
+                                bitz_f128* round_polys, bitz_f128* eval_points, bitz_f128* final_claim,
+                                bitz_f128* evals) {
+    if (!t || !final_claim || !evals || (matrix_vars && (!round_polys || !eval_points)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    if (witness_vars != matrix_vars)
+        return fail(BITZ_ERR_PRODUCT_DIMS, "matrix and witness have different num_vars");
+    bitz_inner* h = nullptr;
+    TRY(bitz_inner_begin(&h, matrix, matrix_vars, witness, witness_vars, flags));
+    return prove_inner_rounds(h, t, initial_claim, matrix_vars, round_polys, eval_points, final_claim, evals);
+}
+
 extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
                                              const bitz_f128* matrix, uint32_t matrix_vars,
                                              const bitz_f128* witness, uint32_t witness_vars,
@@ -2527,6 +2710,18 @@ extern "C" int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initi
                                              bitz_f128* evals) {
     return prove_inner_impl(t, initial_claim, matrix, matrix_vars, witness, witness_vars, flags, round_polys,
                             eval_points, final_claim, evals);
+}
+
+extern "C" int32_t bitz_prove_inner_sumcheck_packed(bitz_transcript* t, bitz_f128 initial_claim,
+                                                    const bitz_f128* matrix, uint32_t matrix_vars,
+                                                    const uint64_t* assignment_bits, uint32_t assignment_vars,
+                                                    uint32_t flags, bitz_f128* round_polys, bitz_f128* eval_points,
+                                                    bitz_f128* final_claim, bitz_f128* evals) {
+    if (!t || !final_claim || !evals || (matrix_vars && (!round_polys || !eval_points)))
+        return fail(BITZ_ERR_ARG, "null pointer");
+    bitz_inner* h = nullptr;
+    TRY(bitz_inner_begin_packed(&h, matrix, matrix_vars, assignment_bits, assignment_vars, flags));
+    return prove_inner_rounds(h, t, initial_claim, matrix_vars, round_polys, eval_points, final_claim, evals);
 }
 
 // Device buffers of the queued inner prover: the inputs, the ping-pong
@@ -3072,6 +3267,35 @@ extern "C" int32_t bitz_r1cs_products(bitz_matrices* h, const bitz_f128* assignm
     return BITZ_OK;
 }
 
+extern "C" int32_t bitz_r1cs_products_packed(bitz_matrices* h, const uint64_t* assignment_bits, bitz_f128* az,
+                                             bitz_f128* bz, bitz_f128* cz, uint32_t flags) {
+    if (!h || !assignment_bits || !az || !bz || !cz) return fail(BITZ_ERR_ARG, "null pointer");
+    std::lock_guard<std::mutex> lock(h->mu);
+    hipStream_t s = h->so.s;
+    Dev din, dout;
+    const uint64_t* bits = assignment_bits;
+    if (!(flags & BITZ_DEVICE)) {
+        const size_t words = (h->cols + 63) / 64;
+        TRY(din.alloc(words * 8));
+        HIP_TRY(hipMemcpyAsync(din.p, assignment_bits, words * 8, hipMemcpyHostToDevice, s));
+        bits = static_cast<const uint64_t*>(din.p);
+        TRY(dout.alloc(3 * h->rows_padded * sizeof(f128)));
+    }
+    bitz_f128* outs[3] = {az, bz, cz};
+    for (int m = 0; m < 3; ++m) {
+        f128* target = (flags & BITZ_DEVICE) ? reinterpret_cast<f128*>(outs[m]) : dout.f() + m * h->rows_padded;
+        hipLaunchKernelGGL(bitz::k_r1cs_rows_bits, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s,
+                           h->rs[m].u32(), h->cs[m].u32(), h->vs[m].f(), bits, h->rows, target, h->rows_padded);
+        TRY(check_launch());
+    }
+    if (!(flags & BITZ_DEVICE))
+        for (int m = 0; m < 3; ++m)
+            HIP_TRY(hipMemcpyAsync(outs[m], dout.f() + m * h->rows_padded, h->rows_padded * sizeof(f128),
+                                   hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BITZ_OK;
+}
+
 extern "C" int32_t bitz_assignment_mle(const uint64_t* words, size_t bit_len, size_t columns, bitz_f128* out,
                                        uint32_t flags) {
     if (!out || (bit_len && !words)) return fail(BITZ_ERR_ARG, "null pointer");
@@ -3095,6 +3319,28 @@ extern "C" int32_t bitz_assignment_mle(const uint64_t* words, size_t bit_len, si
     TRY(check_launch());
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, padded * sizeof(f128), hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipStreamSynchronize(so.s));
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_assignment_pack(const uint64_t* words, size_t bit_len, size_t columns, uint64_t* out,
+                                        uint32_t flags) {
+    if (!out || (bit_len && !words)) return fail(BITZ_ERR_ARG, "null pointer");
+    if (bit_len != columns) return fail(BITZ_ERR_ASSIGNMENT_LEN, "assignment length differs from the columns");
+    if (columns == 0 || !(words[0] & 1)) return fail(BITZ_ERR_ASSIGNMENT_CONST, "the first assignment bit must be one");
+    size_t padded = 1;
+    while (padded < columns) padded <<= 1;
+    const size_t nw = (padded + 63) / 64, used = (bit_len + 63) / 64;
+    std::vector<uint64_t> packed(nw, 0);
+    memcpy(packed.data(), words, used * 8);
+    if (bit_len % 64) packed[used - 1] &= (~0ull) >> (64 - bit_len % 64);  // zero past the assignment
+    if (!(flags & BITZ_DEVICE)) {
+        memcpy(out, packed.data(), nw * 8);
+        return BITZ_OK;
+    }
+    StreamOwner so;  // non-blocking: the copy does not wait for other streams
+    TRY(so.create());
+    HIP_TRY(hipMemcpyAsync(out, packed.data(), nw * 8, hipMemcpyHostToDevice, so.s));
     HIP_TRY(hipStreamSynchronize(so.s));
     return BITZ_OK;
 }
@@ -3150,13 +3396,15 @@ extern "C" int32_t bitz_constraint_matrix_digest(size_t m_rows, size_t m_columns
     return BITZ_OK;
 }
 
-extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
-                                           const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
-                                           uint32_t row_vars, const bitz_f128* assignment, uint32_t col_vars,
-                                           uint32_t flags, bitz_f128* outer_rounds, bitz_f128* outer_claims,
-                                           bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
-                                           bitz_f128* final_claim) {
-    if (!t || !h || !digest || !az || !bz || !cz || !assignment || !outer_claims || !matrix_eval || !final_claim ||
+// prove_spartan_piop over an assignment given as elements or (bits non-null)
+// as its packed bits.
+static int32_t piop_impl(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32], const bitz_f128* az,
+                         const bitz_f128* bz, const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
+                         const uint64_t* bits, uint32_t col_vars, uint32_t flags, bitz_f128* outer_rounds,
+                         bitz_f128* outer_claims, bitz_f128* inner_rounds, bitz_f128* inner_point,
+                         bitz_f128* matrix_eval, bitz_f128* final_claim) {
+    if (!t || !h || !digest || !az || !bz || !cz || !(assignment || bits) || !outer_claims || !matrix_eval ||
+        !final_claim ||
         (row_vars && !outer_rounds) || (col_vars && (!inner_rounds || !inner_point)))
         return fail(BITZ_ERR_ARG, "null pointer");
     // piop.rs:74-83: dimensions before any transcript call.
@@ -3209,17 +3457,57 @@ extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h,
     TRY(bitz_bind_and_batch(h, reinterpret_cast<const bitz_f128*>(drx.p), row_vars, B(rho),
                             reinterpret_cast<bitz_f128*>(dD.p), BITZ_DEVICE));
     // piop.rs:99: the inner sumcheck against the assignment.
-    const bitz_f128* hv = assignment;
-    if (!dev) {
-        TRY(upload(dh, assignment, h->cols, 0, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        hv = reinterpret_cast<const bitz_f128*>(dh.p);
-    }
     bitz_f128 evals[2];
-    TRY(prove_inner_impl(t, B(inner_claim), reinterpret_cast<const bitz_f128*>(dD.p), col_vars, hv, col_vars,
-                         BITZ_DEVICE, inner_rounds, inner_point, final_claim, evals));
+    if (bits) {
+        // The bits' own placement (flags) and D on the device.
+        bitz_inner* ih = nullptr;
+        Dev dbits;
+        const uint64_t* bv = bits;
+        if (!dev) {
+            const size_t words = (h->cols + 63) / 64;
+            TRY(dbits.alloc(words * 8));
+            HIP_TRY(hipMemcpyAsync(dbits.p, bits, words * 8, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            bv = static_cast<const uint64_t*>(dbits.p);
+        }
+        TRY(bitz_inner_begin_packed(&ih, reinterpret_cast<const bitz_f128*>(dD.p), col_vars, bv, col_vars,
+                                    BITZ_DEVICE));
+        TRY(prove_inner_rounds(ih, t, B(inner_claim), col_vars, inner_rounds, inner_point, final_claim, evals));
+    } else {
+        const bitz_f128* hv = assignment;
+        if (!dev) {
+            TRY(upload(dh, assignment, h->cols, 0, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            hv = reinterpret_cast<const bitz_f128*>(dh.p);
+        }
+        TRY(prove_inner_impl(t, B(inner_claim), reinterpret_cast<const bitz_f128*>(dD.p), col_vars, hv, col_vars,
+                             BITZ_DEVICE, inner_rounds, inner_point, final_claim, evals));
+    }
     *matrix_eval = evals[0];  // ScaledMleEvaluationClaim (piop.rs:101-105): D(r_y) scales h(r_y)
     return BITZ_OK;
+}
+
+extern "C" int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
+                                           const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
+                                           uint32_t row_vars, const bitz_f128* assignment, uint32_t col_vars,
+                                           uint32_t flags, bitz_f128* outer_rounds, bitz_f128* outer_claims,
+                                           bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
+                                           bitz_f128* final_claim) {
+    if (!assignment) return fail(BITZ_ERR_ARG, "null pointer");
+    return piop_impl(t, h, digest, az, bz, cz, row_vars, assignment, nullptr, col_vars, flags, outer_rounds,
+                     outer_claims, inner_rounds, inner_point, matrix_eval, final_claim);
+}
+
+extern "C" int32_t bitz_prove_spartan_piop_packed(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
+                                                  const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
+                                                  uint32_t row_vars, const uint64_t* assignment_bits,
+                                                  uint32_t col_vars, uint32_t flags, bitz_f128* outer_rounds,
+                                                  bitz_f128* outer_claims, bitz_f128* inner_rounds,
+                                                  bitz_f128* inner_point, bitz_f128* matrix_eval,
+                                                  bitz_f128* final_claim) {
+    if (!assignment_bits) return fail(BITZ_ERR_ARG, "null pointer");
+    return piop_impl(t, h, digest, az, bz, cz, row_vars, nullptr, assignment_bits, col_vars, flags, outer_rounds,
+                     outer_claims, inner_rounds, inner_point, matrix_eval, final_claim);
 }
 
 extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
@@ -3237,11 +3525,19 @@ namespace {
 
 enum class JobPhase { kOuter, kOuterFinal, kBind, kInner, kInnerFinal, kDone };
 
+// Words of a proof's slot in a step's read-back (a bit round's take
+// kInnerBitsAccWords): round sums, round 0's live length at words 8-9.
+constexpr size_t kEngineSlotWords = 16;
+
 struct PiopJob {
     uint64_t tag = 0;
     bitz_transcript* t = nullptr;
     bitz_matrices* m = nullptr;
     const f128* hv = nullptr;  // the assignment (device)
+    // Or the assignment as packed bits (device): inner rounds 0 .. kb run the
+    // bit kernels, round kb writes h_kb as elements (see bitz_inner).
+    const uint64_t* hbits = nullptr;
+    int kb = -1;
     int n = 0, nc = 0;
     JobPhase phase = JobPhase::kOuter;
     int round = 0;  // next round to launch in the current sumcheck
@@ -3281,7 +3577,8 @@ struct bitz_piop_engine {
     std::string worker_err;
     // HIP events around the step's batched inner fold and around the whole
     // step's device work (when profiling), read once the step has synchronised.
-    Timed t_inner_fold, t_step;
+    Timed t_inner_fold, t_step, t_bits[bitz::kInnerBitsMaxRound + 1];
+    std::vector<size_t> slot_off;  // word offset of each proof's slot in the step in flight (and the total)
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -3310,11 +3607,12 @@ extern "C" int32_t bitz_piop_engine_create(bitz_piop_engine** out) {
 
 extern "C" void bitz_piop_engine_free(bitz_piop_engine* e) { delete e; }
 
-extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
-                                           const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
-                                           const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
-                                           uint32_t col_vars) {
-    if (!e || !t || !m || !digest || !az || !bz || !cz || !assignment) return fail(BITZ_ERR_ARG, "null pointer");
+static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                             const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
+                             uint32_t row_vars, const bitz_f128* assignment, const uint64_t* bits,
+                             uint32_t col_vars) {
+    if (!e || !t || !m || !digest || !az || !bz || !cz || !(assignment || bits))
+        return fail(BITZ_ERR_ARG, "null pointer");
     // piop.rs:74-83: dimensions before any transcript call.
     if ((int)row_vars != m->row_vars) return fail(BITZ_ERR_PRODUCT_DIMS, "products do not span the row domain");
     if ((int)col_vars != m->col_vars) return fail(BITZ_ERR_ASSIGNMENT_DIMS, "assignment does not span the columns");
@@ -3333,6 +3631,10 @@ extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bi
     j->hv = reinterpret_cast<const f128*>(assignment);
     j->n = (int)row_vars;
     j->nc = (int)col_vars;
+    if (bits) {
+        j->hbits = bits;
+        j->kb = std::min<int>(bitz::kInnerBitsMaxRound, (int)col_vars - 1);
+    }
     hipStream_t s = e->so.s;
     // piop.rs:85-90: digest, tau, make_equality_factors (crates/poly/src/eq.rs:107-115).
     t->t.absorb(0, digest, 32);
@@ -3385,6 +3687,29 @@ extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bi
     return BITZ_OK;
 }
 
+extern "C" int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                                           const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
+                                           const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
+                                           uint32_t col_vars) {
+    if (!assignment) return fail(BITZ_ERR_ARG, "null pointer");
+    return engine_submit(e, tag, t, m, digest, az, bz, cz, row_vars, assignment, nullptr, col_vars);
+}
+
+extern "C" int32_t bitz_piop_engine_submit_packed(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t,
+                                                  bitz_matrices* m, const uint8_t digest[32], const bitz_f128* az,
+                                                  const bitz_f128* bz, const bitz_f128* cz, uint32_t row_vars,
+                                                  const uint64_t* assignment_bits, uint32_t col_vars) {
+    if (!assignment_bits) return fail(BITZ_ERR_ARG, "null pointer");
+    return engine_submit(e, tag, t, m, digest, az, bz, cz, row_vars, nullptr, assignment_bits, col_vars);
+}
+
+// Inner round K's challenges r_0 .. r_{K-1} as elements.
+static std::vector<f128> job_inner_point(const PiopJob* j, int K) {
+    std::vector<f128> r((size_t)std::max(K, 1));
+    for (int i = 0; i < K; ++i) r[i] = F(j->inner_point[i]);
+    return r;
+}
+
 extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched) {
     if (!e) return fail(BITZ_ERR_ARG, "null pointer");
     if (e->running && std::this_thread::get_id() != e->worker.get_id())
@@ -3397,10 +3722,18 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
         if (j->phase != JobPhase::kDone) act.push_back(j.get());
     if (launched) *launched = act.size();
     if (act.empty()) return BITZ_OK;
-    constexpr size_t kSlotWords = 16;
-    const size_t slot_bytes = act.size() * kSlotWords * 4;
+    // Each proof's slot for the step's sums: a bit round's copies
+    // (kInnerBitsAccWords), else kEngineSlotWords words.
+    e->slot_off.assign(act.size() + 1, 0);
+    for (size_t i = 0; i < act.size(); ++i) {
+        const PiopJob* j = act[i];
+        const bool bits = j->hbits && ((j->phase == JobPhase::kBind) ||
+                                       (j->phase == JobPhase::kInner && j->round <= j->kb));
+        e->slot_off[i + 1] = e->slot_off[i] + (bits ? bitz::kInnerBitsAccWords : kEngineSlotWords);
+    }
+    const size_t slot_bytes = e->slot_off[act.size()] * 4;
     if (e->slots.bytes < slot_bytes) TRY(e->slots.alloc(slot_bytes));
-    auto slot = [&](size_t i) { return e->slots.u32() + kSlotWords * i; };
+    auto slot = [&](size_t i) { return e->slots.u32() + e->slot_off[i]; };
     // Challenge tables to build: the r each fold round folds by.
     std::vector<f128> consts;
     std::vector<int> table_of(act.size(), -1);
@@ -3421,6 +3754,16 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     std::vector<uint32_t> o_r0_map, o_fold_map;
     std::vector<bitz::InnerRoundDesc> i_r0, i_fold;
     std::vector<uint32_t> i_r0_map, i_fold_map;
+    // Inner rounds over packed bits, by round index K.
+    constexpr int kBitRounds = bitz::kInnerBitsMaxRound + 1;
+    std::vector<bitz::InnerBitsDesc> i_bits[kBitRounds];
+    std::vector<uint32_t> i_bits_map[kBitRounds];
+    auto add_bits = [&](int K, bitz::InnerBitsDesc d) {
+        d.blocks = bits_grid(d.pairs);
+        d.block_base = (uint32_t)i_bits_map[K].size();
+        i_bits_map[K].insert(i_bits_map[K].end(), d.blocks, (uint32_t)i_bits[K].size());
+        i_bits[K].push_back(d);
+    };
     std::vector<size_t> bind_jobs;
     for (size_t i = 0; i < act.size(); ++i) {
         PiopJob* j = act[i];
@@ -3461,6 +3804,43 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
             d.block_base = (uint32_t)map.size();
             map.insert(map.end(), (g.threads + block - 1) / block, (uint32_t)descs.size());
             descs.push_back(d);
+        } else if (j->phase == JobPhase::kBind && j->hbits) {
+            bind_jobs.push_back(i);
+            bitz::InnerBitsDesc d{};
+            d.d = j->dD.f();
+            d.bits = j->hbits;
+            d.pairs = j->ilen / 2;
+            d.live = j->ilen;
+            d.acc = slot(i);
+            j->pend[0] = nullptr;
+            j->pend[1] = nullptr;
+            if (j->kb == 0) {  // one round: h as elements for the final fold
+                d.wout = j->iP.f() + j->icapP;
+                d.mono[0] = hostf::one();
+                j->pend[1] = d.wout;
+            }
+            add_bits(0, d);
+        } else if (j->phase == JobPhase::kInner && j->hbits && j->round <= j->kb) {
+            const int K = j->round;
+            const size_t next_len = j->ilen / 2;
+            Dev& dst = (j->iwhere == 1) ? j->iQ : j->iP;
+            const size_t cap = (j->iwhere == 1) ? j->icapQ : j->icapP;
+            j->pend[0] = dst.f();
+            j->pend[1] = K == j->kb ? dst.f() + cap : nullptr;
+            bitz::InnerBitsDesc d{};
+            d.d = j->icur[0];
+            d.bits = j->hbits;
+            d.dout = j->pend[0];
+            d.wout = j->pend[1];
+            d.pairs = std::min(next_len / 2, ((j->ilive + 1) / 2 + 1) / 2);  // the folded live prefix, in pairs
+            d.live = j->ilive;
+            d.rtab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
+            d.acc = slot(i);
+            if (d.wout) {
+                const std::vector<f128> r = job_inner_point(j, K);
+                monomials(r.data(), K, d.mono);
+            }
+            add_bits(K, d);
         } else if (j->phase == JobPhase::kBind) {
             bind_jobs.push_back(i);
             bitz::InnerRoundDesc d{};
@@ -3510,6 +3890,11 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     const size_t of_o = place(o_fold.size() * sizeof(bitz::RoundDesc)), ofm_o = place(o_fold_map.size() * 4);
     const size_t ir_o = place(i_r0.size() * sizeof(bitz::InnerRoundDesc)), irm_o = place(i_r0_map.size() * 4);
     const size_t if_o = place(i_fold.size() * sizeof(bitz::InnerRoundDesc)), ifm_o = place(i_fold_map.size() * 4);
+    size_t ib_o[kBitRounds], ibm_o[kBitRounds];
+    for (int K = 0; K < kBitRounds; ++K) {
+        ib_o[K] = place(i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
+        ibm_o[K] = place(i_bits_map[K].size() * 4);
+    }
     // One pinned buffer stages the step's uploads, then receives its sums.
     const size_t total = std::max<size_t>(off, 256);
     const size_t need = std::max(total, slot_bytes);
@@ -3538,6 +3923,10 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     memcpy(pin + irm_o, i_r0_map.data(), i_r0_map.size() * 4);
     memcpy(pin + if_o, i_fold.data(), i_fold.size() * sizeof(bitz::InnerRoundDesc));
     memcpy(pin + ifm_o, i_fold_map.data(), i_fold_map.size() * 4);
+    for (int K = 0; K < kBitRounds; ++K) {
+        memcpy(pin + ib_o[K], i_bits[K].data(), i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
+        memcpy(pin + ibm_o[K], i_bits_map[K].data(), i_bits_map[K].size() * 4);
+    }
     e->t_step.begin(s, "piop_engine_step", (double)act.size());  // "bytes": proofs in the step
     HIP_TRY(hipMemcpyAsync(dst, pin, off, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(e->slots.p, 0, slot_bytes, s));
@@ -3588,6 +3977,29 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
         hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)i_r0_map.size()), dim3(bitz::kInnerBlock),
                            bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + ir_o),
                            reinterpret_cast<const uint32_t*>(dst + irm_o));
+    for (int K = 0; K < kBitRounds; ++K) {
+        if (i_bits[K].empty()) continue;
+        const auto* dd = reinterpret_cast<const bitz::InnerBitsDesc*>(dst + ib_o[K]);
+        const auto* mm = reinterpret_cast<const uint32_t*>(dst + ibm_o[K]);
+        const dim3 grid((unsigned)i_bits_map[K].size()), block(bitz::kInnerBlock);
+        const size_t smem = K ? bitz::kInnerBitsSmem : 0;
+        // Algorithmic bytes: round 0 reads D (32 B per pair) and the bits;
+        // round K >= 1 reads 4 entries of D_{K-1} and writes 2 of D_K per
+        // output pair (and 2 of h_K when it writes them).
+        double bits_bytes = 0;
+        for (const auto& d : i_bits[K])
+            bits_bytes += (K ? 96.0 + (d.wout ? 32.0 : 0.0) : 32.0) * (double)d.pairs + (double)d.pairs * (2 << K) / 8;
+        static const char* const kNames[kBitRounds] = {"inner_bits_batch_0", "inner_bits_batch_1",
+                                                       "inner_bits_batch_2", "inner_bits_batch_3"};
+        e->t_bits[K].begin(s, kNames[K], bits_bytes);
+        switch (K) {
+            case 0: hipLaunchKernelGGL(bitz::k_inner_bits_batch<0>, grid, block, smem, s, dd, mm); break;
+            case 1: hipLaunchKernelGGL(bitz::k_inner_bits_batch<1>, grid, block, smem, s, dd, mm); break;
+            case 2: hipLaunchKernelGGL(bitz::k_inner_bits_batch<2>, grid, block, smem, s, dd, mm); break;
+            default: hipLaunchKernelGGL(bitz::k_inner_bits_batch<3>, grid, block, smem, s, dd, mm); break;
+        }
+        e->t_bits[K].end(s);
+    }
     if (!i_fold.empty()) {
         // Algorithmic bytes: per output pair, 4 + 4 inputs of D and h read and
         // 2 + 2 folded entries written.
@@ -3609,7 +4021,6 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
 
 extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_flight) {
     if (!e) return fail(BITZ_ERR_ARG, "null pointer");
-    constexpr size_t kSlotWords = 16;
     std::vector<PiopJob*> act = std::move(e->launched);
     e->launched.clear();
     const auto wait_t0 = std::chrono::steady_clock::now();
@@ -3618,11 +4029,12 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
     if (!act.empty()) prof_add_host("piop_engine_wait", wait_t0);
     e->t_inner_fold.collect();
     e->t_step.collect();
+    for (auto& t : e->t_bits) t.collect();
     const char* pin = static_cast<const char*>(e->pin);
     // Host: every proof's round polynomial into its transcript, its next challenge.
     for (size_t i = 0; i < act.size(); ++i) {
         PiopJob* j = act[i];
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(pin) + kSlotWords * i;
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(pin) + e->slot_off[i];
         Transcript& tr = j->t->t;
         switch (j->phase) {
             case JobPhase::kOuter: {
@@ -3673,21 +4085,29 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
             case JobPhase::kBind:
             case JobPhase::kInner: {
                 const int k = j->round;
+                const bool bit_round = j->hbits && k <= j->kb;
                 if (k > 0) {
                     j->icur[0] = j->pend[0];
                     j->icur[1] = j->pend[1];
                     j->iwhere = (j->iwhere == 1) ? 2 : 1;
                     j->ilen /= 2;
                     j->ilive = (j->ilive + 1) / 2;
-                } else {
-                    // Round 0's live length (1 + the last index where D or h is
-                    // nonzero): the fold rounds skip the zero tail.
-                    unsigned long long live;
-                    memcpy(&live, w + 8, 8);
-                    j->ilive = std::max<size_t>(1, std::min<size_t>((size_t)live, j->ilen));
                 }
                 bitz_f128 co[2];
-                memcpy(co, w, 32);
+                unsigned long long live = 0;
+                if (bit_round) {
+                    const std::vector<f128> r = job_inner_point(j, k);
+                    bits_round_coeffs(w, k, r.data(), co, &live);
+                } else {
+                    memcpy(co, w, 32);
+                    memcpy(&live, w + 8, 8);
+                }
+                if (k == 0) {
+                    // Round 0's live length (1 + the last index where D or h is
+                    // nonzero): the fold rounds skip the zero tail.
+                    j->ilive = std::max<size_t>(1, std::min<size_t>((size_t)live, j->ilen));
+                    if (j->hbits) j->icur[1] = j->pend[1];  // h as elements if round 0 was the last bit round
+                }
                 j->r = recover_round(tr, j->claim, co, 2, j->inner_rounds.data() + 3 * k, j->inner_point.data() + k);
                 j->round = k + 1;
                 j->phase = j->round == j->nc ? JobPhase::kInnerFinal : JobPhase::kInner;

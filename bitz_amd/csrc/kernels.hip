@@ -77,6 +77,49 @@ __device__ __forceinline__ void block_xor_store(g4 (&v)[K], uint32_t* dst, int n
     }
 }
 
+// XOR of W words per thread over the workgroup, XORed into dst[0 .. W) with
+// one atomic per word.  Inside a wave, each butterfly step sends half of the
+// words a lane still carries to its partner and keeps the other half, so W
+// words take W - 1 shuffles in all (instead of 6 W): after log2 W steps lane
+// l holds the wave's XOR of one word, the index its lane bits chose, and the
+// remaining offsets reduce that word over the lanes that share it.
+// W a power of two <= 64; whole waves.
+template <int C, int O, int W>
+__device__ __forceinline__ void xor_scatter_step(uint32_t (&x)[W], int lane, int& idx) {
+    if constexpr (C > 1) {
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for (int i = 0; i < C / 2; ++i) {
+            const uint32_t send = up ? x[i] : x[i + C / 2];
+            const uint32_t keep = up ? x[i + C / 2] : x[i];
+            x[i] = keep ^ (uint32_t)__shfl_xor((int)send, O, 64);
+        }
+        if (up) idx += C / 2;
+        xor_scatter_step<C / 2, O / 2, W>(x, lane, idx);
+    } else {
+#pragma unroll
+        for (int o = O; o >= 1; o >>= 1) x[0] ^= (uint32_t)__shfl_xor((int)x[0], o, 64);
+    }
+}
+template <int W>
+__device__ __forceinline__ void block_xor_words(uint32_t (&x)[W], uint32_t* dst) {
+    static_assert(W >= 1 && W <= 64 && (W & (W - 1)) == 0, "power of two up to 64 words");
+    __shared__ uint32_t red[1024 / 64][W];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int idx = 0;
+    xor_scatter_step<W, 32, W>(x, lane, idx);
+    if ((lane & (64 / W - 1)) == 0) red[wave][idx] = x[0];
+    __syncthreads();
+    const int waves = (int)blockDim.x >> 6;
+    for (int j = threadIdx.x; j < W; j += blockDim.x) {
+        uint32_t v = 0;
+        for (int w = 0; w < waves; ++w) v ^= red[w][j];
+        if (v)
+            __scoped_atomic_fetch_xor((__attribute__((address_space(1))) uint32_t*)(dst + j), v, __ATOMIC_RELAXED,
+                                      __MEMORY_SCOPE_DEVICE);
+    }
+}
+
 __device__ __forceinline__ g4 interp(const g4& z, const g4& o, const g4& r) {
     return g_add(z, g_mul(r, g_add(o, z)));
 }
@@ -1311,6 +1354,154 @@ __global__ __launch_bounds__(kStageBlock) void k_inner_tail(InnerTail A) {
         return;
     }
     if (threadIdx.x < 2) reinterpret_cast<uint4*>(A.fin)[threadIdx.x] = threadIdx.x == 0 ? Ds[0] : Ws[0];
+}
+
+// ============================ inner sumcheck, Boolean assignment (bits) ===
+// Round K over output pair p: D_K(2p), D_K(2p+1) (round 0: D's pair; round
+// K >= 1: four entries of D_{K-1} folded by r_{K-1} on load and written), and
+// the 2^(K+1) assignment bits under them, h(2^(K+1) p + i).  A subset-XOR
+// transform turns each half (2^K bits) into c_S for every S; D_K(2p) is XORed
+// into sum S of c0 where c_S(2p) is set, D_K(2p) + D_K(2p+1) into sum S of c2
+// where c_S(2p) and c_S(2p+1) differ.  No GF(2^128) products besides the
+// fold's two constant-table products (skipped by a wave whose differences
+// are all zero).
+template <int K>
+__device__ __forceinline__ void inner_bits_body(const InnerBitsDesc& A, size_t block, size_t blocks) {
+    constexpr int S = 1 << K;
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    uint4* rt = smem_raw;
+    if constexpr (K > 0) {
+        for (int e = threadIdx.x; e < kConstTable4Entries; e += kInnerBlock) rt[e] = u4_load_hbm(A.rtab4 + e);
+        __syncthreads();
+    }
+    auto nonzero = [](const g4& a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) != 0; };
+    g4 s0[S], s2[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) s0[s] = s2[s] = g_zero();
+    unsigned long long live = 0;
+#pragma unroll 1
+    for (size_t p = block * kInnerBlock + threadIdx.x; p < A.pairs; p += blocks * kInnerBlock) {
+        g4 m0, m1;
+        if constexpr (K == 0) {
+            m0 = g_load_hbm(A.d + 2 * p);
+            m1 = g_load_hbm(A.d + 2 * p + 1);
+        } else {
+            g4 in[4];
+            if (4 * p + 3 < A.live) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) in[j] = g_load_hbm(A.d + 4 * p + j);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) in[j] = 4 * p + j < A.live ? g_load_hbm(A.d + 4 * p + j) : g_zero();
+            }
+            const g4 d01 = g_add(in[0], in[1]), d23 = g_add(in[2], in[3]);
+            g4 t01 = g_zero(), t23 = g_zero();
+            if (__builtin_amdgcn_ballot_w64(nonzero(d01) || nonzero(d23))) {
+                t01 = g_mul_table4(rt, d01);
+                t23 = g_mul_table4(rt, d23);
+            }
+            m0 = g_add(in[0], t01);
+            m1 = g_add(in[2], t23);
+            g_store_hbm(A.dout + 2 * p, m0);
+            g_store_hbm(A.dout + 2 * p + 1, m1);
+        }
+        const size_t bit0 = p << (K + 1);
+        uint32_t v = (uint32_t)(A.bits[bit0 >> 6] >> (bit0 & 63));
+        v &= (2u << (2 * S - 1)) - 1;  // the 2S bits of this pair
+        // Subset XOR within each half: bit x becomes XOR of h(y), y subset of x.
+        if constexpr (K >= 1) v ^= (v << 1) & 0xAAAAAAAAu;
+        if constexpr (K >= 2) v ^= (v << 2) & 0xCCCCCCCCu;
+        if constexpr (K >= 3) v ^= (v << 4) & 0xF0F0F0F0u;
+        const uint32_t vd = v ^ (v >> S);
+        const g4 dm = g_add(m0, m1);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const uint32_t k0 = 0u - ((v >> s) & 1u), kd = 0u - ((vd >> s) & 1u);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                s0[s].w[i] = xor_and(s0[s].w[i], m0.w[i], k0);
+                s2[s].w[i] = xor_and(s2[s].w[i], dm.w[i], kd);
+            }
+        }
+        if (A.wout) {  // h_K(2p), h_K(2p+1) = sum_S r_S c_S
+            g4 w0 = g_zero(), w1 = g_zero();
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const g4 r = g_from(A.mono[s]);
+                const uint32_t k0 = 0u - ((v >> s) & 1u), k1 = 0u - ((v >> (S + s)) & 1u);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    w0.w[i] = xor_and(w0.w[i], r.w[i], k0);
+                    w1.w[i] = xor_and(w1.w[i], r.w[i], k1);
+                }
+            }
+            g_store_hbm(A.wout + 2 * p, w0);
+            g_store_hbm(A.wout + 2 * p + 1, w1);
+        }
+        if constexpr (K == 0) {
+            if (nonzero(m1) || (v & 2u))
+                live = 2 * p + 2;
+            else if (nonzero(m0) || (v & 1u))
+                live = 2 * p + 1;
+        }
+    }
+    uint32_t x[8 * S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[4 * s + i] = s0[s].w[i];
+            x[4 * S + 4 * s + i] = s2[s].w[i];
+        }
+    uint32_t* rep = A.acc + (block % kInnerBitsReplicas) * kInnerBitsReplicaWords;
+    if constexpr (K == 0) {
+        __shared__ unsigned long long lv[kInnerBlock / 64];
+        for (int off = 32; off >= 1; off >>= 1) live = max(live, (unsigned long long)__shfl_xor(live, off, 64));
+        if ((threadIdx.x & 63) == 0) lv[threadIdx.x >> 6] = live;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long m = 0;
+            for (int w = 0; w < kInnerBlock / 64; ++w) m = max(m, lv[w]);
+            if (m) atomicMax(reinterpret_cast<unsigned long long*>(rep + kInnerBitsLiveWord), m);
+        }
+    }
+    block_xor_words<8 * S>(x, rep);
+}
+
+template <int K>
+__global__ __launch_bounds__(kInnerBlock) void k_inner_bits(InnerBitsDesc A) {
+    inner_bits_body<K>(A, blockIdx.x, A.blocks);
+}
+template <int K>
+__global__ __launch_bounds__(kInnerBlock) void k_inner_bits_batch(const InnerBitsDesc* d, const uint32_t* bmap) {
+    const InnerBitsDesc A = d[bmap[blockIdx.x]];
+    inner_bits_body<K>(A, blockIdx.x - A.block_base, A.blocks);
+}
+template __global__ void k_inner_bits<0>(InnerBitsDesc);
+template __global__ void k_inner_bits<1>(InnerBitsDesc);
+template __global__ void k_inner_bits<2>(InnerBitsDesc);
+template __global__ void k_inner_bits<3>(InnerBitsDesc);
+template __global__ void k_inner_bits_batch<0>(const InnerBitsDesc*, const uint32_t*);
+template __global__ void k_inner_bits_batch<1>(const InnerBitsDesc*, const uint32_t*);
+template __global__ void k_inner_bits_batch<2>(const InnerBitsDesc*, const uint32_t*);
+template __global__ void k_inner_bits_batch<3>(const InnerBitsDesc*, const uint32_t*);
+static_assert(kInnerBitsMaxRound == 3, "the instantiations above cover rounds 0..3");
+
+__global__ __launch_bounds__(kBlock) void k_r1cs_rows_bits(const uint32_t* row_starts, const uint32_t* cols,
+                                                           const f128* vals, const uint64_t* bits, size_t rows,
+                                                           f128* out, size_t padded) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < padded; i += (size_t)gridDim.x * blockDim.x) {
+        g4 s = g_zero();
+        if (i < rows)
+            for (uint32_t k = row_starts[i]; k < row_starts[i + 1]; ++k) {
+                const uint32_t c = cols[k];
+                const uint32_t m = 0u - (uint32_t)((bits[c >> 6] >> (c & 63)) & 1u);
+                const g4 a = g_load(vals + k);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) s.w[w] = xor_and(s.w[w], a.w[w], m);
+            }
+        g_store(out + i, s);
+    }
 }
 
 // ================================ GKR layers in parallel (known challenges) ===

@@ -257,6 +257,50 @@ constexpr size_t kInnerTailSmem =
     sizeof(uint4) * (((size_t)2 << kInnerTailMaxLog) + kConstTable4Entries + 4 * 64 * kLaneTableEntries);
 __global__ void k_inner_tail(InnerTail args);
 
+// ---- inner sumcheck against a Boolean assignment (packed bits) ------------
+// With h in {0, 1} (bit j = bits[j / 64] >> (j % 64) & 1), round K's bound
+// table is h_K(j) = sum_S r_S c_S(j) over the subsets S of {0..K-1}, r_S the
+// product of r_i over S and c_S(j) the XOR of h(2^K j + x) over the subsets x
+// of S: bits only.  Round K's sums split the same way,
+//   c0 = sum_S r_S [sum_p c_S(2p) D_K(2p)],
+//   c2 = sum_S r_S [sum_p (c_S(2p) + c_S(2p+1)) (D_K(2p) + D_K(2p+1))],
+// so the kernel only XORs D_K under masks from the bits (2^(K+1) sums) and the
+// host multiplies the sums by r_S.  Rounds 0 .. kInnerBitsMaxRound run so;
+// the last of them also writes h_K as elements for the general kernels.
+constexpr int kInnerBitsMaxRound = 3;
+// The sums go to kInnerBitsReplicas copies (workgroup g adds into copy
+// g mod kInnerBitsReplicas), which the reader XORs: thousands of workgroups
+// that finish together would otherwise serialise on a few words' atomics.
+// A copy: the 2^(K+1) sums (4 words each: 2^K for c0 with S ascending, then
+// 2^K for c2) from word 0, and at words 64-65 (round 0) the maximum of 1 +
+// the last index where D or h is nonzero.
+constexpr int kInnerBitsReplicas = 32;
+constexpr int kInnerBitsReplicaWords = 68;
+constexpr int kInnerBitsLiveWord = 64;
+constexpr size_t kInnerBitsAccWords = (size_t)kInnerBitsReplicas * kInnerBitsReplicaWords;
+struct InnerBitsDesc {
+    const f128* d;          // round 0: D; round K >= 1: D_{K-1}, folded by r_{K-1} on load
+    const uint64_t* bits;   // the packed assignment, 2^n bits (zero past the assignment)
+    f128* dout;             // K >= 1: D_K, 2 entries per output pair
+    f128* wout;             // or null: h_K as elements, 2 entries per output pair
+    size_t pairs;           // output pairs (pairs of D_K)
+    size_t live;            // K >= 1: entries of D_{K-1} below live (the rest is zero)
+    const uint4* rtab4;     // K >= 1: r_{K-1}'s 4-bit table
+    uint32_t* acc;          // kInnerBitsAccWords words, zero on entry (the copies above)
+    uint32_t blocks, block_base;
+    f128 mono[1 << kInnerBitsMaxRound];  // wout: r_S for S < 2^K
+};
+// Workgroups of a bit round: enough to stream at HBM rate, few enough that
+// the sums' atomics stay cheap.
+constexpr unsigned kInnerBitsMaxBlocks = 1024;
+constexpr size_t kInnerBitsSmem = sizeof(uint4) * kConstTable4Entries;
+template <int K> __global__ void k_inner_bits(InnerBitsDesc A);
+template <int K> __global__ void k_inner_bits_batch(const InnerBitsDesc* d, const uint32_t* bmap);
+// Az, Bz, Cz rows over a packed Boolean assignment: the XOR of the row's
+// values at the columns whose bit is set (no products).
+__global__ void k_r1cs_rows_bits(const uint32_t* row_starts, const uint32_t* cols, const f128* vals,
+                                 const uint64_t* bits, size_t rows, f128* out, size_t padded);
+
 // ---- GKR layers in parallel (challenges known in advance) ----------------
 // With every challenge known, each layer's point is known, so the layers'
 // sumchecks are independent: small eq tables and the low/high halves of

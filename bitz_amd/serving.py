@@ -3,8 +3,8 @@ admission by an HBM budget and concurrent requests overlapping on the GPU.
 
 A request is the sumcheck stage of a proof that a chain of B SHA-256
 compressions maps its chaining value through its message blocks: the
-witness (host, C++: bitz_sha256_witness), the assignment MLE, Az/Bz/Cz and
-prove_spartan_piop (outer sumcheck, bind_and_batch, inner sumcheck;
+witness (host, C++: bitz_sha256_witness), the assignment (kept as its packed
+bits), Az/Bz/Cz and prove_spartan_piop (outer sumcheck, bind_and_batch, inner sumcheck;
 crates/spartan/src/piop.rs:67-112), every table on the device.  The circuit
 is the compression chain of the SHA-256 producer: one block is the
 reference's compression_circuit (crates/circuit/src/sha256.rs:476-490), B
@@ -37,7 +37,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (PiopEngine, PreparedConstraintMatrices, SpartanPiopOutput, Transcript, assignment_mle_words, init,
+from . import (PackedAssignment, PiopEngine, PreparedConstraintMatrices, SpartanPiopOutput, Transcript, init,
                prove_spartan_piop)
 from . import sha256 as sha
 
@@ -70,10 +70,11 @@ class Circuit:
         return sum(self.circuit.nnz)
 
     def footprint(self) -> int:
-        """Device bytes a request holds at its peak: the assignment and D
-        (2^col_vars each) with the inner sumcheck's scratch (1.5 x 2^col_vars),
-        the three products (3 x 2^row_vars) with the outer scratch (2.25 x)."""
-        return 16 * int((1 << self.col_vars) * 3.5 + (1 << self.row_vars) * 5.25)
+        """Device bytes a request holds at its peak: the packed assignment
+        (2^col_vars bits), D (2^col_vars elements) with the inner sumcheck's
+        scratch (1.5 x 2^col_vars), the three products (3 x 2^row_vars) with
+        the outer scratch (2.25 x)."""
+        return 16 * int((1 << self.col_vars) * 2.5 + (1 << self.row_vars) * 5.25) + (1 << self.col_vars) // 8
 
 
 def make_circuit(blocks: int) -> Circuit:
@@ -135,16 +136,17 @@ class ProvingServer:
         PIOP with every table on the device.  Returns (proof, output state)."""
         c = self.circuit(blocks)
         w = c.circuit.witness(message, iv=iv)
-        h = assignment_mle_words(w.h, c.columns, c.columns, device=True)
+        h = PackedAssignment(w.h, c.columns, c.columns, device=True)
         products = c.matrices.r1cs_products(h)
         t = Transcript.duplex(b"bitz/serve/%d" % rid)
         return prove_spartan_piop(t, c.matrices, c.digest, products, h), w.digest
 
     def prepare(self, r: "Request"):
-        """A request's device inputs: witness (host), assignment, Az/Bz/Cz."""
+        """A request's device inputs: witness (host), the assignment's packed
+        bits, Az/Bz/Cz."""
         c = self.circuit(r.blocks)
         w = c.circuit.witness(r.message, iv=r.iv)
-        h = assignment_mle_words(w.h, c.columns, c.columns, device=True)
+        h = PackedAssignment(w.h, c.columns, c.columns, device=True)
         return c, h, c.matrices.r1cs_products(h), w.digest
 
     def serve_batched(self, requests: list[Request], preparers: int = 4, engines: int | None = None) -> list[Request]:

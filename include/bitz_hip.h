@@ -324,9 +324,26 @@ int32_t bitz_prove_inner_sumcheck(bitz_transcript* t, bitz_f128 initial_claim,
                                   const bitz_f128* witness, uint32_t witness_vars, uint32_t flags,
                                   bitz_f128* round_polys, bitz_f128* eval_points,
                                   bitz_f128* final_claim, bitz_f128* evals);
+/* prove_inner_sumcheck (sumcheck.rs:442-552) with the assignment h given as
+ * the packed bits of a Boolean table (bit j = bits[j / 64] >> (j % 64) & 1,
+ * 2^assignment_vars bits, zero past the assignment; bitz_assignment_pack
+ * writes them): the table build_assignment_mle (crates/spartan/src/matrix.rs:
+ * 392-420) expands.  Outputs equal bitz_prove_inner_sumcheck's over that
+ * table; the first rounds need no general products (h_k is a combination of
+ * the monomials of r_0 .. r_{k-1} with bit coefficients).  BITZ_DEVICE:
+ * matrix and bits are device arrays. */
+int32_t bitz_prove_inner_sumcheck_packed(bitz_transcript* t, bitz_f128 initial_claim,
+                                         const bitz_f128* matrix, uint32_t matrix_vars,
+                                         const uint64_t* assignment_bits, uint32_t assignment_vars,
+                                         uint32_t flags, bitz_f128* round_polys, bitz_f128* eval_points,
+                                         bitz_f128* final_claim, bitz_f128* evals);
 typedef struct bitz_inner bitz_inner;
 int32_t bitz_inner_begin(bitz_inner** h, const bitz_f128* matrix, uint32_t matrix_vars,
                          const bitz_f128* witness, uint32_t witness_vars, uint32_t flags);
+/* The per-round handle over a packed Boolean assignment (as
+ * bitz_prove_inner_sumcheck_packed); round0 / fold_next / finish as above. */
+int32_t bitz_inner_begin_packed(bitz_inner** h, const bitz_f128* matrix, uint32_t matrix_vars,
+                                const uint64_t* assignment_bits, uint32_t assignment_vars, uint32_t flags);
 int32_t bitz_inner_round0(bitz_inner* h, bitz_f128* coeffs2);
 int32_t bitz_inner_fold_next(bitz_inner* h, bitz_f128 r, bitz_f128* coeffs2);
 int32_t bitz_inner_finish(bitz_inner* h, bitz_f128* evals2);
@@ -354,6 +371,16 @@ int32_t bitz_r1cs_products(bitz_matrices* h, const bitz_f128* assignment, bitz_f
  * elements padded with zeros to 2^ceil(log2 columns).  bit_len != columns ->
  * BITZ_ERR_ASSIGNMENT_LEN; columns == 0 or bit 0 clear -> BITZ_ERR_ASSIGNMENT_CONST. */
 int32_t bitz_assignment_mle(const uint64_t* words, size_t bit_len, size_t columns, bitz_f128* out, uint32_t flags);
+/* build_assignment_mle's checks (matrix.rs:392-420) over the packed
+ * assignment, kept packed: out receives ceil(2^ceil(log2 columns) / 64)
+ * words (at least one), the bits past bit_len cleared.  BITZ_DEVICE: out is
+ * a device array.  Errors as bitz_assignment_mle's. */
+int32_t bitz_assignment_pack(const uint64_t* words, size_t bit_len, size_t columns, uint64_t* out, uint32_t flags);
+/* bitz_r1cs_products over a packed Boolean assignment (bitz_assignment_pack's
+ * form, 2^col_vars bits): each row is the XOR of its values at the columns
+ * whose bit is set.  BITZ_DEVICE: bits and outputs on the device. */
+int32_t bitz_r1cs_products_packed(bitz_matrices* h, const uint64_t* assignment_bits, bitz_f128* az, bitz_f128* bz,
+                                  bitz_f128* cz, uint32_t flags);
 /* constraint_matrix_digest (matrix.rs:562-627): SHA-256 over the domain
  * "bitz/spartan/constraint-matrices/v1", M's positions (m_rows rows of
  * m_columns columns, CSR), then A, B, C row by row (rows x columns, CSR) with
@@ -375,6 +402,15 @@ int32_t bitz_prove_spartan_piop(bitz_transcript* t, bitz_matrices* h, const uint
                                 const bitz_f128* assignment, uint32_t col_vars, uint32_t flags,
                                 bitz_f128* outer_rounds, bitz_f128* outer_claims, bitz_f128* inner_rounds,
                                 bitz_f128* inner_point, bitz_f128* matrix_eval, bitz_f128* final_claim);
+/* bitz_prove_spartan_piop with the assignment as packed bits (the form of
+ * bitz_assignment_pack, 2^col_vars bits, on the side flags names): the same
+ * outputs; the inner sumcheck runs as bitz_prove_inner_sumcheck_packed. */
+int32_t bitz_prove_spartan_piop_packed(bitz_transcript* t, bitz_matrices* h, const uint8_t digest[32],
+                                       const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
+                                       uint32_t row_vars, const uint64_t* assignment_bits, uint32_t col_vars,
+                                       uint32_t flags, bitz_f128* outer_rounds, bitz_f128* outer_claims,
+                                       bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
+                                       bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
 
 /* ---- post-GKR inner-product sumcheck ----------------------------------------- */
@@ -410,6 +446,13 @@ int32_t bitz_piop_engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcri
                                 const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
                                 const bitz_f128* cz, uint32_t row_vars, const bitz_f128* assignment,
                                 uint32_t col_vars);
+/* bitz_piop_engine_submit with the assignment as packed bits on the device
+ * (bitz_assignment_pack's form, 2^col_vars bits); they must live until the
+ * proof is taken.  Outputs as bitz_prove_spartan_piop_packed's. */
+int32_t bitz_piop_engine_submit_packed(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                                       const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
+                                       const bitz_f128* cz, uint32_t row_vars, const uint64_t* assignment_bits,
+                                       uint32_t col_vars);
 /* One round of every proof in flight; *in_flight receives how many remain.
  * step = launch (enqueue the step's work on the engine's stream, no wait;
  * *launched = proofs in it) then finish (wait, then the host's part).  With
