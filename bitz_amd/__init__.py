@@ -133,6 +133,11 @@ def lib():
             "bitz_prove_inner_sumcheck_packed": (I32, [P, F128, P, U32, P, U32, U32, P, P, P, P]),
             "bitz_inner_begin_packed": (I32, [C.POINTER(C.c_void_p), P, U32, P, U32, U32]),
             "bitz_piop_engine_submit_packed": (I32, [P, C.c_uint64, P, P, P, P, P, P, U32, P, U32]),
+            "bitz_piop_engine_submit_assignment": (I32, [P, C.c_uint64, P, P, P, P, SZ, SZ]),
+            "bitz_server_create": (I32, [C.POINTER(C.c_void_p), U32, U32, U32, C.c_uint64]),
+            "bitz_server_add_circuit": (I32, [P, P, P, P, C.c_uint64, C.POINTER(U32)]),
+            "bitz_server_serve": (I32, [P, P, SZ]),
+            "bitz_server_free": (None, [P]),
             "bitz_constraint_matrix_digest": (I32, [SZ, SZ, P, P, SZ, SZ, P, P, P, P]),
             "bitz_prove_spartan_piop": (I32, [P, P, P, P, P, P, U32, P, U32, U32, P, P, P, P, P, P]),
             "bitz_eq_table": (I32, [P, U32, P, U32]),
@@ -1113,6 +1118,29 @@ class PiopEngine:
         try:
             _check(fn(self.h, tag, transcript.h, matrices.h, dg, _p(az), _p(bz), _p(cz), _nvars(na), _p(hv),
                       _nvars(nh)), SumcheckError)
+        except BaseException:
+            if before is None:
+                self._held.pop(tag, None)
+            else:
+                self._held[tag] = before
+            raise
+
+    def submit_assignment(self, tag: int, transcript: Transcript, matrices: "PreparedConstraintMatrices",
+                          digest: bytes, words, bit_len: int, columns: int) -> None:
+        """A proof from the packed assignment on the host (PackedWitness
+        words): the engine uploads it and computes Az, Bz, Cz itself
+        (bitz_piop_engine_submit_assignment)."""
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        if not len(words):
+            words = np.zeros(1, np.uint64)
+        if bit_len > 64 * len(words):
+            raise BitzError(-7, "fewer words than bits")
+        dg = (C.c_uint8 * 32).from_buffer_copy(digest)
+        before = self._held.get(tag)
+        self._held[tag] = (transcript, matrices, None, None, matrices.num_row_vars, matrices.num_column_vars)
+        try:
+            _check(lib().bitz_piop_engine_submit_assignment(self.h, tag, transcript.h, matrices.h, dg, _p(words),
+                                                            bit_len, columns), SumcheckError)
         except BaseException:
             if before is None:
                 self._held.pop(tag, None)

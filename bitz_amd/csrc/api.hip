@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -59,25 +60,32 @@ struct DeviceCache {
     std::mutex mu;
     std::multimap<size_t, void*> free_blocks;
     size_t cached_bytes = 0;
-    void* get(size_t n) {
+    // A block of at least n bytes (at most 2 n when it comes from the
+    // cache); *size receives its true size, which put() must be given back.
+    void* get(size_t n, size_t* size) {
         {
             std::lock_guard<std::mutex> lock(mu);
             auto it = free_blocks.lower_bound(n);
             if (it != free_blocks.end() && it->first <= 2 * n) {
                 void* p = it->second;
+                *size = it->first;
                 cached_bytes -= it->first;
                 free_blocks.erase(it);
                 return p;
             }
         }
+        const auto t0 = std::chrono::steady_clock::now();
         void* p = nullptr;
         if (hipMalloc(&p, n) != hipSuccess) {
             // drop the cache and retry once
             release_all();
             if (hipMalloc(&p, n) != hipSuccess) return nullptr;
         }
+        note_malloc(t0);
+        *size = n;
         return p;
     }
+    static void note_malloc(std::chrono::steady_clock::time_point t0);
     void put(void* p, size_t n) {
         std::lock_guard<std::mutex> lock(mu);
         free_blocks.emplace(n, p);
@@ -114,9 +122,10 @@ struct Dev {
         }
         if (n == 0) n = 16;
         n = (n + 255) & ~(size_t)255;
-        p = g_cache.get(n);
+        size_t got = 0;
+        p = g_cache.get(n, &got);
         if (!p) return fail(BITZ_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
-        bytes = n;
+        bytes = got;
         return BITZ_OK;
     }
     f128* f() const { return static_cast<f128*>(p); }
@@ -147,6 +156,53 @@ struct PinnedPool {
         free_blocks.push_back(p);
     }
 } g_pinned;
+
+// Larger pinned staging: power-of-two classes from 64 KiB, recycled.
+struct PinnedClassPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_blocks;
+    void* get(size_t n, size_t* size) {
+        size_t c = PinnedPool::kBytes;
+        while (c < n) c <<= 1;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = free_blocks.find(c);
+            if (it != free_blocks.end()) {
+                void* p = it->second;
+                free_blocks.erase(it);
+                *size = c;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+        *size = c;
+        return p;
+    }
+    void put(void* p, size_t size) {
+        std::lock_guard<std::mutex> lock(mu);
+        free_blocks.emplace(size, p);
+    }
+} g_pinned_classes;
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t size = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) g_pinned_classes.put(p, size);
+    }
+    int32_t alloc(size_t n) {
+        if (p && size >= n) return BITZ_OK;
+        if (p) g_pinned_classes.put(p, size);
+        p = g_pinned_classes.get(n, &size);
+        if (!p) return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
+        return BITZ_OK;
+    }
+    char* c() const { return static_cast<char*>(p); }
+};
 
 struct Pinned {
     void* p = nullptr;
@@ -280,6 +336,9 @@ static void prof_add_host(const char* name, std::chrono::steady_clock::time_poin
     k.ms += ms;
     k.launches += 1;
 }
+
+// Cache misses: the time of each hipMalloc the device cache had to make.
+void DeviceCache::note_malloc(std::chrono::steady_clock::time_point t0) { prof_add_host("device_malloc", t0); }
 
 // Events come from a pool so that a profiled proof creates none.
 struct EventPool {
@@ -418,8 +477,8 @@ struct Outstanding {
 
 extern "C" int32_t bitz_device_alloc(void** ptr, size_t bytes) {
     if (!ptr) return fail(BITZ_ERR_ARG, "null pointer");
-    const size_t n = bytes ? bytes : 16;
-    void* p = g_cache.get(n);
+    size_t n = 0;
+    void* p = g_cache.get(bytes ? bytes : 16, &n);
     if (!p) return fail(BITZ_ERR_ALLOC, "hipMalloc failed");
     {
         std::lock_guard<std::mutex> lock(g_outstanding.mu);
@@ -1500,7 +1559,6 @@ static int32_t outer_pipeline_prologue(bitz_outer* h, const bitz_f128* challenge
     hipStream_t s = h->so.s;
     const int n = h->n;
     if (n > bitz::kMaxRounds) return fail(BITZ_ERR_ARG, "num_vars too large");
-    const PipeSchedule ps = pipe_schedule(n);
     bitz::PrologueArgs pa{};
     pa.L = h->eq_in[0];
     pa.H = h->eq_in[1];
@@ -3076,7 +3134,16 @@ struct bitz_matrices {
     Dev long_chunks;          // {column, first, end} runs of the long columns
     size_t nnz = 0, n_long_chunks = 0;
     std::mutex mu;            // calls on one handle share its scratch and stream: one at a time
+    // The last bind's work (on whichever stream ran it): the next bind, on
+    // any stream, waits for it before it writes the shared scratch.
+    hipEvent_t bind_done = nullptr;
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
+    ~bitz_matrices() {
+        if (bind_done) {
+            (void)hipEventSynchronize(bind_done);
+            (void)hipEventDestroy(bind_done);
+        }
+    }
 };
 
 extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint32_t row_vars,
@@ -3187,13 +3254,26 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
 }
 
 // Enqueues D = bind_and_batch(r_x, rho) into target (2^col_vars, device) on
-// stream s, the row point already in h->point.  Uses the handle's scratch:
-// calls on one handle must run one after another.
-static int32_t bind_enqueue(bitz_matrices* h, f128 rho, f128* target, hipStream_t s) {
+// stream s, after copying the row point (row_vars entries, host or device)
+// into the handle.  Uses the handle's scratch: callers hold h->mu while
+// enqueueing, and the bind_done event orders the kernels across streams.
+static int32_t bind_enqueue(bitz_matrices* h, const f128* row_point, bool point_on_device, f128 rho, f128* target,
+                            hipStream_t s, const f128* rho_dev = nullptr) {
+    // The point, eq table, scales and products live in the handle: wait for
+    // the previous bind (possibly queued on another stream) first.
+    if (h->bind_done) HIP_TRY(hipStreamWaitEvent(s, h->bind_done, 0));
+    if (h->row_vars)
+        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, (size_t)h->row_vars * sizeof(f128),
+                               point_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     TRY(eq_build(h->point.f(), h->row_vars, h->eq_rows.f(), h->scratch, s));
-    // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value
-    hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(), rho,
-                       hostf::mul(rho, rho), h->row_scale.f(), h->rows_padded);
+    // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value or (rho_dev:
+    // rho, rho^2) from the device
+    if (rho_dev)
+        hipLaunchKernelGGL(bitz::k_row_scales_dev, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s,
+                           h->eq_rows.f(), rho_dev, h->row_scale.f(), h->rows_padded);
+    else
+        hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(),
+                           rho, hostf::mul(rho, rho), h->row_scale.f(), h->rows_padded);
     // every nonzero's product (a converged pass), then each column's XOR over its run
     static bool attr = false;
     if (!attr) {
@@ -3214,7 +3294,10 @@ static int32_t bind_enqueue(bitz_matrices* h, f128 rho, f128* target, hipStream_
     if (h->n_long_chunks)  // the long columns, left zero above: XOR of each run's sum
         hipLaunchKernelGGL(bitz::k_bind_long, dim3((unsigned)h->n_long_chunks), dim3(kBlock), 0, s,
                            h->long_chunks.u32(), h->prod.f(), target);
-    return check_launch();
+    TRY(check_launch());
+    if (!h->bind_done) HIP_TRY(hipEventCreateWithFlags(&h->bind_done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(h->bind_done, s));
+    return BITZ_OK;
 }
 
 extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_point, uint32_t point_len,
@@ -3224,16 +3307,13 @@ extern "C" int32_t bitz_bind_and_batch(bitz_matrices* h, const bitz_f128* row_po
     if (point_len != (uint32_t)h->row_vars) return fail(BITZ_ERR_ROW_POINT, "row point length");
     std::lock_guard<std::mutex> lock(h->mu);
     hipStream_t s = h->so.s;
-    if (point_len)
-        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, point_len * sizeof(f128),
-                               (flags & BITZ_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     Dev dout;
     f128* target = reinterpret_cast<f128*>(out);
     if (!(flags & BITZ_DEVICE)) {
         TRY(dout.alloc(h->cols * sizeof(f128)));
         target = dout.f();
     }
-    TRY(bind_enqueue(h, F(rho), target, s));
+    TRY(bind_enqueue(h, reinterpret_cast<const f128*>(row_point), (flags & BITZ_DEVICE) != 0, F(rho), target, s));
     if (!(flags & BITZ_DEVICE))
         HIP_TRY(hipMemcpyAsync(out, target, h->cols * sizeof(f128), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -3512,6 +3592,10 @@ extern "C" int32_t bitz_prove_spartan_piop_packed(bitz_transcript* t, bitz_matri
 
 extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
 
+// Whether provers with a Keccak transcript draw its challenges on the
+// device (bitz_set_device_fiat_shamir); on by default.
+static std::atomic<bool> g_device_fs{true};
+
 // ============================================================ PIOP engine ==
 // Many prove_spartan_piop proofs advanced together, one round each per step.
 // Every proof keeps its own transcript, tables and state; a step gathers the
@@ -3523,7 +3607,13 @@ extern "C" void bitz_matrices_free(bitz_matrices* h) { delete h; }
 // next step.  The transcript order is bitz_prove_spartan_piop's.
 namespace {
 
-enum class JobPhase { kOuter, kOuterFinal, kBind, kInner, kInnerFinal, kDone };
+// kDevDone: a device-transcript proof whose last action is in the step in
+// flight (its outputs come back with that step).
+enum class JobPhase { kOuter, kOuterFinal, kBind, kInner, kInnerFinal, kDevDone, kDone };
+
+// Positions per step: a device-transcript proof advances by up to this many
+// actions (rounds) per step; a host-transcript proof by one.
+constexpr int kEnginePositions = 8;
 
 // Words of a proof's slot in a step's read-back (a bit round's take
 // kInnerBitsAccWords): round sums, round 0's live length at words 8-9.
@@ -3554,17 +3644,68 @@ struct PiopJob {
     std::vector<bitz_f128> outer_rounds, rx, inner_rounds, inner_point;
     bitz_f128 outer_claims[3] = {}, matrix_eval{}, final_claim{};
     int32_t status = BITZ_OK;
-    ~PiopJob() { bitz_outer_free(outer); }
+    // Device-transcript proofs (a packed assignment and a Keccak transcript
+    // with device Fiat-Shamir on): one device block holds the running state
+    // (bitz::PiopFsState), the outputs, every round's sums and r's table.
+    bool devfs = false;
+    Dev fsb;
+    enum Out { kOutOuter, kOutRx, kOutClaims, kOutInner, kOutPoint, kOutMatrixEval, kOutFinal, kOutCount };
+    size_t out_off[kOutCount] = {};  // element offsets of the outputs, after the state
+    size_t back_bytes = 0;           // the state and the outputs: what comes back at the end
+    size_t so_off = 0, sb_off = 0, sg_off = 0, sf_off = 0, rt_off = 0;  // byte offsets in fsb
+    // Pinned staging of the proof's uploads (kept until the proof is gone, so
+    // the copies are asynchronous): the packed bits; the eq factors and the
+    // state's first image.
+    PinnedBuf pin_bits, pin_setup;
+    // Set-up work (uploads, products, tables) runs on the proof's own stream
+    // from the submitting thread; the engine's stream waits for `ready`
+    // before the proof's first kernels.
+    StreamOwner prep;
+    hipEvent_t ready = nullptr;
+    Dev own_bits, own_prod;  // bitz_piop_engine_submit_assignment: the packed bits and Az, Bz, Cz
+    char* fs_base() const { return static_cast<char*>(fsb.p); }
+    bitz::PiopFsState* fs_state() const { return reinterpret_cast<bitz::PiopFsState*>(fs_base()); }
+    f128* fs_out(int w) const {
+        return reinterpret_cast<f128*>(fs_base() + sizeof(bitz::PiopFsState)) + out_off[w];
+    }
+    uint32_t* fs_slot_outer(int k) const {  // k = n: the final A(r), B(r), C(r)
+        return reinterpret_cast<uint32_t*>(fs_base() + so_off) + 12 * (size_t)k;
+    }
+    uint32_t* fs_slot_bits(int K) const {
+        return reinterpret_cast<uint32_t*>(fs_base() + sb_off) + bitz::kInnerBitsAccWords * (size_t)K;
+    }
+    uint32_t* fs_slot_gen(int k) const { return reinterpret_cast<uint32_t*>(fs_base() + sg_off) + 8 * (size_t)k; }
+    uint32_t* fs_slot_final() const { return reinterpret_cast<uint32_t*>(fs_base() + sf_off); }
+    uint4* fs_rtab() const { return reinterpret_cast<uint4*>(fs_base() + rt_off); }
+    size_t fs_back_bytes() const { return back_bytes; }
+    ~PiopJob() {
+        bitz_outer_free(outer);
+        if (ready) (void)hipEventDestroy(ready);
+    }
 };
 
 }  // namespace
 
+// A step in flight: its proofs, where each host-transcript proof's sums and
+// each finishing proof's outputs land in the pinned buffer `buf`, and the
+// event recorded after its last operation.
+struct EngineStep {
+    std::vector<PiopJob*> act;
+    std::vector<size_t> slot_off;  // word offset of each proof's slot (and the total)
+    std::vector<std::pair<PiopJob*, size_t>> fin_back;
+    int buf = 0;
+};
+
 struct bitz_piop_engine {
     std::vector<std::unique_ptr<PiopJob>> jobs;
-    std::vector<PiopJob*> launched;  // the proofs of the step in flight on the device
-    Dev slots, stage, rt4;
-    void* pin = nullptr;
-    size_t pin_bytes = 0;
+    // Steps in flight, oldest first: one, or two when every proof draws its
+    // challenges on the device (the next step is queued while one runs).
+    std::deque<EngineStep> steps;
+    Dev slots, stage[2], rt4;
+    void* pin[2] = {nullptr, nullptr};  // per step buffer: the uploads, then what comes back
+    size_t pin_bytes[2] = {0, 0};
+    hipEvent_t step_done[2] = {nullptr, nullptr};
+    int next_buf = 0;
     // Background stepping (bitz_piop_engine_run): a thread of the engine's
     // own steps while proofs are in flight; submit hands it proofs through
     // `incoming`, finished ones wait in `finished` for take.
@@ -3577,8 +3718,7 @@ struct bitz_piop_engine {
     std::string worker_err;
     // HIP events around the step's batched inner fold and around the whole
     // step's device work (when profiling), read once the step has synchronised.
-    Timed t_inner_fold, t_step, t_bits[bitz::kInnerBitsMaxRound + 1];
-    std::vector<size_t> slot_off;  // word offset of each proof's slot in the step in flight (and the total)
+    Timed t_inner_fold[kEnginePositions], t_step, t_bits[kEnginePositions][bitz::kInnerBitsMaxRound + 1];
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -3590,7 +3730,10 @@ struct bitz_piop_engine {
             worker.join();
         }
         (void)hipStreamSynchronize(so.s);
-        if (pin) (void)hipHostFree(pin);
+        for (int b = 0; b < 2; ++b) {
+            if (pin[b]) (void)hipHostFree(pin[b]);
+            if (step_done[b]) (void)hipEventDestroy(step_done[b]);
+        }
     }
 };
 
@@ -3607,10 +3750,51 @@ extern "C" int32_t bitz_piop_engine_create(bitz_piop_engine** out) {
 
 extern "C" void bitz_piop_engine_free(bitz_piop_engine* e) { delete e; }
 
+// A device-transcript proof's buffers, all allocated up front (the host
+// never waits for a round): the inner tables, and one block with the state,
+// the outputs, a slot per round (zeroed here) and r's table.
+static int32_t devfs_setup(PiopJob* j, hipStream_t s) {
+    const int n = j->n, nc = j->nc;
+    const size_t N = (size_t)1 << nc;
+    j->ilen = N;
+    j->ilive = N;
+    j->icapP = std::max<size_t>(N / 2, 1);
+    j->icapQ = std::max<size_t>(N / 4, 1);
+    TRY(j->dD.alloc(N * sizeof(f128)));
+    TRY(j->iP.alloc(2 * j->icapP * sizeof(f128)));
+    TRY(j->iQ.alloc(2 * j->icapQ * sizeof(f128)));
+    const size_t counts[PiopJob::kOutCount] = {4 * (size_t)n, (size_t)n, 3, 3 * (size_t)nc, (size_t)nc, 1, 1};
+    size_t o = 0;
+    for (int w = 0; w < PiopJob::kOutCount; ++w) {
+        j->out_off[w] = o;
+        o += counts[w];
+    }
+    j->back_bytes = sizeof(bitz::PiopFsState) + o * sizeof(f128);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    j->so_off = up(j->back_bytes);
+    j->sb_off = j->so_off + up(12 * 4 * (size_t)(n + 1));
+    j->sg_off = j->sb_off + bitz::kInnerBitsAccWords * 4 * (size_t)(j->kb + 1);
+    j->sf_off = j->sg_off + up(8 * 4 * (size_t)nc);
+    j->rt_off = j->sf_off + 256;
+    const size_t total = j->rt_off + kConstTable4Entries * sizeof(uint4);
+    TRY(j->fsb.alloc(total));
+    HIP_TRY(hipMemsetAsync(j->fs_base() + j->so_off, 0, j->rt_off - j->so_off, s));
+    // The state's first image, staged after the eq factors (engine_submit).
+    char* img = j->pin_setup.c() + (j->lo.size() + j->hi.size()) * sizeof(f128);
+    memset(img, 0, sizeof(bitz::PiopFsState));
+    auto* st = reinterpret_cast<bitz::PiopFsState*>(img);
+    st->ts = j->t->t.keccak;  // the duplex after the digest and tau
+    st->S = hostf::one();
+    st->mono[0] = hostf::one();
+    HIP_TRY(hipMemcpyAsync(j->fsb.p, img, sizeof(bitz::PiopFsState), hipMemcpyHostToDevice, s));
+    j->devfs = true;
+    return BITZ_OK;
+}
+
 static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
                              const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz, const bitz_f128* cz,
                              uint32_t row_vars, const bitz_f128* assignment, const uint64_t* bits,
-                             uint32_t col_vars) {
+                             uint32_t col_vars, std::unique_ptr<PiopJob> job = nullptr) {
     if (!e || !t || !m || !digest || !az || !bz || !cz || !(assignment || bits))
         return fail(BITZ_ERR_ARG, "null pointer");
     // piop.rs:74-83: dimensions before any transcript call.
@@ -3624,7 +3808,8 @@ static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript*
             for (const auto& j : *list)
                 if (j->tag == tag) return fail(BITZ_ERR_ARG, "tag in use");
     }
-    auto j = std::make_unique<PiopJob>();
+    std::unique_ptr<PiopJob> j = job ? std::move(job) : std::make_unique<PiopJob>();
+    if (!j->prep.s) TRY(j->prep.create());
     j->tag = tag;
     j->t = t;
     j->m = m;
@@ -3635,7 +3820,7 @@ static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript*
         j->hbits = bits;
         j->kb = std::min<int>(bitz::kInnerBitsMaxRound, (int)col_vars - 1);
     }
-    hipStream_t s = e->so.s;
+    hipStream_t s = j->prep.s;  // set-up on the proof's own stream
     // piop.rs:85-90: digest, tau, make_equality_factors (crates/poly/src/eq.rs:107-115).
     t->t.absorb(0, digest, 32);
     std::vector<f128> tau(row_vars);
@@ -3656,10 +3841,20 @@ static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript*
     };
     j->lo = host_eq(tau.data(), split);
     j->hi = host_eq(tau.data() + split, row_vars - split);
-    TRY(upload(j->dlo, reinterpret_cast<const bitz_f128*>(j->lo.data()), j->lo.size(), 0, s));
-    TRY(upload(j->dhi, reinterpret_cast<const bitz_f128*>(j->hi.data()), j->hi.size(), 0, s));
+    {
+        // Through pinned staging: the copies stay asynchronous.
+        const size_t lb = j->lo.size() * sizeof(f128), hb = j->hi.size() * sizeof(f128);
+        TRY(j->pin_setup.alloc(lb + hb + sizeof(bitz::PiopFsState)));
+        memcpy(j->pin_setup.c(), j->lo.data(), lb);
+        memcpy(j->pin_setup.c() + lb, j->hi.data(), hb);
+        TRY(j->dlo.alloc(lb));
+        TRY(j->dhi.alloc(hb));
+        HIP_TRY(hipMemcpyAsync(j->dlo.p, j->pin_setup.c(), lb, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(j->dhi.p, j->pin_setup.c() + lb, hb, hipMemcpyHostToDevice, s));
+    }
     // The outer sumcheck from claim 0 over the eq-factored kernels; the
     // factors are eq tables of tau by construction, so no device check.
+    const auto t_outer = std::chrono::steady_clock::now();
     TRY(outer_begin_impl(&j->outer, reinterpret_cast<const bitz_f128*>(j->dlo.p), split,
                          reinterpret_cast<const bitz_f128*>(j->dhi.p), row_vars - split, az, row_vars, bz, row_vars,
                          cz, row_vars, BITZ_DEVICE, false));
@@ -3675,6 +3870,12 @@ static int32_t engine_submit(bitz_piop_engine* e, uint64_t tag, bitz_transcript*
     j->rx.resize(row_vars);
     j->inner_rounds.resize(3 * (size_t)col_vars);
     j->inner_point.resize(col_vars);
+    prof_add_host("submit_outer", t_outer);
+    const auto t_devfs = std::chrono::steady_clock::now();
+    if (bits && t->t.type == Transcript::Type::kKeccak && g_device_fs) TRY(devfs_setup(j.get(), s));
+    prof_add_host("submit_devfs", t_devfs);
+    HIP_TRY(hipEventCreateWithFlags(&j->ready, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(j->ready, s));
     if (e->running) {
         {
             std::lock_guard<std::mutex> lock(e->mu);
@@ -3703,6 +3904,46 @@ extern "C" int32_t bitz_piop_engine_submit_packed(bitz_piop_engine* e, uint64_t 
     return engine_submit(e, tag, t, m, digest, az, bz, cz, row_vars, nullptr, assignment_bits, col_vars);
 }
 
+extern "C" int32_t bitz_piop_engine_submit_assignment(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t,
+                                                      bitz_matrices* m, const uint8_t digest[32],
+                                                      const uint64_t* words, size_t bit_len, size_t columns) {
+    if (!e || !t || !m || !digest || (bit_len && !words)) return fail(BITZ_ERR_ARG, "null pointer");
+    // build_assignment_mle (matrix.rs:392-420) checks, then the column domain.
+    if (bit_len != columns) return fail(BITZ_ERR_ASSIGNMENT_LEN, "assignment length differs from the columns");
+    if (columns == 0 || !(words[0] & 1)) return fail(BITZ_ERR_ASSIGNMENT_CONST, "the first assignment bit must be one");
+    size_t padded = 1;
+    while (padded < columns) padded <<= 1;
+    if (padded != m->cols) return fail(BITZ_ERR_ASSIGNMENT_DIMS, "assignment does not span the columns");
+    const auto t_sub = std::chrono::steady_clock::now();
+    auto j = std::make_unique<PiopJob>();
+    TRY(j->prep.create());
+    hipStream_t s = j->prep.s;
+    // The packed bits (zero past the assignment) and Az, Bz, Cz from them.
+    const size_t nw = (padded + 63) / 64, used = (bit_len + 63) / 64;
+    TRY(j->pin_bits.alloc(nw * 8));
+    uint64_t* packed = reinterpret_cast<uint64_t*>(j->pin_bits.p);
+    memcpy(packed, words, used * 8);
+    if (bit_len % 64) packed[used - 1] &= (~0ull) >> (64 - bit_len % 64);
+    memset(packed + used, 0, (nw - used) * 8);
+    TRY(j->own_bits.alloc(nw * 8));
+    const auto t_up = std::chrono::steady_clock::now();
+    HIP_TRY(hipMemcpyAsync(j->own_bits.p, packed, nw * 8, hipMemcpyHostToDevice, s));
+    prof_add_host("submit_upload", t_up);
+    const uint64_t* bits = static_cast<const uint64_t*>(j->own_bits.p);
+    TRY(j->own_prod.alloc(3 * m->rows_padded * sizeof(f128)));
+    for (int k = 0; k < 3; ++k)
+        hipLaunchKernelGGL(bitz::k_r1cs_rows_bits, dim3(grid_for(m->rows_padded)), dim3(kBlock), 0, s, m->rs[k].u32(),
+                           m->cs[k].u32(), m->vs[k].f(), bits, m->rows, j->own_prod.f() + k * m->rows_padded,
+                           m->rows_padded);
+    TRY(check_launch());
+    prof_add_host("submit_prepare", t_sub);
+    const bitz_f128* prod = reinterpret_cast<const bitz_f128*>(j->own_prod.p);
+    const int32_t rc = engine_submit(e, tag, t, m, digest, prod, prod + m->rows_padded, prod + 2 * m->rows_padded,
+                                     (uint32_t)m->row_vars, nullptr, bits, (uint32_t)m->col_vars, std::move(j));
+    prof_add_host("submit_total", t_sub);
+    return rc;
+}
+
 // Inner round K's challenges r_0 .. r_{K-1} as elements.
 static std::vector<f128> job_inner_point(const PiopJob* j, int K) {
     std::vector<f128> r((size_t)std::max(K, 1));
@@ -3710,35 +3951,275 @@ static std::vector<f128> job_inner_point(const PiopJob* j, int K) {
     return r;
 }
 
-extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched) {
-    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
-    if (e->running && std::this_thread::get_id() != e->worker.get_id())
-        return fail(BITZ_ERR_STATE, "the engine steps in the background");
-    if (!e->launched.empty()) return fail(BITZ_ERR_STATE, "a step is already in flight");
+// The kernels of one position of a step: every round of one kind across the
+// proofs at that position in one launch.
+namespace {
+constexpr int kBitRounds = bitz::kInnerBitsMaxRound + 1;
+struct EnginePos {
+    std::vector<bitz::RoundDesc> o_r0, o_fold;
+    std::vector<uint32_t> o_r0_map, o_fold_map;
+    std::vector<bitz::InnerRoundDesc> i_r0, i_fold;
+    std::vector<uint32_t> i_r0_map, i_fold_map;
+    std::vector<bitz::InnerBitsDesc> i_bits[kBitRounds];
+    std::vector<uint32_t> i_bits_map[kBitRounds];
+    std::vector<bitz::FinalFoldDesc> finals;
+    std::vector<bitz::PiopFsDesc> fs;
+    std::vector<size_t> binds;       // proofs (indices into the step's list) that bind here
+    std::vector<size_t> host_final;  // host-transcript proofs' final folds (position 0)
+    // staging offsets
+    size_t or_o = 0, orm_o = 0, of_o = 0, ofm_o = 0, ir_o = 0, irm_o = 0, if_o = 0, ifm_o = 0, fin_o = 0, fs_o = 0;
+    size_t ib_o[kBitRounds] = {}, ibm_o[kBitRounds] = {};
+    void add_bits(int K, bitz::InnerBitsDesc d) {
+        d.blocks = bits_grid(d.pairs);
+        d.block_base = (uint32_t)i_bits_map[K].size();
+        i_bits_map[K].insert(i_bits_map[K].end(), d.blocks, (uint32_t)i_bits[K].size());
+        i_bits[K].push_back(d);
+    }
+    void add_fold(bitz::InnerRoundDesc d) {
+        d.blocks = inner_grid(d.pairs);
+        d.block_base = (uint32_t)i_fold_map.size();
+        i_fold_map.insert(i_fold_map.end(), d.blocks, (uint32_t)i_fold.size());
+        i_fold.push_back(d);
+    }
+    void add_outer(int k, const bitz_outer* h, bitz::RoundDesc d) {
+        const auto& g = h->geo[k];
+        d.a = h->cur[0];
+        d.b = h->cur[1];
+        d.c = h->cur[2];
+        d.la = h->table_ptr(g.la.table);
+        d.lb = h->table_ptr(g.lb.table);
+        d.h_tabs4 = static_cast<const uint4*>(h->htabs4.p) + g.htab_off * kConstTable4Entries;
+        d.threads = g.threads;
+        d.iters = g.iters;
+        d.la_bits = g.la_bits;
+        d.la_shift = g.la.a - h->table_base(g.la.table);
+        d.lb_shift = g.lb.a - h->table_base(g.lb.table);
+        auto& descs = k == 0 ? o_r0 : o_fold;
+        auto& map = k == 0 ? o_r0_map : o_fold_map;
+        const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
+        d.block_base = (uint32_t)map.size();
+        map.insert(map.end(), (g.threads + block - 1) / block, (uint32_t)descs.size());
+        descs.push_back(d);
+    }
+};
+
+// A device-transcript proof's next action, appended to position P: its
+// round's kernels and the step of its transcript (k_piop_fs), then its host
+// cursor moves on (nothing it needs comes back from the device).
+void devfs_emit(PiopJob* j, size_t i, EnginePos& P) {
+    bitz::PiopFsState* st = j->fs_state();
+    switch (j->phase) {
+        case JobPhase::kOuter: {
+            bitz_outer* h = j->outer;
+            const int k = j->round;
+            bitz::RoundDesc d{};
+            if (k > 0) {
+                Dev& dst = (h->where == 1) ? h->Q : h->P;
+                const size_t cap = (h->where == 1) ? h->capQ : h->capP;
+                d.ao = dst.f();
+                d.bo = dst.f() + cap;
+                d.co = dst.f() + 2 * cap;
+                d.r_tab4 = j->fs_rtab();
+            }
+            const bool q1 = k == 0 || hostf::is_zero(h->tau[k]);
+            d.q1 = q1 ? 1 : 0;
+            d.acc = j->fs_slot_outer(k);
+            P.add_outer(k, h, d);
+            bitz::PiopFsDesc f{};
+            f.st = st;
+            f.kind = bitz::kFsOuterRound;
+            f.k = k;
+            f.K = -1;
+            f.have_q1 = q1 ? 1 : 0;
+            f.sums = j->fs_slot_outer(k);
+            f.round_out = j->fs_out(PiopJob::kOutOuter) + 4 * (size_t)k;
+            f.point_out = j->fs_out(PiopJob::kOutRx) + k;
+            f.rtab = j->fs_rtab();
+            f.tau = h->tau[k];
+            f.tau_inv = h->tau_inv[k];
+            f.scale = h->scale_big[k];
+            P.fs.push_back(f);
+            if (k > 0) {
+                h->cur[0] = d.ao;
+                h->cur[1] = d.bo;
+                h->cur[2] = d.co;
+                h->where = (h->where == 1) ? 2 : 1;
+                h->len /= 2;
+                h->round += 1;
+            }
+            j->round = k + 1;
+            if (j->round == j->n) j->phase = JobPhase::kOuterFinal;
+            break;
+        }
+        case JobPhase::kOuterFinal: {
+            bitz_outer* h = j->outer;
+            bitz::FinalFoldDesc d{};
+            for (int u = 0; u < 3; ++u) d.tables[u] = h->cur[u];
+            d.r = j->fs_out(PiopJob::kOutRx) + (j->n - 1);
+            d.out = reinterpret_cast<f128*>(j->fs_slot_outer(j->n));
+            d.count = 3;
+            P.finals.push_back(d);
+            bitz::PiopFsDesc f{};
+            f.st = st;
+            f.kind = bitz::kFsOuterFinal;
+            f.sums = j->fs_slot_outer(j->n);
+            f.round_out = j->fs_out(PiopJob::kOutClaims);
+            P.fs.push_back(f);
+            j->phase = JobPhase::kBind;
+            j->round = 0;
+            break;
+        }
+        case JobPhase::kBind: {
+            // bind_and_batch at the outer point and rho (device), then inner round 0.
+            P.binds.push_back(i);
+            bitz::InnerBitsDesc d{};
+            d.d = j->dD.f();
+            d.bits = j->hbits;
+            d.pairs = j->ilen / 2;
+            d.live = j->ilen;
+            d.acc = j->fs_slot_bits(0);
+            if (j->kb == 0) {  // one round: h as elements for the final fold
+                d.wout = j->iP.f() + j->icapP;
+                d.mono_dev = st->mono;
+            }
+            P.add_bits(0, d);
+            bitz::PiopFsDesc f{};
+            f.st = st;
+            f.kind = bitz::kFsInnerRound;
+            f.k = 0;
+            f.K = 0;
+            f.mono_next = 0 < j->kb;
+            f.sums = j->fs_slot_bits(0);
+            f.round_out = j->fs_out(PiopJob::kOutInner);
+            f.point_out = j->fs_out(PiopJob::kOutPoint);
+            f.rtab = j->fs_rtab();
+            P.fs.push_back(f);
+            j->icur[0] = j->dD.f();
+            j->icur[1] = j->kb == 0 ? d.wout : nullptr;
+            j->round = 1;
+            j->phase = j->nc == 1 ? JobPhase::kInnerFinal : JobPhase::kInner;
+            break;
+        }
+        case JobPhase::kInner: {
+            const int k = j->round;
+            const size_t next_len = j->ilen / 2;
+            Dev& dst = (j->iwhere == 1) ? j->iQ : j->iP;
+            const size_t cap = (j->iwhere == 1) ? j->icapQ : j->icapP;
+            bitz::PiopFsDesc f{};
+            f.st = st;
+            f.kind = bitz::kFsInnerRound;
+            f.k = k;
+            f.round_out = j->fs_out(PiopJob::kOutInner) + 3 * (size_t)k;
+            f.point_out = j->fs_out(PiopJob::kOutPoint) + k;
+            f.rtab = j->fs_rtab();
+            f128* wout = nullptr;
+            if (k <= j->kb) {
+                bitz::InnerBitsDesc d{};
+                d.d = j->icur[0];
+                d.bits = j->hbits;
+                d.dout = dst.f();
+                wout = k == j->kb ? dst.f() + cap : nullptr;
+                d.wout = wout;
+                d.mono_dev = st->mono;
+                d.pairs = next_len / 2;  // bounds; the kernel takes the live length from the device
+                d.live = j->ilen;
+                d.live0 = &st->live;
+                d.rtab4 = j->fs_rtab();
+                d.acc = j->fs_slot_bits(k);
+                P.add_bits(k, d);
+                f.K = k;
+                f.mono_next = k < j->kb;
+                f.sums = j->fs_slot_bits(k);
+            } else {
+                bitz::InnerRoundDesc d{};
+                d.d = j->icur[0];
+                d.w = j->icur[1];
+                d.dout = dst.f();
+                d.wout = dst.f() + cap;
+                wout = d.wout;
+                d.pairs = next_len / 2;
+                d.live = j->ilen;
+                d.live0 = &st->live;
+                d.live_shift = k - 1;
+                d.rtab4 = j->fs_rtab();
+                d.acc = j->fs_slot_gen(k);
+                P.add_fold(d);
+                f.K = -1;
+                f.sums = j->fs_slot_gen(k);
+            }
+            P.fs.push_back(f);
+            j->icur[0] = dst.f();
+            j->icur[1] = wout;
+            j->iwhere = (j->iwhere == 1) ? 2 : 1;
+            j->ilen = next_len;
+            j->round = k + 1;
+            if (j->round == j->nc) j->phase = JobPhase::kInnerFinal;
+            break;
+        }
+        case JobPhase::kInnerFinal: {
+            bitz::FinalFoldDesc d{};
+            d.tables[0] = j->icur[0];
+            d.tables[1] = j->icur[1];
+            d.r = j->fs_out(PiopJob::kOutPoint) + (j->nc - 1);
+            d.out = reinterpret_cast<f128*>(j->fs_slot_final());
+            d.count = 2;
+            P.finals.push_back(d);
+            bitz::PiopFsDesc f{};
+            f.st = st;
+            f.kind = bitz::kFsInnerFinal;
+            f.sums = j->fs_slot_final();
+            f.round_out = j->fs_out(PiopJob::kOutMatrixEval);
+            f.final_out = j->fs_out(PiopJob::kOutFinal);
+            P.fs.push_back(f);
+            j->phase = JobPhase::kDevDone;
+            break;
+        }
+        default:
+            break;
+    }
+}
+}  // namespace
+
+// Queues a step (every active proof) on the engine's stream behind any step
+// in flight; *launched = the proofs in it (none queued when zero).
+static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
     const auto host_t0 = std::chrono::steady_clock::now();
     hipStream_t s = e->so.s;
     std::vector<PiopJob*> act;
+    bool any_devfs = false;
     for (auto& j : e->jobs)
-        if (j->phase != JobPhase::kDone) act.push_back(j.get());
+        if (j->phase != JobPhase::kDone && j->phase != JobPhase::kDevDone) {
+            act.push_back(j.get());
+            any_devfs |= j->devfs;
+        }
     if (launched) *launched = act.size();
     if (act.empty()) return BITZ_OK;
-    // Each proof's slot for the step's sums: a bit round's copies
-    // (kInnerBitsAccWords), else kEngineSlotWords words.
-    e->slot_off.assign(act.size() + 1, 0);
+    EngineStep step;
+    step.buf = e->next_buf;
+    // Positions: a host-transcript proof does one round per step (its
+    // challenge comes from the host after the read-back); a device-transcript
+    // proof does up to kEnginePositions.
+    const int npos = any_devfs ? kEnginePositions : 1;
+    std::vector<EnginePos> pos(npos);
+    // Each host-transcript proof's slot for the step's sums: a bit round's
+    // copies (kInnerBitsAccWords), else kEngineSlotWords words.
+    std::vector<size_t>& slot_off = step.slot_off;
+    slot_off.assign(act.size() + 1, 0);
     for (size_t i = 0; i < act.size(); ++i) {
         const PiopJob* j = act[i];
-        const bool bits = j->hbits && ((j->phase == JobPhase::kBind) ||
-                                       (j->phase == JobPhase::kInner && j->round <= j->kb));
-        e->slot_off[i + 1] = e->slot_off[i] + (bits ? bitz::kInnerBitsAccWords : kEngineSlotWords);
+        const bool bits = !j->devfs && j->hbits &&
+                          ((j->phase == JobPhase::kBind) || (j->phase == JobPhase::kInner && j->round <= j->kb));
+        slot_off[i + 1] = slot_off[i] + (j->devfs ? 0 : bits ? bitz::kInnerBitsAccWords : kEngineSlotWords);
     }
-    const size_t slot_bytes = e->slot_off[act.size()] * 4;
-    if (e->slots.bytes < slot_bytes) TRY(e->slots.alloc(slot_bytes));
-    auto slot = [&](size_t i) { return e->slots.u32() + e->slot_off[i]; };
-    // Challenge tables to build: the r each fold round folds by.
+    const size_t slot_bytes = std::max<size_t>(slot_off[act.size()] * 4, 256);
+    if (slot_off[act.size()] && e->slots.bytes < slot_bytes) TRY(e->slots.alloc(slot_bytes));
+    auto slot = [&](size_t i) { return e->slots.u32() + slot_off[i]; };
+    // Challenge tables to build: the r each host-transcript fold round folds by.
     std::vector<f128> consts;
     std::vector<int> table_of(act.size(), -1);
     for (size_t i = 0; i < act.size(); ++i) {
         const PiopJob* j = act[i];
+        if (j->devfs) continue;
         const bool outer_fold = j->phase == JobPhase::kOuter && j->round > 0;
         const bool inner_fold = j->phase == JobPhase::kInner && j->round > 0;
         if (outer_fold || inner_fold) {
@@ -3749,24 +4230,11 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
     if (e->rt4.bytes < std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4))
         TRY(e->rt4.alloc(std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4)));
     const uint4* rt4 = static_cast<const uint4*>(e->rt4.p);
-    // Descriptors of the batched launches.
-    std::vector<bitz::RoundDesc> o_r0, o_fold;
-    std::vector<uint32_t> o_r0_map, o_fold_map;
-    std::vector<bitz::InnerRoundDesc> i_r0, i_fold;
-    std::vector<uint32_t> i_r0_map, i_fold_map;
-    // Inner rounds over packed bits, by round index K.
-    constexpr int kBitRounds = bitz::kInnerBitsMaxRound + 1;
-    std::vector<bitz::InnerBitsDesc> i_bits[kBitRounds];
-    std::vector<uint32_t> i_bits_map[kBitRounds];
-    auto add_bits = [&](int K, bitz::InnerBitsDesc d) {
-        d.blocks = bits_grid(d.pairs);
-        d.block_base = (uint32_t)i_bits_map[K].size();
-        i_bits_map[K].insert(i_bits_map[K].end(), d.blocks, (uint32_t)i_bits[K].size());
-        i_bits[K].push_back(d);
-    };
-    std::vector<size_t> bind_jobs;
+    // Host-transcript proofs: position 0.
+    EnginePos& P0 = pos[0];
     for (size_t i = 0; i < act.size(); ++i) {
         PiopJob* j = act[i];
+        if (j->devfs) continue;
         if (j->phase == JobPhase::kOuter) {
             bitz_outer* h = j->outer;
             const int k = j->round;
@@ -3784,28 +4252,11 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
                 d.r_tab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
             }
             j->q1 = k == 0 || hostf::is_zero(h->tau[k]);
-            const auto& g = h->geo[k];
-            d.a = h->cur[0];
-            d.b = h->cur[1];
-            d.c = h->cur[2];
-            d.la = h->table_ptr(g.la.table);
-            d.lb = h->table_ptr(g.lb.table);
-            d.h_tabs4 = static_cast<const uint4*>(h->htabs4.p) + g.htab_off * kConstTable4Entries;
             d.acc = slot(i);
-            d.threads = g.threads;
-            d.iters = g.iters;
-            d.la_bits = g.la_bits;
-            d.la_shift = g.la.a - h->table_base(g.la.table);
-            d.lb_shift = g.lb.a - h->table_base(g.lb.table);
             d.q1 = j->q1 ? 1 : 0;
-            auto& descs = k == 0 ? o_r0 : o_fold;
-            auto& map = k == 0 ? o_r0_map : o_fold_map;
-            const unsigned block = k == 0 ? bitz::kR0Block : bitz::kFoldBlock;
-            d.block_base = (uint32_t)map.size();
-            map.insert(map.end(), (g.threads + block - 1) / block, (uint32_t)descs.size());
-            descs.push_back(d);
+            P0.add_outer(k, h, d);
         } else if (j->phase == JobPhase::kBind && j->hbits) {
-            bind_jobs.push_back(i);
+            P0.binds.push_back(i);
             bitz::InnerBitsDesc d{};
             d.d = j->dD.f();
             d.bits = j->hbits;
@@ -3819,7 +4270,7 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
                 d.mono[0] = hostf::one();
                 j->pend[1] = d.wout;
             }
-            add_bits(0, d);
+            P0.add_bits(0, d);
         } else if (j->phase == JobPhase::kInner && j->hbits && j->round <= j->kb) {
             const int K = j->round;
             const size_t next_len = j->ilen / 2;
@@ -3840,9 +4291,9 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
                 const std::vector<f128> r = job_inner_point(j, K);
                 monomials(r.data(), K, d.mono);
             }
-            add_bits(K, d);
+            P0.add_bits(K, d);
         } else if (j->phase == JobPhase::kBind) {
-            bind_jobs.push_back(i);
+            P0.binds.push_back(i);
             bitz::InnerRoundDesc d{};
             d.d = j->dD.f();
             d.w = j->hv;
@@ -3851,9 +4302,9 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
             d.live_out = reinterpret_cast<unsigned long long*>(slot(i) + 8);  // the tables' live length
             d.acc = slot(i);
             d.blocks = inner_grid(d.pairs);
-            d.block_base = (uint32_t)i_r0_map.size();
-            i_r0_map.insert(i_r0_map.end(), d.blocks, (uint32_t)i_r0.size());
-            i_r0.push_back(d);
+            d.block_base = (uint32_t)P0.i_r0_map.size();
+            P0.i_r0_map.insert(P0.i_r0_map.end(), d.blocks, (uint32_t)P0.i_r0.size());
+            P0.i_r0.push_back(d);
         } else if (j->phase == JobPhase::kInner) {
             const size_t next_len = j->ilen / 2;
             Dev& dst = (j->iwhere == 1) ? j->iQ : j->iP;
@@ -3869,13 +4320,21 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
             d.live = j->ilive;
             d.rtab4 = rt4 + (size_t)table_of[i] * kConstTable4Entries;
             d.acc = slot(i);
-            d.blocks = inner_grid(d.pairs);
-            d.block_base = (uint32_t)i_fold_map.size();
-            i_fold_map.insert(i_fold_map.end(), d.blocks, (uint32_t)i_fold.size());
-            i_fold.push_back(d);
+            P0.add_fold(d);
+        } else if (j->phase == JobPhase::kOuterFinal || j->phase == JobPhase::kInnerFinal) {
+            P0.host_final.push_back(i);
         }
     }
-    // Staging: challenges | row points of the binds | descriptors and maps.
+    // Device-transcript proofs: one action per position while any is left.
+    std::vector<size_t> finishing;  // device-transcript proofs that end in this step
+    for (size_t i = 0; i < act.size(); ++i) {
+        PiopJob* j = act[i];
+        if (!j->devfs) continue;
+        for (int u = 0; u < npos && j->phase != JobPhase::kDevDone; ++u) devfs_emit(j, i, pos[u]);
+        if (j->phase == JobPhase::kDevDone) finishing.push_back(i);
+    }
+    // Staging: challenges | host-transcript row points | every position's
+    // descriptors and maps.
     size_t off = 0;
     auto place = [&](size_t bytes) {
         off = (off + 255) & ~(size_t)255;
@@ -3884,75 +4343,83 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
         return o;
     };
     const size_t c_o = place(consts.size() * sizeof(f128));
-    std::vector<size_t> rx_o(bind_jobs.size());
-    for (size_t b = 0; b < bind_jobs.size(); ++b) rx_o[b] = place((size_t)act[bind_jobs[b]]->n * sizeof(f128));
-    const size_t or_o = place(o_r0.size() * sizeof(bitz::RoundDesc)), orm_o = place(o_r0_map.size() * 4);
-    const size_t of_o = place(o_fold.size() * sizeof(bitz::RoundDesc)), ofm_o = place(o_fold_map.size() * 4);
-    const size_t ir_o = place(i_r0.size() * sizeof(bitz::InnerRoundDesc)), irm_o = place(i_r0_map.size() * 4);
-    const size_t if_o = place(i_fold.size() * sizeof(bitz::InnerRoundDesc)), ifm_o = place(i_fold_map.size() * 4);
-    size_t ib_o[kBitRounds], ibm_o[kBitRounds];
-    for (int K = 0; K < kBitRounds; ++K) {
-        ib_o[K] = place(i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
-        ibm_o[K] = place(i_bits_map[K].size() * 4);
+    std::vector<size_t> rx_o(act.size(), 0);
+    for (size_t b : P0.binds)
+        if (!act[b]->devfs) rx_o[b] = place((size_t)act[b]->n * sizeof(f128));
+    for (auto& P : pos) {
+        P.or_o = place(P.o_r0.size() * sizeof(bitz::RoundDesc));
+        P.orm_o = place(P.o_r0_map.size() * 4);
+        P.of_o = place(P.o_fold.size() * sizeof(bitz::RoundDesc));
+        P.ofm_o = place(P.o_fold_map.size() * 4);
+        P.ir_o = place(P.i_r0.size() * sizeof(bitz::InnerRoundDesc));
+        P.irm_o = place(P.i_r0_map.size() * 4);
+        P.if_o = place(P.i_fold.size() * sizeof(bitz::InnerRoundDesc));
+        P.ifm_o = place(P.i_fold_map.size() * 4);
+        for (int K = 0; K < kBitRounds; ++K) {
+            P.ib_o[K] = place(P.i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
+            P.ibm_o[K] = place(P.i_bits_map[K].size() * 4);
+        }
+        P.fin_o = place(P.finals.size() * sizeof(bitz::FinalFoldDesc));
+        P.fs_o = place(P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
-    // One pinned buffer stages the step's uploads, then receives its sums.
+    // Read-back: the host-transcript slots, then each finishing proof's state
+    // and outputs.
+    size_t back = slot_off[act.size()] * 4;
+    std::vector<size_t> fin_back(finishing.size());
+    for (size_t f = 0; f < finishing.size(); ++f) {
+        back = (back + 255) & ~(size_t)255;
+        fin_back[f] = back;
+        back += act[finishing[f]]->fs_back_bytes();
+    }
+    // One pinned buffer stages the step's uploads, then receives what comes back.
     const size_t total = std::max<size_t>(off, 256);
-    const size_t need = std::max(total, slot_bytes);
-    if (e->pin_bytes < need) {
-        if (e->pin) (void)hipHostFree(e->pin);
-        e->pin_bytes = 0;
-        if (hipHostMalloc(&e->pin, 2 * need, hipHostMallocDefault) != hipSuccess) {
-            e->pin = nullptr;
+    const size_t need = std::max(total, back);
+    const int b = step.buf;
+    if (e->pin_bytes[b] < need) {
+        // A buffer the step before last used: that step has been finished.
+        if (e->pin[b]) (void)hipHostFree(e->pin[b]);
+        e->pin_bytes[b] = 0;
+        if (hipHostMalloc(&e->pin[b], 2 * need, hipHostMallocDefault) != hipSuccess) {
+            e->pin[b] = nullptr;
             return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
         }
-        e->pin_bytes = 2 * need;
+        e->pin_bytes[b] = 2 * need;
     }
-    if (e->stage.bytes < total) TRY(e->stage.alloc(total));
-    char* pin = static_cast<char*>(e->pin);
-    char* dst = static_cast<char*>(e->stage.p);
+    if (e->stage[b].bytes < total) TRY(e->stage[b].alloc(total));
+    if (!e->step_done[b]) HIP_TRY(hipEventCreateWithFlags(&e->step_done[b], hipEventDisableTiming));
+    char* pin = static_cast<char*>(e->pin[b]);
+    char* dst = static_cast<char*>(e->stage[b].p);
     memcpy(pin + c_o, consts.data(), consts.size() * sizeof(f128));
-    for (size_t b = 0; b < bind_jobs.size(); ++b) {
-        const PiopJob* j = act[bind_jobs[b]];
-        memcpy(pin + rx_o[b], j->rx.data(), (size_t)j->n * sizeof(f128));
-    }
-    memcpy(pin + or_o, o_r0.data(), o_r0.size() * sizeof(bitz::RoundDesc));
-    memcpy(pin + orm_o, o_r0_map.data(), o_r0_map.size() * 4);
-    memcpy(pin + of_o, o_fold.data(), o_fold.size() * sizeof(bitz::RoundDesc));
-    memcpy(pin + ofm_o, o_fold_map.data(), o_fold_map.size() * 4);
-    memcpy(pin + ir_o, i_r0.data(), i_r0.size() * sizeof(bitz::InnerRoundDesc));
-    memcpy(pin + irm_o, i_r0_map.data(), i_r0_map.size() * 4);
-    memcpy(pin + if_o, i_fold.data(), i_fold.size() * sizeof(bitz::InnerRoundDesc));
-    memcpy(pin + ifm_o, i_fold_map.data(), i_fold_map.size() * 4);
-    for (int K = 0; K < kBitRounds; ++K) {
-        memcpy(pin + ib_o[K], i_bits[K].data(), i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
-        memcpy(pin + ibm_o[K], i_bits_map[K].data(), i_bits_map[K].size() * 4);
+    for (size_t b : P0.binds)
+        if (!act[b]->devfs) memcpy(pin + rx_o[b], act[b]->rx.data(), (size_t)act[b]->n * sizeof(f128));
+    for (auto& P : pos) {
+        memcpy(pin + P.or_o, P.o_r0.data(), P.o_r0.size() * sizeof(bitz::RoundDesc));
+        memcpy(pin + P.orm_o, P.o_r0_map.data(), P.o_r0_map.size() * 4);
+        memcpy(pin + P.of_o, P.o_fold.data(), P.o_fold.size() * sizeof(bitz::RoundDesc));
+        memcpy(pin + P.ofm_o, P.o_fold_map.data(), P.o_fold_map.size() * 4);
+        memcpy(pin + P.ir_o, P.i_r0.data(), P.i_r0.size() * sizeof(bitz::InnerRoundDesc));
+        memcpy(pin + P.irm_o, P.i_r0_map.data(), P.i_r0_map.size() * 4);
+        memcpy(pin + P.if_o, P.i_fold.data(), P.i_fold.size() * sizeof(bitz::InnerRoundDesc));
+        memcpy(pin + P.ifm_o, P.i_fold_map.data(), P.i_fold_map.size() * 4);
+        for (int K = 0; K < kBitRounds; ++K) {
+            memcpy(pin + P.ib_o[K], P.i_bits[K].data(), P.i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
+            memcpy(pin + P.ibm_o[K], P.i_bits_map[K].data(), P.i_bits_map[K].size() * 4);
+        }
+        memcpy(pin + P.fin_o, P.finals.data(), P.finals.size() * sizeof(bitz::FinalFoldDesc));
+        memcpy(pin + P.fs_o, P.fs.data(), P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
     e->t_step.begin(s, "piop_engine_step", (double)act.size());  // "bytes": proofs in the step
+    for (PiopJob* j : act)
+        if (j->ready) {  // a new proof: its set-up, queued on its own stream, first
+            HIP_TRY(hipStreamWaitEvent(s, j->ready, 0));
+            (void)hipEventDestroy(j->ready);
+            j->ready = nullptr;
+        }
     HIP_TRY(hipMemcpyAsync(dst, pin, off, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(e->slots.p, 0, slot_bytes, s));
+    if (slot_off[act.size()]) HIP_TRY(hipMemsetAsync(e->slots.p, 0, slot_off[act.size()] * 4, s));
     if (!consts.empty())
         hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)consts.size()), dim3(512), 0, s,
                            reinterpret_cast<const f128*>(dst + c_o), static_cast<uint4*>(e->rt4.p));
-    // Per proof: binds (then inner round 0 reads D) and the final folds.
-    for (size_t b = 0; b < bind_jobs.size(); ++b) {
-        PiopJob* j = act[bind_jobs[b]];
-        std::lock_guard<std::mutex> lock(j->m->mu);
-        HIP_TRY(hipMemcpyAsync(j->m->point.p, dst + rx_o[b], (size_t)j->n * sizeof(f128), hipMemcpyDeviceToDevice,
-                               s));
-        TRY(bind_enqueue(j->m, j->rho, j->dD.f(), s));
-    }
-    for (size_t i = 0; i < act.size(); ++i) {
-        PiopJob* j = act[i];
-        f128* out = reinterpret_cast<f128*>(slot(i));
-        if (j->phase == JobPhase::kOuterFinal) {
-            bitz_outer* h = j->outer;
-            hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2], out,
-                               out + 1, out + 2, (size_t)1, j->r);
-        } else if (j->phase == JobPhase::kInnerFinal) {
-            hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[0], out, (size_t)1, j->r);
-            hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[1], out + 1, (size_t)1, j->r);
-        }
-    }
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
@@ -3965,76 +4432,163 @@ extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
         attr = true;
     }
-    if (!o_r0.empty())
-        hipLaunchKernelGGL(bitz::k_batch_r0, dim3((unsigned)o_r0_map.size()), dim3(bitz::kR0Block), bitz::kR0Smem,
-                           s, reinterpret_cast<const bitz::RoundDesc*>(dst + or_o),
-                           reinterpret_cast<const uint32_t*>(dst + orm_o));
-    if (!o_fold.empty())
-        hipLaunchKernelGGL(bitz::k_batch_fold, dim3((unsigned)o_fold_map.size()), dim3(bitz::kFoldBlock),
-                           bitz::kFoldSmem, s, reinterpret_cast<const bitz::RoundDesc*>(dst + of_o),
-                           reinterpret_cast<const uint32_t*>(dst + ofm_o));
-    if (!i_r0.empty())
-        hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)i_r0_map.size()), dim3(bitz::kInnerBlock),
-                           bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + ir_o),
-                           reinterpret_cast<const uint32_t*>(dst + irm_o));
-    for (int K = 0; K < kBitRounds; ++K) {
-        if (i_bits[K].empty()) continue;
-        const auto* dd = reinterpret_cast<const bitz::InnerBitsDesc*>(dst + ib_o[K]);
-        const auto* mm = reinterpret_cast<const uint32_t*>(dst + ibm_o[K]);
-        const dim3 grid((unsigned)i_bits_map[K].size()), block(bitz::kInnerBlock);
-        const size_t smem = K ? bitz::kInnerBitsSmem : 0;
-        // Algorithmic bytes: round 0 reads D (32 B per pair) and the bits;
-        // round K >= 1 reads 4 entries of D_{K-1} and writes 2 of D_K per
-        // output pair (and 2 of h_K when it writes them).
-        double bits_bytes = 0;
-        for (const auto& d : i_bits[K])
-            bits_bytes += (K ? 96.0 + (d.wout ? 32.0 : 0.0) : 32.0) * (double)d.pairs + (double)d.pairs * (2 << K) / 8;
-        static const char* const kNames[kBitRounds] = {"inner_bits_batch_0", "inner_bits_batch_1",
-                                                       "inner_bits_batch_2", "inner_bits_batch_3"};
-        e->t_bits[K].begin(s, kNames[K], bits_bytes);
-        switch (K) {
-            case 0: hipLaunchKernelGGL(bitz::k_inner_bits_batch<0>, grid, block, smem, s, dd, mm); break;
-            case 1: hipLaunchKernelGGL(bitz::k_inner_bits_batch<1>, grid, block, smem, s, dd, mm); break;
-            case 2: hipLaunchKernelGGL(bitz::k_inner_bits_batch<2>, grid, block, smem, s, dd, mm); break;
-            default: hipLaunchKernelGGL(bitz::k_inner_bits_batch<3>, grid, block, smem, s, dd, mm); break;
+    for (int u = 0; u < npos; ++u) {
+        EnginePos& P = pos[u];
+        // Binds (inner round 0 then reads D) and the final folds.
+        for (size_t b : P.binds) {
+            PiopJob* j = act[b];
+            std::lock_guard<std::mutex> lock(j->m->mu);
+            if (j->devfs)
+                TRY(bind_enqueue(j->m, j->fs_out(PiopJob::kOutRx), true, f128{0, 0}, j->dD.f(), s,
+                                 &j->fs_state()->rho));
+            else
+                TRY(bind_enqueue(j->m, reinterpret_cast<const f128*>(dst + rx_o[b]), true, j->rho, j->dD.f(), s));
         }
-        e->t_bits[K].end(s);
-    }
-    if (!i_fold.empty()) {
-        // Algorithmic bytes: per output pair, 4 + 4 inputs of D and h read and
-        // 2 + 2 folded entries written.
-        double fold_bytes = 0;
-        for (const auto& d : i_fold) fold_bytes += 192.0 * (double)d.pairs;
-        e->t_inner_fold.begin(s, "inner_fold_batch", fold_bytes);
-        hipLaunchKernelGGL(bitz::k_inner_fold_batch, dim3((unsigned)i_fold_map.size()), dim3(bitz::kInnerBlock),
-                           bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + if_o),
-                           reinterpret_cast<const uint32_t*>(dst + ifm_o));
-        e->t_inner_fold.end(s);
+        for (size_t i : P.host_final) {
+            PiopJob* j = act[i];
+            f128* out = reinterpret_cast<f128*>(slot(i));
+            if (j->phase == JobPhase::kOuterFinal) {
+                bitz_outer* h = j->outer;
+                hipLaunchKernelGGL(bitz::k_fold3, dim3(1), dim3(kBlock), 0, s, h->cur[0], h->cur[1], h->cur[2], out,
+                                   out + 1, out + 2, (size_t)1, j->r);
+            } else {
+                hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[0], out, (size_t)1, j->r);
+                hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[1], out + 1, (size_t)1, j->r);
+            }
+        }
+        if (!P.finals.empty())
+            hipLaunchKernelGGL(bitz::k_final_folds, dim3((unsigned)P.finals.size()), dim3(64), 0, s,
+                               reinterpret_cast<const bitz::FinalFoldDesc*>(dst + P.fin_o));
+        if (!P.o_r0.empty())
+            hipLaunchKernelGGL(bitz::k_batch_r0, dim3((unsigned)P.o_r0_map.size()), dim3(bitz::kR0Block),
+                               bitz::kR0Smem, s, reinterpret_cast<const bitz::RoundDesc*>(dst + P.or_o),
+                               reinterpret_cast<const uint32_t*>(dst + P.orm_o));
+        if (!P.o_fold.empty())
+            hipLaunchKernelGGL(bitz::k_batch_fold, dim3((unsigned)P.o_fold_map.size()), dim3(bitz::kFoldBlock),
+                               bitz::kFoldSmem, s, reinterpret_cast<const bitz::RoundDesc*>(dst + P.of_o),
+                               reinterpret_cast<const uint32_t*>(dst + P.ofm_o));
+        if (!P.i_r0.empty())
+            hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)P.i_r0_map.size()), dim3(bitz::kInnerBlock),
+                               bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + P.ir_o),
+                               reinterpret_cast<const uint32_t*>(dst + P.irm_o));
+        for (int K = 0; K < kBitRounds; ++K) {
+            if (P.i_bits[K].empty()) continue;
+            const auto* dd = reinterpret_cast<const bitz::InnerBitsDesc*>(dst + P.ib_o[K]);
+            const auto* mm = reinterpret_cast<const uint32_t*>(dst + P.ibm_o[K]);
+            const dim3 grid((unsigned)P.i_bits_map[K].size()), block(bitz::kInnerBlock);
+            const size_t smem = K ? bitz::kInnerBitsSmem : 0;
+            // Algorithmic bytes (bounds for device-transcript proofs): round 0
+            // reads D (32 B per pair) and the bits; round K >= 1 reads 4
+            // entries of D_{K-1} and writes 2 of D_K per output pair (and 2
+            // of h_K when it writes them).
+            double bits_bytes = 0;
+            for (const auto& d : P.i_bits[K])
+                bits_bytes +=
+                    (K ? 96.0 + (d.wout ? 32.0 : 0.0) : 32.0) * (double)d.pairs + (double)d.pairs * (2 << K) / 8;
+            static const char* const kNames[kBitRounds] = {"inner_bits_batch_0", "inner_bits_batch_1",
+                                                           "inner_bits_batch_2", "inner_bits_batch_3"};
+            e->t_bits[u][K].begin(s, kNames[K], bits_bytes);
+            switch (K) {
+                case 0: hipLaunchKernelGGL(bitz::k_inner_bits_batch<0>, grid, block, smem, s, dd, mm); break;
+                case 1: hipLaunchKernelGGL(bitz::k_inner_bits_batch<1>, grid, block, smem, s, dd, mm); break;
+                case 2: hipLaunchKernelGGL(bitz::k_inner_bits_batch<2>, grid, block, smem, s, dd, mm); break;
+                default: hipLaunchKernelGGL(bitz::k_inner_bits_batch<3>, grid, block, smem, s, dd, mm); break;
+            }
+            e->t_bits[u][K].end(s);
+        }
+        if (!P.i_fold.empty()) {
+            // Algorithmic bytes: per output pair, 4 + 4 inputs of D and h read
+            // and 2 + 2 folded entries written.
+            double fold_bytes = 0;
+            for (const auto& d : P.i_fold) fold_bytes += 192.0 * (double)d.pairs;
+            e->t_inner_fold[u].begin(s, "inner_fold_batch", fold_bytes);
+            hipLaunchKernelGGL(bitz::k_inner_fold_batch, dim3((unsigned)P.i_fold_map.size()),
+                               dim3(bitz::kInnerBlock), bitz::kInnerGridSmem, s,
+                               reinterpret_cast<const bitz::InnerRoundDesc*>(dst + P.if_o),
+                               reinterpret_cast<const uint32_t*>(dst + P.ifm_o));
+            e->t_inner_fold[u].end(s);
+        }
+        if (!P.fs.empty())
+            hipLaunchKernelGGL(bitz::k_piop_fs, dim3((unsigned)P.fs.size()), dim3(64), 0, s,
+                               reinterpret_cast<const bitz::PiopFsDesc*>(dst + P.fs_o));
     }
     TRY(check_launch());
-    HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_bytes, hipMemcpyDeviceToHost, s));
+    if (slot_off[act.size()])
+        HIP_TRY(hipMemcpyAsync(pin, e->slots.p, slot_off[act.size()] * 4, hipMemcpyDeviceToHost, s));
+    for (size_t f = 0; f < finishing.size(); ++f) {
+        PiopJob* j = act[finishing[f]];
+        HIP_TRY(hipMemcpyAsync(pin + fin_back[f], j->fsb.p, j->fs_back_bytes(), hipMemcpyDeviceToHost, s));
+    }
     e->t_step.end(s);
-    e->launched = std::move(act);
+    HIP_TRY(hipEventRecord(e->step_done[b], s));
+    for (size_t f = 0; f < finishing.size(); ++f) step.fin_back.emplace_back(act[finishing[f]], fin_back[f]);
+    step.act = std::move(act);
+    e->steps.push_back(std::move(step));
+    e->next_buf ^= 1;
     prof_add_host("piop_engine_host_launch", host_t0);
     return BITZ_OK;
 }
 
-extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_flight) {
+extern "C" int32_t bitz_piop_engine_launch(bitz_piop_engine* e, size_t* launched) {
     if (!e) return fail(BITZ_ERR_ARG, "null pointer");
-    std::vector<PiopJob*> act = std::move(e->launched);
-    e->launched.clear();
+    if (e->running && std::this_thread::get_id() != e->worker.get_id())
+        return fail(BITZ_ERR_STATE, "the engine steps in the background");
+    if (!e->steps.empty()) return fail(BITZ_ERR_STATE, "a step is already in flight");
+    return engine_launch_step(e, launched);
+}
+
+// Waits for the oldest step in flight, then the host's part: each finishing
+// device-transcript proof's outputs, each host-transcript proof's round.
+static int32_t engine_finish_step(bitz_piop_engine* e, size_t* in_flight) {
+    std::vector<PiopJob*> act;
+    EngineStep step;
+    if (!e->steps.empty()) {
+        step = std::move(e->steps.front());
+        e->steps.pop_front();
+        act = step.act;
+    }
     const auto wait_t0 = std::chrono::steady_clock::now();
-    if (!act.empty()) HIP_TRY(hipStreamSynchronize(e->so.s));
+    if (!act.empty()) HIP_TRY(hipEventSynchronize(e->step_done[step.buf]));
     const auto host_t0 = std::chrono::steady_clock::now();
     if (!act.empty()) prof_add_host("piop_engine_wait", wait_t0);
-    e->t_inner_fold.collect();
+    for (auto& t : e->t_inner_fold) t.collect();
     e->t_step.collect();
-    for (auto& t : e->t_bits) t.collect();
-    const char* pin = static_cast<const char*>(e->pin);
-    // Host: every proof's round polynomial into its transcript, its next challenge.
+    for (auto& tp : e->t_bits)
+        for (auto& t : tp) t.collect();
+    const char* pin = static_cast<const char*>(e->pin[step.buf]);
+    // Device-transcript proofs that ended in this step: their outputs, and
+    // the duplex handed back to the caller's transcript.
+    for (const auto& fb : step.fin_back) {
+        PiopJob* j = fb.first;
+        const char* b = pin + fb.second;
+        bitz::PiopFsState st;
+        memcpy(&st, b, sizeof st);
+        j->t->t.keccak = st.ts;
+        const bitz_f128* o = reinterpret_cast<const bitz_f128*>(b + sizeof st);
+        auto take = [&](int w, bitz_f128* dst, size_t n) { memcpy(dst, o + j->out_off[w], n * sizeof(bitz_f128)); };
+        take(PiopJob::kOutOuter, j->outer_rounds.data(), j->outer_rounds.size());
+        take(PiopJob::kOutRx, j->rx.data(), j->rx.size());
+        take(PiopJob::kOutClaims, j->outer_claims, 3);
+        take(PiopJob::kOutInner, j->inner_rounds.data(), j->inner_rounds.size());
+        take(PiopJob::kOutPoint, j->inner_point.data(), j->inner_point.size());
+        take(PiopJob::kOutMatrixEval, &j->matrix_eval, 1);
+        take(PiopJob::kOutFinal, &j->final_claim, 1);
+        bitz_outer_free(j->outer);
+        j->outer = nullptr;
+        j->dlo.release();
+        j->dhi.release();
+        j->iP.release();
+        j->iQ.release();
+        j->dD.release();
+        j->fsb.release();
+        j->phase = JobPhase::kDone;
+    }
+    // Host-transcript proofs: the round polynomial into the transcript, the
+    // next challenge.
     for (size_t i = 0; i < act.size(); ++i) {
         PiopJob* j = act[i];
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(pin) + e->slot_off[i];
+        if (j->devfs) continue;
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(pin) + step.slot_off[i];
         Transcript& tr = j->t->t;
         switch (j->phase) {
             case JobPhase::kOuter: {
@@ -4122,6 +4676,7 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
                 j->phase = JobPhase::kDone;
                 break;
             }
+            case JobPhase::kDevDone:
             case JobPhase::kDone:
                 break;
         }
@@ -4133,6 +4688,11 @@ extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_fligh
     }
     if (!act.empty()) prof_add_host("piop_engine_host_finish", host_t0);
     return BITZ_OK;
+}
+
+extern "C" int32_t bitz_piop_engine_finish(bitz_piop_engine* e, size_t* in_flight) {
+    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
+    return engine_finish_step(e, in_flight);
 }
 
 extern "C" int32_t bitz_piop_engine_step(bitz_piop_engine* e, size_t* in_flight) {
@@ -4156,28 +4716,41 @@ extern "C" int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, si
 }
 
 // The background thread: steps while proofs are in flight, sleeps otherwise.
+// The background thread: steps while proofs are in flight, sleeps otherwise.
+// When every active proof draws its challenges on the device (and kernel
+// timing is off), the next step is queued before the one in flight is
+// finished, so the device never waits for the host between steps.
 static void piop_engine_worker(bitz_piop_engine* e) {
+    auto launchable = [&] {
+        for (auto& j : e->jobs)
+            if (j->phase != JobPhase::kDone && j->phase != JobPhase::kDevDone) return true;
+        return false;
+    };
+    auto all_devfs = [&] {
+        for (auto& j : e->jobs)
+            if (!j->devfs && j->phase != JobPhase::kDone) return false;
+        return true;
+    };
     for (;;) {
         {
             std::unique_lock<std::mutex> lock(e->mu);
-            auto active = [&] {
-                for (auto& j : e->jobs)
-                    if (j->phase != JobPhase::kDone) return true;
-                return false;
-            };
-            e->cv.wait(lock, [&] { return e->stop || !e->incoming.empty() || active(); });
+            e->cv.wait(lock, [&] { return e->stop || !e->incoming.empty() || launchable() || !e->steps.empty(); });
             for (auto& j : e->incoming) e->jobs.push_back(std::move(j));
             e->incoming.clear();
-            if (e->stop && !active()) return;
+            if (e->stop && !launchable() && e->steps.empty()) return;
         }
-        size_t left = 0;
-        int32_t rc = bitz_piop_engine_launch(e, nullptr);
-        if (!rc) rc = bitz_piop_engine_finish(e, &left);
+        const size_t depth = (all_devfs() && !g_prof.on) ? 2 : 1;
+        int32_t rc = BITZ_OK;
+        if (e->steps.size() < depth && launchable()) rc = engine_launch_step(e, nullptr);
+        if (!rc && !e->steps.empty() && (e->steps.size() >= depth || !launchable()))
+            rc = engine_finish_step(e, nullptr);
         {
             std::lock_guard<std::mutex> lock(e->mu);
             if (rc) {
                 e->worker_rc = rc;
                 e->worker_err = bitz_last_error();
+                (void)hipStreamSynchronize(e->so.s);
+                e->steps.clear();
                 for (auto& j : e->jobs) {
                     j->status = rc;
                     j->phase = JobPhase::kDone;
@@ -4194,7 +4767,7 @@ static void piop_engine_worker(bitz_piop_engine* e) {
 extern "C" int32_t bitz_piop_engine_run(bitz_piop_engine* e) {
     if (!e) return fail(BITZ_ERR_ARG, "null pointer");
     if (e->running) return BITZ_OK;
-    if (!e->launched.empty()) return fail(BITZ_ERR_STATE, "a step is in flight");
+    if (!e->steps.empty()) return fail(BITZ_ERR_STATE, "a step is in flight");
     e->running = true;
     e->worker = std::thread(piop_engine_worker, e);
     return BITZ_OK;
@@ -4211,8 +4784,7 @@ extern "C" int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_u
 extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds,
                                          bitz_f128* outer_claims, bitz_f128* inner_rounds, bitz_f128* inner_point,
                                          bitz_f128* matrix_eval, bitz_f128* final_claim) {
-    if (!e || !outer_rounds || !outer_claims || !inner_rounds || !inner_point || !matrix_eval || !final_claim)
-        return fail(BITZ_ERR_ARG, "null pointer");
+    if (!e) return fail(BITZ_ERR_ARG, "null pointer");
     std::unique_lock<std::mutex> lock(e->mu);
     auto& list = e->running ? e->finished : e->jobs;
     for (size_t i = 0; i < list.size(); ++i) {
@@ -4220,13 +4792,13 @@ extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz
         if (j->tag != tag) continue;
         if (j->phase != JobPhase::kDone) return fail(BITZ_ERR_STATE, "proof still in flight");
         const int32_t st = j->status;
-        if (st == BITZ_OK) {
-            memcpy(outer_rounds, j->outer_rounds.data(), j->outer_rounds.size() * sizeof(bitz_f128));
-            memcpy(outer_claims, j->outer_claims, 3 * sizeof(bitz_f128));
-            memcpy(inner_rounds, j->inner_rounds.data(), j->inner_rounds.size() * sizeof(bitz_f128));
-            memcpy(inner_point, j->inner_point.data(), j->inner_point.size() * sizeof(bitz_f128));
-            *matrix_eval = j->matrix_eval;
-            *final_claim = j->final_claim;
+        if (st == BITZ_OK) {  // null outputs are not copied
+            if (outer_rounds) memcpy(outer_rounds, j->outer_rounds.data(), j->outer_rounds.size() * sizeof(bitz_f128));
+            if (outer_claims) memcpy(outer_claims, j->outer_claims, 3 * sizeof(bitz_f128));
+            if (inner_rounds) memcpy(inner_rounds, j->inner_rounds.data(), j->inner_rounds.size() * sizeof(bitz_f128));
+            if (inner_point) memcpy(inner_point, j->inner_point.data(), j->inner_point.size() * sizeof(bitz_f128));
+            if (matrix_eval) *matrix_eval = j->matrix_eval;
+            if (final_claim) *final_claim = j->final_claim;
         }
         list.erase(list.begin() + (std::ptrdiff_t)i);
         return st == BITZ_OK ? BITZ_OK : fail(st, "proof failed");
@@ -4451,8 +5023,7 @@ extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_
 }
 
 // Whether provers with a Keccak transcript draw their challenges on the
-// device (bitz_set_device_fiat_shamir); on by default.
-static std::atomic<bool> g_device_fs{true};
+// device (bitz_set_device_fiat_shamir); on by default (g_device_fs, above).
 
 extern "C" int32_t bitz_set_device_fiat_shamir(int32_t on) {
     g_device_fs = on != 0;
@@ -5026,3 +5597,248 @@ extern "C" void bitz_gkr_free(bitz_gkr* h) {
         delete h;
     }
 }
+
+// ============================================================ serving =====
+// A proving server for SHA-256 compression-chain requests, native end to end:
+// admission in arrival order by slots and HBM budget, preparer threads
+// (witness on the host, then an engine proof from the packed assignment),
+// background PIOP engines, and harvester threads that take finished proofs
+// and free their slot.  No reference counterpart (the reference proves one
+// statement per process, tooling/cli/src/cmd/circuit_e2e.rs:48-77).
+namespace {
+struct ServeCircuit {
+    const bitz_sha256* c = nullptr;
+    bitz_matrices* m = nullptr;
+    uint8_t digest[32] = {};
+    uint64_t footprint = 0;
+    size_t columns = 0, witness_bits = 0;
+};
+}  // namespace
+
+struct bitz_server {
+    uint32_t slots = 8, preparers = 4;
+    uint64_t budget = 0;
+    std::vector<bitz_piop_engine*> engines;
+    std::vector<ServeCircuit> circuits;
+    ~bitz_server() {
+        for (auto* e : engines) bitz_piop_engine_free(e);
+    }
+};
+
+extern "C" int32_t bitz_server_create(bitz_server** out, uint32_t slots, uint32_t engines, uint32_t preparers,
+                                      uint64_t hbm_budget) {
+    if (!out) return fail(BITZ_ERR_ARG, "null pointer");
+    if (!slots || !engines || !preparers) return fail(BITZ_ERR_ARG, "slots, engines and preparers must be positive");
+    auto* s = new bitz_server;
+    s->slots = slots;
+    s->preparers = preparers;
+    s->budget = hbm_budget;
+    for (uint32_t k = 0; k < engines; ++k) {
+        bitz_piop_engine* e = nullptr;
+        int32_t rc = bitz_piop_engine_create(&e);
+        if (!rc) rc = bitz_piop_engine_run(e);
+        if (rc) {
+            if (e) bitz_piop_engine_free(e);
+            delete s;
+            return rc;
+        }
+        s->engines.push_back(e);
+    }
+    *out = s;
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_server_add_circuit(bitz_server* s, const bitz_sha256* c, bitz_matrices* m,
+                                           const uint8_t digest[32], uint64_t footprint, uint32_t* id) {
+    if (!s || !c || !m || !digest || !id) return fail(BITZ_ERR_ARG, "null pointer");
+    ServeCircuit sc;
+    sc.c = c;
+    sc.m = m;
+    memcpy(sc.digest, digest, 32);
+    sc.footprint = footprint;
+    size_t rows = 0, mpos = 0, nnz[3];
+    TRY(bitz_sha256_shape(c, &rows, &sc.columns, &sc.witness_bits, nnz, &mpos));
+    *id = (uint32_t)s->circuits.size();
+    s->circuits.push_back(sc);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_server_serve(bitz_server* s, bitz_serve_request* reqs, size_t count) {
+    if (!s || (count && !reqs)) return fail(BITZ_ERR_ARG, "null pointer");
+    for (size_t i = 0; i < count; ++i)
+        if (reqs[i].circuit >= s->circuits.size() || !reqs[i].message)
+            return fail(BITZ_ERR_ARG, "request " + std::to_string(i) + ": no such circuit or no message");
+    std::vector<size_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<size_t> ready;  // admitted, waiting for a preparer
+    std::vector<uint32_t> load(s->engines.size(), 0);
+    std::vector<std::unique_ptr<bitz_transcript>> ts(count);
+    size_t left = count, in_flight = 0;
+    uint64_t used = 0;
+    int32_t err = BITZ_OK;
+    std::string err_msg;
+    bool stop = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now_s = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    auto fail_all = [&](int32_t rc) {  // under mu
+        if (!err) {
+            err = rc;
+            err_msg = bitz_last_error();
+        }
+        cv.notify_all();
+    };
+    // Harvester threads wait on their engine's condition; quit wakes them.
+    std::atomic<bool> quit{false};
+    auto wake_harvesters = [&] {
+        quit = true;
+        for (auto* e : s->engines) {
+            { std::lock_guard<std::mutex> lock(e->mu); }
+            e->cv.notify_all();
+        }
+    };
+    // Preparers: the witness (host), then the proof joins the least loaded engine.
+    auto prepare = [&] {
+        std::vector<uint64_t> f, h;
+        for (;;) {
+            size_t i;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv.wait(lock, [&] { return stop || !ready.empty() || err; });
+                if (stop || err) return;
+                i = ready.front();
+                ready.pop_front();
+            }
+            bitz_serve_request& r = reqs[i];
+            const ServeCircuit& c = s->circuits[r.circuit];
+            f.resize(std::max<size_t>((c.witness_bits + 63) / 64, 1));
+            h.resize(std::max<size_t>((c.columns + 63) / 64, 1));
+            int32_t sat = 0;
+            int32_t rc = bitz_sha256_witness(c.c, r.message, r.iv, f.data(), h.data(), r.digest, &sat);
+            const std::string dom = "bitz/serve/" + std::to_string(r.rid);
+            ts[i].reset(bitz_transcript_keccak(reinterpret_cast<const uint8_t*>(dom.data()), dom.size()));
+            size_t k = 0;
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                for (size_t e = 1; e < load.size(); ++e)
+                    if (load[e] < load[k]) k = e;
+                load[k] += 1;
+                r.engine = (uint32_t)k;
+            }
+            if (!rc && !ts[i]) rc = fail(BITZ_ERR_ALLOC, "transcript creation failed");
+            if (!rc)
+                rc = bitz_piop_engine_submit_assignment(s->engines[k], i, ts[i].get(), c.m, c.digest, h.data(),
+                                                        c.columns, c.columns);
+            if (rc) {
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    r.status = rc;
+                    fail_all(rc);
+                }
+                wake_harvesters();
+                return;
+            }
+        }
+    };
+    // Harvesters: one per engine, take each finished proof, free its slot.
+    auto harvest = [&](size_t k) {
+        bitz_piop_engine* e = s->engines[k];
+        std::vector<uint64_t> tags(64);
+        for (;;) {
+            bool any = false;
+            {
+                std::unique_lock<std::mutex> lock(e->mu);
+                e->cv.wait_for(lock, std::chrono::milliseconds(20),
+                               [&] { return !e->finished.empty() || e->worker_rc || quit.load(); });
+                any = !e->finished.empty() || e->worker_rc;
+                if (!any && quit.load()) return;
+            }
+            if (!any) continue;
+            size_t n = 0;
+            int32_t rc = bitz_piop_engine_done(e, tags.data(), tags.size(), &n);
+            for (size_t q = 0; !rc && q < std::min(n, tags.size()); ++q) {
+                bitz_serve_request& r = reqs[tags[q]];
+                rc = bitz_piop_engine_take(e, tags[q], r.outer_rounds, r.outer_claims, r.inner_rounds, r.inner_point,
+                                           r.matrix_eval, r.final_claim);
+                r.status = rc;
+                r.done_s = now_s();
+                std::lock_guard<std::mutex> lock(mu);
+                used -= s->circuits[r.circuit].footprint;
+                in_flight -= 1;
+                load[k] -= 1;
+                left -= 1;
+                cv.notify_all();
+            }
+            if (rc) {
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    fail_all(rc);
+                }
+                wake_harvesters();
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t p = 0; p < s->preparers; ++p) threads.emplace_back(prepare);
+    for (size_t k = 0; k < s->engines.size(); ++k) threads.emplace_back(harvest, k);
+    {
+        // Admission in arrival order: a slot free and the footprint within
+        // the budget (a request larger than the budget runs alone).
+        std::unique_lock<std::mutex> lock(mu);
+        size_t next = 0;
+        while (left && !err) {
+            const double t = now_s();
+            while (next < count && reqs[order[next]].arrival_s <= t && in_flight < s->slots) {
+                bitz_serve_request& r = reqs[order[next]];
+                const uint64_t fp = s->circuits[r.circuit].footprint;
+                if (in_flight && used + fp > s->budget) break;
+                r.admitted_s = t;
+                used += fp;
+                in_flight += 1;
+                ready.push_back(order[next]);
+                ++next;
+                cv.notify_all();
+            }
+            if (next < count && in_flight < s->slots && (!in_flight || used + s->circuits[reqs[order[next]].circuit].footprint <= s->budget)) {
+                const double wait = reqs[order[next]].arrival_s - now_s();
+                if (wait > 0) cv.wait_for(lock, std::chrono::duration<double>(wait));
+            } else {
+                cv.wait(lock);
+            }
+        }
+        stop = true;
+        cv.notify_all();
+    }
+    wake_harvesters();
+    for (auto& th : threads) th.join();
+    if (err) {
+        // Leave the engines clean: wait for what is in flight and drop it.
+        for (auto* e : s->engines) {
+            size_t waiting = 0, n = 0;
+            std::vector<uint64_t> tags(count ? count : 1);
+            for (int tries = 0; tries < 1000; ++tries) {
+                (void)bitz_piop_engine_wait(e, 1000, &waiting);
+                if (bitz_piop_engine_done(e, tags.data(), tags.size(), &n) == BITZ_OK)
+                    for (size_t q = 0; q < std::min(n, tags.size()); ++q) {
+                        bitz_serve_request& r = reqs[tags[q]];
+                        (void)bitz_piop_engine_take(e, tags[q], r.outer_rounds, r.outer_claims, r.inner_rounds,
+                                                    r.inner_point, r.matrix_eval, r.final_claim);
+                    }
+                size_t in = 0;
+                {
+                    std::lock_guard<std::mutex> lock(e->mu);
+                    in = e->jobs.size() + e->incoming.size() + e->finished.size();
+                }
+                if (!in) break;
+            }
+        }
+        return fail(err, err_msg);
+    }
+    return BITZ_OK;
+}
+
+extern "C" void bitz_server_free(bitz_server* s) { delete s; }

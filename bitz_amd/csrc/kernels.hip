@@ -1205,7 +1205,13 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_lane(const f128* d, co
 // known to be zero is folded over its live prefix only.
 __device__ __forceinline__ void inner_fold_body(const f128* d, const f128* w, f128* dout, f128* wout,
                                                 size_t out_pairs, size_t live, const uint4* rtab4, uint32_t* acc,
-                                                size_t block, size_t blocks) {
+                                                size_t block, size_t blocks,
+                                                const unsigned long long* live0 = nullptr, int live_shift = 0) {
+    if (live0) {  // the live length measured on the device by round 0
+        const unsigned long long l0 = *live0;
+        live = min(live, (size_t)((l0 + (1ull << live_shift) - 1) >> live_shift));
+        out_pairs = min(out_pairs, ((live + 1) / 2 + 1) / 2);
+    }
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
     uint4* rt = smem_raw;
     uint4* col = smem_raw + kConstTable4Entries + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
@@ -1259,7 +1265,8 @@ __global__ __launch_bounds__(kInnerBlock) void k_inner_r0_batch(const InnerRound
 }
 __global__ __launch_bounds__(kInnerBlock) BITZ_FOUR_WAVES void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap) {
     const InnerRoundDesc A = d[bmap[blockIdx.x]];
-    inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.live, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks);
+    inner_fold_body(A.d, A.w, A.dout, A.wout, A.pairs, A.live, A.rtab4, A.acc, blockIdx.x - A.block_base, A.blocks,
+                    A.live0, A.live_shift);
 }
 
 // Rounds k0 .. n-1 in one workgroup: the two tables (folded by r_{k0-1} on
@@ -1379,20 +1386,27 @@ __device__ __forceinline__ void inner_bits_body(const InnerBitsDesc& A, size_t b
 #pragma unroll
     for (int s = 0; s < S; ++s) s0[s] = s2[s] = g_zero();
     unsigned long long live = 0;
+    size_t in_live = A.live, pairs = A.pairs;
+    if (K > 0 && A.live0) {  // the live length measured on the device by round 0
+        const unsigned long long l0 = *A.live0, sh = K - 1;
+        in_live = min(in_live, (size_t)((l0 + (1ull << sh) - 1) >> sh));
+        pairs = min(pairs, ((in_live + 1) / 2 + 1) / 2);
+    }
+    const f128* mono = A.mono_dev ? A.mono_dev : A.mono;
 #pragma unroll 1
-    for (size_t p = block * kInnerBlock + threadIdx.x; p < A.pairs; p += blocks * kInnerBlock) {
+    for (size_t p = block * kInnerBlock + threadIdx.x; p < pairs; p += blocks * kInnerBlock) {
         g4 m0, m1;
         if constexpr (K == 0) {
             m0 = g_load_hbm(A.d + 2 * p);
             m1 = g_load_hbm(A.d + 2 * p + 1);
         } else {
             g4 in[4];
-            if (4 * p + 3 < A.live) {
+            if (4 * p + 3 < in_live) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) in[j] = g_load_hbm(A.d + 4 * p + j);
             } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) in[j] = 4 * p + j < A.live ? g_load_hbm(A.d + 4 * p + j) : g_zero();
+                for (int j = 0; j < 4; ++j) in[j] = 4 * p + j < in_live ? g_load_hbm(A.d + 4 * p + j) : g_zero();
             }
             const g4 d01 = g_add(in[0], in[1]), d23 = g_add(in[2], in[3]);
             g4 t01 = g_zero(), t23 = g_zero();
@@ -1427,7 +1441,7 @@ __device__ __forceinline__ void inner_bits_body(const InnerBitsDesc& A, size_t b
             g4 w0 = g_zero(), w1 = g_zero();
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const g4 r = g_from(A.mono[s]);
+                const g4 r = g_from(mono[s]);
                 const uint32_t k0 = 0u - ((v >> s) & 1u), k1 = 0u - ((v >> (S + s)) & 1u);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -1960,6 +1974,176 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, G
         }
     }
     fs_save(&fs.st->ts, f);
+}
+
+}  // namespace bitz
+
+namespace bitz {
+
+// ============================== prove_spartan_piop: transcript on the device ===
+// One wave per proof: products by the whole wave (g_mul_wave) where they
+// chain, one per lane (g_mul) where many are independent; the duplex as in
+// fs.hpp; r's 4-bit table built in LDS and written for the next round's fold.
+__device__ __forceinline__ g4 fs_lane_xor(g4 v, int lanes) {
+    // XOR of v over lanes [0, lanes) (lanes a power of two <= 64; others hold zero)
+    for (int off = lanes >> 1; off >= 1; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v.w[i] ^= (uint32_t)__shfl_xor((int)v.w[i], off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
+    const PiopFsDesc D = descs[blockIdx.x];
+    PiopFsState* st = D.st;
+    const int lane = (int)threadIdx.x;
+    __shared__ FsLds f;
+    __shared__ uint4 tab[kConstTable4Entries];
+    __shared__ uint4 basis[128];
+    __shared__ uint32_t words[8 << kInnerBitsMaxRound];
+    auto sum_at = [&](int word) {
+        g4 v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v.w[i] = D.sums[word + i];
+        return v;
+    };
+    g4 claim = g_load(&st->claim);
+    if (D.kind == kFsInnerFinal) {  // D(r) and the final claim; the transcript is untouched
+        if (lane == 0) {
+            g_store(D.round_out, sum_at(0));
+            g_store(D.final_out, claim);
+        }
+        return;
+    }
+    fs_load(f, &st->ts);
+    if (D.kind == kFsOuterFinal) {
+        // sumcheck.rs:372 then piop.rs:94-97: absorb [A, B, C](r), squeeze rho,
+        // the inner claim A + rho B + rho^2 C.
+        g4 cl[3] = {sum_at(0), sum_at(4), sum_at(8)};
+        fs_message(f, 0, cl, 3);
+        const g4 rho = fs_squeeze(f);
+        claim = g_add(cl[0], g_mul_wave(rho, g_add(cl[1], g_mul_wave(rho, cl[2]))));
+        const g4 rho2 = g_mul_wave(rho, rho);
+        if (lane == 0) {
+            for (int u = 0; u < 3; ++u) g_store(D.round_out + u, cl[u]);
+            g_store(&st->rho, rho);
+            g_store(&st->rho2, rho2);
+            g_store(&st->claim, claim);
+        }
+        fs_save(&st->ts, f);
+        return;
+    }
+    g4 c[4];
+    int deg;
+    g4 q[3];
+    if (D.kind == kFsOuterRound) {
+        // The eq-factored round: q(0), q(inf) (and q(1)) scaled, c0 = S (1 + tau) q0,
+        // c2 = S ((1 + tau) q2 + q1), c3 = S q2 with q1 = q(1) + q0 + q2.
+        const g4 scale = g_from(D.scale), tau = g_from(D.tau);
+        g4 one_tau = tau;
+        one_tau.w[0] ^= 1u;
+        q[0] = g_mul_wave(sum_at(0), scale);
+        q[2] = g_mul_wave(sum_at(4), scale);
+        const g4 at1 = D.have_q1 ? g_mul_wave(sum_at(8), scale)
+                                 : g_mul_wave(g_add(g_load(&st->sigma), g_mul_wave(one_tau, q[0])), g_from(D.tau_inv));
+        q[1] = g_add3(at1, q[0], q[2]);
+        const g4 S = g_load(&st->S);
+        c[0] = g_mul_wave(S, g_mul_wave(one_tau, q[0]));
+        c[2] = g_mul_wave(S, g_add(g_mul_wave(one_tau, q[2]), q[1]));
+        c[3] = g_mul_wave(S, q[2]);
+        c[1] = g_add3(claim, c[2], c[3]);
+        deg = 3;
+    } else {
+        if (D.K >= 0) {
+            // A bit round: the copies' XOR, then sum_S r_S (sum S of c0 / c2).
+            const int S = 1 << D.K;
+            if (lane < 8 * S) {
+                uint32_t x = 0;
+                for (int r = 0; r < kInnerBitsReplicas; ++r) x ^= D.sums[r * kInnerBitsReplicaWords + lane];
+                words[lane] = x;
+            }
+            if (D.K == 0 && lane == 0) {
+                unsigned long long lv = 0;
+                for (int r = 0; r < kInnerBitsReplicas; ++r)
+                    lv = max(lv, *reinterpret_cast<const unsigned long long*>(D.sums + r * kInnerBitsReplicaWords +
+                                                                              kInnerBitsLiveWord));
+                st->live = lv;
+            }
+            __syncthreads();
+            g4 prod = g_zero();
+            if (lane < 2 * S) {
+                g4 v;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v.w[i] = words[4 * lane + i];
+                prod = g_mul(g_load(&st->mono[lane & (S - 1)]), v);
+            }
+            // c0 over lanes [0, S), c2 over [S, 2S): XOR within each half.
+            const g4 lo = fs_lane_xor(lane < S ? prod : g_zero(), 2 * S);
+            const g4 hi = fs_lane_xor(lane >= S && lane < 2 * S ? prod : g_zero(), 2 * S);
+            c[0] = lo;
+            c[2] = hi;
+            // every lane of the first 2S has the sums; broadcast lane 0's
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                c[0].w[i] = (uint32_t)__shfl((int)c[0].w[i], 0, 64);
+                c[2].w[i] = (uint32_t)__shfl((int)c[2].w[i], 0, 64);
+            }
+        } else {
+            c[0] = sum_at(0);
+            c[2] = sum_at(4);
+        }
+        c[1] = g_add(claim, c[2]);
+        deg = 2;
+    }
+    // sumcheck.rs:803-829: absorb the full round polynomial, squeeze r, claim = g(r).
+    fs_message(f, 0, c, deg + 1);
+    const g4 r = fs_squeeze(f);
+    g4 g = c[deg];
+    for (int i = deg - 1; i >= 0; --i) g = g_add(c[i], g_mul_wave(g, r));
+    claim = g;
+    if (D.kind == kFsOuterRound) {
+        // sigma_{k+1} = q_k(r), S_{k+1} = S_k eq(tau_k, r) = S_k (1 + tau_k + r)
+        const g4 sigma = g_add(q[0], g_mul_wave(r, g_add(q[1], g_mul_wave(r, q[2]))));
+        g4 e = g_add(g_from(D.tau), r);
+        e.w[0] ^= 1u;
+        const g4 S = g_mul_wave(g_load(&st->S), e);
+        if (lane == 0) {
+            g_store(&st->sigma, sigma);
+            g_store(&st->S, S);
+        }
+    } else if (D.mono_next) {
+        // r_S of the challenges so far, for the next bit round: S with bit k set
+        const int half = 1 << D.k;
+        if (lane < half) g_store(&st->mono[half + lane], g_mul(g_load(&st->mono[lane]), r));
+    }
+    if (lane == 0) {
+        for (int i = 0; i <= deg; ++i) g_store(D.round_out + i, c[i]);
+        g_store(D.point_out, r);
+        g_store(&st->claim, claim);
+    }
+    fs_save(&st->ts, f);
+    build_const_table4(tab, basis, r);
+    for (int e = lane; e < kConstTable4Entries; e += 64) D.rtab[e] = tab[e];
+}
+
+__global__ __launch_bounds__(64) void k_final_folds(const FinalFoldDesc* descs) {
+    const FinalFoldDesc D = descs[blockIdx.x];
+    if ((int)threadIdx.x < D.count) {
+        const g4 r = g_load(D.r);
+        const f128* t = D.tables[threadIdx.x];
+        const g4 a = g_load(t), b = g_load(t + 1);
+        g_store(D.out + threadIdx.x, g_add(a, g_mul(r, g_add(a, b))));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_scales_dev(const f128* eq_rows, const f128* rr, f128* row_scale,
+                                                           size_t rows_padded) {
+    const g4 rho = g_load(rr), rho2 = g_load(rr + 1);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < rows_padded; i += (size_t)gridDim.x * blockDim.x) {
+        const g4 w = g_load(eq_rows + i);  // k_row_scales' layout: [w | rho w | rho^2 w]
+        g_store(row_scale + i, w);
+        g_store(row_scale + rows_padded + i, g_mul(w, rho));
+        g_store(row_scale + 2 * rows_padded + i, g_mul(w, rho2));
+    }
 }
 
 }  // namespace bitz

@@ -229,6 +229,10 @@ struct InnerRoundDesc {
     const uint4* rtab4;
     uint32_t* acc;
     uint32_t blocks, block_base;
+    // Fold rounds, or null: `live` and `pairs` are bounds, and the live
+    // length of the input tables is ceil(*live0 / 2^live_shift).
+    const unsigned long long* live0;
+    int live_shift;
 };
 __global__ void k_inner_r0_batch(const InnerRoundDesc* d, const uint32_t* bmap);
 __global__ void k_inner_fold_batch(const InnerRoundDesc* d, const uint32_t* bmap);
@@ -289,6 +293,10 @@ struct InnerBitsDesc {
     uint32_t* acc;          // kInnerBitsAccWords words, zero on entry (the copies above)
     uint32_t blocks, block_base;
     f128 mono[1 << kInnerBitsMaxRound];  // wout: r_S for S < 2^K
+    const f128* mono_dev;   // or null: wout's r_S read from the device instead
+    // Or null: `live` and `pairs` are bounds, and the live length of D_{K-1}
+    // is ceil(*live0 / 2^(K-1)) (live0: round 0's measure on the device).
+    const unsigned long long* live0;
 };
 // Workgroups of a bit round: enough to stream at HBM rate, few enough that
 // the sums' atomics stay cheap.
@@ -296,6 +304,55 @@ constexpr unsigned kInnerBitsMaxBlocks = 1024;
 constexpr size_t kInnerBitsSmem = sizeof(uint4) * kConstTable4Entries;
 template <int K> __global__ void k_inner_bits(InnerBitsDesc A);
 template <int K> __global__ void k_inner_bits_batch(const InnerBitsDesc* d, const uint32_t* bmap);
+// ---- prove_spartan_piop rounds with the transcript on the device ---------
+// A proof's running values: the Keccak duplex, the claim, the eq-factored
+// outer sumcheck's S_k = prod_{i<k} eq(tau_i, r_i) and sigma_k = q_{k-1}(r),
+// the current q(0), q(1) + q(0) + q(inf), q(inf), rho and rho^2, the inner
+// sumcheck's monomials r_S of its challenges so far (bit rounds) and its live
+// length (round 0's measure).
+struct PiopFsState {
+    KeccakState ts;
+    f128 claim, S, sigma, q[3], rho, rho2;
+    f128 mono[1 << kInnerBitsMaxRound];
+    unsigned long long live;
+    uint64_t pad_;
+};
+constexpr int kFsOuterRound = 0, kFsOuterFinal = 1, kFsInnerRound = 2, kFsInnerFinal = 3;
+// One proof's step after its round's kernels (one workgroup of kStageBlock
+// threads each):
+//   outer round k: c0, c2, c3 from the eq-factored sums (q(0), q(inf), q(1)
+//     at sums + 0, 4, 8, times `scale`; without q(1) it comes from sigma),
+//     c1 = claim + c2 + c3, absorb [c0..c3] (public), squeeze r, claim =
+//     g(r), sigma, S; [c0..c3] at round_out, r at point_out, r's table;
+//   outer final: A(r), B(r), C(r) at sums (12 words) to round_out, absorbed
+//     (public), rho squeezed, claim = A + rho B + rho^2 C;
+//   inner round k: [c0, c2] at sums (a bit round K >= 0: the copies of its
+//     monomial sums, weighed by st->mono), c1 = claim + c2, absorb, squeeze,
+//     claim, outputs and r's table; mono_next: st->mono grows by r;
+//   inner final: D(r) (sums) to round_out, the claim to final_out.
+struct PiopFsDesc {
+    PiopFsState* st;
+    const uint32_t* sums;
+    f128* round_out;
+    f128* point_out;
+    uint4* rtab;
+    f128* final_out;
+    f128 tau, tau_inv, scale;  // outer round k: tau_k, 1 / tau_k, the sums' scale
+    int kind, k, K, have_q1, mono_next;
+};
+__global__ void k_piop_fs(const PiopFsDesc* d);
+// The last fold of a PIOP proof's tables by the challenge at *r (device):
+// tables[t][0] + r (tables[t][0] + tables[t][1]) to out[t], t < count.
+struct FinalFoldDesc {
+    const f128* tables[3];
+    const f128* r;
+    f128* out;
+    int count;
+};
+__global__ void k_final_folds(const FinalFoldDesc* d);
+// k_row_scales with rho and rho^2 read from the device (rr[0], rr[1]).
+__global__ void k_row_scales_dev(const f128* eq_rows, const f128* rr, f128* row_scale, size_t rows_padded);
+
 // Az, Bz, Cz rows over a packed Boolean assignment: the XOR of the row's
 // values at the columns whose bit is set (no products).
 __global__ void k_r1cs_rows_bits(const uint32_t* row_starts, const uint32_t* cols, const f128* vals,

@@ -17,11 +17,13 @@ sha256_spartan.rs:45-90).
 Scheduling: requests wait in arrival order; one is admitted when a slot is
 free and its HBM footprint fits the budget beside the requests in flight.
 Two ways to run the admitted ones:
-  - serve_batched (continuous batching): a few threads prepare each request
-    (witness, assignment, products), then it joins a PiopEngine that advances
-    every proof in flight by one round per step, the rounds of one kind
-    across all proofs in one launch; a request joins at the next step and
-    leaves when done.
+  - serve_batched (continuous batching, native: bitz_server_serve): preparer
+    threads compute each admitted request's witness, then it joins a PIOP
+    engine (which uploads the packed assignment and computes Az/Bz/Cz on a
+    stream of the proof's own) that advances every proof in flight by several
+    rounds per step (its Keccak transcript's challenges drawn on the device),
+    the rounds of one kind across all proofs in one launch; a request joins
+    at the next step and leaves when done.
   - serve: every admitted request runs on its own host thread with its own
     HIP streams, so the requests' kernels overlap on the device.
 Each result depends only on its request (its circuit, its message and
@@ -29,6 +31,7 @@ chaining value, its transcript), not on the mix or the mode.
 """
 from __future__ import annotations
 
+import ctypes as C
 import threading
 import time
 from collections import deque
@@ -37,9 +40,47 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (PackedAssignment, PiopEngine, PreparedConstraintMatrices, SpartanPiopOutput, Transcript, init,
-               prove_spartan_piop)
+from . import (PackedAssignment, PreparedConstraintMatrices, SpartanPiopOutput, SumcheckError, Transcript, _check,
+               init, lib, prove_spartan_piop, to_int)
 from . import sha256 as sha
+
+
+class _ServeRequest(C.Structure):
+    """bitz_serve_request (include/bitz_hip.h)."""
+    _fields_ = [("rid", C.c_uint64), ("circuit", C.c_uint32), ("engine", C.c_uint32), ("message", C.c_void_p),
+                ("iv", C.c_uint32 * 8), ("arrival_s", C.c_double), ("admitted_s", C.c_double),
+                ("done_s", C.c_double), ("digest", C.c_uint8 * 32), ("status", C.c_int32),
+                ("outer_rounds", C.c_void_p), ("outer_claims", C.c_void_p), ("inner_rounds", C.c_void_p),
+                ("inner_point", C.c_void_p), ("matrix_eval", C.c_void_p), ("final_claim", C.c_void_p)]
+
+
+class NativeServer:
+    """The library's proving server (bitz_server_*): admission, preparer
+    threads, PIOP engines and harvesters all native, so serving holds no
+    Python lock while it runs."""
+
+    def __init__(self, slots: int, engines: int, preparers: int, hbm_budget: int):
+        init()
+        h = C.c_void_p()
+        _check(lib().bitz_server_create(C.byref(h), slots, engines, preparers, hbm_budget))
+        self.h = h
+        self.ids: dict[int, int] = {}
+        self.config = (slots, engines, preparers, hbm_budget)
+
+    def add(self, c: "Circuit") -> int:
+        if c.blocks not in self.ids:
+            cid = C.c_uint32()
+            dg = (C.c_uint8 * 32).from_buffer_copy(c.digest)
+            _check(lib().bitz_server_add_circuit(self.h, c.circuit.h, c.matrices.h, dg, c.footprint(),
+                                                 C.byref(cid)))
+            self.ids[c.blocks] = cid.value
+        return self.ids[c.blocks]
+
+    def __del__(self):
+        try:
+            lib().bitz_server_free(self.h)
+        except Exception:
+            pass
 
 
 @dataclass
@@ -116,14 +157,14 @@ class ProvingServer:
         self._lock = threading.Lock()
         self.peak_bytes = 0      # largest footprint in flight during the last serve()
         self.peak_in_flight = 0  # most requests in flight during the last serve()
-        self._engines: list[PiopEngine] = []  # kept across serve_batched calls
+        self._native: NativeServer | None = None  # serve_batched's server, kept across calls
 
-    def engines(self, count: int) -> list[PiopEngine]:
-        """count background PIOP engines, created on first use and kept (each
-        holds a stream, a thread and its staging buffers)."""
-        while len(self._engines) < count:
-            self._engines.append(PiopEngine(background=True))
-        return self._engines[:count]
+    @staticmethod
+    def transcript(rid: int) -> Transcript:
+        """A request's transcript: the Keccak duplex, so the engine draws its
+        challenges on the device (several rounds per step, no host round trip
+        per round)."""
+        return Transcript.keccak(b"bitz/serve/%d" % rid)
 
     def circuit(self, blocks: int) -> Circuit:
         with self._lock:
@@ -138,8 +179,7 @@ class ProvingServer:
         w = c.circuit.witness(message, iv=iv)
         h = PackedAssignment(w.h, c.columns, c.columns, device=True)
         products = c.matrices.r1cs_products(h)
-        t = Transcript.duplex(b"bitz/serve/%d" % rid)
-        return prove_spartan_piop(t, c.matrices, c.digest, products, h), w.digest
+        return prove_spartan_piop(self.transcript(rid), c.matrices, c.digest, products, h), w.digest
 
     def prepare(self, r: "Request"):
         """A request's device inputs: witness (host), the assignment's packed
@@ -150,70 +190,60 @@ class ProvingServer:
         return c, h, c.matrices.r1cs_products(h), w.digest
 
     def serve_batched(self, requests: list[Request], preparers: int = 4, engines: int | None = None) -> list[Request]:
-        """Continuous batching: admitted requests are prepared (witness,
-        assignment, products) by a few threads, then join a PiopEngine (the
-        least loaded of `engines`, each with its own stream and thread) that
-        advances every proof it holds by one round per step, each round kind
-        in one launch across the proofs.  A request joins at its engine's next
-        step and leaves when its proof is done; admission is by slots and HBM
-        budget as in serve().  engines defaults to one per 8 slots."""
+        """Continuous batching on the library's native server
+        (bitz_server_serve): requests are admitted in arrival order by slots
+        and HBM budget; preparer threads compute each one's witness, then it
+        joins the least loaded of `engines` PIOP engines, which upload its
+        packed assignment, compute Az/Bz/Cz and advance every proof they hold
+        by up to 8 rounds per step (challenges drawn on the device from the
+        Keccak transcript), each round kind in one launch across the proofs;
+        harvester threads take finished proofs and free their slots.  engines
+        defaults to one per 8 slots."""
         engines = engines or max(1, self.slots // 8)
-        pending = deque(sorted(requests, key=lambda r: r.arrival))
-        for r in pending:
-            self.circuit(r.blocks)
-        pool_engines = self.engines(engines)
-        load = [0] * engines
-        by_tag: dict[int, tuple] = {}
-        used = 0
+        config = (self.slots, engines, preparers, self.hbm_budget)
+        if self._native is None or self._native.config != config:
+            self._native = None
+            self._native = NativeServer(*config)
+        ns = self._native
+        reqs = (_ServeRequest * max(len(requests), 1))()
+        keep = []
+        for i, r in enumerate(requests):
+            c = self.circuit(r.blocks)
+            q = reqs[i]
+            q.rid, q.circuit = r.rid, ns.add(c)
+            msg = C.create_string_buffer(bytes(r.message), max(len(r.message), 1))
+            q.message = C.cast(msg, C.c_void_p)
+            for k in range(8):
+                q.iv[k] = int(r.iv[k]) & 0xffffffff
+            q.arrival_s = float(r.arrival)
+            rv, cv = c.row_vars, c.col_vars
+            out = np.zeros((4 * rv + 3 + 3 * cv + cv + 2, 2), np.uint64)
+            base = out.ctypes.data
+            q.outer_rounds, q.outer_claims = base, base + 64 * rv
+            q.inner_rounds = base + 16 * (4 * rv + 3)
+            q.inner_point = base + 16 * (4 * rv + 3 + 3 * cv)
+            q.matrix_eval = base + 16 * (4 * rv + 3 + 4 * cv)
+            q.final_claim = base + 16 * (4 * rv + 4 + 4 * cv)
+            keep.append((msg, out, rv, cv))
         self.peak_bytes = self.peak_in_flight = 0
-        t0 = time.perf_counter()
-        lock = threading.Lock()
-
-        def admit_one(r: Request, fp: int) -> None:
-            c, h, products, digest = self.prepare(r)
-            r.digest = digest
-            with lock:
-                k = min(range(engines), key=lambda i: load[i])
-                load[k] += 1
-                by_tag[r.rid] = (r, fp, h, products, k)
-            pool_engines[k].submit(r.rid, Transcript.duplex(b"bitz/serve/%d" % r.rid), c.matrices, c.digest,
-                                   products, h)
-
-        in_flight = 0
-        with ThreadPoolExecutor(max_workers=preparers) as pool:
-            futures = []
-            while pending or in_flight:
-                now = time.perf_counter() - t0
-                while pending and pending[0].arrival <= now and in_flight < self.slots:
-                    fp = self.circuits[pending[0].blocks].footprint()
-                    if in_flight and used + fp > self.hbm_budget:
-                        break
-                    r = pending.popleft()
-                    r.admitted = now
-                    used += fp
-                    in_flight += 1
-                    futures.append(pool.submit(admit_one, r, fp))
-                    self.peak_bytes = max(self.peak_bytes, used)
-                    self.peak_in_flight = max(self.peak_in_flight, in_flight)
-                for f in [f for f in futures if f.done()]:
-                    f.result()  # a failed preparation raises here
-                    futures.remove(f)
-                timeout = 0.0005
-                if pending and in_flight < self.slots:
-                    timeout = min(timeout, max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
-                if in_flight and (pool_engines[0].wait(timeout / engines) if engines == 1 else
-                                  sum(e.wait(timeout / engines) for e in pool_engines)):
-                    for k, e in enumerate(pool_engines):
-                        for tag in e.done():
-                            with lock:
-                                r, fp, _, _, _ = by_tag.pop(tag)
-                                load[k] -= 1
-                            r.result = e.take(tag)
-                            r.done = time.perf_counter() - t0
-                            used -= fp
-                            in_flight -= 1
-                elif not in_flight and pending:
-                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+        _check(lib().bitz_server_serve(ns.h, reqs, len(requests)), SumcheckError)
+        for i, r in enumerate(requests):
+            q, (_, out, rv, cv) = reqs[i], keep[i]
+            r.admitted, r.done = q.admitted_s, q.done_s
+            r.digest = bytes(q.digest)
+            o = 4 * rv
+            r.result = SpartanPiopOutput(out[:o].reshape(rv, 4, 2), [to_int(e) for e in out[o:o + 3]],
+                                         out[o + 3:o + 3 + 3 * cv].reshape(cv, 3, 2),
+                                         out[o + 3 + 3 * cv:o + 3 + 4 * cv], to_int(out[o + 3 + 4 * cv]),
+                                         to_int(out[o + 4 + 4 * cv]))
+        # the peaks of the admission that just ran (replayed from the times)
+        ev = sorted([(r.admitted, 1, self.circuits[r.blocks].footprint()) for r in requests] +
+                    [(r.done, -1, -self.circuits[r.blocks].footprint()) for r in requests], key=lambda x: (x[0], x[1]))
+        n = b = 0
+        for _, dn, db in ev:
+            n += dn
+            b += db
+            self.peak_in_flight, self.peak_bytes = max(self.peak_in_flight, n), max(self.peak_bytes, b)
         return requests
 
     def serve(self, requests: list[Request]) -> list[Request]:

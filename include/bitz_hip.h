@@ -433,7 +433,11 @@ int32_t bitz_post_gkr_prove(bitz_transcript* t, const bitz_f128* rows, uint32_t 
  * one round per step: the rounds of one kind across all proofs run as one
  * launch, every proof's sums come back in one read, and each proof's own
  * transcript absorbs its round polynomial and squeezes its next challenge on
- * the host, in bitz_prove_spartan_piop's order.  A proof's outputs equal
+ * the host, in bitz_prove_spartan_piop's order.  A proof with a packed
+ * assignment and a Keccak transcript (bitz_transcript_keccak) while device
+ * Fiat-Shamir is on (bitz_set_device_fiat_shamir) draws its challenges on the
+ * device instead and advances by up to 8 rounds per step; its transcript's
+ * state is handed back when it finishes.  A proof's outputs equal
  * bitz_prove_spartan_piop's for the same transcript and inputs; proofs
  * submitted between steps join at the next step (continuous batching of a
  * proving server).  Products, assignment (device) and the transcript must
@@ -453,6 +457,16 @@ int32_t bitz_piop_engine_submit_packed(bitz_piop_engine* e, uint64_t tag, bitz_t
                                        const uint8_t digest[32], const bitz_f128* az, const bitz_f128* bz,
                                        const bitz_f128* cz, uint32_t row_vars, const uint64_t* assignment_bits,
                                        uint32_t col_vars);
+/* An engine proof from the packed assignment (PackedWitness words on the
+ * host, bit_len bits; build_assignment_mle's checks, matrix.rs:392-420, and
+ * 2^ceil(log2 columns) = the matrices' column domain): the engine uploads the
+ * bits and computes Az, Bz, Cz itself (their XORs of values under the set
+ * bits, on a stream of the proof's own), then proves as
+ * bitz_piop_engine_submit_packed.  The caller keeps nothing alive but the
+ * transcript and the matrices. */
+int32_t bitz_piop_engine_submit_assignment(bitz_piop_engine* e, uint64_t tag, bitz_transcript* t, bitz_matrices* m,
+                                           const uint8_t digest[32], const uint64_t* words, size_t bit_len,
+                                           size_t columns);
 /* One round of every proof in flight; *in_flight receives how many remain.
  * step = launch (enqueue the step's work on the engine's stream, no wait;
  * *launched = proofs in it) then finish (wait, then the host's part).  With
@@ -469,7 +483,8 @@ int32_t bitz_piop_engine_run(bitz_piop_engine* e);
 int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_us, size_t* finished);
 /* Tags of the finished proofs not yet taken (up to cap; *count = all). */
 int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count);
-/* A finished proof's outputs (as bitz_prove_spartan_piop's); releases it. */
+/* A finished proof's outputs (as bitz_prove_spartan_piop's; a null output is
+ * not copied); releases it. */
 int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds, bitz_f128* outer_claims,
                               bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
                               bitz_f128* final_claim);
@@ -587,6 +602,46 @@ int32_t bitz_gkr_prove_with_challenges(bitz_gkr* h, const bitz_f128* point, uint
                                        bitz_f128* layer_evals, bitz_f128* leaf_point, bitz_f128* leaf_claim);
 int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_t cap, size_t* len);
 void bitz_gkr_free(bitz_gkr* h);
+
+/* ---- serving SHA-256 compression chains (no reference counterpart) -------- */
+/* A proving server, native end to end: requests are admitted in arrival
+ * order while a slot is free and their circuit's footprint fits the HBM
+ * budget beside the requests in flight (a request larger than the budget
+ * runs alone); preparer threads compute each admitted request's witness
+ * (bitz_sha256_witness) and hand it to the least loaded of the server's PIOP
+ * engines (bitz_piop_engine_submit_assignment, Keccak transcript with domain
+ * "bitz/serve/<rid>"); harvester threads take finished proofs and free their
+ * slot.  A request's result depends only on the request. */
+typedef struct bitz_server bitz_server;
+typedef struct {
+    uint64_t rid;             /* names the transcript domain */
+    uint32_t circuit;         /* bitz_server_add_circuit's id */
+    uint32_t engine;          /* out: the engine that proved it */
+    const uint8_t* message;   /* 64 bytes per block, read by a preparer */
+    uint32_t iv[8];           /* the chaining value */
+    double arrival_s;         /* seconds after the call starts */
+    double admitted_s, done_s; /* out */
+    uint8_t digest[32];       /* out: the chain's output state */
+    int32_t status;           /* out */
+    /* outputs as bitz_prove_spartan_piop's (row_vars x 4, 3, col_vars x 3,
+     * col_vars, 1, 1 elements), each may be null */
+    bitz_f128* outer_rounds;
+    bitz_f128* outer_claims;
+    bitz_f128* inner_rounds;
+    bitz_f128* inner_point;
+    bitz_f128* matrix_eval;
+    bitz_f128* final_claim;
+} bitz_serve_request;
+int32_t bitz_server_create(bitz_server** s, uint32_t slots, uint32_t engines, uint32_t preparers,
+                           uint64_t hbm_budget);
+/* A compression-chain circuit with its prepared matrices (bitz_sha256_prepare)
+ * and digest; footprint = device bytes a request holds at its peak.  The
+ * circuit and matrices must outlive the server's use of them. */
+int32_t bitz_server_add_circuit(bitz_server* s, const bitz_sha256* circuit, bitz_matrices* m,
+                                const uint8_t digest[32], uint64_t footprint, uint32_t* id);
+/* Serves every request and returns when all are done (or one failed). */
+int32_t bitz_server_serve(bitz_server* s, bitz_serve_request* requests, size_t count);
+void bitz_server_free(bitz_server* s);
 
 #ifdef __cplusplus
 }

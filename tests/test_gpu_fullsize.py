@@ -178,7 +178,8 @@ def test_served_256_block_request_matches_the_oracle():
     big = served[1]
     assert big.blocks == 256
     say("serve: oracle composition of the 256-block proof")
-    _, ref = prove_and_check(srv.circuit(256).circuit, big.message, list(big.iv), label=b"bitz/serve/%d" % big.rid)
+    _, ref = prove_and_check(srv.circuit(256).circuit, big.message, list(big.iv), label=b"bitz/serve/%d" % big.rid,
+                             keccak=True)
     assert same(big.result, ref)
     for r in served:
         if r.blocks != 256:

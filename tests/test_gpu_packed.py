@@ -193,3 +193,77 @@ def test_packed_engine_matches_one_proof_at_a_time():
         assert steps < 400
     for tag in reqs:
         assert same(eng.take(tag), refs[tag]), tag
+
+
+def test_engine_device_transcript_matches_the_host_transcript():
+    """Proofs with a Keccak transcript in the engine (challenges drawn on the
+    device, several rounds per step, no host round trip per round) equal the
+    single prover's with the same transcript drawn on the host, and the
+    caller's transcript continues from the same state; proofs with a host
+    transcript share the engine's steps."""
+    sizes = [1, 2, 8, 1, 4, 32, 2, 64, 1]
+    reqs = {tag: request(b, 500 + tag) for tag, b in enumerate(sizes)}
+    host_fs = {tag for tag in reqs if tag % 4 == 3}
+
+    def transcript(tag):
+        return G.Transcript.duplex(b"fs/%d" % tag) if tag in host_fs else G.Transcript.keccak(b"fs/%d" % tag)
+
+    refs, after = {}, {}
+    for tag, (m, dg, h, packed) in reqs.items():
+        t = transcript(tag)
+        refs[tag] = G.prove_spartan_piop(t, m, dg, m.r1cs_products(packed), packed)
+        after[tag] = t.squeeze()
+    for background in (False, True):
+        eng = G.PiopEngine(background=background)
+        keep = {}
+        for tag, (m, dg, h, packed) in reqs.items():
+            keep[tag] = (transcript(tag), m.r1cs_products(packed))
+            eng.submit(tag, keep[tag][0], m, dg, keep[tag][1], packed)
+        if background:
+            while len(eng.done()) < len(reqs):
+                eng.wait(0.01)
+        else:
+            steps = 0
+            while eng.step():
+                steps += 1
+                assert steps < 200
+        for tag in reqs:
+            assert same(eng.take(tag), refs[tag]), (background, tag)
+            assert keep[tag][0].squeeze() == after[tag], (background, tag)
+
+
+def test_engine_from_the_assignment_words():
+    """bitz_piop_engine_submit_assignment (host words in; the engine uploads
+    them and computes Az, Bz, Cz) gives the single prover's proof, with a
+    Keccak transcript (device challenges) and a duplex one (host challenges);
+    a wrong length or constant bit is rejected before the transcript."""
+    sizes = [1, 4, 2, 16]
+    circuits = {b: (lambda c: (c, *c.prepare()))(S.Sha256Circuit(S.COMPRESSION_CHAIN, b)) for b in set(sizes)}
+    eng = G.PiopEngine(background=True)
+    refs, keep = {}, {}
+    for tag, b in enumerate(sizes):
+        c, m, dg = circuits[b]
+        rng = np.random.default_rng(900 + tag)
+        msg = rng.integers(0, 256, 64 * b, dtype=np.uint8).tobytes()
+        iv = [int(x) for x in rng.integers(0, 2**32, 8, dtype=np.uint64)]
+        w = c.witness(msg, iv=iv)
+        mk = (lambda t: G.Transcript.keccak(b"words/%d" % t)) if tag % 2 == 0 else \
+            (lambda t: G.Transcript.duplex(b"words/%d" % t))
+        packed = G.PackedAssignment(w.h, c.columns, c.columns, device=True)
+        refs[tag] = G.prove_spartan_piop(mk(tag), m, dg, m.r1cs_products(packed), packed)
+        keep[tag] = mk(tag)
+        eng.submit_assignment(tag, keep[tag], m, dg, w.h, c.columns, c.columns)
+    while len(eng.done()) < len(sizes):
+        eng.wait(0.01)
+    for tag in range(len(sizes)):
+        assert same(eng.take(tag), refs[tag]), tag
+    c, m, dg = circuits[1]
+    t = G.Transcript.keccak(b"words/err")
+    before = G.Transcript.keccak(b"words/err").squeeze()
+    with pytest.raises(G.SumcheckError) as e:
+        eng.submit_assignment(99, t, m, dg, np.zeros((c.columns + 63) // 64, np.uint64), c.columns, c.columns)
+    assert e.value.variant == "InvalidAssignmentConstant"
+    with pytest.raises(G.SumcheckError) as e:
+        eng.submit_assignment(99, t, m, dg, np.ones(1000, np.uint64), c.columns - 1, c.columns)
+    assert e.value.variant == "InvalidAssignmentLength"
+    assert t.squeeze() == before  # nothing reached the transcript

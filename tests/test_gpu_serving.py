@@ -76,5 +76,6 @@ def test_a_served_proof_matches_the_oracle():
     reqs = [r for r in S.poisson_requests(rate=5000.0, count=8, seed=5, max_log_blocks=2) if r.blocks == 2][:1]
     assert reqs
     r = srv.serve(reqs)[0]
-    _, ref = prove_and_check(srv.circuit(2).circuit, r.message, list(r.iv), label=b"bitz/serve/%d" % r.rid)
+    _, ref = prove_and_check(srv.circuit(2).circuit, r.message, list(r.iv), label=b"bitz/serve/%d" % r.rid,
+                             keccak=True)
     assert same(r.result, ref)

@@ -40,8 +40,8 @@ def products_host(ex, h_bits, rows, padded):
     return out
 
 
-def oracle_piop(label, digest, ex, rows, row_vars, col_vars, h_elems, az, bz, cz):
-    ot = O.DuplexTranscript(label)
+def oracle_piop(label, digest, ex, rows, row_vars, col_vars, h_elems, az, bz, cz, keccak=False):
+    ot = O.KeccakTranscript(label) if keccak else O.DuplexTranscript(label)
     O.absorb(ot, 0, digest)
     tau = np.array([[(x := O.squeeze(ot)) & (2**64 - 1), x >> 64] for _ in range(row_vars)], np.uint64)
     split = row_vars // 2
@@ -59,14 +59,16 @@ def oracle_piop(label, digest, ex, rows, row_vars, col_vars, h_elems, az, bz, cz
     return ot, outer, inner
 
 
-def prove_and_check(c: S.Sha256Circuit, message: bytes, iv=None, label=b"sha256/gpu"):
+def prove_and_check(c: S.Sha256Circuit, message: bytes, iv=None, label=b"sha256/gpu", keccak=False):
+    """The PIOP of the statement on the device (duplex or Keccak transcript),
+    each output against the oracle's composition in the reference's order."""
     w = c.witness(message, iv=iv)
     assert w.satisfied
     prepared, digest = c.prepare()
     rv, cv = c.num_row_vars, c.num_column_vars
     assignment = G.assignment_mle_words(w.h, c.columns, c.columns, device=True)
     products = prepared.r1cs_products(assignment)
-    gt = G.Transcript.duplex(label)
+    gt = G.Transcript.keccak(label) if keccak else G.Transcript.duplex(label)
     out = G.prove_spartan_piop(gt, prepared, digest, products, assignment)
     ex = c.export()
     h_bits = np.unpackbits(w.h.view(np.uint8), bitorder="little")[: 1 << cv]
@@ -78,7 +80,7 @@ def prove_and_check(c: S.Sha256Circuit, message: bytes, iv=None, label=b"sha256/
     assert np.array_equal(products.az.to_host(), az)
     assert np.array_equal(products.bz.to_host(), bz)
     assert np.array_equal(products.cz.to_host(), cz)
-    ot, outer, inner = oracle_piop(label, digest, ex, c.rows, rv, cv, h_elems, az, bz, cz)
+    ot, outer, inner = oracle_piop(label, digest, ex, c.rows, rv, cv, h_elems, az, bz, cz, keccak)
     assert np.array_equal(out.outer_rounds, outer["rounds"])
     assert out.outer_claims == ints(outer["claims"])
     assert np.array_equal(out.inner_rounds, inner["rounds"])
