@@ -523,7 +523,7 @@ class Transcript:
 
 
 # ------------------------------------------------------------------- poly ---
-MUL_LANE, MUL_LANE2, MUL_CONST = 0x100, 0x200, 0x400  # BITZ_MUL_* (product primitive)
+MUL_LANE, MUL_LANE2, MUL_CONST, MUL_ROWS = 0x100, 0x200, 0x400, 0x800  # BITZ_MUL_* (product primitive)
 
 
 def mul_batch(a, b, mode: int = 0) -> np.ndarray:

@@ -662,7 +662,7 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
     TRY(upload(da, a, n, flags, so.s));
     TRY(upload(db, b, n, flags, so.s));
     TRY(dout.alloc(n * sizeof(f128)));
-    const uint32_t mode = flags & (BITZ_MUL_LANE | BITZ_MUL_LANE2 | BITZ_MUL_CONST);
+    const uint32_t mode = flags & (BITZ_MUL_LANE | BITZ_MUL_LANE2 | BITZ_MUL_CONST | BITZ_MUL_ROWS);
     if (mode == 0) {
         hipLaunchKernelGGL(bitz::k_mul, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
     } else if (mode == BITZ_MUL_LANE || mode == BITZ_MUL_LANE2) {
@@ -676,6 +676,8 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
                            db.f(), dout.f(), n, mode == BITZ_MUL_LANE2 ? 1 : 0);
     } else if (mode == BITZ_MUL_CONST) {
         hipLaunchKernelGGL(bitz::k_mul_const, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
+    } else if (mode == BITZ_MUL_ROWS) {
+        hipLaunchKernelGGL(bitz::k_mul_rows, dim3(grid_for(16 * n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
     } else {
         return fail(BITZ_ERR_ARG, "one product mode at a time");
     }
@@ -4038,6 +4040,12 @@ void devfs_emit(PiopJob* j, size_t i, EnginePos& P) {
             f.tau = h->tau[k];
             f.tau_inv = h->tau_inv[k];
             f.scale = h->scale_big[k];
+            {
+                f128 one_tau = h->tau[k];
+                one_tau.lo ^= 1u;
+                f.scale_t = hostf::mul(f.scale, one_tau);
+                f.scale_ti = q1 ? f128{0, 0} : hostf::mul(f.scale_t, h->tau_inv[k]);
+            }
             P.fs.push_back(f);
             if (k > 0) {
                 h->cur[0] = d.ao;

@@ -175,40 +175,41 @@ __device__ __forceinline__ g4 fs_squeeze(FsLds& f) {
     return r;
 }
 
-// Absorbs the two-element message [m0, m1] of `kind` and squeezes.  At
-// position 0 (after any squeeze) every byte's place is fixed: the framing
-// 'P'/'N' and le64(32) at bytes 0..8, the 32 bytes at 9..40, 'S' at 41, the
-// pad 0x1F at 42 and 0x80 at 135, so wave 0 XORs seven words into the
-// state on scalars and permutes.  Elsewhere every thread takes the byte
-// path.  Called by all threads; m0, m1 are read from wave 0 (from thread 0
-// on the byte path).  Lanes of wave 0 read their state word before any
-// writes it back: a wave's LDS reads and writes complete in order.
-__device__ __forceinline__ g4 fs_message2_squeeze(FsLds& f, int kind, const g4& m0, const g4& m1) {
+// Absorbs the message m[0, n) (n <= 4 elements) of `kind` and squeezes.  At
+// position 0 (after any squeeze) every byte's place is fixed: 'P'/'N' and
+// le64(16 n) at bytes 0..8, the elements' bytes at 9 .. 8 + 16 n, 'S' and the
+// pad 0x1F after them and 0x80 at byte 135, so lane l of wave 0 builds state
+// word l of that block from the payload words P_(l-1), P_(l-2) (one byte
+// apart) and permutes.  Elsewhere every thread takes the byte path.  Called
+// by all threads; m is read from wave 0 (from thread 0 on the byte path).
+// Lanes of wave 0 read their state word before any writes it back: a
+// wave's LDS reads and writes complete in order.
+__device__ __forceinline__ g4 fs_message_squeeze(FsLds& f, int kind, const g4* m, int n) {
     if (f.pos != 0) {
-        const g4 m[2] = {m0, m1};
-        fs_message(f, kind, m, 2);
+        fs_message(f, kind, m, n);
         return fs_squeeze(f);
     }
     if (threadIdx.x < 64) {
-        auto word = [](const g4& x, int k) { return (uint64_t)x.w[2 * k] | ((uint64_t)x.w[2 * k + 1] << 32); };
-        const uint64_t lo0 = word(m0, 0), hi0 = word(m0, 1), lo1 = word(m1, 0), hi1 = word(m1, 1);
         const int l = min((int)threadIdx.x, 24);
-        uint64_t m = 0;  // this lane's word of the framed message, 'S' and the pad
-        switch (l) {
-            case 0: m = (uint64_t)(kind == 0 ? 'P' : 'N') | (32ull << 8); break;
-            case 1: m = lo0 << 8; break;
-            case 2: m = (lo0 >> 56) | (hi0 << 8); break;
-            case 3: m = (hi0 >> 56) | (lo1 << 8); break;
-            case 4: m = (lo1 >> 56) | (hi1 << 8); break;
-            case 5: m = (hi1 >> 56) | ((uint64_t)'S' << 8) | (0x1Full << 16); break;
-            case (kKeccakRate - 1) >> 3: m = 0x80ull << 56; break;
-            default: break;
+        uint64_t cur = 0, prev = 0;  // P_(l-1), P_(l-2), zero outside [0, 2 n)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i < 2 * n) {
+                const g4& e = m[i >> 1];
+                const uint64_t p = (uint64_t)e.w[2 * (i & 1)] | ((uint64_t)e.w[2 * (i & 1) + 1] << 32);
+                if (i == l - 1) cur = p;
+                if (i == l - 2) prev = p;
+            }
         }
-        const uint64_t w = fs_permute_wave(f.a[l] ^ m);
-        if (threadIdx.x < 25) f.a[threadIdx.x] = w;
+        uint64_t w = (cur << 8) | (prev >> 56);
+        if (l == 0) w = (uint64_t)(kind == 0 ? 'P' : 'N') | ((uint64_t)(16 * n) << 8);
+        if (l == 2 * n + 1) w |= ((uint64_t)'S' << 8) | (0x1Full << 16);
+        if (l == (kKeccakRate - 1) >> 3) w |= 0x80ull << 56;
+        const uint64_t a = fs_permute_wave(f.a[l] ^ w);
+        if (threadIdx.x < 25) f.a[threadIdx.x] = a;
         if (threadIdx.x < 2) {
-            f.out[2 * threadIdx.x] = (uint32_t)w;
-            f.out[2 * threadIdx.x + 1] = (uint32_t)(w >> 32);
+            f.out[2 * threadIdx.x] = (uint32_t)a;
+            f.out[2 * threadIdx.x + 1] = (uint32_t)(a >> 32);
         }
     }
     __syncthreads();
@@ -216,6 +217,12 @@ __device__ __forceinline__ g4 fs_message2_squeeze(FsLds& f, int kind, const g4& 
 #pragma unroll
     for (int i = 0; i < 4; ++i) r.w[i] = f.out[i];
     return r;
+}
+
+// The two-element message [m0, m1] of `kind`, then the squeeze.
+__device__ __forceinline__ g4 fs_message2_squeeze(FsLds& f, int kind, const g4& m0, const g4& m1) {
+    const g4 m[2] = {m0, m1};
+    return fs_message_squeeze(f, kind, m, 2);
 }
 
 __device__ __forceinline__ bool g_is_zero(const g4& a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) == 0; }
@@ -247,6 +254,25 @@ __device__ __forceinline__ g4 g_mul_wave(const g4& a, const g4& b) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) t.w[i] ^= (uint32_t)__shfl_xor((int)t.w[i], off, 64);
     return g_reduce(t);
+}
+
+// a[i] * b[i] for i < 4 by g_mul_rows, row i taking product i; every lane
+// gets all four products.  The operands are the same in every lane; a row
+// picks its pair with XOR masks.  The whole wave must be active.
+__device__ __forceinline__ void g_mul4(const g4 (&a)[4], const g4 (&b)[4], g4 (&out)[4]) {
+    const uint32_t g = (threadIdx.x >> 4) & 3u;
+    const uint32_t m1 = 0u - (uint32_t)(g == 1), m2 = 0u - (uint32_t)(g == 2), m3 = 0u - (uint32_t)(g == 3);
+    g4 x, y;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        x.w[k] = a[0].w[k] ^ ((a[0].w[k] ^ a[1].w[k]) & m1) ^ ((a[0].w[k] ^ a[2].w[k]) & m2) ^
+                 ((a[0].w[k] ^ a[3].w[k]) & m3);
+        y.w[k] = b[0].w[k] ^ ((b[0].w[k] ^ b[1].w[k]) & m1) ^ ((b[0].w[k] ^ b[2].w[k]) & m2) ^
+                 ((b[0].w[k] ^ b[3].w[k]) & m3);
+    }
+    const g4 p = g_mul_rows(x, y);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = g_row(p, i);
 }
 
 }  // namespace bitz

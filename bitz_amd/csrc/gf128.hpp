@@ -154,6 +154,58 @@ __device__ __forceinline__ g4 g_reduce(const g8& c) {
 
 __device__ __forceinline__ g4 g_mul(const g4& a, const g4& b) { return g_reduce(clmul(a, b)); }
 
+// x ^ (x of the lane `ctrl` names), a DPP move within a row of 16 lanes.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t xor_dpp(uint32_t x) {
+    return x ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, Ctrl, 0xF, 0xF, false);
+}
+
+// Four products in one wave: the 16 lanes of row g (lanes 16 g .. 16 g + 15)
+// multiply that row's a by its b, and every lane of the row gets the
+// product.  Lane j of a row adds a X^(32 w + j) and a X^(32 w + j + 16) for
+// the words w of b whose bits j, j + 16 are set (8 shifted copies of a);
+// the row's XOR is four DPP steps (quad swaps, then half-row and row
+// mirrors), with no trip through LDS; then the reduction.
+__device__ __forceinline__ g4 g_mul_rows(const g4& a, const g4& b) {
+    const int j = (int)(threadIdx.x & 15);
+    g8 t = w_zero();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int sh = j + 16 * h;  // 0 .. 31, this lane's bit
+        // a << sh over five words
+        uint32_t s[5];
+        s[0] = a.w[0] << sh;
+#pragma unroll
+        for (int i = 1; i < 4; ++i) s[i] = sh ? shl_pair(a.w[i], a.w[i - 1], sh) : a.w[i];
+        s[4] = sh ? a.w[3] >> (32 - sh) : 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)b.w[w], (uint32_t)sh, 1u);  // bit sh of b.w[w], spread
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (w + i < 8) t.w[w + i] = xor_and(t.w[w + i], s[i], m);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t x = t.w[i];
+        x = xor_dpp<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+        x = xor_dpp<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+        x = xor_dpp<0x141>(x);  // row_half_mirror: the other quad of the half row
+        x = xor_dpp<0x140>(x);  // row_mirror: the other half of the row
+        t.w[i] = x;
+    }
+    return g_reduce(t);
+}
+
+// Row g's value of v (lane 16 g), in every lane.
+__device__ __forceinline__ g4 g_row(const g4& v, int g) {
+    g4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.w[i] = (uint32_t)__builtin_amdgcn_readlane((int)v.w[i], 16 * g);
+    return r;
+}
+
 #endif  // __HIPCC__
 
 // ---------------------------------------------------------------- host -----
@@ -288,6 +340,39 @@ __device__ __forceinline__ void build_const_table4(uint4* tab, uint4* basis, con
         tab[e] = x;
     }
     __syncthreads();
+}
+
+// c * X: one shift, X^128 folded back as X^7 + X^2 + X + 1.
+__device__ __forceinline__ g4 g_mul_x(const g4& b) {
+    g4 r;
+    const uint32_t top = b.w[3] >> 31;
+    r.w[3] = shl_pair(b.w[3], b.w[2], 1);
+    r.w[2] = shl_pair(b.w[2], b.w[1], 1);
+    r.w[1] = shl_pair(b.w[1], b.w[0], 1);
+    r.w[0] = (b.w[0] << 1) ^ (0x87u & (0u - top));
+    return r;
+}
+
+// c's 4-bit table (build_const_table4's layout) written by one wave straight
+// to `out` (global or LDS), with no barrier: lane l builds window l / 2 from
+// c X^(4 w) and its three doublings and writes entries 8 (l % 2) .. + 7 of
+// it.  About 150 instructions against build_const_table4's two LDS passes.
+__device__ __forceinline__ void const_table4_wave(uint4* out, const g4& c) {
+    const int lane = (int)(threadIdx.x & 63), w = lane >> 1, h = lane & 1;
+    g4 b[4];
+    b[0] = g_mul_xpow(c, 4 * w);
+    b[1] = g_mul_x(b[0]);
+    b[2] = g_mul_x(b[1]);
+    b[3] = g_mul_x(b[2]);
+    const g4 base = h ? b[3] : g_zero();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        g4 e = base;
+        if (u & 1) e = g_add(e, b[0]);
+        if (u & 2) e = g_add(e, b[1]);
+        if (u & 4) e = g_add(e, b[2]);
+        out[16 * w + 8 * h + u] = make_uint4(e.w[0], e.w[1], e.w[2], e.w[3]);
+    }
 }
 
 // LDS accesses through 32-bit LDS byte addresses, so lookups can build

@@ -187,6 +187,21 @@ __global__ __launch_bounds__(kBlock) void k_mul_lane(const f128* a, const f128* 
         }
     }
 }
+// g_mul_rows on its own (bitz_mul_batch's BITZ_MUL_ROWS): the grid's 16-lane
+// row q multiplies a[q] by b[q]; lane 0 of each wave writes its four rows'
+// products, read with g_row.  The stride keeps a row's lanes together.
+__global__ __launch_bounds__(kBlock) void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n) {
+    const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 4;
+    for (size_t q = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4; q < n; q += stride) {
+        const g4 p = g_mul_rows(g_load(a + q), g_load(b + q));
+        const size_t q0 = q & ~(size_t)3;  // row 0 of this wave (always active when any row is)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const g4 v = g_row(p, g);
+            if ((threadIdx.x & 63) == 0 && q0 + g < n) g_store(out + q0 + g, v);
+        }
+    }
+}
 __global__ __launch_bounds__(kBlock) void k_mul_const(const f128* a, const f128* b, f128* out, size_t n) {
     __shared__ __attribute__((aligned(256))) uint4 tab[kConstTable4Entries];
     __shared__ uint4 basis[128];
@@ -1981,24 +1996,15 @@ __global__ __launch_bounds__(kStageBlock) void k_gkr_tail_fs(const GkrTail* d, G
 namespace bitz {
 
 // ============================== prove_spartan_piop: transcript on the device ===
-// One wave per proof: products by the whole wave (g_mul_wave) where they
-// chain, one per lane (g_mul) where many are independent; the duplex as in
-// fs.hpp; r's 4-bit table built in LDS and written for the next round's fold.
-__device__ __forceinline__ g4 fs_lane_xor(g4 v, int lanes) {
-    // XOR of v over lanes [0, lanes) (lanes a power of two <= 64; others hold zero)
-    for (int off = lanes >> 1; off >= 1; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v.w[i] ^= (uint32_t)__shfl_xor((int)v.w[i], off, 64);
-    return v;
-}
-
+// One wave per proof.  Products come four at a time from g_mul4 (one 16-lane
+// row each), grouped into levels of independent products; the message and
+// squeeze go through the fixed-position duplex path; r's 4-bit table is
+// written straight to the next round's fold by const_table4_wave.
 __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
     const PiopFsDesc D = descs[blockIdx.x];
     PiopFsState* st = D.st;
     const int lane = (int)threadIdx.x;
     __shared__ FsLds f;
-    __shared__ uint4 tab[kConstTable4Entries];
-    __shared__ uint4 basis[128];
     __shared__ uint32_t words[8 << kInnerBitsMaxRound];
     auto sum_at = [&](int word) {
         g4 v;
@@ -2006,6 +2012,7 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
         for (int i = 0; i < 4; ++i) v.w[i] = D.sums[word + i];
         return v;
     };
+    const g4 Z = g_zero();
     g4 claim = g_load(&st->claim);
     if (D.kind == kFsInnerFinal) {  // D(r) and the final claim; the transcript is untouched
         if (lane == 0) {
@@ -2018,15 +2025,16 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
     if (D.kind == kFsOuterFinal) {
         // sumcheck.rs:372 then piop.rs:94-97: absorb [A, B, C](r), squeeze rho,
         // the inner claim A + rho B + rho^2 C.
-        g4 cl[3] = {sum_at(0), sum_at(4), sum_at(8)};
-        fs_message(f, 0, cl, 3);
-        const g4 rho = fs_squeeze(f);
-        claim = g_add(cl[0], g_mul_wave(rho, g_add(cl[1], g_mul_wave(rho, cl[2]))));
-        const g4 rho2 = g_mul_wave(rho, rho);
+        const g4 cl[3] = {sum_at(0), sum_at(4), sum_at(8)};
+        const g4 rho = fs_message_squeeze(f, 0, cl, 3);
+        g4 p[4], p2[4];
+        g_mul4({rho, rho, rho, Z}, {cl[1], cl[2], rho, Z}, p);  // rho B, rho C, rho^2
+        g_mul4({rho, Z, Z, Z}, {p[1], Z, Z, Z}, p2);          // rho^2 C
+        claim = g_add3(cl[0], p[0], p2[0]);
         if (lane == 0) {
             for (int u = 0; u < 3; ++u) g_store(D.round_out + u, cl[u]);
             g_store(&st->rho, rho);
-            g_store(&st->rho2, rho2);
+            g_store(&st->rho2, p[2]);
             g_store(&st->claim, claim);
         }
         fs_save(&st->ts, f);
@@ -2036,25 +2044,32 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
     int deg;
     g4 q[3];
     if (D.kind == kFsOuterRound) {
-        // The eq-factored round: q(0), q(inf) (and q(1)) scaled, c0 = S (1 + tau) q0,
-        // c2 = S ((1 + tau) q2 + q1), c3 = S q2 with q1 = q(1) + q0 + q2.
-        const g4 scale = g_from(D.scale), tau = g_from(D.tau);
-        g4 one_tau = tau;
-        one_tau.w[0] ^= 1u;
-        q[0] = g_mul_wave(sum_at(0), scale);
-        q[2] = g_mul_wave(sum_at(4), scale);
-        const g4 at1 = D.have_q1 ? g_mul_wave(sum_at(8), scale)
-                                 : g_mul_wave(g_add(g_load(&st->sigma), g_mul_wave(one_tau, q[0])), g_from(D.tau_inv));
-        q[1] = g_add3(at1, q[0], q[2]);
+        // The eq-factored round: q0 = raw0 scale, q2 = raw2 scale, q(1) = raw1 scale
+        // or (sigma + (1 + tau) q0) / tau, q1 = q(1) + q0 + q2; c0 = S (1 + tau) q0,
+        // c2 = S ((1 + tau) q2 + q1), c3 = S q2, c1 = claim + c2 + c3.
+        const g4 scale = g_from(D.scale), st_ = g_from(D.scale_t);
+        const g4 raw0 = sum_at(0), raw2 = sum_at(4);
+        g4 p[4], p2[4];
+        g_mul4({raw0, raw2, raw0, raw2}, {scale, scale, st_, st_}, p);  // q0, q2, (1+tau) q0, (1+tau) q2
+        if (D.have_q1)
+            g_mul4({sum_at(8), Z, Z, Z}, {scale, Z, Z, Z}, p2);
+        else
+            g_mul4({g_load(&st->sigma), raw0, Z, Z}, {g_from(D.tau_inv), g_from(D.scale_ti), Z, Z}, p2);
+        q[0] = p[0];
+        q[2] = p[1];
+        q[1] = g_add3(g_add(p2[0], p2[1]), q[0], q[2]);
         const g4 S = g_load(&st->S);
-        c[0] = g_mul_wave(S, g_mul_wave(one_tau, q[0]));
-        c[2] = g_mul_wave(S, g_add(g_mul_wave(one_tau, q[2]), q[1]));
-        c[3] = g_mul_wave(S, q[2]);
+        g4 s3[4];
+        g_mul4({S, S, S, Z}, {p[2], g_add(p[3], q[1]), q[2], Z}, s3);
+        c[0] = s3[0];
+        c[2] = s3[1];
+        c[3] = s3[2];
         c[1] = g_add3(claim, c[2], c[3]);
         deg = 3;
     } else {
         if (D.K >= 0) {
-            // A bit round: the copies' XOR, then sum_S r_S (sum S of c0 / c2).
+            // A bit round: the copies' XOR, then sum_S r_S (sum S of c0 / c2),
+            // products S of c0 in rows of the first pass(es), then those of c2.
             const int S = 1 << D.K;
             if (lane < 8 * S) {
                 uint32_t x = 0;
@@ -2069,24 +2084,23 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
                 st->live = lv;
             }
             __syncthreads();
-            g4 prod = g_zero();
-            if (lane < 2 * S) {
-                g4 v;
+            g4 lo = Z, hi = Z;
+            for (int base = 0; base < 2 * S; base += 4) {
+                const int idx = base + (lane >> 4);  // this row's product
+                g4 x = Z, y = Z;
+                if (idx < 2 * S) {
+                    x = g_load(&st->mono[idx & (S - 1)]);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) v.w[i] = words[4 * lane + i];
-                prod = g_mul(g_load(&st->mono[lane & (S - 1)]), v);
+                    for (int i = 0; i < 4; ++i) y.w[i] = words[4 * idx + i];
+                }
+                const g4 p = g_mul_rows(x, y);
+                if (idx < S)
+                    lo = g_add(lo, p);
+                else
+                    hi = g_add(hi, p);
             }
-            // c0 over lanes [0, S), c2 over [S, 2S): XOR within each half.
-            const g4 lo = fs_lane_xor(lane < S ? prod : g_zero(), 2 * S);
-            const g4 hi = fs_lane_xor(lane >= S && lane < 2 * S ? prod : g_zero(), 2 * S);
-            c[0] = lo;
-            c[2] = hi;
-            // every lane of the first 2S has the sums; broadcast lane 0's
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                c[0].w[i] = (uint32_t)__shfl((int)c[0].w[i], 0, 64);
-                c[2].w[i] = (uint32_t)__shfl((int)c[2].w[i], 0, 64);
-            }
+            c[0] = g_add(g_add(g_row(lo, 0), g_row(lo, 1)), g_add(g_row(lo, 2), g_row(lo, 3)));
+            c[2] = g_add(g_add(g_row(hi, 0), g_row(hi, 1)), g_add(g_row(hi, 2), g_row(hi, 3)));
         } else {
             c[0] = sum_at(0);
             c[2] = sum_at(4);
@@ -2095,25 +2109,37 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
         deg = 2;
     }
     // sumcheck.rs:803-829: absorb the full round polynomial, squeeze r, claim = g(r).
-    fs_message(f, 0, c, deg + 1);
-    const g4 r = fs_squeeze(f);
-    g4 g = c[deg];
-    for (int i = deg - 1; i >= 0; --i) g = g_add(c[i], g_mul_wave(g, r));
-    claim = g;
+    const g4 r = fs_message_squeeze(f, 0, c, deg + 1);
     if (D.kind == kFsOuterRound) {
-        // sigma_{k+1} = q_k(r), S_{k+1} = S_k eq(tau_k, r) = S_k (1 + tau_k + r)
-        const g4 sigma = g_add(q[0], g_mul_wave(r, g_add(q[1], g_mul_wave(r, q[2]))));
+        // claim = c0 + c1 r + (c2 + c3 r) r^2; sigma_{k+1} = q_k(r) = q0 + q1 r + q2 r^2;
+        // S_{k+1} = S_k eq(tau_k, r) = S_k (1 + tau_k + r)
+        g4 p[4], p2[4];
+        g_mul4({r, c[3], c[1], q[1]}, {r, r, r, r}, p);  // r^2, c3 r, c1 r, q1 r
         g4 e = g_add(g_from(D.tau), r);
         e.w[0] ^= 1u;
-        const g4 S = g_mul_wave(g_load(&st->S), e);
+        g_mul4({g_add(c[2], p[1]), q[2], g_load(&st->S), Z}, {p[0], p[0], e, Z}, p2);
+        claim = g_add3(c[0], p[2], p2[0]);
+        const g4 sigma = g_add3(q[0], p[3], p2[1]);
         if (lane == 0) {
             g_store(&st->sigma, sigma);
-            g_store(&st->S, S);
+            g_store(&st->S, p2[2]);
         }
-    } else if (D.mono_next) {
-        // r_S of the challenges so far, for the next bit round: S with bit k set
-        const int half = 1 << D.k;
-        if (lane < half) g_store(&st->mono[half + lane], g_mul(g_load(&st->mono[lane]), r));
+    } else {
+        // claim = c0 + c1 r + c2 r^2; with mono_next, r_S for the next bit round:
+        // mono[half + i] = mono[i] r (S with bit k set), half = 2^k <= 4
+        const int half = D.mono_next ? 1 << D.k : 0;
+        const g4 m0 = half > 0 ? g_load(&st->mono[0]) : Z, m1 = half > 1 ? g_load(&st->mono[1]) : Z;
+        const g4 m2 = half > 2 ? g_load(&st->mono[2]) : Z, m3 = half > 3 ? g_load(&st->mono[3]) : Z;
+        g4 p[4], p2[4];
+        g_mul4({r, c[1], m0, m1}, {r, r, r, r}, p);
+        g_mul4({c[2], m2, m3, Z}, {p[0], r, r, Z}, p2);
+        claim = g_add3(c[0], p[1], p2[0]);
+        if (lane == 0) {
+            if (half > 0) g_store(&st->mono[half], p[2]);
+            if (half > 1) g_store(&st->mono[half + 1], p[3]);
+            if (half > 2) g_store(&st->mono[half + 2], p2[1]);
+            if (half > 3) g_store(&st->mono[half + 3], p2[2]);
+        }
     }
     if (lane == 0) {
         for (int i = 0; i <= deg; ++i) g_store(D.round_out + i, c[i]);
@@ -2121,8 +2147,7 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
         g_store(&st->claim, claim);
     }
     fs_save(&st->ts, f);
-    build_const_table4(tab, basis, r);
-    for (int e = lane; e < kConstTable4Entries; e += 64) D.rtab[e] = tab[e];
+    const_table4_wave(D.rtab, r);
 }
 
 __global__ __launch_bounds__(64) void k_final_folds(const FinalFoldDesc* descs) {

@@ -15,6 +15,7 @@ constexpr int kBlock = 256;
 __global__ void k_mul(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, int two);
 __global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
+__global__ void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_eq_small(const f128* r, int k, f128* table);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
 __global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);
@@ -338,6 +339,7 @@ struct PiopFsDesc {
     uint4* rtab;
     f128* final_out;
     f128 tau, tau_inv, scale;  // outer round k: tau_k, 1 / tau_k, the sums' scale
+    f128 scale_t, scale_ti;    // scale (1 + tau_k), scale (1 + tau_k) / tau_k (zero when tau_k is)
     int kind, k, K, have_q1, mono_next;
 };
 __global__ void k_piop_fs(const PiopFsDesc* d);

@@ -29,10 +29,11 @@ def test_mul_batch_matches_oracle():
     assert np.array_equal(G.mul_batch(A, B), O.mul_batch(A, B))
 
 
-@pytest.mark.parametrize("mode", [G.MUL_LANE, G.MUL_LANE2])
+@pytest.mark.parametrize("mode", [G.MUL_LANE, G.MUL_LANE2, G.MUL_ROWS])
 def test_lane_table_products_match_oracle(mode):
     """The per-lane LDS table product (3-bit windows, a's top two bits added
-    separately) and its two-operand form on every pair of edge values (zero,
+    separately), its two-operand form and the four-per-wave row product on
+    every pair of edge values (zero,
     one, the reduction constant, the top bits) and on random elements, odd
     length included."""
     edges = EDGES + [1 << 126, 3 << 126, (1 << 126) - 1]
