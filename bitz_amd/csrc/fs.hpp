@@ -29,51 +29,87 @@ static __constant__ uint64_t kKeccakRoundConstants[24] = {
     0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
     0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
 
-// Keccak-f[1600] (FIPS 202 section 3.2) spread over a wave: lane l < 25
-// holds lane word l (x = l mod 5, y = l / 5); per round, theta's column
-// parities and D, pi and chi's row neighbours come through cross-lane reads
-// (ds_bpermute), rho is each lane's own rotation, iota lane 0's.  Lanes
-// 25..63 compute values no lane reads.  A wave issues one instruction per 4
-// clocks, so the 24 rounds' ~45 instructions each (against ~210 for the
-// rounds on scalars) set the time: about 6 us.
-struct KeccakLaneConsts {
-    int rho[25];
-    int pi_src[25];  // pi moves lane pi_src[l] to lane l
-};
-constexpr KeccakLaneConsts keccak_lane_consts() {
+// Keccak-f[1600] (FIPS 202 section 3.2) on one wave, with cross-lane moves
+// that stay off LDS except for pi.  Lane t = 8 y + j holds word (x, y) of the
+// state, x = j mod 5: lanes j = 5, 6, 7 of each group of eight repeat
+// x = 0, 1, 2, so that a row's neighbours x + 1, x + 2 (chi) and x - 1,
+// x + 1 (theta's D) are DPP moves within a 16-lane row; lanes 40..63 hold
+// zero.  Per round: theta's column parity C[x] is the XOR over the lanes of
+// equal j (a row rotation by 8, then the permlane32 and permlane16 swaps);
+// D = C[x - 1] ^ rot(C[x + 1], 1) from the lanes four on / one back (x = 0)
+// and one on; rho is each lane's own rotation; pi is one ds_bpermute from
+// lane 8 y' + x' of the source word, which also refills the repeats; chi
+// reads the next two lanes; iota goes to lanes 0 and 5 (word 0 and its
+// repeat).  The repeats in lanes j = 6, 7 go stale in chi, but only their
+// values after pi are ever read.  About 60 instructions and one LDS round
+// trip a round, against 20 ds_bpermutes for one word per lane.
+constexpr uint64_t keccak_rho_pack(int part) {
     constexpr int rho[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
-    KeccakLaneConsts k{};
-    for (int p = 0; p < 25; ++p) {
-        const int x = p % 5, y = p / 5;
-        k.rho[p] = rho[p];
-        k.pi_src[y + 5 * ((2 * x + 3 * y) % 5)] = p;
-    }
-    return k;
-}
-static __constant__ KeccakLaneConsts kKeccakLane = keccak_lane_consts();
-
-__device__ __forceinline__ uint64_t fs_shfl64(uint64_t v, int src) {
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
-    return ((uint64_t)hi << 32) | lo;
+    uint64_t v = 0;
+    for (int i = 0; i < 10 && 10 * part + i < 25; ++i) v |= (uint64_t)rho[10 * part + i] << (6 * i);
+    return v;
 }
 
-__device__ __forceinline__ uint64_t fs_permute_wave(uint64_t a) {
-    const int l = min((int)(threadIdx.x & 63), 24);
-    const int x = l % 5, y = l / 5;
-    const int rot = kKeccakLane.rho[l], src = kKeccakLane.pi_src[l];
-    const int xm1 = (x + 4) % 5, xp1 = (x + 1) % 5, xp2 = (x + 2) % 5;
+// The state word lane t holds (lanes 40..63 hold none).
+__device__ __forceinline__ int keccak_lane_word(int t) { return (t & 7) % 5 + 5 * min(t >> 3, 4); }
+__device__ __forceinline__ bool keccak_lane_owns(int t) { return t < 40 && (t & 7) < 5; }
+
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, Ctrl, 0xF, 0xF, false);
+}
+// XOR of v over the lanes t' with t' mod 8 == t mod 8.
+__device__ __forceinline__ uint32_t keccak_column_xor(uint32_t v) {
+    v ^= dpp_move<0x128>(v);  // row_ror:8, the other group of the row
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    v = p[0] ^ p[1];
+    const auto q = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return q[0] ^ q[1];
+}
+
+// Permutes this lane's word (lo, hi) of the state in the layout above; the
+// whole wave must be active.  DPP moves run for every lane and masks pick
+// among them: a move under a branch would read the disabled lanes as zero.
+__device__ __forceinline__ void keccak_wave(uint32_t& lo, uint32_t& hi) {
+    const int t = (int)(threadIdx.x & 63), j = t & 7, x = j % 5, y = min(t >> 3, 4), word = x + 5 * y;
+    const uint64_t rpack = word < 10 ? keccak_rho_pack(0) : word < 20 ? keccak_rho_pack(1) : keccak_rho_pack(2);
+    const int n = (int)((rpack >> (6 * (word % 10))) & 63), m = n & 31;
+    // rot(w, n) = alignbit pairs of w's halves, swapped first when n >= 32;
+    // n = 0 takes the swap too and alignbit by 0 swaps them back
+    const uint32_t swap = 0u - (uint32_t)((n >= 32) != (m == 0));
+    const uint32_t s = (uint32_t)((32 - m) & 31);
+    const int src = (x + 3 * y) % 5 + 5 * x;  // pi: B[x, y] = A[x + 3 y, x] (FIPS 202 3.2.3)
+    const int addr = 4 * (8 * (src / 5) + src % 5);
+    const uint32_t j0 = 0u - (uint32_t)(j == 0);
+    const uint32_t iota = 0u - (uint32_t)(t == 0 || t == 5), live = 0u - (uint32_t)(t < 40);
 #pragma unroll 1
     for (int round = 0; round < 24; ++round) {
-        const uint64_t c = fs_shfl64(a, x) ^ fs_shfl64(a, x + 5) ^ fs_shfl64(a, x + 10) ^ fs_shfl64(a, x + 15) ^
-                           fs_shfl64(a, x + 20);
-        a ^= fs_shfl64(c, xm1) ^ keccak_rotl(fs_shfl64(c, xp1), 1);
-        const uint64_t r = rot ? (a << rot) | (a >> (64 - rot)) : a;
-        const uint64_t b = fs_shfl64(r, src);
-        a = b ^ (~fs_shfl64(b, xp1 + 5 * y) & fs_shfl64(b, xp2 + 5 * y));
-        if (l == 0) a ^= kKeccakRoundConstants[round];
+        const uint32_t cl = keccak_column_xor(lo), ch = keccak_column_xor(hi);
+        const uint32_t pl = dpp_move<0x101>(cl), ph = dpp_move<0x101>(ch);  // C[x + 1]: row_shl:1
+        const uint32_t ml = (dpp_move<0x104>(cl) & j0) | (dpp_move<0x111>(cl) & ~j0);  // C[x - 1]
+        const uint32_t mh = (dpp_move<0x104>(ch) & j0) | (dpp_move<0x111>(ch) & ~j0);
+        lo ^= ml ^ __builtin_amdgcn_alignbit(pl, ph, 31);
+        hi ^= mh ^ __builtin_amdgcn_alignbit(ph, pl, 31);
+        const uint32_t u = (lo & swap) | (hi & ~swap), v = (hi & swap) | (lo & ~swap);
+        const uint32_t rh = __builtin_amdgcn_alignbit(u, v, s), rl = __builtin_amdgcn_alignbit(v, u, s);
+        const uint32_t bl = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)rl);
+        const uint32_t bh = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)rh);
+        const uint64_t rc = kKeccakRoundConstants[round];
+        lo = ((bl ^ (~dpp_move<0x101>(bl) & dpp_move<0x102>(bl))) ^ ((uint32_t)rc & iota)) & live;
+        hi = ((bh ^ (~dpp_move<0x101>(bh) & dpp_move<0x102>(bh))) ^ ((uint32_t)(rc >> 32) & iota)) & live;
     }
-    return a;
+}
+
+// Wave 0 XORs `m` (its lane's word of a block, zero for none) into the state
+// and permutes it; the owners write it back.  Returns the lane's new word.
+__device__ __forceinline__ uint64_t fs_permute_lds(FsLds& f, uint64_t m) {
+    const int t = (int)(threadIdx.x & 63), word = keccak_lane_word(t);
+    const uint64_t a = t < 40 ? f.a[word] ^ m : 0ull;
+    uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+    keccak_wave(lo, hi);
+    const uint64_t w = ((uint64_t)hi << 32) | lo;
+    if (keccak_lane_owns(t)) f.a[word] = w;
+    return w;
 }
 
 __device__ __forceinline__ void fs_load(FsLds& f, const KeccakState* g) {
@@ -91,13 +127,9 @@ __device__ __forceinline__ void fs_save(KeccakState* g, const FsLds& f) {
     }
 }
 
-// The permutation by wave 0, lane l on word l.
+// The permutation by wave 0.
 __device__ __forceinline__ void fs_permute(FsLds& f) {
-    if (threadIdx.x < 64) {
-        const int l = min((int)threadIdx.x, 24);
-        const uint64_t w = fs_permute_wave(f.a[l]);
-        if (threadIdx.x < 25) f.a[threadIdx.x] = w;
-    }
+    if (threadIdx.x < 64) (void)fs_permute_lds(f, 0ull);
     __syncthreads();
 }
 
@@ -190,7 +222,7 @@ __device__ __forceinline__ g4 fs_message_squeeze(FsLds& f, int kind, const g4* m
         return fs_squeeze(f);
     }
     if (threadIdx.x < 64) {
-        const int l = min((int)threadIdx.x, 24);
+        const int l = keccak_lane_word((int)threadIdx.x);  // this lane's word of the block
         uint64_t cur = 0, prev = 0;  // P_(l-1), P_(l-2), zero outside [0, 2 n)
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -205,9 +237,8 @@ __device__ __forceinline__ g4 fs_message_squeeze(FsLds& f, int kind, const g4* m
         if (l == 0) w = (uint64_t)(kind == 0 ? 'P' : 'N') | ((uint64_t)(16 * n) << 8);
         if (l == 2 * n + 1) w |= ((uint64_t)'S' << 8) | (0x1Full << 16);
         if (l == (kKeccakRate - 1) >> 3) w |= 0x80ull << 56;
-        const uint64_t a = fs_permute_wave(f.a[l] ^ w);
-        if (threadIdx.x < 25) f.a[threadIdx.x] = a;
-        if (threadIdx.x < 2) {
+        const uint64_t a = fs_permute_lds(f, w);
+        if (threadIdx.x < 2) {  // lanes 0, 1 hold words 0, 1
             f.out[2 * threadIdx.x] = (uint32_t)a;
             f.out[2 * threadIdx.x + 1] = (uint32_t)(a >> 32);
         }
