@@ -116,9 +116,10 @@ def main() -> int:
                     help="serve: also time every engine step (device span by HIP events, host launch and "
                          "finish, wait) and report per-step averages")
     ap.add_argument("--chain-blocks", type=int, default=64)
-    ap.add_argument("--serve-slots", type=int, default=32)
-    ap.add_argument("--serve-engines", type=int, default=0, help="serve: PIOP engines (0: one per 8 slots)")
-    ap.add_argument("--serve-preparers", type=int, default=4,
+    ap.add_argument("--serve-slots", type=int, default=96)
+    ap.add_argument("--serve-engines", type=int, default=4,
+                    help="serve: PIOP engines, one stream each (0: one per 8 slots)")
+    ap.add_argument("--serve-preparers", type=int, default=12,
                     help="serve: host threads preparing requests (witness, assignment, Az/Bz/Cz)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,

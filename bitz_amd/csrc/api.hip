@@ -60,9 +60,18 @@ struct DeviceCache {
     std::mutex mu;
     std::multimap<size_t, void*> free_blocks;
     size_t cached_bytes = 0;
+    // Sizes above 64 KiB round up to a quarter-octave class (at most 25%
+    // over), so that the blocks of requests of similar sizes are reused.
+    static size_t size_class(size_t n) {
+        if (n <= ((size_t)64 << 10)) return n;
+        int lg = 63 - __builtin_clzll((unsigned long long)n);
+        const size_t step = (size_t)1 << (lg - 2);
+        return (n + step - 1) & ~(step - 1);
+    }
     // A block of at least n bytes (at most 2 n when it comes from the
     // cache); *size receives its true size, which put() must be given back.
     void* get(size_t n, size_t* size) {
+        n = size_class(n);
         {
             std::lock_guard<std::mutex> lock(mu);
             auto it = free_blocks.lower_bound(n);
@@ -127,6 +136,15 @@ struct Dev {
         if (!p) return fail(BITZ_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
         bytes = got;
         return BITZ_OK;
+    }
+    // At least n bytes, growing geometrically; the old block goes back to the
+    // cache without a device drain, so the caller must know that nothing
+    // queued still uses it.
+    int32_t grow_idle(size_t n) {
+        if (p && bytes >= n) return BITZ_OK;
+        const size_t want = std::max<size_t>({n, 2 * bytes, (size_t)64 << 10});
+        release();
+        return alloc(want);
     }
     f128* f() const { return static_cast<f128*>(p); }
     uint32_t* u32() const { return static_cast<uint32_t*>(p); }
@@ -327,13 +345,14 @@ struct Profiler {
 } g_prof;
 
 // Adds host time to a profiling entry (the engine's host phases).
-static void prof_add_host(const char* name, std::chrono::steady_clock::time_point since) {
+static void prof_add_host(const char* name, std::chrono::steady_clock::time_point since, double bytes = 0) {
     if (!g_prof.on) return;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - since).count();
     std::lock_guard<std::mutex> lock(g_prof.mu);
     if (!g_prof.only.empty() && g_prof.only != name) return;
     KernelStat& k = g_prof.stats[name];
     k.ms += ms;
+    k.bytes += bytes;
     k.launches += 1;
 }
 
@@ -3704,8 +3723,7 @@ struct bitz_piop_engine {
     // challenges on the device (the next step is queued while one runs).
     std::deque<EngineStep> steps;
     Dev slots, stage[2], rt4;
-    void* pin[2] = {nullptr, nullptr};  // per step buffer: the uploads, then what comes back
-    size_t pin_bytes[2] = {0, 0};
+    PinnedBuf pin[2];  // per step buffer: the uploads, then what comes back (pooled across engines)
     hipEvent_t step_done[2] = {nullptr, nullptr};
     int next_buf = 0;
     // Background stepping (bitz_piop_engine_run): a thread of the engine's
@@ -3732,10 +3750,8 @@ struct bitz_piop_engine {
             worker.join();
         }
         (void)hipStreamSynchronize(so.s);
-        for (int b = 0; b < 2; ++b) {
-            if (pin[b]) (void)hipHostFree(pin[b]);
+        for (int b = 0; b < 2; ++b)
             if (step_done[b]) (void)hipEventDestroy(step_done[b]);
-        }
     }
 };
 
@@ -4220,7 +4236,7 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         slot_off[i + 1] = slot_off[i] + (j->devfs ? 0 : bits ? bitz::kInnerBitsAccWords : kEngineSlotWords);
     }
     const size_t slot_bytes = std::max<size_t>(slot_off[act.size()] * 4, 256);
-    if (slot_off[act.size()] && e->slots.bytes < slot_bytes) TRY(e->slots.alloc(slot_bytes));
+    if (slot_off[act.size()] && e->slots.bytes < slot_bytes) TRY(e->slots.alloc(std::max(slot_bytes, 2 * e->slots.bytes)));
     auto slot = [&](size_t i) { return e->slots.u32() + slot_off[i]; };
     // Challenge tables to build: the r each host-transcript fold round folds by.
     std::vector<f128> consts;
@@ -4236,7 +4252,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         }
     }
     if (e->rt4.bytes < std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4))
-        TRY(e->rt4.alloc(std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4)));
+        TRY(e->rt4.alloc(std::max(std::max<size_t>(consts.size(), 1) * kConstTable4Entries * sizeof(uint4),
+                                  2 * e->rt4.bytes)));
     const uint4* rt4 = static_cast<const uint4*>(e->rt4.p);
     // Host-transcript proofs: position 0.
     EnginePos& P0 = pos[0];
@@ -4341,6 +4358,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         for (int u = 0; u < npos && j->phase != JobPhase::kDevDone; ++u) devfs_emit(j, i, pos[u]);
         if (j->phase == JobPhase::kDevDone) finishing.push_back(i);
     }
+    prof_add_host("piop_engine_launch_emit", host_t0);
+    const auto stage_t0 = std::chrono::steady_clock::now();
     // Staging: challenges | host-transcript row points | every position's
     // descriptors and maps.
     size_t off = 0;
@@ -4379,23 +4398,20 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         fin_back[f] = back;
         back += act[finishing[f]]->fs_back_bytes();
     }
+    prof_add_host("piop_engine_launch_place", stage_t0);
+    const auto pin_t0 = std::chrono::steady_clock::now();
     // One pinned buffer stages the step's uploads, then receives what comes back.
     const size_t total = std::max<size_t>(off, 256);
     const size_t need = std::max(total, back);
     const int b = step.buf;
-    if (e->pin_bytes[b] < need) {
-        // A buffer the step before last used: that step has been finished.
-        if (e->pin[b]) (void)hipHostFree(e->pin[b]);
-        e->pin_bytes[b] = 0;
-        if (hipHostMalloc(&e->pin[b], 2 * need, hipHostMallocDefault) != hipSuccess) {
-            e->pin[b] = nullptr;
-            return fail(BITZ_ERR_ALLOC, "hipHostMalloc failed");
-        }
-        e->pin_bytes[b] = 2 * need;
-    }
-    if (e->stage[b].bytes < total) TRY(e->stage[b].alloc(total));
+    TRY(e->pin[b].alloc(need));  // the step before last used it, and has been finished
+    prof_add_host("piop_engine_launch_pin", pin_t0);
+    const auto dev_t0 = std::chrono::steady_clock::now();
+    TRY(e->stage[b].grow_idle(total));  // the step that used it last has been finished
+    prof_add_host("piop_engine_launch_devstage", dev_t0);
     if (!e->step_done[b]) HIP_TRY(hipEventCreateWithFlags(&e->step_done[b], hipEventDisableTiming));
-    char* pin = static_cast<char*>(e->pin[b]);
+    prof_add_host("piop_engine_launch_stage_alloc", stage_t0, (double)need);
+    char* pin = e->pin[b].c();
     char* dst = static_cast<char*>(e->stage[b].p);
     memcpy(pin + c_o, consts.data(), consts.size() * sizeof(f128));
     for (size_t b : P0.binds)
@@ -4416,6 +4432,9 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         memcpy(pin + P.fin_o, P.finals.data(), P.finals.size() * sizeof(bitz::FinalFoldDesc));
         memcpy(pin + P.fs_o, P.fs.data(), P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
+    prof_add_host("piop_engine_launch_stage", stage_t0, (double)off);
+    const auto kern_t0 = std::chrono::steady_clock::now();
+    double bind_ms = 0;
     e->t_step.begin(s, "piop_engine_step", (double)act.size());  // "bytes": proofs in the step
     for (PiopJob* j : act)
         if (j->ready) {  // a new proof: its set-up, queued on its own stream, first
@@ -4444,8 +4463,16 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         EnginePos& P = pos[u];
         // Binds (inner round 0 then reads D) and the final folds.
         for (size_t b : P.binds) {
+            const auto bind_t0 = std::chrono::steady_clock::now();
             PiopJob* j = act[b];
             std::lock_guard<std::mutex> lock(j->m->mu);
+            struct BindTime {
+                std::chrono::steady_clock::time_point t0;
+                double* ms;
+                ~BindTime() {
+                    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                }
+            } bt{bind_t0, &bind_ms};
             if (j->devfs)
                 TRY(bind_enqueue(j->m, j->fs_out(PiopJob::kOutRx), true, f128{0, 0}, j->dD.f(), s,
                                  &j->fs_state()->rho));
@@ -4533,6 +4560,15 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
     step.act = std::move(act);
     e->steps.push_back(std::move(step));
     e->next_buf ^= 1;
+    prof_add_host("piop_engine_launch_kernels", kern_t0);
+    if (g_prof.on) {
+        std::lock_guard<std::mutex> lock(g_prof.mu);
+        if (g_prof.only.empty()) {
+            KernelStat& k = g_prof.stats["piop_engine_launch_binds"];
+            k.ms += bind_ms;
+            k.launches += 1;
+        }
+    }
     prof_add_host("piop_engine_host_launch", host_t0);
     return BITZ_OK;
 }
@@ -4563,7 +4599,7 @@ static int32_t engine_finish_step(bitz_piop_engine* e, size_t* in_flight) {
     e->t_step.collect();
     for (auto& tp : e->t_bits)
         for (auto& t : tp) t.collect();
-    const char* pin = static_cast<const char*>(e->pin[step.buf]);
+    const char* pin = e->pin[step.buf].c();
     // Device-transcript proofs that ended in this step: their outputs, and
     // the duplex handed back to the caller's transcript.
     for (const auto& fb : step.fin_back) {
