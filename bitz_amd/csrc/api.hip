@@ -3278,23 +3278,48 @@ extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint3
 // stream s, after copying the row point (row_vars entries, host or device)
 // into the handle.  Uses the handle's scratch: callers hold h->mu while
 // enqueueing, and the bind_done event orders the kernels across streams.
+// A bind's working buffers (row point, eq table, batching scales, one product
+// per nonzero) for a caller that keeps its own set: the matrices handle's
+// set serves every other bind, one at a time.
+struct BindScratch {
+    Dev point, eq_rows, eq_scratch, row_scale, prod;
+    size_t rows_padded = 0, nnz = 0;
+    int row_vars = -1;
+    int32_t ensure(const bitz_matrices* h) {
+        if (prod.p && rows_padded == h->rows_padded && nnz == h->nnz && row_vars == h->row_vars) return BITZ_OK;
+        rows_padded = h->rows_padded;
+        nnz = h->nnz;
+        row_vars = h->row_vars;
+        TRY(point.alloc(std::max<int>(h->row_vars, 1) * sizeof(f128)));
+        TRY(eq_rows.alloc(h->rows_padded * sizeof(f128)));
+        TRY(row_scale.alloc(3 * h->rows_padded * sizeof(f128)));
+        return prod.alloc(std::max<size_t>(h->nnz, 1) * sizeof(f128));
+    }
+};
+
 static int32_t bind_enqueue(bitz_matrices* h, const f128* row_point, bool point_on_device, f128 rho, f128* target,
-                            hipStream_t s, const f128* rho_dev = nullptr) {
-    // The point, eq table, scales and products live in the handle: wait for
-    // the previous bind (possibly queued on another stream) first.
-    if (h->bind_done) HIP_TRY(hipStreamWaitEvent(s, h->bind_done, 0));
+                            hipStream_t s, const f128* rho_dev = nullptr, BindScratch* own = nullptr) {
+    // The point, eq table, scales and products live in the handle unless the
+    // caller brings its own: then wait for the previous bind (possibly
+    // queued on another stream) first.
+    if (!own && h->bind_done) HIP_TRY(hipStreamWaitEvent(s, h->bind_done, 0));
+    if (own) TRY(own->ensure(h));
+    f128* point = own ? own->point.f() : h->point.f();
+    f128* eq_rows = own ? own->eq_rows.f() : h->eq_rows.f();
+    f128* row_scale = own ? own->row_scale.f() : h->row_scale.f();
+    f128* prod = own ? own->prod.f() : h->prod.f();
     if (h->row_vars)
-        HIP_TRY(hipMemcpyAsync(h->point.p, row_point, (size_t)h->row_vars * sizeof(f128),
+        HIP_TRY(hipMemcpyAsync(point, row_point, (size_t)h->row_vars * sizeof(f128),
                                point_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-    TRY(eq_build(h->point.f(), h->row_vars, h->eq_rows.f(), h->scratch, s));
+    TRY(eq_build(point, h->row_vars, eq_rows, own ? own->eq_scratch : h->scratch, s));
     // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value or (rho_dev:
     // rho, rho^2) from the device
     if (rho_dev)
-        hipLaunchKernelGGL(bitz::k_row_scales_dev, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s,
-                           h->eq_rows.f(), rho_dev, h->row_scale.f(), h->rows_padded);
+        hipLaunchKernelGGL(bitz::k_row_scales_dev, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, eq_rows,
+                           rho_dev, row_scale, h->rows_padded);
     else
-        hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, h->eq_rows.f(),
-                           rho, hostf::mul(rho, rho), h->row_scale.f(), h->rows_padded);
+        hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, eq_rows, rho,
+                           hostf::mul(rho, rho), row_scale, h->rows_padded);
     // every nonzero's product (a converged pass), then each column's XOR over its run
     static bool attr = false;
     if (!attr) {
@@ -3306,16 +3331,17 @@ static int32_t bind_enqueue(bitz_matrices* h, const f128* row_point, bool point_
         const unsigned pgrid = (unsigned)std::max<size_t>(
             1, std::min<size_t>((h->nnz + bitz::kStageBlock - 1) / bitz::kStageBlock, 4096));
         hipLaunchKernelGGL(bitz::k_bind_products, dim3(pgrid), dim3(bitz::kStageBlock), bitz::kBindSmem, s,
-                           h->row_idx.u32(), static_cast<const uint8_t*>(h->which.p), h->values.f(), h->row_scale.f(),
-                           h->rows_padded, h->prod.f(), h->nnz);
+                           h->row_idx.u32(), static_cast<const uint8_t*>(h->which.p), h->values.f(), row_scale,
+                           h->rows_padded, prod, h->nnz);
     }
     const unsigned cgrid = (unsigned)std::max<size_t>(1, std::min<size_t>((h->cols + kBlock - 1) / kBlock, 1u << 20));
-    hipLaunchKernelGGL(bitz::k_bind_segments, dim3(cgrid), dim3(kBlock), 0, s, h->col_starts.u32(), h->prod.f(),
-                       target, h->cols);
+    hipLaunchKernelGGL(bitz::k_bind_segments, dim3(cgrid), dim3(kBlock), 0, s, h->col_starts.u32(), prod, target,
+                       h->cols);
     if (h->n_long_chunks)  // the long columns, left zero above: XOR of each run's sum
         hipLaunchKernelGGL(bitz::k_bind_long, dim3((unsigned)h->n_long_chunks), dim3(kBlock), 0, s,
-                           h->long_chunks.u32(), h->prod.f(), target);
+                           h->long_chunks.u32(), prod, target);
     TRY(check_launch());
+    if (own) return BITZ_OK;
     if (!h->bind_done) HIP_TRY(hipEventCreateWithFlags(&h->bind_done, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(h->bind_done, s));
     return BITZ_OK;
@@ -3723,6 +3749,9 @@ struct bitz_piop_engine {
     // challenges on the device (the next step is queued while one runs).
     std::deque<EngineStep> steps;
     Dev slots, stage[2], rt4;
+    // Bind buffers per circuit, on this engine's stream: binds on different
+    // engines never wait for one another.
+    std::map<const bitz_matrices*, std::unique_ptr<BindScratch>> binds;
     PinnedBuf pin[2];  // per step buffer: the uploads, then what comes back (pooled across engines)
     hipEvent_t step_done[2] = {nullptr, nullptr};
     int next_buf = 0;
@@ -4465,7 +4494,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         for (size_t b : P.binds) {
             const auto bind_t0 = std::chrono::steady_clock::now();
             PiopJob* j = act[b];
-            std::lock_guard<std::mutex> lock(j->m->mu);
+            std::unique_ptr<BindScratch>& own = e->binds[j->m];
+            if (!own) own.reset(new BindScratch);
             struct BindTime {
                 std::chrono::steady_clock::time_point t0;
                 double* ms;
@@ -4475,9 +4505,10 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
             } bt{bind_t0, &bind_ms};
             if (j->devfs)
                 TRY(bind_enqueue(j->m, j->fs_out(PiopJob::kOutRx), true, f128{0, 0}, j->dD.f(), s,
-                                 &j->fs_state()->rho));
+                                 &j->fs_state()->rho, own.get()));
             else
-                TRY(bind_enqueue(j->m, reinterpret_cast<const f128*>(dst + rx_o[b]), true, j->rho, j->dD.f(), s));
+                TRY(bind_enqueue(j->m, reinterpret_cast<const f128*>(dst + rx_o[b]), true, j->rho, j->dD.f(), s,
+                                 nullptr, own.get()));
         }
         for (size_t i : P.host_final) {
             PiopJob* j = act[i];
