@@ -319,8 +319,7 @@ struct PiopFsState {
     uint64_t pad_;
 };
 constexpr int kFsOuterRound = 0, kFsOuterFinal = 1, kFsInnerRound = 2, kFsInnerFinal = 3;
-// One proof's step after its round's kernels (one workgroup of kStageBlock
-// threads each):
+// One proof's step after its round's kernels (one 64-thread workgroup each):
 //   outer round k: c0, c2, c3 from the eq-factored sums (q(0), q(inf), q(1)
 //     at sums + 0, 4, 8, times `scale`; without q(1) it comes from sigma),
 //     c1 = claim + c2 + c3, absorb [c0..c3] (public), squeeze r, claim =

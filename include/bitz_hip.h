@@ -122,7 +122,9 @@ bitz_transcript* bitz_transcript_keccak(const uint8_t* domain, size_t len);
  * default; off: every round's message goes to the host transcript and the
  * challenge comes back).  Same transcript, messages and results either way;
  * no counterpart in the reference, whose transcript is host code.  Today:
- * bitz_gkr_prove without grinding (Bits100). */
+ * bitz_gkr_prove without grinding (Bits100), and the PIOP engine's proofs
+ * over a packed assignment (bitz_piop_engine_submit_packed / _assignment,
+ * bitz_server_serve). */
 int32_t bitz_set_device_fiat_shamir(int32_t on);
 /* One-message Keccak sponge (pad 0x06: SHA3-*, 0x1F: SHAKE*) on the host,
  * or with BITZ_DEVICE by one device thread: the permutation's check against
