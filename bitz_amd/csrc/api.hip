@@ -3745,8 +3745,8 @@ struct EngineStep {
 
 struct bitz_piop_engine {
     std::vector<std::unique_ptr<PiopJob>> jobs;
-    // Steps in flight, oldest first: one, or two when every proof draws its
-    // challenges on the device (the next step is queued while one runs).
+    // Steps in flight, oldest first (the worker keeps one; the two staging
+    // buffers let a caller of bitz_piop_engine_launch alternate them).
     std::deque<EngineStep> steps;
     Dev slots, stage[2], rt4;
     // Bind buffers per circuit, on this engine's stream: binds on different
@@ -4476,8 +4476,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
     if (!consts.empty())
         hipLaunchKernelGGL(bitz::k_const_tables4, dim3((unsigned)consts.size()), dim3(512), 0, s,
                            reinterpret_cast<const f128*>(dst + c_o), static_cast<uint4*>(e->rt4.p));
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr;  // engines step on threads of their own
+    std::call_once(attr, [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_r0),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kR0Smem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_batch_fold),
@@ -4486,8 +4486,7 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_inner_fold_batch),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kInnerGridSmem);
-        attr = true;
-    }
+    });
     for (int u = 0; u < npos; ++u) {
         EnginePos& P = pos[u];
         // Binds (inner round 0 then reads D) and the final folds.
@@ -4814,7 +4813,11 @@ static void piop_engine_worker(bitz_piop_engine* e) {
             e->incoming.clear();
             if (e->stop && !launchable() && e->steps.empty()) return;
         }
-        const size_t depth = (all_devfs() && !g_prof.on) ? 2 : 1;
+        // One step in flight: queueing a second behind it (the host building
+        // step N + 1 while the device runs step N) measured 10-20 % fewer
+        // blocks/s and a higher p50 under load; new proofs and finished ones
+        // then wait a step longer to join and to free their slots.
+        const size_t depth = 1;
         int32_t rc = BITZ_OK;
         if (e->steps.size() < depth && launchable()) rc = engine_launch_step(e, nullptr);
         if (!rc && !e->steps.empty() && (e->steps.size() >= depth || !launchable()))
