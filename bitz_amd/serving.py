@@ -195,7 +195,7 @@ class ProvingServer:
         and HBM budget; preparer threads compute each one's witness, then it
         joins the least loaded of `engines` PIOP engines, which upload its
         packed assignment, compute Az/Bz/Cz and advance every proof they hold
-        by up to 8 rounds per step (challenges drawn on the device from the
+        by up to 6 rounds per step (challenges drawn on the device from the
         Keccak transcript), each round kind in one launch across the proofs;
         harvester threads take finished proofs and free their slots.  engines
         defaults to one per 8 slots."""

@@ -441,7 +441,7 @@ int32_t bitz_post_gkr_prove(bitz_transcript* t, const bitz_f128* rows, uint32_t 
  * the host, in bitz_prove_spartan_piop's order.  A proof with a packed
  * assignment and a Keccak transcript (bitz_transcript_keccak) while device
  * Fiat-Shamir is on (bitz_set_device_fiat_shamir) draws its challenges on the
- * device instead and advances by up to 8 rounds per step; its transcript's
+ * device instead and advances by up to 6 rounds per step; its transcript's
  * state is handed back when it finishes.  A proof's outputs equal
  * bitz_prove_spartan_piop's for the same transcript and inputs; proofs
  * submitted between steps join at the next step (continuous batching of a
