@@ -17,7 +17,10 @@ timed live with HIP events on the engines' streams).
 
 Other workloads (--workload): outer (one 2^log_n outer sumcheck per step),
 batch (256 outer sumchecks of 2^12..2^22), gkr (grand product over 2^22
-leaves), inner, bind, chain (the sha256-chain statement).
+leaves), inner, bind, chain (the sha256-chain statement), products (each
+GF(2^128) product primitive over 2^log_n elements, the constant-table and
+int8-MFMA products by a fixed element among them), mt (M^T apply and
+transpose_inner over the SHA-256 circuits).
 
 Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
 the reference with the bit-serial multiply the reference selects on x86),
@@ -93,7 +96,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["serve", "outer", "batch", "gkr", "inner", "bind", "chain"],
+    ap.add_argument("--workload", choices=["serve", "outer", "batch", "gkr", "inner", "bind", "chain", "products", "mt"],
                     default="serve",
                     help="serve: SHA-256 proving requests served with continuous batching (default); "
                          "outer: one 2^log_n outer sumcheck per step; batch: 256 outer sumchecks "
@@ -165,6 +168,10 @@ def main() -> int:
         return bench_chain(G, args)
     if args.workload == "bind":
         return bench_bind(G, args)
+    if args.workload == "products":
+        return bench_products(G, args)
+    if args.workload == "mt":
+        return bench_mt(G, args)
 
     return bench_outer(G, args)
 
@@ -453,6 +460,7 @@ def bench_gkr(G, args) -> int:
         return float(np.percentile(np.array(ts[1:]) * 1e3, 50))
 
     keccak_device, keccak_host = keccak_p50(True), keccak_p50(False)
+    bittable = bittable_shape(G, args)
     log(f"bench: Keccak transcript: {keccak_device:.2f} ms with device Fiat-Shamir, {keccak_host:.2f} ms through the host")
     achieved = bind["bytes"] / (bind["ms"] * 1e-3) / 1e9 if bind["ms"] else 0.0
     children = sum(1 << (n + 1) for n in range(s, L))
@@ -476,6 +484,7 @@ def bench_gkr(G, args) -> int:
         "leaves_per_s": (1 << L) * args.steps / wall,
         "transcript_mode_p50_ms": float(np.percentile(np.array(tt) * 1e3, 50)),
         "keccak_transcript_p50_ms": {"device_fiat_shamir": keccak_device, "host_round_trips": keccak_host},
+        "bittable_t7_s15": bittable,
         "algorithmic_gb_per_s": 92.0 * children * args.steps / wall / 1e9,
         "roofline": {
             "bound": "hbm",
@@ -493,6 +502,56 @@ def bench_gkr(G, args) -> int:
         line["cpu_baseline"] = cpu_baseline_gkr(20, L, s)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bittable_shape(G, args, t: int = 7, s: int = 15) -> dict:
+    """The grand product over a BitTable at the end-to-end prover's shape
+    (2^t rows of images, 2^s trees: 2^(t+s) leaves, a uniform random table):
+    what expanding the table into dense leaves costs (the k_bittable_leaves
+    kernel and the HBM it takes) beside the tree build and a queued proof,
+    against the memory the reference's lookup-table form would hold."""
+    rng = np.random.default_rng(args.seed + 77)
+    images = rng.integers(0, 2**64, size=(1 << t, 2), dtype=np.uint64)
+    images[(images[:, 0] == 0) & (images[:, 1] == 0), 0] = 1
+    packed = rng.integers(0, 2**64, size=(1 << (t + s)) // 64, dtype=np.uint64)
+    point = rng.integers(0, 2**64, size=(s, 2), dtype=np.uint64)
+    count = sum(n + 1 for n in range(s, t + s))
+    tr = G.Transcript.seeded(args.seed + 3000)
+    ch = np.array([[(x := tr.squeeze()) & (2**64 - 1), x >> 64] for _ in range(count)], dtype=np.uint64)
+    for _ in range(2):
+        _, wit = G.grand_product_bittable(images, packed, t, s)
+        G.gpgkr_prove_with_challenges(point, ch, wit)
+    G.synchronize()
+    G.profile_enable(True)
+    G.profile_only(None)
+    builds, proves, reps = [], [], 5
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, wit = G.grand_product_bittable(images, packed, t, s)
+        t1 = time.perf_counter()
+        G.gpgkr_prove_with_challenges(point, ch, wit)
+        builds.append(t1 - t0)
+        proves.append(time.perf_counter() - t1)
+    ex, ly = G.profile_read("bittable_leaves"), G.profile_read("gkr_layers")
+    G.profile_enable(False)
+    leaves = 1 << (t + s)
+    # The lookup-table form: the table's bits, one 16-entry table per four
+    # images, layer 3 and up in full, and round 1's bound children (as many
+    # as layer 1 has nodes), lib.rs:44-78, 813-894.
+    lut_bytes = leaves // 8 + 16 * 16 * ((1 << t) // 4) + 16 * (leaves // 4) + 16 * (leaves // 2)
+    out = {
+        "log_rows": t, "log_trees": s, "leaves": leaves,
+        "expand_kernel_ms": ex["ms"] / max(ex["launches"], 1),
+        "expand_gbs": ex["bytes"] / max(ex["ms"], 1e-9) / 1e6,
+        "tree_layers_ms": ly["ms"] / max(ly["launches"], 1),
+        "build_call_ms_p50": float(np.percentile(np.array(builds) * 1e3, 50)),
+        "queued_proof_ms_p50": float(np.percentile(np.array(proves) * 1e3, 50)),
+        "dense_tree_bytes": 2 * 16 * leaves,
+        "lut_form_bytes": lut_bytes,
+    }
+    log(f"bench: BitTable t={t} s={s}: expansion {out['expand_kernel_ms'] * 1e3:.1f} us, tree "
+        f"{out['tree_layers_ms'] * 1e3:.1f} us, proof {out['queued_proof_ms_p50']:.2f} ms")
+    return out
 
 
 def bench_inner(G, args) -> int:
@@ -987,6 +1046,155 @@ def bench_bind(G, args) -> int:
         "algorithmic_bytes": algo,
         "parity_projection": {"ms_per_step": pwall / args.steps * 1e3, "p50_ms": float(np.percentile(ptimes, 50)),
                               "nonzero_values": live},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+# Published MI355X peaks: dense int8 matrix ops, and the vector ALU (256 CUs
+# x 64 lanes at 2.4 GHz) that the table products run on.
+INT8_PEAK_TOPS = 5000.0
+VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9
+
+
+def bench_products(G, args) -> int:
+    """GF(2^128) products per second of each product primitive over 2^log_n
+    device-resident elements (one launch each, timed with HIP events on the
+    launch's stream): the comb, the lane-table product and its two-operand
+    form, the four-per-wave row product, and the two ways of multiplying by
+    a fixed element, its 4-bit constant table on the vector ALU and its F2
+    matrix on the int8 matrix cores (the fold-by-challenge product).  The
+    constant-table product is bound by HBM at 32 B per product; the MFMA
+    one by the matrix cores (128 x 128 x 2 = 32768 int8 ops per product)
+    and by the VALU work that lays the bits out as bytes and gathers the
+    parities back."""
+    G.init(0)
+    n = 1 << args.log_n
+    rng = np.random.default_rng(args.seed)
+    a = G.DeviceTable(rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64))
+    b = G.DeviceTable(rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64))
+    out = G.DeviceTable(length=n)
+    modes = [("mul_comb", 0), ("mul_lane", G.MUL_LANE), ("mul_lane2", G.MUL_LANE2), ("mul_rows", G.MUL_ROWS),
+             ("mul_const", G.MUL_CONST), ("mul_mfma", G.MUL_MFMA)]
+    res = {}
+    for name, mode in modes:
+        G.profile_enable(False)
+        for _ in range(args.warmup):
+            G.mul_batch_device(a, b, out, mode)
+        G.profile_enable(True)
+        G.profile_only(name)
+        for _ in range(args.steps):
+            G.mul_batch_device(a, b, out, mode)
+        st = G.profile_read(name)
+        ms = st["ms"] / max(st["launches"], 1)
+        rate = n / (ms * 1e-3)
+        res[name] = {"ms_per_launch": ms, "products_per_s": rate,
+                     "hbm_gbs": st["bytes"] / max(st["launches"], 1) / (ms * 1e-3) / 1e9,
+                     "valu_lane_ops_per_product_at_peak": VALU_LANE_OPS_PER_S / rate}
+        log(f"bench: {name}: {ms:.3f} ms per 2^{args.log_n} products, {rate / 1e9:.1f} G/s")
+    G.profile_enable(False)
+    const, mfma = res["mul_const"], res["mul_mfma"]
+    mfma_tops = mfma["products_per_s"] * 32768 / 1e12
+    line = {
+        "metric": "GF(2^128) multiplications by a fixed element per second, 1 MI355X",
+        "value": const["products_per_s"],
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": const["ms_per_launch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (uniform elements from the seed)",
+        "config": {"workload": f"a[i] * c over 2^{args.log_n} device-resident elements, one launch per step"},
+        "roofline": {"bound": "hbm", "kernel": "k_mul_const", "achieved": const["hbm_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": const["hbm_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None},
+        "mfma_roofline": {"bound": "mfma", "kernel": "k_mul_mfma", "achieved": mfma_tops,
+                          "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)", "frac": mfma_tops / INT8_PEAK_TOPS},
+        "mfma_vs_const_table": mfma["products_per_s"] / const["products_per_s"],
+        "primitives": res,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_mt(G, args) -> int:
+    """M^T over the SHA-256 circuits: apply (r M, one challenge per row of M)
+    and transpose_inner (the row weight split as a low and a high eq table),
+    with device-resident inputs and outputs, each kernel timed with HIP events
+    on its stream.  Circuits: the 2 KiB block-aligned message that the
+    reference's benchmark uses (crates/circuit/benches/sha256_matrix_products.rs)
+    and the 256-block compression chain of the serving workload.  The oracle
+    runs apply on the 2 KiB circuit on one host core as the CPU baseline."""
+    from bitz_amd import sha256 as S
+    from oracle import oracle as O
+    G.init(0)
+    rng = np.random.default_rng(args.seed)
+    cases = {}
+    cpu = None
+    for name, kind, blocks in (("sha256_2kib", S.BLOCK_ALIGNED, 32), ("chain_256", S.COMPRESSION_CHAIN, 256)):
+        c = S.Sha256Circuit(kind, blocks)
+        offsets, rows = c.m_transpose()
+        mt = G.MaterializedMTranspose(c.columns, offsets, rows)
+        nnz, cols = len(rows), len(offsets) - 1
+        ch = rng.integers(0, 2**64, size=(c.columns, 2), dtype=np.uint64)
+        dch, dout = G.DeviceTable(ch), G.DeviceTable(length=cols)
+        vars_ = max(c.columns - 1, 0).bit_length()
+        lo = (vars_ + 1) // 2
+        drw = G.DeviceTable(rng.integers(0, 2**64, size=(1 << lo, 2), dtype=np.uint64))
+        dcw = G.DeviceTable(rng.integers(0, 2**64, size=(1 << (vars_ - lo), 2), dtype=np.uint64))
+        res = {"m_rows": c.columns, "m_columns": cols, "nonzeros": nnz}
+        for kern, call in (("mt_apply", lambda: mt.apply_device(dch, dout)),
+                           ("mt_transpose_inner", lambda: mt.transpose_inner_device(drw, dcw, dout))):
+            G.profile_enable(False)
+            for _ in range(args.warmup):
+                call()
+            G.profile_enable(True)
+            G.profile_only(kern)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                call()
+            wall = time.perf_counter() - t0
+            st = G.profile_read(kern)
+            ms = st["ms"] / max(st["launches"], 1)
+            gbs = st["bytes"] / max(st["launches"], 1) / (ms * 1e-3) / 1e9
+            res[kern] = {"kernel_ms": ms, "call_ms": wall / args.steps * 1e3, "algorithmic_gbs": gbs,
+                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": st["bytes"] / max(st["launches"], 1)}
+            log(f"bench: {name} {kern}: {ms * 1e3:.1f} us per launch, {gbs:.0f} GB/s algorithmic")
+        G.profile_enable(False)
+        if name == "sha256_2kib" and not args.no_cpu:
+            runs = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                O.mt_apply(offsets, rows, ch)
+                runs.append(time.perf_counter() - t0)
+            cpu = {"value": 1.0 / float(np.median(runs)), "unit": "apply/s", "cores": 1, "kind": "port",
+                   "sample": f"oracle mt_apply on the 2 KiB circuit ({cols} columns, {nnz} nonzeros), "
+                             f"1 thread, median of 5: {float(np.median(runs)) * 1e3:.2f} ms"}
+        cases[name] = res
+    head = cases["sha256_2kib"]["mt_apply"]
+    line = {
+        "metric": "M^T apply calls/sec (SHA-256 2 KiB circuit), 1 MI355X",
+        "value": 1e3 / head["kernel_ms"],
+        "unit": "apply/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["kernel_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "SHA-256 circuits from the producer, uniform challenges and weights from the seed",
+        "config": {"workload": "MaterializedMTranspose apply and transpose_inner on the SHA-256 2 KiB "
+                               "block-aligned circuit and the 256-block compression chain"},
+        "roofline": {"bound": "hbm", "kernel": "k_mt_apply", "achieved": head["algorithmic_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": None},
+        "cases": cases,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -523,17 +523,29 @@ class Transcript:
 
 
 # ------------------------------------------------------------------- poly ---
-MUL_LANE, MUL_LANE2, MUL_CONST, MUL_ROWS = 0x100, 0x200, 0x400, 0x800  # BITZ_MUL_* (product primitive)
+MUL_LANE, MUL_LANE2, MUL_CONST, MUL_ROWS, MUL_MFMA = 0x100, 0x200, 0x400, 0x800, 0x1000  # BITZ_MUL_*
 
 
 def mul_batch(a, b, mode: int = 0) -> np.ndarray:
     """a[i] * b[i] on the device; mode selects the product primitive
-    (MUL_CONST: a[i] * b[0] through b[0]'s constant table)."""
+    (MUL_CONST: a[i] * b[0] through b[0]'s constant table; MUL_MFMA: the
+    same product on the int8 matrix cores)."""
     init()
     a, b = elems(a), elems(b)
     out = np.zeros_like(a)
     _check(lib().bitz_mul_batch(_p(a), _p(b), _p(out), len(a), mode))
     return out
+
+
+def mul_batch_device(a: "DeviceTable", b: "DeviceTable", out: "DeviceTable", mode: int = 0) -> None:
+    """mul_batch over tables resident on the device (out[i] = a[i] * b[i], or
+    a[i] * b[0] for MUL_CONST and MUL_MFMA); with profiling on, the launch is
+    timed as mul_comb, mul_lane, mul_lane2, mul_const, mul_rows or mul_mfma."""
+    init()
+    n = min(a.len, out.len)
+    if mode not in (MUL_CONST, MUL_MFMA) and b.len < n:
+        raise BitzError(-7, "b is shorter than a")
+    _check(lib().bitz_mul_batch(C.c_void_p(a.ptr), C.c_void_p(b.ptr), C.c_void_p(out.ptr), n, mode | DEVICE))
 
 
 def eq_table(r) -> np.ndarray:
@@ -1225,6 +1237,21 @@ class MaterializedMTranspose:
         out = np.zeros((self.columns, 2), dtype=np.uint64)
         _check(lib().bitz_mt_transpose_inner(self.h, _p(rw), len(rw), _p(cw), len(cw), _p(out), 0))
         return out
+
+    def apply_device(self, challenges: "DeviceTable", out: "DeviceTable") -> None:
+        """apply with the challenges and the output resident on the device."""
+        if challenges.len != self.row_count or out.len < self.columns:
+            raise BitzError(-7, "one challenge per row of M and one output per column")
+        _check(lib().bitz_mt_apply(self.h, C.c_void_p(challenges.ptr), challenges.len, C.c_void_p(out.ptr), DEVICE))
+
+    def transpose_inner_device(self, row_weights: "DeviceTable", column_weights: "DeviceTable",
+                               out: "DeviceTable") -> None:
+        """transpose_inner with the weights and the output on the device."""
+        if out.len < self.columns:
+            raise BitzError(-7, "one output per column of M")
+        _check(lib().bitz_mt_transpose_inner(self.h, C.c_void_p(row_weights.ptr), row_weights.len,
+                                             C.c_void_p(column_weights.ptr), column_weights.len,
+                                             C.c_void_p(out.ptr), DEVICE))
 
     def __del__(self):
         try:

@@ -681,7 +681,16 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
     TRY(upload(da, a, n, flags, so.s));
     TRY(upload(db, b, n, flags, so.s));
     TRY(dout.alloc(n * sizeof(f128)));
-    const uint32_t mode = flags & (BITZ_MUL_LANE | BITZ_MUL_LANE2 | BITZ_MUL_CONST | BITZ_MUL_ROWS);
+    const uint32_t mode = flags & (BITZ_MUL_LANE | BITZ_MUL_LANE2 | BITZ_MUL_CONST | BITZ_MUL_ROWS | BITZ_MUL_MFMA);
+    // One event pair around the launch (with profiling on): "mul_comb",
+    // "mul_lane", "mul_lane2", "mul_const", "mul_rows" or "mul_mfma", with
+    // 32 B read and 16 B written per product as its algorithmic bytes.
+    static const char* const kNames[] = {"mul_comb", "mul_lane", "mul_lane2", "mul_const", "mul_rows", "mul_mfma"};
+    const int which = mode == 0 ? 0 : mode == BITZ_MUL_LANE ? 1 : mode == BITZ_MUL_LANE2 ? 2
+                    : mode == BITZ_MUL_CONST ? 3 : mode == BITZ_MUL_ROWS ? 4 : mode == BITZ_MUL_MFMA ? 5 : -1;
+    if (which < 0) return fail(BITZ_ERR_ARG, "one product mode at a time");
+    Timed tm;
+    tm.begin(so.s, kNames[which], (which == 3 || which == 5 ? 32.0 : 48.0) * (double)n);
     if (mode == 0) {
         hipLaunchKernelGGL(bitz::k_mul, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
     } else if (mode == BITZ_MUL_LANE || mode == BITZ_MUL_LANE2) {
@@ -698,10 +707,13 @@ extern "C" int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f
     } else if (mode == BITZ_MUL_ROWS) {
         hipLaunchKernelGGL(bitz::k_mul_rows, dim3(grid_for(16 * n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
     } else {
-        return fail(BITZ_ERR_ARG, "one product mode at a time");
+        hipLaunchKernelGGL(bitz::k_mul_mfma, dim3(grid_for(n)), dim3(kBlock), 0, so.s, da.f(), db.f(), dout.f(), n);
     }
+    tm.end(so.s);
     TRY(check_launch());
-    return download(out, dout.f(), n, flags, so.s);
+    TRY(download(out, dout.f(), n, flags, so.s));  // synchronises the stream
+    tm.collect();
+    return BITZ_OK;
 }
 
 extern "C" int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags) {
@@ -4894,8 +4906,10 @@ extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz
 
 // ================================================================ M^T =====
 struct bitz_mt {
-    size_t row_count = 0, cols = 0;
+    size_t row_count = 0, cols = 0, nnz = 0;
     Dev offsets, rows;
+    Dev long_cols;          // columns of more than kMtLongColumn entries
+    uint32_t n_long = 0;
     StreamOwner so;  // last: destroyed (and synchronised) before the buffers
 };
 
@@ -4917,10 +4931,18 @@ extern "C" int32_t bitz_mt_prepare(bitz_mt** out, size_t row_count, size_t colum
     if ((rc = h->so.create())) return cleanup(rc);
     h->row_count = row_count;
     h->cols = columns;
+    h->nnz = nnz;
     if ((rc = h->offsets.alloc((columns + 1) * 4))) return cleanup(rc);
     if ((rc = h->rows.alloc(std::max<size_t>(nnz, 1) * 4))) return cleanup(rc);
     if (hipMemcpy(h->offsets.p, column_offsets, (columns + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
         (nnz && hipMemcpy(h->rows.p, row_indices, nnz * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return cleanup(fail(BITZ_ERR_DEVICE, "upload failed"));
+    std::vector<uint32_t> longs;
+    for (size_t c = 0; c < columns; ++c)
+        if (column_offsets[c + 1] - column_offsets[c] > bitz::kMtLongColumn) longs.push_back((uint32_t)c);
+    h->n_long = (uint32_t)longs.size();
+    if ((rc = h->long_cols.alloc(std::max<size_t>(longs.size(), 1) * 4))) return cleanup(rc);
+    if (!longs.empty() && hipMemcpy(h->long_cols.p, longs.data(), longs.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(BITZ_ERR_DEVICE, "upload failed"));
     *out = h;
     return BITZ_OK;
@@ -4942,11 +4964,23 @@ extern "C" int32_t bitz_mt_apply(bitz_mt* h, const bitz_f128* challenges, size_t
         TRY(dout.alloc(h->cols * sizeof(f128)));
         target = dout.f();
     }
+    // Timed as "mt_apply": the CSC (4 B per column and nonzero), every
+    // challenge read once and the output written once.
+    Timed tm;
+    tm.begin(s, "mt_apply", 4.0 * (double)(h->cols + 1 + h->nnz) + 16.0 * (double)(n + h->cols));
     hipLaunchKernelGGL(bitz::k_mt_apply, dim3(grid_for(h->cols)), dim3(kBlock), 0, s, h->offsets.u32(),
                        h->rows.u32(), dc.f(), target, h->cols);
+    if (h->n_long)
+        hipLaunchKernelGGL(bitz::k_mt_apply_long, dim3(h->n_long), dim3(kBlock), 0, s, h->offsets.u32(),
+                           h->rows.u32(), dc.f(), h->long_cols.u32(), target);
+    tm.end(s);
     TRY(check_launch());
-    return (flags & BITZ_DEVICE) ? (hipStreamSynchronize(s) == hipSuccess ? BITZ_OK : BITZ_ERR_DEVICE)
-                                 : download(out, target, h->cols, 0, s);
+    if (flags & BITZ_DEVICE)
+        HIP_TRY(hipStreamSynchronize(s));
+    else
+        TRY(download(out, target, h->cols, 0, s));
+    tm.collect();
+    return BITZ_OK;
 }
 
 extern "C" int32_t bitz_mt_transpose_inner(bitz_mt* h, const bitz_f128* row_w, size_t n_row,
@@ -4969,11 +5003,32 @@ extern "C" int32_t bitz_mt_transpose_inner(bitz_mt* h, const bitz_f128* row_w, s
         TRY(dout.alloc(h->cols * sizeof(f128)));
         target = dout.f();
     }
-    hipLaunchKernelGGL(bitz::k_mt_transpose_inner, dim3(grid_for(h->cols)), dim3(kBlock), 0, s,
+    // Timed as "mt_transpose_inner": the CSC, both weight tables read once
+    // and the output written once.
+    Timed tm;
+    tm.begin(s, "mt_transpose_inner",
+             4.0 * (double)(h->cols + 1 + h->nnz) + 16.0 * (double)(n_row + n_col + h->cols));
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_mt_transpose_inner),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_mt_inner_long),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem256);
+        attr = true;
+    }
+    hipLaunchKernelGGL(bitz::k_mt_transpose_inner, dim3(grid_for(h->cols)), dim3(kBlock), bitz::kLaneSmem256, s,
                        h->offsets.u32(), h->rows.u32(), dr.f(), log_rows, dc.f(), target, h->cols);
+    if (h->n_long)
+        hipLaunchKernelGGL(bitz::k_mt_inner_long, dim3(h->n_long), dim3(kBlock), bitz::kLaneSmem256, s,
+                           h->offsets.u32(), h->rows.u32(), dr.f(), log_rows, dc.f(), h->long_cols.u32(), target);
+    tm.end(s);
     TRY(check_launch());
-    return (flags & BITZ_DEVICE) ? (hipStreamSynchronize(s) == hipSuccess ? BITZ_OK : BITZ_ERR_DEVICE)
-                                 : download(out, target, h->cols, 0, s);
+    if (flags & BITZ_DEVICE)
+        HIP_TRY(hipStreamSynchronize(s));
+    else
+        TRY(download(out, target, h->cols, 0, s));
+    tm.collect();
+    return BITZ_OK;
 }
 
 extern "C" void bitz_mt_free(bitz_mt* h) { delete h; }
@@ -5028,6 +5083,9 @@ static int32_t gkr_build(bitz_gkr** out, uint32_t log_leaves, uint32_t log_group
     const int j_small = std::max(0, (int)log_leaves - bitz::kGkrSmallTreeLog);
     const bool fused_copy = src && depth > 0 && j_small > 0;
     if (!fused_copy && (rc = fill(h, s))) return cleanup(rc);
+    // Timed as "gkr_layers": every layer read once, the next written once.
+    Timed tl;
+    tl.begin(s, "gkr_layers", 24.0 * (double)(((size_t)1 << log_leaves) - ((size_t)1 << log_groups)));
     for (int j = 0; j < std::min(depth, j_small); ++j) {
         const size_t half = (size_t)1 << (log_leaves - j - 1);
         const bool copy = fused_copy && j == 0;
@@ -5038,9 +5096,11 @@ static int32_t gkr_build(bitz_gkr** out, uint32_t log_leaves, uint32_t log_group
     if (depth > j_small)
         hipLaunchKernelGGL(bitz::k_gkr_layers_small, dim3(1), dim3(bitz::kStageBlock), kSmallTreeSmem, s,
                            h->store.f() + h->off[j_small], (int)log_leaves - j_small, depth - j_small);
+    tl.end(s);
     if ((rc = check_launch())) return cleanup(rc);
     if ((rc = download(output, h->store.f() + h->off.back(), (size_t)1 << log_groups, flags, s)))
         return cleanup(rc);
+    tl.collect();
     if ((rc = h->host.alloc(64))) return cleanup(rc);
     *out = h;
     return BITZ_OK;
@@ -5071,6 +5131,8 @@ extern "C" int32_t bitz_gkr_build_bittable(bitz_gkr** out, const bitz_f128* imag
     const bool dev = (flags & BITZ_DEVICE) != 0;
     const size_t rows = (size_t)1 << log_rows, words = std::max<size_t>(((size_t)1 << (log_rows + log_columns)) / 64, 1);
     Dev dimg, dbits;
+    // Timed as "bittable_leaves": the bits and images read, 16 B per leaf written.
+    Timed te;
     auto fill = [&](bitz_gkr* h, hipStream_t s) -> int32_t {
         const f128* img = reinterpret_cast<const f128*>(images);
         const uint64_t* bits = packed;
@@ -5081,13 +5143,16 @@ extern "C" int32_t bitz_gkr_build_bittable(bitz_gkr** out, const bitz_f128* imag
             img = dimg.f();
             bits = static_cast<const uint64_t*>(dbits.p);
         }
+        te.begin(s, "bittable_leaves", 16.0 * (double)(rows + (rows << log_columns)) + 8.0 * (double)words);
         hipLaunchKernelGGL(bitz::k_bittable_leaves, dim3(grid_for(rows << log_columns)), dim3(kBlock), 0, s, bits,
                            img, (int)log_rows, (int)log_columns, h->store.f());
+        te.end(s);
         return check_launch();
     };
     const int32_t rc = gkr_build(out, log_rows + log_columns, log_columns, nullptr, fill, output, flags);
     // the staged table and images must outlive the expansion kernel
     if (rc == BITZ_OK && !dev) HIP_TRY(hipStreamSynchronize((*out)->so.s));
+    if (rc == BITZ_OK) te.collect();  // gkr_build synchronised the stream
     return rc;
 }
 

@@ -210,6 +210,96 @@ __global__ __launch_bounds__(kBlock) void k_mul_const(const f128* a, const f128*
         g_store(out + i, g_mul_table4(tab, g_load(a + i)));
 }
 
+// a[i] * b[0] on the int8 matrix cores (bitz_mul_batch's BITZ_MUL_MFMA, a
+// measured alternative to k_mul_const): multiplying by c = b[0] is the
+// 128x128 F2 matrix whose column j is c X^j, so 16 elements (the rows of A,
+// one 0/1 byte per bit) times that matrix (B) on v_mfma_i32_16x16x64_i8
+// counts, per output bit, the set terms; the count's parity is the bit.
+// Tile t of the output holds bits 16t..16t+15, K-step s input bits
+// 64s..64s+63: 8 tiles x 2 steps = 16 MFMAs per 16 elements.  Lane l feeds
+// row or column l % 16 and 16 bytes of the K range to both operands; A and B
+// take the same per-lane K order, so the products line up whatever order
+// the hardware sums in.  Output row 4(l / 16) + i, column l % 16 is in
+// register i of lane l; one ballot per register gathers a tile's bits of
+// four rows, and lane m < 16 assembles element m from the ballots.
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+
+// Bit b of v (b < 4) -> byte b, as 0 or 1: v (v0 + v1 2^8 + ...) through
+// the carry-free product v * (1 + 2^7 + 2^14 + 2^21) masked to bytes.
+__device__ __forceinline__ uint32_t nibble_to_bytes(uint32_t v) { return __umul24(v, 0x204081u) & 0x01010101u; }
+
+__global__ __launch_bounds__(kBlock) void k_mul_mfma(const f128* a, const f128* b, f128* out, size_t n) {
+    __shared__ uint4 basis[128];
+    const g4 c = g_load(b);
+    for (int j = threadIdx.x; j < 128; j += blockDim.x) {
+        const g4 x = g_mul_xpow(c, j);
+        basis[j] = make_uint4(x.w[0], x.w[1], x.w[2], x.w[3]);
+    }
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63), q = lane >> 4, col = lane & 15;
+    // B for tile t, step s: byte j of this lane = bit 16t + col of c X^(64s + 16q + j).
+    v4i32 bm[8][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        uint32_t words[16][4];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint4 e = basis[64 * s + 16 * q + j];
+            words[j][0] = e.x; words[j][1] = e.y; words[j][2] = e.z; words[j][3] = e.w;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int bit = 16 * t + col, w = bit >> 5, sh = bit & 31;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) v |= ((words[4 * d + jj][w] >> sh) & 1u) << (8 * jj);
+                bm[t][s][d] = (int)v;
+            }
+        }
+    }
+    const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const size_t waves = ((size_t)gridDim.x * blockDim.x) >> 6;
+    const size_t groups = (n + 15) / 16;
+    const int qq = col >> 2, ii = col & 3;  // the output row this lane writes (lanes < 16)
+    for (size_t g = wave; g < groups; g += waves) {
+        const size_t e = 16 * g + col;
+        const g4 x = e < n ? g_load_hbm(a + e) : g_zero();
+        v4i32 am[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t h = (x.w[2 * s + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) am[s][d] = (int)nibble_to_bytes((h >> (4 * d)) & 15u);
+        }
+        uint64_t bal[8][4];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            v4i32 acc = {0, 0, 0, 0};
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[0], bm[t][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(am[1], bm[t][1], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bal[t][i] = __builtin_amdgcn_ballot_w64((acc[i] & 1) != 0);
+        }
+        if (lane < 16 && e < n) {
+            g4 y;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint64_t lo = bal[2 * w][0], hi = bal[2 * w + 1][0];
+                lo = ii == 1 ? bal[2 * w][1] : lo;
+                hi = ii == 1 ? bal[2 * w + 1][1] : hi;
+                lo = ii == 2 ? bal[2 * w][2] : lo;
+                hi = ii == 2 ? bal[2 * w + 1][2] : hi;
+                lo = ii == 3 ? bal[2 * w][3] : lo;
+                hi = ii == 3 ? bal[2 * w + 1][3] : hi;
+                y.w[w] = (uint32_t)((lo >> (16 * qq)) & 0xFFFFu) | ((uint32_t)((hi >> (16 * qq)) & 0xFFFFu) << 16);
+            }
+            g_store_hbm(out + e, y);
+        }
+    }
+}
+
 // In-place level doubling of a small eq table by one workgroup
 // (crates/poly/src/eq.rs:63-99).
 // Both halves of a split eq table in one launch: workgroup 0 builds eq(r_<kl)
@@ -474,31 +564,63 @@ __global__ __launch_bounds__(kBlock) void k_row_scales(const f128* eq_rows, f128
 
 // r * M over GF(2^128): out[col] = XOR of challenges[row] over the column
 // (MaterializedMTranspose::apply, matrix_transpose.rs:327-375).
+// M^T (crates/circuit/src/matrix_transpose.rs:327-441), one thread per
+// column of M; a column of more than kMtLongColumn entries (the constant
+// column, read by hundreds of rows) is skipped here and summed by a whole
+// workgroup in the _long kernels, so one column does not hold up the grid.
 __global__ __launch_bounds__(kBlock) void k_mt_apply(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
                            f128* out, size_t cols) {
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < cols;
          c += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t lo = offsets[c], hi = offsets[c + 1];
+        if (hi - lo > kMtLongColumn) continue;
         g4 s = g_zero();
-        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) s = g_add(s, g_load(ch + rows[k]));
-        g_store(out + c, s);
+        for (uint32_t k = lo; k < hi; ++k) s = g_add(s, g_load_hbm(ch + rows[k]));
+        g_store_hbm(out + c, s);
     }
 }
-
-// transpose_inner (matrix_transpose.rs:377-441): one unreduced product per
-// nonzero, one reduction per column.
+__global__ __launch_bounds__(kBlock) void k_mt_apply_long(const uint32_t* offsets, const uint32_t* rows,
+                                                          const f128* ch, const uint32_t* long_cols, f128* out) {
+    const uint32_t c = long_cols[blockIdx.x];
+    g4 v[1] = {g_zero()};
+    for (uint32_t k = offsets[c] + threadIdx.x; k < offsets[c + 1]; k += kBlock)
+        v[0] = g_add(v[0], g_load_hbm(ch + rows[k]));
+    block_xor_store<1>(v, reinterpret_cast<uint32_t*>(out + c));
+}
+// transpose_inner: per entry row_w[i & mask] * col_w[i >> log_rows] through
+// this lane's 3-bit table (kLaneSmem256 of dynamic LDS), XORed per column.
 __global__ __launch_bounds__(kBlock) void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* rows,
                                      const f128* row_w, int log_rows, const f128* col_w,
                                      f128* out, size_t cols) {
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
     const uint32_t mask = (1u << log_rows) - 1;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < cols;
          c += (size_t)gridDim.x * blockDim.x) {
-        g8 s = w_zero();
-        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) {
+        const uint32_t lo = offsets[c], hi = offsets[c + 1];
+        if (hi - lo > kMtLongColumn) continue;
+        g4 s = g_zero();
+        for (uint32_t k = lo; k < hi; ++k) {
             const uint32_t idx = rows[k];
-            clmul_acc(s, g_load(row_w + (idx & mask)), g_load(col_w + (idx >> log_rows)));
+            s = g_add(s, g_mul_lane(g_load_hbm(row_w + (idx & mask)), g_load_hbm(col_w + (idx >> log_rows)), col));
         }
-        g_store(out + c, g_reduce(s));
+        g_store_hbm(out + c, s);
     }
+}
+__global__ __launch_bounds__(kBlock) void k_mt_inner_long(const uint32_t* offsets, const uint32_t* rows,
+                                                          const f128* row_w, int log_rows, const f128* col_w,
+                                                          const uint32_t* long_cols, f128* out) {
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    const uint32_t mask = (1u << log_rows) - 1, c = long_cols[blockIdx.x];
+    g4 v[1] = {g_zero()};
+    for (uint32_t k = offsets[c] + threadIdx.x; k < offsets[c + 1]; k += kBlock) {
+        const uint32_t idx = rows[k];
+        v[0] = g_add(v[0], g_mul_lane(g_load_hbm(row_w + (idx & mask)), g_load_hbm(col_w + (idx >> log_rows)), col));
+    }
+    block_xor_store<1>(v, reinterpret_cast<uint32_t*>(out + c));
 }
 
 // 4-bit constant tables, one workgroup per constant, into global memory;

@@ -15,6 +15,7 @@ constexpr int kBlock = 256;
 __global__ void k_mul(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, int two);
 __global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
+__global__ void k_mul_mfma(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_eq_small(const f128* r, int k, f128* table);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
@@ -49,11 +50,18 @@ __global__ void k_row_scales(const f128* eq_rows, f128 rho, f128 rho2, f128* row
 __global__ void k_r1cs_rows(const uint32_t* row_starts, const uint32_t* cols, const f128* vals, const f128* h,
                             size_t rows, f128* out, size_t padded);
 __global__ void k_assignment_bits(const uint64_t* words, size_t bit_len, f128* out, size_t padded);
+// M^T: columns longer than this are summed by one workgroup each (_long).
+constexpr uint32_t kMtLongColumn = 64;
 __global__ void k_mt_apply(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
                            f128* out, size_t cols);
+__global__ void k_mt_apply_long(const uint32_t* offsets, const uint32_t* rows, const f128* ch,
+                                const uint32_t* long_cols, f128* out);
+// transpose_inner with lane-table products: kLaneSmem256 of dynamic LDS.
 __global__ void k_mt_transpose_inner(const uint32_t* offsets, const uint32_t* rows,
                                      const f128* row_w, int log_rows, const f128* col_w,
                                      f128* out, size_t cols);
+__global__ void k_mt_inner_long(const uint32_t* offsets, const uint32_t* rows, const f128* row_w, int log_rows,
+                                const f128* col_w, const uint32_t* long_cols, f128* out);
 
 __global__ void k_eq_check(const f128* L, int ell, const f128* H, int m, uint32_t* out, unsigned blocks_l);
 

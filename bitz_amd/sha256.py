@@ -90,6 +90,18 @@ class Sha256Circuit:
             out[name] = (starts[k], cols[k][:n], coefs[k][:n], par[k][:n])
         return out
 
+    def m_transpose(self):
+        """M in MaterializedMTranspose's layout (CSC, crates/circuit/src/
+        matrix_transpose.rs:282-288): per column of M (the constant, then each
+        witness bit), the assignment entries that read it, ascending.
+        Returns (column_offsets, row_indices) as uint32 arrays."""
+        ms, mp = self.export()["m"]
+        rows_of = np.repeat(np.arange(self.columns, dtype=np.uint32), np.diff(ms.astype(np.int64)))
+        order = np.argsort(mp, kind="stable")
+        counts = np.bincount(mp, minlength=self.witness_bits + 1)
+        offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+        return offsets, rows_of[order]
+
     def witness(self, message: bytes, iv=None) -> Sha256Witness:
         """The witness for `message` (64 bytes per block); iv (8 words) only
         for the compression chain (None = the FIPS IV)."""

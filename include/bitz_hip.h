@@ -184,11 +184,16 @@ int32_t bitz_transcript_proof(const bitz_transcript* t, uint8_t* out, size_t cap
  * two-operand form (one table of a[i], the second chain taking a neighbour's
  * b), BITZ_MUL_CONST out[i] = a[i] * b[0] through b[0]'s 4-bit table,
  * BITZ_MUL_ROWS the four-products-per-wave form (one 16-lane row per
- * product, the row's partial products XORed by DPP moves). */
+ * product, the row's partial products XORed by DPP moves), BITZ_MUL_MFMA
+ * out[i] = a[i] * b[0] as b[0]'s 128x128 F2 matrix on the int8 matrix cores
+ * (counts, then parities).  With profiling on, the launch is timed as
+ * "mul_comb", "mul_lane", "mul_lane2", "mul_const", "mul_rows" or
+ * "mul_mfma". */
 #define BITZ_MUL_LANE 0x100u
 #define BITZ_MUL_LANE2 0x200u
 #define BITZ_MUL_CONST 0x400u
 #define BITZ_MUL_ROWS 0x800u
+#define BITZ_MUL_MFMA 0x1000u
 int32_t bitz_mul_batch(const bitz_f128* a, const bitz_f128* b, bitz_f128* out, size_t n, uint32_t flags);
 /* poly::eq_table (crates/poly/src/eq.rs:63-99): out has 2^k entries. */
 int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out, uint32_t flags);

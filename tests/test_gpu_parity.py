@@ -43,13 +43,15 @@ def test_lane_table_products_match_oracle(mode):
     assert np.array_equal(G.mul_batch(A, B, mode), O.mul_batch(A, B))
 
 
+@pytest.mark.parametrize("mode", [G.MUL_CONST, G.MUL_MFMA])
 @pytest.mark.parametrize("c", EDGES + [0xDEADBEEF << 70 | 12345])
-def test_constant_table_product_matches_oracle(c):
+def test_constant_table_product_matches_oracle(c, mode):
     """a[i] * c through c's 4-bit constant table (the perm-addressed lookups of
-    the fold and weight products)."""
+    the fold and weight products), and as c's F2 matrix on the int8 matrix
+    cores; 3009 elements leave a partial group of 16."""
     a = from_ints(EDGES + ints(elements(13, 3000)))
     b = from_ints([c] * len(a))
-    assert np.array_equal(G.mul_batch(a, b, G.MUL_CONST), O.mul_batch(a, b))
+    assert np.array_equal(G.mul_batch(a, b, mode), O.mul_batch(a, b))
 
 
 @pytest.mark.parametrize("k", [0, 1, 5, 12, 13, 20])
@@ -264,6 +266,40 @@ def test_mt_apply_and_transpose_inner_match_oracle():
     mt = G.MaterializedMTranspose(rows, offsets, idx)
     assert np.array_equal(mt.apply(r), O.mt_apply(offsets, idx, r))
     rw, cw = elements_fast(10, 64), elements_fast(11, 128)
+    assert np.array_equal(mt.transpose_inner(rw, cw), O.mt_transpose_inner(offsets, idx, rw, cw))
+
+
+def test_mt_long_columns_match_oracle():
+    """Columns past the one-thread limit (64 entries) go to a workgroup each:
+    lengths at the limit, one over it, the 2 KiB SHA-256 circuit's constant
+    column (300) and a column longer than a workgroup's stride, beside short
+    and empty ones."""
+    rng = np.random.default_rng(5)
+    rows = 20000
+    lens = [64, 65, 300, 1000, 0, 3, 257, 1, 9, 4096] + [int(x) for x in rng.integers(0, 10, size=500)]
+    offsets, idx = [0], []
+    for n in lens:
+        idx += sorted(rng.choice(rows, size=n, replace=False).tolist())
+        offsets.append(len(idx))
+    mt = G.MaterializedMTranspose(rows, offsets, idx)
+    r = elements_fast(21, rows)
+    assert np.array_equal(mt.apply(r), O.mt_apply(offsets, idx, r))
+    rw, cw = elements_fast(22, 128), elements_fast(23, 256)
+    assert np.array_equal(mt.transpose_inner(rw, cw), O.mt_transpose_inner(offsets, idx, rw, cw))
+
+
+def test_mt_on_the_sha256_2kib_circuit_matches_oracle():
+    """M^T of the SHA-256 2 KiB block-aligned circuit (the reference's matrix
+    benchmark, crates/circuit/benches/sha256_matrix_products.rs): 675049 rows,
+    226793 columns, its constant column 300 entries long."""
+    from bitz_amd import sha256 as S
+    c = S.Sha256Circuit(S.BLOCK_ALIGNED, 32)
+    offsets, idx = c.m_transpose()
+    assert c.columns == 675049 and len(offsets) - 1 == 226793 and offsets[1] - offsets[0] == 300
+    mt = G.MaterializedMTranspose(c.columns, offsets, idx)
+    r = elements_fast(24, c.columns)
+    assert np.array_equal(mt.apply(r), O.mt_apply(offsets, idx, r))
+    rw, cw = elements_fast(25, 1 << 10), elements_fast(26, 1 << 10)
     assert np.array_equal(mt.transpose_inner(rw, cw), O.mt_transpose_inner(offsets, idx, rw, cw))
 
 
