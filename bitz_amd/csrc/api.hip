@@ -68,14 +68,17 @@ struct DeviceCache {
         const size_t step = (size_t)1 << (lg - 2);
         return (n + step - 1) & ~(step - 1);
     }
-    // A block of at least n bytes (at most 2 n when it comes from the
-    // cache); *size receives its true size, which put() must be given back.
+    // A block of at least n bytes: one of n's own class when n is above
+    // 64 KiB (a smaller request taking a larger class's block would leave
+    // the next request of that class to a hipMalloc, which stalls every
+    // thread's HIP calls for hundreds of microseconds), at most 2 n below;
+    // *size receives its true size, which put() must be given back.
     void* get(size_t n, size_t* size) {
         n = size_class(n);
         {
             std::lock_guard<std::mutex> lock(mu);
             auto it = free_blocks.lower_bound(n);
-            if (it != free_blocks.end() && it->first <= 2 * n) {
+            if (it != free_blocks.end() && (n > ((size_t)64 << 10) ? it->first == n : it->first <= 2 * n)) {
                 void* p = it->second;
                 *size = it->first;
                 cached_bytes -= it->first;
@@ -269,9 +272,21 @@ int32_t download(bitz_f128* dst, const f128* src, size_t n, uint32_t flags, hipS
 // eq(b, r) over b in {0,1}^k into out (2^k entries); r is a device array.
 // Up to 2^12 entries one workgroup doubles the table level by level; above,
 // two half tables are built that way and multiplied entrywise.
+// k_eq_rows' dynamic LDS (lane tables of 16 waves) is above the default
+// limit: set once, from whichever thread gets here first.
+static void eq_rows_attr() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&bitz::k_eq_rows),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bitz::kLaneSmem1024);
+    });
+}
+
 int32_t eq_build(const f128* r_dev, int k, f128* out, Dev& scratch, hipStream_t s) {
-    if (k <= 12) {
-        hipLaunchKernelGGL(bitz::k_eq_small, dim3(1), dim3(kBlock), 0, s, r_dev, k, out);
+    if (k <= bitz::kEqRowsMaxVars) {
+        eq_rows_attr();
+        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(1), dim3(1024), bitz::kLaneSmem1024, s, r_dev, k, out,
+                           static_cast<const f128*>(nullptr));
         return check_launch();
     }
     const int kl = k / 2, kh = k - kl;
@@ -3320,16 +3335,29 @@ static int32_t bind_enqueue(bitz_matrices* h, const f128* row_point, bool point_
     f128* eq_rows = own ? own->eq_rows.f() : h->eq_rows.f();
     f128* row_scale = own ? own->row_scale.f() : h->row_scale.f();
     f128* prod = own ? own->prod.f() : h->prod.f();
-    if (h->row_vars)
+    // A caller with its own scratch and the point on the device (the PIOP
+    // engine: the outer sumcheck's challenges, which outlive the bind) is
+    // read in place; otherwise the point is copied first.
+    const f128* pt = point;
+    if (own && point_on_device)
+        pt = row_point;
+    else if (h->row_vars)
         HIP_TRY(hipMemcpyAsync(point, row_point, (size_t)h->row_vars * sizeof(f128),
                                point_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-    TRY(eq_build(point, h->row_vars, eq_rows, own ? own->eq_scratch : h->scratch, s));
     // batching scales 1, rho, rho^2 (matrix.rs:533-545), by value or (rho_dev:
-    // rho, rho^2) from the device
-    if (rho_dev)
+    // rho, rho^2) from the device; small row domains write the scales with
+    // the eq table in one launch
+    if (rho_dev && h->row_vars <= bitz::kEqRowsMaxVars) {
+        eq_rows_attr();
+        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(1), dim3(1024), bitz::kLaneSmem1024, s, pt, h->row_vars, row_scale,
+                           rho_dev);
+    } else {
+        TRY(eq_build(pt, h->row_vars, eq_rows, own ? own->eq_scratch : h->scratch, s));
+    }
+    if (rho_dev && h->row_vars > bitz::kEqRowsMaxVars)
         hipLaunchKernelGGL(bitz::k_row_scales_dev, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, eq_rows,
                            rho_dev, row_scale, h->rows_padded);
-    else
+    else if (!rho_dev)
         hipLaunchKernelGGL(bitz::k_row_scales, dim3(grid_for(h->rows_padded)), dim3(kBlock), 0, s, eq_rows, rho,
                            hostf::mul(rho, rho), row_scale, h->rows_padded);
     // every nonzero's product (a converged pass), then each column's XOR over its run
@@ -3783,7 +3811,7 @@ struct bitz_piop_engine {
     std::string worker_err;
     // HIP events around the step's batched inner fold and around the whole
     // step's device work (when profiling), read once the step has synchronised.
-    Timed t_inner_fold[kEnginePositions], t_step, t_bits[kEnginePositions][bitz::kInnerBitsMaxRound + 1];
+    Timed t_inner_fold[kEnginePositions], t_step, t_bits[kEnginePositions];
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -4017,26 +4045,26 @@ static std::vector<f128> job_inner_point(const PiopJob* j, int K) {
 // The kernels of one position of a step: every round of one kind across the
 // proofs at that position in one launch.
 namespace {
-constexpr int kBitRounds = bitz::kInnerBitsMaxRound + 1;
 struct EnginePos {
     std::vector<bitz::RoundDesc> o_r0, o_fold;
     std::vector<uint32_t> o_r0_map, o_fold_map;
     std::vector<bitz::InnerRoundDesc> i_r0, i_fold;
     std::vector<uint32_t> i_r0_map, i_fold_map;
-    std::vector<bitz::InnerBitsDesc> i_bits[kBitRounds];
-    std::vector<uint32_t> i_bits_map[kBitRounds];
+    std::vector<bitz::InnerBitsDesc> i_bits;  // every bit round K of the position, one launch
+    std::vector<uint32_t> i_bits_map;
     std::vector<bitz::FinalFoldDesc> finals;
     std::vector<bitz::PiopFsDesc> fs;
     std::vector<size_t> binds;       // proofs (indices into the step's list) that bind here
     std::vector<size_t> host_final;  // host-transcript proofs' final folds (position 0)
     // staging offsets
     size_t or_o = 0, orm_o = 0, of_o = 0, ofm_o = 0, ir_o = 0, irm_o = 0, if_o = 0, ifm_o = 0, fin_o = 0, fs_o = 0;
-    size_t ib_o[kBitRounds] = {}, ibm_o[kBitRounds] = {};
+    size_t ib_o = 0, ibm_o = 0;
     void add_bits(int K, bitz::InnerBitsDesc d) {
         d.blocks = bits_grid(d.pairs);
-        d.block_base = (uint32_t)i_bits_map[K].size();
-        i_bits_map[K].insert(i_bits_map[K].end(), d.blocks, (uint32_t)i_bits[K].size());
-        i_bits[K].push_back(d);
+        d.block_base = (uint32_t)i_bits_map.size();
+        d.k = K;
+        i_bits_map.insert(i_bits_map.end(), d.blocks, (uint32_t)i_bits.size());
+        i_bits.push_back(d);
     }
     void add_fold(bitz::InnerRoundDesc d) {
         d.blocks = inner_grid(d.pairs);
@@ -4427,10 +4455,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         P.irm_o = place(P.i_r0_map.size() * 4);
         P.if_o = place(P.i_fold.size() * sizeof(bitz::InnerRoundDesc));
         P.ifm_o = place(P.i_fold_map.size() * 4);
-        for (int K = 0; K < kBitRounds; ++K) {
-            P.ib_o[K] = place(P.i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
-            P.ibm_o[K] = place(P.i_bits_map[K].size() * 4);
-        }
+        P.ib_o = place(P.i_bits.size() * sizeof(bitz::InnerBitsDesc));
+        P.ibm_o = place(P.i_bits_map.size() * 4);
         P.fin_o = place(P.finals.size() * sizeof(bitz::FinalFoldDesc));
         P.fs_o = place(P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
@@ -4470,10 +4496,8 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         memcpy(pin + P.irm_o, P.i_r0_map.data(), P.i_r0_map.size() * 4);
         memcpy(pin + P.if_o, P.i_fold.data(), P.i_fold.size() * sizeof(bitz::InnerRoundDesc));
         memcpy(pin + P.ifm_o, P.i_fold_map.data(), P.i_fold_map.size() * 4);
-        for (int K = 0; K < kBitRounds; ++K) {
-            memcpy(pin + P.ib_o[K], P.i_bits[K].data(), P.i_bits[K].size() * sizeof(bitz::InnerBitsDesc));
-            memcpy(pin + P.ibm_o[K], P.i_bits_map[K].data(), P.i_bits_map[K].size() * 4);
-        }
+        memcpy(pin + P.ib_o, P.i_bits.data(), P.i_bits.size() * sizeof(bitz::InnerBitsDesc));
+        memcpy(pin + P.ibm_o, P.i_bits_map.data(), P.i_bits_map.size() * 4);
         memcpy(pin + P.fin_o, P.finals.data(), P.finals.size() * sizeof(bitz::FinalFoldDesc));
         memcpy(pin + P.fs_o, P.fs.data(), P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
@@ -4552,30 +4576,21 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
             hipLaunchKernelGGL(bitz::k_inner_r0_batch, dim3((unsigned)P.i_r0_map.size()), dim3(bitz::kInnerBlock),
                                bitz::kInnerGridSmem, s, reinterpret_cast<const bitz::InnerRoundDesc*>(dst + P.ir_o),
                                reinterpret_cast<const uint32_t*>(dst + P.irm_o));
-        for (int K = 0; K < kBitRounds; ++K) {
-            if (P.i_bits[K].empty()) continue;
-            const auto* dd = reinterpret_cast<const bitz::InnerBitsDesc*>(dst + P.ib_o[K]);
-            const auto* mm = reinterpret_cast<const uint32_t*>(dst + P.ibm_o[K]);
-            const dim3 grid((unsigned)P.i_bits_map[K].size()), block(bitz::kInnerBlock);
-            const size_t smem = K ? bitz::kInnerBitsSmem : 0;
-            // Algorithmic bytes (bounds for device-transcript proofs): round 0
-            // reads D (32 B per pair) and the bits; round K >= 1 reads 4
-            // entries of D_{K-1} and writes 2 of D_K per output pair (and 2
-            // of h_K when it writes them).
+        if (!P.i_bits.empty()) {
+            // Every bit round of the position in one launch, each descriptor's
+            // K picking its body.  Algorithmic bytes (bounds for
+            // device-transcript proofs): round 0 reads D (32 B per pair) and
+            // the bits; round K >= 1 reads 4 entries of D_{K-1} and writes 2
+            // of D_K per output pair (and 2 of h_K when it writes them).
             double bits_bytes = 0;
-            for (const auto& d : P.i_bits[K])
-                bits_bytes +=
-                    (K ? 96.0 + (d.wout ? 32.0 : 0.0) : 32.0) * (double)d.pairs + (double)d.pairs * (2 << K) / 8;
-            static const char* const kNames[kBitRounds] = {"inner_bits_batch_0", "inner_bits_batch_1",
-                                                           "inner_bits_batch_2", "inner_bits_batch_3"};
-            e->t_bits[u][K].begin(s, kNames[K], bits_bytes);
-            switch (K) {
-                case 0: hipLaunchKernelGGL(bitz::k_inner_bits_batch<0>, grid, block, smem, s, dd, mm); break;
-                case 1: hipLaunchKernelGGL(bitz::k_inner_bits_batch<1>, grid, block, smem, s, dd, mm); break;
-                case 2: hipLaunchKernelGGL(bitz::k_inner_bits_batch<2>, grid, block, smem, s, dd, mm); break;
-                default: hipLaunchKernelGGL(bitz::k_inner_bits_batch<3>, grid, block, smem, s, dd, mm); break;
-            }
-            e->t_bits[u][K].end(s);
+            for (const auto& d : P.i_bits)
+                bits_bytes += (d.k ? 96.0 + (d.wout ? 32.0 : 0.0) : 32.0) * (double)d.pairs +
+                              (double)d.pairs * (2 << d.k) / 8;
+            e->t_bits[u].begin(s, "inner_bits_batch", bits_bytes);
+            hipLaunchKernelGGL(bitz::k_inner_bits_mixed, dim3((unsigned)P.i_bits_map.size()), dim3(bitz::kInnerBlock),
+                               bitz::kInnerBitsSmem, s, reinterpret_cast<const bitz::InnerBitsDesc*>(dst + P.ib_o),
+                               reinterpret_cast<const uint32_t*>(dst + P.ibm_o));
+            e->t_bits[u].end(s);
         }
         if (!P.i_fold.empty()) {
             // Algorithmic bytes: per output pair, 4 + 4 inputs of D and h read
@@ -4643,8 +4658,7 @@ static int32_t engine_finish_step(bitz_piop_engine* e, size_t* in_flight) {
     if (!act.empty()) prof_add_host("piop_engine_wait", wait_t0);
     for (auto& t : e->t_inner_fold) t.collect();
     e->t_step.collect();
-    for (auto& tp : e->t_bits)
-        for (auto& t : tp) t.collect();
+    for (auto& t : e->t_bits) t.collect();
     const char* pin = e->pin[step.buf].c();
     // Device-transcript proofs that ended in this step: their outputs, and
     // the duplex handed back to the caller's transcript.

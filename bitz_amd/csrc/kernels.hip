@@ -342,6 +342,61 @@ __global__ __launch_bounds__(kBlock) void k_eq_small(const f128* r, int k, f128*
     }
 }
 
+// eq(r, .) over k <= kEqRowsMaxVars variables by one 1024-thread workgroup:
+// the low kl = k / 2 and high kh = k - kl variables' tables (at most 64
+// entries each) doubled level by level side by side in LDS (threads 0..63 the
+// low one, 64..127 the high), then entry j = low[j mod 2^kl] * high[j >> kl],
+// four per thread.  Lane-table products throughout (kLaneSmem1024 of dynamic
+// LDS): a level costs one product's latency instead of a comb product's.
+// With rr (rho and rho^2 on the device) it writes the bind's row scales
+// instead, k_row_scales' layout [w | rho w | rho^2 w] over 2^k entries.
+__global__ __launch_bounds__(1024) void k_eq_rows(const f128* r, int k, f128* out, const f128* rr) {
+    extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
+    __shared__ uint4 lo[64], hi[64];
+    uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
+    lane_table_clear(col);
+    auto ld = [](const uint4* p) {
+        const uint4 u = *p;
+        g4 v;
+        v.w[0] = u.x; v.w[1] = u.y; v.w[2] = u.z; v.w[3] = u.w;
+        return v;
+    };
+    auto sto = [](uint4* p, const g4& v) { *p = make_uint4(v.w[0], v.w[1], v.w[2], v.w[3]); };
+    const int kl = k / 2, kh = k - kl, t = (int)threadIdx.x;
+    if (t == 0) {
+        sto(lo, g_one());
+        sto(hi, g_one());
+    }
+    __syncthreads();
+    for (int i = 0; i < kh; ++i) {  // kh >= kl
+        const int half = 1 << i, j = t & 63;
+        const bool high = t >= 64;
+        if (t < 128 && i < (high ? kh : kl) && j < half) {
+            uint4* T = high ? hi : lo;
+            const g4 parent = ld(T + j), one = g_mul_lane(parent, g_load(r + (high ? kl : 0) + i), col);
+            sto(T + j, g_add(parent, one));  // bit i clear: 1 + r_i
+            sto(T + j + half, one);          // bit i set: r_i
+        }
+        __syncthreads();
+    }
+    const size_t n = (size_t)1 << k, lmask = ((size_t)1 << kl) - 1;
+    g4 rho = g_zero(), rho2 = g_zero();
+    if (rr) {
+        rho = g_load(rr);
+        rho2 = g_load(rr + 1);
+    }
+    for (size_t j = t; j < n; j += 1024) {
+        const g4 w = g_mul_lane(ld(lo + (j & lmask)), ld(hi + (j >> kl)), col);
+        g_store_hbm(out + j, w);
+        if (rr) {
+            g4 a, b;
+            g_mul_lane2(w, rho, rho2, col, a, b);
+            g_store_hbm(out + n + j, a);
+            g_store_hbm(out + 2 * n + j, b);
+        }
+    }
+}
+
 // table[j] = low[j & mask] * high[j >> low_bits]: every entry is the product
 // of its per-coordinate factors, split into a low and a high group.
 __global__ __launch_bounds__(kBlock) void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n) {
@@ -1627,6 +1682,18 @@ template <int K>
 __global__ __launch_bounds__(kInnerBlock) void k_inner_bits_batch(const InnerBitsDesc* d, const uint32_t* bmap) {
     const InnerBitsDesc A = d[bmap[blockIdx.x]];
     inner_bits_body<K>(A, blockIdx.x - A.block_base, A.blocks);
+}
+// Bit rounds of different K in one launch (the PIOP engine's positions):
+// each descriptor's K picks the body; kInnerBitsSmem of dynamic LDS.
+__global__ __launch_bounds__(kInnerBlock) void k_inner_bits_mixed(const InnerBitsDesc* d, const uint32_t* bmap) {
+    const InnerBitsDesc A = d[bmap[blockIdx.x]];
+    const size_t block = blockIdx.x - A.block_base;
+    switch (A.k) {
+        case 0: inner_bits_body<0>(A, block, A.blocks); break;
+        case 1: inner_bits_body<1>(A, block, A.blocks); break;
+        case 2: inner_bits_body<2>(A, block, A.blocks); break;
+        default: inner_bits_body<3>(A, block, A.blocks); break;
+    }
 }
 template __global__ void k_inner_bits<0>(InnerBitsDesc);
 template __global__ void k_inner_bits<1>(InnerBitsDesc);

@@ -18,6 +18,11 @@ __global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_mfma(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_eq_small(const f128* r, int k, f128* table);
+// eq tables of up to 2^kEqRowsMaxVars entries in one 1024-thread workgroup
+// (kLaneSmem1024 of dynamic LDS); rr non-null: the bind's row scales.
+constexpr int kEqRowsMaxVars = 12;
+constexpr size_t kLaneSmem1024 = sizeof(uint4) * 16 * 64 * kLaneTableEntries;
+__global__ void k_eq_rows(const f128* r, int k, f128* out, const f128* rr);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
 __global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);
 __global__ void k_fold(const f128* in, f128* out, size_t out_len, f128 r);
@@ -306,6 +311,7 @@ struct InnerBitsDesc {
     // Or null: `live` and `pairs` are bounds, and the live length of D_{K-1}
     // is ceil(*live0 / 2^(K-1)) (live0: round 0's measure on the device).
     const unsigned long long* live0;
+    int k;                  // the round K (k_inner_bits_mixed)
 };
 // Workgroups of a bit round: enough to stream at HBM rate, few enough that
 // the sums' atomics stay cheap.
@@ -313,6 +319,7 @@ constexpr unsigned kInnerBitsMaxBlocks = 1024;
 constexpr size_t kInnerBitsSmem = sizeof(uint4) * kConstTable4Entries;
 template <int K> __global__ void k_inner_bits(InnerBitsDesc A);
 template <int K> __global__ void k_inner_bits_batch(const InnerBitsDesc* d, const uint32_t* bmap);
+__global__ void k_inner_bits_mixed(const InnerBitsDesc* d, const uint32_t* bmap);
 // ---- prove_spartan_piop rounds with the transcript on the device ---------
 // A proof's running values: the Keccak duplex, the claim, the eq-factored
 // outer sumcheck's S_k = prod_{i<k} eq(tau_i, r_i) and sigma_k = q_{k-1}(r),
