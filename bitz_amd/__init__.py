@@ -164,6 +164,9 @@ def lib():
             "bitz_matrices_prepare": (I32, [C.POINTER(C.c_void_p), SZ, U32, U32, P, P, P]),
             "bitz_bind_and_batch": (I32, [P, P, U32, F128, P, U32]),
             "bitz_matrices_free": (None, [P]),
+            "bitz_matrices_device_bytes": (I32, [P, C.POINTER(C.c_uint64)]),
+            "bitz_piop_engine_device_bytes": (I32, [P, C.c_uint64, C.POINTER(C.c_uint64)]),
+            "bitz_server_resident_bytes": (I32, [P, C.POINTER(C.c_uint64)]),
             "bitz_mt_prepare": (I32, [C.POINTER(C.c_void_p), SZ, SZ, P, P]),
             "bitz_mt_apply": (I32, [P, P, SZ, P, U32]),
             "bitz_mt_transpose_inner": (I32, [P, P, SZ, P, SZ, P, U32]),
@@ -944,6 +947,12 @@ class PreparedConstraintMatrices:
         self.h = h
         return self
 
+    def device_bytes(self) -> int:
+        """Device bytes the prepared matrices hold."""
+        b = C.c_uint64()
+        _check(lib().bitz_matrices_device_bytes(self.h, C.byref(b)))
+        return b.value
+
     def bind_and_batch(self, row_point, rho, device: bool = False, out: "DeviceTable | None" = None):
         """matrix.rs:429-468: D(j) = sum_i eq(i, r_x)(A + rho B + rho^2 C)[i, j];
         a host array, or (device=True) a DeviceTable (out, when given, is
@@ -1189,6 +1198,12 @@ class PiopEngine:
         tags = np.zeros(max(n.value, 1), dtype=np.uint64)
         _check(lib().bitz_piop_engine_done(self.h, _p(tags), len(tags), C.byref(n)))
         return [int(t) for t in tags[: n.value]]
+
+    def device_bytes(self, tag: int) -> int:
+        """Device bytes the proof holds from its submission until taken."""
+        b = C.c_uint64()
+        _check(lib().bitz_piop_engine_device_bytes(self.h, tag, C.byref(b)))
+        return b.value
 
     def take(self, tag: int) -> SpartanPiopOutput:
         rv, cv = self._held[tag][4], self._held[tag][5]

@@ -3194,6 +3194,22 @@ struct bitz_matrices {
     }
 };
 
+// Device bytes a prepared matrices handle holds (every buffer as allocated,
+// size classes included).
+static uint64_t matrices_device_bytes(const bitz_matrices* h) {
+    uint64_t b = h->col_starts.bytes + h->row_idx.bytes + h->which.bytes + h->values.bytes + h->eq_rows.bytes +
+                 h->row_scale.bytes + h->scales.bytes + h->point.bytes + h->scratch.bytes + h->prod.bytes +
+                 h->long_chunks.bytes;
+    for (int k = 0; k < 3; ++k) b += h->rs[k].bytes + h->cs[k].bytes + h->vs[k].bytes;
+    return b;
+}
+
+extern "C" int32_t bitz_matrices_device_bytes(const bitz_matrices* h, uint64_t* bytes) {
+    if (!h || !bytes) return fail(BITZ_ERR_ARG, "null pointer");
+    *bytes = matrices_device_bytes(h);
+    return BITZ_OK;
+}
+
 extern "C" int32_t bitz_matrices_prepare(bitz_matrices** out, size_t rows, uint32_t row_vars,
                                          uint32_t col_vars, const uint64_t* const row_starts[3],
                                          const uint32_t* const columns[3],
@@ -4889,6 +4905,31 @@ extern "C" int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_u
     return BITZ_OK;
 }
 
+// Device bytes one proof holds in the engine: its own buffers and its outer
+// sumcheck's, as allocated (size classes included); all are allocated when
+// it is submitted and held until it is taken.
+static uint64_t job_device_bytes(const PiopJob* j) {
+    uint64_t b = j->dlo.bytes + j->dhi.bytes + j->dD.bytes + j->iP.bytes + j->iQ.bytes + j->fsb.bytes +
+                 j->own_bits.bytes + j->own_prod.bytes;
+    if (const bitz_outer* h = j->outer) {
+        b += h->in.bytes + h->P.bytes + h->Q.bytes + h->acc.bytes + h->htabs4.bytes + h->rtabs4.bytes;
+        for (int u = 0; u < 2; ++u) b += h->eqL[u].bytes + h->eqH[u].bytes;
+    }
+    return b;
+}
+
+extern "C" int32_t bitz_piop_engine_device_bytes(bitz_piop_engine* e, uint64_t tag, uint64_t* bytes) {
+    if (!e || !bytes) return fail(BITZ_ERR_ARG, "null pointer");
+    std::lock_guard<std::mutex> lock(e->mu);
+    for (const auto* list : {&e->jobs, &e->incoming, &e->finished})
+        for (const auto& j : *list)
+            if (j->tag == tag) {
+                *bytes = job_device_bytes(j.get());
+                return BITZ_OK;
+            }
+    return fail(BITZ_ERR_ARG, "no proof with this tag");
+}
+
 extern "C" int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds,
                                          bitz_f128* outer_claims, bitz_f128* inner_rounds, bitz_f128* inner_point,
                                          bitz_f128* matrix_eval, bitz_f128* final_claim) {
@@ -5772,6 +5813,7 @@ struct ServeCircuit {
     bitz_matrices* m = nullptr;
     uint8_t digest[32] = {};
     uint64_t footprint = 0;
+    uint64_t resident = 0;  // its matrices and the engines' bind buffers for it
     size_t columns = 0, witness_bits = 0;
 };
 }  // namespace
@@ -5779,6 +5821,9 @@ struct ServeCircuit {
 struct bitz_server {
     uint32_t slots = 8, preparers = 4;
     uint64_t budget = 0;
+    // Resident for the server's life: every circuit's matrices and each
+    // engine's bind buffers for it; the requests share the rest of the budget.
+    uint64_t resident = 0;
     std::vector<bitz_piop_engine*> engines;
     std::vector<ServeCircuit> circuits;
     ~bitz_server() {
@@ -5819,8 +5864,19 @@ extern "C" int32_t bitz_server_add_circuit(bitz_server* s, const bitz_sha256* c,
     sc.footprint = footprint;
     size_t rows = 0, mpos = 0, nnz[3];
     TRY(bitz_sha256_shape(c, &rows, &sc.columns, &sc.witness_bits, nnz, &mpos));
+    // The matrices, and per engine the bind buffers (BindScratch): the point,
+    // the row eq table, three row scales and one product per nonzero.
+    const uint64_t bind = 16ull * ((uint64_t)std::max(m->row_vars, 1) + 4ull * m->rows_padded + m->nnz);
+    sc.resident = matrices_device_bytes(m) + (uint64_t)s->engines.size() * bind;
+    s->resident += sc.resident;
     *id = (uint32_t)s->circuits.size();
     s->circuits.push_back(sc);
+    return BITZ_OK;
+}
+
+extern "C" int32_t bitz_server_resident_bytes(const bitz_server* s, uint64_t* bytes) {
+    if (!s || !bytes) return fail(BITZ_ERR_ARG, "null pointer");
+    *bytes = s->resident;
     return BITZ_OK;
 }
 
@@ -5948,15 +6004,17 @@ extern "C" int32_t bitz_server_serve(bitz_server* s, bitz_serve_request* reqs, s
     for (size_t k = 0; k < s->engines.size(); ++k) threads.emplace_back(harvest, k);
     {
         // Admission in arrival order: a slot free and the footprint within
-        // the budget (a request larger than the budget runs alone).
+        // what the resident circuits leave of the budget (a request larger
+        // than that runs alone).
         std::unique_lock<std::mutex> lock(mu);
+        const uint64_t budget = s->budget > s->resident ? s->budget - s->resident : 0;
         size_t next = 0;
         while (left && !err) {
             const double t = now_s();
             while (next < count && reqs[order[next]].arrival_s <= t && in_flight < s->slots) {
                 bitz_serve_request& r = reqs[order[next]];
                 const uint64_t fp = s->circuits[r.circuit].footprint;
-                if (in_flight && used + fp > s->budget) break;
+                if (in_flight && used + fp > budget) break;
                 r.admitted_s = t;
                 used += fp;
                 in_flight += 1;
@@ -5964,7 +6022,7 @@ extern "C" int32_t bitz_server_serve(bitz_server* s, bitz_serve_request* reqs, s
                 ++next;
                 cv.notify_all();
             }
-            if (next < count && in_flight < s->slots && (!in_flight || used + s->circuits[reqs[order[next]].circuit].footprint <= s->budget)) {
+            if (next < count && in_flight < s->slots && (!in_flight || used + s->circuits[reqs[order[next]].circuit].footprint <= budget)) {
                 const double wait = reqs[order[next]].arrival_s - now_s();
                 if (wait > 0) cv.wait_for(lock, std::chrono::duration<double>(wait));
             } else {

@@ -67,6 +67,13 @@ class NativeServer:
         self.ids: dict[int, int] = {}
         self.config = (slots, engines, preparers, hbm_budget)
 
+    def resident_bytes(self) -> int:
+        """Device bytes held for the server's life (circuits' matrices and the
+        engines' bind buffers); requests share the rest of the budget."""
+        b = C.c_uint64()
+        _check(lib().bitz_server_resident_bytes(self.h, C.byref(b)))
+        return b.value
+
     def add(self, c: "Circuit") -> int:
         if c.blocks not in self.ids:
             cid = C.c_uint32()
@@ -114,8 +121,12 @@ class Circuit:
         """Device bytes a request holds at its peak: the packed assignment
         (2^col_vars bits), D (2^col_vars elements) with the inner sumcheck's
         scratch (1.5 x 2^col_vars), the three products (3 x 2^row_vars) with
-        the outer scratch (2.25 x)."""
-        return 16 * int((1 << self.col_vars) * 2.5 + (1 << self.row_vars) * 5.25) + (1 << self.col_vars) // 8
+        the outer scratch (2.25 x), and 256 KiB of small tables (transcript
+        state and outputs, eq factors, challenge and H(i) tables).  A PIOP
+        engine's measured bytes per request (PiopEngine.device_bytes) stay
+        within it for 1 to 256 blocks: 91 % of it at 1 block, 99.9 % at 256."""
+        return (16 * int((1 << self.col_vars) * 2.5 + (1 << self.row_vars) * 5.25) + (1 << self.col_vars) // 8 +
+                (256 << 10))
 
 
 def make_circuit(blocks: int) -> Circuit:
@@ -156,6 +167,7 @@ class ProvingServer:
         self.circuits: dict[int, Circuit] = {}
         self._lock = threading.Lock()
         self.peak_bytes = 0      # largest footprint in flight during the last serve()
+        self.resident_bytes = 0  # serve_batched: circuits' matrices and bind buffers, held throughout
         self.peak_in_flight = 0  # most requests in flight during the last serve()
         self._native: NativeServer | None = None  # serve_batched's server, kept across calls
 
@@ -236,10 +248,13 @@ class ProvingServer:
                                          out[o + 3:o + 3 + 3 * cv].reshape(cv, 3, 2),
                                          out[o + 3 + 3 * cv:o + 3 + 4 * cv], to_int(out[o + 3 + 4 * cv]),
                                          to_int(out[o + 4 + 4 * cv]))
-        # the peaks of the admission that just ran (replayed from the times)
+        # the peaks of the admission that just ran (replayed from the times),
+        # the resident circuits' bytes included
+        self.resident_bytes = ns.resident_bytes()
         ev = sorted([(r.admitted, 1, self.circuits[r.blocks].footprint()) for r in requests] +
                     [(r.done, -1, -self.circuits[r.blocks].footprint()) for r in requests], key=lambda x: (x[0], x[1]))
-        n = b = 0
+        n, b = 0, self.resident_bytes
+        self.peak_bytes = b
         for _, dn, db in ev:
             n += dn
             b += db

@@ -422,6 +422,8 @@ int32_t bitz_prove_spartan_piop_packed(bitz_transcript* t, bitz_matrices* h, con
                                        bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
                                        bitz_f128* final_claim);
 void bitz_matrices_free(bitz_matrices* h);
+/* Device bytes the handle holds (every buffer as allocated). */
+int32_t bitz_matrices_device_bytes(const bitz_matrices* h, uint64_t* bytes);
 
 /* ---- post-GKR inner-product sumcheck ----------------------------------------- */
 /* post_gkr::prove (crates/post_gkr/src/lib.rs:85-148, sumcheck.rs:99-178):
@@ -495,6 +497,10 @@ int32_t bitz_piop_engine_wait(bitz_piop_engine* e, uint32_t timeout_us, size_t* 
 int32_t bitz_piop_engine_done(bitz_piop_engine* e, uint64_t* tags, size_t cap, size_t* count);
 /* A finished proof's outputs (as bitz_prove_spartan_piop's; a null output is
  * not copied); releases it. */
+/* Device bytes the proof `tag` holds in the engine, from its submission
+ * (which allocates them all) until it is taken: what a serving budget must
+ * set aside per request.  No counterpart in the reference. */
+int32_t bitz_piop_engine_device_bytes(bitz_piop_engine* e, uint64_t tag, uint64_t* bytes);
 int32_t bitz_piop_engine_take(bitz_piop_engine* e, uint64_t tag, bitz_f128* outer_rounds, bitz_f128* outer_claims,
                               bitz_f128* inner_rounds, bitz_f128* inner_point, bitz_f128* matrix_eval,
                               bitz_f128* final_claim);
@@ -649,8 +655,12 @@ int32_t bitz_server_create(bitz_server** s, uint32_t slots, uint32_t engines, ui
  * circuit and matrices must outlive the server's use of them. */
 int32_t bitz_server_add_circuit(bitz_server* s, const bitz_sha256* circuit, bitz_matrices* m,
                                 const uint8_t digest[32], uint64_t footprint, uint32_t* id);
-/* Serves every request and returns when all are done (or one failed). */
+/* Serves every request and returns when all are done (or one failed).  The
+ * requests share what the resident bytes leave of the budget. */
 int32_t bitz_server_serve(bitz_server* s, bitz_serve_request* requests, size_t count);
+/* Device bytes resident for the server's life: every added circuit's
+ * matrices and, per engine, the bind buffers for it. */
+int32_t bitz_server_resident_bytes(const bitz_server* s, uint64_t* bytes);
 void bitz_server_free(bitz_server* s);
 
 #ifdef __cplusplus

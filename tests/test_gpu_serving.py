@@ -7,6 +7,7 @@ import struct
 import numpy as np
 import pytest
 
+import bitz_amd as G
 from bitz_amd import serving as S
 from common import compress_py
 from test_gpu_sha256 import prove_and_check
@@ -52,13 +53,42 @@ def test_batched_serving_gives_the_same_proofs():
 
 
 def test_batched_admission_keeps_the_budget():
-    srv = S.ProvingServer(hbm_budget=1 << 30, slots=8)
+    """The native server's requests share what the resident circuits (their
+    matrices and the engines' bind buffers) leave of the budget: with room
+    for two of the largest requests beside them, the resident bytes plus the
+    footprints in flight stay within the budget."""
+    srv = S.ProvingServer(hbm_budget=1 << 34, slots=8)
     reqs = S.poisson_requests(rate=5000.0, count=10, seed=7, max_log_blocks=3)
     fps = [srv.circuit(r.blocks).footprint() for r in reqs]
-    srv.hbm_budget = 2 * max(fps)
     srv.serve_batched(reqs)
+    resident = srv.resident_bytes
+    assert resident > 0
+    srv.hbm_budget = resident + 2 * max(fps)
+    srv.serve_batched(reqs)
+    assert srv.resident_bytes == resident
     assert srv.peak_bytes <= srv.hbm_budget
     assert all(r.result is not None for r in reqs)
+
+
+@pytest.mark.parametrize("blocks", [1, 8, 64, 256])
+def test_footprint_covers_a_requests_device_bytes(blocks):
+    """What one request holds on the device in a PIOP engine, measured (every
+    buffer as allocated, from its submission until it is taken), is within
+    the footprint admission budgets for it."""
+    c = S.make_circuit(blocks)
+    msg, iv = S.random_input(np.random.default_rng(blocks), blocks)
+    w = c.circuit.witness(msg, iv=list(iv))
+    eng = G.PiopEngine()
+    eng.submit_assignment(1, G.Transcript.keccak(b"fp/%d" % blocks), c.matrices, c.digest, w.h, c.columns,
+                          c.columns)
+    used = eng.device_bytes(1)
+    steps = 0
+    while eng.step():
+        steps += 1
+        assert steps < 100
+    eng.take(1)
+    print(f"blocks {blocks}: {used} device bytes, footprint {c.footprint()} ({used / c.footprint():.3f})")
+    assert 0 < used <= c.footprint()
 
 
 def test_admission_keeps_the_budget():
