@@ -325,23 +325,6 @@ __global__ __launch_bounds__(kBlock) void k_eq_small2(const f128* r, int kl, f12
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_eq_small(const f128* r, int k, f128* table) {
-    if (threadIdx.x == 0) g_store(table, g_one());
-    for (size_t j = 1 + threadIdx.x; j < ((size_t)1 << k); j += blockDim.x) g_store(table + j, g_zero());
-    __syncthreads();
-    for (int i = 0; i < k; ++i) {
-        const size_t half = (size_t)1 << i;
-        const g4 ri = g_load(r + i);
-        for (size_t j = threadIdx.x; j < half; j += blockDim.x) {
-            const g4 parent = g_load(table + j);
-            const g4 one = g_mul(parent, ri);
-            g_store(table + j, g_add(parent, one));
-            g_store(table + j + half, one);
-        }
-        __syncthreads();
-    }
-}
-
 // eq(r, .) over k <= kEqRowsMaxVars variables by one 1024-thread workgroup:
 // the low kl = k / 2 and high kh = k - kl variables' tables (at most 64
 // entries each) doubled level by level side by side in LDS (threads 0..63 the

@@ -17,7 +17,6 @@ __global__ void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, in
 __global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_mfma(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n);
-__global__ void k_eq_small(const f128* r, int k, f128* table);
 // eq tables of up to 2^kEqRowsMaxVars entries in one 1024-thread workgroup
 // (kLaneSmem1024 of dynamic LDS); rr non-null: the bind's row scales.
 constexpr int kEqRowsMaxVars = 12;
