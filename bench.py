@@ -119,7 +119,7 @@ def main() -> int:
                     help="serve: also time every engine step (device span by HIP events, host launch and "
                          "finish, wait) and report per-step averages")
     ap.add_argument("--chain-blocks", type=int, default=64)
-    ap.add_argument("--serve-slots", type=int, default=96)
+    ap.add_argument("--serve-slots", type=int, default=192)
     ap.add_argument("--serve-engines", type=int, default=4,
                     help="serve: PIOP engines, one stream each (0: one per 8 slots)")
     ap.add_argument("--serve-preparers", type=int, default=12,
