@@ -20,7 +20,8 @@ batch (256 outer sumchecks of 2^12..2^22), gkr (grand product over 2^22
 leaves), inner, bind, chain (the sha256-chain statement), products (each
 GF(2^128) product primitive over 2^log_n elements, the constant-table and
 int8-MFMA products by a fixed element among them), mt (M^T apply and
-transpose_inner over the SHA-256 circuits).
+transpose_inner over the SHA-256 circuits), eval (a multilinear's evaluation
+over 2^26 device-resident entries).
 
 Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
 the reference with the bit-serial multiply the reference selects on x86),
@@ -96,7 +97,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-log-n", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["serve", "outer", "batch", "gkr", "inner", "bind", "chain", "products", "mt"],
+    ap.add_argument("--workload", choices=["serve", "outer", "batch", "gkr", "inner", "bind", "chain", "products", "mt", "eval"],
                     default="serve",
                     help="serve: SHA-256 proving requests served with continuous batching (default); "
                          "outer: one 2^log_n outer sumcheck per step; batch: 256 outer sumchecks "
@@ -127,6 +128,7 @@ def main() -> int:
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--inner-log-n", type=int, default=23,
                     help="inner: log2 table length (23: the column count of a 256-block SHA-256 chain)")
+    ap.add_argument("--eval-log-n", type=int, default=26, help="eval: the multilinear's variables")
     ap.add_argument("--gkr-log-leaves", type=int, default=22)
     ap.add_argument("--gkr-log-groups", type=int, default=1,
                     help="gkr: log2 of the number of trees (1: full depth, two outputs; 15: 7 layers)")
@@ -172,6 +174,8 @@ def main() -> int:
         return bench_products(G, args)
     if args.workload == "mt":
         return bench_mt(G, args)
+    if args.workload == "eval":
+        return bench_eval(G, args)
 
     return bench_outer(G, args)
 
@@ -1117,6 +1121,76 @@ def bench_products(G, args) -> int:
         "mfma_vs_const_table": mfma["products_per_s"] / const["products_per_s"],
         "primitives": res,
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_eval(G, args) -> int:
+    """DenseMultilinearExtension::evaluate (crates/poly/src/mle.rs:158-172) of
+    a 2^eval_log_n table resident on the device at a uniform point: the
+    table folded two variables a pass through the challenges' constant
+    tables.  The roofline is the first pass (every input read once, a quarter
+    as many outputs written: 20 B per input), timed with HIP events on its
+    stream; the oracle (bit-serial multiply, one host core) evaluates a 2^20
+    table as the CPU baseline, scaled to the same size."""
+    from oracle import oracle as O
+    n = args.eval_log_n
+    cpu = None
+    if not args.no_cpu:
+        O.set_kernel("portable")
+        rng = np.random.default_rng(3)
+        t = rng.integers(0, 2**64, size=(1 << 20, 2), dtype=np.uint64)
+        p = rng.integers(0, 2**64, size=(20, 2), dtype=np.uint64)
+        t0 = time.perf_counter()
+        O.mle_evaluate(t, p)
+        dt = time.perf_counter() - t0
+        cpu = {"value": 1.0 / (dt * (1 << (n - 20))), "unit": "evaluations/s", "cores": 1, "kind": "port",
+               "sample": f"oracle mle_evaluate, bit-serial portable multiply, 2^20 entries, 1 thread: {dt:.2f} s, "
+                         f"scaled x{1 << (n - 20)} to 2^{n}"}
+    G.init(0)
+    rng = np.random.default_rng(args.seed)
+    table = G.DeviceTable(length=1 << n)
+    chunk = 1 << 22
+    for o in range(0, 1 << n, chunk):  # uniform entries, uploaded in pieces
+        table.copy_from(G.DeviceTable(rng.integers(0, 2**64, size=(min(chunk, 1 << n), 2), dtype=np.uint64)), o)
+    points = [rng.integers(0, 2**64, size=(n, 2), dtype=np.uint64) for _ in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        G.mle_evaluate_device(table, points[i])
+    G.profile_enable(True)
+    G.profile_only("mle_fold_pass")
+    times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        G.mle_evaluate_device(table, points[args.warmup + i])
+        times.append(time.perf_counter() - t0)
+    wall = time.perf_counter() - t_start
+    st = G.profile_read("mle_fold_pass")
+    G.profile_enable(False)
+    ms = st["ms"] / max(st["launches"], 1)
+    achieved = st["bytes"] / max(st["launches"], 1) / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "multilinear evaluations/sec (2^%d entries, device-resident), 1 MI355X" % n,
+        "value": args.steps / wall,
+        "unit": "evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(np.array(times) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(times) * 1e3, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (uniform table and points from the seed)",
+        "config": {"workload": f"DenseMultilinearExtension::evaluate over 2^{n} device-resident entries", "log_n": n},
+        "roofline": {"bound": "hbm", "kernel": "k_fold2_t (the first fold pass: two variables)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "kernel_ms": ms},
+        "cpu_baseline": cpu,
+    }
+    log(f"bench: evaluate 2^{n}: {line['p50_ms']:.3f} ms, first pass {ms * 1e3:.1f} us at {achieved:.0f} GB/s")
     print(json.dumps(line), flush=True)
     return 0
 

@@ -302,11 +302,12 @@ class DeviceTable:
     def __len__(self):
         return self.len
 
-    def copy_from(self, other: "DeviceTable") -> None:
-        """Device-to-device copy of other's elements into this table."""
-        n = min(self.len, other.len)
-        if n:
-            _check(lib().bitz_memcpy_d2d(C.c_void_p(self.ptr), C.c_void_p(other.ptr), n * 16))
+    def copy_from(self, other: "DeviceTable", offset: int = 0) -> None:
+        """Device-to-device copy of other's elements into this table, from
+        entry `offset` on."""
+        n = min(self.len - offset, other.len)
+        if n > 0:
+            _check(lib().bitz_memcpy_d2d(C.c_void_p(self.ptr + 16 * offset), C.c_void_p(other.ptr), n * 16))
 
     def free(self):
         if self.ptr:
@@ -584,6 +585,19 @@ def mle_evaluate(evaluations, point) -> int:
     out = F128()
     _check(lib().bitz_mle_evaluate(_p(ev), _nvars(len(ev)), _p(point), len(point), C.byref(out), 0))
     return to_int(out)
+
+
+def mle_evaluate_device(table: "DeviceTable", point) -> int:
+    """mle_evaluate over a table resident on the device (read in place); with
+    profiling on, its first fold pass is timed as mle_fold_pass."""
+    init()
+    point = elems(point) if len(point) else np.zeros((0, 2), np.uint64)
+    if table.len != 1 << len(point):
+        raise BitzError(-7, "point width differs from num_vars")
+    dpt, dout = DeviceTable(point), DeviceTable(length=1)  # BITZ_DEVICE: every element array on the device
+    _check(lib().bitz_mle_evaluate(C.c_void_p(table.ptr), len(point), C.c_void_p(dpt.ptr), len(point),
+                                   C.c_void_p(dout.ptr), DEVICE))
+    return to_int(dout.to_host()[0])
 
 
 # ---------------------------------------------------------------- spartan ---

@@ -744,32 +744,61 @@ extern "C" int32_t bitz_eq_table(const bitz_f128* r, uint32_t k, bitz_f128* out,
     return download(out, dout.f(), n, flags, so.s);
 }
 
+// Folds the `len` entries at `in` (a device table, read in place and never
+// written) by r[0 .. m), two variables a pass while two or more remain, into
+// the scratch tables s0 (len / 2 entries) and s1 (len / 4); returns the table
+// that holds the len >> m results.  The first pass is timed as
+// "mle_fold_pass" (its algorithmic bytes: every input read, every output
+// written once).
+static const f128* mle_fold_passes(const f128* in, size_t len, const bitz_f128* r, uint32_t m, f128* s0, f128* s1,
+                                   hipStream_t s, Timed& first) {
+    const f128* cur = in;
+    f128* bufs[2] = {s0, s1};
+    int w = 0;
+    for (uint32_t i = 0; i < m;) {
+        f128* out = bufs[w];
+        w ^= 1;
+        const bool two = m - i >= 2;
+        const size_t out_len = two ? len / 4 : len / 2;
+        if (i == 0) first.begin(s, "mle_fold_pass", 16.0 * (double)(len + out_len));
+        if (two)
+            hipLaunchKernelGGL(bitz::k_fold2_t, dim3(grid_for(out_len)), dim3(kBlock), 0, s, cur, out, out_len, F(r[i]),
+                               F(r[i + 1]));
+        else
+            hipLaunchKernelGGL(bitz::k_fold_t, dim3(grid_for(out_len)), dim3(kBlock), 0, s, cur, out, out_len, F(r[i]));
+        if (i == 0) first.end(s);
+        i += two ? 2 : 1;
+        len = out_len;
+        cur = out;
+    }
+    return cur;
+}
+
 extern "C" int32_t bitz_mle_fold(bitz_f128* evals, uint32_t* num_vars, const bitz_f128* r, uint32_t m,
                                  uint32_t flags) {
     if (!evals || !num_vars || (m && !r)) return fail(BITZ_ERR_ARG, "null pointer");
     // mle.rs:136-142: too many challenges is rejected before mutating.
     if (m > *num_vars) return fail(BITZ_ERR_TOO_MANY, "more challenges than variables");
     if (m == 0) return BITZ_OK;
-    Dev a, b;
+    const bool dev = (flags & BITZ_DEVICE) != 0;
+    Dev a, b, c;
     StreamOwner so;
     TRY(so.create());
     const size_t n = (size_t)1 << *num_vars;
-    TRY(upload(a, evals, n, flags, so.s));
-    TRY(b.alloc(n / 2 * sizeof(f128)));
-    std::vector<bitz_f128> rr(m);
-    HIP_TRY(hipMemcpy(rr.data(), r, m * sizeof(f128),
-                      (flags & BITZ_DEVICE) ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
-    f128* cur = a.f();
-    f128* nxt = b.f();
-    size_t len = n;
-    for (uint32_t i = 0; i < m; ++i) {
-        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(len / 2)), dim3(kBlock), 0, so.s, cur, nxt,
-                           len / 2, F(rr[i]));
-        TRY(check_launch());
-        std::swap(cur, nxt);
-        len /= 2;
+    const f128* in = reinterpret_cast<const f128*>(evals);
+    if (!dev) {
+        TRY(upload(c, evals, n, flags, so.s));
+        in = c.f();
     }
-    TRY(download(evals, cur, len, flags, so.s));
+    TRY(a.alloc(std::max<size_t>(n / 2, 1) * sizeof(f128)));
+    TRY(b.alloc(std::max<size_t>(n / 4, 1) * sizeof(f128)));
+    std::vector<bitz_f128> rr(m);
+    HIP_TRY(hipMemcpy(rr.data(), r, m * sizeof(f128), dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    Timed tm;
+    const f128* res = mle_fold_passes(in, n, rr.data(), m, a.f(), b.f(), so.s, tm);
+    TRY(check_launch());
+    TRY(download(evals, res, n >> m, flags, so.s));  // in place: the first n >> m entries
+    tm.collect();
     *num_vars -= m;
     return BITZ_OK;
 }
@@ -778,30 +807,29 @@ extern "C" int32_t bitz_mle_evaluate(const bitz_f128* evals, uint32_t num_vars, 
                                      uint32_t point_len, bitz_f128* out, uint32_t flags) {
     if (!evals || !out || (point_len && !point)) return fail(BITZ_ERR_ARG, "null pointer");
     if (point_len != num_vars) return fail(BITZ_ERR_ARG, "point width differs from num_vars");
-    Dev a, b;
+    const bool dev = (flags & BITZ_DEVICE) != 0;
+    Dev a, b, c;
     StreamOwner so;
     TRY(so.create());
     const size_t n = (size_t)1 << num_vars;
-    TRY(upload(a, evals, n, flags, so.s));
-    TRY(b.alloc(std::max<size_t>(n / 2, 1) * sizeof(f128)));
+    const f128* in = reinterpret_cast<const f128*>(evals);
+    if (!dev) {
+        TRY(upload(c, evals, n, flags, so.s));
+        in = c.f();
+    }
+    TRY(a.alloc(std::max<size_t>(n / 2, 1) * sizeof(f128)));
+    TRY(b.alloc(std::max<size_t>(n / 4, 1) * sizeof(f128)));
     std::vector<bitz_f128> rr(point_len);
     if (point_len)
-        HIP_TRY(hipMemcpy(rr.data(), point, point_len * sizeof(f128),
-                          (flags & BITZ_DEVICE) ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
-    f128* cur = a.f();
-    f128* nxt = b.f();
-    size_t len = n;
-    for (uint32_t i = 0; i < point_len; ++i) {
-        hipLaunchKernelGGL(bitz::k_fold, dim3(grid_for(len / 2)), dim3(kBlock), 0, so.s, cur, nxt,
-                           len / 2, F(rr[i]));
-        TRY(check_launch());
-        std::swap(cur, nxt);
-        len /= 2;
-    }
+        HIP_TRY(hipMemcpy(rr.data(), point, point_len * sizeof(f128), dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    Timed tm;
+    const f128* res = mle_fold_passes(in, n, rr.data(), point_len, a.f(), b.f(), so.s, tm);
+    TRY(check_launch());
     bitz_f128 v;
-    HIP_TRY(hipMemcpyAsync(&v, cur, sizeof v, hipMemcpyDeviceToHost, so.s));
+    HIP_TRY(hipMemcpyAsync(&v, res, sizeof v, hipMemcpyDeviceToHost, so.s));
     HIP_TRY(hipStreamSynchronize(so.s));
-    if (flags & BITZ_DEVICE)
+    tm.collect();
+    if (dev)
         HIP_TRY(hipMemcpy(out, &v, sizeof v, hipMemcpyHostToDevice));
     else
         *out = v;

@@ -397,6 +397,39 @@ __global__ __launch_bounds__(kBlock) void k_fold(const f128* in, f128* out, size
         g_store(out + i, interp(g_load(in + 2 * i), g_load(in + 2 * i + 1), rr));
 }
 
+// MLE folds by a fixed r (DenseMultilinearExtension::fold, mle.rs:136-156)
+// through r's 4-bit table, built in LDS by one wave: one constant-table
+// product per output.  k_fold2_t binds two variables a pass (r0, then r1 on
+// the two half-folded values), reading four inputs per output: 20 B of HBM
+// per input instead of 24 for two single passes, at 3 products per output.
+__global__ __launch_bounds__(kBlock) void k_fold_t(const f128* in, f128* out, size_t out_len, f128 r) {
+    __shared__ __attribute__((aligned(256))) uint4 tab[kConstTable4Entries];
+    if (threadIdx.x < 64) const_table4_wave(tab, g_from(r));
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < out_len;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const g4 a = g_load_hbm(in + 2 * i), b = g_load_hbm(in + 2 * i + 1);
+        g_store_hbm(out + i, g_add(a, g_mul_table4(tab, g_add(a, b))));
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_fold2_t(const f128* in, f128* out, size_t out_len, f128 r0, f128 r1) {
+    __shared__ __attribute__((aligned(256))) uint4 t0[kConstTable4Entries];
+    __shared__ __attribute__((aligned(256))) uint4 t1[kConstTable4Entries];
+    if (threadIdx.x < 64)
+        const_table4_wave(t0, g_from(r0));
+    else if (threadIdx.x < 128)
+        const_table4_wave(t1, g_from(r1));
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < out_len;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const g4 a0 = g_load_hbm(in + 4 * i), a1 = g_load_hbm(in + 4 * i + 1);
+        const g4 a2 = g_load_hbm(in + 4 * i + 2), a3 = g_load_hbm(in + 4 * i + 3);
+        const g4 x = g_add(a0, g_mul_table4(t0, g_add(a0, a1)));
+        const g4 y = g_add(a2, g_mul_table4(t0, g_add(a2, a3)));
+        g_store_hbm(out + i, g_add(x, g_mul_table4(t1, g_add(x, y))));
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fold3(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
                         size_t out_len, f128 r) {
     const g4 rr = g_from(r);

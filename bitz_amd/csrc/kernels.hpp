@@ -25,6 +25,9 @@ __global__ void k_eq_rows(const f128* r, int k, f128* out, const f128* rr);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
 __global__ void k_eq_combine(const f128* low, int low_bits, const f128* high, f128* out, size_t n);
 __global__ void k_fold(const f128* in, f128* out, size_t out_len, f128 r);
+// MLE folds through the challenges' constant tables: one variable, or two a pass.
+__global__ void k_fold_t(const f128* in, f128* out, size_t out_len, f128 r);
+__global__ void k_fold2_t(const f128* in, f128* out, size_t out_len, f128 r0, f128 r1);
 __global__ void k_fold3(const f128* a, const f128* b, const f128* c, f128* ao, f128* bo, f128* co,
                         size_t out_len, f128 r);
 __global__ void k_outer_round0(const f128* a, const f128* b, const f128* c, size_t pairs,

@@ -73,6 +73,22 @@ def test_mle_fold_and_evaluate_match_oracle():
     assert e.value.variant == "TooManyChallenges"
 
 
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 13, 21])
+def test_mle_evaluate_on_device_tables_matches_oracle(n):
+    """Two variables a pass through the challenges' constant tables, one for
+    an odd last variable, the device table read in place (sizes 1 .. 2^21)."""
+    t = elements_fast(40 + n, 1 << n)
+    pt = elements_fast(60 + n, n)
+    d = G.DeviceTable(t)
+    assert G.mle_evaluate_device(d, pt) == O.mle_evaluate(t, pt)
+    assert np.array_equal(d.to_host(), t)  # the input is not written
+    if n >= 3:
+        ref = t
+        for i in range(3):
+            ref = O.mle_fold(ref, pt[i])
+        assert np.array_equal(G.mle_fold(t, pt[:3]), ref)
+
+
 def outer_case(n, split, seed, satisfied=False):
     a = elements_fast(seed, 1 << n)
     b = elements_fast(seed + 1, 1 << n)
