@@ -713,18 +713,34 @@ def _instance_dtype() -> np.dtype:
                      "offsets": [o for _, o in fields.values()], "itemsize": C.sizeof(_OuterInstance)})
 
 
-def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
-    """Many independent outer sumchecks with challenges fixed in advance, in
-    one batched call (bitz_prove_outer_batch).  instances: iterable of
-    (challenges, initial_claim, (eq_low, eq_high), R1csProductMles) with
-    DeviceTables for the products and factors.  Returns one
-    OuterSumcheckOutput per instance, equal to
-    prove_outer_sumcheck_with_challenges on the same inputs.  graph=False
-    launches the kernels directly instead of replaying the captured graph;
-    tune=False keeps the default tile shapes."""
-    init()
-    instances = list(instances)
-    m = len(instances)
+class OuterBatch:
+    """A batch of outer sumchecks marshalled once for bitz_prove_outer_batch
+    (the instance records, challenges and output arrays), so that repeated
+    calls cost the C call alone: prove() runs the batch and fills `rounds`,
+    `points`, `finals` and `claims` in place; outputs() wraps them per
+    instance.  The tables are borrowed and must outlive the object."""
+
+    def __init__(self, instances, graph: bool = True, tune: bool = True):
+        init()
+        self.keep = list(instances)
+        self.rec, self.arrays = _outer_batch_records(self.keep)
+        self.m = len(self.keep)
+        self.flags = DEVICE | (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
+
+    def prove(self) -> None:
+        _check(lib().bitz_prove_outer_batch(C.c_void_p(self.rec.ctypes.data), self.m, self.flags), SumcheckError)
+
+    def outputs(self) -> list:
+        ns, off, _, rounds, points, finals, claims = self.arrays
+        m = self.m
+        fin = [x[0] | (x[1] << 64) for x in finals[:m].tolist()]
+        cl = [x[0] | (x[1] << 64) for x in claims[: 3 * m].tolist()]
+        return [OuterSumcheckOutput(rounds[4 * off[i]: 4 * off[i + 1]].reshape(ns[i], 4, 2),
+                                    points[off[i]: off[i + 1]].copy(), fin[i], cl[3 * i], cl[3 * i + 1],
+                                    cl[3 * i + 2]) for i in range(m)]
+
+
+def _outer_batch_records(instances):
     ns, chs, tabs = [], [], []
     for ch, claim, eq, prods in instances:
         az, bz, cz = prods.az, prods.bz, prods.cz
@@ -736,6 +752,7 @@ def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
         ns.append(n)
         chs.append(_challenges(ch, n)[:n])
         tabs.append((claim, lo, hi, az, bz, cz))
+    m = len(tabs)
     off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
     total = int(off[-1])
     allch = np.ascontiguousarray(np.concatenate(chs) if total else np.zeros((1, 2), np.uint64), dtype=np.uint64)
@@ -759,13 +776,22 @@ def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
         rec["eval_points"][:m] = points.ctypes.data + 16 * off[:-1]
         rec["final_claim"][:m] = finals.ctypes.data + 16 * np.arange(m)
         rec["claims"][:m] = claims.ctypes.data + 48 * np.arange(m)
-    flags = DEVICE | (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
-    _check(lib().bitz_prove_outer_batch(C.c_void_p(rec.ctypes.data), m, flags), SumcheckError)
-    fin = [x[0] | (x[1] << 64) for x in finals[:m].tolist()]
-    cl = [x[0] | (x[1] << 64) for x in claims[: 3 * m].tolist()]
-    return [OuterSumcheckOutput(rounds[4 * off[i]: 4 * off[i + 1]].reshape(ns[i], 4, 2),
-                                points[off[i]: off[i + 1]], fin[i], cl[3 * i], cl[3 * i + 1], cl[3 * i + 2])
-            for i in range(m)]
+    # the factor tables _factor made and the arrays the records point into stay alive with the records
+    return rec, (ns, off, (allch, tabs), rounds, points, finals, claims)
+
+
+def prove_outer_batch(instances, graph: bool = True, tune: bool = True) -> list:
+    """Many independent outer sumchecks with challenges fixed in advance, in
+    one batched call (bitz_prove_outer_batch).  instances: iterable of
+    (challenges, initial_claim, (eq_low, eq_high), R1csProductMles) with
+    DeviceTables for the products and factors.  Returns one
+    OuterSumcheckOutput per instance, equal to
+    prove_outer_sumcheck_with_challenges on the same inputs.  graph=False
+    launches the kernels directly instead of replaying the captured graph;
+    tune=False keeps the default tile shapes."""
+    b = OuterBatch(instances, graph, tune)
+    b.prove()
+    return b.outputs()
 
 
 def outer_batch_config() -> tuple:

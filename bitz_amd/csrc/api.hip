@@ -2355,6 +2355,11 @@ int32_t batch_autotune(const bitz_outer_instance* inst, uint32_t count, bool eag
 }  // namespace
 
 extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint32_t count, uint32_t flags) {
+    const auto t_total = std::chrono::steady_clock::now();
+    struct TotalTime {  // host phases of a batch call, with profiling on
+        std::chrono::steady_clock::time_point t0;
+        ~TotalTime() { prof_add_host("batch_total", t0); }
+    } total_time{t_total};
     if (!inst && count) return fail(BITZ_ERR_ARG, "null pointer");
     if (!(flags & BITZ_DEVICE)) return fail(BITZ_ERR_ARG, "batched proofs take device tables (BITZ_DEVICE)");
     for (uint32_t i = 0; i < count; ++i) {
@@ -2393,9 +2398,18 @@ extern "C" int32_t bitz_prove_outer_batch(const bitz_outer_instance* inst, uint3
     std::vector<uint32_t> alone = P.alone;
     int32_t rc = BITZ_OK;
     if (!P.members.empty()) {
+        const auto t_fill = std::chrono::steady_clock::now();
         const std::vector<uint64_t> q1mask = batch_q1mask(P, inst, count);
         batch_plan_fill(P, inst, q1mask);
+        prof_add_host("batch_fill", t_fill);
+        const auto t_run = std::chrono::steady_clock::now();
         TRY(batch_plan_run(P, eager));
+        prof_add_host("batch_run", t_run);
+        const auto t_finish = std::chrono::steady_clock::now();
+        struct FinishTime {
+            std::chrono::steady_clock::time_point t0;
+            ~FinishTime() { prof_add_host("batch_finish", t0); }
+        } finish_time{t_finish};
         const uint32_t* host_acc = P.host_acc();
         // Per instance: validate, recover the coefficients, check the claim.
         // Instances are independent; a few host threads share them.

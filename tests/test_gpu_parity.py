@@ -652,6 +652,17 @@ def test_outer_batch_of_many_small_instances():
         assert_outer_equal(oracle_with(ch, claim, lo, hi, a, b, c), out)
 
 
+def test_outer_batch_marshalled_once_proves_again():
+    """An OuterBatch's records reused: every call reproves the batch and
+    fills the same outputs (checked against the oracle after each)."""
+    cases = batch_cases([(5, 2), (11, 5), (13, 13)], 1500)
+    b = G.OuterBatch(batch_insts(cases))
+    for _ in range(2):
+        b.prove()
+        for (ch, claim, lo, hi, x, y, z), out in zip(cases, b.outputs()):
+            assert_outer_equal(oracle_with(ch, claim, lo, hi, x, y, z), out)
+
+
 def batch_cases(specs, seed0):
     cases = []
     for j, (n, split) in enumerate(specs):
