@@ -54,6 +54,9 @@ class _ServeRequest(C.Structure):
                 ("inner_point", C.c_void_p), ("matrix_eval", C.c_void_p), ("final_claim", C.c_void_p)]
 
 
+_SERVE_DTYPE = np.dtype(_ServeRequest)  # the same layout as a numpy record
+
+
 class NativeServer:
     """The library's proving server (bitz_server_*): admission, preparer
     threads, PIOP engines and harvesters all native, so serving holds no
@@ -217,42 +220,58 @@ class ProvingServer:
             self._native = None
             self._native = NativeServer(*config)
         ns = self._native
-        reqs = (_ServeRequest * max(len(requests), 1))()
-        keep = []
-        for i, r in enumerate(requests):
-            c = self.circuit(r.blocks)
-            q = reqs[i]
-            q.rid, q.circuit = r.rid, ns.add(c)
-            msg = C.create_string_buffer(bytes(r.message), max(len(r.message), 1))
-            q.message = C.cast(msg, C.c_void_p)
-            for k in range(8):
-                q.iv[k] = int(r.iv[k]) & 0xffffffff
-            q.arrival_s = float(r.arrival)
-            rv, cv = c.row_vars, c.col_vars
-            out = np.zeros((4 * rv + 3 + 3 * cv + cv + 2, 2), np.uint64)
-            base = out.ctypes.data
-            q.outer_rounds, q.outer_claims = base, base + 64 * rv
-            q.inner_rounds = base + 16 * (4 * rv + 3)
-            q.inner_point = base + 16 * (4 * rv + 3 + 3 * cv)
-            q.matrix_eval = base + 16 * (4 * rv + 3 + 4 * cv)
-            q.final_claim = base + 16 * (4 * rv + 4 + 4 * cv)
-            keep.append((msg, out, rv, cv))
+        m = len(requests)
+        # The request records as one numpy array in bitz_serve_request's layout, filled a column at a time;
+        # the messages back to back in one buffer; one output array for every request: rows [outer rounds
+        # (4 rv) | claims (3) | inner rounds (3 cv) | inner point (cv) | matrix eval | final claim] each.
+        reqs = np.zeros(max(m, 1), dtype=_SERVE_DTYPE)
+        circ = [self.circuit(r.blocks) for r in requests]
+        dims = np.array([(c.row_vars, c.col_vars) for c in circ], dtype=np.int64).reshape(-1, 2)
+        rv_a, cv_a = dims[:, 0], dims[:, 1]
+        starts = np.concatenate([[0], np.cumsum(4 * rv_a + 4 * cv_a + 5)]).astype(np.int64)
+        out = np.zeros((max(int(starts[-1]), 1), 2), np.uint64)
+        msgs = np.frombuffer(b"".join(r.message for r in requests) or b"\0", dtype=np.uint8)
+        moff = np.concatenate([[0], np.cumsum([len(r.message) for r in requests])]).astype(np.int64)
+        if m:
+            ids = {b: ns.add(c) for b, c in zip((r.blocks for r in requests), circ)}
+            b = out.ctypes.data + 16 * starts[:-1]
+            reqs["rid"][:m] = [r.rid for r in requests]
+            reqs["circuit"][:m] = [ids[r.blocks] for r in requests]
+            reqs["message"][:m] = msgs.ctypes.data + moff[:-1]
+            reqs["iv"][:m] = np.array([r.iv for r in requests], dtype=np.uint64) & 0xffffffff
+            reqs["arrival_s"][:m] = [r.arrival for r in requests]
+            reqs["outer_rounds"][:m] = b
+            reqs["outer_claims"][:m] = b + 64 * rv_a
+            reqs["inner_rounds"][:m] = b + 16 * (4 * rv_a + 3)
+            reqs["inner_point"][:m] = b + 16 * (4 * rv_a + 3 + 3 * cv_a)
+            reqs["matrix_eval"][:m] = b + 16 * (4 * rv_a + 3 + 4 * cv_a)
+            reqs["final_claim"][:m] = b + 16 * (4 * rv_a + 4 + 4 * cv_a)
         self.peak_bytes = self.peak_in_flight = 0
-        _check(lib().bitz_server_serve(ns.h, reqs, len(requests)), SumcheckError)
-        for i, r in enumerate(requests):
-            q, (_, out, rv, cv) = reqs[i], keep[i]
-            r.admitted, r.done = q.admitted_s, q.done_s
-            r.digest = bytes(q.digest)
-            o = 4 * rv
-            r.result = SpartanPiopOutput(out[:o].reshape(rv, 4, 2), [to_int(e) for e in out[o:o + 3]],
-                                         out[o + 3:o + 3 + 3 * cv].reshape(cv, 3, 2),
-                                         out[o + 3 + 3 * cv:o + 3 + 4 * cv], to_int(out[o + 3 + 4 * cv]),
-                                         to_int(out[o + 4 + 4 * cv]))
+        t0 = time.perf_counter()
+        _check(lib().bitz_server_serve(ns.h, C.c_void_p(reqs.ctypes.data), m), SumcheckError)
+        self.native_seconds = time.perf_counter() - t0  # the native server's part of the call
+        if m:
+            # the five scalar outputs of every request (claims, matrix eval, final claim) as ints in one pass
+            sc = starts[:-1] + 4 * rv_a
+            rows = np.concatenate([np.stack([sc, sc + 1, sc + 2], axis=1).ravel(),
+                                   np.stack([sc + 3 + 4 * cv_a, sc + 4 + 4 * cv_a], axis=1).ravel()])
+            vals = [lo | (hi << 64) for lo, hi in out[rows].tolist()]
+            adm, done = reqs["admitted_s"][:m].tolist(), reqs["done_s"][:m].tolist()
+            digs = reqs["digest"][:m].tobytes()
+            for i, r in enumerate(requests):
+                rv, cv, o = int(rv_a[i]), int(cv_a[i]), int(sc[i])
+                r.admitted, r.done = adm[i], done[i]
+                r.digest = digs[32 * i:32 * i + 32]
+                r.result = SpartanPiopOutput(out[o - 4 * rv:o].reshape(rv, 4, 2), vals[3 * i:3 * i + 3],
+                                             out[o + 3:o + 3 + 3 * cv].reshape(cv, 3, 2),
+                                             out[o + 3 + 3 * cv:o + 3 + 4 * cv], vals[3 * m + 2 * i],
+                                             vals[3 * m + 2 * i + 1])
         # the peaks of the admission that just ran (replayed from the times),
         # the resident circuits' bytes included
         self.resident_bytes = ns.resident_bytes()
-        ev = sorted([(r.admitted, 1, self.circuits[r.blocks].footprint()) for r in requests] +
-                    [(r.done, -1, -self.circuits[r.blocks].footprint()) for r in requests], key=lambda x: (x[0], x[1]))
+        fp = {b: self.circuits[b].footprint() for b in {r.blocks for r in requests}}
+        ev = sorted([(r.admitted, 1, fp[r.blocks]) for r in requests] +
+                    [(r.done, -1, -fp[r.blocks]) for r in requests], key=lambda x: (x[0], x[1]))
         n, b = 0, self.resident_bytes
         self.peak_bytes = b
         for _, dn, db in ev:
