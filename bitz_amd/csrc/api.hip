@@ -4110,7 +4110,6 @@ struct EnginePos {
     std::vector<uint32_t> i_r0_map, i_fold_map;
     std::vector<bitz::InnerBitsDesc> i_bits;  // every bit round K of the position, one launch
     std::vector<uint32_t> i_bits_map;
-    std::vector<bitz::FinalFoldDesc> finals;
     std::vector<bitz::PiopFsDesc> fs;
     std::vector<size_t> binds;       // proofs (indices into the step's list) that bind here
     std::vector<size_t> host_final;  // host-transcript proofs' final folds (position 0)
@@ -4208,13 +4207,10 @@ void devfs_emit(PiopJob* j, size_t i, EnginePos& P) {
         }
         case JobPhase::kOuterFinal: {
             bitz_outer* h = j->outer;
-            bitz::FinalFoldDesc d{};
-            for (int u = 0; u < 3; ++u) d.tables[u] = h->cur[u];
-            d.r = j->fs_out(PiopJob::kOutRx) + (j->n - 1);
-            d.out = reinterpret_cast<f128*>(j->fs_slot_outer(j->n));
-            d.count = 3;
-            P.finals.push_back(d);
             bitz::PiopFsDesc f{};
+            for (int u = 0; u < 3; ++u) f.fin_tab[u] = h->cur[u];
+            f.fin_r = j->fs_out(PiopJob::kOutRx) + (j->n - 1);
+            f.fin_count = 3;
             f.st = st;
             f.kind = bitz::kFsOuterFinal;
             f.sums = j->fs_slot_outer(j->n);
@@ -4312,14 +4308,11 @@ void devfs_emit(PiopJob* j, size_t i, EnginePos& P) {
             break;
         }
         case JobPhase::kInnerFinal: {
-            bitz::FinalFoldDesc d{};
-            d.tables[0] = j->icur[0];
-            d.tables[1] = j->icur[1];
-            d.r = j->fs_out(PiopJob::kOutPoint) + (j->nc - 1);
-            d.out = reinterpret_cast<f128*>(j->fs_slot_final());
-            d.count = 2;
-            P.finals.push_back(d);
             bitz::PiopFsDesc f{};
+            f.fin_tab[0] = j->icur[0];
+            f.fin_tab[1] = j->icur[1];
+            f.fin_r = j->fs_out(PiopJob::kOutPoint) + (j->nc - 1);
+            f.fin_count = 2;
             f.st = st;
             f.kind = bitz::kFsInnerFinal;
             f.sums = j->fs_slot_final();
@@ -4515,7 +4508,6 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         P.ifm_o = place(P.i_fold_map.size() * 4);
         P.ib_o = place(P.i_bits.size() * sizeof(bitz::InnerBitsDesc));
         P.ibm_o = place(P.i_bits_map.size() * 4);
-        P.fin_o = place(P.finals.size() * sizeof(bitz::FinalFoldDesc));
         P.fs_o = place(P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
     // Read-back: the host-transcript slots, then each finishing proof's state
@@ -4556,7 +4548,6 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
         memcpy(pin + P.ifm_o, P.i_fold_map.data(), P.i_fold_map.size() * 4);
         memcpy(pin + P.ib_o, P.i_bits.data(), P.i_bits.size() * sizeof(bitz::InnerBitsDesc));
         memcpy(pin + P.ibm_o, P.i_bits_map.data(), P.i_bits_map.size() * 4);
-        memcpy(pin + P.fin_o, P.finals.data(), P.finals.size() * sizeof(bitz::FinalFoldDesc));
         memcpy(pin + P.fs_o, P.fs.data(), P.fs.size() * sizeof(bitz::PiopFsDesc));
     }
     prof_add_host("piop_engine_launch_stage", stage_t0, (double)off);
@@ -4619,9 +4610,6 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
                 hipLaunchKernelGGL(bitz::k_fold, dim3(1), dim3(kBlock), 0, s, j->icur[1], out + 1, (size_t)1, j->r);
             }
         }
-        if (!P.finals.empty())
-            hipLaunchKernelGGL(bitz::k_final_folds, dim3((unsigned)P.finals.size()), dim3(64), 0, s,
-                               reinterpret_cast<const bitz::FinalFoldDesc*>(dst + P.fin_o));
         if (!P.o_r0.empty())
             hipLaunchKernelGGL(bitz::k_batch_r0, dim3((unsigned)P.o_r0_map.size()), dim3(bitz::kR0Block),
                                bitz::kR0Smem, s, reinterpret_cast<const bitz::RoundDesc*>(dst + P.or_o),

@@ -2218,6 +2218,15 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
         return v;
     };
     const g4 Z = g_zero();
+    if (D.fin_count) {  // the sumcheck's last fold, into the sums slot read below
+        if (lane < D.fin_count) {
+            const f128* t = D.fin_tab[lane];
+            const g4 a = g_load(t), b = g_load(t + 1);
+            g_store(reinterpret_cast<f128*>(const_cast<uint32_t*>(D.sums)) + lane,
+                    g_add(a, g_mul(g_load(D.fin_r), g_add(a, b))));
+        }
+        __syncthreads();
+    }
     g4 claim = g_load(&st->claim);
     if (D.kind == kFsInnerFinal) {  // D(r) and the final claim; the transcript is untouched
         if (lane == 0) {
@@ -2353,16 +2362,6 @@ __global__ __launch_bounds__(64) void k_piop_fs(const PiopFsDesc* descs) {
     }
     fs_save(&st->ts, f);
     const_table4_wave(D.rtab, r);
-}
-
-__global__ __launch_bounds__(64) void k_final_folds(const FinalFoldDesc* descs) {
-    const FinalFoldDesc D = descs[blockIdx.x];
-    if ((int)threadIdx.x < D.count) {
-        const g4 r = g_load(D.r);
-        const f128* t = D.tables[threadIdx.x];
-        const g4 a = g_load(t), b = g_load(t + 1);
-        g_store(D.out + threadIdx.x, g_add(a, g_mul(r, g_add(a, b))));
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_row_scales_dev(const f128* eq_rows, const f128* rr, f128* row_scale,

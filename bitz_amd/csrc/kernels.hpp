@@ -357,17 +357,13 @@ struct PiopFsDesc {
     f128 tau, tau_inv, scale;  // outer round k: tau_k, 1 / tau_k, the sums' scale
     f128 scale_t, scale_ti;    // scale (1 + tau_k), scale (1 + tau_k) / tau_k (zero when tau_k is)
     int kind, k, K, have_q1, mono_next;
+    // Finals: the last fold of the sumcheck's tables (fin_tab[t][0] + r (fin_tab[t][0] + fin_tab[t][1]),
+    // t < fin_count, r at *fin_r on the device) into `sums`, before the step reads them.
+    const f128* fin_tab[3];
+    const f128* fin_r;
+    int fin_count;
 };
 __global__ void k_piop_fs(const PiopFsDesc* d);
-// The last fold of a PIOP proof's tables by the challenge at *r (device):
-// tables[t][0] + r (tables[t][0] + tables[t][1]) to out[t], t < count.
-struct FinalFoldDesc {
-    const f128* tables[3];
-    const f128* r;
-    f128* out;
-    int count;
-};
-__global__ void k_final_folds(const FinalFoldDesc* d);
 // k_row_scales with rho and rho^2 read from the device (rr[0], rr[1]).
 __global__ void k_row_scales_dev(const f128* eq_rows, const f128* rr, f128* row_scale, size_t rows_padded);
 
