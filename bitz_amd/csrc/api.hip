@@ -285,8 +285,8 @@ static void eq_rows_attr() {
 int32_t eq_build(const f128* r_dev, int k, f128* out, Dev& scratch, hipStream_t s) {
     if (k <= bitz::kEqRowsMaxVars) {
         eq_rows_attr();
-        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(1), dim3(1024), bitz::kLaneSmem1024, s, r_dev, k, out,
-                           static_cast<const f128*>(nullptr));
+        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(bitz::eq_rows_grid(k)), dim3(1024), bitz::kLaneSmem1024, s, r_dev, k,
+                           out, static_cast<const f128*>(nullptr));
         return check_launch();
     }
     const int kl = k / 2, kh = k - kl;
@@ -3407,8 +3407,8 @@ static int32_t bind_enqueue(bitz_matrices* h, const f128* row_point, bool point_
     // the eq table in one launch
     if (rho_dev && h->row_vars <= bitz::kEqRowsMaxVars) {
         eq_rows_attr();
-        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(1), dim3(1024), bitz::kLaneSmem1024, s, pt, h->row_vars, row_scale,
-                           rho_dev);
+        hipLaunchKernelGGL(bitz::k_eq_rows, dim3(bitz::eq_rows_grid(h->row_vars)), dim3(1024), bitz::kLaneSmem1024, s,
+                           pt, h->row_vars, row_scale, rho_dev);
     } else {
         TRY(eq_build(pt, h->row_vars, eq_rows, own ? own->eq_scratch : h->scratch, s));
     }

@@ -325,17 +325,18 @@ __global__ __launch_bounds__(kBlock) void k_eq_small2(const f128* r, int kl, f12
     }
 }
 
-// eq(r, .) over k <= kEqRowsMaxVars variables by one 1024-thread workgroup:
-// the low kl = k / 2 and high kh = k - kl variables' tables (at most 64
-// entries each) doubled level by level side by side in LDS (threads 0..63 the
-// low one, 64..127 the high), then entry j = low[j mod 2^kl] * high[j >> kl],
-// four per thread.  Lane-table products throughout (kLaneSmem1024 of dynamic
-// LDS): a level costs one product's latency instead of a comb product's.
-// With rr (rho and rho^2 on the device) it writes the bind's row scales
-// instead, k_row_scales' layout [w | rho w | rho^2 w] over 2^k entries.
+// eq(r, .) over k <= kEqRowsMaxVars variables in 1024-thread workgroups:
+// each builds the low kl = k / 2 and high kh = k - kl variables' tables (at
+// most 256 entries each) level by level side by side in LDS (threads 0..255
+// the low one, 256..511 the high), then its share of the entries j =
+// low[j mod 2^kl] * high[j >> kl], at most kEqRowsPerThread per thread.
+// Lane-table products throughout (kLaneSmem1024 of dynamic LDS): a level
+// costs one product's latency instead of a comb product's.  With rr (rho and
+// rho^2 on the device) it writes the bind's row scales instead, k_row_scales'
+// layout [w | rho w | rho^2 w] over 2^k entries.
 __global__ __launch_bounds__(1024) void k_eq_rows(const f128* r, int k, f128* out, const f128* rr) {
     extern __shared__ __attribute__((aligned(256))) uint4 smem_raw[];
-    __shared__ uint4 lo[64], hi[64];
+    __shared__ uint4 lo[256], hi[256];
     uint4* col = smem_raw + (threadIdx.x >> 6) * (64 * kLaneTableEntries) + (threadIdx.x & 63);
     lane_table_clear(col);
     auto ld = [](const uint4* p) {
@@ -352,9 +353,9 @@ __global__ __launch_bounds__(1024) void k_eq_rows(const f128* r, int k, f128* ou
     }
     __syncthreads();
     for (int i = 0; i < kh; ++i) {  // kh >= kl
-        const int half = 1 << i, j = t & 63;
-        const bool high = t >= 64;
-        if (t < 128 && i < (high ? kh : kl) && j < half) {
+        const int half = 1 << i, j = t & 255;
+        const bool high = t >= 256;
+        if (t < 512 && i < (high ? kh : kl) && j < half) {
             uint4* T = high ? hi : lo;
             const g4 parent = ld(T + j), one = g_mul_lane(parent, g_load(r + (high ? kl : 0) + i), col);
             sto(T + j, g_add(parent, one));  // bit i clear: 1 + r_i
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(1024) void k_eq_rows(const f128* r, int k, f128* ou
         rho = g_load(rr);
         rho2 = g_load(rr + 1);
     }
-    for (size_t j = t; j < n; j += 1024) {
+    for (size_t j = blockIdx.x * (size_t)1024 + t; j < n; j += (size_t)gridDim.x * 1024) {
         const g4 w = g_mul_lane(ld(lo + (j & lmask)), ld(hi + (j >> kl)), col);
         g_store_hbm(out + j, w);
         if (rr) {

@@ -17,9 +17,14 @@ __global__ void k_mul_lane(const f128* a, const f128* b, f128* out, size_t n, in
 __global__ void k_mul_const(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_mfma(const f128* a, const f128* b, f128* out, size_t n);
 __global__ void k_mul_rows(const f128* a, const f128* b, f128* out, size_t n);
-// eq tables of up to 2^kEqRowsMaxVars entries in one 1024-thread workgroup
-// (kLaneSmem1024 of dynamic LDS); rr non-null: the bind's row scales.
-constexpr int kEqRowsMaxVars = 12;
+// eq tables of up to 2^kEqRowsMaxVars entries in 1024-thread workgroups of
+// kEqRowsPerThread entries per thread (kLaneSmem1024 of dynamic LDS); rr
+// non-null: the bind's row scales.
+constexpr int kEqRowsMaxVars = 16;
+constexpr int kEqRowsPerThread = 4;
+constexpr unsigned eq_rows_grid(int k) {
+    return (unsigned)((((size_t)1 << k) + 1024 * kEqRowsPerThread - 1) / (1024 * kEqRowsPerThread));
+}
 constexpr size_t kLaneSmem1024 = sizeof(uint4) * 16 * 64 * kLaneTableEntries;
 __global__ void k_eq_rows(const f128* r, int k, f128* out, const f128* rr);
 __global__ void k_eq_small2(const f128* r, int kl, f128* low, int kh, f128* high);
