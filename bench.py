@@ -116,6 +116,8 @@ def main() -> int:
     ap.add_argument("--capacity-only", action="store_true",
                     help="serve: only the timed capacity steps (no latency curve, slots or plain runs; "
                          "for kernel profiles of the batched mode)")
+    ap.add_argument("--serve-positions", type=int, default=0,
+                    help="serve: rounds an engine step advances each proof by (0: the library's default)")
     ap.add_argument("--engine-stats", action="store_true",
                     help="serve: also time every engine step (device span by HIP events, host launch and "
                          "finish, wait) and report per-step averages")
@@ -657,6 +659,8 @@ def bench_serve(G, args) -> int:
     if not args.no_cpu:
         cpu = cpu_baseline_serve(args)
     from bitz_amd import serving as S
+    if args.serve_positions:
+        G.set_engine_positions(args.serve_positions)
     srv = S.ProvingServer(hbm_budget=int(args.serve_budget_gib * (1 << 30)), slots=args.serve_slots)
     log("bench: preparing the nine SHA-256 circuits (1 .. 256 blocks)")
     rng = np.random.default_rng(args.seed + 1)

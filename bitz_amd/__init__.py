@@ -177,6 +177,7 @@ def lib():
             "bitz_transcript_set_grind": (I32, [P, P]),
             "bitz_transcript_keccak": (P, [P, SZ]),
             "bitz_set_device_fiat_shamir": (I32, [I32]),
+            "bitz_set_engine_positions": (I32, [I32]),
             "bitz_keccak_hash": (I32, [P, SZ, I32, C.c_uint8, P, SZ, U32]),
             "bitz_post_gkr_prove": (I32, [P, P, U32, P, U32, F128, P, SZ, U32, U32, P, P]),
             "bitz_transcript_hint": (I32, [P, P, SZ]),
@@ -1383,6 +1384,13 @@ def gpgkr_prove_with_challenges(point, challenges, witnesses: LayerWitnesses):
                                                 _p(evals), _p(leaf_point), C.byref(claim)))
     return (msgs[: 2 * rounds].reshape(rounds, 2, 2), evals[: 2 * layers].reshape(layers, 2, 2), leaf_point,
             to_int(claim))
+
+
+def set_engine_positions(positions: int) -> None:
+    """Rounds a PIOP engine step may advance each device-transcript proof by
+    (1 to 16), or 0 (the default: 6 while an engine holds fewer than 8
+    proofs, 4 from 8 on); the proofs do not depend on it."""
+    _check(lib().bitz_set_engine_positions(int(positions)))
 
 
 def set_device_fiat_shamir(on: bool) -> None:

@@ -3762,7 +3762,19 @@ enum class JobPhase { kOuter, kOuterFinal, kBind, kInner, kInnerFinal, kDevDone,
 // K blocks/s, with p50 at 1000 requests/s of 1.35, 1.21, 1.16, 1.16, 1.15,
 // 1.13 ms (shorter steps let proofs join and leave sooner; longer ones
 // spread each step's host work over more rounds).
+// Positions (rounds per proof) a step may advance, at most
+// kEngineMaxPositions.  By default (g_engine_positions = 0) a step with few
+// proofs takes kEnginePositions, so that a proof needs fewer steps (latency at
+// low load), and one with kEngineBusyProofs or more takes kEngineBusyPositions,
+// so that proofs join and leave sooner (throughput under load: 485-501 K
+// blocks/s at 4 positions against 467-478 K at 6, while p50 at 1000
+// requests/s was 1.26 ms at 4 against 1.17 at 6); bitz_set_engine_positions
+// fixes it.
 constexpr int kEnginePositions = 6;
+constexpr int kEngineBusyPositions = 4;
+constexpr size_t kEngineBusyProofs = 8;
+constexpr int kEngineMaxPositions = 16;
+static std::atomic<int> g_engine_positions{0};
 
 // Words of a proof's slot in a step's read-back (a bit round's take
 // kInnerBitsAccWords): round sums, round 0's live length at words 8-9.
@@ -3869,7 +3881,7 @@ struct bitz_piop_engine {
     std::string worker_err;
     // HIP events around the step's batched inner fold and around the whole
     // step's device work (when profiling), read once the step has synchronised.
-    Timed t_inner_fold[kEnginePositions], t_step, t_bits[kEnginePositions];
+    Timed t_inner_fold[kEngineMaxPositions], t_step, t_bits[kEngineMaxPositions];
     StreamOwner so;
     ~bitz_piop_engine() {
         if (running) {
@@ -4347,7 +4359,9 @@ static int32_t engine_launch_step(bitz_piop_engine* e, size_t* launched) {
     // Positions: a host-transcript proof does one round per step (its
     // challenge comes from the host after the read-back); a device-transcript
     // proof does up to kEnginePositions.
-    const int npos = any_devfs ? kEnginePositions : 1;
+    const int fixed = g_engine_positions.load();
+    const int npos = !any_devfs ? 1 : fixed ? fixed
+                   : act.size() >= kEngineBusyProofs ? kEngineBusyPositions : kEnginePositions;
     std::vector<EnginePos> pos(npos);
     // Each host-transcript proof's slot for the step's sums: a bit round's
     // copies (kInnerBitsAccWords), else kEngineSlotWords words.
@@ -5256,6 +5270,12 @@ extern "C" int32_t bitz_gkr_layer(bitz_gkr* h, uint32_t j, bitz_f128* out, size_
 
 // Whether provers with a Keccak transcript draw their challenges on the
 // device (bitz_set_device_fiat_shamir); on by default (g_device_fs, above).
+
+extern "C" int32_t bitz_set_engine_positions(int32_t positions) {
+    if (positions < 0 || positions > kEngineMaxPositions) return fail(BITZ_ERR_ARG, "positions out of range");
+    g_engine_positions = positions;
+    return BITZ_OK;
+}
 
 extern "C" int32_t bitz_set_device_fiat_shamir(int32_t on) {
     g_device_fs = on != 0;

@@ -126,6 +126,11 @@ bitz_transcript* bitz_transcript_keccak(const uint8_t* domain, size_t len);
  * over a packed assignment (bitz_piop_engine_submit_packed / _assignment,
  * bitz_server_serve). */
 int32_t bitz_set_device_fiat_shamir(int32_t on);
+/* Rounds a PIOP engine step may advance each device-transcript proof by (1 to
+ * 16), or 0 (the default): 6 while the engine holds fewer than 8 proofs, 4
+ * from 8 on.  A tuning switch with no counterpart in the reference; the
+ * proofs are the same for every value. */
+int32_t bitz_set_engine_positions(int32_t positions);
 /* One-message Keccak sponge (pad 0x06: SHA3-*, 0x1F: SHAKE*) on the host,
  * or with BITZ_DEVICE by one device thread: the permutation's check against
  * FIPS 202 answers. */
